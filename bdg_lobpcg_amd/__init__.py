@@ -1,0 +1,433 @@
+"""bdg_lobpcg_amd: MI355X (gfx950) BdG iLOBPCG hot path.
+
+The product is the in-tree shared library ``libbdg_amd.so`` (HIP kernels +
+host C): the reference's public C API (``include/bdg/bdg.h``), its
+extensions (``include/bdg/bdg_amd.h``) and the device C ABI
+(``include/bdgx.h``).  This module only binds it with ctypes so that Python
+tests, the smoke check and the benchmark can drive it; it never computes
+anything itself and raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbdg_amd.so")
+_lib = None
+
+# operator codes (include/bdgx.h)
+OP = {"kinetic": 0, "K": 1, "M": 2, "Lrep": 3, "Swap": 4, "precondK": 5,
+      "precondM": 6, "precondLrep": 7, "dipolar_conv": 8}
+FP64, FP32 = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+INIT_PLANEWAVE, INIT_WF_WEIGHTED, INIT_REUSE, INIT_CUSTOM = 0, 1, 2, 3
+GEOM_AUTO, GEOM_ELONGATED, GEOM_RING = 0, 1, 2
+
+V_TRAP_FN = C.CFUNCTYPE(C.c_double, C.c_uint64, C.POINTER(C.c_double), C.c_void_p)
+U_INT_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double)
+INIT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p)
+
+
+class BdgError(RuntimeError):
+    pass
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint64), ("iterations_fp32", C.c_uint64),
+                ("iterations_fp64", C.c_uint64), ("column_applies", C.c_uint64),
+                ("restarts", C.c_uint64), ("seconds_total", C.c_double),
+                ("seconds_fp32", C.c_double), ("seconds_fp64", C.c_double),
+                ("seconds_rr", C.c_double), ("status", C.c_int)]
+
+
+def build() -> None:
+    """Compile libbdg_amd.so for gfx950 with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-j8"], check=True)
+
+
+def lib():
+    """Load the native library; raise loudly when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BdgError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P, U64, I, D = C.c_void_p, C.c_uint64, C.c_int, C.c_double
+    sig = {
+        # device ABI
+        "bdgx_ctx_create": (I, [C.POINTER(P), U64, P, P, I]),
+        "bdgx_ctx_destroy": (None, [P]),
+        "bdgx_ctx_size": (U64, [P]),
+        "bdgx_ctx_ksize": (U64, [P]),
+        "bdgx_error_string": (C.c_char_p, [I]),
+        "bdgx_set_fields": (I, [P, P, P, P, P, P, D]),
+        "bdgx_dipolar_set_kernel": (I, [P, D, P, D]),
+        "bdgx_dipolar_get_kernel": (I, [P, P]),
+        "bdgx_dipolar_meanfield": (I, [P, P]),
+        "bdgx_apply": (I, [P, I, I, P, P, U64, U64]),
+        "bdgx_apply_host": (I, [P, I, P, P, U64, U64]),
+        "bdgx_gram": (I, [P, I, P, U64, U64, P, U64, U64, U64, I, P]),
+        "bdgx_combine": (I, [P, I, P, U64, U64, P, P, U64, U64, U64]),
+        "bdgx_residual": (I, [P, I, P, P, P, U64, U64, U64, P, I, P, U64, P]),
+        "bdgx_convert": (I, [P, I, P, U64, I, P, U64, U64, U64]),
+        "bdgx_itp_groundstate": (I, [P, P, D, D, P, D, D, U64, P, P]),
+        "bdgx_malloc": (I, [C.POINTER(P), C.c_size_t]),
+        "bdgx_free": (I, [P]),
+        "bdgx_memcpy_h2d": (I, [P, P, P, C.c_size_t]),
+        "bdgx_memcpy_d2h": (I, [P, P, P, C.c_size_t]),
+        "bdgx_memset": (I, [P, P, I, C.c_size_t]),
+        "bdgx_sync": (I, [P]),
+        "bdgx_stream": (P, [P]),
+        "bdgx_timer_begin": (I, [P]),
+        "bdgx_timer_end": (I, [P, C.POINTER(C.c_float)]),
+        "bdgx_workspace_bytes": (U64, [P]),
+        # public API
+        "bdg_alloc": (P, [U64, P, P, I]),
+        "bdg_free": (None, [C.POINTER(P)]),
+        "bdg_reset": (None, [P]),
+        "bdg_set_system": (None, [P]),
+        "bdg_set_trap": (None, [P, P, P]),
+        "bdg_set_wavefunction": (None, [P, P, U64]),
+        "bdg_set_local_interactions": (None, [P, P, P, P]),
+        "bdg_set_mu": (None, [P, D]),
+        "bdg_set_dipolar": (None, [P, D, P, D]),
+        "bdg_set_solver_params": (None, [P, U64, U64, U64, D]),
+        "bdg_set_init_mode": (None, [P, I, P, P]),
+        "bdg_reuse_modes": (I, [P, D]),
+        "bdg_solve": (I, [P]),
+        "bdg_converged": (U64, [P]),
+        "bdg_eigenvalues": (C.POINTER(D), [P]),
+        "bdg_modes_u": (P, [P]),
+        "bdg_modes_v": (P, [P]),
+        "bdg_load_wavefunction": (I, [P, C.c_char_p]),
+        "bdg_write_eigenvalues": (I, [P, C.c_char_p]),
+        "bdg_write_mode_u": (I, [P, U64, C.c_char_p]),
+        "bdg_write_mode_v": (I, [P, U64, C.c_char_p]),
+        # extensions
+        "bdg_set_precision": (None, [P, I]),
+        "bdg_get_solve_stats": (None, [P, C.POINTER(SolveStats)]),
+        "bdg_residuals": (C.POINTER(D), [P]),
+        "bdg_zero_modes": (C.POINTER(I), [P]),
+        "bdg_device_context": (P, [P]),
+        "bdg_local_term_K": (C.POINTER(D), [P]),
+        "bdg_local_term_M": (C.POINTER(D), [P]),
+        "bdg_precond_weight_K": (C.POINTER(D), [P]),
+        "bdg_precond_weight_M": (C.POINTER(D), [P]),
+        "bdg_mu": (D, [P]),
+        "bdg_set_mixed_params": (None, [P, D, U64]),
+        "bdg_set_verbosity": (None, [P, I]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().bdgx_error_string(rc).decode() if rc > -100 else "rayleigh-ritz failure"
+        raise BdgError(f"{what}: error {rc} ({msg})")
+
+
+# ---------------------------------------------------------------- device --
+class DeviceBuffer:
+    """Raw device allocation owned by Python (freed on close/GC)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check(lib().bdgx_malloc(C.byref(p), int(nbytes)), "bdgx_malloc")
+        self.ptr = p
+        self.nbytes = int(nbytes)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().bdgx_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceContext:
+    """bdgx context: grid, FFT plans, physics fields, HIP stream."""
+
+    def __init__(self, N, L, complex_psi0: bool = False):
+        self.N = tuple(int(n) for n in N)
+        self.L = tuple(float(v) for v in L)
+        self.cplx = bool(complex_psi0)
+        h = C.c_void_p()
+        Na = np.ascontiguousarray(self.N, dtype=np.uint64)
+        La = np.ascontiguousarray(self.L, dtype=np.float64)
+        check(lib().bdgx_ctx_create(C.byref(h), len(self.N), _p(Na), _p(La), int(self.cplx)),
+              "bdgx_ctx_create")
+        self.h = h
+        self.size = int(lib().bdgx_ctx_size(h))
+        self.k_size = int(lib().bdgx_ctx_ksize(h))
+        self.dtype64 = np.complex128 if self.cplx else np.float64
+        self.dtype32 = np.complex64 if self.cplx else np.float32
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bdgx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def dtype(self, prec):
+        return self.dtype64 if prec == FP64 else self.dtype32
+
+    def set_fields(self, ltK=None, ltM=None, wf=None, pK=None, pM=None, mu=0.0):
+        keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return _p(a)
+
+        check(lib().bdgx_set_fields(self.h, arr(ltK, np.float64), arr(ltM, np.float64),
+                                    arr(wf, self.dtype64), arr(pK, np.float64),
+                                    arr(pM, np.float64), float(mu)), "bdgx_set_fields")
+
+    def set_dipolar(self, g_ddi, direction, cutoff):
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        check(lib().bdgx_dipolar_set_kernel(self.h, float(g_ddi), _p(d), float(cutoff)),
+              "bdgx_dipolar_set_kernel")
+
+    def dipolar_kernel(self):
+        out = np.zeros(self.k_size)
+        check(lib().bdgx_dipolar_get_kernel(self.h, _p(out)), "bdgx_dipolar_get_kernel")
+        return out
+
+    def dipolar_meanfield(self):
+        out = np.zeros(self.size)
+        check(lib().bdgx_dipolar_meanfield(self.h, _p(out)), "bdgx_dipolar_meanfield")
+        return out
+
+    def upload(self, a: np.ndarray) -> DeviceBuffer:
+        a = np.asfortranarray(a)
+        b = DeviceBuffer(a.nbytes)
+        check(lib().bdgx_memcpy_h2d(self.h, b.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes), "h2d")
+        return b
+
+    def download(self, b: DeviceBuffer, shape, dtype) -> np.ndarray:
+        out = np.zeros(shape, dtype=dtype, order="F")
+        check(lib().bdgx_memcpy_d2h(self.h, out.ctypes.data_as(C.c_void_p), b.ptr, out.nbytes), "d2h")
+        return out
+
+    def apply(self, op: str, X: np.ndarray, prec: int = FP64) -> np.ndarray:
+        """Upload a column block, apply `op` on the device, download."""
+        X = np.asarray(X)
+        one = X.ndim == 1
+        Xf = np.asfortranarray(X[:, None] if one else X, dtype=self.dtype(prec))
+        dX = self.upload(Xf)
+        dY = DeviceBuffer(Xf.nbytes)
+        check(lib().bdgx_memset(self.h, dY.ptr, 0, Xf.nbytes), "memset")
+        check(lib().bdgx_apply(self.h, OP[op], prec, dX.ptr, dY.ptr, Xf.shape[1], Xf.shape[0]),
+              f"bdgx_apply({op})")
+        Y = self.download(dY, Xf.shape, Xf.dtype)
+        return Y[:, 0] if one else Y
+
+    def apply_device(self, op, prec, dX, dY, ncols, ld):
+        check(lib().bdgx_apply(self.h, OP[op], prec, dX.ptr, dY.ptr, int(ncols), int(ld)),
+              f"bdgx_apply({op})")
+
+    def sync(self):
+        check(lib().bdgx_sync(self.h), "sync")
+
+    def timer_begin(self):
+        check(lib().bdgx_timer_begin(self.h), "timer")
+
+    def timer_end(self) -> float:
+        ms = C.c_float()
+        check(lib().bdgx_timer_end(self.h, C.byref(ms)), "timer")
+        return float(ms.value)
+
+    def gram(self, A: np.ndarray, B: np.ndarray, prec=FP64, swap_b=False) -> np.ndarray:
+        A = np.asfortranarray(A, dtype=self.dtype(prec))
+        B = np.asfortranarray(B, dtype=self.dtype(prec))
+        dA, dB = self.upload(A), self.upload(B)
+        w = 2 if self.cplx else 1
+        dG = DeviceBuffer(8 * w * A.shape[1] * B.shape[1])
+        check(lib().bdgx_gram(self.h, prec, dA.ptr, A.shape[0], A.shape[1], dB.ptr, B.shape[0],
+                              B.shape[1], A.shape[0], int(swap_b), dG.ptr), "gram")
+        G = self.download(dG, (A.shape[1], B.shape[1]), self.dtype64)
+        return G
+
+    def combine(self, A: np.ndarray, Cm: np.ndarray, prec=FP64) -> np.ndarray:
+        A = np.asfortranarray(A, dtype=self.dtype(prec))
+        Cm = np.asfortranarray(Cm, dtype=self.dtype64)
+        dA, dC = self.upload(A), self.upload(Cm)
+        dY = DeviceBuffer(A.shape[0] * Cm.shape[1] * A.itemsize)
+        check(lib().bdgx_combine(self.h, prec, dA.ptr, A.shape[0], A.shape[1], dC.ptr, dY.ptr,
+                                 A.shape[0], Cm.shape[1], A.shape[0]), "combine")
+        return self.download(dY, (A.shape[0], Cm.shape[1]), A.dtype)
+
+    def itp_groundstate(self, psi, omega, g, g_dd, direction, cutoff, dt, nsteps):
+        psi = np.ascontiguousarray(psi, dtype=self.dtype64).copy()
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        mu = C.c_double(0.0)
+        check(lib().bdgx_itp_groundstate(self.h, _p(om), float(g), float(g_dd), _p(d),
+                                         float(cutoff), float(dt), int(nsteps), _p(psi),
+                                         C.cast(C.pointer(mu), C.c_void_p)), "itp")
+        return psi, mu.value
+
+
+# --------------------------------------------------------------- problem --
+class Problem:
+    """The reference's bdg_t lifecycle (include/bdg/bdg.h) driven from Python."""
+
+    def __init__(self, N, L, complex_psi0: bool = False):
+        self.N = tuple(int(n) for n in N)
+        self.L = tuple(float(v) for v in L)
+        self.cplx = bool(complex_psi0)
+        Na = np.ascontiguousarray(self.N, dtype=np.uint64)
+        La = np.ascontiguousarray(self.L, dtype=np.float64)
+        self.h = C.c_void_p(lib().bdg_alloc(len(self.N), _p(Na), _p(La), int(self.cplx)))
+        self.size = int(np.prod(self.N))
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().bdg_free(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # setup, in the reference's order
+    def set_system(self):
+        lib().bdg_set_system(self.h)
+
+    def set_trap_harmonic(self, omega):
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        self._keep.append(om)
+        fn = C.cast(lib().bdg_cb_harmonic_trap, C.c_void_p)
+        lib().bdg_set_trap(self.h, fn, _p(om))
+
+    def set_trap(self, fn_py):
+        cb = V_TRAP_FN(lambda dim, r, p: fn_py([r[i] for i in range(dim)]))
+        self._keep.append(cb)
+        lib().bdg_set_trap(self.h, C.cast(cb, C.c_void_p), None)
+
+    def set_wavefunction(self, wf):
+        a = np.ascontiguousarray(wf, dtype=np.complex128 if self.cplx else np.float64)
+        lib().bdg_set_wavefunction(self.h, _p(a), a.size)
+
+    def set_contact(self, gK, gM=None):
+        gK = np.array([float(gK)])
+        gM = np.array([float(gK[0] if gM is None else gM)])
+        self._keep += [gK, gM]
+        fn = C.cast(lib().bdg_cb_contact, C.c_void_p)
+        # U_intK and U_intM share the callback; the parameter selects g
+        self._keep.append(fn)
+        # two different parameters are needed: use Python trampolines only if
+        # gK != gM, else the native callback with one parameter
+        if gK[0] == gM[0]:
+            lib().bdg_set_local_interactions(self.h, fn, fn, _p(gK))
+        else:
+            cbK = U_INT_FN(lambda p, n: gK[0] * n)
+            cbM = U_INT_FN(lambda p, n: gM[0] * n)
+            self._keep += [cbK, cbM]
+            lib().bdg_set_local_interactions(self.h, C.cast(cbK, C.c_void_p),
+                                             C.cast(cbM, C.c_void_p), None)
+
+    def set_local_interactions(self, fK, fM):
+        cbK = U_INT_FN(lambda p, n: fK(n))
+        cbM = U_INT_FN(lambda p, n: fM(n))
+        self._keep += [cbK, cbM]
+        lib().bdg_set_local_interactions(self.h, C.cast(cbK, C.c_void_p),
+                                         C.cast(cbM, C.c_void_p), None)
+
+    def set_dipolar(self, g_ddi, direction, cutoff):
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        lib().bdg_set_dipolar(self.h, float(g_ddi), _p(d), float(cutoff))
+
+    def set_mu(self, mu):
+        lib().bdg_set_mu(self.h, float(mu))
+
+    def set_solver_params(self, nev, sizeSub, maxIter, tol):
+        lib().bdg_set_solver_params(self.h, int(nev), int(sizeSub), int(maxIter), float(tol))
+
+    def set_precision(self, mode):
+        lib().bdg_set_precision(self.h, int(mode))
+
+    def set_mixed_params(self, tol32, maxit32=0):
+        lib().bdg_set_mixed_params(self.h, float(tol32), int(maxit32))
+
+    def set_verbosity(self, level):
+        lib().bdg_set_verbosity(self.h, int(level))
+
+    def set_init_custom_block(self, X: np.ndarray):
+        """BDG_INIT_CUSTOM with a fixed n x sizeSub block (copied in the callback)."""
+        Xf = np.asfortranarray(X, dtype=np.complex128 if self.cplx else np.float64)
+
+        def fill(bdg, ptr, n, m, param):
+            C.memmove(ptr, Xf.ctypes.data, Xf.nbytes)
+
+        cb = INIT_FN(fill)
+        self._keep += [Xf, cb]
+        lib().bdg_set_init_mode(self.h, INIT_CUSTOM, C.cast(cb, C.c_void_p), None)
+
+    def set_init_mode(self, mode, hint=None):
+        lib().bdg_set_init_mode(self.h, int(mode), None,
+                                C.c_void_p(int(hint)) if hint else None)
+
+    def solve(self) -> int:
+        return int(lib().bdg_solve(self.h))
+
+    # results
+    def converged(self) -> int:
+        return int(lib().bdg_converged(self.h))
+
+    def nev_arr(self, ptr, n, dt=np.float64):
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt).copy()
+
+    def eigenvalues(self, nev) -> np.ndarray:
+        p = lib().bdg_eigenvalues(self.h)
+        return np.ctypeslib.as_array(p, shape=(nev,)).copy()
+
+    def residuals(self, nev) -> np.ndarray:
+        return np.ctypeslib.as_array(lib().bdg_residuals(self.h), shape=(nev,)).copy()
+
+    def zero_modes(self, nev) -> np.ndarray:
+        return np.ctypeslib.as_array(lib().bdg_zero_modes(self.h), shape=(nev,)).astype(bool)
+
+    def modes(self, nev):
+        dt = np.complex128 if self.cplx else np.float64
+        out = []
+        for fn in (lib().bdg_modes_u, lib().bdg_modes_v):
+            p = fn(self.h)
+            a = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)),
+                                      shape=(self.size * nev * (2 if self.cplx else 1),))
+            out.append(a.view(dt).reshape(nev, self.size).T.copy())
+        return out[0], out[1]
+
+    def stats(self) -> dict:
+        s = SolveStats()
+        lib().bdg_get_solve_stats(self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in SolveStats._fields_}
+
+    def arrays(self):
+        S = self.size
+        g = lambda fn: np.ctypeslib.as_array(fn(self.h), shape=(S,)).copy()
+        return dict(ltK=g(lib().bdg_local_term_K), ltM=g(lib().bdg_local_term_M),
+                    pK=g(lib().bdg_precond_weight_K), pM=g(lib().bdg_precond_weight_M),
+                    mu=float(lib().bdg_mu(self.h)))
+
+    def device_context_handle(self):
+        return lib().bdg_device_context(self.h)

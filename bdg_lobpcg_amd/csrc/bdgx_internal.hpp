@@ -1,0 +1,87 @@
+// Internal definitions shared by the HIP translation units of libbdg_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bdgx.h"
+
+#define BDGX_HIP(call)                                   \
+  do {                                                   \
+    hipError_t e__ = (call);                             \
+    if (e__ != hipSuccess) {                             \
+      bdgx_last_hip_error = e__;                         \
+      return BDGX_EHIP;                                  \
+    }                                                    \
+  } while (0)
+
+extern hipError_t bdgx_last_hip_error;
+
+namespace bdgx {
+
+// complex value of precision R
+template <typename R>
+struct cx {
+  R re, im;
+};
+
+// One axis FFT plan: mixed-radix Stockham stages.
+struct AxisPlan {
+  int n;
+  int nst;
+  int rad[12];
+  int ns[12];
+};
+
+constexpr int kMaxChan = 3;
+
+// weights applied when a channel pencil is formed (load side)
+enum PreKind { PRE_NONE = 0, PRE_REAL = 1, PRE_CONJ_PSI = 2, PRE_ABS2 = 3 };
+// coefficient applied when a channel is folded into the output (store side)
+enum PostKind { POST_ONE = 0, POST_REAL = 1, POST_PSI = 2, POST_RE = 3 };
+// k-space multipliers applied between the forward and inverse z transform
+enum MulKind { MUL_ONE = 0, MUL_KIN = 1, MUL_DIP = 2, MUL_PRE = 3, MUL_ITPKIN = 4 };
+
+struct Geo {
+  int nx, ny, nz, n0k;
+  long long S, Kk;
+};
+
+// Device-side copies of the physics fields in one precision.
+template <typename R>
+struct Fields {
+  R *ltK = nullptr, *ltM = nullptr, *pK = nullptr, *pM = nullptr;
+  R *wf = nullptr;    // size (real) or 2*size (complex, interleaved)
+  R *U = nullptr;     // k_size dipolar kernel
+  R *kx2[3] = {nullptr, nullptr, nullptr};
+  cx<R> *tw[3] = {nullptr, nullptr, nullptr};  // per-axis twiddles exp(-2 pi i m/n)
+};
+
+}  // namespace bdgx
+
+struct bdgx_ctx {
+  int dim, cplx, device;
+  uint64_t N[3];
+  double L[3];
+  bdgx::Geo geo;
+  bdgx::AxisPlan plan[3];
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  bdgx::Fields<double> f64;
+  bdgx::Fields<float> f32;
+  double mu, g_ddi;
+  double itp_dt;  // time step of the imaginary-time kinetic multiplier
+  int has_wf, has_U, has_lt, has_p;
+  // spectrum workspace (grown on demand)
+  void *spec;
+  size_t spec_bytes;
+  // small device scratch for gram partials etc.
+  void *scratch;
+  size_t scratch_bytes;
+};
+
+namespace bdgx {
+int ensure_spec(bdgx_ctx *c, size_t bytes);
+int ensure_scratch(bdgx_ctx *c, size_t bytes);
+int make_plan(int n, AxisPlan *p);
+}  // namespace bdgx

@@ -1,0 +1,647 @@
+// Device context and the extern "C" entry points of libbdg_amd's HIP layer.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "bdgx_internal.hpp"
+#include "fft_core.hpp"
+
+hipError_t bdgx_last_hip_error = hipSuccess;
+
+namespace bdgx {
+int apply_dispatch(bdgx_ctx *c, int op, int prec, const void *dX, void *dY,
+                   uint64_t ncols, uint64_t ld);
+int gram_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
+                  const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
+                  double *dG);
+int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
+                     const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows);
+int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,
+                      uint64_t ld, uint64_t ncols, uint64_t nrows, const double *dlam,
+                      int swap_x, void *dR, uint64_t ldr, double *dn2);
+int convert_dispatch(bdgx_ctx *c, int ps, const void *src, uint64_t lds, int pd, void *dst,
+                     uint64_t ldd, uint64_t nrows, uint64_t ncols);
+template <typename R>
+int meanfield_t(bdgx_ctx *c, const R *dpsi, R *dOut, double scale);
+template <typename R>
+int itp_kinetic_t(bdgx_ctx *c, R *dpsi, double dt);
+
+int make_plan(int n, AxisPlan *p) {
+  memset(p, 0, sizeof(*p));
+  p->n = n;
+  int m = n;
+  int f[32], nf = 0;
+  while (m % 8 == 0 && m > 1) { f[nf++] = 8; m /= 8; }
+  while (m % 4 == 0 && m > 1) { f[nf++] = 4; m /= 4; }
+  while (m % 2 == 0 && m > 1) { f[nf++] = 2; m /= 2; }
+  const int odd[] = {3, 5, 7};
+  for (int t = 0; t < 3; ++t)
+    while (m % odd[t] == 0 && m > 1) { f[nf++] = odd[t]; m /= odd[t]; }
+  if (m != 1 || nf > 12) return BDGX_EINVAL;
+  int Ns = 1;
+  for (int s = 0; s < nf; ++s) {
+    p->rad[s] = f[s];
+    p->ns[s] = Ns;
+    Ns *= f[s];
+  }
+  p->nst = nf;
+  return BDGX_OK;
+}
+
+int ensure_spec(bdgx_ctx *c, size_t bytes) {
+  if (bytes <= c->spec_bytes) return BDGX_OK;
+  if (c->spec) {
+    BDGX_HIP(hipStreamSynchronize(c->stream));
+    BDGX_HIP(hipFree(c->spec));
+    c->spec = nullptr;
+    c->spec_bytes = 0;
+  }
+  BDGX_HIP(hipMalloc(&c->spec, bytes));
+  c->spec_bytes = bytes;
+  return BDGX_OK;
+}
+
+int ensure_scratch(bdgx_ctx *c, size_t bytes) {
+  if (bytes <= c->scratch_bytes) return BDGX_OK;
+  if (c->scratch) {
+    BDGX_HIP(hipStreamSynchronize(c->stream));
+    BDGX_HIP(hipFree(c->scratch));
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+  }
+  BDGX_HIP(hipMalloc(&c->scratch, bytes));
+  c->scratch_bytes = bytes;
+  return BDGX_OK;
+}
+
+template <typename R>
+static int upload_conv(bdgx_ctx *c, R **dst, const double *src, size_t n) {
+  if (!*dst) BDGX_HIP(hipMalloc((void **)dst, sizeof(R) * (n ? n : 1)));
+  std::vector<R> tmp(n);
+  for (size_t i = 0; i < n; ++i) tmp[i] = (R)src[i];
+  BDGX_HIP(hipMemcpy(*dst, tmp.data(), sizeof(R) * n, hipMemcpyHostToDevice));
+  return BDGX_OK;
+}
+
+static int upload_both(bdgx_ctx *c, double **d64, float **d32, const double *src, size_t n) {
+  int rc = upload_conv<double>(c, d64, src, n);
+  if (rc) return rc;
+  return upload_conv<float>(c, d32, src, n);
+}
+
+template <typename R>
+static void free_fields(Fields<R> &f) {
+  R **arr[] = {&f.ltK, &f.ltM, &f.pK, &f.pM, &f.wf, &f.U};
+  for (auto p : arr)
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  for (int d = 0; d < 3; ++d) {
+    if (f.kx2[d]) (void)hipFree(f.kx2[d]);
+    if (f.tw[d]) (void)hipFree(f.tw[d]);
+    f.kx2[d] = nullptr;
+    f.tw[d] = nullptr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dipolar kernel U(k) on the device (reference src/dipolar.c:21-72)
+__global__ void k_dipolar_kernel(double *U, int nx, int ny, int nz, int n0k, int cplx,
+                                 double dkx, double dky, double dkz, double d0, double d1,
+                                 double d2, double R) {
+  const long long total = (long long)n0k * ny * nz;
+  const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(e % n0k);
+    const long long rest = e / n0k;
+    const int iy = (int)(rest % ny), iz = (int)(rest / ny);
+    const double kz = dkz * (double)(iz <= nz / 2 ? iz : iz - nz);
+    const double ky = dky * (double)(iy <= ny / 2 ? iy : iy - ny);
+    const double kx = dkx * (double)((cplx && ix > nx / 2) ? ix - nx : ix);
+    const double ks = kx * kx + ky * ky + kz * kz;
+    if (ks < 1e-50) {
+      U[e] = 0.0;
+      continue;
+    }
+    const double kd = kx * d0 + ky * d1 + kz * d2;
+    const double c2 = kd * kd / (ks * dd);
+    const double kR = sqrt(ks) * R, kR2 = kR * kR;
+    const double f = 1.0 + 3.0 * cos(kR) / kR2 - 3.0 * sin(kR) / (kR2 * kR);
+    U[e] = (3.0 * c2 - 1.0) * f;
+  }
+}
+
+__global__ void k_d2f(const double *s, float *d, long long n) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    d[e] = (float)s[e];
+}
+
+// ---------------------------------------------------------------------------
+// imaginary-time helpers
+// psi *= exp(-0.5 dt (V + g|psi|^2 + phi)); accumulate sum |psi|^2
+template <typename R, bool CPLX>
+__global__ void k_itp_pot(R *psi, const double *V, const R *phi, double g, double dt,
+                          long long S, double *norm) {
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < S;
+       i += (long long)gridDim.x * blockDim.x) {
+    double pr, pi = 0.0;
+    if (CPLX) {
+      pr = (double)psi[2 * i];
+      pi = (double)psi[2 * i + 1];
+    } else {
+      pr = (double)psi[i];
+    }
+    const double n = pr * pr + pi * pi;
+    double ve = V[i] + g * n;
+    if (phi) ve += (double)phi[i];
+    const double f = exp(-0.5 * dt * ve);
+    pr *= f;
+    pi *= f;
+    if (CPLX) {
+      psi[2 * i] = (R)pr;
+      psi[2 * i + 1] = (R)pi;
+    } else {
+      psi[i] = (R)pr;
+    }
+    s += pr * pr + pi * pi;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && norm) atomicAdd(norm, red[0]);
+}
+
+template <typename R>
+__global__ void k_scale_by_norm(R *psi, long long n, const double *norm, double dV) {
+  const double s = 1.0 / sqrt(norm[0] * dV);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    psi[i] = (R)((double)psi[i] * s);
+}
+
+// mu integrand: conj(psi) * (T psi + Veff psi), T psi supplied in tpsi
+template <bool CPLX>
+__global__ void k_itp_mu(const double *psi, const double *tpsi, const double *V,
+                         const double *phi, double g, long long S, double *acc) {
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < S;
+       i += (long long)gridDim.x * blockDim.x) {
+    double pr, pi = 0.0, tr, ti = 0.0;
+    if (CPLX) {
+      pr = psi[2 * i];
+      pi = psi[2 * i + 1];
+      tr = tpsi[2 * i];
+      ti = tpsi[2 * i + 1];
+    } else {
+      pr = psi[i];
+      tr = tpsi[i];
+    }
+    const double n = pr * pr + pi * pi;
+    double ve = V[i] + g * n;
+    if (phi) ve += phi[i];
+    s += pr * (tr + ve * pr) + pi * (ti + ve * pi);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+}
+
+}  // namespace bdgx
+
+using namespace bdgx;
+
+extern "C" {
+
+const char *bdgx_error_string(int code) {
+  switch (code) {
+    case BDGX_OK: return "ok";
+    case BDGX_EINVAL: return "invalid argument or unsupported shape";
+    case BDGX_EHIP: return hipGetErrorString(bdgx_last_hip_error);
+    case BDGX_ESTATE: return "required field not uploaded";
+    case BDGX_ENOMEM: return "out of memory";
+    default: return "unknown error";
+  }
+}
+
+int bdgx_ctx_create(bdgx_ctx **out, uint64_t dim, const uint64_t *N, const double *L,
+                    int complex_psi0) {
+  if (!out || dim < 1 || dim > 3 || !N || !L) return BDGX_EINVAL;
+  *out = nullptr;
+  bdgx_ctx *c = new bdgx_ctx();
+  memset(c, 0, sizeof(*c));
+  c->dim = (int)dim;
+  c->cplx = complex_psi0 ? 1 : 0;
+  for (int d = 0; d < 3; ++d) {
+    c->N[d] = d < (int)dim ? N[d] : 1;
+    c->L[d] = d < (int)dim ? L[d] : 1.0;
+    if (c->N[d] < 1 || c->N[d] > 4096) {
+      delete c;
+      return BDGX_EINVAL;
+    }
+  }
+  Geo &g = c->geo;
+  g.nx = (int)c->N[0];
+  g.ny = (int)c->N[1];
+  g.nz = (int)c->N[2];
+  g.n0k = c->cplx ? g.nx : g.nx / 2 + 1;
+  g.S = (long long)g.nx * g.ny * g.nz;
+  g.Kk = (long long)g.n0k * g.ny * g.nz;
+  for (int d = 0; d < 3; ++d)
+    if (make_plan((int)c->N[d], &c->plan[d])) {
+      delete c;
+      return BDGX_EINVAL;
+    }
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return BDGX_EHIP;
+  }
+  // k^2 tables (reference src/matmul_ctx.c:102-113) and twiddles
+  for (int d = 0; d < 3; ++d) {
+    const int n = (int)c->N[d];
+    std::vector<double> k2(n), twr(2 * n);
+    const double dk = 2.0 * M_PI / c->L[d];
+    for (int j = 0; j < n; ++j) {
+      const double kj = (d < (int)dim) ? ((j <= n / 2) ? dk * j : dk * (j - n)) : 0.0;
+      k2[j] = kj * kj;
+      const double a = -2.0 * M_PI * (double)j / (double)n;
+      twr[2 * j] = cos(a);
+      twr[2 * j + 1] = sin(a);
+    }
+    int rc = upload_both(c, &c->f64.kx2[d], &c->f32.kx2[d], k2.data(), n);
+    if (!rc)
+      rc = upload_both(c, reinterpret_cast<double **>(&c->f64.tw[d]),
+                       reinterpret_cast<float **>(&c->f32.tw[d]), twr.data(), 2 * n);
+    if (rc) {
+      bdgx_ctx_destroy(c);
+      return rc;
+    }
+  }
+  *out = c;
+  return BDGX_OK;
+}
+
+void bdgx_ctx_destroy(bdgx_ctx *c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  free_fields(c->f64);
+  free_fields(c->f32);
+  if (c->spec) (void)hipFree(c->spec);
+  if (c->scratch) (void)hipFree(c->scratch);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+uint64_t bdgx_ctx_size(const bdgx_ctx *c) { return (uint64_t)c->geo.S; }
+uint64_t bdgx_ctx_ksize(const bdgx_ctx *c) { return (uint64_t)c->geo.Kk; }
+
+int bdgx_set_fields(bdgx_ctx *c, const double *ltK, const double *ltM, const void *wf,
+                    const double *pK, const double *pM, double mu) {
+  if (!c) return BDGX_EINVAL;
+  BDGX_HIP(hipStreamSynchronize(c->stream));  // kernels in flight may read the fields
+  const size_t S = (size_t)c->geo.S;
+  int rc = 0;
+  if (ltK) rc = rc ? rc : upload_both(c, &c->f64.ltK, &c->f32.ltK, ltK, S);
+  if (ltM) rc = rc ? rc : upload_both(c, &c->f64.ltM, &c->f32.ltM, ltM, S);
+  if (pK) rc = rc ? rc : upload_both(c, &c->f64.pK, &c->f32.pK, pK, S);
+  if (pM) rc = rc ? rc : upload_both(c, &c->f64.pM, &c->f32.pM, pM, S);
+  if (wf)
+    rc = rc ? rc
+            : upload_both(c, &c->f64.wf, &c->f32.wf, (const double *)wf, S * (c->cplx ? 2 : 1));
+  if (rc) return rc;
+  if (ltK && ltM) c->has_lt = 1;
+  if (pK && pM) c->has_p = 1;
+  if (wf) c->has_wf = 1;
+  c->mu = mu;
+  return BDGX_OK;
+}
+
+int bdgx_dipolar_set_kernel(bdgx_ctx *c, double g_ddi, const double *dir, double R) {
+  if (!c || !dir || c->dim != 3) return BDGX_EINVAL;
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  const size_t K = (size_t)c->geo.Kk;
+  if (!c->f64.U) BDGX_HIP(hipMalloc((void **)&c->f64.U, sizeof(double) * K));
+  if (!c->f32.U) BDGX_HIP(hipMalloc((void **)&c->f32.U, sizeof(float) * K));
+  const Geo &g = c->geo;
+  hipLaunchKernelGGL(k_dipolar_kernel, dim3(1024), dim3(256), 0, c->stream, c->f64.U, g.nx,
+                     g.ny, g.nz, g.n0k, c->cplx, 2.0 * M_PI / c->L[0], 2.0 * M_PI / c->L[1],
+                     2.0 * M_PI / c->L[2], dir[0], dir[1], dir[2], R);
+  BDGX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_d2f, dim3(1024), dim3(256), 0, c->stream, c->f64.U, c->f32.U,
+                     (long long)K);
+  BDGX_HIP(hipGetLastError());
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  c->g_ddi = g_ddi;
+  c->has_U = 1;
+  return BDGX_OK;
+}
+
+int bdgx_dipolar_get_kernel(bdgx_ctx *c, double *h) {
+  if (!c || !c->has_U) return BDGX_ESTATE;
+  BDGX_HIP(hipMemcpyAsync(h, c->f64.U, sizeof(double) * c->geo.Kk, hipMemcpyDeviceToHost,
+                          c->stream));
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  return BDGX_OK;
+}
+
+int bdgx_dipolar_meanfield(bdgx_ctx *c, double *h) {
+  if (!c || !c->has_U || !c->has_wf) return BDGX_ESTATE;
+  double *d = nullptr;
+  BDGX_HIP(hipMalloc((void **)&d, sizeof(double) * c->geo.S));
+  int rc = meanfield_t<double>(c, c->f64.wf, d, c->g_ddi);
+  if (!rc) {
+    if (hipMemcpyAsync(h, d, sizeof(double) * c->geo.S, hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = BDGX_EHIP;
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+int bdgx_apply(bdgx_ctx *c, int op, int prec, const void *dX, void *dY, uint64_t ncols,
+               uint64_t ld) {
+  if (!c || !dX || !dY) return BDGX_EINVAL;
+  if (ncols == 0) return BDGX_OK;
+  const uint64_t need = (op == BDGX_OP_LREP || op == BDGX_OP_SWAP || op == BDGX_OP_PRECOND_LREP)
+                            ? 2 * (uint64_t)c->geo.S
+                            : (uint64_t)c->geo.S;
+  if (ld < need) return BDGX_EINVAL;
+  return apply_dispatch(c, op, prec, dX, dY, ncols, ld);
+}
+
+int bdgx_apply_host(bdgx_ctx *c, int op, const void *hX, void *hY, uint64_t ncols,
+                    uint64_t ld) {
+  if (!c || !hX || !hY) return BDGX_EINVAL;
+  const size_t bytes = sizeof(double) * (c->cplx ? 2 : 1) * (size_t)ld * ncols;
+  void *dX = nullptr, *dY = nullptr;
+  BDGX_HIP(hipMalloc(&dX, bytes));
+  if (hipMalloc(&dY, bytes) != hipSuccess) {
+    (void)hipFree(dX);
+    return BDGX_EHIP;
+  }
+  int rc = BDGX_OK;
+  if (hipMemcpyAsync(dX, hX, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = BDGX_EHIP;
+  if (!rc && hipMemsetAsync(dY, 0, bytes, c->stream) != hipSuccess) rc = BDGX_EHIP;
+  if (!rc) rc = bdgx_apply(c, op, BDGX_FP64, dX, dY, ncols, ld);
+  if (!rc && hipMemcpyAsync(hY, dY, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = BDGX_EHIP;
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = BDGX_EHIP;
+  (void)hipFree(dX);
+  (void)hipFree(dY);
+  return rc;
+}
+
+int bdgx_gram(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p, const void *dB,
+              uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b, double *dG) {
+  if (!c || !dA || !dB || !dG) return BDGX_EINVAL;
+  if (swap_b && (nrows & 1)) return BDGX_EINVAL;
+  return gram_dispatch(c, prec, dA, lda, p, dB, ldb, q, nrows, swap_b, dG);
+}
+
+int bdgx_combine(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
+                 const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows) {
+  if (!c || !dA || !dC || !dY) return BDGX_EINVAL;
+  return combine_dispatch(c, prec, dA, lda, k, dC, dY, ldy, m, nrows);
+}
+
+int bdgx_residual(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,
+                  uint64_t ld, uint64_t ncols, uint64_t nrows, const double *dlam, int swap_x,
+                  void *dR, uint64_t ldr, double *dn2) {
+  if (!c || !dAX || !dX || !dlam || !dn2) return BDGX_EINVAL;
+  if (swap_x && (nrows & 1)) return BDGX_EINVAL;
+  return residual_dispatch(c, prec, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2);
+}
+
+int bdgx_convert(bdgx_ctx *c, int ps, const void *src, uint64_t lds, int pd, void *dst,
+                 uint64_t ldd, uint64_t nrows, uint64_t ncols) {
+  if (!c) return BDGX_EINVAL;
+  return convert_dispatch(c, ps, src, lds, pd, dst, ldd, nrows, ncols);
+}
+
+int bdgx_itp_groundstate(bdgx_ctx *c, const double *omega, double g, double g_dd,
+                         const double *dir, double cutoff, double dt, uint64_t nsteps,
+                         void *h_psi, double *mu_out) {
+  if (!c || !omega || !h_psi) return BDGX_EINVAL;
+  if (g_dd != 0.0 && (c->dim != 3 || !dir)) return BDGX_EINVAL;
+  const Geo &G = c->geo;
+  const long long S = G.S;
+  const int W = c->cplx ? 2 : 1;
+  double dV = 1.0;
+  for (int d = 0; d < c->dim; ++d) dV *= c->L[d] / (double)c->N[d];
+  // harmonic trap on the grid of reference src/setup.c:60-81
+  std::vector<double> V((size_t)S);
+  for (int iz = 0; iz < G.nz; ++iz)
+    for (int iy = 0; iy < G.ny; ++iy)
+      for (int ix = 0; ix < G.nx; ++ix) {
+        const int idx[3] = {ix, iy, iz};
+        double v = 0.0;
+        for (int d = 0; d < c->dim; ++d) {
+          const double r = ((double)idx[d] - 0.5 * (double)c->N[d]) * c->L[d] / (double)c->N[d];
+          v += 0.5 * omega[d] * omega[d] * r * r;
+        }
+        V[((size_t)iz * G.ny + iy) * G.nx + ix] = v;
+      }
+  double *dV_ = nullptr, *dpsi = nullptr, *dphi = nullptr, *dnorm = nullptr, *dT = nullptr;
+  int rc = BDGX_OK;
+  const double saved_g = c->g_ddi;
+  const int saved_hasU = c->has_U;
+  auto cleanup = [&]() {
+    if (dV_) (void)hipFree(dV_);
+    if (dpsi) (void)hipFree(dpsi);
+    if (dphi) (void)hipFree(dphi);
+    if (dnorm) (void)hipFree(dnorm);
+    if (dT) (void)hipFree(dT);
+    c->g_ddi = saved_g;
+    c->has_U = saved_hasU;
+  };
+#define ITP_HIP(x)          \
+  do {                      \
+    if ((x) != hipSuccess) { \
+      cleanup();            \
+      return BDGX_EHIP;     \
+    }                       \
+  } while (0)
+  ITP_HIP(hipMalloc((void **)&dV_, sizeof(double) * S));
+  ITP_HIP(hipMalloc((void **)&dpsi, sizeof(double) * S * W));
+  ITP_HIP(hipMalloc((void **)&dphi, sizeof(double) * S));
+  ITP_HIP(hipMalloc((void **)&dnorm, sizeof(double) * 2));
+  ITP_HIP(hipMalloc((void **)&dT, sizeof(double) * S * W));
+  ITP_HIP(hipMemcpyAsync(dV_, V.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
+  ITP_HIP(hipMemcpyAsync(dpsi, h_psi, sizeof(double) * S * W, hipMemcpyHostToDevice, c->stream));
+  const bool dip = g_dd != 0.0;
+  if (dip) {
+    // kernel for the ITP's own dipole parameters (restored on exit)
+    double *save64 = c->f64.U;
+    float *save32 = c->f32.U;
+    c->f64.U = nullptr;
+    c->f32.U = nullptr;
+    rc = bdgx_dipolar_set_kernel(c, g_dd, dir, cutoff);
+    double *itpU64 = c->f64.U;
+    float *itpU32 = c->f32.U;
+    if (rc) {
+      c->f64.U = save64;
+      c->f32.U = save32;
+      cleanup();
+      return rc;
+    }
+    // run the loop with the ITP kernel, then restore the context's own kernel
+    auto potential = [&](double *nrm) -> int {
+      int r = meanfield_t<double>(c, dpsi, dphi, g_dd);
+      if (r) return r;
+      if (nrm && hipMemsetAsync(nrm, 0, sizeof(double), c->stream) != hipSuccess) return BDGX_EHIP;
+      if (c->cplx)
+        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+                           dV_, dphi, g, dt, S, nrm);
+      else
+        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream,
+                           dpsi, dV_, dphi, g, dt, S, nrm);
+      return hipGetLastError() == hipSuccess ? BDGX_OK : BDGX_EHIP;
+    };
+    {
+      // normalise the start vector
+      double *nrm = dnorm;
+      ITP_HIP(hipMemsetAsync(nrm, 0, sizeof(double), c->stream));
+      if (c->cplx)
+        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+                           dV_, (const double *)nullptr, 0.0, 0.0, S, nrm);
+      else
+        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream,
+                           dpsi, dV_, (const double *)nullptr, 0.0, 0.0, S, nrm);
+      hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(1024), dim3(256), 0, c->stream, dpsi,
+                         S * W, (const double *)nrm, dV);
+    }
+    for (uint64_t st = 0; st < nsteps && !rc; ++st) {
+      rc = potential(nullptr);
+      if (!rc) rc = itp_kinetic_t<double>(c, dpsi, dt);
+      if (!rc) rc = potential(dnorm);
+      if (!rc)
+        hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(1024), dim3(256), 0, c->stream, dpsi,
+                           S * W, (const double *)dnorm, dV);
+    }
+    if (!rc) rc = meanfield_t<double>(c, dpsi, dphi, g_dd);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(itpU64);
+    (void)hipFree(itpU32);
+    c->f64.U = save64;
+    c->f32.U = save32;
+  } else {
+    ITP_HIP(hipMemsetAsync(dnorm, 0, sizeof(double), c->stream));
+    if (c->cplx)
+      hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+                         dV_, (const double *)nullptr, 0.0, 0.0, S, dnorm);
+    else
+      hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+                         dV_, (const double *)nullptr, 0.0, 0.0, S, dnorm);
+    hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(1024), dim3(256), 0, c->stream, dpsi,
+                       S * W, (const double *)dnorm, dV);
+    for (uint64_t st = 0; st < nsteps && !rc; ++st) {
+      if (c->cplx)
+        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+                           dV_, (const double *)nullptr, g, dt, S, (double *)nullptr);
+      else
+        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream,
+                           dpsi, dV_, (const double *)nullptr, g, dt, S, (double *)nullptr);
+      rc = itp_kinetic_t<double>(c, dpsi, dt);
+      if (rc) break;
+      ITP_HIP(hipMemsetAsync(dnorm, 0, sizeof(double), c->stream));
+      if (c->cplx)
+        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+                           dV_, (const double *)nullptr, g, dt, S, dnorm);
+      else
+        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream,
+                           dpsi, dV_, (const double *)nullptr, g, dt, S, dnorm);
+      hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(1024), dim3(256), 0, c->stream, dpsi,
+                         S * W, (const double *)dnorm, dV);
+    }
+    ITP_HIP(hipMemsetAsync(dphi, 0, sizeof(double) * S, c->stream));
+  }
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  // mu = sum conj(psi) (T psi + Veff psi) dV
+  ITP_HIP(hipMemcpyAsync(dT, dpsi, sizeof(double) * S * W, hipMemcpyDeviceToDevice, c->stream));
+  rc = itp_kinetic_t<double>(c, dT, -1.0);  // dt < 0 selects the plain k^2/2 multiplier
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  ITP_HIP(hipMemsetAsync(dnorm + 1, 0, sizeof(double), c->stream));
+  if (c->cplx)
+    hipLaunchKernelGGL(k_itp_mu<true>, dim3(1024), dim3(256), 0, c->stream, dpsi, dT, dV_,
+                       dip ? dphi : nullptr, g, S, dnorm + 1);
+  else
+    hipLaunchKernelGGL(k_itp_mu<false>, dim3(1024), dim3(256), 0, c->stream, dpsi, dT, dV_,
+                       dip ? dphi : nullptr, g, S, dnorm + 1);
+  double acc = 0.0;
+  ITP_HIP(hipMemcpyAsync(&acc, dnorm + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ITP_HIP(hipMemcpyAsync(h_psi, dpsi, sizeof(double) * S * W, hipMemcpyDeviceToHost, c->stream));
+  ITP_HIP(hipStreamSynchronize(c->stream));
+#undef ITP_HIP
+  if (mu_out) *mu_out = acc * dV;
+  cleanup();
+  return BDGX_OK;
+}
+
+int bdgx_malloc(void **p, size_t bytes) {
+  if (!p) return BDGX_EINVAL;
+  BDGX_HIP(hipMalloc(p, bytes ? bytes : 1));
+  return BDGX_OK;
+}
+int bdgx_free(void *p) {
+  if (p) BDGX_HIP(hipFree(p));
+  return BDGX_OK;
+}
+int bdgx_memcpy_h2d(bdgx_ctx *c, void *d, const void *s, size_t n) {
+  BDGX_HIP(hipMemcpyAsync(d, s, n, hipMemcpyHostToDevice, c->stream));
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  return BDGX_OK;
+}
+int bdgx_memcpy_d2h(bdgx_ctx *c, void *d, const void *s, size_t n) {
+  BDGX_HIP(hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, c->stream));
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  return BDGX_OK;
+}
+int bdgx_memset(bdgx_ctx *c, void *d, int v, size_t n) {
+  BDGX_HIP(hipMemsetAsync(d, v, n, c->stream));
+  return BDGX_OK;
+}
+int bdgx_sync(bdgx_ctx *c) {
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  return BDGX_OK;
+}
+void *bdgx_stream(bdgx_ctx *c) { return (void *)c->stream; }
+int bdgx_timer_begin(bdgx_ctx *c) {
+  BDGX_HIP(hipEventRecord(c->ev0, c->stream));
+  return BDGX_OK;
+}
+int bdgx_timer_end(bdgx_ctx *c, float *ms) {
+  BDGX_HIP(hipEventRecord(c->ev1, c->stream));
+  BDGX_HIP(hipEventSynchronize(c->ev1));
+  BDGX_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return BDGX_OK;
+}
+uint64_t bdgx_workspace_bytes(const bdgx_ctx *c) {
+  return (uint64_t)(c->spec_bytes + c->scratch_bytes);
+}
+
+}  // extern "C"

@@ -1,0 +1,614 @@
+/*
+ * Host implementation of the public BdG API (include/bdg/bdg.h).
+ *
+ * Lifecycle, setup state machine and result ownership follow the reference
+ * (src/bdg.c, src/setup.c, src/solver.c); the heavy work runs on the GPU
+ * through the bdgx C ABI: the dipolar kernel and mean field at setup, and
+ * the whole eigen-iteration in bdg_solve.
+ */
+#include <complex.h>
+#include <inttypes.h>
+
+#include "bdg_host.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+static void *xcalloc(size_t n, size_t s) {
+  void *p = calloc(n ? n : 1, s ? s : 1);
+  if (!p) BDG_FAIL("out of host memory (%zu x %zu bytes)", n, s);
+  return p;
+}
+
+static void freep(void **p) {
+  free(*p);
+  *p = NULL;
+}
+
+/* ---- lifecycle (reference src/bdg.c:17-84) ------------------------------ */
+
+bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0) {
+  if (dim < 1 || dim > 3 || !N || !L) BDG_FAIL("bdg_alloc: invalid grid (dim %" PRIu64 ")", dim);
+  bdg_t *b = (bdg_t *)xcalloc(1, sizeof(bdg_t));
+  b->dim = dim;
+  b->complex_psi0 = complex_psi0 ? 1 : 0;
+  b->size = 1;
+  for (int d = 0; d < 3; ++d) {
+    b->N[d] = d < (int)dim ? N[d] : 1;
+    b->L[d] = d < (int)dim ? L[d] : 1.0;
+  }
+  for (uint64_t d = 0; d < dim; ++d) b->size *= N[d];
+  b->ltK = (double *)xcalloc(b->size, sizeof(double));
+  b->ltM = (double *)xcalloc(b->size, sizeof(double));
+  b->nev = 3;
+  b->sizeSub = 6;
+  b->maxIter = 500;
+  b->tol = 1e-6;
+  b->precision = BDG_PREC_FP64;
+  b->tol_fp32 = 1e-5;
+  b->maxIter_fp32 = 0;
+  return b;
+}
+
+static void free_results(bdg_t *b) {
+  freep((void **)&b->eigvals);
+  freep(&b->modes_u);
+  freep(&b->modes_v);
+  freep((void **)&b->residuals);
+  freep((void **)&b->zero_modes);
+  b->converged = 0;
+}
+
+void bdg_free(bdg_t **pb) {
+  if (!pb || !*pb) return;
+  bdg_t *b = *pb;
+  free_results(b);
+  freep(&b->reuse_buf);
+  for (int d = 0; d < 3; ++d) freep((void **)&b->kx2[d]);
+  freep((void **)&b->ltK);
+  freep((void **)&b->ltM);
+  freep((void **)&b->pK);
+  freep((void **)&b->pM);
+  freep(&b->wf);
+  if (b->dev) bdgx_ctx_destroy(b->dev);
+  free(b);
+  *pb = NULL;
+}
+
+void bdg_reset(bdg_t *b) {
+  BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_reset");
+  freep(&b->wf);
+  freep((void **)&b->pK);
+  freep((void **)&b->pM);
+  memset(b->ltK, 0, sizeof(double) * b->size);
+  memset(b->ltM, 0, sizeof(double) * b->size);
+  b->mu = 0.0;
+  b->g_ddi = 0.0;
+  b->dipolar = 0;
+  free_results(b);
+  /* the device context keeps grid, plans and workspaces; its physics fields
+   * are re-uploaded by the next bdg_solve / bdg_set_dipolar */
+  if (b->dev) {
+    bdgx_ctx_destroy(b->dev);
+    b->dev = NULL;
+    if (bdgx_ctx_create(&b->dev, b->dim, b->N, b->L, b->complex_psi0) != BDGX_OK)
+      BDG_FAIL("bdg_reset: device context re-creation failed");
+  }
+  b->state = BDG_HAS_SYSTEM;
+}
+
+/* ---- setup ---------------------------------------------------------------- */
+
+void bdg_set_system(bdg_t *b) {
+  /* k-tables (reference src/matmul_ctx.c:102-113) */
+  b->k_size = b->complex_psi0 ? b->size : (b->N[0] / 2 + 1) * b->N[1] * b->N[2];
+  for (int d = 0; d < 3; ++d) {
+    freep((void **)&b->kx2[d]);
+    b->kx2[d] = (double *)xcalloc(b->N[d], sizeof(double));
+    if (d >= (int)b->dim) continue;
+    const double dk = 2.0 * M_PI / b->L[d];
+    for (uint64_t j = 0; j < b->N[d]; ++j) {
+      const double kj = (j <= b->N[d] / 2) ? dk * (double)j : dk * ((double)j - (double)b->N[d]);
+      b->kx2[d][j] = kj * kj;
+    }
+  }
+  if (!b->dev) {
+    const int rc = bdgx_ctx_create(&b->dev, b->dim, b->N, b->L, b->complex_psi0);
+    if (rc) BDG_FAIL("bdg_set_system: device context failed: %s", bdgx_error_string(rc));
+  }
+  b->state |= BDG_HAS_SYSTEM;
+}
+
+/* reference src/setup.c:17-84: r_d = (i - N_d/2) L_d / N_d, additive */
+void bdg_set_trap(bdg_t *b, f64 (*V)(uint64_t, const f64 *, const void *), const void *param) {
+  BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_set_trap");
+  if (!V) {
+    fprintf(stderr, "WARN bdg_set_trap: V_trap is NULL, skipping\n");
+    return;
+  }
+  for (uint64_t iz = 0; iz < b->N[2]; ++iz)
+    for (uint64_t iy = 0; iy < b->N[1]; ++iy)
+      for (uint64_t ix = 0; ix < b->N[0]; ++ix) {
+        const uint64_t idx[3] = {ix, iy, iz};
+        f64 r[3];
+        for (uint64_t d = 0; d < b->dim; ++d)
+          r[d] = ((double)idx[d] - 0.5 * (double)b->N[d]) * b->L[d] / (double)b->N[d];
+        const uint64_t i = (iz * b->N[1] + iy) * b->N[0] + ix;
+        const double v = V(b->dim, r, param);
+        b->ltK[i] += v;
+        b->ltM[i] += v;
+      }
+  b->state |= BDG_HAS_TRAP;
+}
+
+/* reference src/setup.c:89-114 (copies) */
+void bdg_set_wavefunction(bdg_t *b, const void *wf, uint64_t wf_size) {
+  BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_set_wavefunction");
+  if (!wf) BDG_FAIL("bdg_set_wavefunction: wf is NULL");
+  if (wf_size != b->size)
+    BDG_FAIL("bdg_set_wavefunction: wf_size (%" PRIu64 ") != grid size (%" PRIu64 ")", wf_size,
+             b->size);
+  freep(&b->wf);
+  const size_t bytes = b->size * sizeof(double) * (b->complex_psi0 ? 2 : 1);
+  b->wf = xcalloc(1, bytes);
+  memcpy(b->wf, wf, bytes);
+  b->state |= BDG_HAS_WF;
+}
+
+static double density_at(const bdg_t *b, uint64_t i) {
+  if (b->complex_psi0) {
+    const double *w = (const double *)b->wf;
+    return w[2 * i] * w[2 * i] + w[2 * i + 1] * w[2 * i + 1];
+  }
+  const double p = ((const double *)b->wf)[i];
+  return p * p;
+}
+
+/* reference src/setup.c:119-145: K += U_K(n), M += U_K(n) + 2 U_M(n) */
+void bdg_set_local_interactions(bdg_t *b, f64 (*UK)(const void *, f64),
+                                f64 (*UM)(const void *, f64), const void *param) {
+  BDG_REQUIRE(b, BDG_HAS_WF, "bdg_set_local_interactions");
+  BDG_FORBID(b, BDG_HAS_INTERACTIONS, "bdg_set_local_interactions");
+  for (uint64_t i = 0; i < b->size; ++i) {
+    const double n = density_at(b, i);
+    const double uK = UK(param, n), uM = UM(param, n);
+    b->ltK[i] += uK;
+    b->ltM[i] += uK + 2.0 * uM;
+  }
+  b->state |= BDG_HAS_INTERACTIONS;
+}
+
+/* reference src/setup.c:150-177: p = 1/sqrt(max(1e-8, lt)) before lt -= mu */
+void bdg_set_mu(bdg_t *b, f64 mu) {
+  BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_set_mu");
+  if (!b->pK) b->pK = (double *)xcalloc(b->size, sizeof(double));
+  if (!b->pM) b->pM = (double *)xcalloc(b->size, sizeof(double));
+  for (uint64_t i = 0; i < b->size; ++i) {
+    b->pK[i] = 1.0 / sqrt(fmax(1e-8, b->ltK[i]));
+    b->pM[i] = 1.0 / sqrt(fmax(1e-8, b->ltM[i]));
+  }
+  for (uint64_t i = 0; i < b->size; ++i) {
+    b->ltK[i] -= mu;
+    b->ltM[i] -= mu;
+  }
+  b->mu = mu;
+  b->state |= BDG_HAS_MU;
+}
+
+/* reference src/setup.c:182-197 + src/dipolar.c: kernel and mean field, here
+ * computed on the device */
+void bdg_set_dipolar(bdg_t *b, f64 g_ddi, const f64 *dir, f64 cutoff) {
+  BDG_REQUIRE(b, BDG_HAS_WF, "bdg_set_dipolar");
+  BDG_FORBID(b, BDG_HAS_MU, "bdg_set_dipolar");
+  if (3 != b->dim)
+    BDG_FAIL("bdg_set_dipolar: dipolar interactions require dim == 3 (got %" PRIu64 ")", b->dim);
+  int rc = bdgx_set_fields(b->dev, NULL, NULL, b->wf, NULL, NULL, 0.0);
+  if (!rc) rc = bdgx_dipolar_set_kernel(b->dev, g_ddi, dir, cutoff);
+  double *gphi = (double *)xcalloc(b->size, sizeof(double));
+  if (!rc) rc = bdgx_dipolar_meanfield(b->dev, gphi);
+  if (rc) BDG_FAIL("bdg_set_dipolar: device error: %s", bdgx_error_string(rc));
+  for (uint64_t i = 0; i < b->size; ++i) {
+    b->ltK[i] += gphi[i];
+    b->ltM[i] += gphi[i];
+  }
+  free(gphi);
+  b->g_ddi = g_ddi;
+  b->dipolar = 1;
+  b->state |= BDG_HAS_DIPOLAR;
+}
+
+void bdg_set_solver_params(bdg_t *b, uint64_t nev, uint64_t sizeSub, uint64_t maxIter, f64 tol) {
+  b->nev = nev;
+  b->sizeSub = sizeSub;
+  b->maxIter = maxIter;
+  b->tol = tol;
+}
+
+void bdg_set_init_mode(bdg_t *b, bdg_init_mode_t mode, bdg_init_fn fn, void *param) {
+  b->init_mode = mode;
+  b->custom_init_fn = fn;
+  b->custom_init_param = param;
+}
+
+/* reference src/bdg.c:112-207: modes + gaussian noise (seed 12345, Box-Muller);
+ * extra columns |gaussian| * |psi0| */
+int bdg_reuse_modes(bdg_t *b, f64 noise_frac) {
+  if (!b->modes_u || !b->modes_v) return -3;
+  const uint64_t S = b->size, n = 2 * S, nev = b->nev, m = b->sizeSub;
+  const int c = b->complex_psi0;
+  freep(&b->reuse_buf);
+  b->reuse_buf = xcalloc(n * m, sizeof(double) * (c ? 2 : 1));
+  b->reuse_n = n;
+  b->reuse_cols = m;
+  uint32_t seed = 12345;
+  double *buf = (double *)b->reuse_buf;
+  const double *u = (const double *)b->modes_u, *v = (const double *)b->modes_v;
+  for (uint64_t j = 0; j < nev && j < m; ++j) {
+    double cn = 0.0;
+    for (uint64_t i = 0; i < S; ++i) {
+      if (c) {
+        const double *uu = u + 2 * (j * S + i), *vv = v + 2 * (j * S + i);
+        cn += uu[0] * uu[0] + uu[1] * uu[1] + vv[0] * vv[0] + vv[1] * vv[1];
+      } else {
+        cn += u[j * S + i] * u[j * S + i] + v[j * S + i] * v[j * S + i];
+      }
+    }
+    const double sigma = noise_frac * sqrt(cn);
+    for (uint64_t i = 0; i < S; ++i) {
+      double u1 = bdg_xrand(&seed);
+      const double u2 = bdg_xrand(&seed);
+      u1 = fmax(u1, 1e-10);
+      const double z0 = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+      const double z1 = sqrt(-2.0 * log(u1)) * sin(2.0 * M_PI * u2);
+      if (c) {
+        buf[2 * (i + j * n)] = u[2 * (j * S + i)] + sigma * z0;
+        buf[2 * (i + j * n) + 1] = u[2 * (j * S + i) + 1];
+        buf[2 * (i + S + j * n)] = v[2 * (j * S + i)] + sigma * z1;
+        buf[2 * (i + S + j * n) + 1] = v[2 * (j * S + i) + 1];
+      } else {
+        buf[i + j * n] = u[j * S + i] + sigma * z0;
+        buf[i + S + j * n] = v[j * S + i] + sigma * z1;
+      }
+    }
+  }
+  for (uint64_t j = nev; j < m; ++j)
+    for (uint64_t i = 0; i < S; ++i) {
+      double u1 = bdg_xrand(&seed);
+      const double u2 = bdg_xrand(&seed);
+      u1 = fmax(u1, 1e-10);
+      const double wf = b->wf ? sqrt(density_at(b, i)) : 1.0;
+      const double val = fabs(sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2)) * wf;
+      if (c) {
+        buf[2 * (i + j * n)] = val;
+        buf[2 * (i + j * n) + 1] = 0.0;
+        buf[2 * (i + S + j * n)] = val;
+        buf[2 * (i + S + j * n) + 1] = 0.0;
+      } else {
+        buf[i + j * n] = val;
+        buf[i + S + j * n] = val;
+      }
+    }
+  b->init_mode = BDG_INIT_REUSE;
+  return 0;
+}
+
+/* ---- start blocks (reference src/solver.c:79-259, 322-414) -------------- */
+
+typedef struct {
+  double k2;
+  uint64_t idx[3];
+} kvec_t;
+
+static int kvec_cmp(const void *a, const void *b) {
+  const double x = ((const kvec_t *)a)->k2, y = ((const kvec_t *)b)->k2;
+  return (x > y) - (x < y);
+}
+
+static kvec_t *lowest_kvecs(const bdg_t *b, bdg_geom_hint_t hint, uint64_t need, uint64_t *nout) {
+  uint64_t range[3] = {1, 1, 1};
+  if (hint == BDG_GEOM_ELONGATED) {
+    uint64_t ld = 0;
+    for (uint64_t d = 1; d < b->dim; ++d)
+      if (b->L[d] > b->L[ld]) ld = d;
+    range[ld] = b->N[ld] < 2 * need ? b->N[ld] : 2 * need;
+  } else if (hint == BDG_GEOM_RING) {
+    for (uint64_t d = 0; d < b->dim && d < 2; ++d) range[d] = b->N[d] < 2 * need ? b->N[d] : 2 * need;
+  } else {
+    for (uint64_t d = 0; d < b->dim; ++d) range[d] = b->N[d] < 2 * need ? b->N[d] : 2 * need;
+  }
+  const uint64_t total = range[0] * range[1] * range[2];
+  kvec_t *e = (kvec_t *)xcalloc(total, sizeof(kvec_t));
+  uint64_t cnt = 0;
+  for (uint64_t iz = 0; iz < range[2]; ++iz)
+    for (uint64_t iy = 0; iy < range[1]; ++iy)
+      for (uint64_t ix = 0; ix < range[0]; ++ix) {
+        e[cnt].k2 = b->kx2[0][ix] + (b->dim > 1 ? b->kx2[1][iy] : 0.0) +
+                    (b->dim > 2 ? b->kx2[2][iz] : 0.0);
+        e[cnt].idx[0] = ix;
+        e[cnt].idx[1] = iy;
+        e[cnt].idx[2] = iz;
+        ++cnt;
+      }
+  qsort(e, cnt, sizeof(kvec_t), kvec_cmp);
+  *nout = cnt < need ? cnt : need;
+  return e;
+}
+
+static double wf_abs(const bdg_t *b, uint64_t i) { return b->wf ? sqrt(density_at(b, i)) : 1.0; }
+
+static void put(const bdg_t *b, void *X, uint64_t row, uint64_t col, double val) {
+  const uint64_t n = 2 * b->size;
+  if (b->complex_psi0) {
+    double *x = (double *)X;
+    x[2 * (row + col * n)] = val;
+    x[2 * (row + col * n) + 1] = 0.0;
+  } else {
+    ((double *)X)[row + col * n] = val;
+  }
+}
+
+void bdg_fill_start_block(bdg_t *b, void *X) {
+  const uint64_t S = b->size, n = 2 * S, m = b->sizeSub;
+  const size_t es = sizeof(double) * (b->complex_psi0 ? 2 : 1);
+  switch (b->init_mode) {
+    case BDG_INIT_REUSE:
+      if (b->reuse_buf && b->reuse_n == n && b->reuse_cols == m) {
+        memcpy(X, b->reuse_buf, n * m * es);
+        freep(&b->reuse_buf);
+        b->init_mode = BDG_INIT_WF_WEIGHTED;
+        return;
+      }
+      /* fall through */
+    case BDG_INIT_WF_WEIGHTED: {
+      /* |gaussian| * |psi0|, u = v.  Both variants use xorshift32 seed 42 here
+       * (the reference's real variant seeds rand_r with time(NULL),
+       * src/solver.c:178-182, which cannot be reproduced). */
+      uint32_t seed = 42;
+      for (uint64_t j = 0; j < m; ++j)
+        for (uint64_t i = 0; i < S; ++i) {
+          double u1 = bdg_xrand(&seed);
+          const double u2 = bdg_xrand(&seed);
+          u1 = fmax(u1, 1e-10);
+          const double val = fabs(sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2)) * wf_abs(b, i);
+          put(b, X, i, j, val);
+          put(b, X, i + S, j, val);
+        }
+      return;
+    }
+    case BDG_INIT_CUSTOM:
+      if (b->custom_init_fn) b->custom_init_fn(b, X, n, m, b->custom_init_param);
+      return;
+    case BDG_INIT_PLANEWAVE:
+    default: {
+      uint32_t seed = 42;
+      const bdg_geom_hint_t hint =
+          b->custom_init_param ? (bdg_geom_hint_t)(intptr_t)b->custom_init_param : BDG_GEOM_AUTO;
+      const uint64_t need = (m + 1) / 2;
+      uint64_t nk = 0;
+      kvec_t *kv = lowest_kvecs(b, hint, need, &nk);
+      for (uint64_t j = 0; j < m; ++j) {
+        const uint64_t kidx = (j + 1) / 2;
+        const int use_sin = (j % 2) == 1;
+        if (kidx >= nk) {
+          for (uint64_t i = 0; i < S; ++i) {
+            const double val = (bdg_xrand(&seed) - 0.5) * wf_abs(b, i);
+            put(b, X, i, j, val);
+            put(b, X, i + S, j, val);
+          }
+          continue;
+        }
+        double freq[3] = {0, 0, 0};
+        for (uint64_t d = 0; d < b->dim; ++d) {
+          const int64_t f = kv[kidx].idx[d] <= b->N[d] / 2 ? (int64_t)kv[kidx].idx[d]
+                                                           : (int64_t)kv[kidx].idx[d] - (int64_t)b->N[d];
+          freq[d] = 2.0 * M_PI * (double)f / b->L[d];
+        }
+        for (uint64_t i = 0; i < S; ++i) {
+          const uint64_t id[3] = {i % b->N[0], (i / b->N[0]) % b->N[1], i / (b->N[0] * b->N[1])};
+          double kr = 0.0;
+          for (uint64_t d = 0; d < b->dim; ++d) kr += freq[d] * (double)id[d] * b->L[d] / (double)b->N[d];
+          const double pw = use_sin ? sin(kr) : cos(kr);
+          const double pert = 1e-4 * (bdg_xrand(&seed) - 0.5);
+          const double val = (pw + pert) * wf_abs(b, i);
+          put(b, X, i, j, val);
+          put(b, X, i + S, j, val);
+        }
+      }
+      free(kv);
+      return;
+    }
+  }
+}
+
+/* ---- solve ---------------------------------------------------------------- */
+
+static int upload_fields(bdg_t *b) {
+  int rc = bdgx_set_fields(b->dev, b->ltK, b->ltM, b->wf, b->pK, b->pM, b->mu);
+  if (!rc && b->dipolar) rc = BDGX_OK; /* kernel already on the device */
+  return rc;
+}
+
+int bdg_solve(bdg_t *b) {
+  BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_solve");
+  BDG_REQUIRE(b, BDG_HAS_MU, "bdg_solve");
+  const double t_start = bdg_wtime();
+  free_results(b);
+  memset(&b->stats, 0, sizeof(b->stats));
+  const uint64_t S = b->size, n = 2 * S, m = b->sizeSub;
+  uint64_t nev = b->nev;
+  if (m < nev || nev == 0) {
+    b->stats.status = BDGX_EINVAL;
+    return BDGX_EINVAL;
+  }
+  const int c = b->complex_psi0;
+  const size_t es64 = sizeof(double) * (c ? 2 : 1);
+  int rc = upload_fields(b);
+  if (rc) {
+    b->stats.status = rc;
+    return rc;
+  }
+  /* start block [u; v] on the host, v half to the device */
+  void *X = xcalloc(n * m, es64);
+  bdg_fill_start_block(b, X);
+  void *dV0 = NULL;
+  rc = bdgx_malloc(&dV0, S * m * es64);
+  for (uint64_t j = 0; j < m && !rc; ++j)
+    rc = bdgx_memcpy_h2d(b->dev, (char *)dV0 + j * S * es64, (const char *)X + (j * n + S) * es64,
+                         S * es64);
+  free(X);
+  if (rc) {
+    bdgx_free(dV0);
+    b->stats.status = rc;
+    return rc;
+  }
+  lobpcg_cfg cfg;
+  cfg.S = S;
+  cfg.m = (int)m;
+  cfg.nev = (int)nev;
+  cfg.cplx = c;
+  cfg.tol = b->tol;
+  cfg.zero_tol = 1e-3;
+  cfg.verbosity = b->verbosity;
+  lobpcg_result r;
+  memset(&r, 0, sizeof(r));
+  r.w2 = (double *)xcalloc(m, sizeof(double));
+  r.zflag = (int *)xcalloc(m, sizeof(int));
+  r.res = (double *)xcalloc(m, sizeof(double));
+  void *dX = NULL, *dMX = NULL;
+  int prec_final = BDGX_FP64;
+  if (b->precision == BDG_PREC_MIXED) {
+    /* FP32 bulk phase */
+    void *dV32 = NULL;
+    rc = bdgx_malloc(&dV32, S * m * (es64 / 2));
+    if (!rc) rc = bdgx_convert(b->dev, BDGX_FP64, dV0, S, BDGX_FP32, dV32, S, S, m);
+    lobpcg_cfg c32 = cfg;
+    c32.tol = fmax(b->tol, b->tol_fp32);
+    c32.maxiter = (int)(b->maxIter_fp32 ? b->maxIter_fp32 : b->maxIter);
+    const double t0 = bdg_wtime();
+    void *dX32 = NULL, *dMX32 = NULL;
+    if (!rc) rc = bdg_lobpcg_device(b->dev, BDGX_FP32, &c32, dV32, &r, &dX32, &dMX32);
+    b->stats.seconds_fp32 = bdg_wtime() - t0;
+    b->stats.iterations_fp32 = (uint64_t)r.iters;
+    b->stats.column_applies += r.applies;
+    b->stats.restarts += (uint64_t)r.restarts;
+    b->stats.seconds_rr += r.rr_seconds;
+    if (!rc) rc = bdgx_convert(b->dev, BDGX_FP32, dX32, S, BDGX_FP64, dV0, S, S, m);
+    if (!rc) rc = bdgx_sync(b->dev);
+    bdgx_free(dV32);
+    bdgx_free(dX32);
+    bdgx_free(dMX32);
+    if (b->verbosity > 0)
+      fprintf(stderr, "[bdg] fp32 phase: %d iterations, %.3f s, status %d\n", r.iters,
+              b->stats.seconds_fp32, rc);
+  }
+  if (!rc) {
+    cfg.maxiter = (int)b->maxIter;
+    const double t0 = bdg_wtime();
+    rc = bdg_lobpcg_device(b->dev, BDGX_FP64, &cfg, dV0, &r, &dX, &dMX);
+    b->stats.seconds_fp64 = bdg_wtime() - t0;
+    b->stats.iterations_fp64 = (uint64_t)r.iters;
+    b->stats.column_applies += r.applies;
+    b->stats.restarts += (uint64_t)r.restarts;
+    b->stats.seconds_rr += r.rr_seconds;
+    if (b->verbosity > 0)
+      fprintf(stderr, "[bdg] fp64 phase: %d iterations, %.3f s, status %d\n", r.iters,
+              b->stats.seconds_fp64, rc);
+  }
+  (void)prec_final;
+  bdgx_free(dV0);
+  b->stats.iterations = b->stats.iterations_fp32 + b->stats.iterations_fp64;
+  if (rc) {
+    bdgx_free(dX);
+    bdgx_free(dMX);
+    free(r.w2);
+    free(r.zflag);
+    free(r.res);
+    b->stats.status = rc;
+    b->stats.seconds_total = bdg_wtime() - t_start;
+    return rc;
+  }
+  /* results: w = signed sqrt(w2), v = X, u = M X / w, ||[u; v]|| = 1 */
+  double *hX = (double *)xcalloc(S * m, es64);
+  double *hMX = (double *)xcalloc(S * m, es64);
+  rc = bdgx_memcpy_d2h(b->dev, hX, dX, S * m * es64);
+  if (!rc) rc = bdgx_memcpy_d2h(b->dev, hMX, dMX, S * m * es64);
+  bdgx_free(dX);
+  bdgx_free(dMX);
+  b->eigvals = (double *)xcalloc(nev, sizeof(double));
+  b->residuals = (double *)xcalloc(nev, sizeof(double));
+  b->zero_modes = (int *)xcalloc(nev, sizeof(int));
+  b->modes_u = xcalloc(S * nev, es64);
+  b->modes_v = xcalloc(S * nev, es64);
+  double scale = 1.0;
+  for (uint64_t j = 0; j < nev; ++j) {
+    const double w2 = r.w2[j];
+    b->eigvals[j] = w2 >= 0 ? sqrt(w2) : -sqrt(-w2);
+    scale = fmax(scale, fabs(b->eigvals[j]));
+  }
+  uint64_t conv = 0;
+  const int W = c ? 2 : 1;
+  for (uint64_t j = 0; j < nev; ++j) {
+    const double w = b->eigvals[j];
+    b->residuals[j] = r.res[j];
+    b->zero_modes[j] = r.zflag[j] || fabs(w) < 1e-3 * scale;
+    if (b->zero_modes[j] || r.res[j] < b->tol) ++conv;
+    const double inv = fabs(w) > 0 ? 1.0 / w : 1.0;
+    double nrm = 0.0;
+    for (uint64_t i = 0; i < S * W; ++i) {
+      const double uu = hMX[j * S * W + i] * inv, vv = hX[j * S * W + i];
+      nrm += uu * uu + vv * vv;
+    }
+    nrm = nrm > 0 ? 1.0 / sqrt(nrm) : 1.0;
+    double *mu = (double *)b->modes_u + j * S * W, *mv = (double *)b->modes_v + j * S * W;
+    for (uint64_t i = 0; i < S * W; ++i) {
+      mu[i] = hMX[j * S * W + i] * inv * nrm;
+      mv[i] = hX[j * S * W + i] * nrm;
+    }
+  }
+  b->converged = conv;
+  free(hX);
+  free(hMX);
+  free(r.w2);
+  free(r.zflag);
+  free(r.res);
+  b->stats.status = rc;
+  b->stats.seconds_total = bdg_wtime() - t_start;
+  return rc;
+}
+
+/* ---- results ---------------------------------------------------------------- */
+
+uint64_t bdg_converged(const bdg_t *b) { return b->converged; }
+const f64 *bdg_eigenvalues(const bdg_t *b) { return b->eigvals; }
+const void *bdg_modes_u(const bdg_t *b) { return b->modes_u; }
+const void *bdg_modes_v(const bdg_t *b) { return b->modes_v; }
+
+/* ---- extensions (include/bdg/bdg_amd.h) ------------------------------------ */
+
+void bdg_set_precision(bdg_t *b, bdg_precision_t p) { b->precision = (int)p; }
+void bdg_get_solve_stats(const bdg_t *b, bdg_solve_stats_t *out) {
+  if (out) *out = b->stats;
+}
+const f64 *bdg_residuals(const bdg_t *b) { return b->residuals; }
+const int *bdg_zero_modes(const bdg_t *b) { return b->zero_modes; }
+bdgx_ctx *bdg_device_context(bdg_t *b) { return b->dev; }
+const f64 *bdg_local_term_K(const bdg_t *b) { return b->ltK; }
+const f64 *bdg_local_term_M(const bdg_t *b) { return b->ltM; }
+const f64 *bdg_precond_weight_K(const bdg_t *b) { return b->pK; }
+const f64 *bdg_precond_weight_M(const bdg_t *b) { return b->pM; }
+f64 bdg_mu(const bdg_t *b) { return b->mu; }
+void bdg_set_mixed_params(bdg_t *b, f64 tol32, uint64_t maxit32) {
+  b->tol_fp32 = tol32;
+  b->maxIter_fp32 = maxit32;
+}
+void bdg_set_verbosity(bdg_t *b, int level) { b->verbosity = level; }
+
+f64 bdg_cb_harmonic_trap(uint64_t dim, const f64 *r, const void *param) {
+  const f64 *om = (const f64 *)param;
+  double v = 0.0;
+  for (uint64_t d = 0; d < dim; ++d) v += 0.5 * om[d] * om[d] * r[d] * r[d];
+  return v;
+}
+
+f64 bdg_cb_contact(const void *param, f64 density) { return *(const f64 *)param * density; }

@@ -1,0 +1,128 @@
+/* Host-side definitions of the BdG handle (not installed). */
+#ifndef BDG_AMD_HOST_H
+#define BDG_AMD_HOST_H
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "bdg/bdg.h"
+#include "bdg/bdg_amd.h"
+#include "bdgx.h"
+
+/* setup-state flags (same order semantics as reference src/bdg_internal.h:25-44) */
+enum {
+  BDG_HAS_SYSTEM = 1u << 0,
+  BDG_HAS_TRAP = 1u << 1,
+  BDG_HAS_WF = 1u << 2,
+  BDG_HAS_INTERACTIONS = 1u << 3,
+  BDG_HAS_DIPOLAR = 1u << 4,
+  BDG_HAS_MU = 1u << 5
+};
+
+/* Misuse exits with EXIT_FAILURE after a message, as the reference does
+ * (reference src/bdg_log.h:29-37). */
+#define BDG_FAIL(...)                                   \
+  do {                                                  \
+    fprintf(stderr, "ERROR [%s:%d] ", __FILE__, __LINE__); \
+    fprintf(stderr, __VA_ARGS__);                       \
+    fprintf(stderr, "\n");                              \
+    exit(EXIT_FAILURE);                                 \
+  } while (0)
+
+#define BDG_REQUIRE(b, flag, who)                                          \
+  do {                                                                     \
+    if (0 == ((b)->state & (flag)))                                        \
+      BDG_FAIL("%s: prerequisite %s not satisfied", who, #flag);          \
+  } while (0)
+
+#define BDG_FORBID(b, flag, who)                                           \
+  do {                                                                     \
+    if (0 != ((b)->state & (flag)))                                        \
+      BDG_FAIL("%s: %s already set (double-call forbidden)", who, #flag); \
+  } while (0)
+
+struct bdg_t {
+  /* grid */
+  uint64_t dim, N[3];
+  double L[3];
+  uint64_t size, k_size;
+  int complex_psi0;
+  uint32_t state;
+  double *kx2[3]; /* per-axis k_d^2 (reference src/matmul_ctx.c:102-113) */
+  /* physics (host, FP64) */
+  double *ltK, *ltM, *pK, *pM;
+  void *wf;
+  double mu, g_ddi;
+  int dipolar;
+  /* device */
+  bdgx_ctx *dev;
+  /* solver parameters (defaults reference src/bdg.c:23-27) */
+  uint64_t nev, sizeSub, maxIter;
+  double tol;
+  int precision;
+  double tol_fp32;
+  uint64_t maxIter_fp32;
+  int verbosity;
+  /* results */
+  double *eigvals;
+  void *modes_u, *modes_v;
+  uint64_t converged;
+  double *residuals;
+  int *zero_modes;
+  bdg_solve_stats_t stats;
+  /* start-block strategy (survives reset) */
+  bdg_init_mode_t init_mode;
+  bdg_init_fn custom_init_fn;
+  void *custom_init_param;
+  void *reuse_buf;
+  uint64_t reuse_n, reuse_cols;
+};
+
+/* xorshift32 / uniform (0, 1] (reference src/bdg_internal.h:53-65) */
+static inline uint32_t bdg_xorshift32(uint32_t *s) {
+  uint32_t x = *s;
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  *s = x;
+  return x;
+}
+static inline double bdg_xrand(uint32_t *s) {
+  return (double)(bdg_xorshift32(s) & 0x7FFFFFFFu) / (double)0x7FFFFFFF;
+}
+
+/* start block: fills X (n x sizeSub column-major, f64 or interleaved c64) */
+void bdg_fill_start_block(bdg_t *b, void *X);
+
+/* device LOBPCG on (MKM, M); see ilobpcg.c */
+typedef struct {
+  uint64_t S;
+  int m, nev, cplx;
+  int maxiter;
+  double tol;
+  double zero_tol;
+  int verbosity;
+} lobpcg_cfg;
+
+typedef struct {
+  double *w2;      /* m */
+  int *zflag;      /* m */
+  double *res;     /* m */
+  int iters, restarts;
+  uint64_t applies;
+  double rr_seconds;
+  int status;
+} lobpcg_result;
+
+/* Runs from the device start block dX0 (S x m, precision prec, ld S).  On
+ * success *dX and *dMX (device, precision prec, S x m) hold the final block
+ * and its M image; the caller frees them with bdgx_free. */
+int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void *dX0,
+                      lobpcg_result *res, void **dX, void **dMX);
+
+double bdg_wtime(void);
+
+#endif

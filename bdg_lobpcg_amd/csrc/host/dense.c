@@ -1,0 +1,242 @@
+/*
+ * Small dense FP64 linear algebra for the host side of the iLOBPCG step.
+ *
+ * dn_syev: Householder reduction of a symmetric matrix to tridiagonal form
+ * followed by the implicit symmetric QR iteration with Wilkinson shifts and
+ * Givens chasing (Golub & Van Loan, Matrix Computations, 4th ed., Alg. 8.3.1
+ * and 8.3.3), accumulating the orthogonal transforms.
+ *
+ * dn_heev: Hermitian H = A + iB through the real symmetric embedding
+ * [[A, -B], [B, A]], whose spectrum is H's with every eigenvalue doubled;
+ * one complex eigenvector per pair is recovered by complex Gram-Schmidt
+ * inside each (doubled) eigenvalue cluster.
+ *
+ * The Rayleigh-Ritz routine itself lives in rr_impl.inc (compiled once for
+ * real and once for complex scalars).
+ */
+#include "dense.h"
+
+#include <complex.h>
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------------- */
+static void tridiagonalize(int n, double *A, double *Q, double *d, double *e) {
+  double *v = (double *)malloc(sizeof(double) * (size_t)n);
+  double *p = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n * n; ++i) Q[i] = 0.0;
+  for (int i = 0; i < n; ++i) Q[i + i * n] = 1.0;
+  for (int j = 0; j + 2 < n; ++j) {
+    const int m = n - j - 1; /* length of x = A[j+1.., j] */
+    double nx = 0.0;
+    for (int i = 0; i < m; ++i) {
+      v[i] = A[(j + 1 + i) + j * n];
+      nx += v[i] * v[i];
+    }
+    nx = sqrt(nx);
+    if (nx == 0.0) continue;
+    const double alpha = v[0] > 0 ? -nx : nx;
+    v[0] -= alpha;
+    double nv = 0.0;
+    for (int i = 0; i < m; ++i) nv += v[i] * v[i];
+    nv = sqrt(nv);
+    if (nv == 0.0) continue;
+    for (int i = 0; i < m; ++i) v[i] /= nv;
+    /* p = A22 v */
+    for (int i = 0; i < m; ++i) p[i] = 0.0;
+    for (int c = 0; c < m; ++c) {
+      const double vc = v[c];
+      const double *col = A + (j + 1) + (j + 1 + c) * n;
+      for (int i = 0; i < m; ++i) p[i] += col[i] * vc;
+    }
+    double K = 0.0;
+    for (int i = 0; i < m; ++i) K += v[i] * p[i];
+    /* w = 2p - 2Kv ; A22 -= v w^T + w v^T */
+    for (int i = 0; i < m; ++i) p[i] = 2.0 * p[i] - 2.0 * K * v[i];
+    for (int c = 0; c < m; ++c) {
+      double *col = A + (j + 1) + (j + 1 + c) * n;
+      const double vc = v[c], wc = p[c];
+      for (int i = 0; i < m; ++i) col[i] -= v[i] * wc + p[i] * vc;
+    }
+    A[(j + 1) + j * n] = alpha;
+    A[j + (j + 1) * n] = alpha;
+    for (int i = 1; i < m; ++i) {
+      A[(j + 1 + i) + j * n] = 0.0;
+      A[j + (j + 1 + i) * n] = 0.0;
+    }
+    /* Q[:, j+1..] -= 2 (Q[:, j+1..] v) v^T */
+    for (int r = 0; r < n; ++r) {
+      double s = 0.0;
+      for (int i = 0; i < m; ++i) s += Q[r + (j + 1 + i) * n] * v[i];
+      s *= 2.0;
+      for (int i = 0; i < m; ++i) Q[r + (j + 1 + i) * n] -= s * v[i];
+    }
+  }
+  for (int i = 0; i < n; ++i) d[i] = A[i + i * n];
+  for (int i = 0; i + 1 < n; ++i) e[i] = A[(i + 1) + i * n];
+  if (n > 0) e[n - 1] = 0.0;
+  free(v);
+  free(p);
+}
+
+/* implicit QR on the tridiagonal (d, e) with vectors accumulated in Z */
+static int tridiag_qr(int n, double *d, double *e, double *Z) {
+  int q = n - 1;
+  int guard = 0;
+  while (q > 0) {
+    for (int i = 0; i < q; ++i)
+      if (fabs(e[i]) <= DBL_EPSILON * (fabs(d[i]) + fabs(d[i + 1]))) e[i] = 0.0;
+    while (q > 0 && e[q - 1] == 0.0) --q;
+    if (q == 0) break;
+    int p = q - 1;
+    while (p > 0 && e[p - 1] != 0.0) --p;
+    if (++guard > 60 * n) return -1;
+    /* Wilkinson shift from the trailing 2x2 of the block [p, q] */
+    const double dl = 0.5 * (d[q - 1] - d[q]);
+    const double eq = e[q - 1];
+    const double den = dl + (dl >= 0 ? 1.0 : -1.0) * hypot(dl, eq);
+    const double mu = den != 0.0 ? d[q] - eq * eq / den : d[q];
+    double x = d[p] - mu, z = e[p];
+    double bulge = 0.0;
+    for (int k = p; k < q; ++k) {
+      const double r = hypot(x, z);
+      double c = 1.0, s = 0.0;
+      if (r != 0.0) {
+        c = x / r;
+        s = z / r;
+      }
+      if (k > p) e[k - 1] = r;
+      const double a = d[k], b = d[k + 1], o = e[k];
+      d[k] = c * c * a + 2.0 * c * s * o + s * s * b;
+      d[k + 1] = s * s * a - 2.0 * c * s * o + c * c * b;
+      e[k] = c * s * (b - a) + (c * c - s * s) * o;
+      if (k + 1 < q) {
+        bulge = s * e[k + 1];
+        e[k + 1] = c * e[k + 1];
+      }
+      double *zk = Z + (size_t)k * n, *zk1 = Z + (size_t)(k + 1) * n;
+      for (int i = 0; i < n; ++i) {
+        const double u = zk[i], w = zk1[i];
+        zk[i] = c * u + s * w;
+        zk1[i] = -s * u + c * w;
+      }
+      x = e[k];
+      z = bulge;
+    }
+  }
+  return 0;
+}
+
+static void sort_eig(int n, double *w, double *V, int ldv, int vlen) {
+  /* selection sort (n is small) */
+  double *tmp = (double *)malloc(sizeof(double) * (size_t)vlen);
+  for (int i = 0; i < n; ++i) {
+    int k = i;
+    for (int j = i + 1; j < n; ++j)
+      if (w[j] < w[k]) k = j;
+    if (k != i) {
+      const double t = w[i];
+      w[i] = w[k];
+      w[k] = t;
+      memcpy(tmp, V + (size_t)i * ldv, sizeof(double) * vlen);
+      memcpy(V + (size_t)i * ldv, V + (size_t)k * ldv, sizeof(double) * vlen);
+      memcpy(V + (size_t)k * ldv, tmp, sizeof(double) * vlen);
+    }
+  }
+  free(tmp);
+}
+
+int dn_syev(int n, double *A, double *w) {
+  if (n <= 0) return 0;
+  double *Q = (double *)malloc(sizeof(double) * (size_t)n * n);
+  double *e = (double *)malloc(sizeof(double) * (size_t)n);
+  /* symmetrise from the lower triangle */
+  for (int c = 0; c < n; ++c)
+    for (int r = c + 1; r < n; ++r) {
+      const double s = 0.5 * (A[r + c * n] + A[c + r * n]);
+      A[r + c * n] = s;
+      A[c + r * n] = s;
+    }
+  tridiagonalize(n, A, Q, w, e);
+  const int rc = tridiag_qr(n, w, e, Q);
+  sort_eig(n, w, Q, n, n);
+  memcpy(A, Q, sizeof(double) * (size_t)n * n);
+  free(Q);
+  free(e);
+  return rc;
+}
+
+int dn_heev(int n, double *A, double *w) {
+  if (n <= 0) return 0;
+  const int N2 = 2 * n;
+  double *M = (double *)malloc(sizeof(double) * (size_t)N2 * N2);
+  double *w2 = (double *)malloc(sizeof(double) * (size_t)N2);
+  for (int c = 0; c < n; ++c)
+    for (int r = 0; r < n; ++r) {
+      /* Hermitian part: average with the conjugate transpose */
+      const double re = 0.5 * (A[2 * (r + c * n)] + A[2 * (c + r * n)]);
+      const double im = 0.5 * (A[2 * (r + c * n) + 1] - A[2 * (c + r * n) + 1]);
+      M[r + c * N2] = re;
+      M[(r + n) + (c + n) * N2] = re;
+      M[(r + n) + c * N2] = im;
+      M[r + (c + n) * N2] = -im;
+    }
+  int rc = dn_syev(N2, M, w2);
+  double scale = 0.0;
+  for (int i = 0; i < N2; ++i) scale = fmax(scale, fabs(w2[i]));
+  const double tol = 1e-9 * (scale > 0 ? scale : 1.0);
+  int nout = 0;
+  double complex *acc = (double complex *)malloc(sizeof(double complex) * (size_t)n * n);
+  double complex *z = (double complex *)malloc(sizeof(double complex) * (size_t)n);
+  int i = 0;
+  while (i < N2 && nout < n) {
+    int j = i + 1;
+    while (j < N2 && fabs(w2[j] - w2[i]) <= tol) ++j;
+    const int cl_start = nout;
+    const int want = (j - i + 1) / 2;
+    for (int t = i; t < j && nout - cl_start < want && nout < n; ++t) {
+      for (int r = 0; r < n; ++r) z[r] = M[r + (size_t)t * N2] + I * M[(r + n) + (size_t)t * N2];
+      for (int pass = 0; pass < 2; ++pass)
+        for (int a = cl_start; a < nout; ++a) {
+          double complex dot = 0;
+          for (int r = 0; r < n; ++r) dot += conj(acc[r + (size_t)a * n]) * z[r];
+          for (int r = 0; r < n; ++r) z[r] -= dot * acc[r + (size_t)a * n];
+        }
+      double nz = 0.0;
+      for (int r = 0; r < n; ++r) nz += creal(z[r] * conj(z[r]));
+      nz = sqrt(nz);
+      if (nz < 0.3) continue;
+      for (int r = 0; r < n; ++r) acc[r + (size_t)nout * n] = z[r] / nz;
+      w[nout] = w2[t];
+      ++nout;
+    }
+    i = j;
+  }
+  if (nout < n) rc = -2;
+  for (int c = 0; c < nout; ++c)
+    for (int r = 0; r < n; ++r) {
+      A[2 * (r + (size_t)c * n)] = creal(acc[r + (size_t)c * n]);
+      A[2 * (r + (size_t)c * n) + 1] = cimag(acc[r + (size_t)c * n]);
+    }
+  free(M);
+  free(w2);
+  free(acc);
+  free(z);
+  return rc;
+}
+
+/* ---- Rayleigh-Ritz, real and complex instantiations ------------------ */
+#define RR_REAL
+#include "rr_impl.inc"
+#undef RR_REAL
+#define RR_CPLX
+#include "rr_impl.inc"
+#undef RR_CPLX
+
+int dn_rayleigh_ritz(int k, int m, int cplx, const double *GV, const double *GM,
+                     const double *GA, double *w2, double *Y, int *zflags) {
+  if (cplx) return rr_z(k, m, GV, GM, GA, w2, Y, zflags);
+  return rr_d(k, m, GV, GM, GA, w2, Y, zflags);
+}

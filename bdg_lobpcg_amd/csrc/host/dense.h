@@ -1,0 +1,27 @@
+/* Small dense FP64 linear algebra for the Rayleigh-Ritz step (host side).
+ * Matrices are column-major; complex matrices are interleaved (re, im). */
+#ifndef BDG_AMD_DENSE_H
+#define BDG_AMD_DENSE_H
+
+#include <stdint.h>
+
+/* Real symmetric eigendecomposition: on exit A holds the orthonormal
+ * eigenvectors (columns) and w the eigenvalues in ascending order.
+ * Returns 0, or -1 if the QL iteration did not converge. */
+int dn_syev(int n, double *A, double *w);
+
+/* Complex Hermitian eigendecomposition (interleaved A, n x n).  Same output
+ * convention; A is overwritten by the eigenvectors. */
+int dn_heev(int n, double *A, double *w);
+
+/* Rayleigh-Ritz for the Hermitian-definite pencil (G_A, G_M) on a basis
+ * with Euclidean Gram G_V (all k x k, complex interleaved when cplx):
+ *   w2      out: the m smallest Ritz values (w^2), ascending
+ *   Y       out: k x m coefficients (complex when cplx)
+ *   zflags  out (optional): 1 for directions in the null space of G_M (zero
+ *           modes of a singular M), else 0
+ * Returns the number of pairs produced (<= m), or < 0 on failure. */
+int dn_rayleigh_ritz(int k, int m, int cplx, const double *GV, const double *GM,
+                     const double *GA, double *w2, double *Y, int *zflags);
+
+#endif

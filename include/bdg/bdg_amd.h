@@ -1,0 +1,72 @@
+/*
+ * bdg_amd.h -- extensions of the BdG API that have no counterpart in the
+ * reference header (include/bdg/bdg.h).  A program that only uses bdg.h
+ * never needs this file; it gets the plain FP64 device solver.
+ */
+#ifndef BDG_AMD_H
+#define BDG_AMD_H
+
+#include "bdg/bdg.h"
+#include "bdgx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Solver precision policy. */
+typedef enum {
+  BDG_PREC_FP64 = 0,   /* every operator apply and block update in FP64 (default) */
+  BDG_PREC_MIXED = 1   /* FP32 bulk iterations, then an FP64 refinement phase */
+} bdg_precision_t;
+
+void bdg_set_precision(bdg_t *bdg, bdg_precision_t prec);
+
+typedef struct {
+  uint64_t iterations;       /* total LOBPCG iterations                    */
+  uint64_t iterations_fp32;  /* of which in FP32 (mixed policy)            */
+  uint64_t iterations_fp64;  /* of which in FP64                           */
+  uint64_t column_applies;   /* (M then K) column applies, incl. start     */
+  uint64_t restarts;         /* iterations that dropped the P block        */
+  double seconds_total;      /* bdg_solve wall clock                       */
+  double seconds_fp32;
+  double seconds_fp64;
+  double seconds_rr;         /* host Rayleigh-Ritz time                    */
+  int status;                /* 0 ok, else a BDGX_E* code or -100 (RR)    */
+} bdg_solve_stats_t;
+
+void bdg_get_solve_stats(const bdg_t *bdg, bdg_solve_stats_t *out);
+
+/* Final relative residuals ||A x - w B x|| / (||x|| max(1,|w|)), nev values. */
+const f64 *bdg_residuals(const bdg_t *bdg);
+
+/* 1 for modes identified as zero modes (Goldstone), nev values. */
+const int *bdg_zero_modes(const bdg_t *bdg);
+
+/* Device context of the handle (valid after bdg_set_system). */
+bdgx_ctx *bdg_device_context(bdg_t *bdg);
+
+/* Host copies of the setup arrays (size entries; NULL before they exist). */
+const f64 *bdg_local_term_K(const bdg_t *bdg);
+const f64 *bdg_local_term_M(const bdg_t *bdg);
+const f64 *bdg_precond_weight_K(const bdg_t *bdg);
+const f64 *bdg_precond_weight_M(const bdg_t *bdg);
+f64 bdg_mu(const bdg_t *bdg);
+
+/* Residual tolerance of the FP32 phase of the mixed policy (default 1e-5)
+ * and the iteration budget of the FP32 phase (default maxIter). */
+void bdg_set_mixed_params(bdg_t *bdg, f64 tol_fp32, uint64_t max_iter_fp32);
+
+/* 0 silent (default), 1 per-phase summary, 2 per-iteration trace (stderr). */
+void bdg_set_verbosity(bdg_t *bdg, int level);
+
+/* Built-in host callbacks usable with bdg_set_trap / bdg_set_local_interactions
+ * (for callers that cannot pass native function pointers cheaply):
+ *   bdg_cb_harmonic_trap: param = const f64 omega[dim], V = 0.5 sum (w_d r_d)^2
+ *   bdg_cb_contact:       param = const f64 *g,         U = g * density      */
+f64 bdg_cb_harmonic_trap(uint64_t dim, const f64 *r, const void *param);
+f64 bdg_cb_contact(const void *param, f64 density);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

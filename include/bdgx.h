@@ -1,0 +1,137 @@
+/*
+ * bdgx.h -- thin C ABI over the HIP/gfx950 kernels of the BdG hot path.
+ *
+ * The reference applies each operator to one host vector at a time through
+ * `void op(matmul_ctx_t *ctx, const T *x, T *y)` (reference
+ * src/bdg_internal.h:159-215, adapters src/solver.c:26-62).  This layer
+ * replaces that with block-wise device applies: one call transforms `ncols`
+ * column vectors resident in HBM (column-major, leading dimension `ld`
+ * elements) on the context's HIP stream.  Everything is plain C: opaque
+ * handles, raw pointers and sizes.
+ *
+ * Element types follow the context's variant: real psi0 -> real vectors
+ * (f64 for BDGX_FP64, f32 for BDGX_FP32); complex psi0 -> interleaved complex
+ * (c64 / 2 x f32).  All calls return 0 on success and a negative BDGX_E*
+ * code on failure (never exit()).
+ */
+#ifndef BDGX_H
+#define BDGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bdgx_ctx bdgx_ctx;
+
+/* precisions */
+enum { BDGX_FP64 = 0, BDGX_FP32 = 1 };
+
+/* error codes */
+enum {
+  BDGX_OK = 0,
+  BDGX_EINVAL = -1,   /* bad argument / unsupported shape */
+  BDGX_EHIP = -2,     /* HIP runtime error */
+  BDGX_ESTATE = -3,   /* required field not uploaded yet */
+  BDGX_ENOMEM = -4
+};
+
+/* operator codes; each replaces the reference function named beside it */
+enum {
+  BDGX_OP_KINETIC = 0,      /* kinetic_d/_z      src/kinetic_impl.inc:17-30      */
+  BDGX_OP_K = 1,            /* matmulK_d/_z      src/matmulK_impl.inc:11-28      */
+  BDGX_OP_M = 2,            /* matmulM_d/_z      src/matmulM_impl.inc:14-44      */
+  BDGX_OP_LREP = 3,         /* matmulLrep_d/_z   src/matmulLrep_impl.inc:13-18   */
+  BDGX_OP_SWAP = 4,         /* matmulSwap_d/_z   src/matmulSwap_impl.inc:14-26   */
+  BDGX_OP_PRECOND_K = 5,    /* precondK_d/_z     src/precond_impl.inc:21-65      */
+  BDGX_OP_PRECOND_M = 6,    /* precondM_d/_z     src/precond_impl.inc:67-111     */
+  BDGX_OP_PRECOND_LREP = 7, /* precondLrep_d/_z  src/precond_impl.inc:113-118    */
+  BDGX_OP_DIPOLAR_CONV = 8  /* dipolar_conv_d/_z src/dipolar_conv_impl.inc:22-52 */
+};
+
+/* ---- context: replaces matmul_ctx_alloc + matmul_ctx_set_system
+ *      (reference src/matmul_ctx.c:21-42, 88-172) ------------------------ */
+int bdgx_ctx_create(bdgx_ctx **out, uint64_t dim, const uint64_t *N,
+                    const double *L, int complex_psi0);
+void bdgx_ctx_destroy(bdgx_ctx *ctx);
+uint64_t bdgx_ctx_size(const bdgx_ctx *ctx);   /* prod N            */
+uint64_t bdgx_ctx_ksize(const bdgx_ctx *ctx);  /* spectrum length   */
+const char *bdgx_error_string(int code);
+
+/* ---- physics fields (host FP64 arrays; NULL leaves a field untouched).
+ *      Replaces the ctx fields written by src/setup.c:17-177. ---------- */
+int bdgx_set_fields(bdgx_ctx *ctx, const double *ltK, const double *ltM,
+                    const void *wf, const double *pK, const double *pM,
+                    double mu);
+
+/* Dipolar kernel on the device (reference src/dipolar.c:21-72). */
+int bdgx_dipolar_set_kernel(bdgx_ctx *ctx, double g_ddi, const double *dir,
+                            double cutoff_radius);
+int bdgx_dipolar_get_kernel(bdgx_ctx *ctx, double *h_out /* ksize */);
+/* g_ddi * Phi_dd into h_out[size] (reference src/dipolar_meanfield_impl.inc:20-55);
+ * requires the wavefunction and the kernel. */
+int bdgx_dipolar_meanfield(bdgx_ctx *ctx, double *h_out);
+
+/* ---- operator applies ------------------------------------------------- */
+/* Device block apply: dY[:, j] = op(dX[:, j]), j < ncols, on the ctx stream.
+ * Column length is 2*size for LREP/SWAP/PRECOND_LREP, size otherwise. */
+int bdgx_apply(bdgx_ctx *ctx, int op, int prec, const void *dX, void *dY,
+               uint64_t ncols, uint64_t ld);
+/* Host convenience (uploads, applies in FP64, downloads, synchronises). */
+int bdgx_apply_host(bdgx_ctx *ctx, int op, const void *hX, void *hY,
+                    uint64_t ncols, uint64_t ld);
+
+/* ---- dense block kernels (tall-skinny, rows = nrows) ------------------- */
+/* G (p x q, column-major FP64 or complex FP64, ldg = p) = A^H B, computed
+ * with FP64 accumulation.  swap_b != 0 reads B with its two halves of
+ * nrows/2 rows exchanged (gives A^H (Swap B) without materialising it). */
+int bdgx_gram(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64_t p,
+              const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows,
+              int swap_b, double *dG);
+/* dY[:, 0:m] = dA[:, 0:k] * C  (C is k x m column-major FP64 / complex FP64
+ * in device memory, ldc = k); dY must not alias dA. */
+int bdgx_combine(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda,
+                 uint64_t k, const double *dC, void *dY, uint64_t ldy,
+                 uint64_t m, uint64_t nrows);
+/* R[:, j] = AX[:, j] - lambda_j * X'[:, j] with X' = Swap(X) when swap_x,
+ * else X.  dnorms2 (3 * ncols FP64, device) receives ||R_j||^2, ||X_j||^2 and
+ * ||MX_j||^2 (the last only when dMX != NULL, else 0).  dR may be NULL (norms
+ * only).  dAX, dX and dMX share the leading dimension ld. */
+int bdgx_residual(bdgx_ctx *ctx, int prec, const void *dAX, const void *dX,
+                  const void *dMX, uint64_t ld, uint64_t ncols, uint64_t nrows,
+                  const double *dlambda, int swap_x, void *dR, uint64_t ldr,
+                  double *dnorms2);
+/* Element-type conversion FP64 <-> FP32 of an nrows x ncols block. */
+int bdgx_convert(bdgx_ctx *ctx, int prec_src, const void *dsrc, uint64_t lds,
+                 int prec_dst, void *ddst, uint64_t ldd, uint64_t nrows,
+                 uint64_t ncols);
+
+/* ---- psi0 by imaginary-time split-step (no reference counterpart: the
+ *      reference only loads psi0, src/io.c:13-50).  Same step definition as
+ *      the oracle's orc_itp_groundstate. ------------------------------- */
+int bdgx_itp_groundstate(bdgx_ctx *ctx, const double *omega, double g,
+                         double g_dd, const double *dir, double cutoff,
+                         double dt, uint64_t nsteps, void *h_psi /* in/out */,
+                         double *mu_out);
+
+/* ---- memory, streams, timing ------------------------------------------ */
+int bdgx_malloc(void **dptr, size_t bytes);
+int bdgx_free(void *dptr);
+int bdgx_memcpy_h2d(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int bdgx_memcpy_d2h(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int bdgx_memset(bdgx_ctx *ctx, void *dst, int value, size_t bytes);
+int bdgx_sync(bdgx_ctx *ctx);
+void *bdgx_stream(bdgx_ctx *ctx);          /* the ctx's hipStream_t */
+/* HIP-event timer on the ctx stream: begin records, end records + waits and
+ * returns the elapsed milliseconds. */
+int bdgx_timer_begin(bdgx_ctx *ctx);
+int bdgx_timer_end(bdgx_ctx *ctx, float *ms);
+/* Bytes of device workspace the context currently holds. */
+uint64_t bdgx_workspace_bytes(const bdgx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BDGX_H */

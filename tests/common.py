@@ -1,0 +1,78 @@
+"""Shared fixtures: the same seeded physics set up on the CPU oracle and,
+for GPU tests, on the device context."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import oracle as O
+
+
+def grid_coords(N, L):
+    xs = [(np.arange(n) - n / 2) * l / n for n, l in zip(N, L)]
+    if len(N) == 1:
+        return (xs[0],)
+    if len(N) == 2:
+        Y, X = np.meshgrid(xs[1], xs[0], indexing="ij")
+        return X.ravel(), Y.ravel()
+    Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+    return X.ravel(), Y.ravel(), Z.ravel()
+
+
+def gaussian_psi(N, L, cplx=False, width=1.0, vortex=False):
+    r = grid_coords(N, L)
+    r2 = sum(c ** 2 for c in r)
+    psi = np.exp(-r2 / (2 * width ** 2))
+    if cplx:
+        psi = psi.astype(np.complex128)
+        if vortex and len(N) >= 2:
+            psi = psi * np.exp(1j * np.arctan2(r[1], r[0]))
+        else:
+            psi = psi * np.exp(0.3j * r[0])
+    return psi
+
+
+def physics(N, L, cplx=False, dipolar=True, g=20.0, gdd=10.0, cutoff=None, mu=1.0,
+            omega=None):
+    """Oracle context with trap + contact (+ dipolar) + mu; returns (ctx, fields)."""
+    c = O.Ctx(N, L, cplx)
+    S = c.size
+    psi = gaussian_psi(N, L, cplx)
+    ltK, ltM = np.zeros(S), np.zeros(S)
+    c.trap_harmonic(omega or (1.0,) * len(N), ltK, ltM)
+    c.contact(psi, g, g, ltK, ltM)
+    c.set_fields(wf=psi)
+    f = dict(wf=psi, dipolar=False)
+    if dipolar and len(N) == 3:
+        R = cutoff if cutoff is not None else min(L) / 2
+        c.set_dipolar(gdd, (0.0, 0.0, 1.0), R)
+        gphi = c.dipolar_meanfield()
+        ltK += gphi
+        ltM += gphi
+        f.update(dipolar=True, gdd=gdd, R=R)
+    pK, pM = O.setup_mu(mu, ltK, ltM)
+    c.set_fields(ltK=ltK, ltM=ltM, pK=pK, pM=pM, mu=mu)
+    f.update(ltK=ltK, ltM=ltM, pK=pK, pM=pM, mu=mu)
+    return c, f
+
+
+def device_twin(N, L, cplx, f):
+    import bdg_lobpcg_amd as B
+
+    d = B.DeviceContext(N, L, cplx)
+    d.set_fields(wf=f["wf"])
+    if f.get("dipolar"):
+        d.set_dipolar(f["gdd"], (0.0, 0.0, 1.0), f["R"])
+    d.set_fields(ltK=f["ltK"], ltM=f["ltM"], pK=f["pK"], pM=f["pM"], mu=f["mu"])
+    return d
+
+
+def rand_block(rows, cols, cplx, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((rows, cols))
+    if cplx:
+        X = X + 1j * rng.standard_normal((rows, cols))
+    return np.asfortranarray(X)
+
+
+def relerr(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))

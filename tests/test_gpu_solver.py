@@ -1,0 +1,60 @@
+"""GPU solver tests through the public C API (include/bdg/bdg.h)."""
+import numpy as np
+import pytest
+
+from tests.common import physics, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_1d_uniform_bogoliubov(cplx):
+    """reference tests/test_bdg_1d_uniform.c:60-138 and 142-228: uniform gas,
+    N=64, L=2pi, g=n0=mu=1: spectrum 0, E(1) (x2), E(2)."""
+    import bdg_lobpcg_amd as B
+    N, L = 64, 2 * np.pi
+    p = B.Problem((N,), (L,), cplx)
+    p.set_system()
+    p.set_solver_params(4, 10, 200, 1e-10)
+    p.set_wavefunction(np.ones(N))
+    p.set_contact(1.0)
+    p.set_mu(1.0)
+    assert p.solve() == 0
+    assert p.converged() >= 4
+    w = p.eigenvalues(4)
+    E1, E2 = np.sqrt(0.5 * 2.5), np.sqrt(2.0 * 4.0)
+    assert abs(w[0]) < 1e-3
+    np.testing.assert_allclose(w[1:], [E1, E1, E2], rtol=1e-9)
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_3d_dipolar_vs_oracle(prec):
+    """Small 3D dipolar problem: device solve vs the CPU reference solver
+    (numpy iLOBPCG on the oracle operators), same psi0 and start block."""
+    import bdg_lobpcg_amd as B
+    from oracle.ilobpcg_ref import ilobpcg
+    N, L = (16, 16, 16), (12.0, 12.0, 12.0)
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=None or 1.0)
+    S = c.size
+    nev, m = 6, 8
+    X0 = c.start_block_uniform(42, m)
+    ref = ilobpcg(lambda X: c.apply("K", X), lambda X: c.apply("M", X),
+                  lambda X: c.apply("precondK", X), lambda X: c.apply("precondM", X),
+                  X0[S:], nev, tol=1e-11, maxiter=400)
+    p = B.Problem(N, L, False)
+    p.set_system()
+    p.set_trap_harmonic((1.0, 1.0, 1.0))
+    p.set_wavefunction(f["wf"])
+    p.set_contact(20.0)
+    p.set_dipolar(10.0, (0.0, 0.0, 1.0), f["R"])
+    p.set_mu(1.0)
+    p.set_solver_params(nev, m, 400, 1e-10)
+    p.set_precision(prec)
+    p.set_init_custom_block(X0)
+    assert p.solve() == 0
+    w = p.eigenvalues(nev)
+    z = p.zero_modes(nev) | ref["zero"]
+    nz = ~z
+    np.testing.assert_allclose(w[nz], ref["w"][nz], rtol=1e-9)
+    assert np.all(np.abs(w[z] - ref["w"][z]) < 1e-6)
+    assert p.converged() >= nev
