@@ -86,6 +86,7 @@ def lib():
         "bdgx_timer_begin": (I, [P]),
         "bdgx_timer_end": (I, [P, C.POINTER(C.c_float)]),
         "bdgx_workspace_bytes": (U64, [P]),
+        "bdgx_bench_pass": (I, [P, I, I, U64, I, C.POINTER(C.c_float), C.POINTER(D)]),
         # public API
         "bdg_alloc": (P, [U64, P, P, I]),
         "bdg_free": (None, [C.POINTER(P)]),
@@ -256,6 +257,14 @@ class DeviceContext:
         check(lib().bdgx_timer_end(self.h, C.byref(ms)), "timer")
         return float(ms.value)
 
+    def bench_pass(self, pass_id: int, prec: int, nbatch: int, iters: int = 20):
+        """(ms per launch, algorithmic bytes per launch) of one FFT pass kind."""
+        ms = C.c_float()
+        by = C.c_double()
+        check(lib().bdgx_bench_pass(self.h, int(pass_id), int(prec), int(nbatch), int(iters),
+                                    C.byref(ms), C.byref(by)), "bdgx_bench_pass")
+        return float(ms.value), float(by.value)
+
     def gram(self, A: np.ndarray, B: np.ndarray, prec=FP64, swap_b=False) -> np.ndarray:
         A = np.asfortranarray(A, dtype=self.dtype(prec))
         B = np.asfortranarray(B, dtype=self.dtype(prec))
@@ -382,6 +391,13 @@ class Problem:
         cb = INIT_FN(fill)
         self._keep += [Xf, cb]
         lib().bdg_set_init_mode(self.h, INIT_CUSTOM, C.cast(cb, C.c_void_p), None)
+
+    def set_init_uniform(self, seed: int = 42):
+        """BDG_INIT_CUSTOM with the built-in xorshift32 uniform block (u = v)."""
+        s = np.array([seed], dtype=np.uint32)
+        self._keep.append(s)
+        lib().bdg_set_init_mode(self.h, INIT_CUSTOM,
+                                C.cast(lib().bdg_init_uniform_xorshift, C.c_void_p), _p(s))
 
     def set_init_mode(self, mode, hint=None):
         lib().bdg_set_init_mode(self.h, int(mode), None,

@@ -23,6 +23,8 @@ int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, co
                       int swap_x, void *dR, uint64_t ldr, double *dn2);
 int convert_dispatch(bdgx_ctx *c, int ps, const void *src, uint64_t lds, int pd, void *dst,
                      uint64_t ldd, uint64_t nrows, uint64_t ncols);
+int bench_pass_dispatch(bdgx_ctx *c, int pass, int prec, uint64_t nbatch, int iters, float *ms,
+                        double *bytes);
 template <typename R>
 int meanfield_t(bdgx_ctx *c, const R *dpsi, R *dOut, double scale);
 template <typename R>
@@ -640,6 +642,12 @@ int bdgx_timer_end(bdgx_ctx *c, float *ms) {
   BDGX_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
   return BDGX_OK;
 }
+int bdgx_bench_pass(bdgx_ctx *c, int pass, int prec, uint64_t nbatch, int iters, float *ms,
+                    double *bytes) {
+  if (!c || !ms || !bytes || nbatch == 0 || iters < 0) return BDGX_EINVAL;
+  return bench_pass_dispatch(c, pass, prec, nbatch, iters, ms, bytes);
+}
+
 uint64_t bdgx_workspace_bytes(const bdgx_ctx *c) {
   return (uint64_t)(c->spec_bytes + c->scratch_bytes);
 }

@@ -423,9 +423,24 @@ void bdg_fill_start_block(bdg_t *b, void *X) {
 
 /* ---- solve ---------------------------------------------------------------- */
 
+/* The solver's preconditioner weights: the reference's
+ * p = 1/sqrt(max(1e-8, lt)) (src/setup.c:165-166) with the pre-mu local term
+ * lt clamped below at mu instead of 1e-8.  Where an attractive dipolar mean
+ * field drives lt to zero or below, the reference weights reach 1e4 and every
+ * preconditioned residual collapses onto those grid points; inside the
+ * condensate lt ~ mu, so the clamp only changes those points.  The operator
+ * API (bdgx_apply PRECOND_*) keeps the reference weights. */
 static int upload_fields(bdg_t *b) {
-  int rc = bdgx_set_fields(b->dev, b->ltK, b->ltM, b->wf, b->pK, b->pM, b->mu);
-  if (!rc && b->dipolar) rc = BDGX_OK; /* kernel already on the device */
+  const double floor_v = b->mu > 1e-8 ? b->mu : 1e-8;
+  double *pKs = (double *)xcalloc(b->size, sizeof(double));
+  double *pMs = (double *)xcalloc(b->size, sizeof(double));
+  for (uint64_t i = 0; i < b->size; ++i) {
+    pKs[i] = 1.0 / sqrt(fmax(floor_v, b->ltK[i] + b->mu));
+    pMs[i] = 1.0 / sqrt(fmax(floor_v, b->ltM[i] + b->mu));
+  }
+  const int rc = bdgx_set_fields(b->dev, b->ltK, b->ltM, b->wf, pKs, pMs, b->mu);
+  free(pKs);
+  free(pMs);
   return rc;
 }
 
@@ -612,3 +627,14 @@ f64 bdg_cb_harmonic_trap(uint64_t dim, const f64 *r, const void *param) {
 }
 
 f64 bdg_cb_contact(const void *param, f64 density) { return *(const f64 *)param * density; }
+
+void bdg_init_uniform_xorshift(const bdg_t *b, void *X, uint64_t n, uint64_t m, void *param) {
+  uint32_t seed = param ? *(const uint32_t *)param : 42u;
+  const uint64_t S = n / 2;
+  for (uint64_t j = 0; j < m; ++j)
+    for (uint64_t i = 0; i < S; ++i) {
+      const double v = bdg_xrand(&seed) - 0.5;
+      put(b, X, i, j, v);
+      put(b, X, i + S, j, v);
+    }
+}

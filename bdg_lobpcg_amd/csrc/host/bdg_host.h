@@ -111,7 +111,7 @@ typedef struct {
   double *w2;      /* m */
   int *zflag;      /* m */
   double *res;     /* m */
-  int iters, restarts;
+  int iters, restarts, precond_fallbacks;
   uint64_t applies;
   double rr_seconds;
   int status;

@@ -180,6 +180,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void
   res->status = 0;
   res->iters = 0;
   res->restarts = 0;
+  res->precond_fallbacks = 0;
   res->applies = 0;
 
   TRY(alloc3(&A, (size_t)S * kmax * es));
@@ -238,7 +239,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void
       }
       if (!conv[j]) act[nact++] = j;
     }
-    if (cfg->verbosity > 1)
+    if (cfg->verbosity > 1 || (cfg->verbosity == 1 && it % 10 == 0))
       fprintf(stderr, "[bdg] it %d prec %s w0 %.10g wlast %.10g worst res %.3e active %d\n", it,
               prec == BDGX_FP64 ? "fp64" : "fp32", signed_sqrt(w2[0]),
               signed_sqrt(w2[nev - 1]), worst, nact);
@@ -266,8 +267,9 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void
       k += nact;
     }
     TRY(grams(ctx, prec, &A, (uint64_t)k, S, cplx, dG, hGV, hGM, hGA));
-    const double t0 = bdg_wtime();
+    double t0 = bdg_wtime();
     if (haveP && !gram_ok(hGV, k, cplx)) {
+      /* [X, W, P] numerically dependent: drop P for this iteration */
       const int kk = m + nact;
       lead_block(hGV, k, kk, cplx, hT1);
       lead_block(hGM, k, kk, cplx, hT2);
@@ -277,6 +279,23 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void
       memcpy(hGA, hT3, sizeof(double) * (size_t)kk * kk * w);
       k = kk;
       res->restarts++;
+    }
+    if (!gram_ok(hGV, k, cplx)) {
+      /* [X, W] still dependent: the preconditioner collapsed the residuals
+       * (e.g. a 1e8 gain on one mode); use W = R for this iteration */
+      trr += bdg_wtime() - t0;
+      k = m + nact;
+      for (int a = 0; a < nact; ++a)
+        TRY(copy_col(ctx, prec, colp(dTmp, (uint64_t)a, S, es),
+                     colp(A.V, (uint64_t)(m + a), S, es), S));
+      TRY(bdgx_apply(ctx, BDGX_OP_M, prec, colp(A.V, (uint64_t)m, S, es),
+                     colp(A.MV, (uint64_t)m, S, es), (uint64_t)nact, S));
+      TRY(bdgx_apply(ctx, BDGX_OP_K, prec, colp(A.MV, (uint64_t)m, S, es),
+                     colp(A.KMV, (uint64_t)m, S, es), (uint64_t)nact, S));
+      res->applies += (uint64_t)nact;
+      res->precond_fallbacks++;
+      TRY(grams(ctx, prec, &A, (uint64_t)k, S, cplx, dG, hGV, hGM, hGA));
+      t0 = bdg_wtime();
     }
     const int got = dn_rayleigh_ritz(k, m, cplx, hGV, hGM, hGA, w2, hY, zf);
     trr += bdg_wtime() - t0;

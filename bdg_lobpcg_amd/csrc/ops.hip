@@ -467,13 +467,145 @@ static int rows_per_wg(int nx, int npen_per_2rows, bool real) {
 }
 
 template <typename R>
+static XArgs<R> make_xargs(bdgx_ctx *c, const HalfJob &J, int base, int nc, const R *Xc, R *Yc,
+                           long long ld, cx<R> *spec, long long cstride, const R *lt_override) {
+  Fields<R> &F = fields<R>(c);
+  XArgs<R> a;
+  memset(&a, 0, sizeof(a));
+  a.g = c->geo;
+  a.ncols = nc;
+  a.ld = ld;
+  a.X = Xc;
+  a.Y = Yc;
+  a.in_half = J.in_half;
+  a.out_half = J.out_half;
+  a.nch = J.nch;
+  a.chan_base = base;
+  for (int q = 0; q < J.nch; ++q) {
+    a.pre[q] = J.pre[q];
+    a.prew[q] = field_ptr(F, J.prew_sel[q]);
+    a.post[q] = J.post[q];
+    a.postw[q] = field_ptr(F, J.postw_sel[q]);
+    a.posts[q] = (R)J.posts[q];
+  }
+  a.lt = lt_override ? lt_override : field_ptr(F, J.lt_sel);
+  a.out_real = J.out_real;
+  a.spec = spec;
+  a.chan_stride = cstride;
+  a.tw = F.tw[0];
+  a.pl = c->plan[0];
+  return a;
+}
+
+// x-axis pass (forward: pre-weights + transform into the spectrum; inverse:
+// transform back + combine into Y) for one half-job over nc columns.
+template <typename R>
+static int launch_x(bdgx_ctx *c, XArgs<R> &a, bool inverse) {
+  const Geo &g = c->geo;
+  const int rows = g.ny * g.nz;
+  const int nc = a.ncols;
+  if (!c->cplx) {
+    int npen;
+    if (!inverse) {
+      a.rb = rows_per_wg(g.nx, a.nch, true);
+      npen = a.nch * (a.rb / 2);
+    } else if (a.nch == 2) {
+      int rb = 1024 / g.nx;  // one pencil per row (both channels in one transform)
+      if (rb < 2) rb = 2;
+      a.rb = rb & ~1;
+      npen = a.rb;
+    } else {
+      a.rb = rows_per_wg(g.nx, 1, true);
+      npen = a.rb / 2;
+    }
+    const size_t sm = sizeof(cx<R>) * ((size_t)g.nx + 2 * (size_t)npen * g.nx);
+    const int nblk = (rows + a.rb - 1) / a.rb;
+    if (inverse)
+      hipLaunchKernelGGL(k_xinv_real<R>, dim3(nblk * nc), dim3(kThreads), sm, c->stream, a);
+    else
+      hipLaunchKernelGGL(k_xfwd_real<R>, dim3(nblk * nc), dim3(kThreads), sm, c->stream, a);
+  } else {
+    a.rb = rows_per_wg(g.nx, a.nch, false);
+    const int npen = a.nch * a.rb;
+    const size_t sm = sizeof(cx<R>) * ((size_t)g.nx + 2 * (size_t)npen * g.nx);
+    const int nblk = (rows + a.rb - 1) / a.rb;
+    if (inverse)
+      hipLaunchKernelGGL(k_xinv_cplx<R>, dim3(nblk * nc), dim3(kThreads), sm, c->stream, a);
+    else
+      hipLaunchKernelGGL(k_xfwd_cplx<R>, dim3(nblk * nc), dim3(kThreads), sm, c->stream, a);
+  }
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
+template <typename R>
+static YZArgs<R> make_yz(bdgx_ctx *c, const int *mul, int C, int nc, cx<R> *spec) {
+  const Geo &g = c->geo;
+  Fields<R> &F = fields<R>(c);
+  YZArgs<R> yz;
+  memset(&yz, 0, sizeof(yz));
+  yz.g = g;
+  yz.ncols = nc;
+  yz.nb = C * nc;
+  yz.P = sizeof(R) == 8 ? 8 : 16;
+  if (yz.P > g.n0k) yz.P = g.n0k;
+  yz.spec = spec;
+  for (int ch = 0; ch < C; ++ch) yz.mul[ch] = mul[ch];
+  yz.kx2 = F.kx2[0];
+  yz.ky2 = F.kx2[1];
+  yz.kz2 = F.kx2[2];
+  yz.U = F.U;
+  yz.mu = (R)c->mu;
+  yz.invS = (R)(1.0 / (double)g.S);
+  yz.dt = (R)c->itp_dt;
+  return yz;
+}
+
+// y-axis transform over all batches (skipped when Ny == 1)
+template <typename R>
+static int launch_y(bdgx_ctx *c, YZArgs<R> yz, bool inverse) {
+  const Geo &g = c->geo;
+  if (g.ny <= 1) return BDGX_OK;
+  Fields<R> &F = fields<R>(c);
+  yz.tw = F.tw[1];
+  yz.pl = c->plan[1];
+  const int ntile = (g.n0k + yz.P - 1) / yz.P;
+  const size_t sm = sizeof(cx<R>) * ((size_t)g.ny + 2 * (size_t)yz.P * (g.ny + 1));
+  if (inverse)
+    hipLaunchKernelGGL((k_ypass<R, true>), dim3(ntile * g.nz * yz.nb), dim3(kThreads), sm,
+                       c->stream, yz);
+  else
+    hipLaunchKernelGGL((k_ypass<R, false>), dim3(ntile * g.nz * yz.nb), dim3(kThreads), sm,
+                       c->stream, yz);
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
+// fused z-axis forward, k-space multiply, z-axis inverse
+template <typename R>
+static int launch_z(bdgx_ctx *c, YZArgs<R> z) {
+  const Geo &g = c->geo;
+  Fields<R> &F = fields<R>(c);
+  z.tw = F.tw[2];
+  z.pl = c->plan[2];
+  const int ntile = (g.n0k + z.P - 1) / z.P;
+  const size_t sm = sizeof(cx<R>) * ((size_t)g.nz + 2 * (size_t)z.P * (g.nz + 1));
+  hipLaunchKernelGGL(k_zpass<R>, dim3(ntile * g.ny * z.nb), dim3(kThreads), sm, c->stream, z);
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
+template <typename R>
 static int run_jobs(bdgx_ctx *c, const HalfJob *jobs, int njobs, const R *X,
                     R *Y, long long ncols_total, long long ld, const R *lt_override) {
   const Geo &g = c->geo;
-  Fields<R> &F = fields<R>(c);
   int C = 0;
-  for (int j = 0; j < njobs; ++j) C += jobs[j].nch;
-  if (C > kMaxChan) return BDGX_EINVAL;
+  int mul[kMaxChan];
+  for (int j = 0; j < njobs; ++j)
+    for (int q = 0; q < jobs[j].nch; ++q) {
+      if (C >= kMaxChan) return BDGX_EINVAL;
+      mul[C++] = jobs[j].mul[q];
+    }
   // chunk columns so the spectrum workspace stays bounded
   const size_t per_col = (size_t)C * (size_t)g.Kk * sizeof(cx<R>);
   const size_t budget = (size_t)6 << 30;
@@ -489,152 +621,21 @@ static int run_jobs(bdgx_ctx *c, const HalfJob *jobs, int njobs, const R *X,
     const R *Xc = X + c0 * ld * cw;
     R *Yc = Y ? Y + c0 * ld * cw : nullptr;
     const long long cstride = (long long)nc * g.Kk;
-    // forward x
     int base = 0;
     for (int j = 0; j < njobs; ++j) {
-      const HalfJob &J = jobs[j];
-      XArgs<R> a;
-      memset(&a, 0, sizeof(a));
-      a.g = g;
-      a.ncols = nc;
-      a.ld = ld;
-      a.X = Xc;
-      a.Y = Yc;
-      a.in_half = J.in_half;
-      a.out_half = J.out_half;
-      a.nch = J.nch;
-      a.chan_base = base;
-      for (int q = 0; q < J.nch; ++q) {
-        a.pre[q] = J.pre[q];
-        a.prew[q] = field_ptr(F, J.prew_sel[q]);
-        a.post[q] = J.post[q];
-        a.postw[q] = field_ptr(F, J.postw_sel[q]);
-        a.posts[q] = (R)J.posts[q];
-      }
-      a.lt = lt_override ? lt_override : field_ptr(F, J.lt_sel);
-      a.out_real = J.out_real;
-      a.spec = spec;
-      a.chan_stride = cstride;
-      a.tw = F.tw[0];
-      a.pl = c->plan[0];
-      const int rows = g.ny * g.nz;
-      if (!c->cplx) {
-        a.rb = rows_per_wg(g.nx, J.nch, true);
-        const int npen = J.nch * (a.rb / 2);
-        const size_t sm = sizeof(cx<R>) * ((size_t)g.nx + 2 * (size_t)npen * g.nx);
-        const int nblk = (rows + a.rb - 1) / a.rb;
-        hipLaunchKernelGGL(k_xfwd_real<R>, dim3(nblk * nc), dim3(kThreads), sm,
-                           c->stream, a);
-      } else {
-        a.rb = rows_per_wg(g.nx, J.nch, false);
-        const int npen = J.nch * a.rb;
-        const size_t sm = sizeof(cx<R>) * ((size_t)g.nx + 2 * (size_t)npen * g.nx);
-        const int nblk = (rows + a.rb - 1) / a.rb;
-        hipLaunchKernelGGL(k_xfwd_cplx<R>, dim3(nblk * nc), dim3(kThreads), sm,
-                           c->stream, a);
-      }
-      BDGX_HIP(hipGetLastError());
-      base += J.nch;
+      XArgs<R> a = make_xargs<R>(c, jobs[j], base, nc, Xc, Yc, ld, spec, cstride, lt_override);
+      if ((rc = launch_x<R>(c, a, false))) return rc;
+      base += jobs[j].nch;
     }
-    // y / z passes over all C*nc batches
-    YZArgs<R> yz;
-    memset(&yz, 0, sizeof(yz));
-    yz.g = g;
-    yz.ncols = nc;
-    yz.nb = C * nc;
-    yz.P = sizeof(R) == 8 ? 8 : 16;
-    if (yz.P > g.n0k) yz.P = g.n0k;
-    yz.spec = spec;
-    {
-      int ch = 0;
-      for (int j = 0; j < njobs; ++j)
-        for (int q = 0; q < jobs[j].nch; ++q) yz.mul[ch++] = jobs[j].mul[q];
-    }
-    yz.kx2 = F.kx2[0];
-    yz.ky2 = F.kx2[1];
-    yz.kz2 = F.kx2[2];
-    yz.U = F.U;
-    yz.mu = (R)c->mu;
-    yz.invS = (R)(1.0 / (double)g.S);
-    yz.dt = (R)c->itp_dt;
-    const int ntile = (g.n0k + yz.P - 1) / yz.P;
-    if (g.ny > 1) {
-      yz.tw = F.tw[1];
-      yz.pl = c->plan[1];
-      const size_t sm = sizeof(cx<R>) * ((size_t)g.ny + 2 * (size_t)yz.P * (g.ny + 1));
-      hipLaunchKernelGGL((k_ypass<R, false>), dim3(ntile * g.nz * yz.nb), dim3(kThreads),
-                         sm, c->stream, yz);
-      BDGX_HIP(hipGetLastError());
-    }
-    {
-      YZArgs<R> z = yz;
-      z.tw = F.tw[2];
-      z.pl = c->plan[2];
-      const size_t sm = sizeof(cx<R>) * ((size_t)g.nz + 2 * (size_t)z.P * (g.nz + 1));
-      hipLaunchKernelGGL(k_zpass<R>, dim3(ntile * g.ny * z.nb), dim3(kThreads), sm,
-                         c->stream, z);
-      BDGX_HIP(hipGetLastError());
-    }
-    if (g.ny > 1) {
-      yz.tw = F.tw[1];
-      yz.pl = c->plan[1];
-      const size_t sm = sizeof(cx<R>) * ((size_t)g.ny + 2 * (size_t)yz.P * (g.ny + 1));
-      hipLaunchKernelGGL((k_ypass<R, true>), dim3(ntile * g.nz * yz.nb), dim3(kThreads),
-                         sm, c->stream, yz);
-      BDGX_HIP(hipGetLastError());
-    }
-    // inverse x + combine
+    const YZArgs<R> yz = make_yz<R>(c, mul, C, nc, spec);
+    if ((rc = launch_y<R>(c, yz, false))) return rc;
+    if ((rc = launch_z<R>(c, yz))) return rc;
+    if ((rc = launch_y<R>(c, yz, true))) return rc;
     base = 0;
     for (int j = 0; j < njobs; ++j) {
-      const HalfJob &J = jobs[j];
-      XArgs<R> a;
-      memset(&a, 0, sizeof(a));
-      a.g = g;
-      a.ncols = nc;
-      a.ld = ld;
-      a.X = Xc;
-      a.Y = Yc;
-      a.in_half = J.in_half;
-      a.out_half = J.out_half;
-      a.nch = J.nch;
-      a.chan_base = base;
-      for (int q = 0; q < J.nch; ++q) {
-        a.pre[q] = J.pre[q];
-        a.prew[q] = field_ptr(F, J.prew_sel[q]);
-        a.post[q] = J.post[q];
-        a.postw[q] = field_ptr(F, J.postw_sel[q]);
-        a.posts[q] = (R)J.posts[q];
-      }
-      a.lt = lt_override ? lt_override : field_ptr(F, J.lt_sel);
-      a.out_real = J.out_real;
-      a.spec = spec;
-      a.chan_stride = cstride;
-      a.tw = F.tw[0];
-      a.pl = c->plan[0];
-      const int rows = g.ny * g.nz;
-      if (!c->cplx) {
-        a.rb = rows_per_wg(g.nx, 1, true);
-        if (J.nch == 2) {
-          // one pencil per row: ~1024 elements
-          int rb = 1024 / g.nx;
-          if (rb < 2) rb = 2;
-          a.rb = rb & ~1;
-        }
-        const int npen = J.nch == 2 ? a.rb : a.rb / 2;
-        const size_t sm = sizeof(cx<R>) * ((size_t)g.nx + 2 * (size_t)npen * g.nx);
-        const int nblk = (rows + a.rb - 1) / a.rb;
-        hipLaunchKernelGGL(k_xinv_real<R>, dim3(nblk * nc), dim3(kThreads), sm,
-                           c->stream, a);
-      } else {
-        a.rb = rows_per_wg(g.nx, J.nch, false);
-        const int npen = J.nch * a.rb;
-        const size_t sm = sizeof(cx<R>) * ((size_t)g.nx + 2 * (size_t)npen * g.nx);
-        const int nblk = (rows + a.rb - 1) / a.rb;
-        hipLaunchKernelGGL(k_xinv_cplx<R>, dim3(nblk * nc), dim3(kThreads), sm,
-                           c->stream, a);
-      }
-      BDGX_HIP(hipGetLastError());
-      base += J.nch;
+      XArgs<R> a = make_xargs<R>(c, jobs[j], base, nc, Xc, Yc, ld, spec, cstride, lt_override);
+      if ((rc = launch_x<R>(c, a, true))) return rc;
+      base += jobs[j].nch;
     }
   }
   return BDGX_OK;
@@ -780,6 +781,68 @@ int itp_kinetic_t(bdgx_ctx *c, R *dpsi, double dt) {
   if (dt >= 0.0) J.mul[0] = MUL_ITPKIN;
   c->itp_dt = dt >= 0.0 ? dt : 0.0;
   return run_jobs<R>(c, &J, 1, dpsi, dpsi, 1, c->geo.S, nullptr);
+}
+
+// Times one pass kind over nbatch single-channel (kinetic) batches.
+template <typename R>
+static int bench_pass_t(bdgx_ctx *c, int pass, uint64_t nbatch, int iters, float *ms,
+                        double *bytes) {
+  const Geo &g = c->geo;
+  const int nc = (int)nbatch;
+  const int cw = c->cplx ? 2 : 1;
+  const size_t es = sizeof(R) * cw;
+  int rc = ensure_spec(c, sizeof(cx<R>) * (size_t)g.Kk * nbatch);
+  if (rc) return rc;
+  cx<R> *spec = reinterpret_cast<cx<R> *>(c->spec);
+  R *X = nullptr, *Y = nullptr;
+  BDGX_HIP(hipMalloc((void **)&X, es * (size_t)g.S * nbatch));
+  if (hipMalloc((void **)&Y, es * (size_t)g.S * nbatch) != hipSuccess) {
+    (void)hipFree(X);
+    return BDGX_EHIP;
+  }
+  (void)hipMemsetAsync(X, 0, es * (size_t)g.S * nbatch, c->stream);
+  (void)hipMemsetAsync(spec, 0, sizeof(cx<R>) * (size_t)g.Kk * nbatch, c->stream);
+  HalfJob J = job_kin(0, F_NONE);
+  int mul[kMaxChan] = {MUL_KIN, MUL_KIN, MUL_KIN};
+  const long long cstride = (long long)nc * g.Kk;
+  XArgs<R> a = make_xargs<R>(c, J, 0, nc, X, Y, (long long)g.S, spec, cstride, nullptr);
+  const YZArgs<R> yz = make_yz<R>(c, mul, 1, nc, spec);
+  const double specb = (double)sizeof(cx<R>) * (double)g.Kk * (double)nbatch;
+  const double realb = (double)es * (double)g.S * (double)nbatch;
+  switch (pass) {
+    case BDGX_PASS_XFWD: *bytes = realb + specb; break;
+    case BDGX_PASS_XINV: *bytes = specb + realb; break;
+    default: *bytes = 2.0 * specb; break;
+  }
+  auto one = [&]() -> int {
+    switch (pass) {
+      case BDGX_PASS_XFWD: return launch_x<R>(c, a, false);
+      case BDGX_PASS_YFWD: return launch_y<R>(c, yz, false);
+      case BDGX_PASS_Z: return launch_z<R>(c, yz);
+      case BDGX_PASS_YINV: return launch_y<R>(c, yz, true);
+      case BDGX_PASS_XINV: return launch_x<R>(c, a, true);
+      default: return BDGX_EINVAL;
+    }
+  };
+  rc = one();  // warm-up
+  if (!rc && hipEventRecord(c->ev0, c->stream) != hipSuccess) rc = BDGX_EHIP;
+  for (int i = 0; i < iters && !rc; ++i) rc = one();
+  if (!rc && hipEventRecord(c->ev1, c->stream) != hipSuccess) rc = BDGX_EHIP;
+  if (!rc && hipEventSynchronize(c->ev1) != hipSuccess) rc = BDGX_EHIP;
+  float t = 0.f;
+  if (!rc && hipEventElapsedTime(&t, c->ev0, c->ev1) != hipSuccess) rc = BDGX_EHIP;
+  *ms = iters > 0 ? t / (float)iters : 0.f;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(X);
+  (void)hipFree(Y);
+  return rc;
+}
+
+int bench_pass_dispatch(bdgx_ctx *c, int pass, int prec, uint64_t nbatch, int iters, float *ms,
+                        double *bytes) {
+  if (prec == BDGX_FP64) return bench_pass_t<double>(c, pass, nbatch, iters, ms, bytes);
+  if (prec == BDGX_FP32) return bench_pass_t<float>(c, pass, nbatch, iters, ms, bytes);
+  return BDGX_EINVAL;
 }
 
 int apply_dispatch(bdgx_ctx *c, int op, int prec, const void *dX, void *dY,

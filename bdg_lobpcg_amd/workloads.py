@@ -1,0 +1,83 @@
+"""Synthetic BdG workloads (the benchmark configurations) built through the
+public API in the reference's setup order (examples/3d_jens.c:53-67):
+system -> trap -> wavefunction -> local interactions -> dipolar -> mu.
+
+Conventions settled here (the configuration text leaves them open):
+  * psi0: normalised Gaussian of width 1 (sum |psi|^2 dV = 1), relaxed by
+    `itp_steps` imaginary-time split-step steps of `dt` on the device
+    (bdgx_itp_groundstate), mu = <psi| H_GP |psi> of the final state;
+  * contact: U_K = U_M = g * density;
+  * dipolar coupling g_ddi = g * eps_dd (the reference examples' convention,
+    examples/3d_dipolar_atom.c:39-48), dipoles along z, cutoff R as given
+    (configs without one use R = min(L) / 2);
+  * start block: BDG_INIT_CUSTOM with xorshift32 seed 42, u = v = xrand - 0.5.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+CONFIGS = {
+    "cfg0_32c_d": dict(N=(32, 32, 32), L=(16.0, 16.0, 16.0), omega=(1.0, 1.0, 1.0), g=100.0,
+                       edd=0.5, R=8.0, itp_steps=2000, dt=1e-3, nev=10, m=10, tol=1e-8,
+                       cplx=False),
+    "cfg1_128c_d": dict(N=(128, 128, 128), L=(24.0, 24.0, 48.0), omega=(1.0, 1.0, 0.5),
+                        g=2000.0, edd=0.8, R=12.0, itp_steps=4000, dt=1e-3, nev=32, m=32,
+                        tol=1e-8, cplx=False),
+    "cfg2_96x96x192_z": dict(N=(96, 96, 192), L=(20.0, 20.0, 80.0), omega=(3.0, 3.0, 0.3),
+                             g=3000.0, edd=0.95, R=10.0, itp_steps=2000, dt=1e-3, nev=64, m=64,
+                             tol=1e-8, cplx=True, modulation=0.3, mod_period=4.0, vortex=True),
+    "cfg4_256c_d": dict(N=(256, 256, 256), L=(32.0, 32.0, 32.0), omega=(1.0, 1.0, 2.0),
+                        g=5000.0, edd=0.7, R=16.0, itp_steps=2000, dt=1e-3, nev=128, m=128,
+                        tol=1e-8, cplx=False),
+}
+
+
+def coords(N, L):
+    xs = [(np.arange(n) - n / 2) * l / n for n, l in zip(N, L)]
+    Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+    return X.ravel(), Y.ravel(), Z.ravel()
+
+
+def initial_psi(cfg):
+    X, Y, Z = coords(cfg["N"], cfg["L"])
+    psi = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 2.0)
+    if cfg.get("modulation"):
+        psi = psi * (1.0 + cfg["modulation"] * np.cos(2 * np.pi * Z / cfg["mod_period"]))
+    if cfg["cplx"]:
+        psi = psi.astype(np.complex128)
+        if cfg.get("vortex"):
+            psi = psi * np.exp(1j * np.arctan2(Y, X))
+    dV = np.prod([l / n for n, l in zip(cfg["N"], cfg["L"])])
+    return psi / np.sqrt(np.sum(np.abs(psi) ** 2) * dV)
+
+
+def ground_state(cfg, steps=None):
+    """psi0 and mu by device imaginary-time evolution; returns (psi, mu, seconds)."""
+    import bdg_lobpcg_amd as B
+
+    d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
+    t0 = time.perf_counter()
+    psi, mu = d.itp_groundstate(initial_psi(cfg), cfg["omega"], cfg["g"], cfg["g"] * cfg["edd"],
+                                (0.0, 0.0, 1.0), cfg["R"], cfg["dt"],
+                                cfg["itp_steps"] if steps is None else steps)
+    dt = time.perf_counter() - t0
+    d.close()
+    return psi, mu, dt
+
+
+def make_problem(cfg, psi, mu, precision=0, maxiter=1000, tol=None, nev=None, m=None):
+    import bdg_lobpcg_amd as B
+
+    p = B.Problem(cfg["N"], cfg["L"], cfg["cplx"])
+    p.set_system()
+    p.set_trap_harmonic(cfg["omega"])
+    p.set_wavefunction(psi)
+    p.set_contact(cfg["g"])
+    p.set_dipolar(cfg["g"] * cfg["edd"], (0.0, 0.0, 1.0), cfg["R"])
+    p.set_mu(mu)
+    p.set_solver_params(nev or cfg["nev"], m or cfg["m"], maxiter, tol or cfg["tol"])
+    p.set_precision(precision)
+    p.set_init_uniform(42)
+    return p

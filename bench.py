@@ -1,0 +1,172 @@
+"""Benchmark: time-to-solution for 32 BdG modes at 128^3 on one MI355X.
+
+One step = one full bdg_solve (plain FP64 policy and mixed FP32 + FP64
+refinement policy) of the 128^3 dipolar cigar (real psi0, nev = 32, block
+32, tol 1e-8) with psi0 and all fields resident on the device.  `value` is
+the mixed policy's mean time-to-solution; the FP64 policy's time and the
+speed-up are reported beside it.  The FFT-pass roofline is measured live
+with HIP events on the context stream (bdgx_bench_pass); the CPU baseline is
+the FP64 C oracle (one vector at a time, as the reference) on a bounded
+sample, scaled by the column applies of the FP64 device solve.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def rel_err_modes(w, ref, zero):
+    nz = ~zero
+    e_rel = float(np.max(np.abs(w[nz] - ref[nz]) / np.abs(ref[nz]))) if nz.any() else 0.0
+    e_abs0 = float(np.max(np.abs(w[zero] - ref[zero]))) if zero.any() else 0.0
+    return e_rel, e_abs0
+
+
+def solve_timed(cfg, psi, mu, precision, steps, warmup):
+    from bdg_lobpcg_amd.workloads import make_problem
+
+    times, last = [], None
+    for i in range(warmup + steps):
+        p = make_problem(cfg, psi, mu, precision=precision)
+        p.set_verbosity(1)
+        t0 = time.perf_counter()
+        rc = p.solve()
+        t = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"bdg_solve failed with {rc}")
+        st = p.stats()
+        w = p.eigenvalues(cfg["nev"])
+        log(f"precision={precision} solve {i} ({'warmup' if i < warmup else 'timed'}): {t:.3f} s,"
+            f" iters {st['iterations']} (fp32 {st['iterations_fp32']}, fp64 {st['iterations_fp64']}),"
+            f" converged {p.converged()}/{cfg['nev']}")
+        if i >= warmup:
+            times.append(t)
+        last = dict(w=w, zero=p.zero_modes(cfg["nev"]), res=p.residuals(cfg["nev"]), stats=st,
+                    conv=p.converged(), arrays=p.arrays() if i == warmup + steps - 1 else None)
+        p.close()
+    return float(np.mean(times)), last
+
+
+def roofline(cfg):
+    import bdg_lobpcg_amd as B
+
+    d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
+    out = {}
+    for prec, name in ((B.FP32, "fp32"), (B.FP64, "fp64")):
+        for pid, pname in enumerate(["xfwd", "yfwd", "z", "yinv", "xinv"]):
+            ms, by = d.bench_pass(pid, prec, cfg["m"], iters=10)
+            out[f"{name}_{pname}"] = dict(ms=ms, bytes=by, GBs=by / (ms * 1e-3) / 1e9)
+    d.close()
+    return out
+
+
+def cpu_baseline(cfg, psi, arrays, column_applies):
+    """FP64 C oracle: one column of (M then K) and (T_K then T_M) timed on the
+    host cores, scaled by the FP64 device solve's column applies."""
+    from oracle import oracle as O
+
+    c = O.Ctx(cfg["N"], cfg["L"], cfg["cplx"])
+    c.set_fields(wf=psi)
+    c.set_dipolar(cfg["g"] * cfg["edd"], (0.0, 0.0, 1.0), cfg["R"])
+    c.set_fields(ltK=arrays["ltK"], ltM=arrays["ltM"], pK=arrays["pK"], pM=arrays["pM"],
+                 mu=arrays["mu"])
+    rng = np.random.default_rng(0)
+    ncol = 2
+    V = rng.standard_normal((c.size, ncol))
+    t0 = time.perf_counter()
+    for j in range(ncol):
+        c.apply("K", c.apply("M", V[:, j]))
+        c.apply("precondM", c.apply("precondK", V[:, j]))
+    t = (time.perf_counter() - t0) / ncol
+    return dict(value=t * column_applies, unit="s", cores=O.num_threads(), kind="port",
+                sample=f"{ncol} columns of (M then K) + (precondK then precondM) at "
+                       f"{'x'.join(map(str, cfg['N']))} FP64 ({t:.3f} s/column), scaled by "
+                       f"{column_applies} column applies of the FP64 device solve; "
+                       f"operator applies only (Gram/Rayleigh-Ritz excluded)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg1_128c_d")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: the solver runs on a single GPU "
+              "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import bdg_lobpcg_amd as B  # noqa: F401  (fails loudly without the native library)
+    from bdg_lobpcg_amd.workloads import CONFIGS, ground_state
+
+    cfg = CONFIGS[args.config]
+    log(f"config {args.config}: N={cfg['N']} nev={cfg['nev']} ITP {cfg['itp_steps']} steps")
+    psi, mu, t_itp = ground_state(cfg)
+    log(f"psi0 by device ITP: {t_itp:.2f} s, mu = {mu:.10f}")
+    t64, r64 = solve_timed(cfg, psi, mu, 0, args.steps, args.warmup)
+    tmx, rmx = solve_timed(cfg, psi, mu, 1, args.steps, args.warmup)
+    zero = r64["zero"] | rmx["zero"]
+    e_rel, e_abs0 = rel_err_modes(rmx["w"], r64["w"], zero)
+    log(f"FP64 {t64:.3f} s, mixed {tmx:.3f} s, speed-up {t64 / tmx:.2f}x, "
+        f"max rel err mixed vs FP64 {e_rel:.2e}")
+    rf = roofline(cfg)
+    dom = rf["fp32_z"]
+    log("passes: " + ", ".join(f"{k} {v['GBs']:.0f} GB/s" for k, v in rf.items()))
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, psi, r64["arrays"], r64["stats"]["column_applies"])
+        log(f"cpu baseline {cpu['value']:.1f} s ({cpu['cores']} threads)")
+    line = {
+        "metric": "time-to-solution, 32 BdG modes at 128^3: mixed+learned vs plain FP64 on GPU",
+        "value": tmx,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tmx * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (psi0 from device imaginary-time evolution of a Gaussian)",
+        "config": {"workload": f"{args.config}: 128^3 dipolar cigar, real psi0, nev 32, block 32, "
+                               f"tol 1e-8, g 2000, eps_dd 0.8", "grid": list(cfg["N"]),
+                   "nev": cfg["nev"], "parallelism": "single GPU"},
+        "fp64_seconds": t64,
+        "mixed_seconds": tmx,
+        "speedup_vs_fp64": t64 / tmx,
+        "mixed_vs_fp64_max_rel_err": e_rel,
+        "goldstone_abs_diff": e_abs0,
+        "iterations_fp64": r64["stats"]["iterations"],
+        "iterations_mixed": [rmx["stats"]["iterations_fp32"], rmx["stats"]["iterations_fp64"]],
+        "converged": [r64["conv"], rmx["conv"]],
+        "omega_fp64": [float(x) for x in r64["w"]],
+        "psi0_seconds": t_itp,
+        "learned_preconditioner": "not built yet: the mixed policy uses the FFT-diagonal preconditioner",
+        "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_zpass<float> (fused z forward, k-space multiply, z inverse)",
+                     "bytes_per_launch": dom["bytes"], "ms_per_launch": dom["ms"]},
+        "passes": {k: round(v["GBs"], 1) for k, v in rf.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,12 @@ void bdg_set_verbosity(bdg_t *bdg, int level);
 f64 bdg_cb_harmonic_trap(uint64_t dim, const f64 *r, const void *param);
 f64 bdg_cb_contact(const void *param, f64 density);
 
+/* Built-in bdg_init_fn for BDG_INIT_CUSTOM: u = v = xrand - 0.5 from
+ * xorshift32 (reference src/bdg_internal.h:53-65), column by column, with
+ * param = const uint32_t *seed. */
+void bdg_init_uniform_xorshift(const bdg_t *bdg, void *X, uint64_t n, uint64_t sizeSub,
+                               void *param);
+
 #ifdef __cplusplus
 }
 #endif

@@ -131,6 +131,17 @@ int bdgx_timer_end(bdgx_ctx *ctx, float *ms);
 /* Bytes of device workspace the context currently holds. */
 uint64_t bdgx_workspace_bytes(const bdgx_ctx *ctx);
 
+/* Measurement hook (no reference counterpart): launches one FFT pass kind
+ * `iters` times over `nbatch` spectrum batches (one kinetic channel per
+ * batch) between HIP events on the ctx stream and reports the mean launch
+ * time and the pass's algorithmic HBM bytes per launch.
+ *   pass: 0 x-forward (r2c/c2c), 1 y-forward, 2 z fused (fwd, multiply, inv),
+ *         3 y-inverse, 4 x-inverse + combine. */
+enum { BDGX_PASS_XFWD = 0, BDGX_PASS_YFWD = 1, BDGX_PASS_Z = 2, BDGX_PASS_YINV = 3,
+       BDGX_PASS_XINV = 4 };
+int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iters,
+                    float *ms_per_launch, double *bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,10 @@ and runs block LOBPCG (Knyazev 2001) on the pencil (MKM, M) for the v half:
     precondM (src/precond_impl.inc:21-111), i.e. an approximation of
     (M K M)^-1 M applied to R = KMx - w^2 x;
   * soft locking: converged columns get no new W / P; when [X, W, P] is
-    numerically dependent the P block is dropped for one iteration;
+    numerically dependent the P block is dropped for one iteration, and when
+    [X, W] still is, W falls back to the plain residual R for that iteration
+    (guards against preconditioners that blow one direction up, e.g.
+    safe_val(mu + 0) = 1e-8 at k = 0 for mu = 0);
   * a pair converges when ||A x - w B x|| <= tol * ||x|| * max(1, |w|), with
     x = [M v / w; v].  The Goldstone mode (|w| below zero_tol * scale) is
     reported and counted as converged (for a non-stationary psi0 its w^2 is
@@ -92,6 +95,15 @@ def gram_ok(GV):
     return float(np.min(np.abs(np.diag(L)))) ** 2 >= TAU_CHOL
 
 
+def solver_precond_weights(ltK, ltM, mu):
+    """The solver's preconditioner weights: the reference's
+    1/sqrt(max(1e-8, lt)) (src/setup.c:165-166) on the pre-mu local term,
+    with the floor raised to mu (same rule as the device solve)."""
+    fl = mu if mu > 1e-8 else 1e-8
+    return (1.0 / np.sqrt(np.maximum(fl, np.asarray(ltK) + mu)),
+            1.0 / np.sqrt(np.maximum(fl, np.asarray(ltM) + mu)))
+
+
 def signed_sqrt(w2):
     return np.sign(w2) * np.sqrt(np.abs(w2))
 
@@ -120,6 +132,7 @@ def ilobpcg(applyK, applyM, applyTK, applyTM, V0, nev, tol=1e-8, maxiter=500,
     res = None
     hist = []
     restarts = 0
+    precond_fallbacks = 0
     applies = m
     for it in range(1, maxiter + 1):
         R = KMX - X * w2[None, :]
@@ -142,13 +155,28 @@ def ilobpcg(applyK, applyM, applyTK, applyTM, V0, nev, tol=1e-8, maxiter=500,
         blocks = [(X, MX, KMX), (W, MW, KMW)]
         if P is not None:
             blocks.append((P[:, act], MP[:, act], KMP[:, act]))
+        fallback = False
         while True:
             V = np.concatenate([b[0] for b in blocks], axis=1)
             GV = V.conj().T @ V
-            if len(blocks) == 2 or gram_ok(GV):
+            if gram_ok(GV):
                 break
-            blocks = blocks[:2]
-            restarts += 1
+            if len(blocks) == 3:
+                blocks = blocks[:2]
+                restarts += 1
+                continue
+            if fallback:
+                break
+            # the preconditioned block is numerically dependent on X (e.g. a
+            # preconditioner that amplifies one direction by 1e8): use the
+            # plain residual for this iteration
+            W = R[:, act]
+            MW = applyM(W)
+            KMW = applyK(MW)
+            applies += len(act)
+            blocks = [(X, MX, KMX), (W, MW, KMW)]
+            fallback = True
+            precond_fallbacks += 1
         MV = np.concatenate([b[1] for b in blocks], axis=1)
         KMV = np.concatenate([b[2] for b in blocks], axis=1)
         w2, Cc, flags = rayleigh_ritz(GV, V.conj().T @ MV, MV.conj().T @ KMV, m)
@@ -165,4 +193,5 @@ def ilobpcg(applyK, applyM, applyTK, applyTM, V0, nev, tol=1e-8, maxiter=500,
     nrm = np.sqrt(np.linalg.norm(U, axis=0) ** 2 + np.linalg.norm(X, axis=0) ** 2)
     return dict(w=w[:nev], v=(X / nrm)[:, :nev], u=(U / nrm)[:, :nev], iters=it,
                 res=res[:nev], hist=hist, zero=zero[:nev], restarts=restarts,
+                precond_fallbacks=precond_fallbacks,
                 applies=applies, w2=w2[:nev])

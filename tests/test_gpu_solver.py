@@ -32,11 +32,13 @@ def test_3d_dipolar_vs_oracle(prec):
     """Small 3D dipolar problem: device solve vs the CPU reference solver
     (numpy iLOBPCG on the oracle operators), same psi0 and start block."""
     import bdg_lobpcg_amd as B
-    from oracle.ilobpcg_ref import ilobpcg
+    from oracle.ilobpcg_ref import ilobpcg, solver_precond_weights
     N, L = (16, 16, 16), (12.0, 12.0, 12.0)
-    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=None or 1.0)
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
     S = c.size
     nev, m = 6, 8
+    pKs, pMs = solver_precond_weights(f["ltK"], f["ltM"], f["mu"])
+    c.set_fields(pK=pKs, pM=pMs, mu=f["mu"])
     X0 = c.start_block_uniform(42, m)
     ref = ilobpcg(lambda X: c.apply("K", X), lambda X: c.apply("M", X),
                   lambda X: c.apply("precondK", X), lambda X: c.apply("precondM", X),
