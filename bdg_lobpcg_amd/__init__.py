@@ -122,6 +122,9 @@ def lib():
         "bdg_mu": (D, [P]),
         "bdg_set_mixed_params": (None, [P, D, U64]),
         "bdg_set_verbosity": (None, [P, I]),
+        "bdg_start_block": (None, [P, P]),
+        "bdg_load_wavefunction_fmt": (I, [P, C.c_char_p]),
+        "bdgx_reset_fields": (I, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -402,6 +405,12 @@ class Problem:
     def set_init_mode(self, mode, hint=None):
         lib().bdg_set_init_mode(self.h, int(mode), None,
                                 C.c_void_p(int(hint)) if hint else None)
+
+    def start_block(self, nrows_cols) -> np.ndarray:
+        n, m = nrows_cols
+        X = np.zeros((n, m), dtype=np.complex128 if self.cplx else np.float64, order="F")
+        lib().bdg_start_block(self.h, _p(X))
+        return X
 
     def solve(self) -> int:
         return int(lib().bdg_solve(self.h))

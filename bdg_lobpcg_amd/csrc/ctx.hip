@@ -335,6 +335,14 @@ int bdgx_set_fields(bdgx_ctx *c, const double *ltK, const double *ltM, const voi
   return BDGX_OK;
 }
 
+int bdgx_reset_fields(bdgx_ctx *c) {
+  if (!c) return BDGX_EINVAL;
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  c->has_wf = c->has_U = c->has_lt = c->has_p = 0;
+  c->mu = c->g_ddi = 0.0;
+  return BDGX_OK;
+}
+
 int bdgx_dipolar_set_kernel(bdgx_ctx *c, double g_ddi, const double *dir, double R) {
   if (!c || !dir || c->dim != 3) return BDGX_EINVAL;
   BDGX_HIP(hipStreamSynchronize(c->stream));

@@ -88,13 +88,8 @@ void bdg_reset(bdg_t *b) {
   b->dipolar = 0;
   free_results(b);
   /* the device context keeps grid, plans and workspaces; its physics fields
-   * are re-uploaded by the next bdg_solve / bdg_set_dipolar */
-  if (b->dev) {
-    bdgx_ctx_destroy(b->dev);
-    b->dev = NULL;
-    if (bdgx_ctx_create(&b->dev, b->dim, b->N, b->L, b->complex_psi0) != BDGX_OK)
-      BDG_FAIL("bdg_reset: device context re-creation failed");
-  }
+   * are marked absent and re-uploaded by the next bdg_set_dipolar / bdg_solve */
+  if (b->dev && bdgx_reset_fields(b->dev) != BDGX_OK) BDG_FAIL("bdg_reset: device reset failed");
   b->state = BDG_HAS_SYSTEM;
 }
 
@@ -113,11 +108,15 @@ void bdg_set_system(bdg_t *b) {
       b->kx2[d][j] = kj * kj;
     }
   }
-  if (!b->dev) {
-    const int rc = bdgx_ctx_create(&b->dev, b->dim, b->N, b->L, b->complex_psi0);
-    if (rc) BDG_FAIL("bdg_set_system: device context failed: %s", bdgx_error_string(rc));
-  }
+  /* the device context is created on first device use (ensure_device) */
   b->state |= BDG_HAS_SYSTEM;
+}
+
+/* Create the device context on first use: lets the host-side setup run (and
+ * be tested) on machines without a GPU. */
+static int ensure_device(bdg_t *b) {
+  if (b->dev) return BDGX_OK;
+  return bdgx_ctx_create(&b->dev, b->dim, b->N, b->L, b->complex_psi0);
 }
 
 /* reference src/setup.c:17-84: r_d = (i - N_d/2) L_d / N_d, additive */
@@ -203,7 +202,8 @@ void bdg_set_dipolar(bdg_t *b, f64 g_ddi, const f64 *dir, f64 cutoff) {
   BDG_FORBID(b, BDG_HAS_MU, "bdg_set_dipolar");
   if (3 != b->dim)
     BDG_FAIL("bdg_set_dipolar: dipolar interactions require dim == 3 (got %" PRIu64 ")", b->dim);
-  int rc = bdgx_set_fields(b->dev, NULL, NULL, b->wf, NULL, NULL, 0.0);
+  int rc = ensure_device(b);
+  if (!rc) rc = bdgx_set_fields(b->dev, NULL, NULL, b->wf, NULL, NULL, 0.0);
   if (!rc) rc = bdgx_dipolar_set_kernel(b->dev, g_ddi, dir, cutoff);
   double *gphi = (double *)xcalloc(b->size, sizeof(double));
   if (!rc) rc = bdgx_dipolar_meanfield(b->dev, gphi);
@@ -421,6 +421,22 @@ void bdg_fill_start_block(bdg_t *b, void *X) {
   }
 }
 
+void bdg_start_block(bdg_t *b, void *X) {
+  BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_start_block");
+  const bdg_init_mode_t mode = b->init_mode;
+  void *keep = b->reuse_buf;
+  if (mode == BDG_INIT_REUSE && keep) {
+    /* copy without consuming the buffer */
+    const size_t es = sizeof(double) * (b->complex_psi0 ? 2 : 1);
+    if (b->reuse_n == 2 * b->size && b->reuse_cols == b->sizeSub) {
+      memcpy(X, keep, 2 * b->size * b->sizeSub * es);
+      return;
+    }
+  }
+  bdg_fill_start_block(b, X);
+  b->init_mode = mode;
+}
+
 /* ---- solve ---------------------------------------------------------------- */
 
 /* The solver's preconditioner weights: the reference's
@@ -431,6 +447,8 @@ void bdg_fill_start_block(bdg_t *b, void *X) {
  * condensate lt ~ mu, so the clamp only changes those points.  The operator
  * API (bdgx_apply PRECOND_*) keeps the reference weights. */
 static int upload_fields(bdg_t *b) {
+  const int drc = ensure_device(b);
+  if (drc) return drc;
   const double floor_v = b->mu > 1e-8 ? b->mu : 1e-8;
   double *pKs = (double *)xcalloc(b->size, sizeof(double));
   double *pMs = (double *)xcalloc(b->size, sizeof(double));
@@ -607,7 +625,7 @@ void bdg_get_solve_stats(const bdg_t *b, bdg_solve_stats_t *out) {
 }
 const f64 *bdg_residuals(const bdg_t *b) { return b->residuals; }
 const int *bdg_zero_modes(const bdg_t *b) { return b->zero_modes; }
-bdgx_ctx *bdg_device_context(bdg_t *b) { return b->dev; }
+bdgx_ctx *bdg_device_context(bdg_t *b) { return ensure_device(b) == BDGX_OK ? b->dev : NULL; }
 const f64 *bdg_local_term_K(const bdg_t *b) { return b->ltK; }
 const f64 *bdg_local_term_M(const bdg_t *b) { return b->ltM; }
 const f64 *bdg_precond_weight_K(const bdg_t *b) { return b->pK; }

@@ -66,6 +66,11 @@ void bdg_set_verbosity(bdg_t *bdg, int level);
 f64 bdg_cb_harmonic_trap(uint64_t dim, const f64 *r, const void *param);
 f64 bdg_cb_contact(const void *param, f64 density);
 
+/* Fill X (n x sizeSub column-major, f64 or c64) with the start block the next
+ * bdg_solve would use for the current init mode (does not consume a REUSE
+ * buffer).  Needs bdg_set_system. */
+void bdg_start_block(bdg_t *bdg, void *X);
+
 /* Built-in bdg_init_fn for BDG_INIT_CUSTOM: u = v = xrand - 0.5 from
  * xorshift32 (reference src/bdg_internal.h:53-65), column by column, with
  * param = const uint32_t *seed. */

@@ -66,6 +66,10 @@ int bdgx_set_fields(bdgx_ctx *ctx, const double *ltK, const double *ltM,
                     const void *wf, const double *pK, const double *pM,
                     double mu);
 
+/* Mark every physics field absent (allocations are kept); the next uploads
+ * re-define the problem.  Device side of bdg_reset (reference src/bdg.c:55-84). */
+int bdgx_reset_fields(bdgx_ctx *ctx);
+
 /* Dipolar kernel on the device (reference src/dipolar.c:21-72). */
 int bdgx_dipolar_set_kernel(bdgx_ctx *ctx, double g_ddi, const double *dir,
                             double cutoff_radius);

@@ -1,0 +1,200 @@
+"""CPU tests of the product's host layer (include/bdg/bdg.h implemented in
+C): exported symbols, setup arrays, misuse exits, start blocks and file I/O.
+No device call happens here (the device context is created on first use)."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import bdg_lobpcg_amd as B
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for hdr in ("include/bdg/bdg.h", "include/bdg/bdg_amd.h", "include/bdgx.h"):
+        txt = open(os.path.join(ROOT, hdr)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b(bdgx?_[a-z0-9_]+)\s*\(", txt):
+            if not txt[m.end():].lstrip().startswith("*"):
+                names.add(m.group(1))
+    names.discard("bdg_init_fn")
+    return sorted(names)
+
+
+def test_library_exports_every_declared_function():
+    lib = B.lib()
+    missing = [n for n in _declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert len(_declared_functions()) >= 50
+
+
+def test_reference_api_complete():
+    """The 22 functions of reference include/bdg/bdg.h:67-244."""
+    ref = ["bdg_alloc", "bdg_free", "bdg_reset", "bdg_set_system", "bdg_set_trap",
+           "bdg_set_wavefunction", "bdg_set_local_interactions", "bdg_set_mu", "bdg_set_dipolar",
+           "bdg_set_solver_params", "bdg_set_init_mode", "bdg_reuse_modes", "bdg_solve",
+           "bdg_converged", "bdg_eigenvalues", "bdg_modes_u", "bdg_modes_v",
+           "bdg_load_wavefunction", "bdg_load_wavefunction_fmt", "bdg_write_eigenvalues",
+           "bdg_write_mode_u", "bdg_write_mode_v"]
+    assert len(ref) == 22
+    lib = B.lib()
+    assert all(hasattr(lib, n) for n in ref)
+
+
+def test_setup_arrays_match_oracle():
+    """trap + contact + mu through the C host layer vs the oracle
+    (reference tests/test_setup.c:299-500)."""
+    N, L = (12, 10, 8), (6.0, 5.0, 4.0)
+    rng = np.random.default_rng(2)
+    psi = rng.standard_normal(int(np.prod(N)))
+    p = B.Problem(N, L, False)
+    p.set_system()
+    p.set_trap_harmonic((1.0, 0.7, 1.4))
+    p.set_wavefunction(psi)
+    p.set_contact(3.0)
+    p.set_mu(0.9)
+    a = p.arrays()
+    c = O.Ctx(N, L, False)
+    ltK, ltM = np.zeros(c.size), np.zeros(c.size)
+    c.trap_harmonic((1.0, 0.7, 1.4), ltK, ltM)
+    c.contact(psi, 3.0, 3.0, ltK, ltM)
+    pK, pM = O.setup_mu(0.9, ltK, ltM)
+    np.testing.assert_allclose(a["ltK"], ltK, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(a["ltM"], ltM, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(a["pK"], pK, rtol=1e-15)
+    np.testing.assert_allclose(a["pM"], pM, rtol=1e-15)
+    assert a["mu"] == 0.9
+
+
+def test_local_interactions_asymmetric_and_trap_additive():
+    """U_K != U_M: K += U_K, M += U_K + 2 U_M; two traps add
+    (reference tests/test_setup.c:415-443, 505-524)."""
+    p = B.Problem((8,), (4.0,), False)
+    p.set_system()
+    p.set_trap_harmonic((1.0,))
+    p.set_trap_harmonic((1.0,))
+    p.set_wavefunction(np.full(8, 0.5))
+    p.set_local_interactions(lambda n: 2.0 * n, lambda n: 5.0 * n)
+    p.set_mu(0.0)
+    a = p.arrays()
+    x = (np.arange(8) - 4) * 0.5
+    np.testing.assert_allclose(a["ltK"], x ** 2 + 0.5, atol=1e-14)
+    np.testing.assert_allclose(a["ltM"], x ** 2 + 0.5 + 2 * 1.25, atol=1e-14)
+
+
+MISUSE = {
+    "interactions_before_wf": "p.set_contact(1.0)",
+    "trap_before_system": "q = B.Problem((8,), (4.0,), False); q.set_trap_harmonic((1.0,))",
+    "interactions_twice": "p.set_wavefunction(np.ones(8)); p.set_contact(1.0); p.set_contact(1.0)",
+    "dipolar_after_mu": "p.set_wavefunction(np.ones(8)); p.set_mu(1.0); p.set_dipolar(1.0, (0, 0, 1), 1.0)",
+    "solve_without_mu": "p.solve()",
+    "wf_wrong_size": "p.set_wavefunction(np.ones(7))",
+}
+
+
+@pytest.mark.parametrize("case", sorted(MISUSE))
+def test_misuse_exits_with_failure(case):
+    """Out-of-order setup exits with EXIT_FAILURE (reference
+    tests/test_setup.c:531-592, which forks for the same check)."""
+    code = ("import sys, numpy as np; sys.path.insert(0, %r); import bdg_lobpcg_amd as B;"
+            "p = B.Problem((8,), (4.0,), False); p.set_system(); %s; print('no-exit')"
+            % (ROOT, MISUSE[case]))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1, (r.returncode, r.stdout, r.stderr)
+    assert "ERROR" in r.stderr and "no-exit" not in r.stdout
+
+
+def _ref_planewave_block(N, L, psi, m, hint="auto"):
+    """Restatement of reference src/solver.c:79-129, 196-258 in numpy."""
+    dim = len(N)
+    need = (m + 1) // 2
+    kx2 = [((2 * np.pi / l) * np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n)) ** 2
+           for n, l in zip(N, L)]
+    rng3 = [1, 1, 1]
+    for d in range(dim):
+        rng3[d] = min(N[d], 2 * need)
+    ent = []
+    for iz in range(rng3[2]):
+        for iy in range(rng3[1]):
+            for ix in range(rng3[0]):
+                k2 = kx2[0][ix] + (kx2[1][iy] if dim > 1 else 0) + (kx2[2][iz] if dim > 2 else 0)
+                ent.append((k2, (ix, iy, iz)))
+    ent.sort(key=lambda e: e[0])    # stable, like glibc's merge-sort qsort
+    nk = min(len(ent), need)
+    S = int(np.prod(N))
+    X = np.zeros((2 * S, m))
+    seed = [42]
+
+    def xr():
+        s = seed[0]
+        s ^= (s << 13) & 0xFFFFFFFF
+        s ^= s >> 17
+        s ^= (s << 5) & 0xFFFFFFFF
+        seed[0] = s
+        return (s & 0x7FFFFFFF) / 0x7FFFFFFF
+
+    idx = np.arange(S)
+    ids = [idx % N[0], (idx // N[0]) % (N[1] if dim > 1 else 1), idx // (N[0] * (N[1] if dim > 1 else 1))]
+    for j in range(m):
+        kv = (j + 1) // 2
+        if kv >= nk:
+            for i in range(S):
+                X[i, j] = X[i + S, j] = (xr() - 0.5) * abs(psi[i])
+            continue
+        ki = ent[kv][1]
+        kr = np.zeros(S)
+        for d in range(dim):
+            f = ki[d] if ki[d] <= N[d] // 2 else ki[d] - N[d]
+            kr += 2 * np.pi * f / L[d] * ids[d] * L[d] / N[d]
+        pw = np.sin(kr) if j % 2 == 1 else np.cos(kr)
+        for i in range(S):
+            X[i, j] = X[i + S, j] = (pw[i] + 1e-4 * (xr() - 0.5)) * abs(psi[i])
+    return X
+
+
+def test_planewave_start_block_matches_reference_recipe():
+    N, L = (6, 5, 4), (3.0, 4.0, 2.0)
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal(int(np.prod(N)))
+    p = B.Problem(N, L, False)
+    p.set_system()
+    p.set_wavefunction(psi)
+    p.set_solver_params(5, 7, 10, 1e-6)
+    S = psi.size
+    X = p.start_block((2 * S, 7))
+    np.testing.assert_allclose(X, _ref_planewave_block(N, L, psi, 7), rtol=0, atol=1e-13)
+
+
+def test_uniform_custom_block_matches_oracle():
+    N, L = (8, 6, 4), (4.0, 3.0, 2.0)
+    p = B.Problem(N, L, False)
+    p.set_system()
+    p.set_solver_params(3, 4, 10, 1e-6)
+    p.set_init_uniform(42)
+    S = int(np.prod(N))
+    X = p.start_block((2 * S, 4))
+    c = O.Ctx(N, L, False)
+    np.testing.assert_array_equal(X, c.start_block_uniform(42, 4))
+
+
+def test_io_formats_and_error_codes(tmp_path):
+    """reference tests/test_io.c:45-321: loader formats and return codes."""
+    lib = B.lib()
+    p = B.Problem((4,), (2.0,), True)
+    p.set_system()
+    f = tmp_path / "wf.dat"
+    f.write_text("1.0 0.5\n2.0 -1.0\n3.0 0.0\n4.0 2.5\n")
+    assert lib.bdg_load_wavefunction(p.h, str(f).encode()) == 0
+    assert lib.bdg_load_wavefunction(p.h, str(tmp_path / "missing.dat").encode()) == -1
+    g = tmp_path / "short.dat"
+    g.write_text("1.0 0.5\n2.0\n")
+    assert lib.bdg_load_wavefunction(p.h, str(g).encode()) == -2
+    assert lib.bdg_write_eigenvalues(p.h, str(tmp_path / "ev.dat").encode()) == -3
+    assert lib.bdg_write_mode_u(p.h, 0, str(tmp_path / "m").encode()) == -3
+    assert lib.bdg_reuse_modes(p.h, 0.1) == -3
