@@ -78,6 +78,9 @@ struct bdgx_ctx {
   // small device scratch for gram partials etc.
   void *scratch;
   size_t scratch_bytes;
+  // device copy of column indices for gathers
+  int *idx_buf;
+  uint64_t idx_cap;
 };
 
 namespace bdgx {

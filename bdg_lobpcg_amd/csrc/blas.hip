@@ -125,14 +125,19 @@ __global__ void __launch_bounds__(256) k_gram(GramArgs<R, CPLX> a) {
     }
 }
 
-__global__ void k_reduce_chunks(const double *part, int nchunk, long long len,
-                                double *out) {
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
-       e += (long long)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[(long long)c * len + e];
-    out[e] = s;
-  }
+// Sum of nchunk partial Grams: 64 outputs x 4 chunk lanes per workgroup,
+// lanes combined in LDS (deterministic order).
+__global__ void __launch_bounds__(256) k_reduce_chunks(const double *part, int nchunk,
+                                                       long long len, double *out) {
+  __shared__ double red[4][64];
+  const int el = threadIdx.x & 63, lane = threadIdx.x >> 6;
+  const long long e = blockIdx.x * 64LL + el;
+  double s = 0.0;
+  if (e < len)
+    for (int c = lane; c < nchunk; c += 4) s += part[(long long)c * len + e];
+  red[lane][el] = s;
+  __syncthreads();
+  if (lane == 0 && e < len) out[e] = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
 }
 
 template <typename R, bool CPLX>
@@ -142,7 +147,8 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   if (p == 0 || q == 0) return BDGX_OK;
   const int W = CPLX ? 2 : 1;
   const int tiles = (int)(((p + GT - 1) / GT) * ((q + GT - 1) / GT));
-  int nchunk = 2048 / tiles;
+  int nchunk = 1024 / tiles;
+  if (nchunk > 256) nchunk = 256;
   if (nchunk < 1) nchunk = 1;
   long long rpc = ((long long)nrows + nchunk - 1) / nchunk;
   rpc = ((rpc + GK - 1) / GK) * GK;
@@ -168,8 +174,7 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   hipLaunchKernelGGL((k_gram<R, CPLX>), dim3(tiles * nchunk), dim3(256), 0, c->stream, a);
   BDGX_HIP(hipGetLastError());
   const long long len = (long long)p * q * W;
-  int nb = (int)((len + 255) / 256);
-  if (nb > 1024) nb = 1024;
+  const int nb = (int)((len + 63) / 64);
   hipLaunchKernelGGL(k_reduce_chunks, dim3(nb), dim3(256), 0, c->stream,
                      (const double *)c->scratch, nchunk, len, dG);
   BDGX_HIP(hipGetLastError());
@@ -355,6 +360,59 @@ int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, co
 }
 
 // ---------------------------------------------------------------------------
+// column gather: dst[:, j] = src[:, idx[j]]
+template <typename T>
+__global__ void k_gather(const T *src, long long lds, const int *idx, int ncols, T *dst,
+                         long long ldd, long long nrows) {
+  const int j = blockIdx.y;
+  const long long base = (long long)idx[j] * lds;
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nrows;
+       r += (long long)gridDim.x * blockDim.x)
+    dst[r + j * ldd] = src[base + r];
+}
+
+int gather_dispatch(bdgx_ctx *c, int prec, const void *src, uint64_t lds, const int *hidx,
+                    uint64_t ncols, void *dst, uint64_t ldd, uint64_t nrows) {
+  if (ncols == 0) return BDGX_OK;
+  int rc = ensure_scratch(c, 0);
+  (void)rc;
+  // indices go through a small device buffer owned by the context
+  if (!c->idx_buf || c->idx_cap < ncols) {
+    if (c->idx_buf) {
+      BDGX_HIP(hipStreamSynchronize(c->stream));
+      BDGX_HIP(hipFree(c->idx_buf));
+    }
+    c->idx_cap = ncols < 1024 ? 1024 : ncols;
+    BDGX_HIP(hipMalloc((void **)&c->idx_buf, sizeof(int) * c->idx_cap));
+  }
+  BDGX_HIP(hipMemcpyAsync(c->idx_buf, hidx, sizeof(int) * ncols, hipMemcpyHostToDevice,
+                          c->stream));
+  const int W = c->cplx ? 2 : 1;
+  long long nbx = ((long long)nrows + 255) / 256;
+  if (nbx > 1024) nbx = 1024;
+  if (prec == BDGX_FP64) {
+    if (W == 2)
+      hipLaunchKernelGGL(k_gather<double2>, dim3((unsigned)nbx, (unsigned)ncols), dim3(256), 0,
+                         c->stream, (const double2 *)src, (long long)lds, c->idx_buf, (int)ncols,
+                         (double2 *)dst, (long long)ldd, (long long)nrows);
+    else
+      hipLaunchKernelGGL(k_gather<double>, dim3((unsigned)nbx, (unsigned)ncols), dim3(256), 0,
+                         c->stream, (const double *)src, (long long)lds, c->idx_buf, (int)ncols,
+                         (double *)dst, (long long)ldd, (long long)nrows);
+  } else {
+    if (W == 2)
+      hipLaunchKernelGGL(k_gather<float2>, dim3((unsigned)nbx, (unsigned)ncols), dim3(256), 0,
+                         c->stream, (const float2 *)src, (long long)lds, c->idx_buf, (int)ncols,
+                         (float2 *)dst, (long long)ldd, (long long)nrows);
+    else
+      hipLaunchKernelGGL(k_gather<float>, dim3((unsigned)nbx, (unsigned)ncols), dim3(256), 0,
+                         c->stream, (const float *)src, (long long)lds, c->idx_buf, (int)ncols,
+                         (float *)dst, (long long)ldd, (long long)nrows);
+  }
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
 template <typename S, typename D>
 __global__ void k_convert(const S *src, long long lds, D *dst, long long ldd,
                           long long nrows, int ncols) {

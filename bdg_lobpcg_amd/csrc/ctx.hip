@@ -25,6 +25,8 @@ int convert_dispatch(bdgx_ctx *c, int ps, const void *src, uint64_t lds, int pd,
                      uint64_t ldd, uint64_t nrows, uint64_t ncols);
 int bench_pass_dispatch(bdgx_ctx *c, int pass, int prec, uint64_t nbatch, int iters, float *ms,
                         double *bytes);
+int gather_dispatch(bdgx_ctx *c, int prec, const void *src, uint64_t lds, const int *hidx,
+                    uint64_t ncols, void *dst, uint64_t ldd, uint64_t nrows);
 template <typename R>
 int meanfield_t(bdgx_ctx *c, const R *dpsi, R *dOut, double scale);
 template <typename R>
@@ -305,6 +307,7 @@ void bdgx_ctx_destroy(bdgx_ctx *c) {
   free_fields(c->f32);
   if (c->spec) (void)hipFree(c->spec);
   if (c->scratch) (void)hipFree(c->scratch);
+  if (c->idx_buf) (void)hipFree(c->idx_buf);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -439,6 +442,12 @@ int bdgx_residual(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const 
   if (!c || !dAX || !dX || !dlam || !dn2) return BDGX_EINVAL;
   if (swap_x && (nrows & 1)) return BDGX_EINVAL;
   return residual_dispatch(c, prec, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2);
+}
+
+int bdgx_gather_cols(bdgx_ctx *c, int prec, const void *src, uint64_t lds, const int *idx,
+                     uint64_t ncols, void *dst, uint64_t ldd, uint64_t nrows) {
+  if (!c || !src || !dst || (!idx && ncols)) return BDGX_EINVAL;
+  return gather_dispatch(c, prec, src, lds, idx, ncols, dst, ldd, nrows);
 }
 
 int bdgx_convert(bdgx_ctx *c, int ps, const void *src, uint64_t lds, int pd, void *dst,

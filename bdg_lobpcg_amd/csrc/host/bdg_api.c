@@ -496,6 +496,7 @@ int bdg_solve(bdg_t *b) {
     return rc;
   }
   lobpcg_cfg cfg;
+  memset(&cfg, 0, sizeof(cfg));
   cfg.S = S;
   cfg.m = (int)m;
   cfg.nev = (int)nev;
@@ -503,42 +504,51 @@ int bdg_solve(bdg_t *b) {
   cfg.tol = b->tol;
   cfg.zero_tol = 1e-3;
   cfg.verbosity = b->verbosity;
+  cfg.gram_refresh = 25;
   lobpcg_result r;
   memset(&r, 0, sizeof(r));
   r.w2 = (double *)xcalloc(m, sizeof(double));
   r.zflag = (int *)xcalloc(m, sizeof(int));
   r.res = (double *)xcalloc(m, sizeof(double));
-  void *dX = NULL, *dMX = NULL;
-  int prec_final = BDGX_FP64;
+  void *dX = NULL, *dP = NULL, *dMX = NULL, *dP64 = NULL;
   if (b->precision == BDG_PREC_MIXED) {
-    /* FP32 bulk phase */
+    /* FP32 bulk phase: stops at tol_fp32, or when the residual stagnates at
+     * the FP32 floor; X and the last search direction P go on to FP64 */
     void *dV32 = NULL;
     rc = bdgx_malloc(&dV32, S * m * (es64 / 2));
     if (!rc) rc = bdgx_convert(b->dev, BDGX_FP64, dV0, S, BDGX_FP32, dV32, S, S, m);
     lobpcg_cfg c32 = cfg;
     c32.tol = fmax(b->tol, b->tol_fp32);
     c32.maxiter = (int)(b->maxIter_fp32 ? b->maxIter_fp32 : b->maxIter);
+    c32.gram_refresh = 10;
+    c32.image_refresh = 20;
+    c32.stall_window = 25;
+    c32.stall_factor = 0.7;
     const double t0 = bdg_wtime();
-    void *dX32 = NULL, *dMX32 = NULL;
-    if (!rc) rc = bdg_lobpcg_device(b->dev, BDGX_FP32, &c32, dV32, &r, &dX32, &dMX32);
+    void *dX32 = NULL, *dP32 = NULL;
+    if (!rc) rc = bdg_lobpcg_device(b->dev, BDGX_FP32, &c32, dV32, NULL, &r, &dX32, &dP32);
     b->stats.seconds_fp32 = bdg_wtime() - t0;
     b->stats.iterations_fp32 = (uint64_t)r.iters;
     b->stats.column_applies += r.applies;
     b->stats.restarts += (uint64_t)r.restarts;
     b->stats.seconds_rr += r.rr_seconds;
     if (!rc) rc = bdgx_convert(b->dev, BDGX_FP32, dX32, S, BDGX_FP64, dV0, S, S, m);
+    if (!rc && dP32) {
+      rc = bdgx_malloc(&dP64, S * m * es64);
+      if (!rc) rc = bdgx_convert(b->dev, BDGX_FP32, dP32, S, BDGX_FP64, dP64, S, S, m);
+    }
     if (!rc) rc = bdgx_sync(b->dev);
     bdgx_free(dV32);
     bdgx_free(dX32);
-    bdgx_free(dMX32);
+    bdgx_free(dP32);
     if (b->verbosity > 0)
-      fprintf(stderr, "[bdg] fp32 phase: %d iterations, %.3f s, status %d\n", r.iters,
-              b->stats.seconds_fp32, rc);
+      fprintf(stderr, "[bdg] fp32 phase: %d iterations, %.3f s, stalled %d, status %d\n",
+              r.iters, b->stats.seconds_fp32, r.stalled, rc);
   }
   if (!rc) {
     cfg.maxiter = (int)b->maxIter;
     const double t0 = bdg_wtime();
-    rc = bdg_lobpcg_device(b->dev, BDGX_FP64, &cfg, dV0, &r, &dX, &dMX);
+    rc = bdg_lobpcg_device(b->dev, BDGX_FP64, &cfg, dV0, dP64, &r, &dX, &dP);
     b->stats.seconds_fp64 = bdg_wtime() - t0;
     b->stats.iterations_fp64 = (uint64_t)r.iters;
     b->stats.column_applies += r.applies;
@@ -548,8 +558,14 @@ int bdg_solve(bdg_t *b) {
       fprintf(stderr, "[bdg] fp64 phase: %d iterations, %.3f s, status %d\n", r.iters,
               b->stats.seconds_fp64, rc);
   }
-  (void)prec_final;
   bdgx_free(dV0);
+  bdgx_free(dP64);
+  bdgx_free(dP);
+  /* M X for the u half */
+  if (!rc) {
+    rc = bdgx_malloc(&dMX, S * m * es64);
+    if (!rc) rc = bdgx_apply(b->dev, BDGX_OP_M, BDGX_FP64, dX, dMX, m, S);
+  }
   b->stats.iterations = b->stats.iterations_fp32 + b->stats.iterations_fp64;
   if (rc) {
     bdgx_free(dX);

@@ -105,23 +105,28 @@ typedef struct {
   double tol;
   double zero_tol;
   int verbosity;
+  int gram_refresh;  /* full Gram recompute every n iterations (0: never)       */
+  int image_refresh; /* re-apply M, K to X (and drop P) every n iterations      */
+  int stall_window;  /* stop when the worst residual has not dropped below      */
+  double stall_factor; /*   stall_factor x its best for stall_window iterations */
 } lobpcg_cfg;
 
 typedef struct {
   double *w2;      /* m */
   int *zflag;      /* m */
   double *res;     /* m */
-  int iters, restarts, precond_fallbacks;
+  int iters, restarts, precond_fallbacks, stalled;
   uint64_t applies;
   double rr_seconds;
   int status;
 } lobpcg_result;
 
-/* Runs from the device start block dX0 (S x m, precision prec, ld S).  On
- * success *dX and *dMX (device, precision prec, S x m) hold the final block
- * and its M image; the caller frees them with bdgx_free. */
+/* Runs from the device start block dX0 (S x m, precision prec, ld S), with an
+ * optional previous search direction block dP0 (same shape, or NULL).  On
+ * success *dX, *dP (device, precision prec, S x m) hold the final block and
+ * its last search direction; the caller frees them with bdgx_free. */
 int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void *dX0,
-                      lobpcg_result *res, void **dX, void **dMX);
+                      const void *dP0, lobpcg_result *res, void **dX, void **dP);
 
 double bdg_wtime(void);
 

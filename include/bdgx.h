@@ -107,6 +107,12 @@ int bdgx_residual(bdgx_ctx *ctx, int prec, const void *dAX, const void *dX,
                   const void *dMX, uint64_t ld, uint64_t ncols, uint64_t nrows,
                   const double *dlambda, int swap_x, void *dR, uint64_t ldr,
                   double *dnorms2);
+/* dDst[:, j] = dSrc[:, idx[j]] for j < ncols (same precision, one launch);
+ * idx is a host array.  dDst must not overlap the gathered source columns. */
+int bdgx_gather_cols(bdgx_ctx *ctx, int prec, const void *dSrc, uint64_t lds,
+                     const int *idx, uint64_t ncols, void *dDst, uint64_t ldd,
+                     uint64_t nrows);
+
 /* Element-type conversion FP64 <-> FP32 of an nrows x ncols block. */
 int bdgx_convert(bdgx_ctx *ctx, int prec_src, const void *dsrc, uint64_t lds,
                  int prec_dst, void *ddst, uint64_t ldd, uint64_t nrows,
