@@ -37,10 +37,9 @@ class BdgError(RuntimeError):
 
 
 class SolveStats(C.Structure):
-    _fields_ = [("iterations", C.c_uint64), ("iterations_fp32", C.c_uint64),
-                ("iterations_fp64", C.c_uint64), ("column_applies", C.c_uint64),
+    _fields_ = [("iterations", C.c_uint64), ("column_applies", C.c_uint64),
+                ("column_applies_fp32", C.c_uint64), ("image_refreshes", C.c_uint64),
                 ("restarts", C.c_uint64), ("seconds_total", C.c_double),
-                ("seconds_fp32", C.c_double), ("seconds_fp64", C.c_double),
                 ("seconds_rr", C.c_double), ("status", C.c_int)]
 
 
@@ -378,8 +377,8 @@ class Problem:
     def set_precision(self, mode):
         lib().bdg_set_precision(self.h, int(mode))
 
-    def set_mixed_params(self, tol32, maxit32=0):
-        lib().bdg_set_mixed_params(self.h, float(tol32), int(maxit32))
+    def set_mixed_params(self, img_tol=0.0, refresh_every=20):
+        lib().bdg_set_mixed_params(self.h, float(img_tol), int(refresh_every))
 
     def set_verbosity(self, level):
         lib().bdg_set_verbosity(self.h, int(level))

@@ -11,133 +11,230 @@
 namespace bdgx {
 
 // ---------------------------------------------------------------------------
-// Gram: partial[chunk][i + j*p] = sum_{r in chunk} conj(A[r,i]) B[r',j]
-constexpr int GT = 64;   // output tile edge
-constexpr int GK = 32;   // rows per LDS stage
+// Gram on FP64 matrix cores: partial[chunk][i + j*p] = sum_{r in chunk}
+// conj(A[r,i]) B[r',j] with v_mfma_f64_16x16x4f64.
+//
+// The contraction runs over rows, so any bijection of rows onto the MFMA's
+// k slots works as long as A and B use the same one: lane l = c + 16*ks
+// loads four consecutive rows (4*ks .. 4*ks+3 of a 16-row step) of column
+// c of each 16-column fragment, one vector load per fragment, and feeds them
+// to four MFMAs.  Each wave streams its own row range and keeps the whole
+// (16*FP) x (16*FQ) output tile in accumulators, so every element of A and B
+// is read once per tile; the four waves of a workgroup are summed in LDS and
+// one partial per workgroup is written.  Complex data is processed as real
+// columns of 2*nrows interleaved (re, im) rows: Re G is the real Gram of
+// those rows and Im G the real Gram against B with each (re, im) pair turned
+// into (im, -re).  FP32 storage is widened on load (FP64 products are exact).
+typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <typename T>
-struct Elem;
-template <>
-struct Elem<double> {
-  static constexpr bool cplx = false;
-};
-template <>
-struct Elem<float> {
-  static constexpr bool cplx = false;
-};
-
-template <typename R, bool CPLX>
-struct GramArgs {
+template <typename R>
+struct GramMArgs {
   const R *A, *B;
-  long long lda, ldb, nrows, half;
-  int p, q, swap_b, nchunk;
-  long long rows_per_chunk;
-  double *part;  // nchunk * p * q (complex: 2x)
+  long long lda, ldb;      // in R units of the real view
+  long long nrows;         // rows of the real view
+  long long half;          // element rows / 2 (swap_b)
+  long long rows_per_wg;   // multiple of 64
+  int p, q, tiles_p, swap_b, vec, wel;  // wel: R units per element (1 or 2)
+  double *part;            // nchunk * p * q * W
 };
 
-template <typename R, bool CPLX>
-__global__ void __launch_bounds__(256) k_gram(GramArgs<R, CPLX> a) {
-  constexpr int W = CPLX ? 2 : 1;
-  __shared__ double As[GK][GT * W + 1];
-  __shared__ double Bs[GK][GT * W + 1];
-  const int tilesP = (a.p + GT - 1) / GT;
-  const int tilesQ = (a.q + GT - 1) / GT;
-  const int tiles = tilesP * tilesQ;
-  const int chunk = blockIdx.x / tiles;
-  const int tt = blockIdx.x - chunk * tiles;
-  const int tp = tt % tilesP, tq = tt / tilesP;
-  const int i0 = tp * GT, j0 = tq * GT;
-  const long long rs = (long long)chunk * a.rows_per_chunk;
-  long long re = rs + a.rows_per_chunk;
-  if (re > a.nrows) re = a.nrows;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4][W];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-      for (int w = 0; w < W; ++w) acc[u][v][w] = 0.0;
-  for (long long r0 = rs; r0 < re; r0 += GK) {
-    // stage GK rows x GT columns of A and B
-    for (int e = threadIdx.x; e < GK * GT; e += blockDim.x) {
-      const int rr = e % GK, cc = e / GK;
-      const long long r = r0 + rr;
-      const bool okr = r < re;
-      const int ia = i0 + cc, jb = j0 + cc;
-      long long rb = r;
-      if (a.swap_b) rb = (r < a.half) ? r + a.half : r - a.half;
-      if (CPLX) {
-        double ar = 0, ai = 0, br = 0, bi = 0;
-        if (okr && ia < a.p) {
-          ar = (double)a.A[2 * (r + ia * a.lda)];
-          ai = (double)a.A[2 * (r + ia * a.lda) + 1];
-        }
-        if (okr && jb < a.q) {
-          br = (double)a.B[2 * (rb + jb * a.ldb)];
-          bi = (double)a.B[2 * (rb + jb * a.ldb) + 1];
-        }
-        As[rr][2 * cc] = ar;
-        As[rr][2 * cc + 1] = ai;
-        Bs[rr][2 * cc] = br;
-        Bs[rr][2 * cc + 1] = bi;
-      } else {
-        As[rr][cc] = (okr && ia < a.p) ? (double)a.A[r + ia * a.lda] : 0.0;
-        Bs[rr][cc] = (okr && jb < a.q) ? (double)a.B[rb + jb * a.ldb] : 0.0;
-      }
+template <typename R>
+__device__ __forceinline__ void load4(const R *base, long long ld, long long row, int col,
+                                      int ncol, long long rlim, const GramMArgs<R> &a,
+                                      bool swap, double out[4]) {
+  if (col >= ncol) {
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    return;
+  }
+  const R *cp = base + (long long)col * ld;
+  if (!swap && a.vec && row + 4 <= rlim) {
+    if constexpr (sizeof(R) == 8) {
+      const double2 *v = reinterpret_cast<const double2 *>(cp + row);
+      const double2 x = v[0], y = v[1];
+      out[0] = x.x;
+      out[1] = x.y;
+      out[2] = y.x;
+      out[3] = y.y;
+    } else {
+      const float4 x = *reinterpret_cast<const float4 *>(cp + row);
+      out[0] = x.x;
+      out[1] = x.y;
+      out[2] = x.z;
+      out[3] = x.w;
     }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < GK; ++kk) {
-      double av[4][W], bv[4][W];
+    return;
+  }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+  for (int t = 0; t < 4; ++t) {
+    long long r = row + t;
+    if (r >= rlim) {
+      out[t] = 0.0;
+      continue;
+    }
+    if (swap) {
+      const long long e = r / a.wel, part = r - e * a.wel;
+      const long long es = e < a.half ? e + a.half : e - a.half;
+      r = es * a.wel + part;
+    }
+    out[t] = (double)cp[r];
+  }
+}
+
+template <typename R, int FP, int FQ, bool CPLX>
+__global__ void __launch_bounds__(256) k_gram_mfma(GramMArgs<R> a) {
+  constexpr int NACC = FP * FQ * (CPLX ? 2 : 1);
+  extern __shared__ double red[];  // 2 * NACC * 256 doubles
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, ks = lane >> 4;
+  const int tp = blockIdx.y % a.tiles_p, tq = blockIdx.y / a.tiles_p;
+  const int p0 = tp * 16 * FP, q0 = tq * 16 * FQ;
+  const long long wg_r0 = (long long)blockIdx.x * a.rows_per_wg;
+  const long long rpw = a.rows_per_wg >> 2;
+  const long long r_lo = wg_r0 + wave * rpw;
+  long long r_hi = r_lo + rpw;
+  if (r_hi > a.nrows) r_hi = a.nrows;
+  v4d acc[NACC];
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-          av[u][w] = As[kk][(ty + 16 * u) * W + w];
-          bv[u][w] = Bs[kk][(tx + 16 * u) * W + w];
-        }
+  for (int u = 0; u < NACC; ++u) acc[u] = v4d{0.0, 0.0, 0.0, 0.0};
+  const bool swap = a.swap_b != 0;
+  double av[FP][4], bv[FQ][4];
+  if (r_lo < r_hi) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+    for (int f = 0; f < FP; ++f) load4(a.A, a.lda, r_lo + 4 * ks, p0 + 16 * f + c, a.p, r_hi, a, false, av[f]);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          if (CPLX) {
-            // conj(a) * b
-            acc[u][v][0] += av[u][0] * bv[v][0] + av[u][W - 1] * bv[v][W - 1];
-            acc[u][v][W - 1] += av[u][0] * bv[v][W - 1] - av[u][W - 1] * bv[v][0];
-          } else {
-            acc[u][v][0] += av[u][0] * bv[v][0];
+    for (int g = 0; g < FQ; ++g) load4(a.B, a.ldb, r_lo + 4 * ks, q0 + 16 * g + c, a.q, r_hi, a, swap, bv[g]);
+  }
+  for (long long r = r_lo; r < r_hi; r += 16) {
+    // next step's operands are requested before this step's MFMAs
+    double an[FP][4], bn[FQ][4];
+    const long long rn = r + 16 + 4 * ks;
+    const bool more = r + 16 < r_hi;
+    if (more) {
+#pragma unroll
+      for (int f = 0; f < FP; ++f) load4(a.A, a.lda, rn, p0 + 16 * f + c, a.p, r_hi, a, false, an[f]);
+#pragma unroll
+      for (int g = 0; g < FQ; ++g) load4(a.B, a.ldb, rn, q0 + 16 * g + c, a.q, r_hi, a, swap, bn[g]);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int f = 0; f < FP; ++f)
+#pragma unroll
+        for (int g = 0; g < FQ; ++g) {
+          acc[f * FQ + g] =
+              __builtin_amdgcn_mfma_f64_16x16x4f64(av[f][t], bv[g][t], acc[f * FQ + g], 0, 0, 0);
+          if constexpr (CPLX) {
+            // (re, im) pairs of B turned into (im, -re): gives Im conj(a) b
+            const double bt = (t & 1) ? -bv[g][t - 1] : bv[g][t + 1];
+            acc[FP * FQ + f * FQ + g] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                av[f][t], bt, acc[FP * FQ + f * FQ + g], 0, 0, 0);
           }
         }
     }
-    __syncthreads();
+    if (more) {
+#pragma unroll
+      for (int f = 0; f < FP; ++f)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) av[f][t] = an[f][t];
+#pragma unroll
+      for (int g = 0; g < FQ; ++g)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) bv[g][t] = bn[g][t];
+    }
   }
-  double *out = a.part + (long long)chunk * a.p * a.q * W;
+  // sum the four waves' tiles in LDS: waves 2, 3 -> 0, 1, then wave 1 -> 0
+  if (wave >= 2) {
+    double *dst = red + (size_t)(wave - 2) * NACC * 256;
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < NACC; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = i0 + ty + 16 * u, j = j0 + tx + 16 * v;
-      if (i < a.p && j < a.q) {
+      for (int v = 0; v < 4; ++v) dst[(u * 4 + v) * 64 + lane] = acc[u][v];
+  }
+  __syncthreads();
+  if (wave < 2) {
+    const double *src = red + (size_t)wave * NACC * 256;
 #pragma unroll
-        for (int w = 0; w < W; ++w) out[(i + (long long)j * a.p) * W + w] = acc[u][v][w];
+    for (int u = 0; u < NACC; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] += src[(u * 4 + v) * 64 + lane];
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int u = 0; u < NACC; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[(u * 4 + v) * 64 + lane] = acc[u][v];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int u = 0; u < NACC; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] += red[(u * 4 + v) * 64 + lane];
+  constexpr int W = CPLX ? 2 : 1;
+  double *out = a.part + (long long)blockIdx.x * a.p * a.q * W;
+  // accumulator element v of lane (c, ks) holds D[ks + 4*v][c]
+#pragma unroll
+  for (int f = 0; f < FP; ++f)
+#pragma unroll
+    for (int g = 0; g < FQ; ++g) {
+      const int j = q0 + 16 * g + c;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = p0 + 16 * f + ks + 4 * v;
+        if (i < a.p && j < a.q) {
+          const long long o = ((long long)i + (long long)j * a.p) * W;
+          out[o] = acc[f * FQ + g][v];
+          if constexpr (CPLX) out[o + 1] = acc[FP * FQ + f * FQ + g][v];
+        }
       }
     }
 }
 
-// Sum of nchunk partial Grams: 64 outputs x 4 chunk lanes per workgroup,
-// lanes combined in LDS (deterministic order).
-__global__ void __launch_bounds__(256) k_reduce_chunks(const double *part, int nchunk,
+// Sum of n partial vectors of length len: workgroup (x, y) adds chunks
+// [y*cpg, (y+1)*cpg) for 64 outputs with 4 lanes each, combined in LDS in a
+// fixed order (deterministic).
+__global__ void __launch_bounds__(256) k_reduce_chunks(const double *part, int n, int cpg,
                                                        long long len, double *out) {
   __shared__ double red[4][64];
-  const int el = threadIdx.x & 63, lane = threadIdx.x >> 6;
+  const int el = threadIdx.x & 63, ln = threadIdx.x >> 6;
   const long long e = blockIdx.x * 64LL + el;
+  const int c0 = blockIdx.y * cpg;
+  int c1 = c0 + cpg;
+  if (c1 > n) c1 = n;
   double s = 0.0;
   if (e < len)
-    for (int c = lane; c < nchunk; c += 4) s += part[(long long)c * len + e];
-  red[lane][el] = s;
+    for (int c = c0 + ln; c < c1; c += 4) s += part[(long long)c * len + e];
+  red[ln][el] = s;
   __syncthreads();
-  if (lane == 0 && e < len) out[e] = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
+  if (ln == 0 && e < len)
+    out[(long long)blockIdx.y * len + e] = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
+}
+
+static int reduce_partials(bdgx_ctx *c, const double *part, int n, long long len, double *tmp,
+                           double *out) {
+  const int nb = (int)((len + 63) / 64);
+  if (n <= 64) {
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(nb, 1), dim3(256), 0, c->stream, part, n, n, len,
+                       out);
+    BDGX_HIP(hipGetLastError());
+    return BDGX_OK;
+  }
+  const int g = (n + 63) / 64;
+  hipLaunchKernelGGL(k_reduce_chunks, dim3(nb, g), dim3(256), 0, c->stream, part, n, 64, len,
+                     tmp);
+  BDGX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_chunks, dim3(nb, 1), dim3(256), 0, c->stream, (const double *)tmp,
+                     g, g, len, out);
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
+template <typename R, int FP, int FQ, bool CPLX>
+static void launch_gram(const GramMArgs<R> &a, int nchunk, int tiles, hipStream_t s) {
+  constexpr int NACC = FP * FQ * (CPLX ? 2 : 1);
+  const size_t lds = sizeof(double) * 2 * NACC * 256;
+  hipLaunchKernelGGL((k_gram_mfma<R, FP, FQ, CPLX>), dim3(nchunk, tiles), dim3(256), lds, s, a);
 }
 
 template <typename R, bool CPLX>
@@ -146,39 +243,66 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
                   int swap_b, double *dG) {
   if (p == 0 || q == 0) return BDGX_OK;
   const int W = CPLX ? 2 : 1;
-  const int tiles = (int)(((p + GT - 1) / GT) * ((q + GT - 1) / GT));
-  int nchunk = 1024 / tiles;
-  if (nchunk > 256) nchunk = 256;
+  // tile shape: real up to 96 x 32, complex up to 48 x 32 (12 accumulators)
+  int FQ = q <= 16 ? 1 : 2;
+  int FP;
+  if (CPLX)
+    FP = p <= 16 ? 1 : 3;
+  else
+    FP = p <= 32 ? 2 : (p <= 64 ? 4 : 6);
+  const int tiles_p = (int)((p + 16 * FP - 1) / (16 * FP));
+  const int tiles_q = (int)((q + 16 * FQ - 1) / (16 * FQ));
+  const int tiles = tiles_p * tiles_q;
+  const long long rows = (long long)nrows * W;
+  // about 1024 workgroups in total (2 per CU and 4 waves each)
+  long long nchunk = (1024 + tiles - 1) / tiles;
+  long long rpw = (rows + nchunk - 1) / nchunk;
+  rpw = ((rpw + 63) / 64) * 64;
+  if (rpw < 64) rpw = 64;
+  nchunk = (rows + rpw - 1) / rpw;
   if (nchunk < 1) nchunk = 1;
-  long long rpc = ((long long)nrows + nchunk - 1) / nchunk;
-  rpc = ((rpc + GK - 1) / GK) * GK;
-  if (rpc < GK) rpc = GK;
-  nchunk = (int)(((long long)nrows + rpc - 1) / rpc);
-  if (nchunk < 1) nchunk = 1;
-  const size_t need = sizeof(double) * (size_t)nchunk * p * q * W;
+  const long long len = (long long)p * q * W;
+  const int g2 = nchunk > 64 ? (int)((nchunk + 63) / 64) : 0;
+  const size_t need = sizeof(double) * (size_t)(nchunk + g2) * (size_t)len;
   int rc = ensure_scratch(c, need);
   if (rc) return rc;
-  GramArgs<R, CPLX> a;
+  GramMArgs<R> a;
   a.A = reinterpret_cast<const R *>(dA);
   a.B = reinterpret_cast<const R *>(dB);
-  a.lda = (long long)lda;
-  a.ldb = (long long)ldb;
-  a.nrows = (long long)nrows;
+  a.lda = (long long)lda * W;
+  a.ldb = (long long)ldb * W;
+  a.nrows = rows;
   a.half = (long long)nrows / 2;
+  a.rows_per_wg = rpw;
   a.p = (int)p;
   a.q = (int)q;
+  a.tiles_p = tiles_p;
   a.swap_b = swap_b;
-  a.nchunk = nchunk;
-  a.rows_per_chunk = rpc;
+  a.wel = W;
+  const uintptr_t al = (uintptr_t)dA | (uintptr_t)dB;
+  a.vec = (a.lda % 4 == 0) && (a.ldb % 4 == 0) && (al % (4 * sizeof(R)) == 0);
   a.part = reinterpret_cast<double *>(c->scratch);
-  hipLaunchKernelGGL((k_gram<R, CPLX>), dim3(tiles * nchunk), dim3(256), 0, c->stream, a);
+  const int nc = (int)nchunk;
+  if (CPLX) {
+    if (FP == 1 && FQ == 1) launch_gram<R, 1, 1, true>(a, nc, tiles, c->stream);
+    else if (FP == 1) launch_gram<R, 1, 2, true>(a, nc, tiles, c->stream);
+    else if (FQ == 1) launch_gram<R, 3, 1, true>(a, nc, tiles, c->stream);
+    else launch_gram<R, 3, 2, true>(a, nc, tiles, c->stream);
+  } else {
+    if (FP == 2) {
+      if (FQ == 1) launch_gram<R, 2, 1, false>(a, nc, tiles, c->stream);
+      else launch_gram<R, 2, 2, false>(a, nc, tiles, c->stream);
+    } else if (FP == 4) {
+      if (FQ == 1) launch_gram<R, 4, 1, false>(a, nc, tiles, c->stream);
+      else launch_gram<R, 4, 2, false>(a, nc, tiles, c->stream);
+    } else {
+      if (FQ == 1) launch_gram<R, 6, 1, false>(a, nc, tiles, c->stream);
+      else launch_gram<R, 6, 2, false>(a, nc, tiles, c->stream);
+    }
+  }
   BDGX_HIP(hipGetLastError());
-  const long long len = (long long)p * q * W;
-  const int nb = (int)((len + 63) / 64);
-  hipLaunchKernelGGL(k_reduce_chunks, dim3(nb), dim3(256), 0, c->stream,
-                     (const double *)c->scratch, nchunk, len, dG);
-  BDGX_HIP(hipGetLastError());
-  return BDGX_OK;
+  double *part = reinterpret_cast<double *>(c->scratch);
+  return reduce_partials(c, part, nc, len, part + (size_t)nc * len, dG);
 }
 
 int gram_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
@@ -256,11 +380,146 @@ __global__ void __launch_bounds__(256) k_combine(const R *A, long long lda, int 
   }
 }
 
+// Real combine on FP64 matrix cores: D (16 x 16 block of Y) += A_op (16 x 4
+// block of A) * B_op (4 x 16 block of C).  The 16 rows of MFMA tile t
+// (t = 0..3) of a wave are the rows r0 + 4*i + t, i = 0..15, so lane
+// l = c + 16*kk loads the four consecutive rows r0 + 4c .. r0 + 4c + 3 of
+// column k0 + kk with one vector load and feeds element t to tile t; the
+// accumulator element v of tile t in lane l is Y[r0 + 4*(kk + 4v) + t,
+// j0 + c], so the four tiles give each lane four consecutive rows to store
+// with one vector store.  Each wave owns 64 rows and all 16*MF output columns
+// of its column tile (A is read once per tile); C is staged through LDS 64
+// rows at a time; the next k step's A values are loaded before the MFMAs of
+// the current one.
+template <typename R>
+__device__ __forceinline__ void load_rows4(const R *A, long long lda, int col, int k,
+                                           long long row, long long nrows, int vec,
+                                           double out[4]) {
+  if (col >= k) {
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    return;
+  }
+  const R *p = A + row + (long long)col * lda;
+  if (vec && row + 4 <= nrows) {
+    if constexpr (sizeof(R) == 8) {
+      const double2 x = reinterpret_cast<const double2 *>(p)[0];
+      const double2 y = reinterpret_cast<const double2 *>(p)[1];
+      out[0] = x.x;
+      out[1] = x.y;
+      out[2] = y.x;
+      out[3] = y.y;
+    } else {
+      const float4 x = *reinterpret_cast<const float4 *>(p);
+      out[0] = x.x;
+      out[1] = x.y;
+      out[2] = x.z;
+      out[3] = x.w;
+    }
+    return;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) out[t] = (row + t < nrows) ? (double)p[t] : 0.0;
+}
+
+template <typename R, int MF>
+__global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda, int k,
+                                                      const double *C, R *Y, long long ldy,
+                                                      int m, long long nrows, int vec_a,
+                                                      int vec_y) {
+  constexpr int KC = 64;
+  __shared__ double Cs[KC][16 * MF + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, kk = lane >> 4;
+  const int j0 = blockIdx.y * 16 * MF;
+  const long long r0 = ((long long)blockIdx.x * 4 + wave) * 64;
+  const long long row = r0 + 4 * c;
+  v4d acc[4][MF];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int f = 0; f < MF; ++f) acc[t][f] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int kc = 0; kc < k; kc += KC) {
+    const int nk = (k - kc < KC) ? k - kc : KC;
+    __syncthreads();
+    for (int e = threadIdx.x; e < KC * 16 * MF; e += 256) {
+      const int i = e % KC, j = e / KC;
+      Cs[i][j] = (i < nk && j0 + j < m) ? C[(kc + i) + (long long)(j0 + j) * k] : 0.0;
+    }
+    __syncthreads();
+    double av[4], an[4];
+    load_rows4(A, lda, kc + kk, k, row, nrows, vec_a, av);
+    for (int k4 = 0; k4 < nk; k4 += 4) {
+      if (k4 + 4 < nk) load_rows4(A, lda, kc + k4 + 4 + kk, k, row, nrows, vec_a, an);
+      double bv[MF];
+#pragma unroll
+      for (int f = 0; f < MF; ++f) bv[f] = Cs[k4 + kk][16 * f + c];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int f = 0; f < MF; ++f)
+          acc[t][f] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bv[f], acc[t][f], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) av[t] = an[t];
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < MF; ++f) {
+    const int col = j0 + 16 * f + c;
+    if (col >= m) continue;
+    R *yc = Y + (long long)col * ldy;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const long long rr = r0 + 4 * (kk + 4 * v);
+      if (vec_y && rr + 4 <= nrows) {
+        if constexpr (sizeof(R) == 8) {
+          reinterpret_cast<double2 *>(yc + rr)[0] = double2{acc[0][f][v], acc[1][f][v]};
+          reinterpret_cast<double2 *>(yc + rr)[1] = double2{acc[2][f][v], acc[3][f][v]};
+        } else {
+          *reinterpret_cast<float4 *>(yc + rr) = float4{(float)acc[0][f][v], (float)acc[1][f][v],
+                                                        (float)acc[2][f][v], (float)acc[3][f][v]};
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (rr + t < nrows) yc[rr + t] = (R)acc[t][f][v];
+      }
+    }
+  }
+}
+
+template <typename R>
+static int combine_real_mfma(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k,
+                             const double *dC, void *dY, uint64_t ldy, uint64_t m,
+                             uint64_t nrows) {
+  const int MF = m <= 16 ? 1 : (m <= 32 ? 2 : 4);
+  const unsigned gx = (unsigned)((nrows + 255) / 256);
+  const unsigned gy = (unsigned)((m + 16 * MF - 1) / (16 * MF));
+  const int vec_a = (lda % 4 == 0) && ((uintptr_t)dA % (4 * sizeof(R)) == 0);
+  const int vec_y = (ldy % 4 == 0) && ((uintptr_t)dY % (4 * sizeof(R)) == 0);
+  const R *A = reinterpret_cast<const R *>(dA);
+  R *Y = reinterpret_cast<R *>(dY);
+  if (MF == 1)
+    hipLaunchKernelGGL((k_combine_mfma<R, 1>), dim3(gx, gy), dim3(256), 0, c->stream, A,
+                       (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
+                       vec_a, vec_y);
+  else if (MF == 2)
+    hipLaunchKernelGGL((k_combine_mfma<R, 2>), dim3(gx, gy), dim3(256), 0, c->stream, A,
+                       (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
+                       vec_a, vec_y);
+  else
+    hipLaunchKernelGGL((k_combine_mfma<R, 4>), dim3(gx, gy), dim3(256), 0, c->stream, A,
+                       (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
+                       vec_a, vec_y);
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
 template <typename R, bool CPLX>
 static int combine_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k,
                      const double *dC, void *dY, uint64_t ldy, uint64_t m,
                      uint64_t nrows) {
   if (m == 0 || nrows == 0) return BDGX_OK;
+  if (!CPLX) return combine_real_mfma<R>(c, dA, lda, k, dC, dY, ldy, m, nrows);
   const int W = CPLX ? 2 : 1;
   const size_t sm = sizeof(double) * (size_t)k * CJ * W;
   if (sm > 64 * 1024) return BDGX_EINVAL;

@@ -46,8 +46,8 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
   b->maxIter = 500;
   b->tol = 1e-6;
   b->precision = BDG_PREC_FP64;
-  b->tol_fp32 = 1e-5;
-  b->maxIter_fp32 = 0;
+  b->img_tol = 0.0;
+  b->refresh_every = 20;
   return b;
 }
 
@@ -505,68 +505,37 @@ int bdg_solve(bdg_t *b) {
   cfg.zero_tol = 1e-3;
   cfg.verbosity = b->verbosity;
   cfg.gram_refresh = 25;
+  cfg.maxiter = (int)b->maxIter;
+  cfg.w_prec = b->precision == BDG_PREC_MIXED ? BDGX_FP32 : BDGX_FP64;
+  cfg.img_tol = b->img_tol;
+  cfg.refresh_every = (int)b->refresh_every;
+  cfg.mixed_min_cols = (int)(m / 4 > 1 ? m / 4 : 1);
   lobpcg_result r;
   memset(&r, 0, sizeof(r));
   r.w2 = (double *)xcalloc(m, sizeof(double));
   r.zflag = (int *)xcalloc(m, sizeof(int));
   r.res = (double *)xcalloc(m, sizeof(double));
-  void *dX = NULL, *dP = NULL, *dMX = NULL, *dP64 = NULL;
-  if (b->precision == BDG_PREC_MIXED) {
-    /* FP32 bulk phase: stops at tol_fp32, or when the residual stagnates at
-     * the FP32 floor; X and the last search direction P go on to FP64 */
-    void *dV32 = NULL;
-    rc = bdgx_malloc(&dV32, S * m * (es64 / 2));
-    if (!rc) rc = bdgx_convert(b->dev, BDGX_FP64, dV0, S, BDGX_FP32, dV32, S, S, m);
-    lobpcg_cfg c32 = cfg;
-    c32.tol = fmax(b->tol, b->tol_fp32);
-    c32.maxiter = (int)(b->maxIter_fp32 ? b->maxIter_fp32 : b->maxIter);
-    c32.gram_refresh = 10;
-    c32.image_refresh = 20;
-    c32.stall_window = 25;
-    c32.stall_factor = 0.7;
-    const double t0 = bdg_wtime();
-    void *dX32 = NULL, *dP32 = NULL;
-    if (!rc) rc = bdg_lobpcg_device(b->dev, BDGX_FP32, &c32, dV32, NULL, &r, &dX32, &dP32);
-    b->stats.seconds_fp32 = bdg_wtime() - t0;
-    b->stats.iterations_fp32 = (uint64_t)r.iters;
-    b->stats.column_applies += r.applies;
-    b->stats.restarts += (uint64_t)r.restarts;
-    b->stats.seconds_rr += r.rr_seconds;
-    if (!rc) rc = bdgx_convert(b->dev, BDGX_FP32, dX32, S, BDGX_FP64, dV0, S, S, m);
-    if (!rc && dP32) {
-      rc = bdgx_malloc(&dP64, S * m * es64);
-      if (!rc) rc = bdgx_convert(b->dev, BDGX_FP32, dP32, S, BDGX_FP64, dP64, S, S, m);
-    }
-    if (!rc) rc = bdgx_sync(b->dev);
-    bdgx_free(dV32);
-    bdgx_free(dX32);
-    bdgx_free(dP32);
-    if (b->verbosity > 0)
-      fprintf(stderr, "[bdg] fp32 phase: %d iterations, %.3f s, stalled %d, status %d\n",
-              r.iters, b->stats.seconds_fp32, r.stalled, rc);
-  }
-  if (!rc) {
-    cfg.maxiter = (int)b->maxIter;
-    const double t0 = bdg_wtime();
-    rc = bdg_lobpcg_device(b->dev, BDGX_FP64, &cfg, dV0, dP64, &r, &dX, &dP);
-    b->stats.seconds_fp64 = bdg_wtime() - t0;
-    b->stats.iterations_fp64 = (uint64_t)r.iters;
-    b->stats.column_applies += r.applies;
-    b->stats.restarts += (uint64_t)r.restarts;
-    b->stats.seconds_rr += r.rr_seconds;
-    if (b->verbosity > 0)
-      fprintf(stderr, "[bdg] fp64 phase: %d iterations, %.3f s, status %d\n", r.iters,
-              b->stats.seconds_fp64, rc);
-  }
+  void *dX = NULL, *dMX = NULL;
+  rc = bdg_lobpcg_device(b->dev, &cfg, dV0, &r, &dX);
+  b->stats.iterations = (uint64_t)r.iters;
+  b->stats.column_applies = r.applies;
+  b->stats.column_applies_fp32 = r.applies_fp32;
+  b->stats.image_refreshes = r.refreshes;
+  b->stats.restarts = (uint64_t)r.restarts;
+  b->stats.seconds_rr = r.rr_seconds;
+  if (b->verbosity > 0)
+    fprintf(stderr,
+            "[bdg] %s solve: %d iterations, %llu column applies (%llu fp32), %llu refreshes, "
+            "%.3f s, status %d\n",
+            cfg.w_prec == BDGX_FP32 ? "mixed" : "fp64", r.iters, (unsigned long long)r.applies,
+            (unsigned long long)r.applies_fp32, (unsigned long long)r.refreshes,
+            bdg_wtime() - t_start, rc);
   bdgx_free(dV0);
-  bdgx_free(dP64);
-  bdgx_free(dP);
   /* M X for the u half */
   if (!rc) {
     rc = bdgx_malloc(&dMX, S * m * es64);
     if (!rc) rc = bdgx_apply(b->dev, BDGX_OP_M, BDGX_FP64, dX, dMX, m, S);
   }
-  b->stats.iterations = b->stats.iterations_fp32 + b->stats.iterations_fp64;
   if (rc) {
     bdgx_free(dX);
     bdgx_free(dMX);
@@ -647,9 +616,9 @@ const f64 *bdg_local_term_M(const bdg_t *b) { return b->ltM; }
 const f64 *bdg_precond_weight_K(const bdg_t *b) { return b->pK; }
 const f64 *bdg_precond_weight_M(const bdg_t *b) { return b->pM; }
 f64 bdg_mu(const bdg_t *b) { return b->mu; }
-void bdg_set_mixed_params(bdg_t *b, f64 tol32, uint64_t maxit32) {
-  b->tol_fp32 = tol32;
-  b->maxIter_fp32 = maxit32;
+void bdg_set_mixed_params(bdg_t *b, f64 img_tol, uint64_t refresh_every) {
+  b->img_tol = img_tol;
+  b->refresh_every = refresh_every;
 }
 void bdg_set_verbosity(bdg_t *b, int level) { b->verbosity = level; }
 

@@ -63,8 +63,8 @@ struct bdg_t {
   uint64_t nev, sizeSub, maxIter;
   double tol;
   int precision;
-  double tol_fp32;
-  uint64_t maxIter_fp32;
+  double img_tol;
+  uint64_t refresh_every;
   int verbosity;
   /* results */
   double *eigvals;
@@ -105,28 +105,28 @@ typedef struct {
   double tol;
   double zero_tol;
   int verbosity;
-  int gram_refresh;  /* full Gram recompute every n iterations (0: never)       */
-  int image_refresh; /* re-apply M, K to X (and drop P) every n iterations      */
-  int stall_window;  /* stop when the worst residual has not dropped below      */
-  double stall_factor; /*   stall_factor x its best for stall_window iterations */
+  int gram_refresh;      /* full Gram recompute every n iterations (0: always)     */
+  int w_prec;            /* FP32: mixed policy (T in FP32, images while res high)   */
+  double img_tol;        /* mixed: FP32 images of W while the worst residual > this */
+  int refresh_every;     /* mixed: FP64 images of X every n iterations meanwhile    */
+  int mixed_min_cols;    /* FP32 directions only when at least this many active   */
 } lobpcg_cfg;
 
 typedef struct {
   double *w2;      /* m */
   int *zflag;      /* m */
   double *res;     /* m */
-  int iters, restarts, precond_fallbacks, stalled;
-  uint64_t applies;
+  int iters, restarts, precond_fallbacks;
+  uint64_t applies, applies_fp32, refreshes;
   double rr_seconds;
   int status;
 } lobpcg_result;
 
-/* Runs from the device start block dX0 (S x m, precision prec, ld S), with an
- * optional previous search direction block dP0 (same shape, or NULL).  On
- * success *dX, *dP (device, precision prec, S x m) hold the final block and
- * its last search direction; the caller frees them with bdgx_free. */
-int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void *dX0,
-                      const void *dP0, lobpcg_result *res, void **dX, void **dP);
+/* Runs from the FP64 device start block dX0 (S x m, ld S).  On success *dX
+ * (device, FP64, S x m) holds the final block; the caller frees it with
+ * bdgx_free. */
+int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lobpcg_result *res,
+                      void **dX);
 
 double bdg_wtime(void);
 

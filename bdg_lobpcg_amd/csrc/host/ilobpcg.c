@@ -7,19 +7,37 @@
  *     (M K M) v = w^2 M v,   u = M v / w
  * (K u = w v, M v = w u), on the v half only:
  *
- *   basis V = [X | W | P] (S x k, ld S) in HBM with images MV, KMV;
+ *   basis V = [X | P | W] (S x k, ld S, FP64) in HBM with images MV, KMV;
  *   W = T_M T_K R (the reference's precondK / precondM), R = KMX - w^2 X;
  *   Gram matrices V^H V, V^H MV, MV^H KMV with FP64 accumulation;
  *   Rayleigh-Ritz on the host in FP64 (dense.c), block updates on the device.
  *
+ * One combine per block produces the next X (all m columns) and the next P
+ * (the active columns only) side by side, so they land at [X | P] of the next
+ * basis with no copies.
+ *
  * Gram reuse: X and P are linear combinations of the previous basis, so their
- * Gram blocks follow from the previous Gram matrices and the Rayleigh-Ritz
+ * Gram blocks follow from the previous Gram matrices and the combine
  * coefficients (Q^H G Q); only the blocks that involve the new W are formed
  * on the device.  A full device recompute runs every gram_refresh iterations.
  *
- * Soft locking: converged columns get no W / P.  When [X, W, P] is
- * numerically dependent the P block is dropped for that iteration; when
+ * Soft locking: converged columns get no W / P.  When [X, P, W] is
+ * numerically dependent the P block is left out of that Rayleigh-Ritz; when
  * [X, W] still is, W falls back to the plain residual.
+ *
+ * Mixed precision (w_prec = FP32): the basis, its Grams, the residuals and
+ * the Rayleigh-Ritz stay FP64.  T_K and T_M always run in FP32 (W is only a
+ * search direction, so its rounding changes which direction is searched, not
+ * the accuracy of the basis).  Optionally (img_tol > 0), while the worst
+ * residual is above img_tol, the images M W, K M W are FP32 too, of W
+ * projected out of span(X); their error enters the carried images of X in
+ * proportion to the W coefficients, and the images of X are re-applied in
+ * FP64 every refresh_every iterations.  Below img_tol the images are FP64,
+ * and convergence is only accepted on FP64 images.  FP32 images are off by
+ * default: rounding noise in an FP32 image is amplified by the stiffness of
+ * K M (about 1e-3 relative at 128^3), which stops interior modes from
+ * converging while they are on, so they cost more FP64 iterations later
+ * than they save.
  */
 #include <complex.h>
 #include <time.h>
@@ -148,9 +166,43 @@ static int gram_ok(const double *G, int k, int w) {
   return ok;
 }
 
-static void lead_block(const double *G, int k, int kk, int w, double *out) {
-  for (int j = 0; j < kk; ++j)
-    for (int i = 0; i < kk; ++i) setz(out, kk, i, j, w, getz(G, k, i, j, w));
+/* Solve G Y = Z for Hermitian positive definite G (n x n) and n x r right-hand
+ * sides, in place in Z, by Cholesky.  Returns 0, or -1 when a pivot is not
+ * positive. */
+static int chol_solve(const double *G, int n, int w, double *Z, int r) {
+  double complex *L = (double complex *)malloc(sizeof(double complex) * (size_t)n * n);
+  int ok = 0;
+  for (int j = 0; j < n && !ok; ++j) {
+    double complex d = getz(G, n, j, j, w);
+    for (int p = 0; p < j; ++p) d -= L[j + (size_t)p * n] * conj(L[j + (size_t)p * n]);
+    if (!(creal(d) > 0)) {
+      ok = -1;
+      break;
+    }
+    const double l = sqrt(creal(d));
+    L[j + (size_t)j * n] = l;
+    for (int i = j + 1; i < n; ++i) {
+      double complex s = getz(G, n, i, j, w);
+      for (int p = 0; p < j; ++p) s -= L[i + (size_t)p * n] * conj(L[j + (size_t)p * n]);
+      L[i + (size_t)j * n] = s / l;
+    }
+  }
+  if (!ok)
+    for (int c = 0; c < r; ++c) {
+      /* forward L y = z, backward L^H x = y */
+      for (int i = 0; i < n; ++i) {
+        double complex s = getz(Z, n, i, c, w);
+        for (int p = 0; p < i; ++p) s -= L[i + (size_t)p * n] * getz(Z, n, p, c, w);
+        setz(Z, n, i, c, w, s / L[i + (size_t)i * n]);
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        double complex s = getz(Z, n, i, c, w);
+        for (int p = i + 1; p < n; ++p) s -= conj(L[p + (size_t)i * n]) * getz(Z, n, p, c, w);
+        setz(Z, n, i, c, w, s / L[i + (size_t)i * n]);
+      }
+    }
+  free(L);
+  return ok;
 }
 
 static double signed_sqrt(double x) { return x >= 0 ? sqrt(x) : -sqrt(-x); }
@@ -211,19 +263,75 @@ static int grams_cols(bdgx_ctx *ctx, int prec, const blk3 *b, int k, int c0, int
   return rc;
 }
 
-int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void *dX0,
-                      const void *dP0, lobpcg_result *res, void **dXout, void **dPout) {
+/* residual norms of X and the convergence decisions */
+typedef struct {
+  double *resv;
+  int *conv, *act;
+  int nact, allc;
+  double worst;
+} res_state;
+
+static int eval_residuals(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const blk3 *A, const double *w2,
+                          const int *zf, double *dlam, double *dn2, double *hn2, void *dR,
+                          res_state *st) {
+  const uint64_t S = cfg->S;
+  const int m = cfg->m, nev = cfg->nev;
+  int rc = bdgx_memcpy_h2d(ctx, dlam, w2, sizeof(double) * (size_t)m);
+  if (!rc)
+    rc = bdgx_residual(ctx, BDGX_FP64, A->KMV, A->V, A->MV, S, (uint64_t)m, S, dlam, 0, dR, S,
+                       dn2);
+  if (!rc) rc = bdgx_memcpy_d2h(ctx, hn2, dn2, sizeof(double) * 3 * (size_t)m);
+  if (rc) return rc;
+  double scale = 1.0;
+  for (int j = 0; j < nev && j < m; ++j) scale = fmax(scale, fabs(signed_sqrt(w2[j])));
+  st->nact = 0;
+  st->allc = 1;
+  st->worst = 0.0;
+  for (int j = 0; j < m; ++j) {
+    const double wj = signed_sqrt(w2[j]);
+    const double aw = fmax(fabs(wj), 1e-300);
+    const double rn = sqrt(hn2[3 * j]), xn = sqrt(hn2[3 * j + 1]), mn = sqrt(hn2[3 * j + 2]);
+    const double xnorm = sqrt(xn * xn + (mn / aw) * (mn / aw));
+    st->resv[j] = (rn / aw) / (xnorm * fmax(1.0, fabs(wj)));
+    const int zero = zf[j] || fabs(wj) < cfg->zero_tol * scale;
+    st->conv[j] = zero || st->resv[j] < cfg->tol;
+    if (j < nev) {
+      if (!st->conv[j]) st->allc = 0;
+      if (!zero) st->worst = fmax(st->worst, st->resv[j]);
+    }
+    if (!st->conv[j]) st->act[st->nact++] = j;
+  }
+  return 0;
+}
+
+/* re-apply M then K (FP64) to the X block of A */
+static int refresh_images(bdgx_ctx *ctx, const lobpcg_cfg *cfg, blk3 *A, lobpcg_result *res) {
+  int rc = bdgx_apply(ctx, BDGX_OP_M, BDGX_FP64, A->V, A->MV, (uint64_t)cfg->m, cfg->S);
+  if (!rc) rc = bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, A->MV, A->KMV, (uint64_t)cfg->m, cfg->S);
+  res->applies += (uint64_t)cfg->m;
+  res->refreshes++;
+  return rc;
+}
+
+/* symmetric sub-block G[sel, sel] of a k x k set */
+static void sub_block(const double *G, int k, const int *sel, int ks, int w, double *out) {
+  for (int j = 0; j < ks; ++j)
+    for (int i = 0; i < ks; ++i) setz(out, ks, i, j, w, getz(G, k, sel[i], sel[j], w));
+}
+
+int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lobpcg_result *res,
+                      void **dXout) {
   const uint64_t S = cfg->S;
   const int m = cfg->m, nev = cfg->nev, cplx = cfg->cplx;
   const int w = cplx ? 2 : 1;
-  const size_t es = elem_bytes(prec, cplx);
+  const int mixed = cfg->w_prec == BDGX_FP32;
+  const size_t es = elem_bytes(BDGX_FP64, cplx), es32 = elem_bytes(BDGX_FP32, cplx);
   const int kmax = 3 * m;
   int rc = 0;
-  blk3 A, B, P;
+  blk3 A, B;
   memset(&A, 0, sizeof(A));
   memset(&B, 0, sizeof(B));
-  memset(&P, 0, sizeof(P));
-  void *dR = NULL, *dTmp = NULL;
+  void *dR = NULL, *dTmp = NULL, *f1 = NULL, *f2 = NULL, *f3 = NULL;
   double *dG = NULL, *dC = NULL, *dlam = NULL, *dn2 = NULL;
   const size_t gsz = (size_t)kmax * kmax * w;
   gram3 G, Gp;
@@ -234,261 +342,271 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, int prec, const lobpcg_cfg *cfg, const void
   Gp.M = (double *)calloc(gsz, sizeof(double));
   Gp.A = (double *)calloc(gsz, sizeof(double));
   double *hT = (double *)malloc(sizeof(double) * gsz);
-  double *hQ = (double *)malloc(sizeof(double) * (size_t)kmax * kmax * w);
-  double *hY = (double *)malloc(sizeof(double) * (size_t)kmax * m * w);
-  double *hYp = (double *)malloc(sizeof(double) * (size_t)kmax * m * w);
+  double *hS[3];
+  for (int g = 0; g < 3; ++g) hS[g] = (double *)malloc(sizeof(double) * gsz);
+  double *hY = (double *)calloc((size_t)kmax * m * w, sizeof(double));
+  double *hYs = (double *)malloc(sizeof(double) * (size_t)kmax * m * w);
+  double *hCn = (double *)calloc((size_t)kmax * 2 * m * w, sizeof(double)); /* [X' | P'] */
+  double *hCp = (double *)calloc((size_t)kmax * 2 * m * w, sizeof(double)); /* previous */
   double *w2 = (double *)calloc((size_t)m, sizeof(double));
   int *zf = (int *)calloc((size_t)m, sizeof(int));
   double *hn2 = (double *)malloc(sizeof(double) * 3 * (size_t)m);
-  double *resv = (double *)malloc(sizeof(double) * (size_t)m);
-  int *conv = (int *)calloc((size_t)m, sizeof(int));
-  int *act = (int *)malloc(sizeof(int) * (size_t)m);
-  int haveP = 0, it = 0, kprev = 0, gvalid = 0, since_full = 0, since_img = 0;
-  double trr = 0.0, best = 1e300;
-  int best_it = 0;
+  int *sel = (int *)malloc(sizeof(int) * (size_t)kmax);
+  res_state st;
+  st.resv = (double *)malloc(sizeof(double) * (size_t)m);
+  st.conv = (int *)calloc((size_t)m, sizeof(int));
+  st.act = (int *)malloc(sizeof(int) * (size_t)m);
+  int it = 0, np = 0, kprev = 0, ncp = 0, gvalid = 0, since_full = 0, fresh = 1;
+  double trr = 0.0;
+  int refresh_it = 0;
+  const double t_start = bdg_wtime();
   res->status = 0;
   res->iters = 0;
   res->restarts = 0;
   res->precond_fallbacks = 0;
-  res->stalled = 0;
   res->applies = 0;
+  res->applies_fp32 = 0;
+  res->refreshes = 0;
 
   TRY(alloc3(&A, (size_t)S * kmax * es));
   TRY(alloc3(&B, (size_t)S * kmax * es));
-  TRY(alloc3(&P, (size_t)S * m * es));
   TRY(bdgx_malloc(&dR, (size_t)S * m * es));
   TRY(bdgx_malloc(&dTmp, (size_t)S * m * es));
+  if (mixed) {
+    TRY(bdgx_malloc(&f1, (size_t)S * m * es32));
+    TRY(bdgx_malloc(&f2, (size_t)S * m * es32));
+    TRY(bdgx_malloc(&f3, (size_t)S * m * es32));
+  }
   TRY(bdgx_malloc((void **)&dG, sizeof(double) * 3 * gsz));
-  TRY(bdgx_malloc((void **)&dC, sizeof(double) * (size_t)kmax * kmax * w));
+  TRY(bdgx_malloc((void **)&dC, sizeof(double) * (size_t)kmax * 2 * m * w));
   TRY(bdgx_malloc((void **)&dlam, sizeof(double) * (size_t)m));
   TRY(bdgx_malloc((void **)&dn2, sizeof(double) * 3 * (size_t)m));
 
-  /* start: basis [X0] or [X0 | P0], images by M then K, Rayleigh-Ritz */
+  /* start: basis [X0], images by M then K (FP64), Rayleigh-Ritz, X = X0 Y */
   {
-    const int k0 = dP0 ? 2 * m : m;
-    TRY(bdgx_convert(ctx, prec, dX0, S, prec, A.V, S, S, (uint64_t)m));
-    if (dP0) TRY(bdgx_convert(ctx, prec, dP0, S, prec, colp(A.V, (uint64_t)m, S, es), S, S, (uint64_t)m));
-    TRY(bdgx_apply(ctx, BDGX_OP_M, prec, A.V, A.MV, (uint64_t)k0, S));
-    TRY(bdgx_apply(ctx, BDGX_OP_K, prec, A.MV, A.KMV, (uint64_t)k0, S));
-    res->applies += (uint64_t)k0;
-    TRY(grams_full(ctx, prec, &A, k0, S, w, dG, &G));
-    int k = k0;
-    if (dP0 && !gram_ok(G.V, k0, w)) {
-      lead_block(G.V, k0, m, w, hT);
-      memcpy(G.V, hT, sizeof(double) * (size_t)m * m * w);
-      lead_block(G.M, k0, m, w, hT);
-      memcpy(G.M, hT, sizeof(double) * (size_t)m * m * w);
-      lead_block(G.A, k0, m, w, hT);
-      memcpy(G.A, hT, sizeof(double) * (size_t)m * m * w);
-      k = m;
-    }
+    TRY(bdgx_convert(ctx, BDGX_FP64, dX0, S, BDGX_FP64, A.V, S, S, (uint64_t)m));
+    TRY(bdgx_apply(ctx, BDGX_OP_M, BDGX_FP64, A.V, A.MV, (uint64_t)m, S));
+    TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, A.MV, A.KMV, (uint64_t)m, S));
+    res->applies += (uint64_t)m;
+    TRY(grams_full(ctx, BDGX_FP64, &A, m, S, w, dG, &G));
     const double t0 = bdg_wtime();
-    const int got = dn_rayleigh_ritz(k, m, cplx, G.V, G.M, G.A, w2, hY, zf);
+    const int got = dn_rayleigh_ritz(m, m, cplx, G.V, G.M, G.A, w2, hY, zf);
     trr += bdg_wtime() - t0;
     if (got < m) {
       rc = -100;
       goto done;
     }
-    TRY(bdgx_memcpy_h2d(ctx, dC, hY, sizeof(double) * (size_t)k * m * w));
-    TRY(combine3(ctx, prec, &A, 0, (uint64_t)k, dC, &B, 0, (uint64_t)m, S, es));
-    if (k > m) {
-      for (int j = 0; j < m; ++j)
-        for (int i = 0; i < k - m; ++i)
-          for (int q = 0; q < w; ++q)
-            hYp[(i + (size_t)j * (k - m)) * w + q] = hY[((m + i) + (size_t)j * k) * w + q];
-      TRY(bdgx_memcpy_h2d(ctx, dC, hYp, sizeof(double) * (size_t)(k - m) * m * w));
-      TRY(combine3(ctx, prec, &A, (uint64_t)m, (uint64_t)(k - m), dC, &P, 0, (uint64_t)m, S, es));
-      haveP = 1;
-    }
-    memcpy(Gp.V, G.V, sizeof(double) * (size_t)k * k * w);
-    memcpy(Gp.M, G.M, sizeof(double) * (size_t)k * k * w);
-    memcpy(Gp.A, G.A, sizeof(double) * (size_t)k * k * w);
-    kprev = k;
+    TRY(bdgx_memcpy_h2d(ctx, dC, hY, sizeof(double) * (size_t)m * m * w));
+    TRY(combine3(ctx, BDGX_FP64, &A, 0, (uint64_t)m, dC, &B, 0, (uint64_t)m, S, es));
+    memcpy(Gp.V, G.V, sizeof(double) * (size_t)m * m * w);
+    memcpy(Gp.M, G.M, sizeof(double) * (size_t)m * m * w);
+    memcpy(Gp.A, G.A, sizeof(double) * (size_t)m * m * w);
+    memcpy(hCp, hY, sizeof(double) * (size_t)m * m * w);
+    kprev = m;
+    ncp = m;
+    np = 0;
     gvalid = 1;
-    {
-      blk3 t = A;
-      A = B;
-      B = t;
-    }
+    blk3 t = A;
+    A = B;
+    B = t;
   }
 
   for (it = 1; it <= cfg->maxiter; ++it) {
-    /* optional periodic re-application of M, K to X (removes the drift of
-     * images kept by linear combination; P is dropped at that point) */
-    if (cfg->image_refresh > 0 && ++since_img >= cfg->image_refresh) {
-      since_img = 0;
-      TRY(bdgx_apply(ctx, BDGX_OP_M, prec, A.V, A.MV, (uint64_t)m, S));
-      TRY(bdgx_apply(ctx, BDGX_OP_K, prec, A.MV, A.KMV, (uint64_t)m, S));
-      res->applies += (uint64_t)m;
+    TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st));
+    /* mixed: FP64 images of X every refresh_every iterations while new
+     * directions get FP32 images, when the residual reaches img_tol (FP64
+     * images from then on), and before convergence is accepted */
+    if (mixed && !fresh &&
+        (st.allc || st.worst <= cfg->img_tol ||
+         (cfg->refresh_every > 0 && it - refresh_it >= cfg->refresh_every))) {
+      TRY(refresh_images(ctx, cfg, &A, res));
+      fresh = 1;
       gvalid = 0;
+      refresh_it = it;
+      TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st));
     }
-    /* residuals R = KMX - w2 X and norms */
-    TRY(bdgx_memcpy_h2d(ctx, dlam, w2, sizeof(double) * (size_t)m));
-    TRY(bdgx_residual(ctx, prec, A.KMV, A.V, A.MV, S, (uint64_t)m, S, dlam, 0, dR, S, dn2));
-    TRY(bdgx_memcpy_d2h(ctx, hn2, dn2, sizeof(double) * 3 * (size_t)m));
-    double scale = 1.0;
-    for (int j = 0; j < nev && j < m; ++j) scale = fmax(scale, fabs(signed_sqrt(w2[j])));
-    int nact = 0, allc = 1;
-    double worst = 0.0;
-    for (int j = 0; j < m; ++j) {
-      const double wj = signed_sqrt(w2[j]);
-      const double aw = fmax(fabs(wj), 1e-300);
-      const double rn = sqrt(hn2[3 * j]), xn = sqrt(hn2[3 * j + 1]), mn = sqrt(hn2[3 * j + 2]);
-      const double xnorm = sqrt(xn * xn + (mn / aw) * (mn / aw));
-      resv[j] = (rn / aw) / (xnorm * fmax(1.0, fabs(wj)));
-      const int zero = zf[j] || fabs(wj) < cfg->zero_tol * scale;
-      conv[j] = zero || resv[j] < cfg->tol;
-      if (j < nev) {
-        if (!conv[j]) allc = 0;
-        if (!zero) worst = fmax(worst, resv[j]);
+    const int nact = st.nact;
+    const int *act = st.act;
+    if (cfg->verbosity > 1 || (cfg->verbosity == 1 && it % 10 == 0)) {
+      fprintf(stderr, "[bdg] it %d t %.3f w0 %.10g wlast %.10g worst res %.3e active %d%s", it,
+              bdg_wtime() - t_start, signed_sqrt(w2[0]), signed_sqrt(w2[nev - 1]), st.worst,
+              nact, fresh ? "" : " (fp32 images)");
+      if (nact <= 8) {
+        fprintf(stderr, " [");
+        for (int j = 0; j < nact; ++j)
+          fprintf(stderr, "%s%d:%.4g/%.2e", j ? " " : "", act[j], signed_sqrt(w2[act[j]]),
+                  st.resv[act[j]]);
+        fprintf(stderr, "]");
       }
-      if (!conv[j]) act[nact++] = j;
+      fprintf(stderr, "\n");
     }
-    if (cfg->verbosity > 1 || (cfg->verbosity == 1 && it % 10 == 0))
-      fprintf(stderr, "[bdg] it %d prec %s w0 %.10g wlast %.10g worst res %.3e active %d\n", it,
-              prec == BDGX_FP64 ? "fp64" : "fp32", signed_sqrt(w2[0]),
-              signed_sqrt(w2[nev - 1]), worst, nact);
-    if (allc) break;
+    if (st.allc) break;
     if (it == cfg->maxiter) break;
-    if (cfg->stall_window > 0) {
-      if (worst < cfg->stall_factor * best) {
-        best = worst;
-        best_it = it;
-      } else if (it - best_it >= cfg->stall_window || worst > 10.0 * best) {
-        res->stalled = 1;
-        break;
-      }
+
+    /* W = T_M T_K R[:, act] at columns [m + np, m + np + nact) */
+    const int wc = m + np;
+    const int k = m + np + nact;
+    const void *Rsrc = dR;
+    if (nact < m) {
+      TRY(bdgx_gather_cols(ctx, BDGX_FP64, dR, S, act, (uint64_t)nact, dTmp, S, S));
+      Rsrc = dTmp;
     }
-    /* W = T_M T_K R[:, act] into A.V[:, m : m+nact] */
-    TRY(bdgx_gather_cols(ctx, prec, dR, S, act, (uint64_t)nact, dTmp, S, S));
-    TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_K, prec, dTmp, dR, (uint64_t)nact, S));
-    TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_M, prec, dR, colp(A.V, (uint64_t)m, S, es),
-                   (uint64_t)nact, S));
-    TRY(bdgx_apply(ctx, BDGX_OP_M, prec, colp(A.V, (uint64_t)m, S, es),
-                   colp(A.MV, (uint64_t)m, S, es), (uint64_t)nact, S));
-    TRY(bdgx_apply(ctx, BDGX_OP_K, prec, colp(A.MV, (uint64_t)m, S, es),
-                   colp(A.KMV, (uint64_t)m, S, es), (uint64_t)nact, S));
-    res->applies += (uint64_t)nact;
-    const int np = haveP ? nact : 0;
-    int k = m + nact + np;
-    if (np) {
-      TRY(bdgx_gather_cols(ctx, prec, P.V, S, act, (uint64_t)nact,
-                           colp(A.V, (uint64_t)(m + nact), S, es), S, S));
-      TRY(bdgx_gather_cols(ctx, prec, P.MV, S, act, (uint64_t)nact,
-                           colp(A.MV, (uint64_t)(m + nact), S, es), S, S));
-      TRY(bdgx_gather_cols(ctx, prec, P.KMV, S, act, (uint64_t)nact,
-                           colp(A.KMV, (uint64_t)(m + nact), S, es), S, S));
-    }
-    /* Gram set of [X | W | P_act] */
+    void *other = (Rsrc == dR) ? dTmp : dR;
     const int full = !gvalid || cfg->gram_refresh <= 0 || ++since_full >= cfg->gram_refresh;
-    if (full) {
-      since_full = 0;
-      TRY(grams_full(ctx, prec, &A, k, S, w, dG, &G));
-    } else {
-      /* X = V_prev C, P_act = V_prev[:, m:] C[m:, act]: blocks from Q^H G_prev Q */
+    if (!full) {
+      /* [X | P] = V_prev [C_X | C_P]: Gram blocks from Q^H G_prev Q */
       const int q = m + np;
-      for (int j = 0; j < q; ++j)
-        for (int i = 0; i < kprev; ++i) {
-          double complex v = 0;
-          if (j < m)
-            v = getz(hY, kprev, i, j, w);
-          else if (i >= m)
-            v = getz(hY, kprev, i, act[j - m], w);
-          setz(hQ, kprev, i, j, w, v);
-        }
       double *gp[3] = {Gp.V, Gp.M, Gp.A}, *gn[3] = {G.V, G.M, G.A};
       for (int g = 0; g < 3; ++g) {
-        qhgq(gp[g], kprev, hQ, q, w, hT);
+        qhgq(gp[g], kprev, hCp, q, w, hT);
         for (int j = 0; j < q; ++j)
-          for (int i = 0; i < q; ++i) {
-            const int ii = i < m ? i : i + nact, jj = j < m ? j : j + nact;
-            setz(gn[g], k, ii, jj, w, getz(hT, q, i, j, w));
-          }
+          for (int i = 0; i < q; ++i) setz(gn[g], k, i, j, w, getz(hT, q, i, j, w));
       }
-      TRY(grams_cols(ctx, prec, &A, k, m, nact, S, w, es, dG, &G));
     }
-    double t0 = bdg_wtime();
-    if (np && !gram_ok(G.V, k, w)) {
-      /* [X, W, P] numerically dependent: drop P for this iteration */
-      const int kk = m + nact;
-      double *gn[3] = {G.V, G.M, G.A};
-      for (int g = 0; g < 3; ++g) {
-        lead_block(gn[g], k, kk, w, hT);
-        memcpy(gn[g], hT, sizeof(double) * (size_t)kk * kk * w);
+    const int img32 =
+        mixed && cfg->img_tol > 0 && nact >= cfg->mixed_min_cols && st.worst > cfg->img_tol;
+    void *wdst = colp(A.V, (uint64_t)wc, S, es);
+    if (!mixed) {
+      TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_K, BDGX_FP64, Rsrc, other, (uint64_t)nact, S));
+      TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_M, BDGX_FP64, other, wdst, (uint64_t)nact, S));
+    } else {
+      /* T_K, T_M in FP32: W is a search direction, its rounding only changes
+       * which direction is searched, not the accuracy of the basis */
+      TRY(bdgx_convert(ctx, BDGX_FP64, Rsrc, S, BDGX_FP32, f1, S, S, (uint64_t)nact));
+      TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_K, BDGX_FP32, f1, f2, (uint64_t)nact, S));
+      TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_M, BDGX_FP32, f2, f1, (uint64_t)nact, S));
+      TRY(bdgx_convert(ctx, BDGX_FP32, f1, S, BDGX_FP64, wdst, S, S, (uint64_t)nact));
+    }
+    if (!img32) {
+      TRY(bdgx_apply(ctx, BDGX_OP_M, BDGX_FP64, wdst, colp(A.MV, (uint64_t)wc, S, es),
+                     (uint64_t)nact, S));
+      TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, colp(A.MV, (uint64_t)wc, S, es),
+                     colp(A.KMV, (uint64_t)wc, S, es), (uint64_t)nact, S));
+    } else {
+      /* FP32 images of W, with W first projected out of span(X) (Euclidean,
+       * FP64) so that the image error is relative to the new part of W */
+      const size_t zsz = (size_t)m * nact * w;
+      TRY(bdgx_gram(ctx, BDGX_FP64, A.V, S, (uint64_t)m, wdst, S, (uint64_t)nact, S, 0, dG));
+      if (full)
+        TRY(bdgx_gram(ctx, BDGX_FP64, A.V, S, (uint64_t)m, A.V, S, (uint64_t)m, S, 0, dG + zsz));
+      TRY(bdgx_memcpy_d2h(ctx, hT, dG, sizeof(double) * (zsz + (full ? (size_t)m * m * w : 0))));
+      if (full) {
+        memcpy(hS[1], hT + zsz, sizeof(double) * (size_t)m * m * w);
+      } else {
+        for (int j = 0; j < m; ++j)
+          for (int i = 0; i < m; ++i) setz(hS[1], m, i, j, w, getz(G.V, k, i, j, w));
       }
-      k = kk;
+      if (chol_solve(hS[1], m, w, hT, nact) == 0) {
+        /* W <- [X | P | W] [-Y; 0; I] into `other`, then back into the basis */
+        memset(hCn, 0, sizeof(double) * (size_t)k * nact * w);
+        for (int j = 0; j < nact; ++j) {
+          for (int i = 0; i < m; ++i) setz(hCn, k, i, j, w, -getz(hT, m, i, j, w));
+          setz(hCn, k, wc + j, j, w, 1.0);
+        }
+        TRY(bdgx_memcpy_h2d(ctx, dC, hCn, sizeof(double) * (size_t)k * nact * w));
+        TRY(bdgx_combine(ctx, BDGX_FP64, A.V, S, (uint64_t)k, dC, other, S, (uint64_t)nact, S));
+        TRY(bdgx_convert(ctx, BDGX_FP64, other, S, BDGX_FP64, wdst, S, S, (uint64_t)nact));
+        TRY(bdgx_convert(ctx, BDGX_FP64, other, S, BDGX_FP32, f1, S, S, (uint64_t)nact));
+      }
+      TRY(bdgx_apply(ctx, BDGX_OP_M, BDGX_FP32, f1, f2, (uint64_t)nact, S));
+      TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP32, f2, f3, (uint64_t)nact, S));
+      TRY(bdgx_convert(ctx, BDGX_FP32, f2, S, BDGX_FP64, colp(A.MV, (uint64_t)wc, S, es), S, S,
+                       (uint64_t)nact));
+      TRY(bdgx_convert(ctx, BDGX_FP32, f3, S, BDGX_FP64, colp(A.KMV, (uint64_t)wc, S, es), S, S,
+                       (uint64_t)nact));
+      res->applies_fp32 += (uint64_t)nact;
+      fresh = 0;
+    }
+    res->applies += (uint64_t)nact;
+
+    /* Gram set of [X | P | W] */
+    if (full) {
+      since_full = 0;
+      TRY(grams_full(ctx, BDGX_FP64, &A, k, S, w, dG, &G));
+    } else {
+      TRY(grams_cols(ctx, BDGX_FP64, &A, k, wc, nact, S, w, es, dG, &G));
+    }
+    /* Rayleigh-Ritz on a column selection: all, or without P when [X P W]
+     * is numerically dependent */
+    double t0 = bdg_wtime();
+    int ks = 0;
+    for (int i = 0; i < k; ++i) sel[ks++] = i;
+    sub_block(G.V, k, sel, ks, w, hS[0]);
+    if (np && !gram_ok(hS[0], ks, w)) {
+      ks = 0;
+      for (int i = 0; i < m; ++i) sel[ks++] = i;
+      for (int i = wc; i < k; ++i) sel[ks++] = i;
+      sub_block(G.V, k, sel, ks, w, hS[0]);
       res->restarts++;
     }
-    if (!gram_ok(G.V, k, w)) {
+    if (!gram_ok(hS[0], ks, w)) {
       /* [X, W] still dependent: the preconditioner collapsed the residuals
        * (e.g. a 1e8 gain on one mode); use W = R for this iteration */
       trr += bdg_wtime() - t0;
-      k = m + nact;
-      TRY(bdgx_convert(ctx, prec, dTmp, S, prec, colp(A.V, (uint64_t)m, S, es), S, S,
+      TRY(bdgx_convert(ctx, BDGX_FP64, Rsrc, S, BDGX_FP64, colp(A.V, (uint64_t)wc, S, es), S, S,
                        (uint64_t)nact));
-      TRY(bdgx_apply(ctx, BDGX_OP_M, prec, colp(A.V, (uint64_t)m, S, es),
-                     colp(A.MV, (uint64_t)m, S, es), (uint64_t)nact, S));
-      TRY(bdgx_apply(ctx, BDGX_OP_K, prec, colp(A.MV, (uint64_t)m, S, es),
-                     colp(A.KMV, (uint64_t)m, S, es), (uint64_t)nact, S));
+      TRY(bdgx_apply(ctx, BDGX_OP_M, BDGX_FP64, colp(A.V, (uint64_t)wc, S, es),
+                     colp(A.MV, (uint64_t)wc, S, es), (uint64_t)nact, S));
+      TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, colp(A.MV, (uint64_t)wc, S, es),
+                     colp(A.KMV, (uint64_t)wc, S, es), (uint64_t)nact, S));
       res->applies += (uint64_t)nact;
       res->precond_fallbacks++;
-      TRY(grams_full(ctx, prec, &A, k, S, w, dG, &G));
+      TRY(grams_full(ctx, BDGX_FP64, &A, k, S, w, dG, &G));
       since_full = 0;
+      ks = 0;
+      for (int i = 0; i < m; ++i) sel[ks++] = i;
+      for (int i = wc; i < k; ++i) sel[ks++] = i;
+      sub_block(G.V, k, sel, ks, w, hS[0]);
       t0 = bdg_wtime();
     }
-    const int got = dn_rayleigh_ritz(k, m, cplx, G.V, G.M, G.A, w2, hY, zf);
+    sub_block(G.M, k, sel, ks, w, hS[1]);
+    sub_block(G.A, k, sel, ks, w, hS[2]);
+    const int got = dn_rayleigh_ritz(ks, m, cplx, hS[0], hS[1], hS[2], w2, hYs, zf);
     trr += bdg_wtime() - t0;
     if (got < m) {
       rc = -100;
       goto done;
     }
-    /* X' = V C, P' = V[:, m:k] C[m:k, :] */
-    TRY(bdgx_memcpy_h2d(ctx, dC, hY, sizeof(double) * (size_t)k * m * w));
-    TRY(combine3(ctx, prec, &A, 0, (uint64_t)k, dC, &B, 0, (uint64_t)m, S, es));
+    /* coefficients on the full basis: [C_X | C_P], C_P = rows m.. of the
+     * active columns of C_X (P' = [P W] part of the update) */
+    const int nc = m + nact;
+    memset(hCn, 0, sizeof(double) * (size_t)k * nc * w);
     for (int j = 0; j < m; ++j)
-      for (int i = 0; i < k - m; ++i)
-        for (int q = 0; q < w; ++q)
-          hYp[(i + (size_t)j * (k - m)) * w + q] = hY[((m + i) + (size_t)j * k) * w + q];
-    TRY(bdgx_memcpy_h2d(ctx, dC, hYp, sizeof(double) * (size_t)(k - m) * m * w));
-    TRY(combine3(ctx, prec, &A, (uint64_t)m, (uint64_t)(k - m), dC, &P, 0, (uint64_t)m, S, es));
-    haveP = 1;
+      for (int i = 0; i < ks; ++i) setz(hCn, k, sel[i], j, w, getz(hYs, ks, i, j, w));
+    for (int j = 0; j < nact; ++j)
+      for (int i = m; i < k; ++i) setz(hCn, k, i, m + j, w, getz(hCn, k, i, act[j], w));
+    TRY(bdgx_memcpy_h2d(ctx, dC, hCn, sizeof(double) * (size_t)k * nc * w));
+    TRY(combine3(ctx, BDGX_FP64, &A, 0, (uint64_t)k, dC, &B, 0, (uint64_t)nc, S, es));
     memcpy(Gp.V, G.V, sizeof(double) * (size_t)k * k * w);
     memcpy(Gp.M, G.M, sizeof(double) * (size_t)k * k * w);
     memcpy(Gp.A, G.A, sizeof(double) * (size_t)k * k * w);
+    memcpy(hCp, hCn, sizeof(double) * (size_t)k * nc * w);
     kprev = k;
+    ncp = nc;
+    np = nact;
     gvalid = 1;
-    {
-      blk3 t = A;
-      A = B;
-      B = t;
-    }
+    blk3 t = A;
+    A = B;
+    B = t;
   }
+  (void)ncp;
   if (it > cfg->maxiter) it = cfg->maxiter;
   TRY(bdgx_sync(ctx));
-  /* hand back X and P (copies into fresh buffers) */
   {
-    void *x = NULL, *p = NULL;
+    void *x = NULL;
     TRY(bdgx_malloc(&x, (size_t)S * m * es));
-    rc = bdgx_malloc(&p, (size_t)S * m * es);
-    if (!rc) rc = bdgx_convert(ctx, prec, A.V, S, prec, x, S, S, (uint64_t)m);
-    if (!rc && haveP) rc = bdgx_convert(ctx, prec, P.V, S, prec, p, S, S, (uint64_t)m);
-    if (!rc && !haveP) {
-      bdgx_free(p);
-      p = NULL;
-    }
+    rc = bdgx_convert(ctx, BDGX_FP64, A.V, S, BDGX_FP64, x, S, S, (uint64_t)m);
     if (!rc) rc = bdgx_sync(ctx);
     if (rc) {
       bdgx_free(x);
-      bdgx_free(p);
       goto done;
     }
     *dXout = x;
-    *dPout = p;
   }
   for (int j = 0; j < m; ++j) {
     res->w2[j] = w2[j];
     res->zflag[j] = zf[j];
-    res->res[j] = resv[j];
+    res->res[j] = st.resv[j];
   }
 done:
   res->iters = it;
@@ -496,9 +614,11 @@ done:
   res->status = rc;
   free3(&A);
   free3(&B);
-  free3(&P);
   bdgx_free(dR);
   bdgx_free(dTmp);
+  bdgx_free(f1);
+  bdgx_free(f2);
+  bdgx_free(f3);
   bdgx_free(dG);
   bdgx_free(dC);
   bdgx_free(dlam);
@@ -510,14 +630,17 @@ done:
   free(Gp.M);
   free(Gp.A);
   free(hT);
-  free(hQ);
+  for (int g = 0; g < 3; ++g) free(hS[g]);
   free(hY);
-  free(hYp);
+  free(hYs);
+  free(hCn);
+  free(hCp);
   free(w2);
   free(zf);
   free(hn2);
-  free(resv);
-  free(conv);
-  free(act);
+  free(sel);
+  free(st.resv);
+  free(st.conv);
+  free(st.act);
   return rc;
 }

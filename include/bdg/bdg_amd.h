@@ -15,23 +15,24 @@ extern "C" {
 
 /* Solver precision policy. */
 typedef enum {
-  BDG_PREC_FP64 = 0,   /* every operator apply and block update in FP64 (default) */
-  BDG_PREC_MIXED = 1   /* FP32 bulk iterations, then an FP64 refinement phase */
+  BDG_PREC_FP64 = 0,  /* every operator apply and block update in FP64 (default)   */
+  BDG_PREC_MIXED = 1  /* FP64 basis, Grams, residuals, images and Rayleigh-Ritz;
+                         the preconditioner in FP32 (optionally also the images
+                         of new directions while the residual is large, see
+                         bdg_set_mixed_params) */
 } bdg_precision_t;
 
 void bdg_set_precision(bdg_t *bdg, bdg_precision_t prec);
 
 typedef struct {
-  uint64_t iterations;       /* total LOBPCG iterations                    */
-  uint64_t iterations_fp32;  /* of which in FP32 (mixed policy)            */
-  uint64_t iterations_fp64;  /* of which in FP64                           */
-  uint64_t column_applies;   /* (M then K) column applies, incl. start     */
-  uint64_t restarts;         /* iterations that dropped the P block        */
-  double seconds_total;      /* bdg_solve wall clock                       */
-  double seconds_fp32;
-  double seconds_fp64;
-  double seconds_rr;         /* host Rayleigh-Ritz time                    */
-  int status;                /* 0 ok, else a BDGX_E* code or -100 (RR)    */
+  uint64_t iterations;          /* LOBPCG iterations                           */
+  uint64_t column_applies;      /* (M then K) column applies, incl. start      */
+  uint64_t column_applies_fp32; /* of which in FP32 (mixed policy)             */
+  uint64_t image_refreshes;     /* FP64 re-applications of M, K to X (mixed)   */
+  uint64_t restarts;            /* iterations that left the P block out        */
+  double seconds_total;         /* bdg_solve wall clock                        */
+  double seconds_rr;            /* host Rayleigh-Ritz time                     */
+  int status;                   /* 0 ok, else a BDGX_E* code or -100 (RR)     */
 } bdg_solve_stats_t;
 
 void bdg_get_solve_stats(const bdg_t *bdg, bdg_solve_stats_t *out);
@@ -52,9 +53,11 @@ const f64 *bdg_precond_weight_K(const bdg_t *bdg);
 const f64 *bdg_precond_weight_M(const bdg_t *bdg);
 f64 bdg_mu(const bdg_t *bdg);
 
-/* Residual tolerance of the FP32 phase of the mixed policy (default 1e-5)
- * and the iteration budget of the FP32 phase (default maxIter). */
-void bdg_set_mixed_params(bdg_t *bdg, f64 tol_fp32, uint64_t max_iter_fp32);
+/* Mixed policy: new search directions get FP32 images while the worst
+ * relative residual is above img_tol (default 0: never); meanwhile the images
+ * of X are re-applied in FP64 every refresh_every iterations (default 20, 0:
+ * only when the residual reaches img_tol). */
+void bdg_set_mixed_params(bdg_t *bdg, f64 img_tol, uint64_t refresh_every);
 
 /* 0 silent (default), 1 per-phase summary, 2 per-iteration trace (stderr). */
 void bdg_set_verbosity(bdg_t *bdg, int level);
