@@ -97,20 +97,21 @@ __device__ __forceinline__ void fixed_stage(const SRC &src, const DST &dst, cons
   }
 }
 
-// Stage dispatcher over the compile-time plan.  Stage 0 reads `first`, the
-// last stage writes `last` (through `mul`), intermediate stages ping-pong
-// between the LDS buffers A and B.  Barriers separate every stage.
+// Stage dispatcher over the compile-time plan.  Stage 0 reads `first` and
+// writes X, stage 1 reads X and writes Y, and so on alternating; the last
+// stage writes `last` (through `mul`) instead.  Barriers follow every stage.
+// With nst stages the last LDS buffer written before `last` is X when nst is
+// even, Y when odd (for nst == 1 none is).
 template <typename R, bool INV, int N, int P, int NT, int S, class FIRST, class LAST, class MUL>
-__device__ __forceinline__ void fixed_stages(const FIRST &first, const LAST &last, cx<R> *A,
-                                             cx<R> *B, const cx<R> *tw, const MUL &mul) {
+__device__ __forceinline__ void fixed_stages(const FIRST &first, const LAST &last, cx<R> *X,
+                                             cx<R> *Y, const cx<R> *tw, const MUL &mul) {
   constexpr FixedPlan pl = make_fixed_plan(N);
   if constexpr (S < pl.nst) {
     constexpr int RAD = pl.rad[S];
     constexpr int NS = pl.ns[S];
     constexpr bool is_first = S == 0, is_last = S == pl.nst - 1;
-    // ping-pong: stage S writes A when S is even, B when odd; reads the other
-    LdsIO<R> in{(S % 2 == 0) ? B : A};
-    LdsIO<R> out{(S % 2 == 0) ? A : B};
+    LdsIO<R> in{(S % 2 == 1) ? X : Y};
+    LdsIO<R> out{(S % 2 == 0) ? X : Y};
     if constexpr (is_first && is_last)
       fixed_stage<R, INV, N, RAD, NS, P, NT>(first, last, tw, mul);
     else if constexpr (is_first)
@@ -120,7 +121,7 @@ __device__ __forceinline__ void fixed_stages(const FIRST &first, const LAST &las
     else
       fixed_stage<R, INV, N, RAD, NS, P, NT>(in, out, tw, NoMul{});
     __syncthreads();
-    fixed_stages<R, INV, N, P, NT, S + 1>(first, last, A, B, tw, mul);
+    fixed_stages<R, INV, N, P, NT, S + 1>(first, last, X, Y, tw, mul);
   }
 }
 

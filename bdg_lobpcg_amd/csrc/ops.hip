@@ -26,46 +26,11 @@
 
 #include "bdgx_internal.hpp"
 #include "fft_core.hpp"
+#include "plane.hpp"
 
 namespace bdgx {
 
 constexpr int kThreads = 256;
-
-template <typename R>
-struct XArgs {
-  Geo g;
-  int ncols;
-  long long ld;
-  const R *X;
-  R *Y;
-  int in_half, out_half;
-  int nch, chan_base;
-  int pre[kMaxChan];
-  const R *prew[kMaxChan];
-  int post[kMaxChan];
-  const R *postw[kMaxChan];
-  R posts[kMaxChan];
-  const R *lt;
-  int out_real;
-  cx<R> *spec;
-  long long chan_stride;
-  const cx<R> *tw;
-  AxisPlan pl;
-  int rb;  // rows per workgroup
-};
-
-template <typename R>
-struct YZArgs {
-  Geo g;
-  int ncols, nb, P;
-  cx<R> *spec;
-  const cx<R> *tw;
-  AxisPlan pl;
-  // z pass only
-  int mul[kMaxChan];
-  const R *kx2, *ky2, *kz2, *U;
-  R mu, invS, dt;
-};
 
 // ---------------------------------------------------------------------------
 // x-axis forward, real input (two-for-one)
@@ -493,6 +458,7 @@ static XArgs<R> make_xargs(bdgx_ctx *c, const HalfJob &J, int base, int nc, cons
   a.spec = spec;
   a.chan_stride = cstride;
   a.tw = F.tw[0];
+  a.tw_y = F.tw[1];
   a.pl = c->plan[0];
   return a;
 }
@@ -588,6 +554,7 @@ static int launch_z(bdgx_ctx *c, YZArgs<R> z) {
   Fields<R> &F = fields<R>(c);
   z.tw = F.tw[2];
   z.pl = c->plan[2];
+  if (zpass_fixed_supported<R>(c)) return launch_zpass_fixed<R>(c, z);
   const int ntile = (g.n0k + z.P - 1) / z.P;
   const size_t sm = sizeof(cx<R>) * ((size_t)g.nz + 2 * (size_t)z.P * (g.nz + 1));
   hipLaunchKernelGGL(k_zpass<R>, dim3(ntile * g.ny * z.nb), dim3(kThreads), sm, c->stream, z);
@@ -621,20 +588,23 @@ static int run_jobs(bdgx_ctx *c, const HalfJob *jobs, int njobs, const R *X,
     const R *Xc = X + c0 * ld * cw;
     R *Yc = Y ? Y + c0 * ld * cw : nullptr;
     const long long cstride = (long long)nc * g.Kk;
+    // whole-plane x/y kernels where the spectrum plane fits in LDS, else
+    // separate x and y pencil passes
+    const bool plane = plane_supported<R>(c);
     int base = 0;
     for (int j = 0; j < njobs; ++j) {
       XArgs<R> a = make_xargs<R>(c, jobs[j], base, nc, Xc, Yc, ld, spec, cstride, lt_override);
-      if ((rc = launch_x<R>(c, a, false))) return rc;
+      if ((rc = plane ? launch_plane_x<R>(c, a, false) : launch_x<R>(c, a, false))) return rc;
       base += jobs[j].nch;
     }
     const YZArgs<R> yz = make_yz<R>(c, mul, C, nc, spec);
-    if ((rc = launch_y<R>(c, yz, false))) return rc;
+    if (!plane && (rc = launch_y<R>(c, yz, false))) return rc;
     if ((rc = launch_z<R>(c, yz))) return rc;
-    if ((rc = launch_y<R>(c, yz, true))) return rc;
+    if (!plane && (rc = launch_y<R>(c, yz, true))) return rc;
     base = 0;
     for (int j = 0; j < njobs; ++j) {
       XArgs<R> a = make_xargs<R>(c, jobs[j], base, nc, Xc, Yc, ld, spec, cstride, lt_override);
-      if ((rc = launch_x<R>(c, a, true))) return rc;
+      if ((rc = plane ? launch_plane_x<R>(c, a, true) : launch_x<R>(c, a, true))) return rc;
       base += jobs[j].nch;
     }
   }
@@ -810,9 +780,16 @@ static int bench_pass_t(bdgx_ctx *c, int pass, uint64_t nbatch, int iters, float
   const double specb = (double)sizeof(cx<R>) * (double)g.Kk * (double)nbatch;
   const double realb = (double)es * (double)g.S * (double)nbatch;
   switch (pass) {
-    case BDGX_PASS_XFWD: *bytes = realb + specb; break;
-    case BDGX_PASS_XINV: *bytes = specb + realb; break;
+    case BDGX_PASS_XFWD:
+    case BDGX_PASS_PLANE_FWD: *bytes = realb + specb; break;
+    case BDGX_PASS_XINV:
+    case BDGX_PASS_PLANE_INV: *bytes = specb + realb; break;
     default: *bytes = 2.0 * specb; break;
+  }
+  if ((pass == BDGX_PASS_PLANE_FWD || pass == BDGX_PASS_PLANE_INV) && !plane_supported<R>(c)) {
+    (void)hipFree(X);
+    (void)hipFree(Y);
+    return BDGX_EINVAL;
   }
   auto one = [&]() -> int {
     switch (pass) {
@@ -821,6 +798,8 @@ static int bench_pass_t(bdgx_ctx *c, int pass, uint64_t nbatch, int iters, float
       case BDGX_PASS_Z: return launch_z<R>(c, yz);
       case BDGX_PASS_YINV: return launch_y<R>(c, yz, true);
       case BDGX_PASS_XINV: return launch_x<R>(c, a, true);
+      case BDGX_PASS_PLANE_FWD: return launch_plane_x<R>(c, a, false);
+      case BDGX_PASS_PLANE_INV: return launch_plane_x<R>(c, a, true);
       default: return BDGX_EINVAL;
     }
   };

@@ -1,0 +1,426 @@
+// Whole-plane x/y transforms for real vectors on gfx950.
+//
+// For grids whose (y, kx) spectrum plane fits in LDS (Nx = Ny = 32, 64, 128
+// in FP32 and FP64: 137 KB at 128^2 FP64 of the 160 KB per CU), one
+// workgroup owns one z-plane of one column:
+//
+//   forward  read the real plane (contiguous, coalesced) once into registers;
+//            per channel: pre-weight, x transform of the row pairs packed as
+//            complex rows (two-for-one), untangle into the Nx/2+1 half
+//            spectrum, y transform, write the spectrum plane (contiguous);
+//   inverse  per channel: read the spectrum plane, y inverse, repack row
+//            pairs, x inverse, fold the real rows into the output kept in
+//            registers; then add lt * x and write the output plane.
+//
+// This replaces the x pass and the y pass of each direction (five launches
+// per transform pair become three with the fused z pass), and every HBM
+// access of these two kernels is a contiguous plane.  The transforms run
+// in place (decimation in frequency forward, in time inverse), so a single
+// LDS plane suffices.  All sizes are compile-time constants.
+#include <string.h>
+
+#include "bdgx_internal.hpp"
+#include "fft_core.hpp"
+#include "fft_fixed.hpp"
+#include "plane.hpp"
+
+namespace bdgx {
+
+// In-place mixed-radix transforms over NP pencils, element i of pencil p at
+// buf[p*PST + i*EST].  Decimation in frequency (DIF) takes natural order and
+// leaves X[k] at position dif_pos<N>(k) (digits of k in the plan's radices,
+// first radix least significant, reversed); decimation in time (DIT) takes
+// that order and returns natural order.  Every butterfly reads and writes the
+// same RAD positions, so a stage needs no staging buffer and only one barrier;
+// each thread keeps one butterfly in registers at a time.  PFAST maps
+// consecutive threads to consecutive pencils (pencils adjacent in memory),
+// otherwise to consecutive butterflies of one pencil.
+template <int N>
+__device__ __forceinline__ int dif_pos(int k) {
+  constexpr FixedPlan pl = make_fixed_plan(N);
+  int p = 0, span = N;
+#pragma unroll
+  for (int s = 0; s < pl.nst; ++s) {
+    span /= pl.rad[s];
+    p += (k % pl.rad[s]) * span;
+    k /= pl.rad[s];
+  }
+  return p;
+}
+
+template <typename R, bool INV, bool DIT, int N, int RAD, int L, int NP, int PST, int EST,
+          int T, bool PFAST>
+__device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw) {
+  constexpr int LR = L / RAD;
+  constexpr int NBF = N / RAD;
+  constexpr int TOTAL = NP * NBF;
+  constexpr int TSTEP = N / L;
+  for (int t = threadIdx.x; t < TOTAL; t += T) {
+    const int p = PFAST ? t % NP : t / NBF;
+    const int u = PFAST ? t / NP : t % NBF;
+    const int blk = u / LR, m = u % LR;
+    cx<R> *base = buf + p * PST + (blk * L + m) * EST;
+    cx<R> v[RAD];
+#pragma unroll
+    for (int j = 0; j < RAD; ++j) v[j] = base[j * LR * EST];
+    if (DIT && LR > 1 && m) {
+#pragma unroll
+      for (int j = 1; j < RAD; ++j) {
+        cx<R> w = tw[j * m * TSTEP];
+        if (INV) w.im = -w.im;
+        v[j] = cmul(v[j], w);
+      }
+    }
+    Bfly<R, INV, RAD>::run(v, tw, N);
+    if (!DIT && LR > 1 && m) {
+#pragma unroll
+      for (int q = 1; q < RAD; ++q) {
+        cx<R> w = tw[q * m * TSTEP];
+        if (INV) w.im = -w.im;
+        v[q] = cmul(v[q], w);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RAD; ++q) base[q * LR * EST] = v[q];
+  }
+  __syncthreads();
+}
+
+// DIF: stages S = 0 .. nst-1 with block length L = N / ns[S]
+template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int S>
+__device__ __forceinline__ void dif_fft(cx<R> *buf, const cx<R> *tw) {
+  constexpr FixedPlan pl = make_fixed_plan(N);
+  if constexpr (S < pl.nst) {
+    ip_stage<R, INV, false, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST>(buf, tw);
+    dif_fft<R, INV, N, NP, PST, EST, T, PFAST, S + 1>(buf, tw);
+  }
+}
+
+// DIT: the same stages in reverse order (S = nst-1 .. 0)
+template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int S>
+__device__ __forceinline__ void dit_fft(cx<R> *buf, const cx<R> *tw) {
+  constexpr FixedPlan pl = make_fixed_plan(N);
+  if constexpr (S >= 0) {
+    ip_stage<R, INV, true, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST>(buf, tw);
+    dit_fft<R, INV, N, NP, PST, EST, T, PFAST, S - 1>(buf, tw);
+  }
+}
+
+template <int N>
+constexpr int last_stage() {
+  return make_fixed_plan(N).nst - 1;
+}
+
+// rows handled per round of the in-place untangle / repack: the largest
+// power of two that keeps at most `per` staged values per thread
+template <int N, int T, int per>
+constexpr int group_rows(int row_len) {
+  int p = 1;
+  while (2 * p <= N / 2 && 2 * p * row_len <= per * T) p *= 2;
+  return p;
+}
+
+template <typename R, int N, int T>
+__global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
+  constexpr int NK = N / 2 + 1;
+  __shared__ cx<R> twx[N];
+  __shared__ cx<R> twy[N];
+  __shared__ cx<R> buf[N * NK];
+  const int nz = a.g.nz;
+  const int z = blockIdx.x % nz;
+  const int col = blockIdx.x / nz;
+  for (int i = threadIdx.x; i < N; i += T) {
+    twx[i] = a.tw[i];
+    twy[i] = a.tw_y[i];
+  }
+  const long long plane0 = (long long)z * N * N;
+  const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S + plane0;
+  for (int c = 0; c < a.nch; ++c) {
+    const int pre = a.pre[c];
+    const R *pw = a.prew[c] ? a.prew[c] + plane0 : nullptr;
+    // rows 2q, 2q+1 of the weighted plane as one complex row q (natural order)
+    for (int g = threadIdx.x; g < N * N; g += T) {
+      R w = xp[g];
+      if (pre == PRE_REAL || pre == PRE_CONJ_PSI)
+        w *= pw[g];
+      else if (pre == PRE_ABS2)
+        w *= w;
+      const int y = g / N, i = g % N;
+      reinterpret_cast<R *>(&buf[(y >> 1) * N + i])[y & 1] = w;
+    }
+    __syncthreads();
+    dif_fft<R, false, N, N / 2, N, 1, T, false, 0>(buf, twx);
+    // untangle Z_q (digit-reversed in k) into the half spectra of rows 2q,
+    // 2q+1 (natural kx, row stride NK); rounds of G rows from the top down,
+    // so that no row is overwritten before it is read
+    constexpr int G = group_rows<N, T, 5>(NK);
+    constexpr int NU = G * NK;
+    constexpr int BU = (NU + T - 1) / T;
+    for (int q0 = N / 2 - G; q0 >= 0; q0 -= G) {
+      cx<R> ua[BU], ub[BU];
+#pragma unroll
+      for (int b = 0; b < BU; ++b) {
+        const int t = threadIdx.x + b * T;
+        if (NU % T == 0 || t < NU) {
+          const int q = q0 + t / NK, k = t % NK;
+          const cx<R> zk = buf[q * N + dif_pos<N>(k)];
+          const cx<R> zn = buf[q * N + dif_pos<N>(k == 0 ? 0 : N - k)];
+          ua[b] = cmk<R>(R(0.5) * (zk.re + zn.re), R(0.5) * (zk.im - zn.im));
+          ub[b] = cmk<R>(R(0.5) * (zk.im + zn.im), R(-0.5) * (zk.re - zn.re));
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < BU; ++b) {
+        const int t = threadIdx.x + b * T;
+        if (NU % T == 0 || t < NU) {
+          const int q = q0 + t / NK, k = t % NK;
+          buf[(2 * q) * NK + k] = ua[b];
+          buf[(2 * q + 1) * NK + k] = ub[b];
+        }
+      }
+      __syncthreads();
+    }
+    // y transform over the NK adjacent pencils; rows come out digit-reversed
+    dif_fft<R, false, N, NK, 1, NK, T, true, 0>(buf, twy);
+    cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride + (long long)col * a.g.Kk +
+                (long long)z * N * NK;
+    for (int e = threadIdx.x; e < N * NK; e += T) {
+      const int y = e / NK, k = e % NK;
+      sp[e] = buf[dif_pos<N>(y) * NK + k];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R, int N, int T>
+__global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
+  constexpr int NK = N / 2 + 1;
+  constexpr int EPT = (N * N + T - 1) / T;
+  __shared__ cx<R> twx[N];
+  __shared__ cx<R> twy[N];
+  __shared__ cx<R> buf[N * NK];
+  const int nz = a.g.nz;
+  const int z = blockIdx.x % nz;
+  const int col = blockIdx.x / nz;
+  for (int i = threadIdx.x; i < N; i += T) {
+    twx[i] = a.tw[i];
+    twy[i] = a.tw_y[i];
+  }
+  const long long plane0 = (long long)z * N * N;
+  R acc[EPT];
+#pragma unroll
+  for (int b = 0; b < EPT; ++b) acc[b] = R(0);
+  for (int c = 0; c < a.nch; ++c) {
+    const cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride +
+                      (long long)col * a.g.Kk + (long long)z * N * NK;
+    // rows into digit-reversed order for the DIT y inverse
+    for (int e = threadIdx.x; e < N * NK; e += T) {
+      const int y = e / NK, k = e % NK;
+      buf[dif_pos<N>(y) * NK + k] = sp[e];
+    }
+    __syncthreads();
+    dit_fft<R, true, N, NK, 1, NK, T, true, last_stage<N>()>(buf, twy);
+    // repack: complex row q = spectrum(row 2q) + i spectrum(row 2q+1),
+    // extended to all kx by Hermitian symmetry, digit-reversed in x; rounds
+    // of G rows from the bottom up
+    constexpr int G = group_rows<N, T, 8>(N);
+    constexpr int NQ = G * N;
+    constexpr int BQ = (NQ + T - 1) / T;
+    for (int q0 = 0; q0 < N / 2; q0 += G) {
+      cx<R> zq[BQ];
+#pragma unroll
+      for (int b = 0; b < BQ; ++b) {
+        const int t = threadIdx.x + b * T;
+        if (NQ % T == 0 || t < NQ) {
+          const int q = q0 + t / N, k = t % N;
+          const bool cj = k >= NK;
+          const int kk = cj ? N - k : k;
+          cx<R> A0 = buf[(2 * q) * NK + kk], A1 = buf[(2 * q + 1) * NK + kk];
+          if (cj) {
+            A0.im = -A0.im;
+            A1.im = -A1.im;
+          }
+          if (kk == 0 || kk == N / 2) {
+            A0.im = 0;
+            A1.im = 0;
+          }
+          zq[b] = cmk<R>(A0.re - A1.im, A0.im + A1.re);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < BQ; ++b) {
+        const int t = threadIdx.x + b * T;
+        if (NQ % T == 0 || t < NQ) {
+          const int q = q0 + t / N, k = t % N;
+          buf[q * N + dif_pos<N>(k)] = zq[b];
+        }
+      }
+      __syncthreads();
+    }
+    dit_fft<R, true, N, N / 2, N, 1, T, false, last_stage<N>()>(buf, twx);
+    const int post = a.post[c];
+    const R ps = a.posts[c];
+    const R *pw = a.postw[c] ? a.postw[c] + plane0 : nullptr;
+#pragma unroll
+    for (int b = 0; b < EPT; ++b) {
+      const int g = threadIdx.x + b * T;
+      if (N * N % T == 0 || g < N * N) {
+        const int y = g / N, i = g % N;
+        const R *zv = reinterpret_cast<const R *>(&buf[(y >> 1) * N + i]);
+        R v = ps * zv[y & 1];
+        if (post != POST_ONE) v *= pw[g];
+        acc[b] += v;
+      }
+    }
+    __syncthreads();
+  }
+  const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S + plane0;
+  R *yp = a.Y + (long long)col * a.ld + (long long)a.out_half * a.g.S + plane0;
+  const R *lt = a.lt ? a.lt + plane0 : nullptr;
+#pragma unroll
+  for (int b = 0; b < EPT; ++b) {
+    const int g = threadIdx.x + b * T;
+    if (N * N % T == 0 || g < N * N) {
+      R out = acc[b];
+      if (lt) out += lt[g] * xp[g];
+      yp[g] = out;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused z pass with compile-time Nz: P adjacent kx pencils per workgroup held
+// element-major in LDS (element z of pencil p at z*P + p).  The first forward
+// stage reads its butterfly inputs straight from the spectrum in HBM (P
+// adjacent kx per z: one contiguous segment), the channel's k-space factor is
+// applied as the last forward stage stores, and the last inverse stage writes
+// straight back to HBM.
+template <typename R>
+struct ZMul {
+  const YZArgs<R> *a;
+  int code, kx0, y;
+  __device__ __forceinline__ cx<R> operator()(cx<R> v, int z, int p) const {
+    const int kx = kx0 + p;
+    if (kx >= a->g.n0k) return v;
+    const R k2 = a->kx2[kx] + a->ky2[y] + a->kz2[z];
+    R f;
+    switch (code) {
+      case MUL_KIN:
+        f = k2 * (R(0.5) * a->invS);
+        break;
+      case MUL_DIP:
+        f = a->U[((long long)z * a->g.ny + y) * a->g.n0k + kx] * a->invS;
+        break;
+      case MUL_PRE: {
+        R d = a->mu + R(0.5) * k2;
+        if (fabs((double)d) < 1e-8) d = R(1e-8);
+        f = a->invS / d;
+        break;
+      }
+      case MUL_ITPKIN:
+        f = R(exp(-(double)a->dt * 0.5 * (double)k2)) * a->invS;
+        break;
+      default:
+        f = a->invS;
+    }
+    return cscale(v, f);
+  }
+};
+
+template <typename R, int NZ, int P, int T>
+__global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
+  __shared__ cx<R> tw[NZ];
+  __shared__ cx<R> bufA[NZ * P];
+  __shared__ cx<R> bufB[NZ * P];
+  const int ny = a.g.ny, n0k = a.g.n0k;
+  const int ntile = (n0k + P - 1) / P;
+  const int t = blockIdx.x % ntile;
+  const int rest = blockIdx.x / ntile;
+  const int y = rest % ny;
+  const int b = rest / ny;
+  const int kx0 = t * P;
+  for (int i = threadIdx.x; i < NZ; i += T) tw[i] = a.tw[i];
+  __syncthreads();
+  cx<R> *base = a.spec + (long long)b * a.g.Kk + (long long)y * n0k + kx0;
+  const long long zs = (long long)n0k * ny;
+  const int valid = n0k - kx0 < P ? n0k - kx0 : P;
+  GlobIO<R> g{base, zs, valid};
+  ZMul<R> mul{&a, a.mul[b / a.ncols], kx0, y};
+  constexpr int nst = make_fixed_plan(NZ).nst;
+  cx<R> *F = (nst % 2 == 1) ? bufA : bufB;  // last LDS buffer of the forward
+  cx<R> *O = (F == bufA) ? bufB : bufA;
+  if constexpr (nst == 1) {
+    fixed_stages<R, false, NZ, P, T, 0>(g, LdsIO<R>{bufA}, bufA, bufB, tw, mul);
+    fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{bufA}, g, bufB, bufA, tw, NoMul{});
+  } else {
+    // forward: global -> A -> B ... ; the last stage stores into the buffer
+    // it does not read
+    fixed_stages<R, false, NZ, P, T, 0>(g, LdsIO<R>{F}, bufA, bufB, tw, mul);
+    // inverse: F -> O -> F ... -> global
+    fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{F}, g, O, F, tw, NoMul{});
+  }
+}
+
+template <typename R>
+bool zpass_fixed_supported(const bdgx_ctx *c) {
+  const int nz = c->geo.nz;
+  return nz == 32 || nz == 64 || nz == 128;
+}
+
+template <typename R>
+int launch_zpass_fixed(bdgx_ctx *c, const YZArgs<R> &z) {
+  constexpr int P = sizeof(R) == 8 ? 8 : 16;
+  const Geo &g = c->geo;
+  const int ntile = (g.n0k + P - 1) / P;
+  const dim3 grid((unsigned)(ntile * g.ny * z.nb));
+  switch (g.nz) {
+    case 32: hipLaunchKernelGGL((k_zpass_fixed<R, 32, P, 128>), grid, dim3(128), 0, c->stream, z); break;
+    case 64: hipLaunchKernelGGL((k_zpass_fixed<R, 64, P, 128>), grid, dim3(128), 0, c->stream, z); break;
+    case 128: hipLaunchKernelGGL((k_zpass_fixed<R, 128, P, 256>), grid, dim3(256), 0, c->stream, z); break;
+    default: return BDGX_EINVAL;
+  }
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
+template bool zpass_fixed_supported<float>(const bdgx_ctx *);
+template bool zpass_fixed_supported<double>(const bdgx_ctx *);
+template int launch_zpass_fixed<float>(bdgx_ctx *, const YZArgs<float> &);
+template int launch_zpass_fixed<double>(bdgx_ctx *, const YZArgs<double> &);
+
+template <typename R, int N, int T>
+static void launch_plane(const XArgs<R> &a, bool inverse, hipStream_t s) {
+  const unsigned grid = (unsigned)(a.g.nz * a.ncols);
+  if (inverse)
+    hipLaunchKernelGGL((k_plane_inv_real<R, N, T>), dim3(grid), dim3(T), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_plane_fwd_real<R, N, T>), dim3(grid), dim3(T), 0, s, a);
+}
+
+template <typename R>
+bool plane_supported(const bdgx_ctx *c) {
+  const Geo &g = c->geo;
+  if (c->cplx || c->dim != 3 || g.nx != g.ny) return false;
+  return g.nx == 32 || g.nx == 64 || g.nx == 128;
+}
+
+template <typename R>
+int launch_plane_x(bdgx_ctx *c, const XArgs<R> &a, bool inverse) {
+  switch (c->geo.nx) {
+    case 32: launch_plane<R, 32, 256>(a, inverse, c->stream); break;
+    case 64: launch_plane<R, 64, 512>(a, inverse, c->stream); break;
+    case 128: launch_plane<R, 128, 512>(a, inverse, c->stream); break;
+    default: return BDGX_EINVAL;
+  }
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
+template bool plane_supported<float>(const bdgx_ctx *);
+template bool plane_supported<double>(const bdgx_ctx *);
+template int launch_plane_x<float>(bdgx_ctx *, const XArgs<float> &, bool);
+template int launch_plane_x<double>(bdgx_ctx *, const XArgs<double> &, bool);
+
+}  // namespace bdgx

@@ -146,9 +146,11 @@ uint64_t bdgx_workspace_bytes(const bdgx_ctx *ctx);
  * batch) between HIP events on the ctx stream and reports the mean launch
  * time and the pass's algorithmic HBM bytes per launch.
  *   pass: 0 x-forward (r2c/c2c), 1 y-forward, 2 z fused (fwd, multiply, inv),
- *         3 y-inverse, 4 x-inverse + combine. */
+ *         3 y-inverse, 4 x-inverse + combine, 5 whole-plane x+y forward,
+ *         6 whole-plane y+x inverse + combine (5 and 6 only for the grids
+ *         that use them: real psi0, Nx = Ny in {32, 64, 128}; else EINVAL). */
 enum { BDGX_PASS_XFWD = 0, BDGX_PASS_YFWD = 1, BDGX_PASS_Z = 2, BDGX_PASS_YINV = 3,
-       BDGX_PASS_XINV = 4 };
+       BDGX_PASS_XINV = 4, BDGX_PASS_PLANE_FWD = 5, BDGX_PASS_PLANE_INV = 6 };
 int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iters,
                     float *ms_per_launch, double *bytes_per_launch);
 

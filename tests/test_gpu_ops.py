@@ -14,6 +14,9 @@ GRIDS = [
     ((32, 16, 24), (16.0, 12.0, 14.0)),
     ((64,), (2 * np.pi,)),
     ((16, 12), (8.0, 6.0)),
+    ((32, 32, 12), (16.0, 16.0, 10.0)),    # whole-plane x/y kernels (real)
+    ((64, 64, 6), (16.0, 16.0, 6.0)),
+    ((128, 128, 4), (24.0, 24.0, 4.0)),
 ]
 OPS_HALF = ["kinetic", "K", "M", "precondK", "precondM", "dipolar_conv"]
 OPS_FULL = ["Lrep", "Swap", "precondLrep"]
@@ -41,8 +44,9 @@ def test_operator_parity(grid, cplx, prec):
 
 
 @pytest.mark.parametrize("cplx", [False, True])
-def test_dipolar_kernel_and_meanfield(cplx):
-    N, L = (16, 12, 10), (12.0, 10.0, 9.0)
+@pytest.mark.parametrize("N", [(16, 12, 10), (32, 32, 10)], ids=["pencil", "plane"])
+def test_dipolar_kernel_and_meanfield(cplx, N):
+    L = (12.0, 10.0, 9.0)
     c, f = physics(N, L, cplx, dipolar=True)
     d = device_twin(N, L, cplx, f)
     assert relerr(d.dipolar_kernel(), c.dipolar_kernel()) < 1e-13
