@@ -510,6 +510,7 @@ int bdg_solve(bdg_t *b) {
   cfg.img_tol = b->img_tol;
   cfg.refresh_every = (int)b->refresh_every;
   cfg.mixed_min_cols = (int)(m / 4 > 1 ? m / 4 : 1);
+  cfg.k_equals_m = !b->dipolar && memcmp(b->ltK, b->ltM, sizeof(double) * S) == 0;
   lobpcg_result r;
   memset(&r, 0, sizeof(r));
   r.w2 = (double *)xcalloc(m, sizeof(double));

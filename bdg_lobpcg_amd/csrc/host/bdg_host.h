@@ -110,6 +110,7 @@ typedef struct {
   double img_tol;        /* mixed: FP32 images of W while the worst residual > this */
   int refresh_every;     /* mixed: FP64 images of X every n iterations meanwhile    */
   int mixed_min_cols;    /* FP32 directions only when at least this many active   */
+  int k_equals_m;        /* K == M: standard form K^2 v = w^2 v (B = I)            */
 } lobpcg_cfg;
 
 typedef struct {

@@ -25,6 +25,11 @@
  * numerically dependent the P block is left out of that Rayleigh-Ritz; when
  * [X, W] still is, W falls back to the plain residual.
  *
+ * When K == M (no interactions, no dipolar term) M is as singular as K on
+ * the Goldstone direction and the product form divides rounding noise by a
+ * vanishing M-norm; that case runs the same iteration on the standard form
+ * K^2 v = w^2 v (B = I): images K V and K^2 V, Grams V^H V and V^H K^2 V.
+ *
  * Mixed precision (w_prec = FP32): the basis, its Grams, the residuals and
  * the Rayleigh-Ritz stay FP64.  T_K and T_M always run in FP32 (W is only a
  * search direction, so its rounding changes which direction is searched, not
@@ -213,20 +218,24 @@ typedef struct {
   double *V, *M, *A; /* k x k each (complex interleaved when w = 2) */
 } gram3;
 
-/* full device Grams on V[:, 0:k] */
-static int grams_full(bdgx_ctx *ctx, int prec, const blk3 *b, int k, uint64_t S, int w,
+/* full device Grams on V[:, 0:k].  Product form: V^H V, V^H MV, MV^H KMV.
+ * Standard form (std != 0, the "M" images hold K V and the "KM" images K^2 V):
+ * V^H V, V^H V, V^H K^2 V. */
+static int grams_full(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, uint64_t S, int w,
                       double *dG, gram3 *h) {
   const size_t gsz = (size_t)k * k * w;
   int rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, b->V, S, (uint64_t)k, S, 0, dG);
-  if (!rc) rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, b->MV, S, (uint64_t)k, S, 0, dG + gsz);
+  if (!rc && !std)
+    rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, b->MV, S, (uint64_t)k, S, 0, dG + gsz);
   if (!rc)
-    rc = bdgx_gram(ctx, prec, b->MV, S, (uint64_t)k, b->KMV, S, (uint64_t)k, S, 0, dG + 2 * gsz);
+    rc = bdgx_gram(ctx, prec, std ? b->V : b->MV, S, (uint64_t)k, b->KMV, S, (uint64_t)k, S, 0,
+                   dG + 2 * gsz);
   if (rc) return rc;
   double *tmp = (double *)malloc(sizeof(double) * 3 * gsz);
   rc = bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * 3 * gsz);
   if (!rc) {
     memcpy(h->V, tmp, sizeof(double) * gsz);
-    memcpy(h->M, tmp + gsz, sizeof(double) * gsz);
+    memcpy(h->M, std ? tmp : tmp + gsz, sizeof(double) * gsz);
     memcpy(h->A, tmp + 2 * gsz, sizeof(double) * gsz);
   }
   free(tmp);
@@ -235,21 +244,22 @@ static int grams_full(bdgx_ctx *ctx, int prec, const blk3 *b, int k, uint64_t S,
 
 /* device Grams of V[:, 0:k] against the columns [c0, c0+nc) of V; written
  * into columns c0.. of the full k x k sets, and mirrored into the rows. */
-static int grams_cols(bdgx_ctx *ctx, int prec, const blk3 *b, int k, int c0, int nc, uint64_t S,
-                      int w, size_t es, double *dG, gram3 *h) {
+static int grams_cols(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, int c0, int nc,
+                      uint64_t S, int w, size_t es, double *dG, gram3 *h) {
   const size_t csz = (size_t)k * nc * w;
   int rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, colp(b->V, (uint64_t)c0, S, es), S,
                      (uint64_t)nc, S, 0, dG);
-  if (!rc)
+  if (!rc && !std)
     rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, colp(b->MV, (uint64_t)c0, S, es), S,
                    (uint64_t)nc, S, 0, dG + csz);
   if (!rc)
-    rc = bdgx_gram(ctx, prec, b->MV, S, (uint64_t)k, colp(b->KMV, (uint64_t)c0, S, es), S,
-                   (uint64_t)nc, S, 0, dG + 2 * csz);
+    rc = bdgx_gram(ctx, prec, std ? b->V : b->MV, S, (uint64_t)k,
+                   colp(b->KMV, (uint64_t)c0, S, es), S, (uint64_t)nc, S, 0, dG + 2 * csz);
   if (rc) return rc;
   double *tmp = (double *)malloc(sizeof(double) * 3 * csz);
   rc = bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * 3 * csz);
   if (!rc) {
+    if (std) memcpy(tmp + csz, tmp, sizeof(double) * csz);
     double *dst[3] = {h->V, h->M, h->A};
     for (int g = 0; g < 3; ++g)
       for (int j = 0; j < nc; ++j)
@@ -306,7 +316,8 @@ static int eval_residuals(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const blk3 *A, c
 
 /* re-apply M then K (FP64) to the X block of A */
 static int refresh_images(bdgx_ctx *ctx, const lobpcg_cfg *cfg, blk3 *A, lobpcg_result *res) {
-  int rc = bdgx_apply(ctx, BDGX_OP_M, BDGX_FP64, A->V, A->MV, (uint64_t)cfg->m, cfg->S);
+  const int opM = cfg->k_equals_m ? BDGX_OP_K : BDGX_OP_M;
+  int rc = bdgx_apply(ctx, opM, BDGX_FP64, A->V, A->MV, (uint64_t)cfg->m, cfg->S);
   if (!rc) rc = bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, A->MV, A->KMV, (uint64_t)cfg->m, cfg->S);
   res->applies += (uint64_t)cfg->m;
   res->refreshes++;
@@ -325,6 +336,10 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   const int m = cfg->m, nev = cfg->nev, cplx = cfg->cplx;
   const int w = cplx ? 2 : 1;
   const int mixed = cfg->w_prec == BDGX_FP32;
+  /* standard form when K == M: (K^2) v = w^2 v with B = I; the "M" images
+   * then hold K V */
+  const int std = cfg->k_equals_m;
+  const int opM = std ? BDGX_OP_K : BDGX_OP_M;
   const size_t es = elem_bytes(BDGX_FP64, cplx), es32 = elem_bytes(BDGX_FP32, cplx);
   const int kmax = 3 * m;
   int rc = 0;
@@ -385,10 +400,10 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   /* start: basis [X0], images by M then K (FP64), Rayleigh-Ritz, X = X0 Y */
   {
     TRY(bdgx_convert(ctx, BDGX_FP64, dX0, S, BDGX_FP64, A.V, S, S, (uint64_t)m));
-    TRY(bdgx_apply(ctx, BDGX_OP_M, BDGX_FP64, A.V, A.MV, (uint64_t)m, S));
+    TRY(bdgx_apply(ctx, opM, BDGX_FP64, A.V, A.MV, (uint64_t)m, S));
     TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, A.MV, A.KMV, (uint64_t)m, S));
     res->applies += (uint64_t)m;
-    TRY(grams_full(ctx, BDGX_FP64, &A, m, S, w, dG, &G));
+    TRY(grams_full(ctx, BDGX_FP64, std, &A, m, S, w, dG, &G));
     const double t0 = bdg_wtime();
     const int got = dn_rayleigh_ritz(m, m, cplx, G.V, G.M, G.A, w2, hY, zf);
     trr += bdg_wtime() - t0;
@@ -478,7 +493,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       TRY(bdgx_convert(ctx, BDGX_FP32, f1, S, BDGX_FP64, wdst, S, S, (uint64_t)nact));
     }
     if (!img32) {
-      TRY(bdgx_apply(ctx, BDGX_OP_M, BDGX_FP64, wdst, colp(A.MV, (uint64_t)wc, S, es),
+      TRY(bdgx_apply(ctx, opM, BDGX_FP64, wdst, colp(A.MV, (uint64_t)wc, S, es),
                      (uint64_t)nact, S));
       TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, colp(A.MV, (uint64_t)wc, S, es),
                      colp(A.KMV, (uint64_t)wc, S, es), (uint64_t)nact, S));
@@ -508,7 +523,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
         TRY(bdgx_convert(ctx, BDGX_FP64, other, S, BDGX_FP64, wdst, S, S, (uint64_t)nact));
         TRY(bdgx_convert(ctx, BDGX_FP64, other, S, BDGX_FP32, f1, S, S, (uint64_t)nact));
       }
-      TRY(bdgx_apply(ctx, BDGX_OP_M, BDGX_FP32, f1, f2, (uint64_t)nact, S));
+      TRY(bdgx_apply(ctx, opM, BDGX_FP32, f1, f2, (uint64_t)nact, S));
       TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP32, f2, f3, (uint64_t)nact, S));
       TRY(bdgx_convert(ctx, BDGX_FP32, f2, S, BDGX_FP64, colp(A.MV, (uint64_t)wc, S, es), S, S,
                        (uint64_t)nact));
@@ -522,9 +537,9 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     /* Gram set of [X | P | W] */
     if (full) {
       since_full = 0;
-      TRY(grams_full(ctx, BDGX_FP64, &A, k, S, w, dG, &G));
+      TRY(grams_full(ctx, BDGX_FP64, std, &A, k, S, w, dG, &G));
     } else {
-      TRY(grams_cols(ctx, BDGX_FP64, &A, k, wc, nact, S, w, es, dG, &G));
+      TRY(grams_cols(ctx, BDGX_FP64, std, &A, k, wc, nact, S, w, es, dG, &G));
     }
     /* Rayleigh-Ritz on a column selection: all, or without P when [X P W]
      * is numerically dependent */
@@ -545,13 +560,13 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       trr += bdg_wtime() - t0;
       TRY(bdgx_convert(ctx, BDGX_FP64, Rsrc, S, BDGX_FP64, colp(A.V, (uint64_t)wc, S, es), S, S,
                        (uint64_t)nact));
-      TRY(bdgx_apply(ctx, BDGX_OP_M, BDGX_FP64, colp(A.V, (uint64_t)wc, S, es),
+      TRY(bdgx_apply(ctx, opM, BDGX_FP64, colp(A.V, (uint64_t)wc, S, es),
                      colp(A.MV, (uint64_t)wc, S, es), (uint64_t)nact, S));
       TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, colp(A.MV, (uint64_t)wc, S, es),
                      colp(A.KMV, (uint64_t)wc, S, es), (uint64_t)nact, S));
       res->applies += (uint64_t)nact;
       res->precond_fallbacks++;
-      TRY(grams_full(ctx, BDGX_FP64, &A, k, S, w, dG, &G));
+      TRY(grams_full(ctx, BDGX_FP64, std, &A, k, S, w, dG, &G));
       since_full = 0;
       ks = 0;
       for (int i = 0; i < m; ++i) sel[ks++] = i;

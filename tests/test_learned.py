@@ -1,0 +1,60 @@
+"""Learned preconditioner: the numpy restatement (oracle/learned_ref.py) pinned
+on the CPU, and (GPU) the device forward and parameter gradient against it."""
+import numpy as np
+import pytest
+
+from oracle import learned_ref as LR
+from tests.common import physics, rand_block, relerr
+
+
+def _small(N=(8, 6, 4), L=(6.0, 5.0, 4.0), seed=0):
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
+    net = LR.LearnedRef(N, [c.kx2(d) for d in range(3)], f["pK"], f["pM"], f["wf"], f["mu"])
+    return c, f, net
+
+
+def test_zero_theta_is_the_analytic_preconditioner():
+    """With v = 0 and alpha = beta = kappa = 0 the network is T_M T_K exactly
+    (reference src/precond_impl.inc:21-111 as restated by the C oracle)."""
+    c, f, net = _small()
+    th = LR.init_theta(seed=1)  # random stencil weights act only through v = 0
+    R = rand_block(c.size, 3, False, seed=4)
+    ref = c.apply("precondM", c.apply("precondK", R))
+    assert relerr(net.forward(th, R), ref) < 1e-12
+
+
+def test_gradient_matches_finite_differences():
+    c, f, net = _small()
+    rng = np.random.default_rng(7)
+    th = LR.init_theta(seed=2, wscale=0.5)
+    th[:8] = 0.3 * rng.standard_normal(8)
+    th[8:8 + 2 * LR.NKNOT] = 0.2 * rng.standard_normal(2 * LR.NKNOT)
+    o = 8 + 2 * LR.NKNOT + 27 * LR.HID
+    th[o:] = 0.3 * rng.standard_normal(2 * LR.HID)
+    R = rand_block(c.size, 2, False, seed=5)
+    dW = rand_block(c.size, 2, False, seed=6)
+    g = net.grad(th, R, dW)
+    f0 = lambda t: float(np.sum(dW * net.forward(t, R)))
+    h = 1e-5
+    idx = list(range(0, 8 + 2 * LR.NKNOT)) + list(rng.choice(np.arange(o - 27 * LR.HID, o), 20,
+                                                              replace=False)) + list(range(o, LR.NPARAM))
+    fd = np.zeros(len(idx))
+    for n, i in enumerate(idx):
+        e = np.zeros_like(th)
+        e[i] = h
+        fd[n] = (f0(th + e) - f0(th - e)) / (2 * h)
+    np.testing.assert_allclose(g[idx], fd, rtol=2e-5, atol=1e-7 * np.abs(fd).max())
+
+
+def test_cosine_loss_gradient():
+    rng = np.random.default_rng(3)
+    W, E = rng.standard_normal((50, 3)), rng.standard_normal((50, 3))
+    loss, dW = LR.cosine_loss(W, E)
+    h = 1e-6
+    for (i, j) in [(0, 0), (7, 1), (49, 2)]:
+        Wp, Wm = W.copy(), W.copy()
+        Wp[i, j] += h
+        Wm[i, j] -= h
+        fd = (LR.cosine_loss(Wp, E)[0] - LR.cosine_loss(Wm, E)[0]) / (2 * h)
+        assert abs(fd - dW[i, j]) < 1e-7
+    assert LR.cosine_loss(E * 3.0, E)[0] < 1e-14
