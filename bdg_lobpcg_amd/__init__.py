@@ -23,6 +23,8 @@ _lib = None
 OP = {"kinetic": 0, "K": 1, "M": 2, "Lrep": 3, "Swap": 4, "precondK": 5,
       "precondM": 6, "precondLrep": 7, "dipolar_conv": 8}
 FP64, FP32 = 0, 1
+LP_NKNOT, LP_HIDDEN = 16, 16
+LP_NPARAM = 8 + 2 * LP_NKNOT + 29 * LP_HIDDEN
 PREC_FP64, PREC_MIXED = 0, 1
 INIT_PLANEWAVE, INIT_WF_WEIGHTED, INIT_REUSE, INIT_CUSTOM = 0, 1, 2, 3
 GEOM_AUTO, GEOM_ELONGATED, GEOM_RING = 0, 1, 2
@@ -39,7 +41,8 @@ class BdgError(RuntimeError):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_uint64), ("column_applies", C.c_uint64),
                 ("column_applies_fp32", C.c_uint64), ("image_refreshes", C.c_uint64),
-                ("restarts", C.c_uint64), ("seconds_total", C.c_double),
+                ("restarts", C.c_uint64), ("learned_iterations", C.c_uint64),
+                ("learned_fallback_at", C.c_int), ("seconds_total", C.c_double),
                 ("seconds_rr", C.c_double), ("status", C.c_int)]
 
 
@@ -86,6 +89,9 @@ def lib():
         "bdgx_timer_end": (I, [P, C.POINTER(C.c_float)]),
         "bdgx_workspace_bytes": (U64, [P]),
         "bdgx_bench_pass": (I, [P, I, I, U64, I, C.POINTER(C.c_float), C.POINTER(D)]),
+        "bdgx_lp_set_params": (I, [P, P]),
+        "bdgx_lp_apply": (I, [P, I, P, P, U64, U64]),
+        "bdgx_lp_backward": (I, [P, P, P, U64, U64, P]),
         # public API
         "bdg_alloc": (P, [U64, P, P, I]),
         "bdg_free": (None, [C.POINTER(P)]),
@@ -121,6 +127,8 @@ def lib():
         "bdg_mu": (D, [P]),
         "bdg_set_mixed_params": (None, [P, D, U64]),
         "bdg_set_verbosity": (None, [P, I]),
+        "bdg_set_learned_preconditioner": (None, [P, P]),
+        "bdg_set_learned_fallback": (None, [P, I, D]),
         "bdg_start_block": (None, [P, P]),
         "bdg_load_wavefunction_fmt": (I, [P, C.c_char_p]),
         "bdgx_reset_fields": (I, [P]),
@@ -175,14 +183,31 @@ class DeviceContext:
         check(lib().bdgx_ctx_create(C.byref(h), len(self.N), _p(Na), _p(La), int(self.cplx)),
               "bdgx_ctx_create")
         self.h = h
-        self.size = int(lib().bdgx_ctx_size(h))
-        self.k_size = int(lib().bdgx_ctx_ksize(h))
+        self._owned = True
+        self._init_sizes()
+
+    def _init_sizes(self):
+        self.size = int(lib().bdgx_ctx_size(self.h))
+        self.k_size = int(lib().bdgx_ctx_ksize(self.h))
         self.dtype64 = np.complex128 if self.cplx else np.float64
         self.dtype32 = np.complex64 if self.cplx else np.float32
 
+    @classmethod
+    def wrap(cls, handle, N, L, complex_psi0=False):
+        """Non-owning view of an existing context (e.g. a Problem's)."""
+        self = cls.__new__(cls)
+        self.N = tuple(int(n) for n in N)
+        self.L = tuple(float(v) for v in L)
+        self.cplx = bool(complex_psi0)
+        self.h = C.c_void_p(handle)
+        self._owned = False
+        self._init_sizes()
+        return self
+
     def close(self):
         if getattr(self, "h", None):
-            lib().bdgx_ctx_destroy(self.h)
+            if getattr(self, "_owned", True):
+                lib().bdgx_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -287,6 +312,36 @@ class DeviceContext:
                                  A.shape[0], Cm.shape[1], A.shape[0]), "combine")
         return self.download(dY, (A.shape[0], Cm.shape[1]), A.dtype)
 
+    # -- learned preconditioner (FP32, real psi0) ---------------------------
+    def lp_set_params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        if th.size != LP_NPARAM:
+            raise ValueError(f"theta must have {LP_NPARAM} entries")
+        check(lib().bdgx_lp_set_params(self.h, th.ctypes.data_as(C.c_void_p)), "bdgx_lp_set_params")
+
+    def lp_apply_device(self, dR, dW, ncols):
+        check(lib().bdgx_lp_apply(self.h, FP32, dR.ptr, dW.ptr, int(ncols), self.size),
+              "bdgx_lp_apply")
+
+    def lp_backward_device(self, dR, dG, ncols) -> np.ndarray:
+        g = np.zeros(LP_NPARAM, dtype=np.float32)
+        check(lib().bdgx_lp_backward(self.h, dR.ptr, dG.ptr, int(ncols), self.size,
+                                     g.ctypes.data_as(C.c_void_p)), "bdgx_lp_backward")
+        return g
+
+    def lp_apply(self, R: np.ndarray) -> np.ndarray:
+        R = np.asfortranarray(R, dtype=np.float32)
+        dR = self.upload(R)
+        dW = DeviceBuffer(R.nbytes)
+        self.lp_apply_device(dR, dW, R.shape[1])
+        return self.download(dW, R.shape, np.float32)
+
+    def lp_backward(self, R: np.ndarray, G: np.ndarray) -> np.ndarray:
+        R = np.asfortranarray(R, dtype=np.float32)
+        G = np.asfortranarray(G, dtype=np.float32)
+        dR, dG = self.upload(R), self.upload(G)
+        return self.lp_backward_device(dR, dG, R.shape[1])
+
     def itp_groundstate(self, psi, omega, g, g_dd, direction, cutoff, dt, nsteps):
         psi = np.ascontiguousarray(psi, dtype=self.dtype64).copy()
         om = np.ascontiguousarray(omega, dtype=np.float64)
@@ -383,6 +438,19 @@ class Problem:
     def set_verbosity(self, level):
         lib().bdg_set_verbosity(self.h, int(level))
 
+    def set_learned_fallback(self, window=30, factor=0.5):
+        lib().bdg_set_learned_fallback(self.h, int(window), float(factor))
+
+    def set_learned(self, theta):
+        """Learned preconditioner parameters (LP_NPARAM floats), or None."""
+        if theta is None:
+            lib().bdg_set_learned_preconditioner(self.h, None)
+            return
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        if th.size != LP_NPARAM:
+            raise ValueError(f"theta must have {LP_NPARAM} entries")
+        lib().bdg_set_learned_preconditioner(self.h, th.ctypes.data_as(C.c_void_p))
+
     def set_init_custom_block(self, X: np.ndarray):
         """BDG_INIT_CUSTOM with a fixed n x sizeSub block (copied in the callback)."""
         Xf = np.asfortranarray(X, dtype=np.complex128 if self.cplx else np.float64)
@@ -455,3 +523,7 @@ class Problem:
 
     def device_context_handle(self):
         return lib().bdg_device_context(self.h)
+
+    def device_context(self):
+        """Non-owning DeviceContext of this problem (fields as of the last solve)."""
+        return DeviceContext.wrap(self.device_context_handle(), self.N, self.L, self.cplx)

@@ -40,7 +40,8 @@ enum PreKind { PRE_NONE = 0, PRE_REAL = 1, PRE_CONJ_PSI = 2, PRE_ABS2 = 3 };
 // coefficient applied when a channel is folded into the output (store side)
 enum PostKind { POST_ONE = 0, POST_REAL = 1, POST_PSI = 2, POST_RE = 3 };
 // k-space multipliers applied between the forward and inverse z transform
-enum MulKind { MUL_ONE = 0, MUL_KIN = 1, MUL_DIP = 2, MUL_PRE = 3, MUL_ITPKIN = 4 };
+enum MulKind { MUL_ONE = 0, MUL_KIN = 1, MUL_DIP = 2, MUL_PRE = 3, MUL_ITPKIN = 4,
+               MUL_TABK = 5, MUL_TABM = 6 };  // TAB*: learned k-space tables
 
 struct Geo {
   int nx, ny, nz, n0k;
@@ -55,6 +56,9 @@ struct Fields {
   R *U = nullptr;     // k_size dipolar kernel
   R *kx2[3] = {nullptr, nullptr, nullptr};
   cx<R> *tw[3] = {nullptr, nullptr, nullptr};  // per-axis twiddles exp(-2 pi i m/n)
+  // learned preconditioner (FP32 only): pointwise weights a, b and k-space
+  // tables g (1/S folded in) of its two spectral filters, [0] K and [1] M
+  R *lpa[2] = {nullptr, nullptr}, *lpb[2] = {nullptr, nullptr}, *lpg[2] = {nullptr, nullptr};
 };
 
 }  // namespace bdgx
@@ -81,6 +85,13 @@ struct bdgx_ctx {
   // device copy of column indices for gathers
   int *idx_buf;
   uint64_t idx_cap;
+  // learned preconditioner: stencil parameters (w, d, v), the stencil
+  // branch's pointwise scale p_K p_M, features rho, workspace
+  float *lp_conv;
+  float *lp_cs, *lp_rho;
+  void *lp_tmp;
+  size_t lp_tmp_bytes;
+  int has_lp;
 };
 
 namespace bdgx {

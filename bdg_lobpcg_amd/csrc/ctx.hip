@@ -97,7 +97,8 @@ static int upload_both(bdgx_ctx *c, double **d64, float **d32, const double *src
 
 template <typename R>
 static void free_fields(Fields<R> &f) {
-  R **arr[] = {&f.ltK, &f.ltM, &f.pK, &f.pM, &f.wf, &f.U};
+  R **arr[] = {&f.ltK, &f.ltM, &f.pK, &f.pM, &f.wf, &f.U, &f.lpa[0], &f.lpa[1],
+               &f.lpb[0], &f.lpb[1], &f.lpg[0], &f.lpg[1]};
   for (auto p : arr)
     if (*p) {
       (void)hipFree(*p);
@@ -308,6 +309,10 @@ void bdgx_ctx_destroy(bdgx_ctx *c) {
   if (c->spec) (void)hipFree(c->spec);
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->idx_buf) (void)hipFree(c->idx_buf);
+  if (c->lp_conv) (void)hipFree(c->lp_conv);
+  if (c->lp_cs) (void)hipFree(c->lp_cs);
+  if (c->lp_rho) (void)hipFree(c->lp_rho);
+  if (c->lp_tmp) (void)hipFree(c->lp_tmp);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);

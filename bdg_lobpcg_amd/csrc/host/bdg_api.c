@@ -48,6 +48,8 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
   b->precision = BDG_PREC_FP64;
   b->img_tol = 0.0;
   b->refresh_every = 20;
+  b->lp_window = 30;
+  b->lp_factor = 0.5;
   return b;
 }
 
@@ -71,6 +73,7 @@ void bdg_free(bdg_t **pb) {
   freep((void **)&b->pK);
   freep((void **)&b->pM);
   freep(&b->wf);
+  freep((void **)&b->lp_theta);
   if (b->dev) bdgx_ctx_destroy(b->dev);
   free(b);
   *pb = NULL;
@@ -456,9 +459,10 @@ static int upload_fields(bdg_t *b) {
     pKs[i] = 1.0 / sqrt(fmax(floor_v, b->ltK[i] + b->mu));
     pMs[i] = 1.0 / sqrt(fmax(floor_v, b->ltM[i] + b->mu));
   }
-  const int rc = bdgx_set_fields(b->dev, b->ltK, b->ltM, b->wf, pKs, pMs, b->mu);
+  int rc = bdgx_set_fields(b->dev, b->ltK, b->ltM, b->wf, pKs, pMs, b->mu);
   free(pKs);
   free(pMs);
+  if (!rc && b->lp_theta) rc = bdgx_lp_set_params(b->dev, b->lp_theta);
   return rc;
 }
 
@@ -511,6 +515,9 @@ int bdg_solve(bdg_t *b) {
   cfg.refresh_every = (int)b->refresh_every;
   cfg.mixed_min_cols = (int)(m / 4 > 1 ? m / 4 : 1);
   cfg.k_equals_m = !b->dipolar && memcmp(b->ltK, b->ltM, sizeof(double) * S) == 0;
+  cfg.learned = b->lp_theta != NULL && !c && !cfg.k_equals_m;
+  cfg.lp_window = b->lp_window;
+  cfg.lp_factor = b->lp_factor;
   lobpcg_result r;
   memset(&r, 0, sizeof(r));
   r.w2 = (double *)xcalloc(m, sizeof(double));
@@ -522,6 +529,8 @@ int bdg_solve(bdg_t *b) {
   b->stats.column_applies = r.applies;
   b->stats.column_applies_fp32 = r.applies_fp32;
   b->stats.image_refreshes = r.refreshes;
+  b->stats.learned_iterations = (uint64_t)r.learned_its;
+  b->stats.learned_fallback_at = r.learned_fallback_at;
   b->stats.restarts = (uint64_t)r.restarts;
   b->stats.seconds_rr = r.rr_seconds;
   if (b->verbosity > 0)
@@ -622,6 +631,17 @@ void bdg_set_mixed_params(bdg_t *b, f64 img_tol, uint64_t refresh_every) {
   b->refresh_every = refresh_every;
 }
 void bdg_set_verbosity(bdg_t *b, int level) { b->verbosity = level; }
+void bdg_set_learned_fallback(bdg_t *b, int window, f64 factor) {
+  b->lp_window = window > 0 ? window : 0;
+  b->lp_factor = factor;
+}
+void bdg_set_learned_preconditioner(bdg_t *b, const float *theta) {
+  freep((void **)&b->lp_theta);
+  if (theta) {
+    b->lp_theta = (float *)xcalloc(BDGX_LP_NPARAM, sizeof(float));
+    memcpy(b->lp_theta, theta, sizeof(float) * BDGX_LP_NPARAM);
+  }
+}
 
 f64 bdg_cb_harmonic_trap(uint64_t dim, const f64 *r, const void *param) {
   const f64 *om = (const f64 *)param;

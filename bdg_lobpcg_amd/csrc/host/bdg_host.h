@@ -65,6 +65,9 @@ struct bdg_t {
   int precision;
   double img_tol;
   uint64_t refresh_every;
+  float *lp_theta; /* learned preconditioner parameters (NULL: analytic) */
+  int lp_window;
+  double lp_factor;
   int verbosity;
   /* results */
   double *eigvals;
@@ -111,6 +114,10 @@ typedef struct {
   int refresh_every;     /* mixed: FP64 images of X every n iterations meanwhile    */
   int mixed_min_cols;    /* FP32 directions only when at least this many active   */
   int k_equals_m;        /* K == M: standard form K^2 v = w^2 v (B = I)            */
+  int learned;           /* W from the learned network (bdgx_lp_apply, FP32)      */
+  int lp_window;         /* learned: back to T_M T_K when the worst residual has  */
+  double lp_factor;      /*   not dropped below lp_factor x its value lp_window   */
+                         /*   iterations earlier                                   */
 } lobpcg_cfg;
 
 typedef struct {
@@ -119,6 +126,7 @@ typedef struct {
   double *res;     /* m */
   int iters, restarts, precond_fallbacks;
   uint64_t applies, applies_fp32, refreshes;
+  int learned_its, learned_fallback_at;
   double rr_seconds;
   int status;
 } lobpcg_result;

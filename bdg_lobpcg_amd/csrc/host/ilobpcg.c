@@ -382,12 +382,16 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   res->applies = 0;
   res->applies_fp32 = 0;
   res->refreshes = 0;
+  res->learned_its = 0;
+  res->learned_fallback_at = 0;
+  int lp_on = cfg->learned;
+  double *worst_hist = (double *)calloc((size_t)cfg->maxiter + 2, sizeof(double));
 
   TRY(alloc3(&A, (size_t)S * kmax * es));
   TRY(alloc3(&B, (size_t)S * kmax * es));
   TRY(bdgx_malloc(&dR, (size_t)S * m * es));
   TRY(bdgx_malloc(&dTmp, (size_t)S * m * es));
-  if (mixed) {
+  if (mixed || cfg->learned) {
     TRY(bdgx_malloc(&f1, (size_t)S * m * es32));
     TRY(bdgx_malloc(&f2, (size_t)S * m * es32));
     TRY(bdgx_malloc(&f3, (size_t)S * m * es32));
@@ -457,6 +461,14 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     }
     if (st.allc) break;
     if (it == cfg->maxiter) break;
+    worst_hist[it] = st.worst;
+    if (lp_on && cfg->lp_window > 0 && it > cfg->lp_window &&
+        !(st.worst < cfg->lp_factor * worst_hist[it - cfg->lp_window])) {
+      lp_on = 0;  /* the network stopped helping: analytic T_M T_K from here on */
+      res->learned_fallback_at = it;
+      if (cfg->verbosity > 0)
+        fprintf(stderr, "[bdg] it %d: learned preconditioner stalled, back to T_M T_K\n", it);
+    }
 
     /* W = T_M T_K R[:, act] at columns [m + np, m + np + nact) */
     const int wc = m + np;
@@ -481,7 +493,13 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     const int img32 =
         mixed && cfg->img_tol > 0 && nact >= cfg->mixed_min_cols && st.worst > cfg->img_tol;
     void *wdst = colp(A.V, (uint64_t)wc, S, es);
-    if (!mixed) {
+    if (lp_on) {
+      /* learned network on the residuals (FP32) */
+      TRY(bdgx_convert(ctx, BDGX_FP64, Rsrc, S, BDGX_FP32, f1, S, S, (uint64_t)nact));
+      TRY(bdgx_lp_apply(ctx, BDGX_FP32, f1, f2, (uint64_t)nact, S));
+      TRY(bdgx_convert(ctx, BDGX_FP32, f2, S, BDGX_FP64, wdst, S, S, (uint64_t)nact));
+      res->learned_its++;
+    } else if (!mixed) {
       TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_K, BDGX_FP64, Rsrc, other, (uint64_t)nact, S));
       TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_M, BDGX_FP64, other, wdst, (uint64_t)nact, S));
     } else {
@@ -657,5 +675,6 @@ done:
   free(st.resv);
   free(st.conv);
   free(st.act);
+  free(worst_hist);
   return rc;
 }

@@ -1,0 +1,501 @@
+// Learned preconditioner for the iLOBPCG search directions (FP32, real psi0).
+//
+//   N(r) = T'_M(T'_K(r)) + c sigma * sum_h v_h tanh(sum_nb w_{h,nb} r(x+nb)/sigma + d_h)
+//   T'_X v = a_X * F^-1[ g_X * F(b_X * v) ],  X in {K, M}
+//   a_X = p_X exp(alpha_X . phi),  b_X = p_X exp(beta_X . phi),  phi = (rho, rho^2)
+//   g_X(k) = exp(interp(kappa_X, |k|)) / (S * safe(mu + k^2/2))
+//
+// with sigma = rms(r) per column (N is positively homogeneous in r, like the
+// preconditioner it replaces) and c = (p_K p_M)^2 / safe(mu)^2, the low-k
+// scale of T_M T_K.  Two spectral layers that generalise the analytic
+// preconditioner T_M T_K
+// (reference src/precond_impl.inc:21-111; it is N at theta = 0 with v = 0),
+// plus a 3x3x3 stencil layer with HID hidden channels.  The spectral layers run
+// through the same FFT pipeline as every operator (pre-weight, k-space table,
+// post-weight); the stencil layer is one fused kernel that reads the 27
+// neighbours of each point once per hidden channel and never stores the
+// hidden activations.  The parameter gradient (forward recomputed, then the
+// adjoint: the filters are self-adjoint because g is real and even in k)
+// needs two more table filters, two pairs of forward spectra for the kappa
+// gradients, and reductions; everything runs on the device and the sums are
+// accumulated in FP64.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "bdgx.h"
+#include "bdgx_internal.hpp"
+#include "fft_core.hpp"
+
+namespace bdgx {
+
+int lp_filter_f32(bdgx_ctx *c, int w, int mode, const float *X, float *Y, long long ncols,
+                  long long ld);
+int lp_spectra_f32(bdgx_ctx *c, int w, const float *X2, long long ncols, long long ld2);
+
+constexpr int LPK = BDGX_LP_NKNOT;
+constexpr int LPH = BDGX_LP_HIDDEN;
+constexpr int LP_OFF_W = 8 + 2 * LPK;
+constexpr int LP_NCONV = 29 * LPH;
+
+struct LpCoef {
+  float aK[2], bK[2], aM[2], bM[2];
+  float kK[LPK], kM[LPK];
+};
+
+// ---------------------------------------------------------------------------
+// setup kernels
+__global__ void k_lp_rho(const float *wf, long long S, float *rho) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < S;
+       i += (long long)gridDim.x * blockDim.x)
+    rho[i] = wf[i] * wf[i];
+}
+
+// max of a non-negative array into *out (float bit patterns order like ints)
+__global__ void k_lp_max(const float *x, long long n, unsigned int *out) {
+  __shared__ float red[256];
+  float m = 0.f;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    m = fmaxf(m, x[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
+}
+
+__global__ void k_lp_weights(LpCoef cf, const float *pK, const float *pM, float *rho,
+                             const unsigned int *rmax, long long S, float inv_mu2, float *aK,
+                             float *bK, float *aM, float *bM, float *cs) {
+  const float inv = 1.0f / fmaxf(__uint_as_float(*rmax), 1e-30f);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < S;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float r = rho[i] * inv;
+    rho[i] = r;
+    const float r2 = r * r;
+    aK[i] = pK[i] * expf(cf.aK[0] * r + cf.aK[1] * r2);
+    bK[i] = pK[i] * expf(cf.bK[0] * r + cf.bK[1] * r2);
+    aM[i] = pM[i] * expf(cf.aM[0] * r + cf.aM[1] * r2);
+    bM[i] = pM[i] * expf(cf.bM[0] * r + cf.bM[1] * r2);
+    const float pp = pK[i] * pM[i];
+    cs[i] = pp * pp * inv_mu2;
+  }
+}
+
+__device__ __forceinline__ void lp_knot(float kabs, float kmax, int &i, float &f) {
+  const float t = kmax > 0.f ? kabs / kmax * (float)(LPK - 1) : 0.f;
+  i = (int)floorf(t);
+  if (i > LPK - 2) i = LPK - 2;
+  if (i < 0) i = 0;
+  f = t - (float)i;
+}
+
+__global__ void k_lp_tables(LpCoef cf, Geo g, const float *kx2, const float *ky2,
+                            const float *kz2, double mu, float kmax, float *gK, float *gM) {
+  const long long total = g.Kk;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int kx = (int)(e % g.n0k);
+    const long long r = e / g.n0k;
+    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    const double k2 = (double)kx2[kx] + (double)ky2[y] + (double)kz2[z];
+    double d = mu + 0.5 * k2;
+    if (fabs(d) < 1e-8) d = 1e-8;
+    const double base = 1.0 / ((double)g.S * d);
+    int i;
+    float f;
+    lp_knot((float)sqrt(k2), kmax, i, f);
+    gK[e] = (float)(base * exp((double)((1.f - f) * cf.kK[i] + f * cf.kK[i + 1])));
+    gM[e] = (float)(base * exp((double)((1.f - f) * cf.kM[i] + f * cf.kM[i + 1])));
+  }
+}
+
+// sig[col] = rms of column col (one workgroup per column)
+__global__ void __launch_bounds__(256) k_lp_colnorm(const float *R, long long ld, long long S,
+                                                    float *sig) {
+  const float *r = R + (long long)blockIdx.x * ld;
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < S; i += blockDim.x) s += (double)r[i] * r[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sig[blockIdx.x] = (float)fmax(sqrt(red[0] / (double)S), 1e-30);
+}
+
+// ---------------------------------------------------------------------------
+// stencil layer
+__device__ __forceinline__ long long lp_nb(long long i, int dx, int dy, int dz, const Geo &g) {
+  const int x = (int)(i % g.nx);
+  const long long r = i / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int xx = x + dx, yy = y + dy, zz = z + dz;
+  xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
+  yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
+  zz = zz < 0 ? zz + g.nz : (zz >= g.nz ? zz - g.nz : zz);
+  return ((long long)zz * g.ny + yy) * g.nx + xx;
+}
+
+// W[:, col] += cs * sum_h v_h tanh(z_h), z_h = sum_nb w_{h,nb} r(x+nb) + d_h
+__global__ void __launch_bounds__(256) k_lp_conv_fwd(const float *R, long long ld, int ncols,
+                                                     Geo g, const float *prm, const float *cs,
+                                                     const float *sig, float *W) {
+  __shared__ float sp[LP_NCONV];
+  for (int i = threadIdx.x; i < LP_NCONV; i += blockDim.x) sp[i] = prm[i];
+  __syncthreads();
+  const float *w = sp, *d = sp + 27 * LPH, *v = sp + 28 * LPH;
+  const long long total = g.S * ncols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(e / g.S);
+    const long long i = e - (long long)col * g.S;
+    const float *r = R + (long long)col * ld;
+    const float sg = sig[col], isg = 1.f / sg;
+    float nb[27];
+    int n = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) nb[n++] = r[lp_nb(i, dx, dy, dz, g)] * isg;
+    float s = 0.f;
+#pragma unroll 4
+    for (int h = 0; h < LPH; ++h) {
+      float z = d[h];
+#pragma unroll
+      for (int q = 0; q < 27; ++q) z += w[h * 27 + q] * nb[q];
+      s += v[h] * tanhf(z);
+    }
+    W[i + (long long)col * ld] += cs[i] * sg * s;
+  }
+}
+
+// d(w, d, v) of sum(dW * cs * s): grid.y = hidden channel h
+__global__ void __launch_bounds__(256) k_lp_conv_bwd(const float *R, const float *dW, long long ld,
+                                                     int ncols, Geo g, const float *prm,
+                                                     const float *cs, const float *sig,
+                                                     double *grad) {
+  const int h = blockIdx.y;
+  const float *w = prm + h * 27;
+  const float dh = prm[27 * LPH + h], vh = prm[28 * LPH + h];
+  float gw[27], gd = 0.f, gv = 0.f;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) gw[q] = 0.f;
+  const long long total = g.S * ncols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(e / g.S);
+    const long long i = e - (long long)col * g.S;
+    const float *r = R + (long long)col * ld;
+    const float sg = sig[col], isg = 1.f / sg;
+    float nb[27];
+    int n = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) nb[n++] = r[lp_nb(i, dx, dy, dz, g)] * isg;
+    float z = dh;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) z += w[q] * nb[q];
+    const float t = tanhf(z);
+    const float ds = cs[i] * sg * dW[i + (long long)col * ld];
+    gv += ds * t;
+    const float gz = ds * vh * (1.f - t * t);
+    gd += gz;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) gw[q] += gz * nb[q];
+  }
+  // block sum of the 29 partials (two rounds through LDS)
+  __shared__ float red[256];
+  for (int q = 0; q < 29; ++q) {
+    const float val = q < 27 ? gw[q] : (q == 27 ? gd : gv);
+    red[threadIdx.x] = val;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const int o = q < 27 ? LP_OFF_W + h * 27 + q
+                           : (q == 27 ? LP_OFF_W + 27 * LPH + h : LP_OFF_W + 28 * LPH + h);
+      atomicAdd(grad + o, (double)red[0]);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions and elementwise helpers
+// out[0..1] += sum_{x, col} P Q wgt (rho, rho^2)   (wgt may be null)
+__global__ void __launch_bounds__(256) k_lp_pointgrad(const float *P, const float *Q,
+                                                      const float *wgt, const float *rho,
+                                                      long long S, int ncols, long long ld,
+                                                      double *out) {
+  double s0 = 0.0, s1 = 0.0;
+  const long long total = S * ncols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(e / S);
+    const long long i = e - (long long)col * S;
+    double t = (double)P[i + col * ld] * (double)Q[i + col * ld];
+    if (wgt) t *= (double)wgt[i];
+    const double r = rho[i];
+    s0 += t * r;
+    s1 += t * r * r;
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) atomicAdd(out + threadIdx.x, red[threadIdx.x][0]);
+}
+
+__global__ void k_lp_mulw(float *X, const float *w, long long S, int ncols, long long ld) {
+  const long long total = S * ncols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(e / S);
+    const long long i = e - (long long)col * S;
+    X[i + col * ld] *= w[i];
+  }
+}
+
+// X2[:, col] = [A[:, col]; B[:, col]] (ld2 = 2 S)
+__global__ void k_lp_pack2(const float *A, const float *B, long long S, int ncols, long long ld,
+                           float *X2) {
+  const long long total = S * ncols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(e / S);
+    const long long i = e - (long long)col * S;
+    X2[2 * S * col + i] = A[i + col * ld];
+    X2[2 * S * col + S + i] = B[i + col * ld];
+  }
+}
+
+// kappa gradient: out[knot] += sum_{k, col} wt(kx) Re(conj(A_k) B_k) g_k * basis
+__global__ void __launch_bounds__(256) k_lp_knotgrad(const cx<float> *spec, int ncols, Geo g,
+                                                     const float *kx2, const float *ky2,
+                                                     const float *kz2, float kmax,
+                                                     const float *gt, double *out) {
+  __shared__ double hist[LPK];
+  if (threadIdx.x < LPK) hist[threadIdx.x] = 0.0;
+  __syncthreads();
+  const long long total = g.Kk * ncols;
+  const long long chan = g.Kk * ncols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long kidx = e % g.Kk;
+    const int kx = (int)(kidx % g.n0k);
+    const long long r = kidx / g.n0k;
+    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    const cx<float> A = spec[e], B = spec[chan + e];
+    const bool paired = kx > 0 && !((g.nx & 1) == 0 && kx == g.nx / 2);
+    const double q = (paired ? 2.0 : 1.0) * ((double)A.re * B.re + (double)A.im * B.im) *
+                     (double)gt[kidx];
+    int i;
+    float f;
+    lp_knot(sqrtf(kx2[kx] + ky2[y] + kz2[z]), kmax, i, f);
+    atomicAdd(&hist[i], (1.0 - f) * q);
+    atomicAdd(&hist[i + 1], (double)f * q);
+  }
+  __syncthreads();
+  if (threadIdx.x < LPK) atomicAdd(out + threadIdx.x, hist[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+static int lp_alloc(float **p, size_t n) {
+  if (!*p) BDGX_HIP(hipMalloc((void **)p, sizeof(float) * (n ? n : 1)));
+  return BDGX_OK;
+}
+
+static int ensure_lp_tmp(bdgx_ctx *c, size_t bytes) {
+  if (bytes <= c->lp_tmp_bytes) return BDGX_OK;
+  if (c->lp_tmp) {
+    BDGX_HIP(hipStreamSynchronize(c->stream));
+    BDGX_HIP(hipFree(c->lp_tmp));
+    c->lp_tmp = nullptr;
+    c->lp_tmp_bytes = 0;
+  }
+  BDGX_HIP(hipMalloc(&c->lp_tmp, bytes));
+  c->lp_tmp_bytes = bytes;
+  return BDGX_OK;
+}
+
+static float lp_kmax(const bdgx_ctx *c) {
+  double s = 0.0;
+  for (int d = 0; d < c->dim; ++d) {
+    const double kn = M_PI * (double)c->N[d] / c->L[d];  // largest |k_d| on the grid
+    s += kn * kn;
+  }
+  return (float)sqrt(s);
+}
+
+static unsigned grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace bdgx
+
+using namespace bdgx;
+
+extern "C" {
+
+int bdgx_lp_set_params(bdgx_ctx *c, const float *theta) {
+  if (!c || !theta) return BDGX_EINVAL;
+  if (c->cplx) return BDGX_EINVAL;
+  if (!c->has_wf || !c->has_p) return BDGX_ESTATE;
+  const long long S = c->geo.S;
+  Fields<float> &F = c->f32;
+  int rc = 0;
+  for (int w = 0; w < 2 && !rc; ++w) {
+    rc = lp_alloc(&F.lpa[w], (size_t)S);
+    if (!rc) rc = lp_alloc(&F.lpb[w], (size_t)S);
+    if (!rc) rc = lp_alloc(&F.lpg[w], (size_t)c->geo.Kk);
+  }
+  if (!rc) rc = lp_alloc(&c->lp_cs, (size_t)S);
+  if (!rc) rc = lp_alloc(&c->lp_rho, (size_t)S);
+  if (!rc) rc = lp_alloc(&c->lp_conv, (size_t)LP_NCONV);
+  if (rc) return rc;
+  LpCoef cf;
+  memcpy(cf.aK, theta + 0, 2 * sizeof(float));
+  memcpy(cf.bK, theta + 2, 2 * sizeof(float));
+  memcpy(cf.aM, theta + 4, 2 * sizeof(float));
+  memcpy(cf.bM, theta + 6, 2 * sizeof(float));
+  memcpy(cf.kK, theta + 8, LPK * sizeof(float));
+  memcpy(cf.kM, theta + 8 + LPK, LPK * sizeof(float));
+  BDGX_HIP(hipStreamSynchronize(c->stream));  // earlier applies may read the tables
+  BDGX_HIP(hipMemcpyAsync(c->lp_conv, theta + LP_OFF_W, sizeof(float) * LP_NCONV,
+                          hipMemcpyHostToDevice, c->stream));
+  rc = ensure_scratch(c, sizeof(unsigned int));
+  if (rc) return rc;
+  unsigned int *dmax = reinterpret_cast<unsigned int *>(c->scratch);
+  BDGX_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned int), c->stream));
+  hipLaunchKernelGGL(k_lp_rho, dim3(grid_for(S)), dim3(256), 0, c->stream, F.wf, S, c->lp_rho);
+  hipLaunchKernelGGL(k_lp_max, dim3(grid_for(S)), dim3(256), 0, c->stream, c->lp_rho, S, dmax);
+  const double smu = fabs(c->mu) < 1e-8 ? 1e-8 : c->mu;
+  hipLaunchKernelGGL(k_lp_weights, dim3(grid_for(S)), dim3(256), 0, c->stream, cf, F.pK, F.pM,
+                     c->lp_rho, dmax, S, (float)(1.0 / (smu * smu)), F.lpa[0], F.lpb[0], F.lpa[1],
+                     F.lpb[1], c->lp_cs);
+  hipLaunchKernelGGL(k_lp_tables, dim3(grid_for(c->geo.Kk)), dim3(256), 0, c->stream, cf, c->geo,
+                     F.kx2[0], F.kx2[1], F.kx2[2], c->mu, lp_kmax(c), F.lpg[0], F.lpg[1]);
+  BDGX_HIP(hipGetLastError());
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  c->has_lp = 1;
+  return BDGX_OK;
+}
+
+int bdgx_lp_apply(bdgx_ctx *c, int prec, const void *dR, void *dW, uint64_t ncols, uint64_t ld) {
+  if (!c || prec != BDGX_FP32 || c->cplx || dR == dW || ld != (uint64_t)c->geo.S)
+    return BDGX_EINVAL;
+  if (!c->has_lp) return BDGX_ESTATE;
+  if (ncols == 0) return BDGX_OK;
+  const long long S = c->geo.S;
+  int rc = ensure_lp_tmp(c, sizeof(float) * ((size_t)S * ncols + ncols));
+  if (rc) return rc;
+  float *U = reinterpret_cast<float *>(c->lp_tmp);
+  float *sig = U + (size_t)S * ncols;
+  const float *R = reinterpret_cast<const float *>(dR);
+  float *W = reinterpret_cast<float *>(dW);
+  hipLaunchKernelGGL(k_lp_colnorm, dim3((unsigned)ncols), dim3(256), 0, c->stream, R,
+                     (long long)ld, S, sig);
+  rc = lp_filter_f32(c, 0, 0, R, U, (long long)ncols, S);
+  if (!rc) rc = lp_filter_f32(c, 1, 0, U, W, (long long)ncols, (long long)ld);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_lp_conv_fwd, dim3(grid_for(S * (long long)ncols)), dim3(256), 0, c->stream,
+                     R, (long long)ld, (int)ncols, c->geo, c->lp_conv, c->lp_cs,
+                     (const float *)sig, W);
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
+int bdgx_lp_backward(bdgx_ctx *c, const float *dR, const float *dGW, uint64_t ncols, uint64_t ld,
+                     float *h_grad) {
+  if (!c || !dR || !dGW || !h_grad || c->cplx || ld != (uint64_t)c->geo.S) return BDGX_EINVAL;
+  if (!c->has_lp) return BDGX_ESTATE;
+  if (ncols == 0) return BDGX_OK;
+  const long long S = c->geo.S, n = (long long)ncols, L = (long long)ld;
+  Fields<float> &F = c->f32;
+  // U, Wsp, T1, T2 (S x n each, ld S) and X2 (2S x n)
+  int rc = ensure_lp_tmp(c, sizeof(float) * ((size_t)S * n * 6 + n));
+  if (rc) return rc;
+  float *U = reinterpret_cast<float *>(c->lp_tmp);
+  float *Wsp = U + S * n, *T1 = Wsp + S * n, *T2 = T1 + S * n, *X2 = T2 + S * n;
+  float *sig = X2 + 2 * S * n;
+  hipLaunchKernelGGL(k_lp_colnorm, dim3((unsigned)n), dim3(256), 0, c->stream, dR, L, S, sig);
+  rc = ensure_scratch(c, sizeof(double) * BDGX_LP_NPARAM);
+  if (rc) return rc;
+  double *dg = reinterpret_cast<double *>(c->scratch);
+  BDGX_HIP(hipMemsetAsync(dg, 0, sizeof(double) * BDGX_LP_NPARAM, c->stream));
+  const unsigned gb = grid_for(S * n);
+  const float kmax = lp_kmax(c);
+  // forward intermediates
+  rc = lp_filter_f32(c, 0, 0, dR, U, n, L);
+  if (!rc) rc = lp_filter_f32(c, 1, 0, U, Wsp, n, S);
+  // stencil branch
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_conv_bwd, dim3(gb > 1024 ? 1024 : gb, LPH), dim3(256), 0, c->stream,
+                       dR, dGW, L, (int)n, c->geo, c->lp_conv, c->lp_cs, (const float *)sig, dg);
+    if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
+  }
+  // M stage: alpha_M, beta_M, kappa_M; dU = b_M * C_M(a_M dW)
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, c->stream, dGW, Wsp,
+                       (const float *)nullptr, c->lp_rho, S, (int)n, L, dg + 4);
+    rc = lp_filter_f32(c, 1, 1, dGW, T1, n, L);
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, c->stream, U, T1,
+                       (const float *)F.lpb[1], c->lp_rho, S, (int)n, S, dg + 6);
+    hipLaunchKernelGGL(k_lp_pack2, dim3(gb), dim3(256), 0, c->stream, dGW, U, S, (int)n, L, X2);
+    rc = lp_spectra_f32(c, 1, X2, n, 2 * S);
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_knotgrad, dim3(grid_for(c->geo.Kk * n)), dim3(256), 0, c->stream,
+                       reinterpret_cast<const cx<float> *>(c->spec), (int)n, c->geo, F.kx2[0],
+                       F.kx2[1], F.kx2[2], kmax, (const float *)F.lpg[1], dg + 8 + LPK);
+    hipLaunchKernelGGL(k_lp_mulw, dim3(gb), dim3(256), 0, c->stream, T1, (const float *)F.lpb[1],
+                       S, (int)n, S);
+    // K stage with upstream dU = T1
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, c->stream, T1, U,
+                       (const float *)nullptr, c->lp_rho, S, (int)n, S, dg + 0);
+    rc = lp_filter_f32(c, 0, 1, T1, T2, n, S);
+  }
+  if (!rc) {
+    // sum R T2 b_K phi: R has leading dimension L, T2 has S; pack R first
+    hipLaunchKernelGGL(k_lp_pack2, dim3(gb), dim3(256), 0, c->stream, T1, dR, S, (int)n, L, X2);
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, c->stream, dR, T2,
+                       (const float *)F.lpb[0], c->lp_rho, S, (int)n, L, dg + 2);
+    rc = lp_spectra_f32(c, 0, X2, n, 2 * S);
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_knotgrad, dim3(grid_for(c->geo.Kk * n)), dim3(256), 0, c->stream,
+                       reinterpret_cast<const cx<float> *>(c->spec), (int)n, c->geo, F.kx2[0],
+                       F.kx2[1], F.kx2[2], kmax, (const float *)F.lpg[0], dg + 8);
+    if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
+  }
+  std::vector<double> hg(BDGX_LP_NPARAM, 0.0);
+  if (!rc && hipMemcpyAsync(hg.data(), dg, sizeof(double) * BDGX_LP_NPARAM, hipMemcpyDeviceToHost,
+                            c->stream) != hipSuccess)
+    rc = BDGX_EHIP;
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = BDGX_EHIP;
+  if (rc) return rc;
+  for (int i = 0; i < BDGX_LP_NPARAM; ++i) h_grad[i] = (float)hg[i];
+  return BDGX_OK;
+}
+
+}  // extern "C"

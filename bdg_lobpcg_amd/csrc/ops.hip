@@ -320,6 +320,9 @@ __device__ __forceinline__ R kmul(const YZArgs<R> &a, int code, int kx, int y,
     }
     case MUL_ITPKIN:
       return R(exp(-(double)a.dt * 0.5 * (double)k2)) * a.invS;
+    case MUL_TABK:
+    case MUL_TABM:
+      return a.tab[code - MUL_TABK][((long long)z * a.g.ny + y) * a.g.n0k + kx];
     default:
       return a.invS;
   }
@@ -351,6 +354,14 @@ __global__ void __launch_bounds__(kThreads) k_zpass(YZArgs<R> a) {
   }
   __syncthreads();
   cx<R> *res = lds_fft<R, false>(bufA, bufB, tw, a.pl, P, ps);
+  if (a.fwd_only) {
+    for (int e = threadIdx.x; e < P * nz; e += blockDim.x) {
+      const int z = e / P, p = e - z * P;
+      const int kx = kx0 + p;
+      if (kx < n0k) base[z * zs + kx] = res[p * ps + z];
+    }
+    return;
+  }
   cx<R> *oth = res == bufA ? bufB : bufA;
   for (int e = threadIdx.x; e < P * nz; e += blockDim.x) {
     const int z = e / P, p = e - z * P;
@@ -396,7 +407,7 @@ struct HalfJob {
   int out_real;
 };
 
-enum FieldSel { F_NONE = 0, F_LTK, F_LTM, F_PK, F_PM, F_WF };
+enum FieldSel { F_NONE = 0, F_LTK, F_LTM, F_PK, F_PM, F_WF, F_LAK, F_LBK, F_LAM, F_LBM };
 
 template <typename R>
 static const R *field_ptr(const Fields<R> &f, int sel) {
@@ -406,6 +417,10 @@ static const R *field_ptr(const Fields<R> &f, int sel) {
     case F_PK: return f.pK;
     case F_PM: return f.pM;
     case F_WF: return f.wf;
+    case F_LAK: return f.lpa[0];
+    case F_LBK: return f.lpb[0];
+    case F_LAM: return f.lpa[1];
+    case F_LBM: return f.lpb[1];
     default: return nullptr;
   }
 }
@@ -521,6 +536,8 @@ static YZArgs<R> make_yz(bdgx_ctx *c, const int *mul, int C, int nc, cx<R> *spec
   yz.ky2 = F.kx2[1];
   yz.kz2 = F.kx2[2];
   yz.U = F.U;
+  yz.tab[0] = F.lpg[0];
+  yz.tab[1] = F.lpg[1];
   yz.mu = (R)c->mu;
   yz.invS = (R)(1.0 / (double)g.S);
   yz.dt = (R)c->itp_dt;
@@ -564,7 +581,8 @@ static int launch_z(bdgx_ctx *c, YZArgs<R> z) {
 
 template <typename R>
 static int run_jobs(bdgx_ctx *c, const HalfJob *jobs, int njobs, const R *X,
-                    R *Y, long long ncols_total, long long ld, const R *lt_override) {
+                    R *Y, long long ncols_total, long long ld, const R *lt_override,
+                    bool fwd_only = false) {
   const Geo &g = c->geo;
   int C = 0;
   int mul[kMaxChan];
@@ -597,9 +615,15 @@ static int run_jobs(bdgx_ctx *c, const HalfJob *jobs, int njobs, const R *X,
       if ((rc = plane ? launch_plane_x<R>(c, a, false) : launch_x<R>(c, a, false))) return rc;
       base += jobs[j].nch;
     }
-    const YZArgs<R> yz = make_yz<R>(c, mul, C, nc, spec);
+    YZArgs<R> yz = make_yz<R>(c, mul, C, nc, spec);
+    yz.fwd_only = fwd_only ? 1 : 0;
     if (!plane && (rc = launch_y<R>(c, yz, false))) return rc;
     if ((rc = launch_z<R>(c, yz))) return rc;
+    if (fwd_only) {
+      // the caller reads the spectrum; a single chunk is required
+      if (chunk < ncols_total) return BDGX_EINVAL;
+      return BDGX_OK;
+    }
     if (!plane && (rc = launch_y<R>(c, yz, true))) return rc;
     base = 0;
     for (int j = 0; j < njobs; ++j) {
@@ -751,6 +775,40 @@ int itp_kinetic_t(bdgx_ctx *c, R *dpsi, double dt) {
   if (dt >= 0.0) J.mul[0] = MUL_ITPKIN;
   c->itp_dt = dt >= 0.0 ? dt : 0.0;
   return run_jobs<R>(c, &J, 1, dpsi, dpsi, 1, c->geo.S, nullptr);
+}
+
+// Learned-preconditioner building blocks (FP32; see learned.hip).
+// lp_filter: Y = a_w * C_w(b_w * X) (mode 0) or Y = C_w(a_w * X) (mode 1),
+// C_w the k-space table filter of stage w (0: K, 1: M).
+int lp_filter_f32(bdgx_ctx *c, int w, int mode, const float *X, float *Y, long long ncols,
+                  long long ld) {
+  HalfJob J;
+  memset(&J, 0, sizeof(J));
+  J.nch = 1;
+  J.pre[0] = PRE_REAL;
+  J.prew_sel[0] = mode == 0 ? (w ? F_LBM : F_LBK) : (w ? F_LAM : F_LAK);
+  J.mul[0] = w ? MUL_TABM : MUL_TABK;
+  J.post[0] = mode == 0 ? POST_REAL : POST_ONE;
+  J.postw_sel[0] = mode == 0 ? (w ? F_LAM : F_LAK) : F_NONE;
+  J.posts[0] = 1.0;
+  return run_jobs<float>(c, &J, 1, X, Y, ncols, ld, nullptr);
+}
+
+// Forward 3D spectra of a_w * P (channel 0) and b_w * Q (channel 1) for the
+// columns of X2 = [P; Q] (2*size rows, ld2), left in the context's spectrum
+// workspace: spec[(chan * ncols + col) * Kk + kidx].
+int lp_spectra_f32(bdgx_ctx *c, int w, const float *X2, long long ncols, long long ld2) {
+  HalfJob J[2];
+  memset(J, 0, sizeof(J));
+  for (int q = 0; q < 2; ++q) {
+    J[q].in_half = q;
+    J[q].nch = 1;
+    J[q].pre[0] = PRE_REAL;
+    J[q].prew_sel[0] = q == 0 ? (w ? F_LAM : F_LAK) : (w ? F_LBM : F_LBK);
+    J[q].mul[0] = MUL_ONE;
+    J[q].posts[0] = 1.0;
+  }
+  return run_jobs<float>(c, J, 2, X2, nullptr, ncols, ld2, nullptr, true);
 }
 
 // Times one pass kind over nbatch single-channel (kinetic) batches.

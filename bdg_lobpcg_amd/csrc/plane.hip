@@ -322,6 +322,10 @@ struct ZMul {
       case MUL_ITPKIN:
         f = R(exp(-(double)a->dt * 0.5 * (double)k2)) * a->invS;
         break;
+      case MUL_TABK:
+      case MUL_TABM:
+        f = a->tab[code - MUL_TABK][((long long)z * a->g.ny + y) * a->g.n0k + kx];
+        break;
       default:
         f = a->invS;
     }
@@ -351,6 +355,10 @@ __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
   constexpr int nst = make_fixed_plan(NZ).nst;
   cx<R> *F = (nst % 2 == 1) ? bufA : bufB;  // last LDS buffer of the forward
   cx<R> *O = (F == bufA) ? bufB : bufA;
+  if (a.fwd_only) {
+    fixed_stages<R, false, NZ, P, T, 0>(g, g, bufA, bufB, tw, NoMul{});
+    return;
+  }
   if constexpr (nst == 1) {
     fixed_stages<R, false, NZ, P, T, 0>(g, LdsIO<R>{bufA}, bufA, bufB, tw, mul);
     fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{bufA}, g, bufB, bufA, tw, NoMul{});

@@ -39,7 +39,9 @@ struct YZArgs {
   AxisPlan pl;
   // z pass only
   int mul[kMaxChan];
+  int fwd_only;  // z forward transform only: the spectrum stays in HBM
   const R *kx2, *ky2, *kz2, *U;
+  const R *tab[2];  // learned k-space tables (MUL_TABK, MUL_TABM)
   R mu, invS, dt;
 };
 

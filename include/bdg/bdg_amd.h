@@ -30,6 +30,8 @@ typedef struct {
   uint64_t column_applies_fp32; /* of which in FP32 (mixed policy)             */
   uint64_t image_refreshes;     /* FP64 re-applications of M, K to X (mixed)   */
   uint64_t restarts;            /* iterations that left the P block out        */
+  uint64_t learned_iterations;  /* iterations whose W came from the network    */
+  int learned_fallback_at;      /* iteration of the switch back to T_M T_K, or 0 */
   double seconds_total;         /* bdg_solve wall clock                        */
   double seconds_rr;            /* host Rayleigh-Ritz time                     */
   int status;                   /* 0 ok, else a BDGX_E* code or -100 (RR)     */
@@ -58,6 +60,15 @@ f64 bdg_mu(const bdg_t *bdg);
  * of X are re-applied in FP64 every refresh_every iterations (default 20, 0:
  * only when the residual reaches img_tol). */
 void bdg_set_mixed_params(bdg_t *bdg, f64 img_tol, uint64_t refresh_every);
+
+/* Learned preconditioner for the search directions (real psi0): theta holds
+ * BDGX_LP_NPARAM floats (layout in bdgx.h), copied; NULL switches back to the
+ * analytic T_M T_K.  The solver falls back to T_M T_K for the rest of a solve
+ * when the worst residual has not dropped below factor x its value `window`
+ * iterations earlier (bdg_set_learned_fallback; default 30 and 0.5, window 0:
+ * never). */
+void bdg_set_learned_preconditioner(bdg_t *bdg, const float *theta);
+void bdg_set_learned_fallback(bdg_t *bdg, int window, f64 factor);
 
 /* 0 silent (default), 1 per-phase summary, 2 per-iteration trace (stderr). */
 void bdg_set_verbosity(bdg_t *bdg, int level);

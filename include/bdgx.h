@@ -154,6 +154,27 @@ enum { BDGX_PASS_XFWD = 0, BDGX_PASS_YFWD = 1, BDGX_PASS_Z = 2, BDGX_PASS_YINV =
 int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iters,
                     float *ms_per_launch, double *bytes_per_launch);
 
+/* ---- learned preconditioner (no reference counterpart; its analytic base
+ *      is precondK/precondM, reference src/precond_impl.inc:21-111) --------
+ *   N(r) = T'_M(T'_K(r)) + p_K p_M sum_h v_h tanh(sum_nb w_{h,nb} r(x+nb) + d_h)
+ *   T'_X v = a_X F^-1[g_X F(b_X v)],  a_X = p_X exp(alpha_X.phi),
+ *   b_X = p_X exp(beta_X.phi),  phi = (rho, rho^2),  rho = psi0^2 / max psi0^2,
+ *   g_X(k) = exp(interp(kappa_X, |k|)) / (S safe(mu + k^2/2))
+ * theta (float, BDGX_LP_NPARAM): alpha_K[2] beta_K[2] alpha_M[2] beta_M[2]
+ * kappa_K[NKNOT] kappa_M[NKNOT] w[HIDDEN][27] d[HIDDEN] v[HIDDEN]; the 27
+ * stencil offsets in the order (dz+1)*9 + (dy+1)*3 + (dx+1), periodic.
+ * Real psi0 only; FP32; columns with leading dimension size. */
+enum { BDGX_LP_NKNOT = 16, BDGX_LP_HIDDEN = 16,
+       BDGX_LP_NPARAM = 8 + 2 * BDGX_LP_NKNOT + 29 * BDGX_LP_HIDDEN };
+/* Set the parameters (needs psi0 and the preconditioner weights uploaded). */
+int bdgx_lp_set_params(bdgx_ctx *ctx, const float *theta);
+/* dW = N(dR) for ncols FP32 columns (prec must be BDGX_FP32, dW != dR). */
+int bdgx_lp_apply(bdgx_ctx *ctx, int prec, const void *dR, void *dW, uint64_t ncols,
+                  uint64_t ld);
+/* h_grad = d/dtheta sum(dGW * N(dR)) (device forward + adjoint, FP64 sums). */
+int bdgx_lp_backward(bdgx_ctx *ctx, const float *dR, const float *dGW, uint64_t ncols,
+                     uint64_t ld, float *h_grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,9 @@ weights from src/setup.c:150-177):
 
 and the network generalises it as
 
-    N(r) = T_M'(T_K'(r)) + p_K p_M * sum_h v_h tanh(sum_nb w_{h,nb} r(x+nb) + d_h)
+    N(r) = T_M'(T_K'(r)) + c * sigma * sum_h v_h tanh(sum_nb w_{h,nb} r(x+nb)/sigma + d_h)
+    sigma = rms(r) per column (the network is positively homogeneous in r),
+    c = (p_K p_M)^2 / safe(mu)^2 (the low-k scale of T_M T_K)
     T_X' v = a_X * S * irfft(g_X * rfft(b_X * v))
     a_X = p_X exp(alpha_X . phi),  b_X = p_X exp(beta_X . phi)
     g_X(k) = exp(interp(kappa_X, |k|)) / (S * safe(mu + k^2/2))
@@ -80,6 +82,8 @@ class LearnedRef:
         self.base = 1.0 / (self.S * d)
         self.pK = np.asarray(pK, float).reshape(self.shape)
         self.pM = np.asarray(pM, float).reshape(self.shape)
+        smu = mu if abs(mu) >= 1e-8 else 1e-8
+        self.cs = (self.pK * self.pM) ** 2 / (smu * smu)
         rho = np.asarray(wf, float).reshape(self.shape) ** 2
         rho = rho / rho.max()
         self.phi = np.stack([rho, rho ** 2])
@@ -121,8 +125,9 @@ class LearnedRef:
             r = R[:, j].reshape(self.shape)
             u = T["aK"] * self.filt(T["gK"], T["bK"] * r)
             w = T["aM"] * self.filt(T["gM"], T["bM"] * u)
-            s = np.tensordot(P["v"], np.tanh(self.conv_z(P, r)), axes=(0, 0))
-            out[:, j] = (w + self.pK * self.pM * s).ravel()
+            sig = np.sqrt(np.mean(r * r))
+            s = np.tensordot(P["v"], np.tanh(self.conv_z(P, r / sig)), axes=(0, 0))
+            out[:, j] = (w + self.cs * sig * s).ravel()
         return out
 
     def grad(self, theta, R, dW):
@@ -152,15 +157,17 @@ class LearnedRef:
             B = np.fft.rfftn(T["bK"] * r)
             q = self.wt * np.real(np.conj(A) * B) * T["gK"]
             g["kK"] += self.knot_sum(q)
-            # stencil branch
-            z = self.conv_z(P, r)
+            # stencil branch (sigma depends on r only)
+            sig = np.sqrt(np.mean(r * r))
+            rn = r / sig
+            z = self.conv_z(P, rn)
             th = np.tanh(z)
-            ds = self.pK * self.pM * dw
+            ds = self.cs * sig * dw
             g["v"] += np.tensordot(th, ds, axes=([1, 2, 3], [0, 1, 2]))
             gz = P["v"][:, None, None, None] * (1 - th ** 2) * ds[None]
             g["d"] += gz.sum(axis=(1, 2, 3))
             for n, (dz, dy, dx) in enumerate(offsets()):
-                rs = np.roll(r, shift=(-dz, -dy, -dx), axis=(0, 1, 2))
+                rs = np.roll(rn, shift=(-dz, -dy, -dx), axis=(0, 1, 2))
                 g["w"][:, n] += np.tensordot(gz, rs, axes=([1, 2, 3], [0, 1, 2]))
         return np.concatenate([g["aK"], g["bK"], g["aM"], g["bM"], g["kK"], g["kM"],
                                g["w"].ravel(), g["d"], g["v"]])

@@ -58,3 +58,60 @@ def test_cosine_loss_gradient():
         fd = (LR.cosine_loss(Wp, E)[0] - LR.cosine_loss(Wm, E)[0]) / (2 * h)
         assert abs(fd - dW[i, j]) < 1e-7
     assert LR.cosine_loss(E * 3.0, E)[0] < 1e-14
+
+
+def _device(N, L):
+    from tests.common import device_twin
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
+    d = device_twin(N, L, False, f)
+    net = LR.LearnedRef(N, [c.kx2(k) for k in range(3)], f["pK"], f["pM"], f["wf"], f["mu"])
+    return c, f, d, net
+
+
+def _theta(seed):
+    rng = np.random.default_rng(seed)
+    th = LR.init_theta(seed=seed, wscale=0.5)
+    th[:8] = 0.3 * rng.standard_normal(8)
+    th[8:8 + 2 * LR.NKNOT] = 0.2 * rng.standard_normal(2 * LR.NKNOT)
+    o = 8 + 2 * LR.NKNOT + 27 * LR.HID
+    th[o:] = 0.3 * rng.standard_normal(2 * LR.HID)
+    return th
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,L", [((16, 12, 10), (12.0, 10.0, 9.0)), ((32, 32, 32), (16.0, 16.0, 12.0))],
+                         ids=["pencil", "plane"])
+def test_device_forward_and_gradient(N, L):
+    """Device network forward and parameter gradient (hand-written HIP forward
+    and adjoint kernels) against the numpy restatement, FP32 tolerances."""
+    c, f, d, net = _device(N, L)
+    th = _theta(11)
+    d.lp_set_params(th.astype(np.float32))
+    R = rand_block(c.size, 3, False, seed=21)
+    G = rand_block(c.size, 3, False, seed=22)
+    print(f"case N={N}", flush=True)
+    W = d.lp_apply(R)
+    Wr = net.forward(th.astype(np.float32).astype(np.float64), R.astype(np.float32).astype(np.float64))
+    assert relerr(W, Wr) < 2e-5
+    g = d.lp_backward(R, G)
+    gr = net.grad(th.astype(np.float32).astype(np.float64), R.astype(np.float32).astype(np.float64),
+                  G.astype(np.float32).astype(np.float64))
+    scale = np.abs(gr).max()
+    # per parameter group
+    K = LR.NKNOT
+    o = 8 + 2 * K
+    for name, sl in [("pointwise", slice(0, 8)), ("knots", slice(8, o)),
+                     ("stencil", slice(o, LR.NPARAM))]:
+        err = np.abs(g[sl] - gr[sl]).max() / max(np.abs(gr[sl]).max(), 1e-30)
+        assert err < 1e-3, f"{name}: rel err {err:.2e}"
+    assert np.abs(g - gr).max() < 1e-3 * scale
+
+
+@pytest.mark.gpu
+def test_device_zero_theta_is_the_analytic_preconditioner():
+    c, f, d, net = _device((16, 16, 16), (12.0, 12.0, 12.0))
+    th = LR.init_theta(seed=3)
+    d.lp_set_params(th.astype(np.float32))
+    R = rand_block(c.size, 4, False, seed=5)
+    ref = c.apply("precondM", c.apply("precondK", R))
+    assert relerr(d.lp_apply(R), ref) < 2e-5
