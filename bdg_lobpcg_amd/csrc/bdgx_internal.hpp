@@ -85,12 +85,13 @@ struct bdgx_ctx {
   // device copy of column indices for gathers
   int *idx_buf;
   uint64_t idx_cap;
-  // learned preconditioner: stencil parameters (w, d, v), the stencil
-  // branch's pointwise scale p_K p_M, features rho, workspace
+  // learned preconditioner: stencil parameters (w, d, v), features rho,
+  // workspace
   float *lp_conv;
-  float *lp_cs, *lp_rho;
+  float *lp_rho;
   void *lp_tmp;
   size_t lp_tmp_bytes;
+  double lp_s0;  // stencil-layer scale mean(p_K^2) / safe(mu)
   int has_lp;
 };
 

@@ -310,7 +310,6 @@ void bdgx_ctx_destroy(bdgx_ctx *c) {
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->idx_buf) (void)hipFree(c->idx_buf);
   if (c->lp_conv) (void)hipFree(c->lp_conv);
-  if (c->lp_cs) (void)hipFree(c->lp_cs);
   if (c->lp_rho) (void)hipFree(c->lp_rho);
   if (c->lp_tmp) (void)hipFree(c->lp_tmp);
   (void)hipEventDestroy(c->ev0);

@@ -466,6 +466,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
         !(st.worst < cfg->lp_factor * worst_hist[it - cfg->lp_window])) {
       lp_on = 0;  /* the network stopped helping: analytic T_M T_K from here on */
       res->learned_fallback_at = it;
+      np = 0;     /* and restart the search-direction history */
       if (cfg->verbosity > 0)
         fprintf(stderr, "[bdg] it %d: learned preconditioner stalled, back to T_M T_K\n", it);
     }

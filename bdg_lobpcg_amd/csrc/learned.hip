@@ -1,24 +1,25 @@
 // Learned preconditioner for the iLOBPCG search directions (FP32, real psi0).
 //
-//   N(r) = T'_M(T'_K(r)) + c sigma * sum_h v_h tanh(sum_nb w_{h,nb} r(x+nb)/sigma + d_h)
-//   T'_X v = a_X * F^-1[ g_X * F(b_X * v) ],  X in {K, M}
+//   N(r) = T'_M(V),  V = U + s0 sig sum_h v_h tanh(sum_nb w_{h,nb} U(x+nb)/(s0 sig) + d_h),
+//   U = T'_K(r),  T'_X v = a_X * F^-1[ g_X * F(b_X * v) ],  X in {K, M}
 //   a_X = p_X exp(alpha_X . phi),  b_X = p_X exp(beta_X . phi),  phi = (rho, rho^2)
 //   g_X(k) = exp(interp(kappa_X, |k|)) / (S * safe(mu + k^2/2))
+//   sig = rms(r) per column,  s0 = mean(p_K^2) / safe(mu)
 //
-// with sigma = rms(r) per column (N is positively homogeneous in r, like the
-// preconditioner it replaces) and c = (p_K p_M)^2 / safe(mu)^2, the low-k
-// scale of T_M T_K.  Two spectral layers that generalise the analytic
-// preconditioner T_M T_K
-// (reference src/precond_impl.inc:21-111; it is N at theta = 0 with v = 0),
-// plus a 3x3x3 stencil layer with HID hidden channels.  The spectral layers run
-// through the same FFT pipeline as every operator (pre-weight, k-space table,
-// post-weight); the stencil layer is one fused kernel that reads the 27
-// neighbours of each point once per hidden channel and never stores the
-// hidden activations.  The parameter gradient (forward recomputed, then the
-// adjoint: the filters are self-adjoint because g is real and even in k)
-// needs two more table filters, two pairs of forward spectra for the kappa
-// gradients, and reductions; everything runs on the device and the sums are
-// accumulated in FP64.
+// A spectral layer, a 3x3x3 stencil layer with HID hidden channels and a
+// residual connection, and a second spectral layer.  The spectral layers
+// generalise the analytic preconditioner T_M T_K (reference
+// src/precond_impl.inc:21-111; N is exactly T_M T_K at theta = 0 with v = 0)
+// and run through the same FFT pipeline as every operator (pre-weight,
+// k-space table, post-weight).  The stencil acts on the already smoothed U and
+// its output is smoothed again by T'_M, so it cannot put unfiltered high-k
+// content into a search direction; N is positively homogeneous in r.  The
+// stencil kernels read the 27 neighbours of a point once per hidden channel
+// and never store the hidden activations (the adjoint stores one channel at a
+// time).  The parameter gradient recomputes the forward and runs the adjoint
+// (the filters are self-adjoint because g is real and even in k): two more
+// table filters, two pairs of forward spectra for the kappa gradients, the
+// transposed stencil, and reductions accumulated in FP64, all on the device.
 #include <math.h>
 #include <string.h>
 
@@ -68,9 +69,25 @@ __global__ void k_lp_max(const float *x, long long n, unsigned int *out) {
   if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
 }
 
+// out[0] += sum x^2
+__global__ void k_lp_sumsq(const float *x, long long n, double *out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    s += (double)x[i] * x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
 __global__ void k_lp_weights(LpCoef cf, const float *pK, const float *pM, float *rho,
-                             const unsigned int *rmax, long long S, float inv_mu2, float *aK,
-                             float *bK, float *aM, float *bM, float *cs) {
+                             const unsigned int *rmax, long long S, float *aK, float *bK,
+                             float *aM, float *bM) {
   const float inv = 1.0f / fmaxf(__uint_as_float(*rmax), 1e-30f);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < S;
        i += (long long)gridDim.x * blockDim.x) {
@@ -81,8 +98,6 @@ __global__ void k_lp_weights(LpCoef cf, const float *pK, const float *pM, float 
     bK[i] = pK[i] * expf(cf.bK[0] * r + cf.bK[1] * r2);
     aM[i] = pM[i] * expf(cf.aM[0] * r + cf.aM[1] * r2);
     bM[i] = pM[i] * expf(cf.bM[0] * r + cf.bM[1] * r2);
-    const float pp = pK[i] * pM[i];
-    cs[i] = pp * pp * inv_mu2;
   }
 }
 
@@ -143,10 +158,10 @@ __device__ __forceinline__ long long lp_nb(long long i, int dx, int dy, int dz, 
   return ((long long)zz * g.ny + yy) * g.nx + xx;
 }
 
-// W[:, col] += cs * sum_h v_h tanh(z_h), z_h = sum_nb w_{h,nb} r(x+nb) + d_h
-__global__ void __launch_bounds__(256) k_lp_conv_fwd(const float *R, long long ld, int ncols,
-                                                     Geo g, const float *prm, const float *cs,
-                                                     const float *sig, float *W) {
+// V = U + s0 sig_col sum_h v_h tanh(z_h),  z_h = sum_nb w_{h,nb} U(x+nb)/(s0 sig_col) + d_h
+__global__ void __launch_bounds__(256) k_lp_stencil_fwd(const float *U, int ncols, Geo g,
+                                                        const float *prm, float s0,
+                                                        const float *sig, float *V) {
   __shared__ float sp[LP_NCONV];
   for (int i = threadIdx.x; i < LP_NCONV; i += blockDim.x) sp[i] = prm[i];
   __syncthreads();
@@ -156,13 +171,13 @@ __global__ void __launch_bounds__(256) k_lp_conv_fwd(const float *R, long long l
        e += (long long)gridDim.x * blockDim.x) {
     const int col = (int)(e / g.S);
     const long long i = e - (long long)col * g.S;
-    const float *r = R + (long long)col * ld;
-    const float sg = sig[col], isg = 1.f / sg;
+    const float *u = U + (long long)col * g.S;
+    const float sc = s0 * sig[col], isc = 1.f / sc;
     float nb[27];
     int n = 0;
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) nb[n++] = r[lp_nb(i, dx, dy, dz, g)] * isg;
+        for (int dx = -1; dx <= 1; ++dx) nb[n++] = u[lp_nb(i, dx, dy, dz, g)] * isc;
     float s = 0.f;
 #pragma unroll 4
     for (int h = 0; h < LPH; ++h) {
@@ -171,16 +186,18 @@ __global__ void __launch_bounds__(256) k_lp_conv_fwd(const float *R, long long l
       for (int q = 0; q < 27; ++q) z += w[h * 27 + q] * nb[q];
       s += v[h] * tanhf(z);
     }
-    W[i + (long long)col * ld] += cs[i] * sg * s;
+    V[e] = u[i] + sc * s;
   }
 }
 
-// d(w, d, v) of sum(dW * cs * s): grid.y = hidden channel h
-__global__ void __launch_bounds__(256) k_lp_conv_bwd(const float *R, const float *dW, long long ld,
-                                                     int ncols, Geo g, const float *prm,
-                                                     const float *cs, const float *sig,
-                                                     double *grad) {
-  const int h = blockIdx.y;
+// Parameter gradient of the stencil layer for hidden channel h = blockIdx.y,
+// and Q = dV v_h (1 - tanh(z_h)^2) for channel `qh` (when Q != null), the
+// input of the transposed stencil.
+__global__ void __launch_bounds__(256) k_lp_stencil_bwd(const float *U, const float *dV,
+                                                        int ncols, Geo g, const float *prm,
+                                                        float s0, const float *sig,
+                                                        double *grad, int qh, float *Q) {
+  const int h = Q ? qh : blockIdx.y;
   const float *w = prm + h * 27;
   const float dh = prm[27 * LPH + h], vh = prm[28 * LPH + h];
   float gw[27], gd = 0.f, gv = 0.f;
@@ -191,28 +208,33 @@ __global__ void __launch_bounds__(256) k_lp_conv_bwd(const float *R, const float
        e += (long long)gridDim.x * blockDim.x) {
     const int col = (int)(e / g.S);
     const long long i = e - (long long)col * g.S;
-    const float *r = R + (long long)col * ld;
-    const float sg = sig[col], isg = 1.f / sg;
+    const float *u = U + (long long)col * g.S;
+    const float sc = s0 * sig[col], isc = 1.f / sc;
     float nb[27];
     int n = 0;
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) nb[n++] = r[lp_nb(i, dx, dy, dz, g)] * isg;
+        for (int dx = -1; dx <= 1; ++dx) nb[n++] = u[lp_nb(i, dx, dy, dz, g)] * isc;
     float z = dh;
 #pragma unroll
     for (int q = 0; q < 27; ++q) z += w[q] * nb[q];
     const float t = tanhf(z);
-    const float ds = cs[i] * sg * dW[i + (long long)col * ld];
-    gv += ds * t;
-    const float gz = ds * vh * (1.f - t * t);
-    gd += gz;
+    const float dvv = dV[e];
+    const float q = dvv * vh * (1.f - t * t);
+    if (Q) {
+      Q[e] = q;
+      continue;
+    }
+    gv += sc * dvv * t;
+    gd += sc * q;
 #pragma unroll
-    for (int q = 0; q < 27; ++q) gw[q] += gz * nb[q];
+    for (int k = 0; k < 27; ++k) gw[k] += sc * q * nb[k];
   }
-  // block sum of the 29 partials (two rounds through LDS)
+  if (Q) return;
+  // block sum of the 29 partials through LDS
   __shared__ float red[256];
-  for (int q = 0; q < 29; ++q) {
-    const float val = q < 27 ? gw[q] : (q == 27 ? gd : gv);
+  for (int k = 0; k < 29; ++k) {
+    const float val = k < 27 ? gw[k] : (k == 27 ? gd : gv);
     red[threadIdx.x] = val;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
@@ -220,11 +242,32 @@ __global__ void __launch_bounds__(256) k_lp_conv_bwd(const float *R, const float
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      const int o = q < 27 ? LP_OFF_W + h * 27 + q
-                           : (q == 27 ? LP_OFF_W + 27 * LPH + h : LP_OFF_W + 28 * LPH + h);
+      const int o = k < 27 ? LP_OFF_W + h * 27 + k
+                           : (k == 27 ? LP_OFF_W + 27 * LPH + h : LP_OFF_W + 28 * LPH + h);
       atomicAdd(grad + o, (double)red[0]);
     }
     __syncthreads();
+  }
+}
+
+// dU(y) += sum_nb w_{h,nb} Q(y - nb)   (transposed stencil of channel h)
+__global__ void __launch_bounds__(256) k_lp_stencil_t(const float *Q, int ncols, Geo g,
+                                                      const float *prm, int h, float *dU) {
+  __shared__ float ws[27];
+  if (threadIdx.x < 27) ws[threadIdx.x] = prm[h * 27 + threadIdx.x];
+  __syncthreads();
+  const long long total = g.S * ncols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(e / g.S);
+    const long long i = e - (long long)col * g.S;
+    const float *q = Q + (long long)col * g.S;
+    float acc = 0.f;
+    int n = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) acc += ws[n++] * q[lp_nb(i, -dx, -dy, -dz, g)];
+    dU[e] += acc;
   }
 }
 
@@ -367,7 +410,6 @@ int bdgx_lp_set_params(bdgx_ctx *c, const float *theta) {
     if (!rc) rc = lp_alloc(&F.lpb[w], (size_t)S);
     if (!rc) rc = lp_alloc(&F.lpg[w], (size_t)c->geo.Kk);
   }
-  if (!rc) rc = lp_alloc(&c->lp_cs, (size_t)S);
   if (!rc) rc = lp_alloc(&c->lp_rho, (size_t)S);
   if (!rc) rc = lp_alloc(&c->lp_conv, (size_t)LP_NCONV);
   if (rc) return rc;
@@ -381,20 +423,24 @@ int bdgx_lp_set_params(bdgx_ctx *c, const float *theta) {
   BDGX_HIP(hipStreamSynchronize(c->stream));  // earlier applies may read the tables
   BDGX_HIP(hipMemcpyAsync(c->lp_conv, theta + LP_OFF_W, sizeof(float) * LP_NCONV,
                           hipMemcpyHostToDevice, c->stream));
-  rc = ensure_scratch(c, sizeof(unsigned int));
+  rc = ensure_scratch(c, 2 * sizeof(double));
   if (rc) return rc;
   unsigned int *dmax = reinterpret_cast<unsigned int *>(c->scratch);
-  BDGX_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned int), c->stream));
+  double *dsum = reinterpret_cast<double *>(c->scratch) + 1;
+  BDGX_HIP(hipMemsetAsync(c->scratch, 0, 2 * sizeof(double), c->stream));
   hipLaunchKernelGGL(k_lp_rho, dim3(grid_for(S)), dim3(256), 0, c->stream, F.wf, S, c->lp_rho);
   hipLaunchKernelGGL(k_lp_max, dim3(grid_for(S)), dim3(256), 0, c->stream, c->lp_rho, S, dmax);
-  const double smu = fabs(c->mu) < 1e-8 ? 1e-8 : c->mu;
+  hipLaunchKernelGGL(k_lp_sumsq, dim3(grid_for(S)), dim3(256), 0, c->stream, F.pK, S, dsum);
   hipLaunchKernelGGL(k_lp_weights, dim3(grid_for(S)), dim3(256), 0, c->stream, cf, F.pK, F.pM,
-                     c->lp_rho, dmax, S, (float)(1.0 / (smu * smu)), F.lpa[0], F.lpb[0], F.lpa[1],
-                     F.lpb[1], c->lp_cs);
+                     c->lp_rho, dmax, S, F.lpa[0], F.lpb[0], F.lpa[1], F.lpb[1]);
   hipLaunchKernelGGL(k_lp_tables, dim3(grid_for(c->geo.Kk)), dim3(256), 0, c->stream, cf, c->geo,
                      F.kx2[0], F.kx2[1], F.kx2[2], c->mu, lp_kmax(c), F.lpg[0], F.lpg[1]);
   BDGX_HIP(hipGetLastError());
+  double hsum = 0.0;
+  BDGX_HIP(hipMemcpyAsync(&hsum, dsum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   BDGX_HIP(hipStreamSynchronize(c->stream));
+  const double smu = fabs(c->mu) < 1e-8 ? 1e-8 : c->mu;
+  c->lp_s0 = hsum / (double)S / smu;
   c->has_lp = 1;
   return BDGX_OK;
 }
@@ -404,23 +450,22 @@ int bdgx_lp_apply(bdgx_ctx *c, int prec, const void *dR, void *dW, uint64_t ncol
     return BDGX_EINVAL;
   if (!c->has_lp) return BDGX_ESTATE;
   if (ncols == 0) return BDGX_OK;
-  const long long S = c->geo.S;
-  int rc = ensure_lp_tmp(c, sizeof(float) * ((size_t)S * ncols + ncols));
+  const long long S = c->geo.S, n = (long long)ncols;
+  // U, V (S x n each) and sig (n)
+  int rc = ensure_lp_tmp(c, sizeof(float) * ((size_t)S * n * 2 + n));
   if (rc) return rc;
   float *U = reinterpret_cast<float *>(c->lp_tmp);
-  float *sig = U + (size_t)S * ncols;
+  float *V = U + S * n, *sig = V + S * n;
   const float *R = reinterpret_cast<const float *>(dR);
   float *W = reinterpret_cast<float *>(dW);
-  hipLaunchKernelGGL(k_lp_colnorm, dim3((unsigned)ncols), dim3(256), 0, c->stream, R,
-                     (long long)ld, S, sig);
-  rc = lp_filter_f32(c, 0, 0, R, U, (long long)ncols, S);
-  if (!rc) rc = lp_filter_f32(c, 1, 0, U, W, (long long)ncols, (long long)ld);
+  hipLaunchKernelGGL(k_lp_colnorm, dim3((unsigned)n), dim3(256), 0, c->stream, R, S, S, sig);
+  rc = lp_filter_f32(c, 0, 0, R, U, n, S);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_lp_conv_fwd, dim3(grid_for(S * (long long)ncols)), dim3(256), 0, c->stream,
-                     R, (long long)ld, (int)ncols, c->geo, c->lp_conv, c->lp_cs,
-                     (const float *)sig, W);
+  hipLaunchKernelGGL(k_lp_stencil_fwd, dim3(grid_for(S * n)), dim3(256), 0, c->stream,
+                     (const float *)U, (int)n, c->geo, c->lp_conv, (float)c->lp_s0,
+                     (const float *)sig, V);
   BDGX_HIP(hipGetLastError());
-  return BDGX_OK;
+  return lp_filter_f32(c, 1, 0, V, W, n, S);
 }
 
 int bdgx_lp_backward(bdgx_ctx *c, const float *dR, const float *dGW, uint64_t ncols, uint64_t ld,
@@ -428,71 +473,87 @@ int bdgx_lp_backward(bdgx_ctx *c, const float *dR, const float *dGW, uint64_t nc
   if (!c || !dR || !dGW || !h_grad || c->cplx || ld != (uint64_t)c->geo.S) return BDGX_EINVAL;
   if (!c->has_lp) return BDGX_ESTATE;
   if (ncols == 0) return BDGX_OK;
-  const long long S = c->geo.S, n = (long long)ncols, L = (long long)ld;
+  const long long S = c->geo.S, n = (long long)ncols;
   Fields<float> &F = c->f32;
-  // U, Wsp, T1, T2 (S x n each, ld S) and X2 (2S x n)
-  int rc = ensure_lp_tmp(c, sizeof(float) * ((size_t)S * n * 6 + n));
+  // U, V, Wsp, dV, dU, T, Q (S x n each), X2 (2S x n), sig (n)
+  int rc = ensure_lp_tmp(c, sizeof(float) * ((size_t)S * n * 9 + n));
   if (rc) return rc;
   float *U = reinterpret_cast<float *>(c->lp_tmp);
-  float *Wsp = U + S * n, *T1 = Wsp + S * n, *T2 = T1 + S * n, *X2 = T2 + S * n;
-  float *sig = X2 + 2 * S * n;
-  hipLaunchKernelGGL(k_lp_colnorm, dim3((unsigned)n), dim3(256), 0, c->stream, dR, L, S, sig);
+  float *V = U + S * n, *Wsp = V + S * n, *dV = Wsp + S * n, *dU = dV + S * n;
+  float *T = dU + S * n, *Q = T + S * n, *X2 = Q + S * n, *sig = X2 + 2 * S * n;
   rc = ensure_scratch(c, sizeof(double) * BDGX_LP_NPARAM);
   if (rc) return rc;
   double *dg = reinterpret_cast<double *>(c->scratch);
   BDGX_HIP(hipMemsetAsync(dg, 0, sizeof(double) * BDGX_LP_NPARAM, c->stream));
   const unsigned gb = grid_for(S * n);
-  const float kmax = lp_kmax(c);
+  const float kmax = lp_kmax(c), s0 = (float)c->lp_s0;
+  const hipStream_t st = c->stream;
   // forward intermediates
-  rc = lp_filter_f32(c, 0, 0, dR, U, n, L);
-  if (!rc) rc = lp_filter_f32(c, 1, 0, U, Wsp, n, S);
-  // stencil branch
+  hipLaunchKernelGGL(k_lp_colnorm, dim3((unsigned)n), dim3(256), 0, st, dR, S, S, sig);
+  rc = lp_filter_f32(c, 0, 0, dR, U, n, S);
   if (!rc) {
-    hipLaunchKernelGGL(k_lp_conv_bwd, dim3(gb > 1024 ? 1024 : gb, LPH), dim3(256), 0, c->stream,
-                       dR, dGW, L, (int)n, c->geo, c->lp_conv, c->lp_cs, (const float *)sig, dg);
-    if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
+    hipLaunchKernelGGL(k_lp_stencil_fwd, dim3(gb), dim3(256), 0, st, (const float *)U, (int)n,
+                       c->geo, c->lp_conv, s0, (const float *)sig, V);
+    rc = lp_filter_f32(c, 1, 0, V, Wsp, n, S);
   }
-  // M stage: alpha_M, beta_M, kappa_M; dU = b_M * C_M(a_M dW)
+  // M layer: alpha_M, beta_M, kappa_M; dV = b_M * C_M(a_M dW)
   if (!rc) {
-    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, c->stream, dGW, Wsp,
-                       (const float *)nullptr, c->lp_rho, S, (int)n, L, dg + 4);
-    rc = lp_filter_f32(c, 1, 1, dGW, T1, n, L);
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, st, dGW, (const float *)Wsp,
+                       (const float *)nullptr, c->lp_rho, S, (int)n, S, dg + 4);
+    rc = lp_filter_f32(c, 1, 1, dGW, dV, n, S);
   }
   if (!rc) {
-    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, c->stream, U, T1,
-                       (const float *)F.lpb[1], c->lp_rho, S, (int)n, S, dg + 6);
-    hipLaunchKernelGGL(k_lp_pack2, dim3(gb), dim3(256), 0, c->stream, dGW, U, S, (int)n, L, X2);
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, st, (const float *)V,
+                       (const float *)dV, (const float *)F.lpb[1], c->lp_rho, S, (int)n, S, dg + 6);
+    hipLaunchKernelGGL(k_lp_pack2, dim3(gb), dim3(256), 0, st, dGW, (const float *)V, S, (int)n, S,
+                       X2);
     rc = lp_spectra_f32(c, 1, X2, n, 2 * S);
   }
   if (!rc) {
-    hipLaunchKernelGGL(k_lp_knotgrad, dim3(grid_for(c->geo.Kk * n)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_lp_knotgrad, dim3(grid_for(c->geo.Kk * n)), dim3(256), 0, st,
                        reinterpret_cast<const cx<float> *>(c->spec), (int)n, c->geo, F.kx2[0],
                        F.kx2[1], F.kx2[2], kmax, (const float *)F.lpg[1], dg + 8 + LPK);
-    hipLaunchKernelGGL(k_lp_mulw, dim3(gb), dim3(256), 0, c->stream, T1, (const float *)F.lpb[1],
-                       S, (int)n, S);
-    // K stage with upstream dU = T1
-    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, c->stream, T1, U,
-                       (const float *)nullptr, c->lp_rho, S, (int)n, S, dg + 0);
-    rc = lp_filter_f32(c, 0, 1, T1, T2, n, S);
+    hipLaunchKernelGGL(k_lp_mulw, dim3(gb), dim3(256), 0, st, dV, (const float *)F.lpb[1], S,
+                       (int)n, S);
+    // stencil layer: parameters, then dU = dV + transposed stencil of each channel
+    hipLaunchKernelGGL(k_lp_stencil_bwd, dim3(gb > 1024 ? 1024 : gb, LPH), dim3(256), 0, st,
+                       (const float *)U, (const float *)dV, (int)n, c->geo, c->lp_conv, s0,
+                       (const float *)sig, dg, 0, (float *)nullptr);
+    if (hipMemcpyAsync(dU, dV, sizeof(float) * S * n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      rc = BDGX_EHIP;
+    for (int h = 0; h < LPH && !rc; ++h) {
+      hipLaunchKernelGGL(k_lp_stencil_bwd, dim3(gb), dim3(256), 0, st, (const float *)U,
+                         (const float *)dV, (int)n, c->geo, c->lp_conv, s0, (const float *)sig, dg,
+                         h, Q);
+      hipLaunchKernelGGL(k_lp_stencil_t, dim3(gb), dim3(256), 0, st, (const float *)Q, (int)n,
+                         c->geo, c->lp_conv, h, dU);
+    }
+    if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
+  }
+  // K layer with upstream dU
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, st, (const float *)dU,
+                       (const float *)U, (const float *)nullptr, c->lp_rho, S, (int)n, S, dg + 0);
+    rc = lp_filter_f32(c, 0, 1, dU, T, n, S);
   }
   if (!rc) {
-    // sum R T2 b_K phi: R has leading dimension L, T2 has S; pack R first
-    hipLaunchKernelGGL(k_lp_pack2, dim3(gb), dim3(256), 0, c->stream, T1, dR, S, (int)n, L, X2);
-    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, c->stream, dR, T2,
-                       (const float *)F.lpb[0], c->lp_rho, S, (int)n, L, dg + 2);
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, st, dR, (const float *)T,
+                       (const float *)F.lpb[0], c->lp_rho, S, (int)n, S, dg + 2);
+    hipLaunchKernelGGL(k_lp_pack2, dim3(gb), dim3(256), 0, st, (const float *)dU, dR, S, (int)n, S,
+                       X2);
     rc = lp_spectra_f32(c, 0, X2, n, 2 * S);
   }
   if (!rc) {
-    hipLaunchKernelGGL(k_lp_knotgrad, dim3(grid_for(c->geo.Kk * n)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_lp_knotgrad, dim3(grid_for(c->geo.Kk * n)), dim3(256), 0, st,
                        reinterpret_cast<const cx<float> *>(c->spec), (int)n, c->geo, F.kx2[0],
                        F.kx2[1], F.kx2[2], kmax, (const float *)F.lpg[0], dg + 8);
     if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
   }
   std::vector<double> hg(BDGX_LP_NPARAM, 0.0);
   if (!rc && hipMemcpyAsync(hg.data(), dg, sizeof(double) * BDGX_LP_NPARAM, hipMemcpyDeviceToHost,
-                            c->stream) != hipSuccess)
+                            st) != hipSuccess)
     rc = BDGX_EHIP;
-  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = BDGX_EHIP;
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = BDGX_EHIP;
   if (rc) return rc;
   for (int i = 0; i < BDGX_LP_NPARAM; ++i) h_grad[i] = (float)hg[i];
   return BDGX_OK;

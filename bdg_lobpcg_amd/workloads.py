@@ -81,3 +81,28 @@ def make_problem(cfg, psi, mu, precision=0, maxiter=1000, tol=None, nev=None, m=
     p.set_precision(precision)
     p.set_init_uniform(42)
     return p
+
+
+def train_learned(cfg, steps=300, lr=1e-3, reg=0.3, snapshots=(4, 8, 14, 22, 32, 48, 70),
+                  log=None):
+    """Learned-preconditioner parameters for cfg's grid, trained on the GPU on
+    pairs from a related problem (g x 0.75, eps_dd x 0.875: same grid and trap,
+    different condensate), so that the evaluation problem is not in the
+    training set.  Returns (theta, seconds, info)."""
+    from bdg_lobpcg_amd import learned as LP
+
+    t0 = time.perf_counter()
+    tcfg = dict(cfg, g=cfg["g"] * 0.75, edd=cfg["edd"] * 0.875)
+    psi_t, mu_t, _ = ground_state(tcfg)
+    mk = lambda maxiter: make_problem(tcfg, psi_t, mu_t, maxiter=maxiter)
+    ref, d, snaps = LP.collect(mk, tcfg["nev"], snapshots=snapshots)
+    th0 = LP.init_theta(0)
+    loss0 = LP.evaluate(d, th0, snaps)
+    th, hist = LP.train(d, snaps, th0, steps=steps, lr=lr, reg=reg, log=log)
+    loss1 = LP.evaluate(d, th, snaps)
+    d.close()
+    ref.close()
+    return th, time.perf_counter() - t0, dict(loss_analytic=loss0, loss_learned=loss1,
+                                              steps=steps, lr=lr, reg=reg,
+                                              snapshots=list(snapshots),
+                                              train_problem=f"g={tcfg['g']}, eps_dd={tcfg['edd']}")

@@ -1,13 +1,19 @@
 """Benchmark: time-to-solution for 32 BdG modes at 128^3 on one MI355X.
 
-One step = one full bdg_solve (plain FP64 policy and mixed FP32 + FP64
-refinement policy) of the 128^3 dipolar cigar (real psi0, nev = 32, block
-32, tol 1e-8) with psi0 and all fields resident on the device.  `value` is
-the mixed policy's mean time-to-solution; the FP64 policy's time and the
-speed-up are reported beside it.  The FFT-pass roofline is measured live
+One step = one full bdg_solve of the 128^3 dipolar cigar (real psi0, nev = 32,
+block 32, tol 1e-8) with psi0 and all fields resident on the device, under
+three policies: plain FP64 (the reference's algorithm), mixed (preconditioner
+in FP32), and mixed + learned (the learned preconditioner, trained on the GPU
+beforehand on pairs from a related problem; its training time is reported
+separately).  `value` is the mixed + learned time-to-solution; the FP64 and
+mixed times, iteration counts and the eigenvalue error against FP64 are
+reported beside it.  The roofline of the dominant kernel is measured live
 with HIP events on the context stream (bdgx_bench_pass); the CPU baseline is
 the FP64 C oracle (one vector at a time, as the reference) on a bounded
 sample, scaled by the column applies of the FP64 device solve.
+
+`--probe` only launches the dominant kernel and a copy kernel of known size a
+few times (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE traffic runs).
 """
 from __future__ import annotations
 
@@ -36,13 +42,15 @@ def rel_err_modes(w, ref, zero):
     return e_rel, e_abs0
 
 
-def solve_timed(cfg, psi, mu, precision, steps, warmup):
+def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None):
     from bdg_lobpcg_amd.workloads import make_problem
 
     times, last = [], None
     for i in range(warmup + steps):
         p = make_problem(cfg, psi, mu, precision=precision)
         p.set_verbosity(1)
+        if theta is not None:
+            p.set_learned(theta)
         t0 = time.perf_counter()
         rc = p.solve()
         t = time.perf_counter() - t0
@@ -50,9 +58,10 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup):
             raise RuntimeError(f"bdg_solve failed with {rc}")
         st = p.stats()
         w = p.eigenvalues(cfg["nev"])
-        log(f"precision={precision} solve {i} ({'warmup' if i < warmup else 'timed'}): {t:.3f} s,"
+        tag = "mixed+learned" if theta is not None else ("mixed" if precision else "fp64")
+        log(f"{tag} solve {i} ({'warmup' if i < warmup else 'timed'}): {t:.3f} s,"
             f" iters {st['iterations']}, column applies {st['column_applies']}"
-            f" (fp32 {st['column_applies_fp32']}), refreshes {st['image_refreshes']},"
+            f" (fp32 {st['column_applies_fp32']}), learned iterations {st['learned_iterations']},"
             f" converged {p.converged()}/{cfg['nev']}")
         if i >= warmup:
             times.append(t)
@@ -62,17 +71,47 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup):
     return float(np.mean(times)), last
 
 
+PASSES = ["xfwd", "yfwd", "z", "yinv", "xinv", "plane_fwd", "plane_inv"]
+KERNELS = {"plane_fwd": "k_plane_fwd_real<double, 128, 512> (x+y forward, whole z-planes)",
+           "plane_inv": "k_plane_inv_real<double, 128, 512> (y+x inverse + combine)",
+           "z": "k_zpass_fixed<double, 128, 8, 256> (z forward, k-space multiply, z inverse)"}
+
+
 def roofline(cfg):
+    """Per-launch time and algorithmic bytes of every FFT pass kind (32
+    spectrum batches, 10 launches between HIP events on the context stream)."""
     import bdg_lobpcg_amd as B
 
     d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
     out = {}
     for prec, name in ((B.FP32, "fp32"), (B.FP64, "fp64")):
-        for pid, pname in enumerate(["xfwd", "yfwd", "z", "yinv", "xinv"]):
-            ms, by = d.bench_pass(pid, prec, cfg["m"], iters=10)
+        for pid, pname in enumerate(PASSES):
+            try:
+                ms, by = d.bench_pass(pid, prec, cfg["m"], iters=10)
+            except B.BdgError:
+                continue
             out[f"{name}_{pname}"] = dict(ms=ms, bytes=by, GBs=by / (ms * 1e-3) / 1e9)
     d.close()
     return out
+
+
+def probe(cfg):
+    """Launch the dominant kernel (FP64 whole-plane forward, 32 batches) and
+    a 1 GiB device copy (k_convert<double, double>) three times each."""
+    import ctypes as C
+
+    import bdg_lobpcg_amd as B
+
+    d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
+    d.bench_pass(PASSES.index("plane_fwd"), B.FP64, cfg["m"], iters=3)
+    n = (1 << 30) // 8
+    src, dst = B.DeviceBuffer(8 * n), B.DeviceBuffer(8 * n)
+    for _ in range(3):
+        B.check(B.lib().bdgx_convert(d.h, B.FP64, src.ptr, n, B.FP64, dst.ptr, n, n, 1), "copy")
+    d.sync()
+    print(json.dumps({"probe": "plane_fwd fp64 x3 (+1 warm-up), copy 1 GiB x3",
+                      "copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n}), flush=True)
+    d.close()
 
 
 def cpu_baseline(cfg, psi, arrays, column_applies):
@@ -103,66 +142,83 @@ def cpu_baseline(cfg, psi, arrays, column_applies):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg1_128c_d")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the solver runs on a single GPU "
               "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import bdg_lobpcg_amd as B  # noqa: F401  (fails loudly without the native library)
-    from bdg_lobpcg_amd.workloads import CONFIGS, ground_state
+    from bdg_lobpcg_amd.workloads import CONFIGS, ground_state, train_learned
 
     cfg = CONFIGS[args.config]
+    if args.probe:
+        probe(cfg)
+        return
     log(f"config {args.config}: N={cfg['N']} nev={cfg['nev']} ITP {cfg['itp_steps']} steps")
     psi, mu, t_itp = ground_state(cfg)
     log(f"psi0 by device ITP: {t_itp:.2f} s, mu = {mu:.10f}")
     t64, r64 = solve_timed(cfg, psi, mu, 0, args.steps, args.warmup)
     tmx, rmx = solve_timed(cfg, psi, mu, 1, args.steps, args.warmup)
-    zero = r64["zero"] | rmx["zero"]
-    e_rel, e_abs0 = rel_err_modes(rmx["w"], r64["w"], zero)
-    log(f"FP64 {t64:.3f} s, mixed {tmx:.3f} s, speed-up {t64 / tmx:.2f}x, "
-        f"max rel err mixed vs FP64 {e_rel:.2e}")
+    theta, t_train, tinfo = train_learned(cfg)
+    log(f"learned preconditioner trained in {t_train:.1f} s: {tinfo}")
+    tlr, rlr = solve_timed(cfg, psi, mu, 1, args.steps, args.warmup, theta=theta)
+    zero = r64["zero"] | rlr["zero"] | rmx["zero"]
+    e_rel, e_abs0 = rel_err_modes(rlr["w"], r64["w"], zero)
+    e_rel_mx, _ = rel_err_modes(rmx["w"], r64["w"], zero)
+    log(f"FP64 {t64:.3f} s, mixed {tmx:.3f} s ({t64 / tmx:.2f}x), mixed+learned {tlr:.3f} s "
+        f"({t64 / tlr:.2f}x); max rel err vs FP64: mixed {e_rel_mx:.2e}, mixed+learned {e_rel:.2e}")
     rf = roofline(cfg)
-    dom = rf["fp32_z"]
     log("passes: " + ", ".join(f"{k} {v['GBs']:.0f} GB/s" for k, v in rf.items()))
+    dom_key = "fp64_plane_fwd" if "fp64_plane_fwd" in rf else "fp64_z"
+    dom = rf[dom_key]
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, psi, r64["arrays"], r64["stats"]["column_applies"])
         log(f"cpu baseline {cpu['value']:.1f} s ({cpu['cores']} threads)")
     line = {
         "metric": "time-to-solution, 32 BdG modes at 128^3: mixed+learned vs plain FP64 on GPU",
-        "value": tmx,
+        "value": tlr,
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": tmx * 1e3,
+        "ms_per_step": tlr * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64+f32",
         "data": "synthetic (psi0 from device imaginary-time evolution of a Gaussian)",
-        "config": {"workload": f"{args.config}: 128^3 dipolar cigar, real psi0, nev 32, block 32, "
-                               f"tol 1e-8, g 2000, eps_dd 0.8", "grid": list(cfg["N"]),
+        "config": {"workload": f"{args.config}: {'x'.join(map(str, cfg['N']))} dipolar cigar, "
+                               f"real psi0, nev {cfg['nev']}, block {cfg['m']}, tol {cfg['tol']}, "
+                               f"g {cfg['g']}, eps_dd {cfg['edd']}", "grid": list(cfg["N"]),
                    "nev": cfg["nev"], "parallelism": "single GPU"},
         "fp64_seconds": t64,
         "mixed_seconds": tmx,
-        "speedup_vs_fp64": t64 / tmx,
-        "mixed_vs_fp64_max_rel_err": e_rel,
+        "mixed_learned_seconds": tlr,
+        "speedup_vs_fp64": t64 / tlr,
+        "speedup_mixed_only": t64 / tmx,
+        "mixed_learned_vs_fp64_max_rel_err": e_rel,
+        "mixed_vs_fp64_max_rel_err": e_rel_mx,
         "goldstone_abs_diff": e_abs0,
-        "iterations_fp64": r64["stats"]["iterations"],
-        "iterations_mixed": rmx["stats"]["iterations"],
-        "column_applies": [r64["stats"]["column_applies"], rmx["stats"]["column_applies"]],
-        "converged": [r64["conv"], rmx["conv"]],
+        "iterations": {"fp64": r64["stats"]["iterations"], "mixed": rmx["stats"]["iterations"],
+                       "mixed_learned": rlr["stats"]["iterations"]},
+        "column_applies": {"fp64": r64["stats"]["column_applies"],
+                           "mixed": rmx["stats"]["column_applies"],
+                           "mixed_learned": rlr["stats"]["column_applies"]},
+        "learned": dict(tinfo, training_seconds=t_train,
+                        learned_iterations=rlr["stats"]["learned_iterations"],
+                        fallback_at=rlr["stats"]["learned_fallback_at"]),
+        "converged": [r64["conv"], rmx["conv"], rlr["conv"]],
         "omega_fp64": [float(x) for x in r64["w"]],
         "psi0_seconds": t_itp,
-        "learned_preconditioner": "not built yet: the mixed policy uses the FFT-diagonal preconditioner",
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_zpass<float> (fused z forward, k-space multiply, z inverse)",
+                     "kernel": KERNELS.get(dom_key.split("_", 1)[1], dom_key),
                      "bytes_per_launch": dom["bytes"], "ms_per_launch": dom["ms"]},
         "passes": {k: round(v["GBs"], 1) for k, v in rf.items()},
         "cpu_baseline": cpu,

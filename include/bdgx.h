@@ -156,7 +156,8 @@ int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iter
 
 /* ---- learned preconditioner (no reference counterpart; its analytic base
  *      is precondK/precondM, reference src/precond_impl.inc:21-111) --------
- *   N(r) = T'_M(T'_K(r)) + p_K p_M sum_h v_h tanh(sum_nb w_{h,nb} r(x+nb) + d_h)
+ *   N(r) = T'_M(V),  V = U + s0 sig sum_h v_h tanh(sum_nb w_{h,nb} U(x+nb)/(s0 sig) + d_h),
+ *   U = T'_K(r),  sig = rms(r) per column,  s0 = mean(p_K^2) / safe(mu),
  *   T'_X v = a_X F^-1[g_X F(b_X v)],  a_X = p_X exp(alpha_X.phi),
  *   b_X = p_X exp(beta_X.phi),  phi = (rho, rho^2),  rho = psi0^2 / max psi0^2,
  *   g_X(k) = exp(interp(kappa_X, |k|)) / (S safe(mu + k^2/2))

@@ -10,14 +10,18 @@ weights from src/setup.c:150-177):
 
 and the network generalises it as
 
-    N(r) = T_M'(T_K'(r)) + c * sigma * sum_h v_h tanh(sum_nb w_{h,nb} r(x+nb)/sigma + d_h)
-    sigma = rms(r) per column (the network is positively homogeneous in r),
-    c = (p_K p_M)^2 / safe(mu)^2 (the low-k scale of T_M T_K)
+    N(r) = T_M'(V),  V = U + s0 sig * sum_h v_h tanh(sum_nb w_{h,nb} U(x+nb)/(s0 sig) + d_h),
+    U = T_K'(r),  sig = rms(r) per column,  s0 = mean(p_K^2) / safe(mu)
     T_X' v = a_X * S * irfft(g_X * rfft(b_X * v))
     a_X = p_X exp(alpha_X . phi),  b_X = p_X exp(beta_X . phi)
     g_X(k) = exp(interp(kappa_X, |k|)) / (S * safe(mu + k^2/2))
     phi = (rho, rho^2),  rho = |psi0|^2 / max |psi0|^2
 
+i.e. spectral layer, 3x3x3 stencil layer with HID hidden channels and a
+residual connection, spectral layer.  The stencil acts on the already
+smoothed U and its output is smoothed again by T_M', so it cannot add
+unfiltered high-k content to the search direction; N is positively
+homogeneous in r (sig depends on r only, s0 on the fields only).
 with nb over the 27 offsets of a 3x3x3 stencil (periodic), in the order
 idx = (dz+1)*9 + (dy+1)*3 + (dx+1), and kappa interpolated linearly on
 NKNOT knots spanning |k| in [0, max |k|].  At theta = 0 (except the stencil
@@ -83,7 +87,7 @@ class LearnedRef:
         self.pK = np.asarray(pK, float).reshape(self.shape)
         self.pM = np.asarray(pM, float).reshape(self.shape)
         smu = mu if abs(mu) >= 1e-8 else 1e-8
-        self.cs = (self.pK * self.pM) ** 2 / (smu * smu)
+        self.s0 = float(np.mean(self.pK ** 2)) / smu
         rho = np.asarray(wf, float).reshape(self.shape) ** 2
         rho = rho / rho.max()
         self.phi = np.stack([rho, rho ** 2])
@@ -116,18 +120,22 @@ class LearnedRef:
         return z + P["d"][:, None, None, None]
 
     # -- network ------------------------------------------------------------
+    def stencil(self, P, U, sig):
+        un = U / (self.s0 * sig)
+        th = np.tanh(self.conv_z(P, un))
+        return un, th, U + self.s0 * sig * np.tensordot(P["v"], th, axes=(0, 0))
+
     def forward(self, theta, R):
         """R: (S, ncols) -> W: (S, ncols)."""
-        P, T = unpack(theta), None
+        P = unpack(theta)
         T = self.tables(P)
         out = np.zeros_like(np.asarray(R, float))
         for j in range(R.shape[1]):
             r = R[:, j].reshape(self.shape)
-            u = T["aK"] * self.filt(T["gK"], T["bK"] * r)
-            w = T["aM"] * self.filt(T["gM"], T["bM"] * u)
             sig = np.sqrt(np.mean(r * r))
-            s = np.tensordot(P["v"], np.tanh(self.conv_z(P, r / sig)), axes=(0, 0))
-            out[:, j] = (w + self.cs * sig * s).ravel()
+            u = T["aK"] * self.filt(T["gK"], T["bK"] * r)
+            _, _, v = self.stencil(P, u, sig)
+            out[:, j] = (T["aM"] * self.filt(T["gM"], T["bM"] * v)).ravel()
         return out
 
     def grad(self, theta, R, dW):
@@ -138,37 +146,37 @@ class LearnedRef:
         for j in range(R.shape[1]):
             r = R[:, j].reshape(self.shape)
             dw = dW[:, j].reshape(self.shape)
+            sig = np.sqrt(np.mean(r * r))
             u = T["aK"] * self.filt(T["gK"], T["bK"] * r)
-            wsp = T["aM"] * self.filt(T["gM"], T["bM"] * u)
-            # M stage
+            un, th, v = self.stencil(P, u, sig)
+            wsp = T["aM"] * self.filt(T["gM"], T["bM"] * v)
+            # M layer
             g["aM"] += [np.sum(dw * wsp * f) for f in self.phi]
             tM = self.filt(T["gM"], T["aM"] * dw)
-            g["bM"] += [np.sum(u * tM * T["bM"] * f) for f in self.phi]
+            g["bM"] += [np.sum(v * tM * T["bM"] * f) for f in self.phi]
             A = np.fft.rfftn(T["aM"] * dw)
-            B = np.fft.rfftn(T["bM"] * u)
-            q = self.wt * np.real(np.conj(A) * B) * T["gM"]
-            g["kM"] += self.knot_sum(q)
-            du = T["bM"] * tM
-            # K stage
+            B = np.fft.rfftn(T["bM"] * v)
+            g["kM"] += self.knot_sum(self.wt * np.real(np.conj(A) * B) * T["gM"])
+            dv = T["bM"] * tM
+            # stencil layer (residual connection): parameters and dU
+            ds = self.s0 * sig * dv
+            g["v"] += np.tensordot(th, ds, axes=([1, 2, 3], [0, 1, 2]))
+            q = P["v"][:, None, None, None] * (1 - th ** 2) * dv[None]
+            g["d"] += self.s0 * sig * q.sum(axis=(1, 2, 3))
+            du = dv.copy()
+            for n, (dz, dy, dx) in enumerate(offsets()):
+                rs = np.roll(un, shift=(-dz, -dy, -dx), axis=(0, 1, 2))
+                g["w"][:, n] += self.s0 * sig * np.tensordot(q, rs, axes=([1, 2, 3], [0, 1, 2]))
+                # dU(y) += sum_h w_{h,nb} q_h(y - nb)
+                qs = np.roll(q, shift=(dz, dy, dx), axis=(1, 2, 3))
+                du += np.tensordot(P["w"][:, n], qs, axes=(0, 0))
+            # K layer
             g["aK"] += [np.sum(du * u * f) for f in self.phi]
             tK = self.filt(T["gK"], T["aK"] * du)
             g["bK"] += [np.sum(r * tK * T["bK"] * f) for f in self.phi]
             A = np.fft.rfftn(T["aK"] * du)
             B = np.fft.rfftn(T["bK"] * r)
-            q = self.wt * np.real(np.conj(A) * B) * T["gK"]
-            g["kK"] += self.knot_sum(q)
-            # stencil branch (sigma depends on r only)
-            sig = np.sqrt(np.mean(r * r))
-            rn = r / sig
-            z = self.conv_z(P, rn)
-            th = np.tanh(z)
-            ds = self.cs * sig * dw
-            g["v"] += np.tensordot(th, ds, axes=([1, 2, 3], [0, 1, 2]))
-            gz = P["v"][:, None, None, None] * (1 - th ** 2) * ds[None]
-            g["d"] += gz.sum(axis=(1, 2, 3))
-            for n, (dz, dy, dx) in enumerate(offsets()):
-                rs = np.roll(rn, shift=(-dz, -dy, -dx), axis=(0, 1, 2))
-                g["w"][:, n] += np.tensordot(gz, rs, axes=([1, 2, 3], [0, 1, 2]))
+            g["kK"] += self.knot_sum(self.wt * np.real(np.conj(A) * B) * T["gK"])
         return np.concatenate([g["aK"], g["bK"], g["aM"], g["bM"], g["kK"], g["kM"],
                                g["w"].ravel(), g["d"], g["v"]])
 
