@@ -446,20 +446,30 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
       Cs[i][j] = (i < nk && j0 + j < m) ? C[(kc + i) + (long long)(j0 + j) * k] : 0.0;
     }
     __syncthreads();
-    double av[4], an[4];
-    load_rows4(A, lda, kc + kk, k, row, nrows, vec_a, av);
-    for (int k4 = 0; k4 < nk; k4 += 4) {
-      if (k4 + 4 < nk) load_rows4(A, lda, kc + k4 + 4 + kk, k, row, nrows, vec_a, an);
-      double bv[MF];
+    // ring of PD k steps of A in flight ahead of the MFMAs
+    constexpr int PD = 4;
+    double av[PD][4];
 #pragma unroll
-      for (int f = 0; f < MF; ++f) bv[f] = Cs[k4 + kk][16 * f + c];
+    for (int q = 0; q < PD; ++q)
+      if (4 * q < nk) load_rows4(A, lda, kc + 4 * q + kk, k, row, nrows, vec_a, av[q]);
+    for (int k4 = 0; k4 < nk; k4 += 4 * PD) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
+      for (int q = 0; q < PD; ++q) {
+        const int kq = k4 + 4 * q;
+        if (kq >= nk) break;
+        double bv[MF];
 #pragma unroll
-        for (int f = 0; f < MF; ++f)
-          acc[t][f] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bv[f], acc[t][f], 0, 0, 0);
+        for (int f = 0; f < MF; ++f) bv[f] = Cs[kq + kk][16 * f + c];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) av[t] = an[t];
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int f = 0; f < MF; ++f)
+            acc[t][f] =
+                __builtin_amdgcn_mfma_f64_16x16x4f64(av[q][t], bv[f], acc[t][f], 0, 0, 0);
+        // refill this slot with the step PD ahead
+        if (kq + 4 * PD < nk)
+          load_rows4(A, lda, kc + kq + 4 * PD + kk, k, row, nrows, vec_a, av[q]);
+      }
     }
   }
 #pragma unroll
@@ -491,7 +501,7 @@ template <typename R>
 static int combine_real_mfma(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k,
                              const double *dC, void *dY, uint64_t ldy, uint64_t m,
                              uint64_t nrows) {
-  const int MF = m <= 16 ? 1 : (m <= 32 ? 2 : 4);
+  const int MF = m <= 16 ? 1 : (m <= 32 ? 2 : (m <= 48 ? 3 : 4));
   const unsigned gx = (unsigned)((nrows + 255) / 256);
   const unsigned gy = (unsigned)((m + 16 * MF - 1) / (16 * MF));
   const int vec_a = (lda % 4 == 0) && ((uintptr_t)dA % (4 * sizeof(R)) == 0);
@@ -504,6 +514,10 @@ static int combine_real_mfma(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t
                        vec_a, vec_y);
   else if (MF == 2)
     hipLaunchKernelGGL((k_combine_mfma<R, 2>), dim3(gx, gy), dim3(256), 0, c->stream, A,
+                       (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
+                       vec_a, vec_y);
+  else if (MF == 3)
+    hipLaunchKernelGGL((k_combine_mfma<R, 3>), dim3(gx, gy), dim3(256), 0, c->stream, A,
                        (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
                        vec_a, vec_y);
   else
