@@ -48,8 +48,32 @@ __device__ __forceinline__ int dif_pos(int k) {
   return p;
 }
 
+// LDS index of element e of the row-major x layout (rows of N complex
+// values), for ES-byte elements (0 = no swizzle).  A pass of 16 FP64 (32
+// FP32) complex values covers the 64 LDS banks once; within each such chunk
+// the low bits of e are XORed with g(h), h = chunk index mod 8:
+//   FP64: g(h) = 2h + (h & 1)            (chunks of 16)
+//   FP32: g(h) = (5h + 4 (h & 1)) mod 32 (chunks of 32)
+// With these the lanes of a pass hit distinct bank groups in every x stage
+// (radices 8, 8, 2 at N = 128), in the digit-reversed untangle and repack
+// and in the row loads (FP32: at most 2-way), where the plain layout has up
+// to 8-way conflicts.  The permutation stays inside its chunk, so a row keeps
+// its LDS footprint.
+template <int ES>
+__device__ __forceinline__ int swz(int e) {
+  if constexpr (ES == 16) {
+    const int h = (e >> 4) & 7;
+    return e ^ (2 * h + (h & 1));
+  } else if constexpr (ES == 8) {
+    const int h = (e >> 5) & 7;
+    return e ^ ((5 * h + 4 * (h & 1)) & 31);
+  } else {
+    return e;
+  }
+}
+
 template <typename R, bool INV, bool DIT, int N, int RAD, int L, int NP, int PST, int EST,
-          int T, bool PFAST>
+          int T, bool PFAST, int SWZ>
 __device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw) {
   constexpr int LR = L / RAD;
   constexpr int NBF = N / RAD;
@@ -59,10 +83,10 @@ __device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw) {
     const int p = PFAST ? t % NP : t / NBF;
     const int u = PFAST ? t / NP : t % NBF;
     const int blk = u / LR, m = u % LR;
-    cx<R> *base = buf + p * PST + (blk * L + m) * EST;
+    const int e0 = p * PST + (blk * L + m) * EST;
     cx<R> v[RAD];
 #pragma unroll
-    for (int j = 0; j < RAD; ++j) v[j] = base[j * LR * EST];
+    for (int j = 0; j < RAD; ++j) v[j] = buf[swz<SWZ>(e0 + j * LR * EST)];
     if (DIT && LR > 1 && m) {
 #pragma unroll
       for (int j = 1; j < RAD; ++j) {
@@ -81,28 +105,30 @@ __device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw) {
       }
     }
 #pragma unroll
-    for (int q = 0; q < RAD; ++q) base[q * LR * EST] = v[q];
+    for (int q = 0; q < RAD; ++q) buf[swz<SWZ>(e0 + q * LR * EST)] = v[q];
   }
   __syncthreads();
 }
 
 // DIF: stages S = 0 .. nst-1 with block length L = N / ns[S]
-template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int S>
+template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int SWZ,
+          int S>
 __device__ __forceinline__ void dif_fft(cx<R> *buf, const cx<R> *tw) {
   constexpr FixedPlan pl = make_fixed_plan(N);
   if constexpr (S < pl.nst) {
-    ip_stage<R, INV, false, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST>(buf, tw);
-    dif_fft<R, INV, N, NP, PST, EST, T, PFAST, S + 1>(buf, tw);
+    ip_stage<R, INV, false, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST, SWZ>(buf, tw);
+    dif_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S + 1>(buf, tw);
   }
 }
 
 // DIT: the same stages in reverse order (S = nst-1 .. 0)
-template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int S>
+template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int SWZ,
+          int S>
 __device__ __forceinline__ void dit_fft(cx<R> *buf, const cx<R> *tw) {
   constexpr FixedPlan pl = make_fixed_plan(N);
   if constexpr (S >= 0) {
-    ip_stage<R, INV, true, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST>(buf, tw);
-    dit_fft<R, INV, N, NP, PST, EST, T, PFAST, S - 1>(buf, tw);
+    ip_stage<R, INV, true, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST, SWZ>(buf, tw);
+    dit_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S - 1>(buf, tw);
   }
 }
 
@@ -123,6 +149,7 @@ constexpr int group_rows(int row_len) {
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
   constexpr int NK = N / 2 + 1;
+  constexpr int SW = (int)sizeof(cx<R>);
   __shared__ cx<R> twx[N];
   __shared__ cx<R> twy[N];
   __shared__ cx<R> buf[N * NK];
@@ -139,17 +166,21 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
     const int pre = a.pre[c];
     const R *pw = a.prew[c] ? a.prew[c] + plane0 : nullptr;
     // rows 2q, 2q+1 of the weighted plane as one complex row q (natural order)
-    for (int g = threadIdx.x; g < N * N; g += T) {
-      R w = xp[g];
-      if (pre == PRE_REAL || pre == PRE_CONJ_PSI)
-        w *= pw[g];
-      else if (pre == PRE_ABS2)
-        w *= w;
-      const int y = g / N, i = g % N;
-      reinterpret_cast<R *>(&buf[(y >> 1) * N + i])[y & 1] = w;
+    for (int g = threadIdx.x; g < N * N / 2; g += T) {
+      const int q = g / N, i = g % N;
+      const int g0 = 2 * q * N + i;
+      R w0 = xp[g0], w1 = xp[g0 + N];
+      if (pre == PRE_REAL || pre == PRE_CONJ_PSI) {
+        w0 *= pw[g0];
+        w1 *= pw[g0 + N];
+      } else if (pre == PRE_ABS2) {
+        w0 *= w0;
+        w1 *= w1;
+      }
+      buf[swz<SW>(g)] = cmk<R>(w0, w1);
     }
     __syncthreads();
-    dif_fft<R, false, N, N / 2, N, 1, T, false, 0>(buf, twx);
+    dif_fft<R, false, N, N / 2, N, 1, T, false, SW, 0>(buf, twx);
     // untangle Z_q (digit-reversed in k) into the half spectra of rows 2q,
     // 2q+1 (natural kx, row stride NK); rounds of G rows from the top down,
     // so that no row is overwritten before it is read
@@ -163,8 +194,8 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
         const int t = threadIdx.x + b * T;
         if (NU % T == 0 || t < NU) {
           const int q = q0 + t / NK, k = t % NK;
-          const cx<R> zk = buf[q * N + dif_pos<N>(k)];
-          const cx<R> zn = buf[q * N + dif_pos<N>(k == 0 ? 0 : N - k)];
+          const cx<R> zk = buf[swz<SW>(q * N + dif_pos<N>(k))];
+          const cx<R> zn = buf[swz<SW>(q * N + dif_pos<N>(k == 0 ? 0 : N - k))];
           ua[b] = cmk<R>(R(0.5) * (zk.re + zn.re), R(0.5) * (zk.im - zn.im));
           ub[b] = cmk<R>(R(0.5) * (zk.im + zn.im), R(-0.5) * (zk.re - zn.re));
         }
@@ -182,7 +213,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
       __syncthreads();
     }
     // y transform over the NK adjacent pencils; rows come out digit-reversed
-    dif_fft<R, false, N, NK, 1, NK, T, true, 0>(buf, twy);
+    dif_fft<R, false, N, NK, 1, NK, T, true, 0, 0>(buf, twy);
     cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride + (long long)col * a.g.Kk +
                 (long long)z * N * NK;
     for (int e = threadIdx.x; e < N * NK; e += T) {
@@ -196,7 +227,9 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
   constexpr int NK = N / 2 + 1;
-  constexpr int EPT = (N * N + T - 1) / T;
+  constexpr int SW = (int)sizeof(cx<R>);
+  // output rows 2q and 2q+1 at column i for g = q * N + i, g < N * N / 2
+  constexpr int EPT = (N * N / 2 + T - 1) / T;
   __shared__ cx<R> twx[N];
   __shared__ cx<R> twy[N];
   __shared__ cx<R> buf[N * NK];
@@ -208,9 +241,12 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
     twy[i] = a.tw_y[i];
   }
   const long long plane0 = (long long)z * N * N;
-  R acc[EPT];
+  R acc0[EPT], acc1[EPT];
 #pragma unroll
-  for (int b = 0; b < EPT; ++b) acc[b] = R(0);
+  for (int b = 0; b < EPT; ++b) {
+    acc0[b] = R(0);
+    acc1[b] = R(0);
+  }
   for (int c = 0; c < a.nch; ++c) {
     const cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride +
                       (long long)col * a.g.Kk + (long long)z * N * NK;
@@ -220,7 +256,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
       buf[dif_pos<N>(y) * NK + k] = sp[e];
     }
     __syncthreads();
-    dit_fft<R, true, N, NK, 1, NK, T, true, last_stage<N>()>(buf, twy);
+    dit_fft<R, true, N, NK, 1, NK, T, true, 0, last_stage<N>()>(buf, twy);
     // repack: complex row q = spectrum(row 2q) + i spectrum(row 2q+1),
     // extended to all kx by Hermitian symmetry, digit-reversed in x; rounds
     // of G rows from the bottom up
@@ -254,24 +290,28 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
         const int t = threadIdx.x + b * T;
         if (NQ % T == 0 || t < NQ) {
           const int q = q0 + t / N, k = t % N;
-          buf[q * N + dif_pos<N>(k)] = zq[b];
+          buf[swz<SW>(q * N + dif_pos<N>(k))] = zq[b];
         }
       }
       __syncthreads();
     }
-    dit_fft<R, true, N, N / 2, N, 1, T, false, last_stage<N>()>(buf, twx);
+    dit_fft<R, true, N, N / 2, N, 1, T, false, SW, last_stage<N>()>(buf, twx);
     const int post = a.post[c];
     const R ps = a.posts[c];
     const R *pw = a.postw[c] ? a.postw[c] + plane0 : nullptr;
 #pragma unroll
     for (int b = 0; b < EPT; ++b) {
       const int g = threadIdx.x + b * T;
-      if (N * N % T == 0 || g < N * N) {
-        const int y = g / N, i = g % N;
-        const R *zv = reinterpret_cast<const R *>(&buf[(y >> 1) * N + i]);
-        R v = ps * zv[y & 1];
-        if (post != POST_ONE) v *= pw[g];
-        acc[b] += v;
+      if (N * N / 2 % T == 0 || g < N * N / 2) {
+        const int g0 = 2 * (g / N) * N + g % N;
+        const cx<R> zv = buf[swz<SW>(g)];
+        R v0 = ps * zv.re, v1 = ps * zv.im;
+        if (post != POST_ONE) {
+          v0 *= pw[g0];
+          v1 *= pw[g0 + N];
+        }
+        acc0[b] += v0;
+        acc1[b] += v1;
       }
     }
     __syncthreads();
@@ -282,10 +322,15 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
 #pragma unroll
   for (int b = 0; b < EPT; ++b) {
     const int g = threadIdx.x + b * T;
-    if (N * N % T == 0 || g < N * N) {
-      R out = acc[b];
-      if (lt) out += lt[g] * xp[g];
-      yp[g] = out;
+    if (N * N / 2 % T == 0 || g < N * N / 2) {
+      const int g0 = 2 * (g / N) * N + g % N;
+      R o0 = acc0[b], o1 = acc1[b];
+      if (lt) {
+        o0 += lt[g0] * xp[g0];
+        o1 += lt[g0 + N] * xp[g0 + N];
+      }
+      yp[g0] = o0;
+      yp[g0 + N] = o1;
     }
   }
 }

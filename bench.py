@@ -12,8 +12,10 @@ with HIP events on the context stream (bdgx_bench_pass); the CPU baseline is
 the FP64 C oracle (one vector at a time, as the reference) on a bounded
 sample, scaled by the column applies of the FP64 device solve.
 
-`--probe` only launches the dominant kernel and a copy kernel of known size a
-few times (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE traffic runs).
+`--blas` times the Gram and block-combine kernels at the solver's shapes
+(FP64 MFMA utilisation).  `--probe` only launches the FP64 FFT pass kernels and a copy kernel of known
+size a few times (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE traffic runs,
+whose summary in profiles/ supplies `roofline.traffic`).
 """
 from __future__ import annotations
 
@@ -72,7 +74,7 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None):
 
 
 PASSES = ["xfwd", "yfwd", "z", "yinv", "xinv", "plane_fwd", "plane_inv"]
-KERNELS = {"plane_fwd": "k_plane_fwd_real<double, 128, 512> (x+y forward, whole z-planes)",
+KERNELS = {"plane_fwd": "k_plane_fwd_real<double, 128, 1024> (x+y forward, whole z-planes)",
            "plane_inv": "k_plane_inv_real<double, 128, 512> (y+x inverse + combine)",
            "z": "k_zpass_fixed<double, 128, 8, 256> (z forward, k-space multiply, z inverse)"}
 
@@ -96,22 +98,106 @@ def roofline(cfg):
 
 
 def probe(cfg):
-    """Launch the dominant kernel (FP64 whole-plane forward, 32 batches) and
-    a 1 GiB device copy (k_convert<double, double>) three times each."""
-    import ctypes as C
-
+    """Launch the three FP64 FFT pass kernels (32 spectrum batches) and a
+    1 GiB device copy (k_convert<double, double>) three times each."""
     import bdg_lobpcg_amd as B
 
     d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
-    d.bench_pass(PASSES.index("plane_fwd"), B.FP64, cfg["m"], iters=3)
+    info = {}
+    for name in ("plane_fwd", "z", "plane_inv"):
+        _, by = d.bench_pass(PASSES.index(name), B.FP64, cfg["m"], iters=3)
+        info[name] = by
     n = (1 << 30) // 8
     src, dst = B.DeviceBuffer(8 * n), B.DeviceBuffer(8 * n)
     for _ in range(3):
         B.check(B.lib().bdgx_convert(d.h, B.FP64, src.ptr, n, B.FP64, dst.ptr, n, n, 1), "copy")
     d.sync()
-    print(json.dumps({"probe": "plane_fwd fp64 x3 (+1 warm-up), copy 1 GiB x3",
-                      "copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n}), flush=True)
+    print(json.dumps({"probe": "fp64 plane_fwd, z, plane_inv x3 each (+1 warm-up), copy 1 GiB x3",
+                      "algorithmic_bytes": info, "copy_bytes_read": 8 * n,
+                      "copy_bytes_written": 8 * n}), flush=True)
     d.close()
+
+
+GRAM_SHAPES = [(96, 32, False), (96, 96, False), (96, 96, True), (36, 2, False)]
+COMBINE_SHAPES = [(96, 64), (96, 32), (36, 34)]
+FP64_MFMA_PEAK_TFS = 78.6
+
+
+def blas(cfg, iters=10):
+    """Gram V^T Y and combine V C at the solver's shapes for this grid (S rows,
+    basis [X|P|W] of 3m columns), FP64, HIP events on the context stream:
+    achieved TF/s against the FP64 MFMA peak and GB/s against HBM."""
+    import bdg_lobpcg_amd as B
+
+    d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
+    S = int(np.prod(cfg["N"]))
+    rng = np.random.default_rng(1)
+    kmax = 96
+    A = np.asfortranarray(rng.standard_normal((S, kmax)))
+    Bm = np.asfortranarray(rng.standard_normal((S, kmax)))
+    dA, dB = d.upload(A), d.upload(Bm)
+    dG = B.DeviceBuffer(8 * kmax * kmax)
+    out = []
+    for p, q, same in GRAM_SHAPES:
+        dY = dA if same else dB
+
+        def run():
+            B.check(B.lib().bdgx_gram(d.h, B.FP64, dA.ptr, S, p, dY.ptr, S, q, S, 0, dG.ptr), "gram")
+
+        run()
+        G = d.download(dG, (p, q), np.float64)
+        ref = A[:, :p].T @ (A if same else Bm)[:, :q]
+        err = float(np.abs(G - ref).max() / np.abs(ref).max())
+        d.timer_begin()
+        for _ in range(iters):
+            run()
+        ms = d.timer_end() / iters
+        by = S * (p + (0 if same else q)) * 8
+        fl = 2.0 * S * p * q
+        out.append(dict(kernel="gram", shape=f"{p}x{q}" + (" (A^T A)" if same else ""), us=ms * 1e3,
+                        GBs=by / ms / 1e6, TFs=fl / ms / 1e9,
+                        mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS, rel_err=err))
+    for k, m in COMBINE_SHAPES:
+        Cm = np.asfortranarray(rng.standard_normal((k, m)))
+        dC = d.upload(Cm)
+        dY = B.DeviceBuffer(S * m * 8)
+
+        def run():
+            B.check(B.lib().bdgx_combine(d.h, B.FP64, dA.ptr, S, k, dC.ptr, dY.ptr, S, m, S),
+                    "combine")
+
+        run()
+        Y = d.download(dY, (S, m), np.float64)
+        rows = slice(0, S, 997)
+        ref = A[rows, :k] @ Cm
+        err = float(np.abs(Y[rows] - ref).max() / np.abs(ref).max())
+        d.timer_begin()
+        for _ in range(iters):
+            run()
+        ms = d.timer_end() / iters
+        by = S * (k + m) * 8
+        fl = 2.0 * S * k * m
+        out.append(dict(kernel="combine", shape=f"{k}->{m}", us=ms * 1e3, GBs=by / ms / 1e6,
+                        TFs=fl / ms / 1e9, mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS,
+                        rel_err=err))
+    d.close()
+    for r in out:
+        print(json.dumps(r), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a
+    1 GiB copy), or None if the summary does not list it."""
+    path = os.path.join(HERE, "profiles", "traffic_fft_passes_128c.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("| `") and kernel in line:
+                    return float(line.split("|")[-2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(cfg, psi, arrays, column_applies):
@@ -147,6 +233,8 @@ def main():
     ap.add_argument("--config", default="cfg1_128c_d")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--blas", action="store_true",
+                    help="time the Gram and combine kernels at the solver's shapes and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the solver runs on a single GPU "
@@ -158,6 +246,9 @@ def main():
     cfg = CONFIGS[args.config]
     if args.probe:
         probe(cfg)
+        return
+    if args.blas:
+        blas(cfg)
         return
     log(f"config {args.config}: N={cfg['N']} nev={cfg['nev']} ITP {cfg['itp_steps']} steps")
     psi, mu, t_itp = ground_state(cfg)
@@ -217,7 +308,8 @@ def main():
         "omega_fp64": [float(x) for x in r64["w"]],
         "psi0_seconds": t_itp,
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": dom["GBs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(KERNELS[dom_key.split("_", 1)[1]].split(" (")[0]),
                      "kernel": KERNELS.get(dom_key.split("_", 1)[1], dom_key),
                      "bytes_per_launch": dom["bytes"], "ms_per_launch": dom["ms"]},
         "passes": {k: round(v["GBs"], 1) for k, v in rf.items()},
