@@ -47,17 +47,18 @@ struct LdsIO {
 };
 
 template <typename R>
-struct GlobIO {  // element i of pencil p at base[i * stride + p]; p < valid
+struct GlobIO {  // element i of pencil p at base[i * stride + p * pstride]; p < valid
   cx<R> *base;
   long long stride;
   int valid;
+  long long pstride = 1;
   template <int P>
   __device__ __forceinline__ cx<R> ld(int i, int p) const {
-    return p < valid ? base[i * stride + p] : cmk<R>(0, 0);
+    return p < valid ? base[i * stride + p * pstride] : cmk<R>(0, 0);
   }
   template <int P>
   __device__ __forceinline__ void st(int i, int p, cx<R> v) const {
-    if (p < valid) base[i * stride + p] = v;
+    if (p < valid) base[i * stride + p * pstride] = v;
   }
 };
 

@@ -129,12 +129,19 @@ __global__ void k_lp_tables(LpCoef cf, Geo g, const float *kx2, const float *ky2
   }
 }
 
-// sig[col] = rms of column col (one workgroup per column)
-__global__ void __launch_bounds__(256) k_lp_colnorm(const float *R, long long ld, long long S,
-                                                    float *sig) {
-  const float *r = R + (long long)blockIdx.x * ld;
+// Column rms in two launches: part[col * LP_NCHUNK + chunk] = sum of squares
+// over one of LP_NCHUNK contiguous chunks (grid LP_NCHUNK x ncols), then
+// sig[col] = sqrt(sum_chunk part / S) in a fixed order (deterministic).
+constexpr int LP_NCHUNK = 64;
+
+__global__ void __launch_bounds__(256) k_lp_colsq(const float *R, long long ld, long long S,
+                                                  double *part) {
+  const float *r = R + (long long)blockIdx.y * ld;
+  const long long len = (S + LP_NCHUNK - 1) / LP_NCHUNK;
+  const long long i0 = (long long)blockIdx.x * len;
+  const long long i1 = i0 + len < S ? i0 + len : S;
   double s = 0.0;
-  for (long long i = threadIdx.x; i < S; i += blockDim.x) s += (double)r[i] * r[i];
+  for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += (double)r[i] * r[i];
   __shared__ double red[256];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -142,20 +149,46 @@ __global__ void __launch_bounds__(256) k_lp_colnorm(const float *R, long long ld
     if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) sig[blockIdx.x] = (float)fmax(sqrt(red[0] / (double)S), 1e-30);
+  if (threadIdx.x == 0) part[(long long)blockIdx.y * LP_NCHUNK + blockIdx.x] = red[0];
+}
+
+__global__ void k_lp_colfinal(const double *part, int ncols, long long S, float *sig) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncols) return;
+  double s = 0.0;
+  for (int k = 0; k < LP_NCHUNK; ++k) s += part[(long long)col * LP_NCHUNK + k];
+  sig[col] = (float)fmax(sqrt(s / (double)S), 1e-30);
 }
 
 // ---------------------------------------------------------------------------
 // stencil layer
-__device__ __forceinline__ long long lp_nb(long long i, int dx, int dy, int dz, const Geo &g) {
-  const int x = (int)(i % g.nx);
-  const long long r = i / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
-  int xx = x + dx, yy = y + dy, zz = z + dz;
-  xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
-  yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
-  zz = zz < 0 ? zz + g.nz : (zz >= g.nz ? zz - g.nz : zz);
-  return ((long long)zz * g.ny + yy) * g.nx + xx;
+// The 27 periodic neighbours of point i: index of offset (dz, dy, dx) is
+// row[(dz + 1) * 3 + dy + 1] + col[dx + 1]; neighbour n = (dz+1)*9 + (dy+1)*3
+// + (dx+1) and the opposite offset is 26 - n.  One division chain per point.
+struct LpNb {
+  long long row[9];
+  int col[3];
+  __device__ __forceinline__ LpNb(long long i, const Geo &g) {
+    const int x = (int)(i % g.nx);
+    const long long r = i / g.nx;
+    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
+    const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
+    const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      col[a] = xs[a];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) row[a * 3 + b] = ((long long)zs[a] * g.ny + ys[b]) * g.nx;
+    }
+  }
+  __device__ __forceinline__ long long at(int n) const { return row[n / 3] + col[n % 3]; }
+};
+
+// tanh from one exponential: 1 - 2 / (exp(2z) + 1), exact limits +-1
+__device__ __forceinline__ float lp_tanh(float z) {
+  const float e = __expf(2.f * z);
+  return 1.f - 2.f / (e + 1.f);
 }
 
 // V = U + s0 sig_col sum_h v_h tanh(z_h),  z_h = sum_nb w_{h,nb} U(x+nb)/(s0 sig_col) + d_h
@@ -173,18 +206,17 @@ __global__ void __launch_bounds__(256) k_lp_stencil_fwd(const float *U, int ncol
     const long long i = e - (long long)col * g.S;
     const float *u = U + (long long)col * g.S;
     const float sc = s0 * sig[col], isc = 1.f / sc;
+    const LpNb nbi(i, g);
     float nb[27];
-    int n = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) nb[n++] = u[lp_nb(i, dx, dy, dz, g)] * isc;
+#pragma unroll
+    for (int n = 0; n < 27; ++n) nb[n] = u[nbi.at(n)] * isc;
     float s = 0.f;
 #pragma unroll 4
     for (int h = 0; h < LPH; ++h) {
       float z = d[h];
 #pragma unroll
       for (int q = 0; q < 27; ++q) z += w[h * 27 + q] * nb[q];
-      s += v[h] * tanhf(z);
+      s += v[h] * lp_tanh(z);
     }
     V[e] = u[i] + sc * s;
   }
@@ -210,15 +242,14 @@ __global__ void __launch_bounds__(256) k_lp_stencil_bwd(const float *U, const fl
     const long long i = e - (long long)col * g.S;
     const float *u = U + (long long)col * g.S;
     const float sc = s0 * sig[col], isc = 1.f / sc;
+    const LpNb nbi(i, g);
     float nb[27];
-    int n = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) nb[n++] = u[lp_nb(i, dx, dy, dz, g)] * isc;
+#pragma unroll
+    for (int n = 0; n < 27; ++n) nb[n] = u[nbi.at(n)] * isc;
     float z = dh;
 #pragma unroll
     for (int q = 0; q < 27; ++q) z += w[q] * nb[q];
-    const float t = tanhf(z);
+    const float t = lp_tanh(z);
     const float dvv = dV[e];
     const float q = dvv * vh * (1.f - t * t);
     if (Q) {
@@ -262,11 +293,10 @@ __global__ void __launch_bounds__(256) k_lp_stencil_t(const float *Q, int ncols,
     const int col = (int)(e / g.S);
     const long long i = e - (long long)col * g.S;
     const float *q = Q + (long long)col * g.S;
+    const LpNb nbi(i, g);
     float acc = 0.f;
-    int n = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) acc += ws[n++] * q[lp_nb(i, -dx, -dy, -dz, g)];
+#pragma unroll
+    for (int n = 0; n < 27; ++n) acc += ws[n] * q[nbi.at(26 - n)];
     dU[e] += acc;
   }
 }
@@ -396,6 +426,14 @@ static unsigned grid_for(long long n) {
 
 using namespace bdgx;
 
+// sig[col] = rms of column col of R (S x n, ld S)
+static void lp_colnorm(const float *R, long long S, long long n, double *part, float *sig,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_lp_colsq, dim3(LP_NCHUNK, (unsigned)n), dim3(256), 0, st, R, S, S, part);
+  hipLaunchKernelGGL(k_lp_colfinal, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
+                     (const double *)part, (int)n, S, sig);
+}
+
 extern "C" {
 
 int bdgx_lp_set_params(bdgx_ctx *c, const float *theta) {
@@ -451,14 +489,16 @@ int bdgx_lp_apply(bdgx_ctx *c, int prec, const void *dR, void *dW, uint64_t ncol
   if (!c->has_lp) return BDGX_ESTATE;
   if (ncols == 0) return BDGX_OK;
   const long long S = c->geo.S, n = (long long)ncols;
-  // U, V (S x n each) and sig (n)
-  int rc = ensure_lp_tmp(c, sizeof(float) * ((size_t)S * n * 2 + n));
+  // column partial sums (n x LP_NCHUNK doubles), U, V (S x n each), sig (n)
+  int rc = ensure_lp_tmp(c, sizeof(double) * (size_t)n * LP_NCHUNK +
+                                sizeof(float) * ((size_t)S * n * 2 + n));
   if (rc) return rc;
-  float *U = reinterpret_cast<float *>(c->lp_tmp);
+  double *part = reinterpret_cast<double *>(c->lp_tmp);
+  float *U = reinterpret_cast<float *>(part + n * LP_NCHUNK);
   float *V = U + S * n, *sig = V + S * n;
   const float *R = reinterpret_cast<const float *>(dR);
   float *W = reinterpret_cast<float *>(dW);
-  hipLaunchKernelGGL(k_lp_colnorm, dim3((unsigned)n), dim3(256), 0, c->stream, R, S, S, sig);
+  lp_colnorm(R, S, n, part, sig, c->stream);
   rc = lp_filter_f32(c, 0, 0, R, U, n, S);
   if (rc) return rc;
   hipLaunchKernelGGL(k_lp_stencil_fwd, dim3(grid_for(S * n)), dim3(256), 0, c->stream,
@@ -475,10 +515,13 @@ int bdgx_lp_backward(bdgx_ctx *c, const float *dR, const float *dGW, uint64_t nc
   if (ncols == 0) return BDGX_OK;
   const long long S = c->geo.S, n = (long long)ncols;
   Fields<float> &F = c->f32;
-  // U, V, Wsp, dV, dU, T, Q (S x n each), X2 (2S x n), sig (n)
-  int rc = ensure_lp_tmp(c, sizeof(float) * ((size_t)S * n * 9 + n));
+  // column partial sums (n x LP_NCHUNK doubles), U, V, Wsp, dV, dU, T, Q
+  // (S x n each), X2 (2S x n), sig (n)
+  int rc = ensure_lp_tmp(c, sizeof(double) * (size_t)n * LP_NCHUNK +
+                                sizeof(float) * ((size_t)S * n * 9 + n));
   if (rc) return rc;
-  float *U = reinterpret_cast<float *>(c->lp_tmp);
+  double *part = reinterpret_cast<double *>(c->lp_tmp);
+  float *U = reinterpret_cast<float *>(part + n * LP_NCHUNK);
   float *V = U + S * n, *Wsp = V + S * n, *dV = Wsp + S * n, *dU = dV + S * n;
   float *T = dU + S * n, *Q = T + S * n, *X2 = Q + S * n, *sig = X2 + 2 * S * n;
   rc = ensure_scratch(c, sizeof(double) * BDGX_LP_NPARAM);
@@ -489,7 +532,7 @@ int bdgx_lp_backward(bdgx_ctx *c, const float *dR, const float *dGW, uint64_t nc
   const float kmax = lp_kmax(c), s0 = (float)c->lp_s0;
   const hipStream_t st = c->stream;
   // forward intermediates
-  hipLaunchKernelGGL(k_lp_colnorm, dim3((unsigned)n), dim3(256), 0, st, dR, S, S, sig);
+  lp_colnorm(dR, S, n, part, sig, st);
   rc = lp_filter_f32(c, 0, 0, dR, U, n, S);
   if (!rc) {
     hipLaunchKernelGGL(k_lp_stencil_fwd, dim3(gb), dim3(256), 0, st, (const float *)U, (int)n,

@@ -345,10 +345,12 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
 template <typename R>
 struct ZMul {
   const YZArgs<R> *a;
-  int code, kx0, y;
+  int code, kx0, y0;
+  bool ystep;  // pencil p at (kx0, y0 + p) instead of (kx0 + p, y0)
   __device__ __forceinline__ cx<R> operator()(cx<R> v, int z, int p) const {
-    const int kx = kx0 + p;
-    if (kx >= a->g.n0k) return v;
+    const int kx = ystep ? kx0 : kx0 + p;
+    const int y = ystep ? y0 + p : y0;
+    if (kx >= a->g.n0k || y >= a->g.ny) return v;
     const R k2 = a->kx2[kx] + a->ky2[y] + a->kz2[z];
     R f;
     switch (code) {
@@ -378,25 +380,48 @@ struct ZMul {
   }
 };
 
+// Workgroups of a batch: ny * (n0k / P) tiles of P adjacent kx at one y,
+// then, when n0k = P * full + 1 (the r2c Nyquist column kx = n0k - 1), that
+// column in tiles of P adjacent y; otherwise a last partial kx tile per y.
+template <int P>
+__host__ __device__ __forceinline__ int z_tiles_per_batch(int n0k, int ny) {
+  const int full = n0k / P, rem = n0k % P;
+  if (rem == 0) return full * ny;
+  if (rem == 1) return full * ny + (ny + P - 1) / P;
+  return (full + 1) * ny;
+}
+
 template <typename R, int NZ, int P, int T>
 __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
   __shared__ cx<R> tw[NZ];
   __shared__ cx<R> bufA[NZ * P];
   __shared__ cx<R> bufB[NZ * P];
   const int ny = a.g.ny, n0k = a.g.n0k;
-  const int ntile = (n0k + P - 1) / P;
-  const int t = blockIdx.x % ntile;
-  const int rest = blockIdx.x / ntile;
-  const int y = rest % ny;
-  const int b = rest / ny;
-  const int kx0 = t * P;
+  const int per_b = z_tiles_per_batch<P>(n0k, ny);
+  const int b = blockIdx.x / per_b;
+  const int r = blockIdx.x % per_b;
+  const int full = n0k / P;
+  const bool nyq = n0k % P == 1 && r >= full * ny;
+  int kx0, y0, valid;
+  long long pstride;
+  if (nyq) {  // Nyquist column, P adjacent y
+    kx0 = n0k - 1;
+    y0 = (r - full * ny) * P;
+    valid = ny - y0 < P ? ny - y0 : P;
+    pstride = n0k;
+  } else {
+    const int ntile = n0k % P == 1 ? full : (n0k + P - 1) / P;
+    kx0 = (r % ntile) * P;
+    y0 = r / ntile;
+    valid = n0k - kx0 < P ? n0k - kx0 : P;
+    pstride = 1;
+  }
   for (int i = threadIdx.x; i < NZ; i += T) tw[i] = a.tw[i];
   __syncthreads();
-  cx<R> *base = a.spec + (long long)b * a.g.Kk + (long long)y * n0k + kx0;
+  cx<R> *base = a.spec + (long long)b * a.g.Kk + (long long)y0 * n0k + kx0;
   const long long zs = (long long)n0k * ny;
-  const int valid = n0k - kx0 < P ? n0k - kx0 : P;
-  GlobIO<R> g{base, zs, valid};
-  ZMul<R> mul{&a, a.mul[b / a.ncols], kx0, y};
+  GlobIO<R> g{base, zs, valid, pstride};
+  ZMul<R> mul{&a, a.mul[b / a.ncols], kx0, y0, nyq};
   constexpr int nst = make_fixed_plan(NZ).nst;
   cx<R> *F = (nst % 2 == 1) ? bufA : bufB;  // last LDS buffer of the forward
   cx<R> *O = (F == bufA) ? bufB : bufA;
@@ -426,8 +451,7 @@ template <typename R>
 int launch_zpass_fixed(bdgx_ctx *c, const YZArgs<R> &z) {
   constexpr int P = sizeof(R) == 8 ? 8 : 16;
   const Geo &g = c->geo;
-  const int ntile = (g.n0k + P - 1) / P;
-  const dim3 grid((unsigned)(ntile * g.ny * z.nb));
+  const dim3 grid((unsigned)(z_tiles_per_batch<P>(g.n0k, g.ny) * z.nb));
   switch (g.nz) {
     case 32: hipLaunchKernelGGL((k_zpass_fixed<R, 32, P, 128>), grid, dim3(128), 0, c->stream, z); break;
     case 64: hipLaunchKernelGGL((k_zpass_fixed<R, 64, P, 128>), grid, dim3(128), 0, c->stream, z); break;

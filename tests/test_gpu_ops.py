@@ -17,6 +17,9 @@ GRIDS = [
     ((32, 32, 12), (16.0, 16.0, 10.0)),    # whole-plane x/y kernels (real)
     ((64, 64, 6), (16.0, 16.0, 6.0)),
     ((128, 128, 4), (24.0, 24.0, 4.0)),
+    ((16, 12, 32), (12.0, 10.0, 16.0)),    # fixed-size z pass: Nyquist-column tiles
+    ((32, 32, 64), (16.0, 16.0, 24.0)),    # plane kernels + fixed z pass
+    ((24, 8, 128), (12.0, 8.0, 30.0)),     # fixed z pass, partial kx tiles
 ]
 OPS_HALF = ["kinetic", "K", "M", "precondK", "precondM", "dipolar_conv"]
 OPS_FULL = ["Lrep", "Swap", "precondLrep"]
