@@ -74,12 +74,12 @@ __device__ __forceinline__ int swz(int e) {
 
 template <typename R, bool INV, bool DIT, int N, int RAD, int L, int NP, int PST, int EST,
           int T, bool PFAST, int SWZ>
-__device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw) {
+__device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw, int tid) {
   constexpr int LR = L / RAD;
   constexpr int NBF = N / RAD;
   constexpr int TOTAL = NP * NBF;
   constexpr int TSTEP = N / L;
-  for (int t = threadIdx.x; t < TOTAL; t += T) {
+  for (int t = tid; t < TOTAL; t += T) {
     const int p = PFAST ? t % NP : t / NBF;
     const int u = PFAST ? t / NP : t % NBF;
     const int blk = u / LR, m = u % LR;
@@ -113,22 +113,22 @@ __device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw) {
 // DIF: stages S = 0 .. nst-1 with block length L = N / ns[S]
 template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int SWZ,
           int S>
-__device__ __forceinline__ void dif_fft(cx<R> *buf, const cx<R> *tw) {
+__device__ __forceinline__ void dif_fft(cx<R> *buf, const cx<R> *tw, int tid) {
   constexpr FixedPlan pl = make_fixed_plan(N);
   if constexpr (S < pl.nst) {
-    ip_stage<R, INV, false, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST, SWZ>(buf, tw);
-    dif_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S + 1>(buf, tw);
+    ip_stage<R, INV, false, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST, SWZ>(buf, tw, tid);
+    dif_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S + 1>(buf, tw, tid);
   }
 }
 
 // DIT: the same stages in reverse order (S = nst-1 .. 0)
 template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int SWZ,
           int S>
-__device__ __forceinline__ void dit_fft(cx<R> *buf, const cx<R> *tw) {
+__device__ __forceinline__ void dit_fft(cx<R> *buf, const cx<R> *tw, int tid) {
   constexpr FixedPlan pl = make_fixed_plan(N);
   if constexpr (S >= 0) {
-    ip_stage<R, INV, true, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST, SWZ>(buf, tw);
-    dit_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S - 1>(buf, tw);
+    ip_stage<R, INV, true, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST, SWZ>(buf, tw, tid);
+    dit_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S - 1>(buf, tw, tid);
   }
 }
 
@@ -163,10 +163,13 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
   const long long plane0 = (long long)z * N * N;
   const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S + plane0;
   for (int c = 0; c < a.nch; ++c) {
+    // (the compiler hoists the swizzled LDS addresses of the body out of
+    // this loop; measured faster here than recomputing them, see the inverse)
+    const int tid = threadIdx.x;
     const int pre = a.pre[c];
     const R *pw = a.prew[c] ? a.prew[c] + plane0 : nullptr;
     // rows 2q, 2q+1 of the weighted plane as one complex row q (natural order)
-    for (int g = threadIdx.x; g < N * N / 2; g += T) {
+    for (int g = tid; g < N * N / 2; g += T) {
       const int q = g / N, i = g % N;
       const int g0 = 2 * q * N + i;
       R w0 = xp[g0], w1 = xp[g0 + N];
@@ -180,7 +183,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
       buf[swz<SW>(g)] = cmk<R>(w0, w1);
     }
     __syncthreads();
-    dif_fft<R, false, N, N / 2, N, 1, T, false, SW, 0>(buf, twx);
+    dif_fft<R, false, N, N / 2, N, 1, T, false, SW, 0>(buf, twx, tid);
     // untangle Z_q (digit-reversed in k) into the half spectra of rows 2q,
     // 2q+1 (natural kx, row stride NK); rounds of G rows from the top down,
     // so that no row is overwritten before it is read
@@ -191,7 +194,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
       cx<R> ua[BU], ub[BU];
 #pragma unroll
       for (int b = 0; b < BU; ++b) {
-        const int t = threadIdx.x + b * T;
+        const int t = tid + b * T;
         if (NU % T == 0 || t < NU) {
           const int q = q0 + t / NK, k = t % NK;
           const cx<R> zk = buf[swz<SW>(q * N + dif_pos<N>(k))];
@@ -203,7 +206,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
       __syncthreads();
 #pragma unroll
       for (int b = 0; b < BU; ++b) {
-        const int t = threadIdx.x + b * T;
+        const int t = tid + b * T;
         if (NU % T == 0 || t < NU) {
           const int q = q0 + t / NK, k = t % NK;
           buf[(2 * q) * NK + k] = ua[b];
@@ -213,10 +216,10 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
       __syncthreads();
     }
     // y transform over the NK adjacent pencils; rows come out digit-reversed
-    dif_fft<R, false, N, NK, 1, NK, T, true, 0, 0>(buf, twy);
+    dif_fft<R, false, N, NK, 1, NK, T, true, 0, 0>(buf, twy, tid);
     cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride + (long long)col * a.g.Kk +
                 (long long)z * N * NK;
-    for (int e = threadIdx.x; e < N * NK; e += T) {
+    for (int e = tid; e < N * NK; e += T) {
       const int y = e / NK, k = e % NK;
       sp[e] = buf[dif_pos<N>(y) * NK + k];
     }
@@ -248,15 +251,20 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
     acc1[b] = R(0);
   }
   for (int c = 0; c < a.nch; ++c) {
+    // thread index opaque to the compiler inside the channel loop: keeps it
+    // from hoisting every swizzled LDS address of the body out of a loop that
+    // usually runs once or twice (register spills at 1024 threads)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride +
                       (long long)col * a.g.Kk + (long long)z * N * NK;
     // rows into digit-reversed order for the DIT y inverse
-    for (int e = threadIdx.x; e < N * NK; e += T) {
+    for (int e = tid; e < N * NK; e += T) {
       const int y = e / NK, k = e % NK;
       buf[dif_pos<N>(y) * NK + k] = sp[e];
     }
     __syncthreads();
-    dit_fft<R, true, N, NK, 1, NK, T, true, 0, last_stage<N>()>(buf, twy);
+    dit_fft<R, true, N, NK, 1, NK, T, true, 0, last_stage<N>()>(buf, twy, tid);
     // repack: complex row q = spectrum(row 2q) + i spectrum(row 2q+1),
     // extended to all kx by Hermitian symmetry, digit-reversed in x; rounds
     // of G rows from the bottom up
@@ -267,7 +275,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
       cx<R> zq[BQ];
 #pragma unroll
       for (int b = 0; b < BQ; ++b) {
-        const int t = threadIdx.x + b * T;
+        const int t = tid + b * T;
         if (NQ % T == 0 || t < NQ) {
           const int q = q0 + t / N, k = t % N;
           const bool cj = k >= NK;
@@ -287,7 +295,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
       __syncthreads();
 #pragma unroll
       for (int b = 0; b < BQ; ++b) {
-        const int t = threadIdx.x + b * T;
+        const int t = tid + b * T;
         if (NQ % T == 0 || t < NQ) {
           const int q = q0 + t / N, k = t % N;
           buf[swz<SW>(q * N + dif_pos<N>(k))] = zq[b];
@@ -295,13 +303,13 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
       }
       __syncthreads();
     }
-    dit_fft<R, true, N, N / 2, N, 1, T, false, SW, last_stage<N>()>(buf, twx);
+    dit_fft<R, true, N, N / 2, N, 1, T, false, SW, last_stage<N>()>(buf, twx, tid);
     const int post = a.post[c];
     const R ps = a.posts[c];
     const R *pw = a.postw[c] ? a.postw[c] + plane0 : nullptr;
 #pragma unroll
     for (int b = 0; b < EPT; ++b) {
-      const int g = threadIdx.x + b * T;
+      const int g = tid + b * T;
       if (N * N / 2 % T == 0 || g < N * N / 2) {
         const int g0 = 2 * (g / N) * N + g % N;
         const cx<R> zv = buf[swz<SW>(g)];
@@ -489,10 +497,9 @@ int launch_plane_x(bdgx_ctx *c, const XArgs<R> &a, bool inverse) {
     case 32: launch_plane<R, 32, 256>(a, inverse, c->stream); break;
     case 64: launch_plane<R, 64, 512>(a, inverse, c->stream); break;
     case 128:
-      // thread counts measured per kernel: the FP64 forward runs faster
-      // with 1024 threads (128 VGPRs, no spills), the FP64 inverse spills at
-      // 1024 and stays at 512; FP32 is the other way round
-      if ((sizeof(R) == 8) != inverse)
+      // thread counts measured per kernel: FP64 (one 133 KB plane per CU)
+      // runs faster with 1024 threads, FP32 (two 67 KB planes per CU) with 512
+      if (sizeof(R) == 8)
         launch_plane<R, 128, 1024>(a, inverse, c->stream);
       else
         launch_plane<R, 128, 512>(a, inverse, c->stream);
