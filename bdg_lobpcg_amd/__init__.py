@@ -38,11 +38,33 @@ class BdgError(RuntimeError):
     pass
 
 
+class Graph:
+    """A captured device chain of a DeviceContext (bdgx_graph)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+
+    def launch(self):
+        check(lib().bdgx_graph_launch(self.ctx.h, self.h), "bdgx_graph_launch")
+
+    def close(self):
+        if self.h:
+            lib().bdgx_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_uint64), ("column_applies", C.c_uint64),
                 ("column_applies_fp32", C.c_uint64), ("image_refreshes", C.c_uint64),
                 ("restarts", C.c_uint64), ("learned_iterations", C.c_uint64),
-                ("learned_fallback_at", C.c_int), ("seconds_total", C.c_double),
+                ("learned_fallback_at", C.c_int), ("graph_launches", C.c_uint64),
+                ("graph_captures", C.c_uint64), ("seconds_total", C.c_double),
                 ("seconds_rr", C.c_double), ("status", C.c_int)]
 
 
@@ -89,6 +111,9 @@ def lib():
         "bdgx_timer_end": (I, [P, C.POINTER(C.c_float)]),
         "bdgx_workspace_bytes": (U64, [P]),
         "bdgx_bench_pass": (I, [P, I, I, U64, I, C.POINTER(C.c_float), C.POINTER(D)]),
+        "bdgx_autotune": (I, [P, I, U64, I]),
+        "bdgx_get_launch_variant": (I, [P, I, I, C.POINTER(I)]),
+        "bdgx_set_launch_variant": (I, [P, I, I, I]),
         "bdgx_lp_set_params": (I, [P, P]),
         "bdgx_lp_apply": (I, [P, I, P, P, U64, U64]),
         "bdgx_lp_backward": (I, [P, P, P, U64, U64, P]),
@@ -127,6 +152,12 @@ def lib():
         "bdg_mu": (D, [P]),
         "bdg_set_mixed_params": (None, [P, D, U64]),
         "bdg_set_verbosity": (None, [P, I]),
+        "bdg_set_graph_capture": (None, [P, I]),
+        "bdgx_graph_begin": (I, [P]),
+        "bdgx_graph_end": (I, [P, C.POINTER(P)]),
+        "bdgx_graph_launch": (I, [P, P]),
+        "bdgx_graph_destroy": (None, [P]),
+        "bdgx_reserve": (I, [P, U64, U64]),
         "bdg_set_learned_preconditioner": (None, [P, P]),
         "bdg_set_learned_fallback": (None, [P, I, D]),
         "bdg_start_block": (None, [P, P]),
@@ -284,6 +315,38 @@ class DeviceContext:
         check(lib().bdgx_timer_end(self.h, C.byref(ms)), "timer")
         return float(ms.value)
 
+    def autotune(self, prec: int, nbatch: int = 32, force: bool = False):
+        """Time the launch variants of the fixed-size FFT kernels and keep the
+        fastest (bdgx_autotune); returns the chosen variants."""
+        check(lib().bdgx_autotune(self.h, int(prec), int(nbatch), int(force)), "bdgx_autotune")
+        return self.launch_variants(prec)
+
+    def launch_variants(self, prec: int) -> dict:
+        out = {}
+        for kind, name in enumerate(("plane_fwd", "plane_inv", "z")):
+            v = C.c_int()
+            check(lib().bdgx_get_launch_variant(self.h, kind, int(prec), C.byref(v)),
+                  "bdgx_get_launch_variant")
+            out[name] = int(v.value)
+        return out
+
+    def set_launch_variant(self, kind: int, prec: int, variant: int):
+        check(lib().bdgx_set_launch_variant(self.h, int(kind), int(prec), int(variant)),
+              "bdgx_set_launch_variant")
+
+    def capture(self, fn):
+        """Record the device work that fn() issues on this context into a
+        graph (bdgx_graph_begin / _end); returns a Graph (launch(), close())."""
+        check(lib().bdgx_graph_begin(self.h), "bdgx_graph_begin")
+        try:
+            fn()
+        except Exception:
+            lib().bdgx_graph_end(self.h, None)
+            raise
+        g = C.c_void_p()
+        check(lib().bdgx_graph_end(self.h, C.byref(g)), "bdgx_graph_end")
+        return Graph(self, g)
+
     def bench_pass(self, pass_id: int, prec: int, nbatch: int, iters: int = 20):
         """(ms per launch, algorithmic bytes per launch) of one FFT pass kind."""
         ms = C.c_float()
@@ -437,6 +500,9 @@ class Problem:
 
     def set_verbosity(self, level):
         lib().bdg_set_verbosity(self.h, int(level))
+
+    def set_graph_capture(self, on: bool):
+        lib().bdg_set_graph_capture(self.h, int(bool(on)))
 
     def set_learned_fallback(self, window=30, factor=0.5):
         lib().bdg_set_learned_fallback(self.h, int(window), float(factor))

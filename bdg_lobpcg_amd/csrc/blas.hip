@@ -657,6 +657,7 @@ int gather_dispatch(bdgx_ctx *c, int prec, const void *src, uint64_t lds, const 
     }
     c->idx_cap = ncols < 1024 ? 1024 : ncols;
     BDGX_HIP(hipMalloc((void **)&c->idx_buf, sizeof(int) * c->idx_cap));
+    c->ws_gen++;
   }
   BDGX_HIP(hipMemcpyAsync(c->idx_buf, hidx, sizeof(int) * ncols, hipMemcpyHostToDevice,
                           c->stream));

@@ -7,6 +7,7 @@
 
 #include "bdgx_internal.hpp"
 #include "fft_core.hpp"
+#include "plane.hpp"
 
 hipError_t bdgx_last_hip_error = hipSuccess;
 
@@ -23,6 +24,7 @@ int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, co
                       int swap_x, void *dR, uint64_t ldr, double *dn2);
 int convert_dispatch(bdgx_ctx *c, int ps, const void *src, uint64_t lds, int pd, void *dst,
                      uint64_t ldd, uint64_t nrows, uint64_t ncols);
+int autotune_dispatch(bdgx_ctx *c, int prec, uint64_t nbatch);
 int bench_pass_dispatch(bdgx_ctx *c, int pass, int prec, uint64_t nbatch, int iters, float *ms,
                         double *bytes);
 int gather_dispatch(bdgx_ctx *c, int prec, const void *src, uint64_t lds, const int *hidx,
@@ -64,6 +66,7 @@ int ensure_spec(bdgx_ctx *c, size_t bytes) {
   }
   BDGX_HIP(hipMalloc(&c->spec, bytes));
   c->spec_bytes = bytes;
+  c->ws_gen++;
   return BDGX_OK;
 }
 
@@ -77,6 +80,7 @@ int ensure_scratch(bdgx_ctx *c, size_t bytes) {
   }
   BDGX_HIP(hipMalloc(&c->scratch, bytes));
   c->scratch_bytes = bytes;
+  c->ws_gen++;
   return BDGX_OK;
 }
 
@@ -266,6 +270,7 @@ int bdgx_ctx_create(bdgx_ctx **out, uint64_t dim, const uint64_t *N, const doubl
   g.n0k = c->cplx ? g.nx : g.nx / 2 + 1;
   g.S = (long long)g.nx * g.ny * g.nz;
   g.Kk = (long long)g.n0k * g.ny * g.nz;
+  tune_defaults(c);
   for (int d = 0; d < 3; ++d)
     if (make_plan((int)c->N[d], &c->plan[d])) {
       delete c;
@@ -663,6 +668,75 @@ int bdgx_timer_end(bdgx_ctx *c, float *ms) {
   BDGX_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
   return BDGX_OK;
 }
+struct bdgx_graph {
+  hipGraphExec_t exec;
+  unsigned gen;
+};
+
+int bdgx_graph_begin(bdgx_ctx *c) {
+  if (!c) return BDGX_EINVAL;
+  BDGX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  return BDGX_OK;
+}
+
+int bdgx_graph_end(bdgx_ctx *c, bdgx_graph **out) {
+  if (!c) return BDGX_EINVAL;
+  if (out) *out = nullptr;
+  hipGraph_t g = nullptr;
+  BDGX_HIP(hipStreamEndCapture(c->stream, &g));
+  if (!out) {  // abandon the capture
+    if (g) (void)hipGraphDestroy(g);
+    return BDGX_OK;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return BDGX_EHIP;
+  *out = new bdgx_graph{exec, c->ws_gen};
+  return BDGX_OK;
+}
+
+int bdgx_graph_launch(bdgx_ctx *c, const bdgx_graph *g) {
+  if (!c || !g) return BDGX_EINVAL;
+  if (g->gen != c->ws_gen) return BDGX_ESTATE;  // workspace moved since capture
+  BDGX_HIP(hipGraphLaunch(g->exec, c->stream));
+  return BDGX_OK;
+}
+
+void bdgx_graph_destroy(bdgx_graph *g) {
+  if (!g) return;
+  (void)hipGraphExecDestroy(g->exec);
+  delete g;
+}
+
+int bdgx_reserve(bdgx_ctx *c, uint64_t spec_bytes, uint64_t scratch_bytes) {
+  if (!c) return BDGX_EINVAL;
+  int rc = ensure_spec(c, (size_t)spec_bytes);
+  if (!rc) rc = ensure_scratch(c, (size_t)scratch_bytes);
+  return rc;
+}
+
+int bdgx_autotune(bdgx_ctx *c, int prec, uint64_t nbatch, int force) {
+  if (!c || (prec != BDGX_FP64 && prec != BDGX_FP32)) return BDGX_EINVAL;
+  if (c->tuned[prec == BDGX_FP64 ? 0 : 1] && !force) return BDGX_OK;
+  return autotune_dispatch(c, prec, nbatch);
+}
+
+int bdgx_get_launch_variant(const bdgx_ctx *c, int kind, int prec, int *variant) {
+  if (!c || !variant || kind < 0 || kind > 2 || (prec != BDGX_FP64 && prec != BDGX_FP32))
+    return BDGX_EINVAL;
+  *variant = c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind];
+  return BDGX_OK;
+}
+
+int bdgx_set_launch_variant(bdgx_ctx *c, int kind, int prec, int variant) {
+  if (!c || kind < 0 || kind > 2 || (prec != BDGX_FP64 && prec != BDGX_FP32) || variant < 0 ||
+      variant > 1)
+    return BDGX_EINVAL;
+  c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind] = variant;
+  return BDGX_OK;
+}
+
 int bdgx_bench_pass(bdgx_ctx *c, int pass, int prec, uint64_t nbatch, int iters, float *ms,
                     double *bytes) {
   if (!c || !ms || !bytes || nbatch == 0 || iters < 0) return BDGX_EINVAL;

@@ -68,6 +68,7 @@ struct bdg_t {
   float *lp_theta; /* learned preconditioner parameters (NULL: analytic) */
   int lp_window;
   double lp_factor;
+  int graphs; /* replay the per-iteration device chain as HIP graphs (default 1) */
   int verbosity;
   /* results */
   double *eigvals;
@@ -118,6 +119,7 @@ typedef struct {
   int lp_window;         /* learned: back to T_M T_K when the worst residual has  */
   double lp_factor;      /*   not dropped below lp_factor x its value lp_window   */
                          /*   iterations earlier                                   */
+  int graphs;            /* replay the per-iteration device chain as HIP graphs   */
 } lobpcg_cfg;
 
 typedef struct {
@@ -127,6 +129,7 @@ typedef struct {
   int iters, restarts, precond_fallbacks;
   uint64_t applies, applies_fp32, refreshes;
   int learned_its, learned_fallback_at;
+  uint64_t graph_launches, graph_captures;
   double rr_seconds;
   int status;
 } lobpcg_result;

@@ -221,8 +221,8 @@ typedef struct {
 /* full device Grams on V[:, 0:k].  Product form: V^H V, V^H MV, MV^H KMV.
  * Standard form (std != 0, the "M" images hold K V and the "KM" images K^2 V):
  * V^H V, V^H V, V^H K^2 V. */
-static int grams_full(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, uint64_t S, int w,
-                      double *dG, gram3 *h) {
+static int grams_full_dev(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, uint64_t S,
+                          int w, double *dG) {
   const size_t gsz = (size_t)k * k * w;
   int rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, b->V, S, (uint64_t)k, S, 0, dG);
   if (!rc && !std)
@@ -230,9 +230,13 @@ static int grams_full(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, ui
   if (!rc)
     rc = bdgx_gram(ctx, prec, std ? b->V : b->MV, S, (uint64_t)k, b->KMV, S, (uint64_t)k, S, 0,
                    dG + 2 * gsz);
-  if (rc) return rc;
+  return rc;
+}
+
+static int grams_full_host(bdgx_ctx *ctx, int std, int k, int w, const double *dG, gram3 *h) {
+  const size_t gsz = (size_t)k * k * w;
   double *tmp = (double *)malloc(sizeof(double) * 3 * gsz);
-  rc = bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * 3 * gsz);
+  int rc = bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * 3 * gsz);
   if (!rc) {
     memcpy(h->V, tmp, sizeof(double) * gsz);
     memcpy(h->M, std ? tmp : tmp + gsz, sizeof(double) * gsz);
@@ -242,10 +246,16 @@ static int grams_full(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, ui
   return rc;
 }
 
+static int grams_full(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, uint64_t S, int w,
+                      double *dG, gram3 *h) {
+  int rc = grams_full_dev(ctx, prec, std, b, k, S, w, dG);
+  return rc ? rc : grams_full_host(ctx, std, k, w, dG, h);
+}
+
 /* device Grams of V[:, 0:k] against the columns [c0, c0+nc) of V; written
  * into columns c0.. of the full k x k sets, and mirrored into the rows. */
-static int grams_cols(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, int c0, int nc,
-                      uint64_t S, int w, size_t es, double *dG, gram3 *h) {
+static int grams_cols_dev(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, int c0, int nc,
+                          uint64_t S, int w, size_t es, double *dG) {
   const size_t csz = (size_t)k * nc * w;
   int rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, colp(b->V, (uint64_t)c0, S, es), S,
                      (uint64_t)nc, S, 0, dG);
@@ -255,9 +265,14 @@ static int grams_cols(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, in
   if (!rc)
     rc = bdgx_gram(ctx, prec, std ? b->V : b->MV, S, (uint64_t)k,
                    colp(b->KMV, (uint64_t)c0, S, es), S, (uint64_t)nc, S, 0, dG + 2 * csz);
-  if (rc) return rc;
+  return rc;
+}
+
+static int grams_cols_host(bdgx_ctx *ctx, int std, int k, int c0, int nc, int w,
+                           const double *dG, gram3 *h) {
+  const size_t csz = (size_t)k * nc * w;
   double *tmp = (double *)malloc(sizeof(double) * 3 * csz);
-  rc = bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * 3 * csz);
+  int rc = bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * 3 * csz);
   if (!rc) {
     if (std) memcpy(tmp + csz, tmp, sizeof(double) * csz);
     double *dst[3] = {h->V, h->M, h->A};
@@ -330,6 +345,139 @@ static void sub_block(const double *G, int k, const int *sel, int ks, int w, dou
     for (int i = 0; i < ks; ++i) setz(out, ks, i, j, w, getz(G, k, sel[i], sel[j], w));
 }
 
+/* ---- the per-iteration device chain ------------------------------------ */
+
+/* W into column wc.. of A.V from the residual columns Rsrc: the learned
+ * network (FP32), T_M T_K in FP64, or T_M T_K in FP32 (mixed) */
+typedef struct {
+  const void *rsrc;
+  void *other, *f1, *f2;
+  int wc, nact, k, full, lp, mixed, std, opM, w;
+  uint64_t S;
+  size_t es;
+  double *dG;
+} chain_args;
+
+static int chain_w(bdgx_ctx *ctx, const blk3 *A, const chain_args *c) {
+  const uint64_t S = c->S, n = (uint64_t)c->nact;
+  void *wdst = colp(A->V, (uint64_t)c->wc, S, c->es);
+  int rc;
+  if (c->lp) {
+    rc = bdgx_convert(ctx, BDGX_FP64, c->rsrc, S, BDGX_FP32, c->f1, S, S, n);
+    if (!rc) rc = bdgx_lp_apply(ctx, BDGX_FP32, c->f1, c->f2, n, S);
+    if (!rc) rc = bdgx_convert(ctx, BDGX_FP32, c->f2, S, BDGX_FP64, wdst, S, S, n);
+  } else if (!c->mixed) {
+    rc = bdgx_apply(ctx, BDGX_OP_PRECOND_K, BDGX_FP64, c->rsrc, c->other, n, S);
+    if (!rc) rc = bdgx_apply(ctx, BDGX_OP_PRECOND_M, BDGX_FP64, c->other, wdst, n, S);
+  } else {
+    /* T_K, T_M in FP32: W is a search direction, its rounding only changes
+     * which direction is searched, not the accuracy of the basis */
+    rc = bdgx_convert(ctx, BDGX_FP64, c->rsrc, S, BDGX_FP32, c->f1, S, S, n);
+    if (!rc) rc = bdgx_apply(ctx, BDGX_OP_PRECOND_K, BDGX_FP32, c->f1, c->f2, n, S);
+    if (!rc) rc = bdgx_apply(ctx, BDGX_OP_PRECOND_M, BDGX_FP32, c->f2, c->f1, n, S);
+    if (!rc) rc = bdgx_convert(ctx, BDGX_FP32, c->f1, S, BDGX_FP64, wdst, S, S, n);
+  }
+  return rc;
+}
+
+/* FP64 images M W, K M W of the W columns */
+static int chain_images(bdgx_ctx *ctx, const blk3 *A, const chain_args *c) {
+  const uint64_t S = c->S, n = (uint64_t)c->nact, wc = (uint64_t)c->wc;
+  int rc = bdgx_apply(ctx, c->opM, BDGX_FP64, colp(A->V, wc, S, c->es), colp(A->MV, wc, S, c->es),
+                      n, S);
+  if (!rc)
+    rc = bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, colp(A->MV, wc, S, c->es),
+                    colp(A->KMV, wc, S, c->es), n, S);
+  return rc;
+}
+
+static int chain_grams(bdgx_ctx *ctx, const blk3 *A, const chain_args *c) {
+  if (c->full) return grams_full_dev(ctx, BDGX_FP64, c->std, A, c->k, c->S, c->w, c->dG);
+  return grams_cols_dev(ctx, BDGX_FP64, c->std, A, c->k, c->wc, c->nact, c->S, c->w, c->es,
+                        c->dG);
+}
+
+static int chain_all(bdgx_ctx *ctx, const blk3 *A, const chain_args *c) {
+  int rc = chain_w(ctx, A, c);
+  if (!rc) rc = chain_images(ctx, A, c);
+  if (!rc) rc = chain_grams(ctx, A, c);
+  return rc;
+}
+
+/* Graph cache: the chain W -> images -> Grams is captured once per distinct
+ * argument set (basis buffer, residual source, column counts, Gram kind,
+ * preconditioner) after one eager run of that set, and replayed from then
+ * on.  Launch is refused when the ctx workspace moved after the capture; the
+ * slot is then captured again. */
+#define NGSLOT 8
+typedef struct {
+  const void *v, *r;
+  int wc, nact, k, full, lp;
+} gkey;
+typedef struct {
+  gkey key;
+  bdgx_graph *g;
+  int used, seen;
+  unsigned long stamp;
+} gslot;
+
+static int gkey_eq(const gkey *a, const gkey *b) {
+  return a->v == b->v && a->r == b->r && a->wc == b->wc && a->nact == b->nact &&
+         a->k == b->k && a->full == b->full && a->lp == b->lp;
+}
+
+static int chain_run(bdgx_ctx *ctx, const blk3 *A, const chain_args *c, int use_graph,
+                     gslot *slots, unsigned long *clock, lobpcg_result *res) {
+  if (!use_graph) return chain_all(ctx, A, c);
+  gkey key;
+  memset(&key, 0, sizeof(key));
+  key.v = A->V;
+  key.r = c->rsrc;
+  key.wc = c->wc;
+  key.nact = c->nact;
+  key.k = c->k;
+  key.full = c->full;
+  key.lp = c->lp;
+  gslot *sl = NULL;
+  for (int i = 0; i < NGSLOT; ++i)
+    if (slots[i].used && gkey_eq(&slots[i].key, &key)) sl = &slots[i];
+  if (!sl) { /* least recently used slot */
+    sl = &slots[0];
+    for (int i = 1; i < NGSLOT; ++i)
+      if (!slots[i].used || (sl->used && slots[i].stamp < sl->stamp)) sl = &slots[i];
+    bdgx_graph_destroy(sl->g);
+    memset(sl, 0, sizeof(*sl));
+    sl->key = key;
+    sl->used = 1;
+  }
+  sl->stamp = ++*clock;
+  if (sl->g) {
+    const int rc = bdgx_graph_launch(ctx, sl->g);
+    if (rc != BDGX_ESTATE) {
+      if (!rc) res->graph_launches++;
+      return rc;
+    }
+    bdgx_graph_destroy(sl->g);
+    sl->g = NULL;
+    sl->seen = 0;
+  }
+  if (sl->seen++ == 0) return chain_all(ctx, A, c); /* first use: eager (sizes workspaces) */
+  if (bdgx_graph_begin(ctx)) return chain_all(ctx, A, c);
+  int rc = chain_all(ctx, A, c);
+  if (rc) {
+    (void)bdgx_graph_end(ctx, NULL);
+    return chain_all(ctx, A, c);
+  }
+  if (bdgx_graph_end(ctx, &sl->g) || !sl->g) {
+    sl->g = NULL;
+    return chain_all(ctx, A, c);
+  }
+  res->graph_captures++;
+  rc = bdgx_graph_launch(ctx, sl->g);
+  if (!rc) res->graph_launches++;
+  return rc;
+}
+
 int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lobpcg_result *res,
                       void **dXout) {
   const uint64_t S = cfg->S;
@@ -385,8 +533,17 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   res->learned_its = 0;
   res->learned_fallback_at = 0;
   int lp_on = cfg->learned;
+  gslot gslots[NGSLOT];
+  unsigned long gclock = 0;
+  memset(gslots, 0, sizeof(gslots));
+  res->graph_launches = 0;
+  res->graph_captures = 0;
   double *worst_hist = (double *)calloc((size_t)cfg->maxiter + 2, sizeof(double));
 
+  /* launch shapes of the FFT kernels for this grid and block width (once per
+   * context and precision) */
+  TRY(bdgx_autotune(ctx, BDGX_FP64, (uint64_t)(m < 32 ? m : 32), 0));
+  if (mixed || cfg->learned) TRY(bdgx_autotune(ctx, BDGX_FP32, (uint64_t)(m < 32 ? m : 32), 0));
   TRY(alloc3(&A, (size_t)S * kmax * es));
   TRY(alloc3(&B, (size_t)S * kmax * es));
   TRY(bdgx_malloc(&dR, (size_t)S * m * es));
@@ -494,29 +651,29 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     const int img32 =
         mixed && cfg->img_tol > 0 && nact >= cfg->mixed_min_cols && st.worst > cfg->img_tol;
     void *wdst = colp(A.V, (uint64_t)wc, S, es);
-    if (lp_on) {
-      /* learned network on the residuals (FP32) */
-      TRY(bdgx_convert(ctx, BDGX_FP64, Rsrc, S, BDGX_FP32, f1, S, S, (uint64_t)nact));
-      TRY(bdgx_lp_apply(ctx, BDGX_FP32, f1, f2, (uint64_t)nact, S));
-      TRY(bdgx_convert(ctx, BDGX_FP32, f2, S, BDGX_FP64, wdst, S, S, (uint64_t)nact));
-      res->learned_its++;
-    } else if (!mixed) {
-      TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_K, BDGX_FP64, Rsrc, other, (uint64_t)nact, S));
-      TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_M, BDGX_FP64, other, wdst, (uint64_t)nact, S));
-    } else {
-      /* T_K, T_M in FP32: W is a search direction, its rounding only changes
-       * which direction is searched, not the accuracy of the basis */
-      TRY(bdgx_convert(ctx, BDGX_FP64, Rsrc, S, BDGX_FP32, f1, S, S, (uint64_t)nact));
-      TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_K, BDGX_FP32, f1, f2, (uint64_t)nact, S));
-      TRY(bdgx_apply(ctx, BDGX_OP_PRECOND_M, BDGX_FP32, f2, f1, (uint64_t)nact, S));
-      TRY(bdgx_convert(ctx, BDGX_FP32, f1, S, BDGX_FP64, wdst, S, S, (uint64_t)nact));
-    }
+    chain_args ca;
+    ca.rsrc = Rsrc;
+    ca.other = other;
+    ca.f1 = f1;
+    ca.f2 = f2;
+    ca.wc = wc;
+    ca.nact = nact;
+    ca.k = k;
+    ca.full = full;
+    ca.lp = lp_on;
+    ca.mixed = mixed;
+    ca.std = std;
+    ca.opM = opM;
+    ca.w = w;
+    ca.S = S;
+    ca.es = es;
+    ca.dG = dG;
+    if (lp_on) res->learned_its++;
     if (!img32) {
-      TRY(bdgx_apply(ctx, opM, BDGX_FP64, wdst, colp(A.MV, (uint64_t)wc, S, es),
-                     (uint64_t)nact, S));
-      TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, colp(A.MV, (uint64_t)wc, S, es),
-                     colp(A.KMV, (uint64_t)wc, S, es), (uint64_t)nact, S));
+      /* W, FP64 images, Gram columns: one device chain (graph-replayed) */
+      TRY(chain_run(ctx, &A, &ca, cfg->graphs, gslots, &gclock, res));
     } else {
+      TRY(chain_w(ctx, &A, &ca));
       /* FP32 images of W, with W first projected out of span(X) (Euclidean,
        * FP64) so that the image error is relative to the new part of W */
       const size_t zsz = (size_t)m * nact * w;
@@ -550,15 +707,16 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
                        (uint64_t)nact));
       res->applies_fp32 += (uint64_t)nact;
       fresh = 0;
+      TRY(chain_grams(ctx, &A, &ca));
     }
     res->applies += (uint64_t)nact;
 
-    /* Gram set of [X | P | W] */
+    /* Gram set of [X | P | W] on the host */
     if (full) {
       since_full = 0;
-      TRY(grams_full(ctx, BDGX_FP64, std, &A, k, S, w, dG, &G));
+      TRY(grams_full_host(ctx, std, k, w, dG, &G));
     } else {
-      TRY(grams_cols(ctx, BDGX_FP64, std, &A, k, wc, nact, S, w, es, dG, &G));
+      TRY(grams_cols_host(ctx, std, k, wc, nact, w, dG, &G));
     }
     /* Rayleigh-Ritz on a column selection: all, or without P when [X P W]
      * is numerically dependent */
@@ -643,6 +801,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     res->res[j] = st.resv[j];
   }
 done:
+  for (int i = 0; i < NGSLOT; ++i) bdgx_graph_destroy(gslots[i].g);
   res->iters = it;
   res->rr_seconds = trr;
   res->status = rc;

@@ -403,6 +403,7 @@ static int ensure_lp_tmp(bdgx_ctx *c, size_t bytes) {
   }
   BDGX_HIP(hipMalloc(&c->lp_tmp, bytes));
   c->lp_tmp_bytes = bytes;
+  c->ws_gen++;
   return BDGX_OK;
 }
 

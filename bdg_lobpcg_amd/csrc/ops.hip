@@ -875,6 +875,76 @@ static int bench_pass_t(bdgx_ctx *c, int pass, uint64_t nbatch, int iters, float
   return rc;
 }
 
+// Launch-shape autotuner: every variant of every fixed-size kernel kind the
+// grid uses, timed on nbatch kinetic batches (1 warm-up + 3 timed launches,
+// HIP events on the ctx stream); the fastest variant is kept per kind.
+template <typename R>
+static int autotune_t(bdgx_ctx *c, uint64_t nbatch) {
+  const int pi = sizeof(R) == 8 ? 0 : 1;
+  const bool plane = plane_supported<R>(c), zfix = zpass_fixed_supported<R>(c);
+  if ((!plane && !zfix) || nbatch == 0) {
+    c->tuned[pi] = 1;
+    return BDGX_OK;
+  }
+  const Geo &g = c->geo;
+  const int nc = (int)nbatch;
+  const size_t es = sizeof(R) * (c->cplx ? 2 : 1);
+  int rc = ensure_spec(c, sizeof(cx<R>) * (size_t)g.Kk * nbatch);
+  if (rc) return rc;
+  cx<R> *spec = reinterpret_cast<cx<R> *>(c->spec);
+  R *X = nullptr, *Y = nullptr;
+  BDGX_HIP(hipMalloc((void **)&X, es * (size_t)g.S * nbatch));
+  if (hipMalloc((void **)&Y, es * (size_t)g.S * nbatch) != hipSuccess) {
+    (void)hipFree(X);
+    return BDGX_EHIP;
+  }
+  (void)hipMemsetAsync(X, 0, es * (size_t)g.S * nbatch, c->stream);
+  (void)hipMemsetAsync(spec, 0, sizeof(cx<R>) * (size_t)g.Kk * nbatch, c->stream);
+  HalfJob J = job_kin(0, F_NONE);
+  int mul[kMaxChan] = {MUL_KIN, MUL_KIN, MUL_KIN};
+  XArgs<R> a = make_xargs<R>(c, J, 0, nc, X, Y, (long long)g.S, spec, (long long)nc * g.Kk,
+                             nullptr);
+  const YZArgs<R> yz = make_yz<R>(c, mul, 1, nc, spec);
+  const int kinds[3] = {BDGX_TUNE_PLANE_FWD, BDGX_TUNE_Z, BDGX_TUNE_PLANE_INV};
+  for (int q = 0; q < 3 && !rc; ++q) {
+    const int kind = kinds[q];
+    if (kind == BDGX_TUNE_Z ? !zfix : !plane) continue;
+    auto one = [&]() -> int {
+      // launch_z fills in the z twiddles and plan before the fixed z pass
+      if (kind == BDGX_TUNE_Z) return launch_z<R>(c, yz);
+      return launch_plane_x<R>(c, a, kind == BDGX_TUNE_PLANE_INV);
+    };
+    int best = c->tune_var[pi][kind];
+    float best_ms = 0.f;
+    for (int v = 0; v < 2 && !rc; ++v) {
+      c->tune_var[pi][kind] = v;
+      rc = one();
+      if (!rc && hipEventRecord(c->ev0, c->stream) != hipSuccess) rc = BDGX_EHIP;
+      for (int i = 0; i < 3 && !rc; ++i) rc = one();
+      if (!rc && hipEventRecord(c->ev1, c->stream) != hipSuccess) rc = BDGX_EHIP;
+      if (!rc && hipEventSynchronize(c->ev1) != hipSuccess) rc = BDGX_EHIP;
+      float t = 0.f;
+      if (!rc && hipEventElapsedTime(&t, c->ev0, c->ev1) != hipSuccess) rc = BDGX_EHIP;
+      if (!rc && (v == 0 || t < best_ms)) {
+        best = v;
+        best_ms = t;
+      }
+    }
+    c->tune_var[pi][kind] = best;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(X);
+  (void)hipFree(Y);
+  if (!rc) c->tuned[pi] = 1;
+  return rc;
+}
+
+int autotune_dispatch(bdgx_ctx *c, int prec, uint64_t nbatch) {
+  if (prec == BDGX_FP64) return autotune_t<double>(c, nbatch);
+  if (prec == BDGX_FP32) return autotune_t<float>(c, nbatch);
+  return BDGX_EINVAL;
+}
+
 int bench_pass_dispatch(bdgx_ctx *c, int pass, int prec, uint64_t nbatch, int iters, float *ms,
                         double *bytes) {
   if (prec == BDGX_FP64) return bench_pass_t<double>(c, pass, nbatch, iters, ms, bytes);

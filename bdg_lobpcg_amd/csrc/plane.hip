@@ -455,15 +455,36 @@ bool zpass_fixed_supported(const bdgx_ctx *c) {
   return nz == 32 || nz == 64 || nz == 128;
 }
 
+// Launch shapes of the fixed z pass: variant 0 = (P, T) with T = 256 at
+// NZ = 128 and 128 below; variant 1 halves T (two butterflies per thread in
+// the radix-8 stages).  P = 8 (FP64) or 16 (FP32) adjacent kx pencils: one
+// 128-byte segment per z row.
+template <typename R, int NZ, int T>
+static void launch_z_nt(const YZArgs<R> &z, dim3 grid, hipStream_t s) {
+  constexpr int P = sizeof(R) == 8 ? 8 : 16;
+  hipLaunchKernelGGL((k_zpass_fixed<R, NZ, P, T>), grid, dim3(T), 0, s, z);
+}
+
 template <typename R>
 int launch_zpass_fixed(bdgx_ctx *c, const YZArgs<R> &z) {
   constexpr int P = sizeof(R) == 8 ? 8 : 16;
   const Geo &g = c->geo;
+  if (!z.tw || !z.spec || !z.kx2 || !z.ky2 || !z.kz2 || z.nb < 1) return BDGX_EINVAL;
   const dim3 grid((unsigned)(z_tiles_per_batch<P>(g.n0k, g.ny) * z.nb));
+  const int v = c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_Z];
   switch (g.nz) {
-    case 32: hipLaunchKernelGGL((k_zpass_fixed<R, 32, P, 128>), grid, dim3(128), 0, c->stream, z); break;
-    case 64: hipLaunchKernelGGL((k_zpass_fixed<R, 64, P, 128>), grid, dim3(128), 0, c->stream, z); break;
-    case 128: hipLaunchKernelGGL((k_zpass_fixed<R, 128, P, 256>), grid, dim3(256), 0, c->stream, z); break;
+    case 32:
+      if (v) launch_z_nt<R, 32, 64>(z, grid, c->stream);
+      else launch_z_nt<R, 32, 128>(z, grid, c->stream);
+      break;
+    case 64:
+      if (v) launch_z_nt<R, 64, 64>(z, grid, c->stream);
+      else launch_z_nt<R, 64, 128>(z, grid, c->stream);
+      break;
+    case 128:
+      if (v) launch_z_nt<R, 128, 128>(z, grid, c->stream);
+      else launch_z_nt<R, 128, 256>(z, grid, c->stream);
+      break;
     default: return BDGX_EINVAL;
   }
   BDGX_HIP(hipGetLastError());
@@ -491,23 +512,43 @@ bool plane_supported(const bdgx_ctx *c) {
   return g.nx == 32 || g.nx == 64 || g.nx == 128;
 }
 
+// Launch shapes of the whole-plane kernels: variant 0 = 4N threads,
+// variant 1 = 8N threads (one workgroup per plane either way).
+template <typename R, int N>
+static void launch_plane_v(const XArgs<R> &a, bool inverse, int v, hipStream_t s) {
+  if (v)
+    launch_plane<R, N, 8 * N>(a, inverse, s);
+  else
+    launch_plane<R, N, 4 * N>(a, inverse, s);
+}
+
 template <typename R>
 int launch_plane_x(bdgx_ctx *c, const XArgs<R> &a, bool inverse) {
+  if (!a.tw || !a.tw_y || !a.spec || !a.X || (inverse && !a.Y) || a.ncols < 1)
+    return BDGX_EINVAL;
+  const int v = c->tune_var[sizeof(R) == 8 ? 0 : 1][inverse ? BDGX_TUNE_PLANE_INV
+                                                              : BDGX_TUNE_PLANE_FWD];
   switch (c->geo.nx) {
-    case 32: launch_plane<R, 32, 256>(a, inverse, c->stream); break;
-    case 64: launch_plane<R, 64, 512>(a, inverse, c->stream); break;
-    case 128:
-      // thread counts measured per kernel: FP64 (one 133 KB plane per CU)
-      // runs faster with 1024 threads, FP32 (two 67 KB planes per CU) with 512
-      if (sizeof(R) == 8)
-        launch_plane<R, 128, 1024>(a, inverse, c->stream);
-      else
-        launch_plane<R, 128, 512>(a, inverse, c->stream);
-      break;
+    case 32: launch_plane_v<R, 32>(a, inverse, v, c->stream); break;
+    case 64: launch_plane_v<R, 64>(a, inverse, v, c->stream); break;
+    case 128: launch_plane_v<R, 128>(a, inverse, v, c->stream); break;
     default: return BDGX_EINVAL;
   }
   BDGX_HIP(hipGetLastError());
   return BDGX_OK;
+}
+
+// Measured defaults (MI355X, 32 columns): at N = 128 FP64 (one 133 KB
+// plane per CU) runs faster with 8N threads, FP32 (two 67 KB planes per CU)
+// with 4N; 8N at N = 32, 64.  The z pass default is variant 0.
+void tune_defaults(bdgx_ctx *c) {
+  for (int p = 0; p < 2; ++p) {
+    const int big = c->geo.nx == 128 ? (p == 0) : 1;
+    c->tune_var[p][BDGX_TUNE_PLANE_FWD] = big;
+    c->tune_var[p][BDGX_TUNE_PLANE_INV] = big;
+    c->tune_var[p][BDGX_TUNE_Z] = 0;
+    c->tuned[p] = 0;
+  }
 }
 
 template bool plane_supported<float>(const bdgx_ctx *);

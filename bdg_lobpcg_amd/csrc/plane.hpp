@@ -55,5 +55,7 @@ template <typename R>
 bool zpass_fixed_supported(const bdgx_ctx *c);
 template <typename R>
 int launch_zpass_fixed(bdgx_ctx *c, const YZArgs<R> &z);
+// measured default launch variants of the plane and fixed z kernels
+void tune_defaults(bdgx_ctx *c);
 
 }  // namespace bdgx

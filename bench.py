@@ -74,9 +74,23 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None):
 
 
 PASSES = ["xfwd", "yfwd", "z", "yinv", "xinv", "plane_fwd", "plane_inv"]
-KERNELS = {"plane_fwd": "k_plane_fwd_real<double, 128, 1024> (x+y forward, whole z-planes)",
-           "plane_inv": "k_plane_inv_real<double, 128, 512> (y+x inverse + combine)",
-           "z": "k_zpass_fixed<double, 128, 8, 256> (z forward, k-space multiply, z inverse)"}
+def kernel_name(pname, cfg, variants):
+    """Name of the FP64 kernel that runs pass `pname` under the tuned launch
+    variants (plane: 4N or 8N threads; z: 256 or 128 threads at Nz = 128)."""
+    nx, nz = cfg["N"][0], cfg["N"][2]
+    v = variants.get(pname, 0)
+    if pname in ("plane_fwd", "plane_inv"):
+        k = "k_plane_fwd_real" if pname == "plane_fwd" else "k_plane_inv_real"
+        return f"{k}<double, {nx}, {(8 if v else 4) * nx}>"
+    if pname == "z":
+        t = (256 if nz == 128 else 128) >> v
+        return f"k_zpass_fixed<double, {nz}, 8, {t}>"
+    return pname
+
+
+KERNEL_ROLE = {"plane_fwd": "x+y forward, whole z-planes",
+               "plane_inv": "y+x inverse + combine",
+               "z": "z forward, k-space multiply, z inverse"}
 
 
 def roofline(cfg):
@@ -87,6 +101,7 @@ def roofline(cfg):
     d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
     out = {}
     for prec, name in ((B.FP32, "fp32"), (B.FP64, "fp64")):
+        out[f"{name}_variants"] = d.autotune(prec, cfg["m"])
         for pid, pname in enumerate(PASSES):
             try:
                 ms, by = d.bench_pass(pid, prec, cfg["m"], iters=10)
@@ -103,6 +118,7 @@ def probe(cfg):
     import bdg_lobpcg_amd as B
 
     d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
+    d.autotune(B.FP64, cfg["m"])
     info = {}
     for name in ("plane_fwd", "z", "plane_inv"):
         _, by = d.bench_pass(PASSES.index(name), B.FP64, cfg["m"], iters=3)
@@ -264,9 +280,13 @@ def main():
     log(f"FP64 {t64:.3f} s, mixed {tmx:.3f} s ({t64 / tmx:.2f}x), mixed+learned {tlr:.3f} s "
         f"({t64 / tlr:.2f}x); max rel err vs FP64: mixed {e_rel_mx:.2e}, mixed+learned {e_rel:.2e}")
     rf = roofline(cfg)
+    variants = {k: rf.pop(k) for k in [k for k in rf if k.endswith("_variants")]}
     log("passes: " + ", ".join(f"{k} {v['GBs']:.0f} GB/s" for k, v in rf.items()))
+    log(f"autotuned launch variants: {variants}")
     dom_key = "fp64_plane_fwd" if "fp64_plane_fwd" in rf else "fp64_z"
     dom = rf[dom_key]
+    dom_pass = dom_key.split("_", 1)[1]
+    dom_kernel = kernel_name(dom_pass, cfg, variants.get("fp64_variants", {}))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, psi, r64["arrays"], r64["stats"]["column_applies"])
@@ -309,10 +329,11 @@ def main():
         "psi0_seconds": t_itp,
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["GBs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(KERNELS[dom_key.split("_", 1)[1]].split(" (")[0]),
-                     "kernel": KERNELS.get(dom_key.split("_", 1)[1], dom_key),
+                     "traffic": measured_traffic(dom_kernel),
+                     "kernel": f"{dom_kernel} ({KERNEL_ROLE.get(dom_pass, dom_pass)})",
                      "bytes_per_launch": dom["bytes"], "ms_per_launch": dom["ms"]},
         "passes": {k: round(v["GBs"], 1) for k, v in rf.items()},
+        "launch_variants": variants,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -32,6 +32,8 @@ typedef struct {
   uint64_t restarts;            /* iterations that left the P block out        */
   uint64_t learned_iterations;  /* iterations whose W came from the network    */
   int learned_fallback_at;      /* iteration of the switch back to T_M T_K, or 0 */
+  uint64_t graph_launches;      /* iterations whose device chain was a graph replay */
+  uint64_t graph_captures;      /* graphs captured during the solve            */
   double seconds_total;         /* bdg_solve wall clock                        */
   double seconds_rr;            /* host Rayleigh-Ritz time                     */
   int status;                   /* 0 ok, else a BDGX_E* code or -100 (RR)     */
@@ -71,6 +73,11 @@ void bdg_set_learned_preconditioner(bdg_t *bdg, const float *theta);
 void bdg_set_learned_fallback(bdg_t *bdg, int window, f64 factor);
 
 /* 0 silent (default), 1 per-phase summary, 2 per-iteration trace (stderr). */
+/* Replay the per-iteration device chain (preconditioner, images, Gram
+ * columns) as captured HIP graphs (default on; 0 runs every launch eagerly).
+ * Results are the same either way. */
+void bdg_set_graph_capture(bdg_t *bdg, int on);
+
 void bdg_set_verbosity(bdg_t *bdg, int level);
 
 /* Built-in host callbacks usable with bdg_set_trap / bdg_set_local_interactions

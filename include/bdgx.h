@@ -154,6 +154,39 @@ enum { BDGX_PASS_XFWD = 0, BDGX_PASS_YFWD = 1, BDGX_PASS_Z = 2, BDGX_PASS_YINV =
 int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iters,
                     float *ms_per_launch, double *bytes_per_launch);
 
+/* Launch-shape autotuner (no reference counterpart; it takes the place of
+ * the reference's FFTW_PATIENT planning, src/matmul_ctx.c:134-171).  The
+ * fixed-size FFT kernels (whole-plane forward / inverse, fixed z pass) have
+ * launch variants: plane 0 = 4 Nx threads, 1 = 8 Nx threads per plane; z
+ * 0 = 256 (Nz = 128) or 128 threads, 1 = half of that.  bdgx_autotune times
+ * every variant of every kind for `prec` on `nbatch` kinetic batches (1
+ * warm-up + 3 timed launches each, HIP events on the ctx stream) and keeps
+ * the fastest; it returns at once when `prec` is already tuned and `force`
+ * is 0, and does nothing (returns 0) on grids that use the pencil passes.
+ * The variants compute identical results up to rounding order. */
+enum { BDGX_TUNE_PLANE_FWD = 0, BDGX_TUNE_PLANE_INV = 1, BDGX_TUNE_Z = 2 };
+int bdgx_autotune(bdgx_ctx *ctx, int prec, uint64_t nbatch, int force);
+int bdgx_get_launch_variant(const bdgx_ctx *ctx, int kind, int prec, int *variant);
+int bdgx_set_launch_variant(bdgx_ctx *ctx, int kind, int prec, int variant);
+
+/* Graph capture of device work on the ctx stream (no reference counterpart;
+ * used by the solver to replay its per-iteration operator / Gram chain).
+ * Between bdgx_graph_begin and bdgx_graph_end the bdgx calls on this ctx are
+ * recorded, not run; calls that allocate, free, synchronise or copy through
+ * host memory are not allowed in between (they fail, and the capture must
+ * then be ended with out == NULL, which discards it).  The workspace must
+ * not need to grow during capture: reserve it first (bdgx_reserve).  A graph
+ * replays exactly the recorded kernels on the recorded pointers; launching
+ * it after the ctx workspace has moved returns BDGX_ESTATE. */
+typedef struct bdgx_graph bdgx_graph;
+int bdgx_graph_begin(bdgx_ctx *ctx);
+int bdgx_graph_end(bdgx_ctx *ctx, bdgx_graph **out);
+int bdgx_graph_launch(bdgx_ctx *ctx, const bdgx_graph *g);
+void bdgx_graph_destroy(bdgx_graph *g);
+/* Grow the spectrum workspace and the small scratch to at least the given
+ * sizes now (so that later calls of up to that size do not reallocate). */
+int bdgx_reserve(bdgx_ctx *ctx, uint64_t spec_bytes, uint64_t scratch_bytes);
+
 /* ---- learned preconditioner (no reference counterpart; its analytic base
  *      is precondK/precondM, reference src/precond_impl.inc:21-111) --------
  *   N(r) = T'_M(V),  V = U + s0 sig sum_h v_h tanh(sum_nb w_{h,nb} U(x+nb)/(s0 sig) + d_h),

@@ -94,3 +94,71 @@ def test_gram_combine(cplx, prec):
     Y = d.combine(A, Cm, prec)
     refY = A32.astype(ref.dtype) @ Cm
     assert relerr(Y, refY) < (1e-13 if prec == 0 else 1e-6)
+
+
+@pytest.mark.parametrize("grid", [((32, 32, 64), (16.0, 16.0, 24.0)),
+                                  ((16, 12, 32), (12.0, 10.0, 16.0)),
+                                  ((128, 128, 4), (24.0, 24.0, 4.0))],
+                         ids=lambda g: "x".join(map(str, g[0])))
+def test_launch_variants_and_autotune(grid):
+    """Every launch variant of the plane and fixed z kernels gives the
+    oracle's result; the autotuner picks valid variants."""
+    import bdg_lobpcg_amd as B
+    N, L = grid
+    c, f = physics(N, L, False, dipolar=True)
+    d = device_twin(N, L, False, f)
+    X = rand_block(c.size, 3, False, seed=5)
+    refs = {op: c.apply(op, X) for op in ("K", "M", "precondK")}
+    for prec in (0, 1):
+        for kind in range(3):
+            for v in (0, 1):
+                d.set_launch_variant(kind, prec, v)
+                for op, ref in refs.items():
+                    e = relerr(d.apply(op, X, prec), ref)
+                    assert e < TOL[prec], f"kind {kind} variant {v} prec {prec} {op}: {e:.2e}"
+        chosen = d.autotune(prec, 8, force=True)
+        assert set(chosen.values()) <= {0, 1}
+        for op, ref in refs.items():
+            assert relerr(d.apply(op, X, prec), ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+def test_graph_replay_matches_eager(prec):
+    """A captured chain (operator applies + Gram) replayed several times, with
+    the program's own eager work and synchronisations between the replays,
+    reproduces the eager result exactly on every replay."""
+    import bdg_lobpcg_amd as B
+    N, L = (32, 32, 64), (16.0, 16.0, 24.0)
+    c, f = physics(N, L, False, dipolar=True)
+    d = device_twin(N, L, False, f)
+    S, n = c.size, 4
+    dt = np.float64 if prec == 0 else np.float32
+    X = np.asfortranarray(rand_block(S, n, False, seed=11).astype(dt))
+    dX = d.upload(X)
+    dY, dZ = B.DeviceBuffer(X.nbytes), B.DeviceBuffer(X.nbytes)
+    dG = B.DeviceBuffer(8 * n * n)
+
+    def chain():
+        d.apply_device("precondK", prec, dX, dY, n, S)
+        d.apply_device("M", prec, dY, dZ, n, S)
+        d.apply_device("K", prec, dZ, dY, n, S)
+        B.check(B.lib().bdgx_gram(d.h, prec, dX.ptr, S, n, dY.ptr, S, n, S, 0, dG.ptr), "gram")
+
+    chain()
+    d.sync()
+    Y_ref = d.download(dY, (S, n), dt)
+    G_ref = d.download(dG, (n, n), np.float64)
+    g = d.capture(chain)
+    junk = d.upload(np.asfortranarray(rand_block(S, n, False, seed=12).astype(dt)))
+    for r in range(4):
+        # eager work in between: overwrite the outputs, apply another operator
+        d.apply_device("precondM", prec, junk, dY, n, S)
+        d.apply_device("kinetic", prec, junk, dZ, n, S)
+        d.sync()
+        g.launch()
+        d.sync()
+        Y = d.download(dY, (S, n), dt)
+        G = d.download(dG, (n, n), np.float64)
+        assert np.array_equal(Y, Y_ref), f"replay {r}: operator chain differs"
+        assert np.array_equal(G, G_ref), f"replay {r}: Gram differs"
+    g.close()

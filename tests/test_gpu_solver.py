@@ -60,3 +60,32 @@ def test_3d_dipolar_vs_oracle(prec):
     np.testing.assert_allclose(w[nz], ref["w"][nz], rtol=1e-9)
     assert np.all(np.abs(w[z] - ref["w"][z]) < 1e-6)
     assert p.converged() >= nev
+
+
+def test_graph_capture_gives_identical_solve():
+    """The solver's graph-replayed iteration chain gives the same iterations
+    and eigenvalues as the eager chain."""
+    import bdg_lobpcg_amd as B
+    N, L = (32, 32, 32), (14.0, 14.0, 14.0)
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
+    X0 = c.start_block_uniform(42, 8)
+    out = {}
+    for on in (False, True):
+        p = B.Problem(N, L, False)
+        p.set_system()
+        p.set_trap_harmonic((1.0, 1.0, 1.0))
+        p.set_wavefunction(f["wf"])
+        p.set_contact(20.0)
+        p.set_dipolar(10.0, (0.0, 0.0, 1.0), f["R"])
+        p.set_mu(1.0)
+        p.set_solver_params(6, 8, 300, 1e-9)
+        p.set_init_custom_block(X0)
+        p.set_graph_capture(on)
+        assert p.solve() == 0
+        out[on] = (p.eigenvalues(6), p.stats())
+        p.close()
+    (w0, s0), (w1, s1) = out[False], out[True]
+    assert s0["graph_launches"] == 0
+    assert s1["graph_launches"] > 0 and s1["graph_captures"] > 0
+    assert s0["iterations"] == s1["iterations"]
+    np.testing.assert_array_equal(w0, w1)
