@@ -564,7 +564,7 @@ template <typename R, bool CPLX>
 __global__ void __launch_bounds__(256) k_residual(const R *AX, const R *X, const R *MX,
                                                   long long ld, long long nrows,
                                                   const double *lam, int swap_x, R *Rr,
-                                                  long long ldr, double *n2) {
+                                                  long long ldr, double *part) {
   constexpr int W = CPLX ? 2 : 1;
   const int col = blockIdx.y;
   const long long half = nrows / 2;
@@ -600,7 +600,9 @@ __global__ void __launch_bounds__(256) k_residual(const R *AX, const R *X, const
     }
     __syncthreads();
   }
-  if (threadIdx.x < 3) atomicAdd(n2 + 3 * col + threadIdx.x, red[threadIdx.x][0]);
+  // per-block partials, summed in a fixed order by reduce_partials
+  if (threadIdx.x < 3)
+    part[(long long)blockIdx.x * (3 * gridDim.y) + 3 * col + threadIdx.x] = red[threadIdx.x][0];
 }
 
 template <typename R, bool CPLX>
@@ -608,16 +610,20 @@ static int residual_t(bdgx_ctx *c, const void *dAX, const void *dX, const void *
                       uint64_t ld, uint64_t ncols, uint64_t nrows, const double *dlam,
                       int swap_x, void *dR, uint64_t ldr, double *dn2) {
   if (ncols == 0) return BDGX_OK;
-  BDGX_HIP(hipMemsetAsync(dn2, 0, sizeof(double) * 3 * ncols, c->stream));
   long long nbx = ((long long)nrows + 255) / 256;
   if (nbx > 512) nbx = 512;
+  // block partials (nbx x 3 ncols) and the reduction's tmp (8 x 3 ncols)
+  const long long len = 3 * (long long)ncols;
+  int rc = ensure_scratch(c, sizeof(double) * (size_t)((nbx + 8) * len));
+  if (rc) return rc;
+  double *part = reinterpret_cast<double *>(c->scratch);
   hipLaunchKernelGGL((k_residual<R, CPLX>), dim3((unsigned)nbx, (unsigned)ncols), dim3(256),
                      0, c->stream, reinterpret_cast<const R *>(dAX),
                      reinterpret_cast<const R *>(dX), reinterpret_cast<const R *>(dMX),
                      (long long)ld, (long long)nrows, dlam, swap_x, reinterpret_cast<R *>(dR),
-                     (long long)ldr, dn2);
+                     (long long)ldr, part);
   BDGX_HIP(hipGetLastError());
-  return BDGX_OK;
+  return reduce_partials(c, part, (int)nbx, len, part + nbx * len, dn2);
 }
 
 int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,

@@ -187,15 +187,40 @@ __global__ void k_itp_pot(R *psi, const double *V, const R *phi, double g, doubl
     if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && norm) atomicAdd(norm, red[0]);
+  if (threadIdx.x == 0 && norm) norm[blockIdx.x] = red[0];  // block partial
+}
+
+// Sum of the kItpBlocks block partials in a fixed order (every caller gets
+// the same bits); blockDim.x == 256.
+constexpr int kItpBlocks = 1024;
+
+__device__ __forceinline__ double itp_sum_parts(const double *part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < kItpBlocks; k += 256) s += part[k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  const double t = red[0];
+  __syncthreads();
+  return t;
 }
 
 template <typename R>
-__global__ void k_scale_by_norm(R *psi, long long n, const double *norm, double dV) {
-  const double s = 1.0 / sqrt(norm[0] * dV);
+__global__ void __launch_bounds__(256) k_scale_by_norm(R *psi, long long n, const double *part,
+                                                       double dV) {
+  const double s = 1.0 / sqrt(itp_sum_parts(part) * dV);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     psi[i] = (R)((double)psi[i] * s);
+}
+
+__global__ void __launch_bounds__(256) k_sum_parts(const double *part, double *out) {
+  const double t = itp_sum_parts(part);
+  if (threadIdx.x == 0) out[0] = t;
 }
 
 // mu integrand: conj(psi) * (T psi + Veff psi), T psi supplied in tpsi
@@ -227,7 +252,7 @@ __global__ void k_itp_mu(const double *psi, const double *tpsi, const double *V,
     if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+  if (threadIdx.x == 0) acc[blockIdx.x] = red[0];  // block partial
 }
 
 }  // namespace bdgx
@@ -511,7 +536,8 @@ int bdgx_itp_groundstate(bdgx_ctx *c, const double *omega, double g, double g_dd
   ITP_HIP(hipMalloc((void **)&dV_, sizeof(double) * S));
   ITP_HIP(hipMalloc((void **)&dpsi, sizeof(double) * S * W));
   ITP_HIP(hipMalloc((void **)&dphi, sizeof(double) * S));
-  ITP_HIP(hipMalloc((void **)&dnorm, sizeof(double) * 2));
+  // dnorm: block partials of the norm (kItpBlocks), of mu (kItpBlocks), mu
+  ITP_HIP(hipMalloc((void **)&dnorm, sizeof(double) * (2 * kItpBlocks + 1)));
   ITP_HIP(hipMalloc((void **)&dT, sizeof(double) * S * W));
   ITP_HIP(hipMemcpyAsync(dV_, V.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
   ITP_HIP(hipMemcpyAsync(dpsi, h_psi, sizeof(double) * S * W, hipMemcpyHostToDevice, c->stream));
@@ -535,26 +561,24 @@ int bdgx_itp_groundstate(bdgx_ctx *c, const double *omega, double g, double g_dd
     auto potential = [&](double *nrm) -> int {
       int r = meanfield_t<double>(c, dpsi, dphi, g_dd);
       if (r) return r;
-      if (nrm && hipMemsetAsync(nrm, 0, sizeof(double), c->stream) != hipSuccess) return BDGX_EHIP;
       if (c->cplx)
-        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                            dV_, dphi, g, dt, S, nrm);
       else
-        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(kItpBlocks), dim3(256), 0, c->stream,
                            dpsi, dV_, dphi, g, dt, S, nrm);
       return hipGetLastError() == hipSuccess ? BDGX_OK : BDGX_EHIP;
     };
     {
       // normalise the start vector
       double *nrm = dnorm;
-      ITP_HIP(hipMemsetAsync(nrm, 0, sizeof(double), c->stream));
       if (c->cplx)
-        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                            dV_, (const double *)nullptr, 0.0, 0.0, S, nrm);
       else
-        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(kItpBlocks), dim3(256), 0, c->stream,
                            dpsi, dV_, (const double *)nullptr, 0.0, 0.0, S, nrm);
-      hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(1024), dim3(256), 0, c->stream, dpsi,
+      hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                          S * W, (const double *)nrm, dV);
     }
     for (uint64_t st = 0; st < nsteps && !rc; ++st) {
@@ -562,7 +586,7 @@ int bdgx_itp_groundstate(bdgx_ctx *c, const double *omega, double g, double g_dd
       if (!rc) rc = itp_kinetic_t<double>(c, dpsi, dt);
       if (!rc) rc = potential(dnorm);
       if (!rc)
-        hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(1024), dim3(256), 0, c->stream, dpsi,
+        hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                            S * W, (const double *)dnorm, dV);
     }
     if (!rc) rc = meanfield_t<double>(c, dpsi, dphi, g_dd);
@@ -572,32 +596,30 @@ int bdgx_itp_groundstate(bdgx_ctx *c, const double *omega, double g, double g_dd
     c->f64.U = save64;
     c->f32.U = save32;
   } else {
-    ITP_HIP(hipMemsetAsync(dnorm, 0, sizeof(double), c->stream));
     if (c->cplx)
-      hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+      hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                          dV_, (const double *)nullptr, 0.0, 0.0, S, dnorm);
     else
-      hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+      hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                          dV_, (const double *)nullptr, 0.0, 0.0, S, dnorm);
-    hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(1024), dim3(256), 0, c->stream, dpsi,
+    hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                        S * W, (const double *)dnorm, dV);
     for (uint64_t st = 0; st < nsteps && !rc; ++st) {
       if (c->cplx)
-        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                            dV_, (const double *)nullptr, g, dt, S, (double *)nullptr);
       else
-        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(kItpBlocks), dim3(256), 0, c->stream,
                            dpsi, dV_, (const double *)nullptr, g, dt, S, (double *)nullptr);
       rc = itp_kinetic_t<double>(c, dpsi, dt);
       if (rc) break;
-      ITP_HIP(hipMemsetAsync(dnorm, 0, sizeof(double), c->stream));
       if (c->cplx)
-        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(1024), dim3(256), 0, c->stream, dpsi,
+        hipLaunchKernelGGL((k_itp_pot<double, true>), dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                            dV_, (const double *)nullptr, g, dt, S, dnorm);
       else
-        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(1024), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL((k_itp_pot<double, false>), dim3(kItpBlocks), dim3(256), 0, c->stream,
                            dpsi, dV_, (const double *)nullptr, g, dt, S, dnorm);
-      hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(1024), dim3(256), 0, c->stream, dpsi,
+      hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi,
                          S * W, (const double *)dnorm, dV);
     }
     ITP_HIP(hipMemsetAsync(dphi, 0, sizeof(double) * S, c->stream));
@@ -613,15 +635,17 @@ int bdgx_itp_groundstate(bdgx_ctx *c, const double *omega, double g, double g_dd
     cleanup();
     return rc;
   }
-  ITP_HIP(hipMemsetAsync(dnorm + 1, 0, sizeof(double), c->stream));
   if (c->cplx)
-    hipLaunchKernelGGL(k_itp_mu<true>, dim3(1024), dim3(256), 0, c->stream, dpsi, dT, dV_,
-                       dip ? dphi : nullptr, g, S, dnorm + 1);
+    hipLaunchKernelGGL(k_itp_mu<true>, dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi, dT, dV_,
+                       dip ? dphi : nullptr, g, S, dnorm + kItpBlocks);
   else
-    hipLaunchKernelGGL(k_itp_mu<false>, dim3(1024), dim3(256), 0, c->stream, dpsi, dT, dV_,
-                       dip ? dphi : nullptr, g, S, dnorm + 1);
+    hipLaunchKernelGGL(k_itp_mu<false>, dim3(kItpBlocks), dim3(256), 0, c->stream, dpsi, dT, dV_,
+                       dip ? dphi : nullptr, g, S, dnorm + kItpBlocks);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream,
+                     (const double *)(dnorm + kItpBlocks), dnorm + 2 * kItpBlocks);
   double acc = 0.0;
-  ITP_HIP(hipMemcpyAsync(&acc, dnorm + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ITP_HIP(hipMemcpyAsync(&acc, dnorm + 2 * kItpBlocks, sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
   ITP_HIP(hipMemcpyAsync(h_psi, dpsi, sizeof(double) * S * W, hipMemcpyDeviceToHost, c->stream));
   ITP_HIP(hipStreamSynchronize(c->stream));
 #undef ITP_HIP
