@@ -26,6 +26,7 @@
 static void tridiagonalize(int n, double *A, double *Q, double *d, double *e) {
   double *v = (double *)malloc(sizeof(double) * (size_t)n);
   double *p = (double *)malloc(sizeof(double) * (size_t)n);
+  double *qv = (double *)malloc(sizeof(double) * (size_t)n);
   for (int i = 0; i < n * n; ++i) Q[i] = 0.0;
   for (int i = 0; i < n; ++i) Q[i + i * n] = 1.0;
   for (int j = 0; j + 2 < n; ++j) {
@@ -66,12 +67,17 @@ static void tridiagonalize(int n, double *A, double *Q, double *d, double *e) {
       A[(j + 1 + i) + j * n] = 0.0;
       A[j + (j + 1 + i) * n] = 0.0;
     }
-    /* Q[:, j+1..] -= 2 (Q[:, j+1..] v) v^T */
-    for (int r = 0; r < n; ++r) {
-      double s = 0.0;
-      for (int i = 0; i < m; ++i) s += Q[r + (j + 1 + i) * n] * v[i];
-      s *= 2.0;
-      for (int i = 0; i < m; ++i) Q[r + (j + 1 + i) * n] -= s * v[i];
+    /* Q[:, j+1..] -= 2 (Q[:, j+1..] v) v^T, column by column (contiguous) */
+    for (int r = 0; r < n; ++r) qv[r] = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double vi = v[i];
+      const double *qc = Q + (size_t)(j + 1 + i) * n;
+      for (int r = 0; r < n; ++r) qv[r] += qc[r] * vi;
+    }
+    for (int i = 0; i < m; ++i) {
+      const double vi = 2.0 * v[i];
+      double *qc = Q + (size_t)(j + 1 + i) * n;
+      for (int r = 0; r < n; ++r) qc[r] -= qv[r] * vi;
     }
   }
   for (int i = 0; i < n; ++i) d[i] = A[i + i * n];
@@ -79,6 +85,7 @@ static void tridiagonalize(int n, double *A, double *Q, double *d, double *e) {
   if (n > 0) e[n - 1] = 0.0;
   free(v);
   free(p);
+  free(qv);
 }
 
 /* implicit QR on the tridiagonal (d, e) with vectors accumulated in Z */
