@@ -158,6 +158,7 @@ def lib():
         "bdgx_graph_launch": (I, [P, P]),
         "bdgx_graph_destroy": (None, [P]),
         "bdgx_reserve": (I, [P, U64, U64]),
+        "bdgx_set_chunk_bytes": (I, [P, U64]),
         "bdg_set_learned_preconditioner": (None, [P, P]),
         "bdg_set_learned_fallback": (None, [P, I, D]),
         "bdg_start_block": (None, [P, P]),
@@ -323,7 +324,7 @@ class DeviceContext:
 
     def launch_variants(self, prec: int) -> dict:
         out = {}
-        for kind, name in enumerate(("plane_fwd", "plane_inv", "z")):
+        for kind, name in enumerate(("plane_fwd", "plane_inv", "z", "combine")):
             v = C.c_int()
             check(lib().bdgx_get_launch_variant(self.h, kind, int(prec), C.byref(v)),
                   "bdgx_get_launch_variant")

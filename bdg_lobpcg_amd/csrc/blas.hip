@@ -391,37 +391,56 @@ __global__ void __launch_bounds__(256) k_combine(const R *A, long long lda, int 
 // of its column tile (A is read once per tile); C is staged through LDS 64
 // rows at a time; the next k step's A values are loaded before the MFMAs of
 // the current one.
-template <typename R>
-__device__ __forceinline__ void load_rows4(const R *A, long long lda, int col, int k,
-                                           long long row, long long nrows, int vec,
-                                           double out[4]) {
+// RT consecutive rows (r .. r+RT-1) of column `col`, zero past k / nrows
+template <typename R, int RT>
+__device__ __forceinline__ void load_rows(const R *A, long long lda, int col, int k,
+                                          long long row, long long nrows, int vec,
+                                          double out[RT]) {
   if (col >= k) {
-    out[0] = out[1] = out[2] = out[3] = 0.0;
+#pragma unroll
+    for (int t = 0; t < RT; ++t) out[t] = 0.0;
     return;
   }
   const R *p = A + row + (long long)col * lda;
-  if (vec && row + 4 <= nrows) {
-    if constexpr (sizeof(R) == 8) {
+  if (vec && row + RT <= nrows) {
+    if constexpr (sizeof(R) == 8 && RT == 4) {
       const double2 x = reinterpret_cast<const double2 *>(p)[0];
       const double2 y = reinterpret_cast<const double2 *>(p)[1];
       out[0] = x.x;
       out[1] = x.y;
       out[2] = y.x;
       out[3] = y.y;
-    } else {
+    } else if constexpr (sizeof(R) == 8 && RT == 2) {
+      const double2 x = reinterpret_cast<const double2 *>(p)[0];
+      out[0] = x.x;
+      out[1] = x.y;
+    } else if constexpr (RT == 4) {
       const float4 x = *reinterpret_cast<const float4 *>(p);
       out[0] = x.x;
       out[1] = x.y;
       out[2] = x.z;
       out[3] = x.w;
+    } else {
+      const float2 x = *reinterpret_cast<const float2 *>(p);
+      out[0] = x.x;
+      out[1] = x.y;
     }
     return;
   }
 #pragma unroll
-  for (int t = 0; t < 4; ++t) out[t] = (row + t < nrows) ? (double)p[t] : 0.0;
+  for (int t = 0; t < RT; ++t) out[t] = (row + t < nrows) ? (double)p[t] : 0.0;
 }
 
-template <typename R, int MF>
+// Y = A C, A nrows x k (ld lda), C k x m (FP64, ld k), Y nrows x m.  Each
+// wave owns 16 RT rows and the 16 MF output columns of blockIdx.y: lane
+// (c, kk) loads rows r0 + RT c + t (t < RT) of column kc + kk, which feed RT
+// MFMAs (v_mfma_f64_16x16x4f64) as MFMA row c -- the contraction runs over
+// columns, so this row permutation is free and every load is one vector of
+// RT consecutive rows.  The accumulator element v of lane (c, kk) then holds
+// rows r0 + RT (kk + 4 v) + t of output column c: RT consecutive rows, one
+// vector store.  C is staged in LDS KC rows at a time; A runs PD k steps
+// ahead of the MFMAs.
+template <typename R, int MF, int RT>
 __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda, int k,
                                                       const double *C, R *Y, long long ldy,
                                                       int m, long long nrows, int vec_a,
@@ -431,11 +450,11 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = lane & 15, kk = lane >> 4;
   const int j0 = blockIdx.y * 16 * MF;
-  const long long r0 = ((long long)blockIdx.x * 4 + wave) * 64;
-  const long long row = r0 + 4 * c;
-  v4d acc[4][MF];
+  const long long r0 = ((long long)blockIdx.x * 4 + wave) * (16 * RT);
+  const long long row = r0 + RT * c;
+  v4d acc[RT][MF];
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+  for (int t = 0; t < RT; ++t)
 #pragma unroll
     for (int f = 0; f < MF; ++f) acc[t][f] = v4d{0.0, 0.0, 0.0, 0.0};
   for (int kc = 0; kc < k; kc += KC) {
@@ -446,12 +465,11 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
       Cs[i][j] = (i < nk && j0 + j < m) ? C[(kc + i) + (long long)(j0 + j) * k] : 0.0;
     }
     __syncthreads();
-    // ring of PD k steps of A in flight ahead of the MFMAs
     constexpr int PD = 4;
-    double av[PD][4];
+    double av[PD][RT];
 #pragma unroll
     for (int q = 0; q < PD; ++q)
-      if (4 * q < nk) load_rows4(A, lda, kc + 4 * q + kk, k, row, nrows, vec_a, av[q]);
+      if (4 * q < nk) load_rows<R, RT>(A, lda, kc + 4 * q + kk, k, row, nrows, vec_a, av[q]);
     for (int k4 = 0; k4 < nk; k4 += 4 * PD) {
 #pragma unroll
       for (int q = 0; q < PD; ++q) {
@@ -461,14 +479,14 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
 #pragma unroll
         for (int f = 0; f < MF; ++f) bv[f] = Cs[kq + kk][16 * f + c];
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < RT; ++t)
 #pragma unroll
           for (int f = 0; f < MF; ++f)
             acc[t][f] =
                 __builtin_amdgcn_mfma_f64_16x16x4f64(av[q][t], bv[f], acc[t][f], 0, 0, 0);
         // refill this slot with the step PD ahead
         if (kq + 4 * PD < nk)
-          load_rows4(A, lda, kc + kq + 4 * PD + kk, k, row, nrows, vec_a, av[q]);
+          load_rows<R, RT>(A, lda, kc + kq + 4 * PD + kk, k, row, nrows, vec_a, av[q]);
       }
     }
   }
@@ -479,51 +497,65 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
     R *yc = Y + (long long)col * ldy;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const long long rr = r0 + 4 * (kk + 4 * v);
-      if (vec_y && rr + 4 <= nrows) {
-        if constexpr (sizeof(R) == 8) {
+      const long long rr = r0 + RT * (kk + 4 * v);
+      if (vec_y && rr + RT <= nrows) {
+        if constexpr (sizeof(R) == 8 && RT == 4) {
           reinterpret_cast<double2 *>(yc + rr)[0] = double2{acc[0][f][v], acc[1][f][v]};
           reinterpret_cast<double2 *>(yc + rr)[1] = double2{acc[2][f][v], acc[3][f][v]};
-        } else {
+        } else if constexpr (sizeof(R) == 8 && RT == 2) {
+          reinterpret_cast<double2 *>(yc + rr)[0] = double2{acc[0][f][v], acc[1][f][v]};
+        } else if constexpr (RT == 4) {
           *reinterpret_cast<float4 *>(yc + rr) = float4{(float)acc[0][f][v], (float)acc[1][f][v],
                                                         (float)acc[2][f][v], (float)acc[3][f][v]};
+        } else {
+          *reinterpret_cast<float2 *>(yc + rr) = float2{(float)acc[0][f][v], (float)acc[1][f][v]};
         }
       } else {
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < RT; ++t)
           if (rr + t < nrows) yc[rr + t] = (R)acc[t][f][v];
       }
     }
   }
 }
 
+template <typename R, int MF>
+static void launch_combine(int rt, dim3 grid, hipStream_t s, const R *A, long long lda, int k,
+                           const double *C, R *Y, long long ldy, int m, long long nrows,
+                           int vec_a, int vec_y) {
+  if (rt == 2)
+    hipLaunchKernelGGL((k_combine_mfma<R, MF, 2>), grid, dim3(256), 0, s, A, lda, k, C, Y, ldy, m,
+                       nrows, vec_a, vec_y);
+  else
+    hipLaunchKernelGGL((k_combine_mfma<R, MF, 4>), grid, dim3(256), 0, s, A, lda, k, C, Y, ldy, m,
+                       nrows, vec_a, vec_y);
+}
+
+// Launch variants (autotuned): 0 = 4 rows per lane (64 per wave), 1 = 2 rows
+// per lane (half the accumulators: more waves in flight per SIMD).
 template <typename R>
 static int combine_real_mfma(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k,
                              const double *dC, void *dY, uint64_t ldy, uint64_t m,
                              uint64_t nrows) {
   const int MF = m <= 16 ? 1 : (m <= 32 ? 2 : (m <= 48 ? 3 : 4));
-  const unsigned gx = (unsigned)((nrows + 255) / 256);
+  const int rt = c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_COMBINE] ? 2 : 4;
+  const unsigned gx = (unsigned)((nrows + 64 * rt - 1) / (64 * rt));
   const unsigned gy = (unsigned)((m + 16 * MF - 1) / (16 * MF));
   const int vec_a = (lda % 4 == 0) && ((uintptr_t)dA % (4 * sizeof(R)) == 0);
   const int vec_y = (ldy % 4 == 0) && ((uintptr_t)dY % (4 * sizeof(R)) == 0);
   const R *A = reinterpret_cast<const R *>(dA);
   R *Y = reinterpret_cast<R *>(dY);
-  if (MF == 1)
-    hipLaunchKernelGGL((k_combine_mfma<R, 1>), dim3(gx, gy), dim3(256), 0, c->stream, A,
-                       (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
-                       vec_a, vec_y);
-  else if (MF == 2)
-    hipLaunchKernelGGL((k_combine_mfma<R, 2>), dim3(gx, gy), dim3(256), 0, c->stream, A,
-                       (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
-                       vec_a, vec_y);
-  else if (MF == 3)
-    hipLaunchKernelGGL((k_combine_mfma<R, 3>), dim3(gx, gy), dim3(256), 0, c->stream, A,
-                       (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
-                       vec_a, vec_y);
-  else
-    hipLaunchKernelGGL((k_combine_mfma<R, 4>), dim3(gx, gy), dim3(256), 0, c->stream, A,
-                       (long long)lda, (int)k, dC, Y, (long long)ldy, (int)m, (long long)nrows,
-                       vec_a, vec_y);
+  const dim3 grid(gx, gy);
+  switch (MF) {
+    case 1: launch_combine<R, 1>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,
+                                 (long long)ldy, (int)m, (long long)nrows, vec_a, vec_y); break;
+    case 2: launch_combine<R, 2>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,
+                                 (long long)ldy, (int)m, (long long)nrows, vec_a, vec_y); break;
+    case 3: launch_combine<R, 3>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,
+                                 (long long)ldy, (int)m, (long long)nrows, vec_a, vec_y); break;
+    default: launch_combine<R, 4>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,
+                                  (long long)ldy, (int)m, (long long)nrows, vec_a, vec_y); break;
+  }
   BDGX_HIP(hipGetLastError());
   return BDGX_OK;
 }

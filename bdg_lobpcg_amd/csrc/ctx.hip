@@ -733,6 +733,12 @@ void bdgx_graph_destroy(bdgx_graph *g) {
   delete g;
 }
 
+int bdgx_set_chunk_bytes(bdgx_ctx *c, uint64_t bytes) {
+  if (!c) return BDGX_EINVAL;
+  c->chunk_bytes = (size_t)bytes;
+  return BDGX_OK;
+}
+
 int bdgx_reserve(bdgx_ctx *c, uint64_t spec_bytes, uint64_t scratch_bytes) {
   if (!c) return BDGX_EINVAL;
   int rc = ensure_spec(c, (size_t)spec_bytes);
@@ -747,14 +753,14 @@ int bdgx_autotune(bdgx_ctx *c, int prec, uint64_t nbatch, int force) {
 }
 
 int bdgx_get_launch_variant(const bdgx_ctx *c, int kind, int prec, int *variant) {
-  if (!c || !variant || kind < 0 || kind > 2 || (prec != BDGX_FP64 && prec != BDGX_FP32))
+  if (!c || !variant || kind < 0 || kind > 3 || (prec != BDGX_FP64 && prec != BDGX_FP32))
     return BDGX_EINVAL;
   *variant = c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind];
   return BDGX_OK;
 }
 
 int bdgx_set_launch_variant(bdgx_ctx *c, int kind, int prec, int variant) {
-  if (!c || kind < 0 || kind > 2 || (prec != BDGX_FP64 && prec != BDGX_FP32) || variant < 0 ||
+  if (!c || kind < 0 || kind > 3 || (prec != BDGX_FP64 && prec != BDGX_FP32) || variant < 0 ||
       variant > 1)
     return BDGX_EINVAL;
   c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind] = variant;

@@ -593,7 +593,7 @@ static int run_jobs(bdgx_ctx *c, const HalfJob *jobs, int njobs, const R *X,
     }
   // chunk columns so the spectrum workspace stays bounded
   const size_t per_col = (size_t)C * (size_t)g.Kk * sizeof(cx<R>);
-  const size_t budget = (size_t)6 << 30;
+  const size_t budget = c->chunk_bytes ? c->chunk_bytes : (size_t)6 << 30;
   long long chunk = (long long)(budget / (per_col ? per_col : 1));
   if (chunk < 1) chunk = 1;
   if (chunk > ncols_total) chunk = ncols_total;
@@ -875,14 +875,19 @@ static int bench_pass_t(bdgx_ctx *c, int pass, uint64_t nbatch, int iters, float
   return rc;
 }
 
-// Launch-shape autotuner: every variant of every fixed-size kernel kind the
-// grid uses, timed on nbatch kinetic batches (1 warm-up + 3 timed launches,
-// HIP events on the ctx stream); the fastest variant is kept per kind.
+int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
+                     const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows);
+
+// Launch-shape autotuner: every variant of every kernel kind with variants
+// that this grid uses (plane and fixed z passes on nbatch kinetic batches,
+// the real block combine nbatch -> nbatch), 1 warm-up + 3 timed launches,
+// HIP events on the ctx stream; the fastest variant is kept per kind.
 template <typename R>
 static int autotune_t(bdgx_ctx *c, uint64_t nbatch) {
   const int pi = sizeof(R) == 8 ? 0 : 1;
   const bool plane = plane_supported<R>(c), zfix = zpass_fixed_supported<R>(c);
-  if ((!plane && !zfix) || nbatch == 0) {
+  const bool comb = !c->cplx;
+  if ((!plane && !zfix && !comb) || nbatch == 0) {
     c->tuned[pi] = 1;
     return BDGX_OK;
   }
@@ -892,26 +897,41 @@ static int autotune_t(bdgx_ctx *c, uint64_t nbatch) {
   int rc = ensure_spec(c, sizeof(cx<R>) * (size_t)g.Kk * nbatch);
   if (rc) return rc;
   cx<R> *spec = reinterpret_cast<cx<R> *>(c->spec);
+  // X holds 3 nbatch columns and Y 2 nbatch: the combine is timed at the
+  // solver's main shape [X | P | W] -> [X | P] (3 nbatch -> 2 nbatch)
   R *X = nullptr, *Y = nullptr;
-  BDGX_HIP(hipMalloc((void **)&X, es * (size_t)g.S * nbatch));
-  if (hipMalloc((void **)&Y, es * (size_t)g.S * nbatch) != hipSuccess) {
+  BDGX_HIP(hipMalloc((void **)&X, es * (size_t)g.S * nbatch * 3));
+  if (hipMalloc((void **)&Y, es * (size_t)g.S * nbatch * 2) != hipSuccess) {
     (void)hipFree(X);
     return BDGX_EHIP;
   }
-  (void)hipMemsetAsync(X, 0, es * (size_t)g.S * nbatch, c->stream);
+  double *dC = nullptr;
+  if (hipMalloc((void **)&dC, sizeof(double) * nbatch * nbatch * 6) != hipSuccess) {
+    (void)hipFree(X);
+    (void)hipFree(Y);
+    return BDGX_EHIP;
+  }
+  (void)hipMemsetAsync(dC, 0, sizeof(double) * nbatch * nbatch * 6, c->stream);
+  (void)hipMemsetAsync(X, 0, es * (size_t)g.S * nbatch * 3, c->stream);
   (void)hipMemsetAsync(spec, 0, sizeof(cx<R>) * (size_t)g.Kk * nbatch, c->stream);
   HalfJob J = job_kin(0, F_NONE);
   int mul[kMaxChan] = {MUL_KIN, MUL_KIN, MUL_KIN};
   XArgs<R> a = make_xargs<R>(c, J, 0, nc, X, Y, (long long)g.S, spec, (long long)nc * g.Kk,
                              nullptr);
   const YZArgs<R> yz = make_yz<R>(c, mul, 1, nc, spec);
-  const int kinds[3] = {BDGX_TUNE_PLANE_FWD, BDGX_TUNE_Z, BDGX_TUNE_PLANE_INV};
-  for (int q = 0; q < 3 && !rc; ++q) {
+  const int kinds[4] = {BDGX_TUNE_PLANE_FWD, BDGX_TUNE_Z, BDGX_TUNE_PLANE_INV,
+                        BDGX_TUNE_COMBINE};
+  const int prec = sizeof(R) == 8 ? BDGX_FP64 : BDGX_FP32;
+  for (int q = 0; q < 4 && !rc; ++q) {
     const int kind = kinds[q];
-    if (kind == BDGX_TUNE_Z ? !zfix : !plane) continue;
+    const bool on = kind == BDGX_TUNE_Z ? zfix : (kind == BDGX_TUNE_COMBINE ? comb : plane);
+    if (!on) continue;
     auto one = [&]() -> int {
       // launch_z fills in the z twiddles and plan before the fixed z pass
       if (kind == BDGX_TUNE_Z) return launch_z<R>(c, yz);
+      if (kind == BDGX_TUNE_COMBINE)
+        return combine_dispatch(c, prec, X, (uint64_t)g.S, 3 * nbatch, dC, Y, (uint64_t)g.S,
+                                2 * nbatch, (uint64_t)g.S);
       return launch_plane_x<R>(c, a, kind == BDGX_TUNE_PLANE_INV);
     };
     int best = c->tune_var[pi][kind];
@@ -935,6 +955,7 @@ static int autotune_t(bdgx_ctx *c, uint64_t nbatch) {
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(X);
   (void)hipFree(Y);
+  (void)hipFree(dC);
   if (!rc) c->tuned[pi] = 1;
   return rc;
 }

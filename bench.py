@@ -146,6 +146,7 @@ def blas(cfg, iters=10):
     import bdg_lobpcg_amd as B
 
     d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
+    variants = d.autotune(B.FP64, cfg["m"])
     S = int(np.prod(cfg["N"]))
     rng = np.random.default_rng(1)
     kmax = 96
@@ -193,7 +194,9 @@ def blas(cfg, iters=10):
         ms = d.timer_end() / iters
         by = S * (k + m) * 8
         fl = 2.0 * S * k * m
-        out.append(dict(kernel="combine", shape=f"{k}->{m}", us=ms * 1e3, GBs=by / ms / 1e6,
+        out.append(dict(kernel="combine", shape=f"{k}->{m}",
+                        rows_per_lane=2 if variants["combine"] else 4, us=ms * 1e3,
+                        GBs=by / ms / 1e6,
                         TFs=fl / ms / 1e9, mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS,
                         rel_err=err))
     d.close()

@@ -156,15 +156,18 @@ int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iter
 
 /* Launch-shape autotuner (no reference counterpart; it takes the place of
  * the reference's FFTW_PATIENT planning, src/matmul_ctx.c:134-171).  The
- * fixed-size FFT kernels (whole-plane forward / inverse, fixed z pass) have
- * launch variants: plane 0 = 4 Nx threads, 1 = 8 Nx threads per plane; z
- * 0 = 256 (Nz = 128) or 128 threads, 1 = half of that.  bdgx_autotune times
- * every variant of every kind for `prec` on `nbatch` kinetic batches (1
- * warm-up + 3 timed launches each, HIP events on the ctx stream) and keeps
- * the fastest; it returns at once when `prec` is already tuned and `force`
- * is 0, and does nothing (returns 0) on grids that use the pencil passes.
- * The variants compute identical results up to rounding order. */
-enum { BDGX_TUNE_PLANE_FWD = 0, BDGX_TUNE_PLANE_INV = 1, BDGX_TUNE_Z = 2 };
+ * fixed-size FFT kernels (whole-plane forward / inverse, fixed z pass) and
+ * the block combine have launch variants: plane 0 = 4 Nx threads, 1 = 8 Nx
+ * threads per plane; z 0 = 256 (Nz = 128) or 128 threads, 1 = half of that;
+ * combine 0 = 4 rows per lane, 1 = 2 rows per lane.  bdgx_autotune times
+ * every variant of every kind for `prec` on `nbatch` columns (kinetic
+ * batches for the FFT kinds, an nbatch x nbatch combine; 1 warm-up + 3 timed
+ * launches each, HIP events on the ctx stream) and keeps the fastest; it
+ * returns at once when `prec` is already tuned and `force` is 0.  The FFT
+ * kinds are skipped on grids that use the pencil passes.  The variants
+ * compute identical results (same operations in the same order). */
+enum { BDGX_TUNE_PLANE_FWD = 0, BDGX_TUNE_PLANE_INV = 1, BDGX_TUNE_Z = 2,
+       BDGX_TUNE_COMBINE = 3 };
 int bdgx_autotune(bdgx_ctx *ctx, int prec, uint64_t nbatch, int force);
 int bdgx_get_launch_variant(const bdgx_ctx *ctx, int kind, int prec, int *variant);
 int bdgx_set_launch_variant(bdgx_ctx *ctx, int kind, int prec, int variant);
@@ -186,6 +189,11 @@ void bdgx_graph_destroy(bdgx_graph *g);
 /* Grow the spectrum workspace and the small scratch to at least the given
  * sizes now (so that later calls of up to that size do not reallocate). */
 int bdgx_reserve(bdgx_ctx *ctx, uint64_t spec_bytes, uint64_t scratch_bytes);
+/* Spectrum bytes per column chunk of an operator apply (0 = 6 GiB, the
+ * default).  A chunk's three passes run back to back, so a chunk small
+ * enough for the 256 MB Infinity Cache keeps its spectrum there between the
+ * forward, z and inverse passes. */
+int bdgx_set_chunk_bytes(bdgx_ctx *ctx, uint64_t bytes);
 
 /* ---- learned preconditioner (no reference counterpart; its analytic base
  *      is precondK/precondM, reference src/precond_impl.inc:21-111) --------

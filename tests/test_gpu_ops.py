@@ -116,6 +116,13 @@ def test_launch_variants_and_autotune(grid):
                 for op, ref in refs.items():
                     e = relerr(d.apply(op, X, prec), ref)
                     assert e < TOL[prec], f"kind {kind} variant {v} prec {prec} {op}: {e:.2e}"
+        rng = np.random.default_rng(3)
+        A = rng.standard_normal((c.size, 20))
+        Cm = rng.standard_normal((20, 36))
+        for v in (0, 1):
+            d.set_launch_variant(3, prec, v)  # block combine
+            e = relerr(d.combine(A, Cm, prec), A @ Cm)
+            assert e < (1e-14 if prec == 0 else 1e-6), f"combine variant {v} prec {prec}: {e:.2e}"
         chosen = d.autotune(prec, 8, force=True)
         assert set(chosen.values()) <= {0, 1}
         for op, ref in refs.items():
