@@ -153,6 +153,7 @@ def lib():
         "bdg_set_mixed_params": (None, [P, D, U64]),
         "bdg_set_verbosity": (None, [P, I]),
         "bdg_set_graph_capture": (None, [P, I]),
+        "bdg_solve_batch": (I, [C.POINTER(P), I, I, C.POINTER(I)]),
         "bdgx_graph_begin": (I, [P]),
         "bdgx_graph_end": (I, [P, C.POINTER(P)]),
         "bdgx_graph_launch": (I, [P, P]),
@@ -548,6 +549,16 @@ class Problem:
 
     def solve(self) -> int:
         return int(lib().bdg_solve(self.h))
+
+    @staticmethod
+    def solve_batch(problems, nthreads: int = 0):
+        """bdg_solve_batch: solve independent problems concurrently (one
+        worker thread and HIP stream each); returns the per-problem codes."""
+        n = len(problems)
+        hs = (C.c_void_p * n)(*[p.h for p in problems])
+        rc = (C.c_int * n)()
+        lib().bdg_solve_batch(hs, n, int(nthreads), rc)
+        return [int(x) for x in rc]
 
     # results
     def converged(self) -> int:

@@ -6,6 +6,7 @@
  * through the bdgx C ABI: the dipolar kernel and mean field at setup, and
  * the whole eigen-iteration in bdg_solve.
  */
+#include <pthread.h>
 #include <complex.h>
 #include <inttypes.h>
 
@@ -619,6 +620,68 @@ const void *bdg_modes_v(const bdg_t *b) { return b->modes_v; }
 /* ---- extensions (include/bdg/bdg_amd.h) ------------------------------------ */
 
 void bdg_set_precision(bdg_t *b, bdg_precision_t p) { b->precision = (int)p; }
+/* ---- batches of independent problems ---------------------------------- */
+
+typedef struct {
+  bdg_t **probs;
+  int n;
+  int *rc;
+  int next; /* next problem to take, under lock */
+  pthread_mutex_t lock;
+} batch_state;
+
+static void *batch_worker(void *arg) {
+  batch_state *bs = (batch_state *)arg;
+  for (;;) {
+    pthread_mutex_lock(&bs->lock);
+    const int i = bs->next < bs->n ? bs->next++ : -1;
+    pthread_mutex_unlock(&bs->lock);
+    if (i < 0) break;
+    bs->rc[i] = bdg_solve(bs->probs[i]);
+  }
+  return NULL;
+}
+
+int bdg_solve_batch(bdg_t **probs, int n, int nthreads, int *rc) {
+  if (!probs || n <= 0) return BDGX_EINVAL;
+  int *codes = rc ? rc : (int *)xcalloc((size_t)n, sizeof(int));
+  /* devices first, one at a time: context creation then happens before the
+   * workers start */
+  for (int i = 0; i < n; ++i) {
+    codes[i] = probs[i] ? ensure_device(probs[i]) : BDGX_EINVAL;
+  }
+  int nt = nthreads > 0 && nthreads < n ? nthreads : n;
+  batch_state bs;
+  bs.probs = probs;
+  bs.n = n;
+  bs.rc = codes;
+  bs.next = 0;
+  pthread_mutex_init(&bs.lock, NULL);
+  int first_err = 0;
+  for (int i = 0; i < n; ++i)
+    if (codes[i]) {
+      first_err = codes[i];
+      break;
+    }
+  if (!first_err) {
+    pthread_t *th = (pthread_t *)xcalloc((size_t)nt, sizeof(pthread_t));
+    int started = 0;
+    for (int t = 0; t < nt; ++t)
+      if (pthread_create(&th[t], NULL, batch_worker, &bs) == 0) ++started;
+    if (started == 0) batch_worker(&bs); /* no threads: run them here */
+    for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+    free(th);
+    for (int i = 0; i < n; ++i)
+      if (codes[i]) {
+        first_err = codes[i];
+        break;
+      }
+  }
+  pthread_mutex_destroy(&bs.lock);
+  if (!rc) free(codes);
+  return first_err;
+}
+
 void bdg_get_solve_stats(const bdg_t *b, bdg_solve_stats_t *out) {
   if (out) *out = b->stats;
 }

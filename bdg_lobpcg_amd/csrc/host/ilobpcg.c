@@ -37,7 +37,8 @@
  * residual is above img_tol, the images M W, K M W are FP32 too, of W
  * projected out of span(X); their error enters the carried images of X in
  * proportion to the W coefficients, and the images of X are re-applied in
- * FP64 every refresh_every iterations.  Below img_tol the images are FP64,
+ * FP64 every refresh_every iterations.  Once the worst residual reaches
+ * img_tol the images are FP64 for the rest of the solve,
  * and convergence is only accepted on FP64 images.  FP32 images are off by
  * default: rounding noise in an FP32 image is amplified by the stiffness of
  * K M (about 1e-3 relative at 128^3), which stops interior modes from
@@ -522,6 +523,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   int it = 0, np = 0, kprev = 0, ncp = 0, gvalid = 0, since_full = 0, fresh = 1;
   double trr = 0.0;
   int refresh_it = 0;
+  int img_done = 0; /* the FP32-image phase ends for good at img_tol */
   const double t_start = bdg_wtime();
   res->status = 0;
   res->iters = 0;
@@ -595,6 +597,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     if (mixed && !fresh &&
         (st.allc || st.worst <= cfg->img_tol ||
          (cfg->refresh_every > 0 && it - refresh_it >= cfg->refresh_every))) {
+      if (st.allc || st.worst <= cfg->img_tol) img_done = 1;
       TRY(refresh_images(ctx, cfg, &A, res));
       fresh = 1;
       gvalid = 0;
@@ -648,8 +651,9 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
           for (int i = 0; i < q; ++i) setz(gn[g], k, i, j, w, getz(hT, q, i, j, w));
       }
     }
-    const int img32 =
-        mixed && cfg->img_tol > 0 && nact >= cfg->mixed_min_cols && st.worst > cfg->img_tol;
+    if (mixed && st.worst <= cfg->img_tol) img_done = 1;
+    const int img32 = mixed && !img_done && cfg->img_tol > 0 && nact >= cfg->mixed_min_cols &&
+                      st.worst > cfg->img_tol;
     void *wdst = colp(A.V, (uint64_t)wc, S, es);
     chain_args ca;
     ca.rsrc = Rsrc;

@@ -83,6 +83,36 @@ def make_problem(cfg, psi, mu, precision=0, maxiter=1000, tol=None, nev=None, m=
     return p
 
 
+def batch_configs(n=16, seed=20260601):
+    """The many-problems workload: n independent problems with grids drawn from
+    {48^3, 64^3, 80^3, 96^3, 64^2 x 128}, eps_dd ~ U[0.3, 0.95], g ~ U[500, 3000]
+    and trap aspect w_z / w_perp ~ U[0.25, 4] (w_perp = 1), by xorshift32 from
+    `seed`; nev = 16.  Box: 16 across, along z 16 / sqrt(aspect) (elongated
+    traps get longer boxes), times Nz / Nx for the 64^2 x 128 grid."""
+    grids = [(48, 48, 48), (64, 64, 64), (80, 80, 80), (96, 96, 96), (64, 64, 128)]
+    st = [seed & 0xFFFFFFFF or 1]
+
+    def xr():
+        x = st[0]
+        x ^= (x << 13) & 0xFFFFFFFF
+        x ^= x >> 17
+        x ^= (x << 5) & 0xFFFFFFFF
+        st[0] = x
+        return (x & 0x7FFFFFFF) / 0x7FFFFFFF
+
+    out = []
+    for i in range(n):
+        N = grids[min(int(xr() * len(grids)), len(grids) - 1)]
+        edd = 0.3 + 0.65 * xr()
+        g = 500.0 + 2500.0 * xr()
+        asp = 0.25 * 16.0 ** xr()  # log-uniform on [0.25, 4]
+        lz = 16.0 / np.sqrt(asp) * N[2] / N[0]
+        out.append(dict(N=N, L=(16.0, 16.0, lz), omega=(1.0, 1.0, asp), g=g, edd=edd,
+                        R=min(16.0, lz) / 2, itp_steps=2000, dt=1e-3, nev=16, m=16, tol=1e-8,
+                        cplx=False, name=f"batch{i}"))
+    return out
+
+
 def train_learned(cfg, steps=300, lr=1e-3, reg=0.3, snapshots=(4, 8, 14, 22, 32, 48, 70),
                   log=None):
     """Learned-preconditioner parameters for cfg's grid, trained on the GPU on

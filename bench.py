@@ -204,6 +204,56 @@ def blas(cfg, iters=10):
         print(json.dumps(r), flush=True)
 
 
+def batch(nprob=16, precision=1, nthreads=0):
+    """The many-problems workload (workloads.batch_configs): psi0 per problem by
+    device ITP, then all solves one after another and again concurrently with
+    bdg_solve_batch (one worker thread and HIP stream per problem).  Prints one
+    JSON line with both wall times and checks that the concurrent eigenvalues
+    equal the sequential ones bit for bit."""
+    import bdg_lobpcg_amd as B
+    from bdg_lobpcg_amd.workloads import batch_configs, ground_state, make_problem
+
+    cfgs = batch_configs(nprob)
+    states = []
+    t0 = time.perf_counter()
+    for c in cfgs:
+        psi, mu, _ = ground_state(c)
+        states.append((psi, mu))
+    t_itp = time.perf_counter() - t0
+    log(f"batch: {nprob} ground states in {t_itp:.1f} s")
+
+    def build():
+        return [make_problem(c, psi, mu, precision=precision) for c, (psi, mu) in zip(cfgs, states)]
+
+    probs = build()
+    t0 = time.perf_counter()
+    rc_seq = [p.solve() for p in probs]
+    t_seq = time.perf_counter() - t0
+    w_seq = [p.eigenvalues(c["nev"]) for p, c in zip(probs, cfgs)]
+    its = [p.stats()["iterations"] for p in probs]
+    conv = [p.converged() for p in probs]
+    for p in probs:
+        p.close()
+    probs = build()
+    t0 = time.perf_counter()
+    rc_bat = B.Problem.solve_batch(probs, nthreads)
+    t_bat = time.perf_counter() - t0
+    same = all(np.array_equal(p.eigenvalues(c["nev"]), w) for p, c, w in zip(probs, cfgs, w_seq))
+    for p in probs:
+        p.close()
+    log(f"batch: sequential {t_seq:.2f} s, concurrent {t_bat:.2f} s ({t_seq / t_bat:.2f}x), "
+        f"identical eigenvalues {same}")
+    print(json.dumps({
+        "workload": f"{nprob} problems, grids {{48^3, 64^3, 80^3, 96^3, 64^2x128}}, nev 16, "
+                    f"{'mixed' if precision else 'fp64'}",
+        "problems": [dict(grid=list(c["N"]), g=round(c["g"], 1), eps_dd=round(c["edd"], 3),
+                          aspect=round(c["omega"][2], 3), iterations=i, converged=cv, rc=r)
+                     for c, i, cv, r in zip(cfgs, its, conv, rc_seq)],
+        "psi0_seconds": t_itp, "sequential_seconds": t_seq, "concurrent_seconds": t_bat,
+        "speedup": t_seq / t_bat, "concurrent_rc": rc_bat, "identical_eigenvalues": same,
+    }), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a
@@ -252,6 +302,8 @@ def main():
     ap.add_argument("--config", default="cfg1_128c_d")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="solve the N-problem batch workload sequentially and concurrently")
     ap.add_argument("--blas", action="store_true",
                     help="time the Gram and combine kernels at the solver's shapes and exit")
     args = ap.parse_args()
@@ -268,6 +320,9 @@ def main():
         return
     if args.blas:
         blas(cfg)
+        return
+    if args.batch:
+        batch(args.batch)
         return
     log(f"config {args.config}: N={cfg['N']} nev={cfg['nev']} ITP {cfg['itp_steps']} steps")
     psi, mu, t_itp = ground_state(cfg)

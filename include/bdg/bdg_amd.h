@@ -78,6 +78,15 @@ void bdg_set_learned_fallback(bdg_t *bdg, int window, f64 factor);
  * Results are the same either way. */
 void bdg_set_graph_capture(bdg_t *bdg, int on);
 
+/* Solve n independent, fully set-up problems concurrently: worker threads
+ * (nthreads, 0 = one per problem) take problems in order and call bdg_solve
+ * on each; every handle has its own HIP stream, so the device overlaps the
+ * problems' kernels and the hosts' Rayleigh-Ritz steps run in parallel.
+ * Per-problem return codes go to rc[n] (may be NULL); returns 0 when all
+ * solves returned 0, else the first nonzero code.  Handles must be
+ * distinct; results are read per handle as after bdg_solve. */
+int bdg_solve_batch(bdg_t **probs, int n, int nthreads, int *rc);
+
 void bdg_set_verbosity(bdg_t *bdg, int level);
 
 /* Built-in host callbacks usable with bdg_set_trap / bdg_set_local_interactions

@@ -89,3 +89,38 @@ def test_graph_capture_gives_identical_solve():
     assert s1["graph_launches"] > 0 and s1["graph_captures"] > 0
     assert s0["iterations"] == s1["iterations"]
     np.testing.assert_array_equal(w0, w1)
+
+
+def test_solve_batch_matches_sequential():
+    """bdg_solve_batch (concurrent worker threads, one stream per problem)
+    gives the sequential solves' eigenvalues bit for bit."""
+    import bdg_lobpcg_amd as B
+    setups = [((16, 16, 16), (12.0, 12.0, 12.0), 20.0, 10.0),
+              ((12, 12, 16), (10.0, 10.0, 14.0), 30.0, 5.0),
+              ((16, 16, 12), (12.0, 12.0, 10.0), 10.0, 8.0)]
+
+    def build():
+        out = []
+        for N, L, g, gdd in setups:
+            c, f = physics(N, L, False, dipolar=True, g=g, gdd=gdd, mu=1.0)
+            p = B.Problem(N, L, False)
+            p.set_system()
+            p.set_trap_harmonic((1.0, 1.0, 1.0))
+            p.set_wavefunction(f["wf"])
+            p.set_contact(g)
+            p.set_dipolar(gdd, (0.0, 0.0, 1.0), f["R"])
+            p.set_mu(1.0)
+            p.set_solver_params(4, 6, 300, 1e-9)
+            p.set_init_uniform(42)
+            out.append(p)
+        return out
+
+    seq = build()
+    assert [p.solve() for p in seq] == [0, 0, 0]
+    w_seq = [p.eigenvalues(4) for p in seq]
+    st_seq = [(p.stats()["iterations"], p.converged()) for p in seq]
+    bat = build()
+    assert B.Problem.solve_batch(bat, 0) == [0, 0, 0]
+    for p, w, st in zip(bat, w_seq, st_seq):
+        np.testing.assert_array_equal(p.eigenvalues(4), w)
+        assert (p.stats()["iterations"], p.converged()) == st
