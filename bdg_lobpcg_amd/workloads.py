@@ -67,6 +67,11 @@ def ground_state(cfg, steps=None):
     return psi, mu, dt
 
 
+# graph replay of the solver's iteration chain for the problems built here
+# (bdg_set_graph_capture; on by default, as in the library)
+GRAPHS = True
+
+
 def make_problem(cfg, psi, mu, precision=0, maxiter=1000, tol=None, nev=None, m=None):
     import bdg_lobpcg_amd as B
 
@@ -80,6 +85,7 @@ def make_problem(cfg, psi, mu, precision=0, maxiter=1000, tol=None, nev=None, m=
     p.set_solver_params(nev or cfg["nev"], m or cfg["m"], maxiter, tol or cfg["tol"])
     p.set_precision(precision)
     p.set_init_uniform(42)
+    p.set_graph_capture(GRAPHS)
     return p
 
 

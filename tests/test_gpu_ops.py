@@ -169,3 +169,23 @@ def test_graph_replay_matches_eager(prec):
         assert np.array_equal(Y, Y_ref), f"replay {r}: operator chain differs"
         assert np.array_equal(G, G_ref), f"replay {r}: Gram differs"
     g.close()
+
+
+def test_column_chunking_and_reserve():
+    """Operator applies split into column chunks (bdgx_set_chunk_bytes) give
+    the unchunked result exactly; bdgx_reserve grows the workspaces."""
+    import bdg_lobpcg_amd as B
+    N, L = (32, 32, 64), (16.0, 16.0, 24.0)
+    c, f = physics(N, L, False, dipolar=True)
+    d = device_twin(N, L, False, f)
+    X = rand_block(c.size, 7, False, seed=21)
+    full = {op: d.apply(op, X, 0) for op in ("K", "M", "precondM")}
+    kk = c.size // 32 * 17 * 16  # spectrum bytes of one FP64 channel column
+    B.check(B.lib().bdgx_set_chunk_bytes(d.h, 2 * kk), "chunk")
+    for op, ref in full.items():
+        assert np.array_equal(d.apply(op, X, 0), ref), op
+    B.check(B.lib().bdgx_set_chunk_bytes(d.h, 0), "chunk")
+    B.check(B.lib().bdgx_reserve(d.h, 64 << 20, 1 << 20), "reserve")
+    assert B.lib().bdgx_workspace_bytes(d.h) >= (64 << 20)
+    for op, ref in full.items():
+        assert np.array_equal(d.apply(op, X, 0), ref), op
