@@ -325,7 +325,7 @@ class DeviceContext:
 
     def launch_variants(self, prec: int) -> dict:
         out = {}
-        for kind, name in enumerate(("plane_fwd", "plane_inv", "z", "combine")):
+        for kind, name in enumerate(("plane_fwd", "plane_inv", "z", "combine", "gram")):
             v = C.c_int()
             check(lib().bdgx_get_launch_variant(self.h, kind, int(prec), C.byref(v)),
                   "bdgx_get_launch_variant")

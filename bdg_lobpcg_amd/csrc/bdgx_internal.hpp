@@ -94,9 +94,9 @@ struct bdgx_ctx {
   double lp_s0;  // stencil-layer scale mean(p_K^2) / safe(mu)
   int has_lp;
   // launch-shape variants of the fixed-size FFT kernels, per precision
-  // (index 0 FP64, 1 FP32): [BDGX_TUNE_PLANE_FWD, _PLANE_INV, _Z, _COMBINE];
-  // set to measured defaults at creation, re-chosen by bdgx_autotune
-  int tune_var[2][4];
+  // (index 0 FP64, 1 FP32): [BDGX_TUNE_PLANE_FWD, _PLANE_INV, _Z, _COMBINE,
+  // _GRAM]; set to measured defaults at creation, re-chosen by bdgx_autotune
+  int tune_var[2][5];
   int tuned[2];
   // bumped whenever spec / scratch / lp_tmp / idx_buf move: graphs captured
   // before a move hold stale pointers and must not be launched

@@ -243,11 +243,16 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
                   int swap_b, double *dG) {
   if (p == 0 || q == 0) return BDGX_OK;
   const int W = CPLX ? 2 : 1;
-  // tile shape: real up to 96 x 32, complex up to 48 x 32 (12 accumulators)
+  // tile shape: real up to 96 x 32, complex up to 48 x 32 (12 accumulators);
+  // launch variant 1 (autotuned) caps real tiles at 48 rows of p: more p
+  // tiles re-read B, but each wave holds half the accumulators and operands
+  // (two waves per SIMD instead of one)
   int FQ = q <= 16 ? 1 : 2;
   int FP;
   if (CPLX)
     FP = p <= 16 ? 1 : 3;
+  else if (c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_GRAM])
+    FP = p <= 32 ? 2 : 3;
   else
     FP = p <= 32 ? 2 : (p <= 64 ? 4 : 6);
   const int tiles_p = (int)((p + 16 * FP - 1) / (16 * FP));
@@ -292,6 +297,9 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
     if (FP == 2) {
       if (FQ == 1) launch_gram<R, 2, 1, false>(a, nc, tiles, c->stream);
       else launch_gram<R, 2, 2, false>(a, nc, tiles, c->stream);
+    } else if (FP == 3) {
+      if (FQ == 1) launch_gram<R, 3, 1, false>(a, nc, tiles, c->stream);
+      else launch_gram<R, 3, 2, false>(a, nc, tiles, c->stream);
     } else if (FP == 4) {
       if (FQ == 1) launch_gram<R, 4, 1, false>(a, nc, tiles, c->stream);
       else launch_gram<R, 4, 2, false>(a, nc, tiles, c->stream);

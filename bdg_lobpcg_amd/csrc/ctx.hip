@@ -753,14 +753,14 @@ int bdgx_autotune(bdgx_ctx *c, int prec, uint64_t nbatch, int force) {
 }
 
 int bdgx_get_launch_variant(const bdgx_ctx *c, int kind, int prec, int *variant) {
-  if (!c || !variant || kind < 0 || kind > 3 || (prec != BDGX_FP64 && prec != BDGX_FP32))
+  if (!c || !variant || kind < 0 || kind > 4 || (prec != BDGX_FP64 && prec != BDGX_FP32))
     return BDGX_EINVAL;
   *variant = c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind];
   return BDGX_OK;
 }
 
 int bdgx_set_launch_variant(bdgx_ctx *c, int kind, int prec, int variant) {
-  if (!c || kind < 0 || kind > 3 || (prec != BDGX_FP64 && prec != BDGX_FP32) || variant < 0 ||
+  if (!c || kind < 0 || kind > 4 || (prec != BDGX_FP64 && prec != BDGX_FP32) || variant < 0 ||
       variant > 1)
     return BDGX_EINVAL;
   c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind] = variant;

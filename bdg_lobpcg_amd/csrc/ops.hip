@@ -877,6 +877,9 @@ static int bench_pass_t(bdgx_ctx *c, int pass, uint64_t nbatch, int iters, float
 
 int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
                      const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows);
+int gram_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
+                  const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
+                  double *dG);
 
 // Launch-shape autotuner: every variant of every kernel kind with variants
 // that this grid uses (plane and fixed z passes on nbatch kinetic batches,
@@ -919,12 +922,15 @@ static int autotune_t(bdgx_ctx *c, uint64_t nbatch) {
   XArgs<R> a = make_xargs<R>(c, J, 0, nc, X, Y, (long long)g.S, spec, (long long)nc * g.Kk,
                              nullptr);
   const YZArgs<R> yz = make_yz<R>(c, mul, 1, nc, spec);
-  const int kinds[4] = {BDGX_TUNE_PLANE_FWD, BDGX_TUNE_Z, BDGX_TUNE_PLANE_INV,
-                        BDGX_TUNE_COMBINE};
+  const int kinds[5] = {BDGX_TUNE_PLANE_FWD, BDGX_TUNE_Z, BDGX_TUNE_PLANE_INV,
+                        BDGX_TUNE_COMBINE, BDGX_TUNE_GRAM};
   const int prec = sizeof(R) == 8 ? BDGX_FP64 : BDGX_FP32;
-  for (int q = 0; q < 4 && !rc; ++q) {
+  // Gram output (3 nbatch x nbatch doubles): the head of dC
+  for (int q = 0; q < 5 && !rc; ++q) {
     const int kind = kinds[q];
-    const bool on = kind == BDGX_TUNE_Z ? zfix : (kind == BDGX_TUNE_COMBINE ? comb : plane);
+    const bool on = kind == BDGX_TUNE_Z ? zfix
+                    : (kind == BDGX_TUNE_COMBINE || kind == BDGX_TUNE_GRAM) ? comb
+                                                                             : plane;
     if (!on) continue;
     auto one = [&]() -> int {
       // launch_z fills in the z twiddles and plan before the fixed z pass
@@ -932,6 +938,9 @@ static int autotune_t(bdgx_ctx *c, uint64_t nbatch) {
       if (kind == BDGX_TUNE_COMBINE)
         return combine_dispatch(c, prec, X, (uint64_t)g.S, 3 * nbatch, dC, Y, (uint64_t)g.S,
                                 2 * nbatch, (uint64_t)g.S);
+      if (kind == BDGX_TUNE_GRAM)
+        return gram_dispatch(c, prec, X, (uint64_t)g.S, 3 * nbatch, Y, (uint64_t)g.S, nbatch,
+                             (uint64_t)g.S, 0, dC);
       return launch_plane_x<R>(c, a, kind == BDGX_TUNE_PLANE_INV);
     };
     int best = c->tune_var[pi][kind];

@@ -548,6 +548,7 @@ void tune_defaults(bdgx_ctx *c) {
     c->tune_var[p][BDGX_TUNE_PLANE_INV] = big;
     c->tune_var[p][BDGX_TUNE_Z] = 0;
     c->tune_var[p][BDGX_TUNE_COMBINE] = 0;
+    c->tune_var[p][BDGX_TUNE_GRAM] = 0;
     c->tuned[p] = 0;
   }
 }

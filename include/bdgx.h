@@ -159,15 +159,17 @@ int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iter
  * fixed-size FFT kernels (whole-plane forward / inverse, fixed z pass) and
  * the block combine have launch variants: plane 0 = 4 Nx threads, 1 = 8 Nx
  * threads per plane; z 0 = 256 (Nz = 128) or 128 threads, 1 = half of that;
- * combine 0 = 4 rows per lane, 1 = 2 rows per lane.  bdgx_autotune times
- * every variant of every kind for `prec` on `nbatch` columns (kinetic
- * batches for the FFT kinds, an nbatch x nbatch combine; 1 warm-up + 3 timed
+ * combine 0 = 4 rows per lane, 1 = 2 rows per lane; real Gram 0 = tiles of
+ * up to 96 rows of p, 1 = up to 48.  bdgx_autotune times every variant of
+ * every kind for `prec` on `nbatch` columns (kinetic batches for the FFT
+ * kinds, a 3 nbatch -> 2 nbatch combine and a 3 nbatch x nbatch Gram, the
+ * solver's main shapes; 1 warm-up + 3 timed
  * launches each, HIP events on the ctx stream) and keeps the fastest; it
  * returns at once when `prec` is already tuned and `force` is 0.  The FFT
  * kinds are skipped on grids that use the pencil passes.  The variants
  * compute identical results (same operations in the same order). */
 enum { BDGX_TUNE_PLANE_FWD = 0, BDGX_TUNE_PLANE_INV = 1, BDGX_TUNE_Z = 2,
-       BDGX_TUNE_COMBINE = 3 };
+       BDGX_TUNE_COMBINE = 3, BDGX_TUNE_GRAM = 4 };
 int bdgx_autotune(bdgx_ctx *ctx, int prec, uint64_t nbatch, int force);
 int bdgx_get_launch_variant(const bdgx_ctx *ctx, int kind, int prec, int *variant);
 int bdgx_set_launch_variant(bdgx_ctx *ctx, int kind, int prec, int variant);

@@ -123,6 +123,15 @@ def test_launch_variants_and_autotune(grid):
             d.set_launch_variant(3, prec, v)  # block combine
             e = relerr(d.combine(A, Cm, prec), A @ Cm)
             assert e < (1e-14 if prec == 0 else 1e-6), f"combine variant {v} prec {prec}: {e:.2e}"
+        Ag = rng.standard_normal((c.size, 90))
+        Bg = rng.standard_normal((c.size, 20))
+        for v in (0, 1):
+            d.set_launch_variant(4, prec, v)  # Gram tile height
+            G = d.gram(Ag, Bg, prec)
+            ref = (Ag.astype(np.float32 if prec else np.float64).astype(np.float64).T
+                   @ Bg.astype(np.float32 if prec else np.float64).astype(np.float64))
+            e = relerr(G, ref)
+            assert e < 1e-12, f"gram variant {v} prec {prec}: {e:.2e}"
         chosen = d.autotune(prec, 8, force=True)
         assert set(chosen.values()) <= {0, 1}
         for op, ref in refs.items():
