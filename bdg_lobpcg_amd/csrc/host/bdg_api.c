@@ -6,7 +6,9 @@
  * through the bdgx C ABI: the dipolar kernel and mean field at setup, and
  * the whole eigen-iteration in bdg_solve.
  */
+#define _DEFAULT_SOURCE /* madvise, MADV_HUGEPAGE */
 #include <pthread.h>
+#include <sys/mman.h>
 #include <complex.h>
 #include <inttypes.h>
 
@@ -19,6 +21,26 @@
 static void *xcalloc(size_t n, size_t s) {
   void *p = calloc(n ? n : 1, s ? s : 1);
   if (!p) BDG_FAIL("out of host memory (%zu x %zu bytes)", n, s);
+  return p;
+}
+
+static void *xmalloc(size_t n) {
+  void *p = malloc(n ? n : 1);
+  if (!p) BDG_FAIL("out of host memory (%zu bytes)", n);
+  return p;
+}
+
+/* large host blocks (start block, result modes): 2 MiB aligned and marked for
+ * transparent huge pages, so that first touch faults once per 2 MiB instead
+ * of once per 4 KiB page (a 1 GB block otherwise costs ~0.25 s of faults);
+ * freed with free() */
+static void *xmalloc_big(size_t n) {
+  const size_t align = (size_t)2 << 20;
+  if (n < align) return xmalloc(n);
+  void *p = NULL;
+  if (posix_memalign(&p, align, n) != 0 || !p)
+    BDG_FAIL("out of host memory (%zu bytes)", n);
+  (void)madvise(p, n, MADV_HUGEPAGE);
   return p;
 }
 
@@ -487,15 +509,19 @@ int bdg_solve(bdg_t *b) {
     b->stats.status = rc;
     return rc;
   }
-  /* start block [u; v] on the host, v half to the device */
-  void *X = xcalloc(n * m, es64);
+  /* start block [u; v] on the host (zero-filled anonymous mapping, huge
+   * pages), v half to the device */
+  const size_t xbytes = n * m * es64;
+  void *X = mmap(NULL, xbytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+  if (X == MAP_FAILED) BDG_FAIL("out of host memory (%zu bytes)", xbytes);
+  (void)madvise(X, xbytes, MADV_HUGEPAGE);
   bdg_fill_start_block(b, X);
   void *dV0 = NULL;
   rc = bdgx_malloc(&dV0, S * m * es64);
   for (uint64_t j = 0; j < m && !rc; ++j)
     rc = bdgx_memcpy_h2d(b->dev, (char *)dV0 + j * S * es64, (const char *)X + (j * n + S) * es64,
                          S * es64);
-  free(X);
+  munmap(X, xbytes);
   if (rc) {
     bdgx_free(dV0);
     b->stats.status = rc;
@@ -561,18 +587,12 @@ int bdg_solve(bdg_t *b) {
     b->stats.seconds_total = bdg_wtime() - t_start;
     return rc;
   }
-  /* results: w = signed sqrt(w2), v = X, u = M X / w, ||[u; v]|| = 1 */
-  double *hX = (double *)xcalloc(S * m, es64);
-  double *hMX = (double *)xcalloc(S * m, es64);
-  rc = bdgx_memcpy_d2h(b->dev, hX, dX, S * m * es64);
-  if (!rc) rc = bdgx_memcpy_d2h(b->dev, hMX, dMX, S * m * es64);
-  bdgx_free(dX);
-  bdgx_free(dMX);
+  /* results: w = signed sqrt(w2), v = X, u = M X / w, ||[u; v]|| = 1; the
+   * columns are scaled on the device and copied straight into the result
+   * arrays */
   b->eigvals = (double *)xcalloc(nev, sizeof(double));
   b->residuals = (double *)xcalloc(nev, sizeof(double));
   b->zero_modes = (int *)xcalloc(nev, sizeof(int));
-  b->modes_u = xcalloc(S * nev, es64);
-  b->modes_v = xcalloc(S * nev, es64);
   double scale = 1.0;
   for (uint64_t j = 0; j < nev; ++j) {
     const double w2 = r.w2[j];
@@ -580,28 +600,68 @@ int bdg_solve(bdg_t *b) {
     scale = fmax(scale, fabs(b->eigvals[j]));
   }
   uint64_t conv = 0;
-  const int W = c ? 2 : 1;
   for (uint64_t j = 0; j < nev; ++j) {
-    const double w = b->eigvals[j];
     b->residuals[j] = r.res[j];
-    b->zero_modes[j] = r.zflag[j] || fabs(w) < 1e-3 * scale;
+    b->zero_modes[j] = r.zflag[j] || fabs(b->eigvals[j]) < 1e-3 * scale;
     if (b->zero_modes[j] || r.res[j] < b->tol) ++conv;
-    const double inv = fabs(w) > 0 ? 1.0 / w : 1.0;
-    double nrm = 0.0;
-    for (uint64_t i = 0; i < S * W; ++i) {
-      const double uu = hMX[j * S * W + i] * inv, vv = hX[j * S * W + i];
-      nrm += uu * uu + vv * vv;
-    }
-    nrm = nrm > 0 ? 1.0 / sqrt(nrm) : 1.0;
-    double *mu = (double *)b->modes_u + j * S * W, *mv = (double *)b->modes_v + j * S * W;
-    for (uint64_t i = 0; i < S * W; ++i) {
-      mu[i] = hMX[j * S * W + i] * inv * nrm;
-      mv[i] = hX[j * S * W + i] * nrm;
-    }
   }
+  /* column norms ||X_j||^2 and ||M X_j||^2 (the residual kernel with w = 0) */
+  double *dzero = NULL, *dn2 = NULL, *dcu = NULL, *dcv = NULL;
+  void *dU = NULL;
+  double *hn2 = (double *)xcalloc(3 * nev, sizeof(double));
+  double *cu = (double *)xcalloc(nev * nev, sizeof(double));
+  double *cv = (double *)xcalloc(nev * nev, sizeof(double));
+  rc = bdgx_malloc((void **)&dzero, sizeof(double) * nev);
+  if (!rc) rc = bdgx_malloc((void **)&dn2, sizeof(double) * 3 * nev);
+  if (!rc) rc = bdgx_memset(b->dev, dzero, 0, sizeof(double) * nev);
+  if (!rc)
+    rc = bdgx_residual(b->dev, BDGX_FP64, dMX, dX, dMX, S, nev, S, dzero, 0, NULL, S, dn2);
+  if (!rc) rc = bdgx_memcpy_d2h(b->dev, hn2, dn2, sizeof(double) * 3 * nev);
+  if (!rc) {
+    for (uint64_t j = 0; j < nev; ++j) {
+      const double w = b->eigvals[j];
+      const double inv = fabs(w) > 0 ? 1.0 / w : 1.0;
+      const double nrm2 = hn2[3 * j + 2] * inv * inv + hn2[3 * j + 1];
+      const double nrm = nrm2 > 0 ? 1.0 / sqrt(nrm2) : 1.0;
+      cu[j + j * nev] = inv * nrm;
+      cv[j + j * nev] = nrm;
+    }
+    rc = bdgx_malloc((void **)&dcu, sizeof(double) * nev * nev * (c ? 2 : 1));
+  }
+  if (!rc) rc = bdgx_malloc((void **)&dcv, sizeof(double) * nev * nev * (c ? 2 : 1));
+  if (!rc && c) {
+    /* complex combine coefficients: interleaved (re, im) */
+    double *cz = (double *)xcalloc(2 * nev * nev, sizeof(double));
+    for (uint64_t j = 0; j < nev * nev; ++j) cz[2 * j] = cu[j];
+    rc = bdgx_memcpy_h2d(b->dev, dcu, cz, sizeof(double) * 2 * nev * nev);
+    for (uint64_t j = 0; j < nev * nev; ++j) cz[2 * j] = cv[j];
+    if (!rc) rc = bdgx_memcpy_h2d(b->dev, dcv, cz, sizeof(double) * 2 * nev * nev);
+    free(cz);
+  } else if (!rc) {
+    rc = bdgx_memcpy_h2d(b->dev, dcu, cu, sizeof(double) * nev * nev);
+    if (!rc) rc = bdgx_memcpy_h2d(b->dev, dcv, cv, sizeof(double) * nev * nev);
+  }
+  /* u = M X diag(inv nrm) into a fresh buffer, v = X diag(nrm) into M X */
+  if (!rc) rc = bdgx_malloc(&dU, S * nev * es64);
+  if (!rc) rc = bdgx_combine(b->dev, BDGX_FP64, dMX, S, nev, dcu, dU, S, nev, S);
+  if (!rc) rc = bdgx_combine(b->dev, BDGX_FP64, dX, S, nev, dcv, dMX, S, nev, S);
+  if (!rc) {
+    b->modes_u = xmalloc_big(S * nev * es64);
+    b->modes_v = xmalloc_big(S * nev * es64);
+    rc = bdgx_memcpy_d2h(b->dev, b->modes_u, dU, S * nev * es64);
+    if (!rc) rc = bdgx_memcpy_d2h(b->dev, b->modes_v, dMX, S * nev * es64);
+  }
+  bdgx_free(dzero);
+  bdgx_free(dn2);
+  bdgx_free(dcu);
+  bdgx_free(dcv);
+  bdgx_free(dU);
+  bdgx_free(dX);
+  bdgx_free(dMX);
+  free(hn2);
+  free(cu);
+  free(cv);
   b->converged = conv;
-  free(hX);
-  free(hMX);
   free(r.w2);
   free(r.zflag);
   free(r.res);
