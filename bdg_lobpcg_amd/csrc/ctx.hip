@@ -3,6 +3,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "bdgx_internal.hpp"
@@ -746,10 +747,58 @@ int bdgx_reserve(bdgx_ctx *c, uint64_t spec_bytes, uint64_t scratch_bytes) {
   return rc;
 }
 
+// Process-wide record of tuned launch variants, keyed by device, grid, psi0
+// kind, precision and batch width (like planner wisdom): a new context for a
+// shape already tuned in this process takes the recorded choice instead of
+// timing again.  Guarded by a mutex (contexts may tune from several threads).
+namespace {
+struct TuneRecord {
+  int device, cplx, prec;
+  int N[3];
+  uint64_t nbatch;
+  int var[5];
+};
+std::mutex g_tune_lock;
+std::vector<TuneRecord> g_tune_records;
+
+bool tune_key_eq(const TuneRecord &r, const bdgx_ctx *c, int prec, uint64_t nbatch) {
+  return r.device == c->device && r.cplx == c->cplx && r.prec == prec && r.nbatch == nbatch &&
+         r.N[0] == c->geo.nx && r.N[1] == c->geo.ny && r.N[2] == c->geo.nz;
+}
+}  // namespace
+
 int bdgx_autotune(bdgx_ctx *c, int prec, uint64_t nbatch, int force) {
   if (!c || (prec != BDGX_FP64 && prec != BDGX_FP32)) return BDGX_EINVAL;
-  if (c->tuned[prec == BDGX_FP64 ? 0 : 1] && !force) return BDGX_OK;
-  return autotune_dispatch(c, prec, nbatch);
+  const int pi = prec == BDGX_FP64 ? 0 : 1;
+  if (c->tuned[pi] && !force) return BDGX_OK;
+  if (!force) {
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    for (const TuneRecord &r : g_tune_records)
+      if (tune_key_eq(r, c, prec, nbatch)) {
+        for (int k = 0; k < 5; ++k) c->tune_var[pi][k] = r.var[k];
+        c->tuned[pi] = 1;
+        return BDGX_OK;
+      }
+  }
+  const int rc = autotune_dispatch(c, prec, nbatch);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(g_tune_lock);
+  TuneRecord rec;
+  rec.device = c->device;
+  rec.cplx = c->cplx;
+  rec.prec = prec;
+  rec.N[0] = c->geo.nx;
+  rec.N[1] = c->geo.ny;
+  rec.N[2] = c->geo.nz;
+  rec.nbatch = nbatch;
+  for (int k = 0; k < 5; ++k) rec.var[k] = c->tune_var[pi][k];
+  for (TuneRecord &r : g_tune_records)
+    if (tune_key_eq(r, c, prec, nbatch)) {
+      r = rec;
+      return BDGX_OK;
+    }
+  g_tune_records.push_back(rec);
+  return BDGX_OK;
 }
 
 int bdgx_get_launch_variant(const bdgx_ctx *c, int kind, int prec, int *variant) {

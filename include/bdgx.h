@@ -165,9 +165,12 @@ int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iter
  * kinds, a 3 nbatch -> 2 nbatch combine and a 3 nbatch x nbatch Gram, the
  * solver's main shapes; 1 warm-up + 3 timed
  * launches each, HIP events on the ctx stream) and keeps the fastest; it
- * returns at once when `prec` is already tuned and `force` is 0.  The FFT
- * kinds are skipped on grids that use the pencil passes.  The variants
- * compute identical results (same operations in the same order). */
+ * returns at once when `prec` is already tuned and `force` is 0.  Choices
+ * are recorded process-wide per device, grid, psi0 kind, precision and
+ * nbatch: with `force` 0 a new context of a recorded shape takes them without
+ * timing.  The FFT kinds are skipped on grids that use the pencil passes.
+ * The variants compute identical results (same operations in the same
+ * order). */
 enum { BDGX_TUNE_PLANE_FWD = 0, BDGX_TUNE_PLANE_INV = 1, BDGX_TUNE_Z = 2,
        BDGX_TUNE_COMBINE = 3, BDGX_TUNE_GRAM = 4 };
 int bdgx_autotune(bdgx_ctx *ctx, int prec, uint64_t nbatch, int force);
