@@ -604,11 +604,14 @@ template <typename R, bool CPLX>
 __global__ void __launch_bounds__(256) k_residual(const R *AX, const R *X, const R *MX,
                                                   long long ld, long long nrows,
                                                   const double *lam, int swap_x, R *Rr,
-                                                  long long ldr, double *part) {
+                                                  long long ldr, double *part,
+                                                  const int *cols) {
   constexpr int W = CPLX ? 2 : 1;
-  const int col = blockIdx.y;
+  // output column j = blockIdx.y reads source column cols[j] (or j)
+  const int j = blockIdx.y;
+  const int col = cols ? cols[j] : j;
   const long long half = nrows / 2;
-  const double l = lam[col];
+  const double l = lam[j];
   double sr = 0.0, sx = 0.0, sm = 0.0;
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nrows;
        r += (long long)gridDim.x * blockDim.x) {
@@ -619,7 +622,7 @@ __global__ void __launch_bounds__(256) k_residual(const R *AX, const R *X, const
       const double xs = (double)X[(rs + col * ld) * W + w];
       const double xv = (double)X[(r + col * ld) * W + w];
       const double rv = ax - l * xs;
-      if (Rr) Rr[(r + col * ldr) * W + w] = (R)rv;
+      if (Rr) Rr[(r + j * ldr) * W + w] = (R)rv;
       sr += rv * rv;
       sx += xv * xv;
       if (MX) {
@@ -642,13 +645,13 @@ __global__ void __launch_bounds__(256) k_residual(const R *AX, const R *X, const
   }
   // per-block partials, summed in a fixed order by reduce_partials
   if (threadIdx.x < 3)
-    part[(long long)blockIdx.x * (3 * gridDim.y) + 3 * col + threadIdx.x] = red[threadIdx.x][0];
+    part[(long long)blockIdx.x * (3 * gridDim.y) + 3 * j + threadIdx.x] = red[threadIdx.x][0];
 }
 
 template <typename R, bool CPLX>
 static int residual_t(bdgx_ctx *c, const void *dAX, const void *dX, const void *dMX,
                       uint64_t ld, uint64_t ncols, uint64_t nrows, const double *dlam,
-                      int swap_x, void *dR, uint64_t ldr, double *dn2) {
+                      int swap_x, void *dR, uint64_t ldr, double *dn2, const int *dcols) {
   if (ncols == 0) return BDGX_OK;
   long long nbx = ((long long)nrows + 255) / 256;
   if (nbx > 512) nbx = 512;
@@ -661,21 +664,37 @@ static int residual_t(bdgx_ctx *c, const void *dAX, const void *dX, const void *
                      0, c->stream, reinterpret_cast<const R *>(dAX),
                      reinterpret_cast<const R *>(dX), reinterpret_cast<const R *>(dMX),
                      (long long)ld, (long long)nrows, dlam, swap_x, reinterpret_cast<R *>(dR),
-                     (long long)ldr, part);
+                     (long long)ldr, part, dcols);
   BDGX_HIP(hipGetLastError());
   return reduce_partials(c, part, (int)nbx, len, part + nbx * len, dn2);
 }
 
 int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,
                       uint64_t ld, uint64_t ncols, uint64_t nrows, const double *dlam,
-                      int swap_x, void *dR, uint64_t ldr, double *dn2) {
+                      int swap_x, void *dR, uint64_t ldr, double *dn2, const int *dcols) {
   if (prec == BDGX_FP64)
-    return c->cplx ? residual_t<double, true>(c, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2)
-                   : residual_t<double, false>(c, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2);
+    return c->cplx ? residual_t<double, true>(c, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2, dcols)
+                   : residual_t<double, false>(c, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2, dcols);
   if (prec == BDGX_FP32)
-    return c->cplx ? residual_t<float, true>(c, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2)
-                   : residual_t<float, false>(c, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2);
+    return c->cplx ? residual_t<float, true>(c, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2, dcols)
+                   : residual_t<float, false>(c, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2, dcols);
   return BDGX_EINVAL;
+}
+
+// host column list into the ctx index buffer (grown on demand)
+int upload_cols(bdgx_ctx *c, const int *hidx, uint64_t ncols) {
+  if (!c->idx_buf || c->idx_cap < ncols) {
+    if (c->idx_buf) {
+      BDGX_HIP(hipStreamSynchronize(c->stream));
+      BDGX_HIP(hipFree(c->idx_buf));
+    }
+    c->idx_cap = ncols < 1024 ? 1024 : ncols;
+    BDGX_HIP(hipMalloc((void **)&c->idx_buf, sizeof(int) * c->idx_cap));
+    c->ws_gen++;
+  }
+  BDGX_HIP(hipMemcpyAsync(c->idx_buf, hidx, sizeof(int) * ncols, hipMemcpyHostToDevice,
+                          c->stream));
+  return BDGX_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -696,17 +715,8 @@ int gather_dispatch(bdgx_ctx *c, int prec, const void *src, uint64_t lds, const 
   int rc = ensure_scratch(c, 0);
   (void)rc;
   // indices go through a small device buffer owned by the context
-  if (!c->idx_buf || c->idx_cap < ncols) {
-    if (c->idx_buf) {
-      BDGX_HIP(hipStreamSynchronize(c->stream));
-      BDGX_HIP(hipFree(c->idx_buf));
-    }
-    c->idx_cap = ncols < 1024 ? 1024 : ncols;
-    BDGX_HIP(hipMalloc((void **)&c->idx_buf, sizeof(int) * c->idx_cap));
-    c->ws_gen++;
-  }
-  BDGX_HIP(hipMemcpyAsync(c->idx_buf, hidx, sizeof(int) * ncols, hipMemcpyHostToDevice,
-                          c->stream));
+  rc = upload_cols(c, hidx, ncols);
+  if (rc) return rc;
   const int W = c->cplx ? 2 : 1;
   long long nbx = ((long long)nrows + 255) / 256;
   if (nbx > 1024) nbx = 1024;

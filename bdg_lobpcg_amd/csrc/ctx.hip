@@ -22,7 +22,8 @@ int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64
                      const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows);
 int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,
                       uint64_t ld, uint64_t ncols, uint64_t nrows, const double *dlam,
-                      int swap_x, void *dR, uint64_t ldr, double *dn2);
+                      int swap_x, void *dR, uint64_t ldr, double *dn2, const int *dcols);
+int upload_cols(bdgx_ctx *c, const int *hidx, uint64_t ncols);
 int convert_dispatch(bdgx_ctx *c, int ps, const void *src, uint64_t lds, int pd, void *dst,
                      uint64_t ldd, uint64_t nrows, uint64_t ncols);
 int autotune_dispatch(bdgx_ctx *c, int prec, uint64_t nbatch);
@@ -476,7 +477,20 @@ int bdgx_residual(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const 
                   void *dR, uint64_t ldr, double *dn2) {
   if (!c || !dAX || !dX || !dlam || !dn2) return BDGX_EINVAL;
   if (swap_x && (nrows & 1)) return BDGX_EINVAL;
-  return residual_dispatch(c, prec, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2);
+  return residual_dispatch(c, prec, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2,
+                           nullptr);
+}
+
+int bdgx_residual_cols(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,
+                       uint64_t ld, const int *cols, uint64_t ncols, uint64_t nrows,
+                       const double *dlam, int swap_x, void *dR, uint64_t ldr, double *dn2) {
+  if (!c || !dAX || !dX || !dlam || !dn2 || (ncols && !cols)) return BDGX_EINVAL;
+  if (swap_x && (nrows & 1)) return BDGX_EINVAL;
+  if (ncols == 0) return BDGX_OK;
+  int rc = upload_cols(c, cols, ncols);
+  if (rc) return rc;
+  return residual_dispatch(c, prec, dAX, dX, dMX, ld, ncols, nrows, dlam, swap_x, dR, ldr, dn2,
+                           c->idx_buf);
 }
 
 int bdgx_gather_cols(bdgx_ctx *c, int prec, const void *src, uint64_t lds, const int *idx,

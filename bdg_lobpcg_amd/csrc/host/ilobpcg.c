@@ -293,39 +293,67 @@ static int grams_cols_host(bdgx_ctx *ctx, int std, int k, int c0, int nc, int w,
 typedef struct {
   double *resv;
   int *conv, *act;
+  int *rpos;      /* column of dR holding the residual of act[i] */
+  int *sel;       /* columns evaluated by a partial evaluation */
+  double *lsel;   /* their w^2 */
   int nact, allc;
   double worst;
 } res_state;
 
+/* Residuals R = KMX - w^2 X and the convergence decisions.  Full (nsel = 0):
+ * every column, dR column j = column j.  Partial (nsel > 0): only the
+ * columns sel[0..nsel), packed into dR columns 0..nsel-1; the other columns
+ * keep the state of the last full evaluation (they had converged). */
 static int eval_residuals(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const blk3 *A, const double *w2,
                           const int *zf, double *dlam, double *dn2, double *hn2, void *dR,
-                          res_state *st) {
+                          res_state *st, int nsel) {
   const uint64_t S = cfg->S;
   const int m = cfg->m, nev = cfg->nev;
-  int rc = bdgx_memcpy_h2d(ctx, dlam, w2, sizeof(double) * (size_t)m);
-  if (!rc)
-    rc = bdgx_residual(ctx, BDGX_FP64, A->KMV, A->V, A->MV, S, (uint64_t)m, S, dlam, 0, dR, S,
-                       dn2);
-  if (!rc) rc = bdgx_memcpy_d2h(ctx, hn2, dn2, sizeof(double) * 3 * (size_t)m);
+  int rc;
+  if (nsel == 0) {
+    rc = bdgx_memcpy_h2d(ctx, dlam, w2, sizeof(double) * (size_t)m);
+    if (!rc)
+      rc = bdgx_residual(ctx, BDGX_FP64, A->KMV, A->V, A->MV, S, (uint64_t)m, S, dlam, 0, dR, S,
+                         dn2);
+    if (!rc) rc = bdgx_memcpy_d2h(ctx, hn2, dn2, sizeof(double) * 3 * (size_t)m);
+  } else {
+    for (int i = 0; i < nsel; ++i) st->lsel[i] = w2[st->sel[i]];
+    rc = bdgx_memcpy_h2d(ctx, dlam, st->lsel, sizeof(double) * (size_t)nsel);
+    if (!rc)
+      rc = bdgx_residual_cols(ctx, BDGX_FP64, A->KMV, A->V, A->MV, S, st->sel, (uint64_t)nsel, S,
+                              dlam, 0, dR, S, dn2);
+    if (!rc) rc = bdgx_memcpy_d2h(ctx, hn2, dn2, sizeof(double) * 3 * (size_t)nsel);
+  }
   if (rc) return rc;
   double scale = 1.0;
   for (int j = 0; j < nev && j < m; ++j) scale = fmax(scale, fabs(signed_sqrt(w2[j])));
+  const int ncols = nsel ? nsel : m;
+  for (int i = 0; i < ncols; ++i) {
+    const int j = nsel ? st->sel[i] : i;
+    const double wj = signed_sqrt(w2[j]);
+    const double aw = fmax(fabs(wj), 1e-300);
+    const double rn = sqrt(hn2[3 * i]), xn = sqrt(hn2[3 * i + 1]), mn = sqrt(hn2[3 * i + 2]);
+    const double xnorm = sqrt(xn * xn + (mn / aw) * (mn / aw));
+    st->resv[j] = (rn / aw) / (xnorm * fmax(1.0, fabs(wj)));
+  }
   st->nact = 0;
   st->allc = 1;
   st->worst = 0.0;
+  int si = 0;
   for (int j = 0; j < m; ++j) {
     const double wj = signed_sqrt(w2[j]);
-    const double aw = fmax(fabs(wj), 1e-300);
-    const double rn = sqrt(hn2[3 * j]), xn = sqrt(hn2[3 * j + 1]), mn = sqrt(hn2[3 * j + 2]);
-    const double xnorm = sqrt(xn * xn + (mn / aw) * (mn / aw));
-    st->resv[j] = (rn / aw) / (xnorm * fmax(1.0, fabs(wj)));
     const int zero = zf[j] || fabs(wj) < cfg->zero_tol * scale;
-    st->conv[j] = zero || st->resv[j] < cfg->tol;
+    const int evaluated = !nsel || (si < nsel && st->sel[si] == j);
+    if (evaluated) st->conv[j] = zero || st->resv[j] < cfg->tol;
     if (j < nev) {
       if (!st->conv[j]) st->allc = 0;
       if (!zero) st->worst = fmax(st->worst, st->resv[j]);
     }
-    if (!st->conv[j]) st->act[st->nact++] = j;
+    if (!st->conv[j]) {
+      st->rpos[st->nact] = nsel ? si : j;
+      st->act[st->nact++] = j;
+    }
+    if (nsel && evaluated) ++si;
   }
   return 0;
 }
@@ -520,6 +548,10 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   st.resv = (double *)malloc(sizeof(double) * (size_t)m);
   st.conv = (int *)calloc((size_t)m, sizeof(int));
   st.act = (int *)malloc(sizeof(int) * (size_t)m);
+  st.rpos = (int *)malloc(sizeof(int) * (size_t)m);
+  st.sel = (int *)malloc(sizeof(int) * (size_t)m);
+  st.lsel = (double *)malloc(sizeof(double) * (size_t)m);
+  int last_full = 0; /* iteration of the last full residual evaluation */
   int it = 0, np = 0, kprev = 0, ncp = 0, gvalid = 0, since_full = 0, fresh = 1;
   double trr = 0.0;
   int refresh_it = 0;
@@ -590,7 +622,21 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   }
 
   for (it = 1; it <= cfg->maxiter; ++it) {
-    TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st));
+    /* few active columns: evaluate only those, with a full evaluation every
+     * 10 iterations and before convergence is accepted (converged columns
+     * still move a little under the Rayleigh-Ritz) */
+    int nsel = 0;
+    if (it > 1 && st.nact > 0 && st.nact <= m / 4 && it - last_full < 10) {
+      nsel = st.nact;
+      for (int i = 0; i < nsel; ++i) st.sel[i] = st.act[i];
+    }
+    TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st, nsel));
+    if (nsel == 0) {
+      last_full = it;
+    } else if (st.allc) {
+      TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st, 0));
+      last_full = it;
+    }
     /* mixed: FP64 images of X every refresh_every iterations while new
      * directions get FP32 images, when the residual reaches img_tol (FP64
      * images from then on), and before convergence is accepted */
@@ -602,7 +648,8 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       fresh = 1;
       gvalid = 0;
       refresh_it = it;
-      TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st));
+      TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st, 0));
+      last_full = it;
     }
     const int nact = st.nact;
     const int *act = st.act;
@@ -634,9 +681,13 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     /* W = T_M T_K R[:, act] at columns [m + np, m + np + nact) */
     const int wc = m + np;
     const int k = m + np + nact;
+    /* the active residual columns, packed: dR itself when they already are */
     const void *Rsrc = dR;
-    if (nact < m) {
-      TRY(bdgx_gather_cols(ctx, BDGX_FP64, dR, S, act, (uint64_t)nact, dTmp, S, S));
+    int packed = 1;
+    for (int i = 0; i < nact; ++i)
+      if (st.rpos[i] != i) packed = 0;
+    if (!packed) {
+      TRY(bdgx_gather_cols(ctx, BDGX_FP64, dR, S, st.rpos, (uint64_t)nact, dTmp, S, S));
       Rsrc = dTmp;
     }
     void *other = (Rsrc == dR) ? dTmp : dR;
@@ -839,6 +890,9 @@ done:
   free(st.resv);
   free(st.conv);
   free(st.act);
+  free(st.rpos);
+  free(st.sel);
+  free(st.lsel);
   free(worst_hist);
   return rc;
 }

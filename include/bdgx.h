@@ -107,6 +107,13 @@ int bdgx_residual(bdgx_ctx *ctx, int prec, const void *dAX, const void *dX,
                   const void *dMX, uint64_t ld, uint64_t ncols, uint64_t nrows,
                   const double *dlambda, int swap_x, void *dR, uint64_t ldr,
                   double *dnorms2);
+/* bdgx_residual on the columns cols[0..ncols) (a host array) of dAX, dX, dMX
+ * only: output column j of dR and norms j of dnorms2 belong to source column
+ * cols[j], with eigenvalue dlambda[j]. */
+int bdgx_residual_cols(bdgx_ctx *ctx, int prec, const void *dAX, const void *dX,
+                       const void *dMX, uint64_t ld, const int *cols, uint64_t ncols,
+                       uint64_t nrows, const double *dlambda, int swap_x, void *dR,
+                       uint64_t ldr, double *dnorms2);
 /* dDst[:, j] = dSrc[:, idx[j]] for j < ncols (same precision, one launch);
  * idx is a host array.  dDst must not overlap the gathered source columns. */
 int bdgx_gather_cols(bdgx_ctx *ctx, int prec, const void *dSrc, uint64_t lds,

@@ -198,3 +198,38 @@ def test_column_chunking_and_reserve():
     assert B.lib().bdgx_workspace_bytes(d.h) >= (64 << 20)
     for op, ref in full.items():
         assert np.array_equal(d.apply(op, X, 0), ref), op
+
+
+def test_residual_on_selected_columns():
+    """bdgx_residual_cols gives the selected columns of bdgx_residual (packed
+    output, eigenvalues in selection order) bit for bit."""
+    import ctypes as C
+    import bdg_lobpcg_amd as B
+    N, L = (16, 16, 16), (12.0, 12.0, 12.0)
+    c, f = physics(N, L, False, dipolar=True)
+    d = device_twin(N, L, False, f)
+    S, m = c.size, 9
+    rng = np.random.default_rng(4)
+    AX, X, MX = (np.asfortranarray(rng.standard_normal((S, m))) for _ in range(3))
+    lam = rng.standard_normal(m)
+    dAX, dX, dMX, dlam = d.upload(AX), d.upload(X), d.upload(MX), d.upload(lam)
+    dR, dn = B.DeviceBuffer(8 * S * m), B.DeviceBuffer(8 * 3 * m)
+    B.check(B.lib().bdgx_residual(d.h, 0, dAX.ptr, dX.ptr, dMX.ptr, S, m, S, dlam.ptr, 0, dR.ptr,
+                                  S, dn.ptr), "residual")
+    R = d.download(dR, (S, m), np.float64)
+    n2 = d.download(dn, (3 * m,), np.float64)
+    np.testing.assert_allclose(R, AX - X * lam[None, :], rtol=0, atol=1e-14)
+    sel = np.array([1, 4, 5, 8], dtype=np.int32)
+    dls = d.upload(lam[sel])
+    dRs, dns = B.DeviceBuffer(8 * S * len(sel)), B.DeviceBuffer(8 * 3 * len(sel))
+    B.lib().bdgx_residual_cols.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                           C.c_uint64, C.c_void_p, C.c_int, C.c_void_p,
+                                           C.c_uint64, C.c_void_p]
+    B.check(B.lib().bdgx_residual_cols(d.h, 0, dAX.ptr, dX.ptr, dMX.ptr, S,
+                                       sel.ctypes.data_as(C.c_void_p), len(sel), S, dls.ptr, 0,
+                                       dRs.ptr, S, dns.ptr), "residual_cols")
+    Rs = d.download(dRs, (S, len(sel)), np.float64)
+    ns = d.download(dns, (3 * len(sel),), np.float64)
+    assert np.array_equal(Rs, R[:, sel])
+    assert np.array_equal(ns.reshape(-1, 3), n2.reshape(-1, 3)[sel])
