@@ -103,6 +103,7 @@ struct bdgx_ctx {
   unsigned ws_gen;
   // spectrum bytes per column chunk of an operator apply (0: 6 GiB)
   size_t chunk_bytes;
+  int ncu;  // compute units of the device (persistent launches)
 };
 
 namespace bdgx {

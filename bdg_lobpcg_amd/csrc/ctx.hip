@@ -303,7 +303,10 @@ int bdgx_ctx_create(bdgx_ctx **out, uint64_t dim, const uint64_t *N, const doubl
       delete c;
       return BDGX_EINVAL;
     }
-  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+          hipSuccess ||
+      hipStreamCreate(&c->stream) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
     return BDGX_EHIP;
@@ -824,7 +827,7 @@ int bdgx_get_launch_variant(const bdgx_ctx *c, int kind, int prec, int *variant)
 
 int bdgx_set_launch_variant(bdgx_ctx *c, int kind, int prec, int variant) {
   if (!c || kind < 0 || kind > 4 || (prec != BDGX_FP64 && prec != BDGX_FP32) || variant < 0 ||
-      variant > 1)
+      variant > (kind == BDGX_TUNE_PLANE_FWD ? 3 : 1))
     return BDGX_EINVAL;
   c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind] = variant;
   return BDGX_OK;

@@ -945,7 +945,8 @@ static int autotune_t(bdgx_ctx *c, uint64_t nbatch) {
     };
     int best = c->tune_var[pi][kind];
     float best_ms = 0.f;
-    for (int v = 0; v < 2 && !rc; ++v) {
+    const int nvar = kind == BDGX_TUNE_PLANE_FWD ? 4 : 2;
+    for (int v = 0; v < nvar && !rc; ++v) {
       c->tune_var[pi][kind] = v;
       rc = one();
       if (!rc && hipEventRecord(c->ev0, c->stream) != hipSuccess) rc = BDGX_EHIP;

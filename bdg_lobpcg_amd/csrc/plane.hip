@@ -146,6 +146,57 @@ constexpr int group_rows(int row_len) {
   return p;
 }
 
+// x transform, untangle, y transform of the weighted plane of channel c held
+// in LDS (complex rows of row pairs, swizzled), then the spectrum plane out
+template <typename R, int N, int T>
+__device__ __forceinline__ void plane_fwd_transform(const XArgs<R> &a, cx<R> *buf,
+                                                    const cx<R> *twx, const cx<R> *twy, int c,
+                                                    int col, int z, int tid) {
+  constexpr int NK = N / 2 + 1;
+  constexpr int SW = (int)sizeof(cx<R>);
+  dif_fft<R, false, N, N / 2, N, 1, T, false, SW, 0>(buf, twx, tid);
+  // untangle Z_q (digit-reversed in k) into the half spectra of rows 2q,
+  // 2q+1 (natural kx, row stride NK); rounds of G rows from the top down,
+  // so that no row is overwritten before it is read
+  constexpr int G = group_rows<N, T, 5>(NK);
+  constexpr int NU = G * NK;
+  constexpr int BU = (NU + T - 1) / T;
+  for (int q0 = N / 2 - G; q0 >= 0; q0 -= G) {
+    cx<R> ua[BU], ub[BU];
+#pragma unroll
+    for (int b = 0; b < BU; ++b) {
+      const int t = tid + b * T;
+      if (NU % T == 0 || t < NU) {
+        const int q = q0 + t / NK, k = t % NK;
+        const cx<R> zk = buf[swz<SW>(q * N + dif_pos<N>(k))];
+        const cx<R> zn = buf[swz<SW>(q * N + dif_pos<N>(k == 0 ? 0 : N - k))];
+        ua[b] = cmk<R>(R(0.5) * (zk.re + zn.re), R(0.5) * (zk.im - zn.im));
+        ub[b] = cmk<R>(R(0.5) * (zk.im + zn.im), R(-0.5) * (zk.re - zn.re));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BU; ++b) {
+      const int t = tid + b * T;
+      if (NU % T == 0 || t < NU) {
+        const int q = q0 + t / NK, k = t % NK;
+        buf[(2 * q) * NK + k] = ua[b];
+        buf[(2 * q + 1) * NK + k] = ub[b];
+      }
+    }
+    __syncthreads();
+  }
+  // y transform over the NK adjacent pencils; rows come out digit-reversed
+  dif_fft<R, false, N, NK, 1, NK, T, true, 0, 0>(buf, twy, tid);
+  cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride + (long long)col * a.g.Kk +
+              (long long)z * N * NK;
+  for (int e = tid; e < N * NK; e += T) {
+    const int y = e / NK, k = e % NK;
+    sp[e] = buf[dif_pos<N>(y) * NK + k];
+  }
+  __syncthreads();
+}
+
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
   constexpr int NK = N / 2 + 1;
@@ -183,47 +234,69 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
       buf[swz<SW>(g)] = cmk<R>(w0, w1);
     }
     __syncthreads();
-    dif_fft<R, false, N, N / 2, N, 1, T, false, SW, 0>(buf, twx, tid);
-    // untangle Z_q (digit-reversed in k) into the half spectra of rows 2q,
-    // 2q+1 (natural kx, row stride NK); rounds of G rows from the top down,
-    // so that no row is overwritten before it is read
-    constexpr int G = group_rows<N, T, 5>(NK);
-    constexpr int NU = G * NK;
-    constexpr int BU = (NU + T - 1) / T;
-    for (int q0 = N / 2 - G; q0 >= 0; q0 -= G) {
-      cx<R> ua[BU], ub[BU];
+    plane_fwd_transform<R, N, T>(a, buf, twx, twy, c, col, z, tid);
+  }
+}
+
+// Persistent form: each workgroup takes planes blockIdx.x, + gridDim.x, ...
+// and loads the next plane's values into registers while the current plane
+// is transformed in LDS, so that its HBM reads overlap the LDS work.
+template <typename R, int N, int T>
+__global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes) {
+  constexpr int NK = N / 2 + 1;
+  constexpr int SW = (int)sizeof(cx<R>);
+  constexpr int EP = N * N / 2 / T;  // row-pair values per thread
+  static_assert(EP * T == N * N / 2, "whole row pairs per thread");
+  __shared__ cx<R> twx[N];
+  __shared__ cx<R> twy[N];
+  __shared__ cx<R> buf[N * NK];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += T) {
+    twx[i] = a.tw[i];
+    twy[i] = a.tw_y[i];
+  }
+  const int nz = a.g.nz;
+  R x0[EP], x1[EP];
+  auto load = [&](int p) {
+    const int z = p % nz, col = p / nz;
+    const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S +
+                  (long long)z * N * N;
 #pragma unroll
-      for (int b = 0; b < BU; ++b) {
-        const int t = tid + b * T;
-        if (NU % T == 0 || t < NU) {
-          const int q = q0 + t / NK, k = t % NK;
-          const cx<R> zk = buf[swz<SW>(q * N + dif_pos<N>(k))];
-          const cx<R> zn = buf[swz<SW>(q * N + dif_pos<N>(k == 0 ? 0 : N - k))];
-          ua[b] = cmk<R>(R(0.5) * (zk.re + zn.re), R(0.5) * (zk.im - zn.im));
-          ub[b] = cmk<R>(R(0.5) * (zk.im + zn.im), R(-0.5) * (zk.re - zn.re));
+    for (int b = 0; b < EP; ++b) {
+      const int g = tid + b * T;
+      const int g0 = 2 * (g / N) * N + g % N;
+      x0[b] = xp[g0];
+      x1[b] = xp[g0 + N];
+    }
+  };
+  int p = blockIdx.x;
+  if (p < nplanes) load(p);
+  for (; p < nplanes; p += gridDim.x) {
+    const int z = p % nz, col = p / nz;
+    const long long plane0 = (long long)z * N * N;
+    for (int c = 0; c < a.nch; ++c) {
+      const int pre = a.pre[c];
+      const R *pw = a.prew[c] ? a.prew[c] + plane0 : nullptr;
+#pragma unroll
+      for (int b = 0; b < EP; ++b) {
+        const int g = tid + b * T;
+        const int g0 = 2 * (g / N) * N + g % N;
+        R w0 = x0[b], w1 = x1[b];
+        if (pre == PRE_REAL || pre == PRE_CONJ_PSI) {
+          w0 *= pw[g0];
+          w1 *= pw[g0 + N];
+        } else if (pre == PRE_ABS2) {
+          w0 *= w0;
+          w1 *= w1;
         }
+        buf[swz<SW>(g)] = cmk<R>(w0, w1);
       }
       __syncthreads();
-#pragma unroll
-      for (int b = 0; b < BU; ++b) {
-        const int t = tid + b * T;
-        if (NU % T == 0 || t < NU) {
-          const int q = q0 + t / NK, k = t % NK;
-          buf[(2 * q) * NK + k] = ua[b];
-          buf[(2 * q + 1) * NK + k] = ub[b];
-        }
-      }
-      __syncthreads();
+      // the registers are free after the last channel's LDS write: fetch
+      // the next plane now, its loads in flight during the transform
+      if (c == a.nch - 1 && p + (int)gridDim.x < nplanes) load(p + gridDim.x);
+      plane_fwd_transform<R, N, T>(a, buf, twx, twy, c, col, z, tid);
     }
-    // y transform over the NK adjacent pencils; rows come out digit-reversed
-    dif_fft<R, false, N, NK, 1, NK, T, true, 0, 0>(buf, twy, tid);
-    cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride + (long long)col * a.g.Kk +
-                (long long)z * N * NK;
-    for (int e = tid; e < N * NK; e += T) {
-      const int y = e / NK, k = e % NK;
-      sp[e] = buf[dif_pos<N>(y) * NK + k];
-    }
-    __syncthreads();
   }
 }
 
@@ -514,9 +587,33 @@ bool plane_supported(const bdgx_ctx *c) {
 
 // Launch shapes of the whole-plane kernels: variant 0 = 4N threads,
 // variant 1 = 8N threads (one workgroup per plane either way).
+// persistent forward: as many workgroups as fit on the device at once
+template <typename R, int N, int T>
+static void launch_plane_persist(const bdgx_ctx *c, const XArgs<R> &a, hipStream_t s) {
+  static const int per_cu = [] {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&k_plane_fwd_persist<R, N, T>), T, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    return nb;
+  }();
+  const int nplanes = a.g.nz * a.ncols;
+  int grid = c->ncu * per_cu;
+  if (grid > nplanes) grid = nplanes;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((k_plane_fwd_persist<R, N, T>), dim3(grid), dim3(T), 0, s, a, nplanes);
+}
+
 template <typename R, int N>
-static void launch_plane_v(const XArgs<R> &a, bool inverse, int v, hipStream_t s) {
-  if (v)
+static void launch_plane_v(const bdgx_ctx *c, const XArgs<R> &a, bool inverse, int v,
+                           hipStream_t s) {
+  if (!inverse && v == 2)
+    launch_plane_persist<R, N, 4 * N>(c, a, s);
+  else if (!inverse && v == 3)
+    launch_plane_persist<R, N, 8 * N>(c, a, s);
+  else if (v)
     launch_plane<R, N, 8 * N>(a, inverse, s);
   else
     launch_plane<R, N, 4 * N>(a, inverse, s);
@@ -529,9 +626,9 @@ int launch_plane_x(bdgx_ctx *c, const XArgs<R> &a, bool inverse) {
   const int v = c->tune_var[sizeof(R) == 8 ? 0 : 1][inverse ? BDGX_TUNE_PLANE_INV
                                                               : BDGX_TUNE_PLANE_FWD];
   switch (c->geo.nx) {
-    case 32: launch_plane_v<R, 32>(a, inverse, v, c->stream); break;
-    case 64: launch_plane_v<R, 64>(a, inverse, v, c->stream); break;
-    case 128: launch_plane_v<R, 128>(a, inverse, v, c->stream); break;
+    case 32: launch_plane_v<R, 32>(c, a, inverse, v, c->stream); break;
+    case 64: launch_plane_v<R, 64>(c, a, inverse, v, c->stream); break;
+    case 128: launch_plane_v<R, 128>(c, a, inverse, v, c->stream); break;
     default: return BDGX_EINVAL;
   }
   BDGX_HIP(hipGetLastError());
