@@ -300,10 +300,103 @@ __global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes
   }
 }
 
+// y inverse, repack, x inverse of channel c's spectrum plane held in LDS
+// (rows digit-reversed), then acc += post-weighted real rows
+template <typename R, int N, int T, int EPT>
+__device__ __forceinline__ void plane_inv_transform(const XArgs<R> &a, cx<R> *buf,
+                                                    const cx<R> *twx, const cx<R> *twy, int c,
+                                                    long long plane0, R (&acc0)[EPT],
+                                                    R (&acc1)[EPT], int tid) {
+  constexpr int NK = N / 2 + 1;
+  constexpr int SW = (int)sizeof(cx<R>);
+  dit_fft<R, true, N, NK, 1, NK, T, true, 0, last_stage<N>()>(buf, twy, tid);
+  // repack: complex row q = spectrum(row 2q) + i spectrum(row 2q+1),
+  // extended to all kx by Hermitian symmetry, digit-reversed in x; rounds
+  // of G rows from the bottom up
+  constexpr int G = group_rows<N, T, 8>(N);
+  constexpr int NQ = G * N;
+  constexpr int BQ = (NQ + T - 1) / T;
+  for (int q0 = 0; q0 < N / 2; q0 += G) {
+    cx<R> zq[BQ];
+#pragma unroll
+    for (int b = 0; b < BQ; ++b) {
+      const int t = tid + b * T;
+      if (NQ % T == 0 || t < NQ) {
+        const int q = q0 + t / N, k = t % N;
+        const bool cj = k >= NK;
+        const int kk = cj ? N - k : k;
+        cx<R> A0 = buf[(2 * q) * NK + kk], A1 = buf[(2 * q + 1) * NK + kk];
+        if (cj) {
+          A0.im = -A0.im;
+          A1.im = -A1.im;
+        }
+        if (kk == 0 || kk == N / 2) {
+          A0.im = 0;
+          A1.im = 0;
+        }
+        zq[b] = cmk<R>(A0.re - A1.im, A0.im + A1.re);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BQ; ++b) {
+      const int t = tid + b * T;
+      if (NQ % T == 0 || t < NQ) {
+        const int q = q0 + t / N, k = t % N;
+        buf[swz<SW>(q * N + dif_pos<N>(k))] = zq[b];
+      }
+    }
+    __syncthreads();
+  }
+  dit_fft<R, true, N, N / 2, N, 1, T, false, SW, last_stage<N>()>(buf, twx, tid);
+  const int post = a.post[c];
+  const R ps = a.posts[c];
+  const R *pw = a.postw[c] ? a.postw[c] + plane0 : nullptr;
+#pragma unroll
+  for (int b = 0; b < EPT; ++b) {
+    const int g = tid + b * T;
+    if (N * N / 2 % T == 0 || g < N * N / 2) {
+      const int g0 = 2 * (g / N) * N + g % N;
+      const cx<R> zv = buf[swz<SW>(g)];
+      R v0 = ps * zv.re, v1 = ps * zv.im;
+      if (post != POST_ONE) {
+        v0 *= pw[g0];
+        v1 *= pw[g0 + N];
+      }
+      acc0[b] += v0;
+      acc1[b] += v1;
+    }
+  }
+  __syncthreads();
+}
+
+// Y = acc + lt * X for one plane
+template <typename R, int N, int T, int EPT>
+__device__ __forceinline__ void plane_inv_store(const XArgs<R> &a, int col, long long plane0,
+                                                const R (&acc0)[EPT], const R (&acc1)[EPT],
+                                                int tid) {
+  const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S + plane0;
+  R *yp = a.Y + (long long)col * a.ld + (long long)a.out_half * a.g.S + plane0;
+  const R *lt = a.lt ? a.lt + plane0 : nullptr;
+#pragma unroll
+  for (int b = 0; b < EPT; ++b) {
+    const int g = tid + b * T;
+    if (N * N / 2 % T == 0 || g < N * N / 2) {
+      const int g0 = 2 * (g / N) * N + g % N;
+      R o0 = acc0[b], o1 = acc1[b];
+      if (lt) {
+        o0 += lt[g0] * xp[g0];
+        o1 += lt[g0 + N] * xp[g0 + N];
+      }
+      yp[g0] = o0;
+      yp[g0 + N] = o1;
+    }
+  }
+}
+
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
   constexpr int NK = N / 2 + 1;
-  constexpr int SW = (int)sizeof(cx<R>);
   // output rows 2q and 2q+1 at column i for g = q * N + i, g < N * N / 2
   constexpr int EPT = (N * N / 2 + T - 1) / T;
   __shared__ cx<R> twx[N];
@@ -337,82 +430,93 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
       buf[dif_pos<N>(y) * NK + k] = sp[e];
     }
     __syncthreads();
-    dit_fft<R, true, N, NK, 1, NK, T, true, 0, last_stage<N>()>(buf, twy, tid);
-    // repack: complex row q = spectrum(row 2q) + i spectrum(row 2q+1),
-    // extended to all kx by Hermitian symmetry, digit-reversed in x; rounds
-    // of G rows from the bottom up
-    constexpr int G = group_rows<N, T, 8>(N);
-    constexpr int NQ = G * N;
-    constexpr int BQ = (NQ + T - 1) / T;
-    for (int q0 = 0; q0 < N / 2; q0 += G) {
-      cx<R> zq[BQ];
+    plane_inv_transform<R, N, T, EPT>(a, buf, twx, twy, c, plane0, acc0, acc1, tid);
+  }
+  plane_inv_store<R, N, T, EPT>(a, col, plane0, acc0, acc1, threadIdx.x);
+}
+
+// Persistent form: workgroups take planes blockIdx.x, + gridDim.x, ...; the
+// first channel's spectrum plane of the next plane is loaded into registers
+// while the current plane is transformed, so its HBM reads overlap the LDS
+// work (further channels are read directly).
+template <typename R, int N, int T>
+__global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes) {
+  constexpr int NK = N / 2 + 1;
+  constexpr int EPT = (N * N / 2 + T - 1) / T;
+  constexpr int ES = (N * NK + T - 1) / T;  // spectrum values per thread
+  __shared__ cx<R> twx[N];
+  __shared__ cx<R> twy[N];
+  __shared__ cx<R> buf[N * NK];
+  for (int i = threadIdx.x; i < N; i += T) {
+    twx[i] = a.tw[i];
+    twy[i] = a.tw_y[i];
+  }
+  const int nz = a.g.nz;
+  // next plane's first-channel spectrum, real and imaginary parts apart (a
+  // plain register array: no aggregate to keep in scratch)
+  R sre[ES], sim[ES];
+  int p = blockIdx.x;
+  if (p < nplanes) {
+    const cx<R> *sp = a.spec + (long long)a.chan_base * a.chan_stride +
+                      (long long)(p / nz) * a.g.Kk + (long long)(p % nz) * N * NK;
 #pragma unroll
-      for (int b = 0; b < BQ; ++b) {
-        const int t = tid + b * T;
-        if (NQ % T == 0 || t < NQ) {
-          const int q = q0 + t / N, k = t % N;
-          const bool cj = k >= NK;
-          const int kk = cj ? N - k : k;
-          cx<R> A0 = buf[(2 * q) * NK + kk], A1 = buf[(2 * q + 1) * NK + kk];
-          if (cj) {
-            A0.im = -A0.im;
-            A1.im = -A1.im;
-          }
-          if (kk == 0 || kk == N / 2) {
-            A0.im = 0;
-            A1.im = 0;
-          }
-          zq[b] = cmk<R>(A0.re - A1.im, A0.im + A1.re);
-        }
+    for (int b = 0; b < ES; ++b) {
+      const int e = threadIdx.x + b * T;
+      if (N * NK % T == 0 || e < N * NK) {
+        const cx<R> v = sp[e];
+        sre[b] = v.re;
+        sim[b] = v.im;
       }
-      __syncthreads();
-#pragma unroll
-      for (int b = 0; b < BQ; ++b) {
-        const int t = tid + b * T;
-        if (NQ % T == 0 || t < NQ) {
-          const int q = q0 + t / N, k = t % N;
-          buf[swz<SW>(q * N + dif_pos<N>(k))] = zq[b];
-        }
-      }
-      __syncthreads();
     }
-    dit_fft<R, true, N, N / 2, N, 1, T, false, SW, last_stage<N>()>(buf, twx, tid);
-    const int post = a.post[c];
-    const R ps = a.posts[c];
-    const R *pw = a.postw[c] ? a.postw[c] + plane0 : nullptr;
+  }
+  for (; p < nplanes; p += gridDim.x) {
+    const int z = p % nz, col = p / nz;
+    const long long plane0 = (long long)z * N * N;
+    R acc0[EPT], acc1[EPT];
 #pragma unroll
     for (int b = 0; b < EPT; ++b) {
-      const int g = tid + b * T;
-      if (N * N / 2 % T == 0 || g < N * N / 2) {
-        const int g0 = 2 * (g / N) * N + g % N;
-        const cx<R> zv = buf[swz<SW>(g)];
-        R v0 = ps * zv.re, v1 = ps * zv.im;
-        if (post != POST_ONE) {
-          v0 *= pw[g0];
-          v1 *= pw[g0 + N];
-        }
-        acc0[b] += v0;
-        acc1[b] += v1;
-      }
+      acc0[b] = R(0);
+      acc1[b] = R(0);
     }
-    __syncthreads();
-  }
-  const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S + plane0;
-  R *yp = a.Y + (long long)col * a.ld + (long long)a.out_half * a.g.S + plane0;
-  const R *lt = a.lt ? a.lt + plane0 : nullptr;
+    for (int c = 0; c < a.nch; ++c) {
+      int tid = threadIdx.x;
+      asm volatile("" : "+v"(tid));
+      if (c == 0) {
 #pragma unroll
-  for (int b = 0; b < EPT; ++b) {
-    const int g = threadIdx.x + b * T;
-    if (N * N / 2 % T == 0 || g < N * N / 2) {
-      const int g0 = 2 * (g / N) * N + g % N;
-      R o0 = acc0[b], o1 = acc1[b];
-      if (lt) {
-        o0 += lt[g0] * xp[g0];
-        o1 += lt[g0 + N] * xp[g0 + N];
+        for (int b = 0; b < ES; ++b) {
+          const int e = tid + b * T;
+          if (N * NK % T == 0 || e < N * NK) {
+            const int y = e / NK, k = e % NK;
+            buf[dif_pos<N>(y) * NK + k] = cmk<R>(sre[b], sim[b]);
+          }
+        }
+      } else {
+        const cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride +
+                          (long long)col * a.g.Kk + (long long)z * N * NK;
+        for (int e = tid; e < N * NK; e += T) {
+          const int y = e / NK, k = e % NK;
+          buf[dif_pos<N>(y) * NK + k] = sp[e];
+        }
       }
-      yp[g0] = o0;
-      yp[g0 + N] = o1;
+      __syncthreads();
+      // the registers are free after the first channel's LDS write
+      const int pn = p + (int)gridDim.x;
+      if (c == 0 && pn < nplanes) {
+        const cx<R> *sp = a.spec + (long long)a.chan_base * a.chan_stride +
+                          (long long)(pn / nz) * a.g.Kk + (long long)(pn % nz) * N * NK;
+#pragma unroll
+        for (int b = 0; b < ES; ++b) {
+          const int e = threadIdx.x + b * T;
+          if (N * NK % T == 0 || e < N * NK) {
+            const cx<R> v = sp[e];
+            sre[b] = v.re;
+            sim[b] = v.im;
+          }
+        }
+      }
+      plane_inv_transform<R, N, T, EPT>(a, buf, twx, twy, c, plane0, acc0, acc1, tid);
     }
+    plane_inv_store<R, N, T, EPT>(a, col, plane0, acc0, acc1, threadIdx.x);
   }
 }
 
@@ -587,15 +691,14 @@ bool plane_supported(const bdgx_ctx *c) {
 
 // Launch shapes of the whole-plane kernels: variant 0 = 4N threads,
 // variant 1 = 8N threads (one workgroup per plane either way).
-// persistent forward: as many workgroups as fit on the device at once
-template <typename R, int N, int T>
+// persistent kernels: as many workgroups as fit on the device at once
+template <typename R, int N, int T, bool INV>
 static void launch_plane_persist(const bdgx_ctx *c, const XArgs<R> &a, hipStream_t s) {
   static const int per_cu = [] {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&k_plane_fwd_persist<R, N, T>), T, 0) !=
-            hipSuccess ||
-        nb < 1)
+    const void *fn = INV ? reinterpret_cast<const void *>(&k_plane_inv_persist<R, N, T>)
+                         : reinterpret_cast<const void *>(&k_plane_fwd_persist<R, N, T>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, T, 0) != hipSuccess || nb < 1)
       nb = 1;
     return nb;
   }();
@@ -603,16 +706,24 @@ static void launch_plane_persist(const bdgx_ctx *c, const XArgs<R> &a, hipStream
   int grid = c->ncu * per_cu;
   if (grid > nplanes) grid = nplanes;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((k_plane_fwd_persist<R, N, T>), dim3(grid), dim3(T), 0, s, a, nplanes);
+  if (INV)
+    hipLaunchKernelGGL((k_plane_inv_persist<R, N, T>), dim3(grid), dim3(T), 0, s, a, nplanes);
+  else
+    hipLaunchKernelGGL((k_plane_fwd_persist<R, N, T>), dim3(grid), dim3(T), 0, s, a, nplanes);
 }
 
+// variants: 0 = 4N threads, 1 = 8N threads, 2 / 3 = persistent with 4N / 8N
 template <typename R, int N>
 static void launch_plane_v(const bdgx_ctx *c, const XArgs<R> &a, bool inverse, int v,
                            hipStream_t s) {
-  if (!inverse && v == 2)
-    launch_plane_persist<R, N, 4 * N>(c, a, s);
-  else if (!inverse && v == 3)
-    launch_plane_persist<R, N, 8 * N>(c, a, s);
+  if (v == 2 && inverse)
+    launch_plane_persist<R, N, 4 * N, true>(c, a, s);
+  else if (v == 3 && inverse)
+    launch_plane_persist<R, N, 8 * N, true>(c, a, s);
+  else if (v == 2)
+    launch_plane_persist<R, N, 4 * N, false>(c, a, s);
+  else if (v == 3)
+    launch_plane_persist<R, N, 8 * N, false>(c, a, s);
   else if (v)
     launch_plane<R, N, 8 * N>(a, inverse, s);
   else

@@ -111,7 +111,7 @@ def test_launch_variants_and_autotune(grid):
     refs = {op: c.apply(op, X) for op in ("K", "M", "precondK")}
     for prec in (0, 1):
         for kind in range(3):
-            for v in ((0, 1, 2, 3) if kind == 0 else (0, 1)):  # forward: + persistent forms
+            for v in ((0, 1, 2, 3) if kind <= 1 else (0, 1)):  # planes: + persistent forms
                 d.set_launch_variant(kind, prec, v)
                 for op, ref in refs.items():
                     e = relerr(d.apply(op, X, prec), ref)
@@ -133,8 +133,8 @@ def test_launch_variants_and_autotune(grid):
             e = relerr(G, ref)
             assert e < 1e-12, f"gram variant {v} prec {prec}: {e:.2e}"
         chosen = d.autotune(prec, 8, force=True)
-        assert chosen["plane_fwd"] in (0, 1, 2, 3)
-        assert all(v in (0, 1) for k, v in chosen.items() if k != "plane_fwd")
+        assert chosen["plane_fwd"] in (0, 1, 2, 3) and chosen["plane_inv"] in (0, 1, 2, 3)
+        assert all(v in (0, 1) for k, v in chosen.items() if not k.startswith("plane"))
         for op, ref in refs.items():
             assert relerr(d.apply(op, X, prec), ref) < TOL[prec]
 
