@@ -178,10 +178,14 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def error_string(rc: int) -> str:
+    """Text of a library status code."""
+    return lib().bdgx_error_string(rc).decode() if rc > -100 else "rayleigh-ritz failure"
+
+
 def check(rc: int, what: str = "") -> None:
     if rc != 0:
-        msg = lib().bdgx_error_string(rc).decode() if rc > -100 else "rayleigh-ritz failure"
-        raise BdgError(f"{what}: error {rc} ({msg})")
+        raise BdgError(f"{what}: error {rc} ({error_string(rc)})")
 
 
 # ---------------------------------------------------------------- device --
@@ -567,15 +571,20 @@ class Problem:
     def nev_arr(self, ptr, n, dt=np.float64):
         return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt).copy()
 
+    @staticmethod
+    def _result(p, nev, what):
+        if not p:
+            raise RuntimeError(f"{what}: no results (the last solve did not finish)")
+        return np.ctypeslib.as_array(p, shape=(nev,))
+
     def eigenvalues(self, nev) -> np.ndarray:
-        p = lib().bdg_eigenvalues(self.h)
-        return np.ctypeslib.as_array(p, shape=(nev,)).copy()
+        return self._result(lib().bdg_eigenvalues(self.h), nev, "eigenvalues").copy()
 
     def residuals(self, nev) -> np.ndarray:
-        return np.ctypeslib.as_array(lib().bdg_residuals(self.h), shape=(nev,)).copy()
+        return self._result(lib().bdg_residuals(self.h), nev, "residuals").copy()
 
     def zero_modes(self, nev) -> np.ndarray:
-        return np.ctypeslib.as_array(lib().bdg_zero_modes(self.h), shape=(nev,)).astype(bool)
+        return self._result(lib().bdg_zero_modes(self.h), nev, "zero_modes").astype(bool)
 
     def modes(self, nev):
         dt = np.complex128 if self.cplx else np.float64

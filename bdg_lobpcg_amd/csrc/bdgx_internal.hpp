@@ -15,7 +15,7 @@
     }                                                    \
   } while (0)
 
-extern hipError_t bdgx_last_hip_error;
+extern thread_local hipError_t bdgx_last_hip_error;
 
 namespace bdgx {
 

@@ -10,7 +10,7 @@
 #include "fft_core.hpp"
 #include "plane.hpp"
 
-hipError_t bdgx_last_hip_error = hipSuccess;
+thread_local hipError_t bdgx_last_hip_error = hipSuccess;
 
 namespace bdgx {
 int apply_dispatch(bdgx_ctx *c, int op, int prec, const void *dX, void *dY,
@@ -91,7 +91,8 @@ static int upload_conv(bdgx_ctx *c, R **dst, const double *src, size_t n) {
   if (!*dst) BDGX_HIP(hipMalloc((void **)dst, sizeof(R) * (n ? n : 1)));
   std::vector<R> tmp(n);
   for (size_t i = 0; i < n; ++i) tmp[i] = (R)src[i];
-  BDGX_HIP(hipMemcpy(*dst, tmp.data(), sizeof(R) * n, hipMemcpyHostToDevice));
+  BDGX_HIP(hipMemcpyAsync(*dst, tmp.data(), sizeof(R) * n, hipMemcpyHostToDevice, c->stream));
+  BDGX_HIP(hipStreamSynchronize(c->stream));
   return BDGX_OK;
 }
 
@@ -306,7 +307,7 @@ int bdgx_ctx_create(bdgx_ctx **out, uint64_t dim, const uint64_t *N, const doubl
   if (hipGetDevice(&c->device) != hipSuccess ||
       hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
           hipSuccess ||
-      hipStreamCreate(&c->stream) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
     return BDGX_EHIP;

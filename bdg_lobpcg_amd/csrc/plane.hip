@@ -521,29 +521,27 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
 }
 
 // ---------------------------------------------------------------------------
-// Fused z pass with compile-time Nz: P adjacent kx pencils per workgroup held
-// element-major in LDS (element z of pencil p at z*P + p).  The first forward
-// stage reads its butterfly inputs straight from the spectrum in HBM (P
-// adjacent kx per z: one contiguous segment), the channel's k-space factor is
+// Fused z pass with compile-time Nz: P consecutive (y, kx) pencils per
+// workgroup held element-major in LDS (element z of pencil p at z*P + p).  The
+// first forward stage reads its butterfly inputs straight from the spectrum in
+// HBM (one contiguous segment per z), the channel's k-space factor is
 // applied as the last forward stage stores, and the last inverse stage writes
 // straight back to HBM.
 template <typename R>
 struct ZMul {
   const YZArgs<R> *a;
-  int code, kx0, y0;
-  bool ystep;  // pencil p at (kx0, y0 + p) instead of (kx0 + p, y0)
+  int code, f0, valid;
+  const R *kxy2;  // kx^2 + ky^2 of pencil p (LDS)
   __device__ __forceinline__ cx<R> operator()(cx<R> v, int z, int p) const {
-    const int kx = ystep ? kx0 : kx0 + p;
-    const int y = ystep ? y0 + p : y0;
-    if (kx >= a->g.n0k || y >= a->g.ny) return v;
-    const R k2 = a->kx2[kx] + a->ky2[y] + a->kz2[z];
+    if (p >= valid) return v;
+    const R k2 = kxy2[p] + a->kz2[z];
     R f;
     switch (code) {
       case MUL_KIN:
         f = k2 * (R(0.5) * a->invS);
         break;
       case MUL_DIP:
-        f = a->U[((long long)z * a->g.ny + y) * a->g.n0k + kx] * a->invS;
+        f = a->U[(long long)z * a->g.ny * a->g.n0k + f0 + p] * a->invS;
         break;
       case MUL_PRE: {
         R d = a->mu + R(0.5) * k2;
@@ -556,7 +554,7 @@ struct ZMul {
         break;
       case MUL_TABK:
       case MUL_TABM:
-        f = a->tab[code - MUL_TABK][((long long)z * a->g.ny + y) * a->g.n0k + kx];
+        f = a->tab[code - MUL_TABK][(long long)z * a->g.ny * a->g.n0k + f0 + p];
         break;
       default:
         f = a->invS;
@@ -565,15 +563,15 @@ struct ZMul {
   }
 };
 
-// Workgroups of a batch: ny * (n0k / P) tiles of P adjacent kx at one y,
-// then, when n0k = P * full + 1 (the r2c Nyquist column kx = n0k - 1), that
-// column in tiles of P adjacent y; otherwise a last partial kx tile per y.
+// Workgroups of a batch: the ny * n0k (y, kx) pencils of one spectrum, taken
+// in memory order (flat index f = y * n0k + kx), P consecutive pencils per
+// workgroup.  A z row of a workgroup is then P consecutive elements starting
+// at a multiple of P: one whole aligned 128-byte line when ny * n0k is a
+// multiple of P, so no line is split between two workgroups (which may run
+// on different XCDs, each fetching the line into its own L2).
 template <int P>
 __host__ __device__ __forceinline__ int z_tiles_per_batch(int n0k, int ny) {
-  const int full = n0k / P, rem = n0k % P;
-  if (rem == 0) return full * ny;
-  if (rem == 1) return full * ny + (ny + P - 1) / P;
-  return (full + 1) * ny;
+  return (n0k * ny + P - 1) / P;
 }
 
 template <typename R, int NZ, int P, int T>
@@ -581,32 +579,23 @@ __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
   __shared__ cx<R> tw[NZ];
   __shared__ cx<R> bufA[NZ * P];
   __shared__ cx<R> bufB[NZ * P];
+  __shared__ R kxy2[P];
   const int ny = a.g.ny, n0k = a.g.n0k;
+  const int npen = ny * n0k;
   const int per_b = z_tiles_per_batch<P>(n0k, ny);
   const int b = blockIdx.x / per_b;
-  const int r = blockIdx.x % per_b;
-  const int full = n0k / P;
-  const bool nyq = n0k % P == 1 && r >= full * ny;
-  int kx0, y0, valid;
-  long long pstride;
-  if (nyq) {  // Nyquist column, P adjacent y
-    kx0 = n0k - 1;
-    y0 = (r - full * ny) * P;
-    valid = ny - y0 < P ? ny - y0 : P;
-    pstride = n0k;
-  } else {
-    const int ntile = n0k % P == 1 ? full : (n0k + P - 1) / P;
-    kx0 = (r % ntile) * P;
-    y0 = r / ntile;
-    valid = n0k - kx0 < P ? n0k - kx0 : P;
-    pstride = 1;
-  }
+  const int f0 = (blockIdx.x % per_b) * P;
+  const int valid = npen - f0 < P ? npen - f0 : P;
   for (int i = threadIdx.x; i < NZ; i += T) tw[i] = a.tw[i];
+  if (threadIdx.x < P) {
+    const int f = f0 + (int)threadIdx.x < npen ? f0 + (int)threadIdx.x : npen - 1;
+    const int y = f / n0k;
+    kxy2[threadIdx.x] = a.kx2[f - y * n0k] + a.ky2[y];
+  }
   __syncthreads();
-  cx<R> *base = a.spec + (long long)b * a.g.Kk + (long long)y0 * n0k + kx0;
-  const long long zs = (long long)n0k * ny;
-  GlobIO<R> g{base, zs, valid, pstride};
-  ZMul<R> mul{&a, a.mul[b / a.ncols], kx0, y0, nyq};
+  cx<R> *base = a.spec + (long long)b * a.g.Kk + f0;
+  GlobIO<R> g{base, (long long)npen, valid, 1};
+  ZMul<R> mul{&a, a.mul[b / a.ncols], f0, valid, kxy2};
   constexpr int nst = make_fixed_plan(NZ).nst;
   cx<R> *F = (nst % 2 == 1) ? bufA : bufB;  // last LDS buffer of the forward
   cx<R> *O = (F == bufA) ? bufB : bufA;

@@ -76,12 +76,14 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None):
 PASSES = ["xfwd", "yfwd", "z", "yinv", "xinv", "plane_fwd", "plane_inv"]
 def kernel_name(pname, cfg, variants):
     """Name of the FP64 kernel that runs pass `pname` under the tuned launch
-    variants (plane: 4N or 8N threads; z: 256 or 128 threads at Nz = 128)."""
+    variants (plane: 0/1 one plane per workgroup with 4N/8N threads, 2/3 the
+    persistent kernel with 4N/8N threads; z: 256 or 128 threads at Nz = 128)."""
     nx, nz = cfg["N"][0], cfg["N"][2]
     v = variants.get(pname, 0)
     if pname in ("plane_fwd", "plane_inv"):
-        k = "k_plane_fwd_real" if pname == "plane_fwd" else "k_plane_inv_real"
-        return f"{k}<double, {nx}, {(8 if v else 4) * nx}>"
+        d = "fwd" if pname == "plane_fwd" else "inv"
+        k = f"k_plane_{d}_persist" if v >= 2 else f"k_plane_{d}_real"
+        return f"{k}<double, {nx}, {(8 if v & 1 else 4) * nx}>"
     if pname == "z":
         t = (256 if nz == 128 else 128) >> v
         return f"k_zpass_fixed<double, {nz}, 8, {t}>"
@@ -238,6 +240,10 @@ def batch(nprob=16, precision=1, nthreads=0):
     t0 = time.perf_counter()
     rc_bat = B.Problem.solve_batch(probs, nthreads)
     t_bat = time.perf_counter() - t0
+    if any(rc_seq) or any(rc_bat):
+        log(f"batch: solve codes sequential {rc_seq}, concurrent {rc_bat}: "
+            + "; ".join(sorted({B.error_string(r) for r in rc_seq + rc_bat if r})))
+        sys.exit(1)
     same = all(np.array_equal(p.eigenvalues(c["nev"]), w) for p, c, w in zip(probs, cfgs, w_seq))
     for p in probs:
         p.close()
