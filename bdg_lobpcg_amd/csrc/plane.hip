@@ -197,13 +197,37 @@ __device__ __forceinline__ void plane_fwd_transform(const XArgs<R> &a, cx<R> *bu
   __syncthreads();
 }
 
+// Register budget of the one-plane kernels: FP32 planes at N = 128 (67 KB
+// of LDS) run two inverse workgroups per CU when each 4N-thread workgroup
+// keeps to 128 VGPRs (4 waves per SIMD), with loads grouped by 2 instead of
+// 4.
+template <typename R, int N, int T>
+constexpr bool plane_two_per_cu() {
+  return sizeof(R) == 4 && N == 128 && T == 4 * N;
+}
+template <typename R, int N, int T>
+constexpr int plane_waves_per_simd() {
+  return plane_two_per_cu<R, N, T>() ? 4 : 1;
+}
+template <typename R, int N, int T>
+constexpr int plane_load_group() {
+  return plane_two_per_cu<R, N, T>() ? 2 : 4;
+}
+
+// One plane per workgroup, nothing held across the transform (fewer
+// registers: more workgroups per CU where the LDS allows it).  Each channel
+// reads its input values in groups of LG per thread, a group's loads all in
+// flight together.
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
-  constexpr int NK = N / 2 + 1;
   constexpr int SW = (int)sizeof(cx<R>);
+  constexpr int EP = N * N / 2 / T;  // row-pair values per thread
+  static_assert(EP * T == N * N / 2, "whole row pairs per thread");
+  constexpr int LG = EP < plane_load_group<R, N, T>() ? EP : plane_load_group<R, N, T>();
+  static_assert(EP % LG == 0, "load groups");
   __shared__ cx<R> twx[N];
   __shared__ cx<R> twy[N];
-  __shared__ cx<R> buf[N * NK];
+  __shared__ cx<R> buf[N * (N / 2 + 1)];
   const int nz = a.g.nz;
   const int z = blockIdx.x % nz;
   const int col = blockIdx.x / nz;
@@ -214,33 +238,54 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
   const long long plane0 = (long long)z * N * N;
   const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S + plane0;
   for (int c = 0; c < a.nch; ++c) {
-    // (the compiler hoists the swizzled LDS addresses of the body out of
-    // this loop; measured faster here than recomputing them, see the inverse)
     const int tid = threadIdx.x;
     const int pre = a.pre[c];
-    const R *pw = a.prew[c] ? a.prew[c] + plane0 : nullptr;
-    // rows 2q, 2q+1 of the weighted plane as one complex row q (natural order)
-    for (int g = tid; g < N * N / 2; g += T) {
-      const int q = g / N, i = g % N;
-      const int g0 = 2 * q * N + i;
-      R w0 = xp[g0], w1 = xp[g0 + N];
-      if (pre == PRE_REAL || pre == PRE_CONJ_PSI) {
-        w0 *= pw[g0];
-        w1 *= pw[g0 + N];
-      } else if (pre == PRE_ABS2) {
-        w0 *= w0;
-        w1 *= w1;
+    const bool weighted = pre == PRE_REAL || pre == PRE_CONJ_PSI;
+    const R *pw = a.prew[c] + plane0;
+#pragma unroll
+    for (int b0 = 0; b0 < EP; b0 += LG) {
+      R x0[LG], x1[LG], q0[LG], q1[LG];
+#pragma unroll
+      for (int i = 0; i < LG; ++i) {
+        const int g = tid + (b0 + i) * T;
+        const int g0 = 2 * (g / N) * N + g % N;
+        x0[i] = xp[g0];
+        x1[i] = xp[g0 + N];
       }
-      buf[swz<SW>(g)] = cmk<R>(w0, w1);
+      if (weighted) {
+#pragma unroll
+        for (int i = 0; i < LG; ++i) {
+          const int g = tid + (b0 + i) * T;
+          const int g0 = 2 * (g / N) * N + g % N;
+          q0[i] = pw[g0];
+          q1[i] = pw[g0 + N];
+        }
+      } else if (pre == PRE_ABS2) {
+#pragma unroll
+        for (int i = 0; i < LG; ++i) {
+          q0[i] = x0[i];
+          q1[i] = x1[i];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < LG; ++i) {
+          q0[i] = R(1);
+          q1[i] = R(1);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < LG; ++i)
+        buf[swz<SW>(tid + (b0 + i) * T)] = cmk<R>(x0[i] * q0[i], x1[i] * q1[i]);
     }
     __syncthreads();
     plane_fwd_transform<R, N, T>(a, buf, twx, twy, c, col, z, tid);
   }
 }
 
-// Persistent form: each workgroup takes planes blockIdx.x, + gridDim.x, ...
-// and loads the next plane's values into registers while the current plane
-// is transformed in LDS, so that its HBM reads overlap the LDS work.
+// Persistent forward kernel: each workgroup takes planes blockIdx.x,
+// + gridDim.x, ... and loads the next plane's values into registers while the
+// current plane is transformed in LDS, so that its HBM reads overlap the LDS
+// work.
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes) {
   constexpr int NK = N / 2 + 1;
@@ -276,20 +321,26 @@ __global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes
     const long long plane0 = (long long)z * N * N;
     for (int c = 0; c < a.nch; ++c) {
       const int pre = a.pre[c];
-      const R *pw = a.prew[c] ? a.prew[c] + plane0 : nullptr;
+      if (pre == PRE_REAL || pre == PRE_CONJ_PSI) {
+        // all weights requested before the first is used: one HBM round
+        // trip per channel, not one per value
+        const R *pw = a.prew[c] + plane0;
+        R q0[EP], q1[EP];
 #pragma unroll
-      for (int b = 0; b < EP; ++b) {
-        const int g = tid + b * T;
-        const int g0 = 2 * (g / N) * N + g % N;
-        R w0 = x0[b], w1 = x1[b];
-        if (pre == PRE_REAL || pre == PRE_CONJ_PSI) {
-          w0 *= pw[g0];
-          w1 *= pw[g0 + N];
-        } else if (pre == PRE_ABS2) {
-          w0 *= w0;
-          w1 *= w1;
+        for (int b = 0; b < EP; ++b) {
+          const int g = tid + b * T;
+          const int g0 = 2 * (g / N) * N + g % N;
+          q0[b] = pw[g0];
+          q1[b] = pw[g0 + N];
         }
-        buf[swz<SW>(g)] = cmk<R>(w0, w1);
+#pragma unroll
+        for (int b = 0; b < EP; ++b) buf[swz<SW>(tid + b * T)] = cmk<R>(x0[b] * q0[b], x1[b] * q1[b]);
+      } else if (pre == PRE_ABS2) {
+#pragma unroll
+        for (int b = 0; b < EP; ++b) buf[swz<SW>(tid + b * T)] = cmk<R>(x0[b] * x0[b], x1[b] * x1[b]);
+      } else {
+#pragma unroll
+        for (int b = 0; b < EP; ++b) buf[swz<SW>(tid + b * T)] = cmk<R>(x0[b], x1[b]);
       }
       __syncthreads();
       // the registers are free after the last channel's LDS write: fetch
@@ -302,7 +353,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes
 
 // y inverse, repack, x inverse of channel c's spectrum plane held in LDS
 // (rows digit-reversed), then acc += post-weighted real rows
-template <typename R, int N, int T, int EPT>
+template <typename R, int N, int T, int EPT, int PGMAX>
 __device__ __forceinline__ void plane_inv_transform(const XArgs<R> &a, cx<R> *buf,
                                                     const cx<R> *twx, const cx<R> *twy, int c,
                                                     long long plane0, R (&acc0)[EPT],
@@ -349,56 +400,124 @@ __device__ __forceinline__ void plane_inv_transform(const XArgs<R> &a, cx<R> *bu
     __syncthreads();
   }
   dit_fft<R, true, N, N / 2, N, 1, T, false, SW, last_stage<N>()>(buf, twx, tid);
-  const int post = a.post[c];
+  const bool weighted = a.post[c] != POST_ONE;
   const R ps = a.posts[c];
-  const R *pw = a.postw[c] ? a.postw[c] + plane0 : nullptr;
+  const R *pw = a.postw[c] + plane0;
+  if constexpr (PGMAX > 1) {
+    // weights requested in groups of PG values per thread, a group's loads
+    // all in flight before the first is used
+    constexpr int PG = EPT < PGMAX ? EPT : PGMAX;
+    static_assert(EPT % PG == 0, "weight groups");
 #pragma unroll
-  for (int b = 0; b < EPT; ++b) {
-    const int g = tid + b * T;
-    if (N * N / 2 % T == 0 || g < N * N / 2) {
-      const int g0 = 2 * (g / N) * N + g % N;
-      const cx<R> zv = buf[swz<SW>(g)];
-      R v0 = ps * zv.re, v1 = ps * zv.im;
-      if (post != POST_ONE) {
-        v0 *= pw[g0];
-        v1 *= pw[g0 + N];
+    for (int b0 = 0; b0 < EPT; b0 += PG) {
+      R q0[PG], q1[PG];
+      if (weighted) {
+#pragma unroll
+        for (int i = 0; i < PG; ++i) {
+          const int g = tid + (b0 + i) * T;
+          const int g0 = 2 * (g / N) * N + g % N;
+          const bool in = N * N / 2 % T == 0 || g < N * N / 2;
+          q0[i] = ps * pw[in ? g0 : 0];
+          q1[i] = ps * pw[in ? g0 + N : 0];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < PG; ++i) {
+          q0[i] = ps;
+          q1[i] = ps;
+        }
       }
-      acc0[b] += v0;
-      acc1[b] += v1;
+#pragma unroll
+      for (int i = 0; i < PG; ++i) {
+        const int g = tid + (b0 + i) * T;
+        if (N * N / 2 % T == 0 || g < N * N / 2) {
+          const cx<R> zv = buf[swz<SW>(g)];
+          acc0[b0 + i] += zv.re * q0[i];
+          acc1[b0 + i] += zv.im * q1[i];
+        }
+      }
+    }
+  } else {  // one weight at a time
+#pragma unroll
+    for (int b = 0; b < EPT; ++b) {
+      const int g = tid + b * T;
+      if (N * N / 2 % T == 0 || g < N * N / 2) {
+        const int g0 = 2 * (g / N) * N + g % N;
+        const cx<R> zv = buf[swz<SW>(g)];
+        R v0 = ps * zv.re, v1 = ps * zv.im;
+        if (weighted) {
+          v0 *= pw[g0];
+          v1 *= pw[g0 + N];
+        }
+        acc0[b] += v0;
+        acc1[b] += v1;
+      }
     }
   }
   __syncthreads();
 }
 
-// Y = acc + lt * X for one plane
-template <typename R, int N, int T, int EPT>
+// Y = acc + lt * X for one plane; lt and X are read in groups of SG values
+// per thread, all loads of a group in flight together (Y may alias X, so
+// the loads of a group are issued before its stores)
+template <typename R, int N, int T, int EPT, int SGMAX>
 __device__ __forceinline__ void plane_inv_store(const XArgs<R> &a, int col, long long plane0,
                                                 const R (&acc0)[EPT], const R (&acc1)[EPT],
                                                 int tid) {
+  constexpr int SG = EPT < SGMAX ? EPT : SGMAX;
+  static_assert(EPT % SG == 0, "store groups");
   const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S + plane0;
   R *yp = a.Y + (long long)col * a.ld + (long long)a.out_half * a.g.S + plane0;
   const R *lt = a.lt ? a.lt + plane0 : nullptr;
 #pragma unroll
-  for (int b = 0; b < EPT; ++b) {
-    const int g = tid + b * T;
-    if (N * N / 2 % T == 0 || g < N * N / 2) {
-      const int g0 = 2 * (g / N) * N + g % N;
-      R o0 = acc0[b], o1 = acc1[b];
-      if (lt) {
-        o0 += lt[g0] * xp[g0];
-        o1 += lt[g0 + N] * xp[g0 + N];
+  for (int b0 = 0; b0 < EPT; b0 += SG) {
+    R o0[SG], o1[SG];
+    if (lt) {
+      R l0[SG], l1[SG], u0[SG], u1[SG];
+#pragma unroll
+      for (int i = 0; i < SG; ++i) {
+        const int g = tid + (b0 + i) * T;
+        if (N * N / 2 % T == 0 || g < N * N / 2) {
+          const int g0 = 2 * (g / N) * N + g % N;
+          l0[i] = lt[g0];
+          l1[i] = lt[g0 + N];
+          u0[i] = xp[g0];
+          u1[i] = xp[g0 + N];
+        }
       }
-      yp[g0] = o0;
-      yp[g0 + N] = o1;
+#pragma unroll
+      for (int i = 0; i < SG; ++i) {
+        o0[i] = acc0[b0 + i] + l0[i] * u0[i];
+        o1[i] = acc1[b0 + i] + l1[i] * u1[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < SG; ++i) {
+        o0[i] = acc0[b0 + i];
+        o1[i] = acc1[b0 + i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SG; ++i) {
+      const int g = tid + (b0 + i) * T;
+      if (N * N / 2 % T == 0 || g < N * N / 2) {
+        const int g0 = 2 * (g / N) * N + g % N;
+        yp[g0] = o0[i];
+        yp[g0 + N] = o1[i];
+      }
     }
   }
 }
 
+// One plane per workgroup, nothing held across the transform but the
+// output accumulators; each channel's spectrum plane is read in groups of
+// LG values per thread, a group's loads all in flight together.
 template <typename R, int N, int T>
-__global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
+__global__ void __launch_bounds__(T, (plane_waves_per_simd<R, N, T>())) k_plane_inv_real(XArgs<R> a) {
   constexpr int NK = N / 2 + 1;
-  // output rows 2q and 2q+1 at column i for g = q * N + i, g < N * N / 2
   constexpr int EPT = (N * N / 2 + T - 1) / T;
+  constexpr int ES = (N * NK + T - 1) / T;  // spectrum values per thread
+  constexpr int LG = plane_load_group<R, N, T>();
   __shared__ cx<R> twx[N];
   __shared__ cx<R> twy[N];
   __shared__ cx<R> buf[N * NK];
@@ -419,31 +538,50 @@ __global__ void __launch_bounds__(T) k_plane_inv_real(XArgs<R> a) {
   for (int c = 0; c < a.nch; ++c) {
     // thread index opaque to the compiler inside the channel loop: keeps it
     // from hoisting every swizzled LDS address of the body out of a loop that
-    // usually runs once or twice (register spills at 1024 threads)
+    // usually runs once or twice
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride +
                       (long long)col * a.g.Kk + (long long)z * N * NK;
     // rows into digit-reversed order for the DIT y inverse
-    for (int e = tid; e < N * NK; e += T) {
-      const int y = e / NK, k = e % NK;
-      buf[dif_pos<N>(y) * NK + k] = sp[e];
+#pragma unroll
+    for (int b0 = 0; b0 < ES; b0 += LG) {
+      cx<R> v[LG];
+#pragma unroll
+      for (int i = 0; i < LG; ++i) {
+        const int e = tid + (b0 + i) * T;
+        if (b0 + i < ES && (N * NK % T == 0 || e < N * NK)) v[i] = sp[e];
+      }
+#pragma unroll
+      for (int i = 0; i < LG; ++i) {
+        const int e = tid + (b0 + i) * T;
+        if (b0 + i < ES && (N * NK % T == 0 || e < N * NK)) {
+          const int y = e / NK, k = e % NK;
+          buf[dif_pos<N>(y) * NK + k] = v[i];
+        }
+      }
     }
     __syncthreads();
-    plane_inv_transform<R, N, T, EPT>(a, buf, twx, twy, c, plane0, acc0, acc1, tid);
+    plane_inv_transform<R, N, T, EPT, LG>(a, buf, twx, twy, c, plane0, acc0, acc1, tid);
   }
-  plane_inv_store<R, N, T, EPT>(a, col, plane0, acc0, acc1, threadIdx.x);
+  plane_inv_store<R, N, T, EPT, LG>(a, col, plane0, acc0, acc1, threadIdx.x);
 }
 
-// Persistent form: workgroups take planes blockIdx.x, + gridDim.x, ...; the
-// first channel's spectrum plane of the next plane is loaded into registers
-// while the current plane is transformed, so its HBM reads overlap the LDS
-// work (further channels are read directly).
+// Persistent inverse kernel: workgroups take planes blockIdx.x, + gridDim.x,
+// ...  A channel's spectrum plane passes through registers on its way into
+// LDS (all of a thread's loads in flight at once); the next plane's first
+// channel is loaded while the last channel of the current plane is
+// transformed, so those HBM reads overlap the LDS work.
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes) {
   constexpr int NK = N / 2 + 1;
   constexpr int EPT = (N * N / 2 + T - 1) / T;
   constexpr int ES = (N * NK + T - 1) / T;  // spectrum values per thread
+  // post-weight and store loads in groups of WG / SG per thread (FP64: no
+  // registers to spare for weight groups next to the accumulators and the
+  // next plane's spectrum)
+  constexpr int WG = sizeof(R) == 8 ? 1 : 4;
+  constexpr int SG = 4;
   __shared__ cx<R> twx[N];
   __shared__ cx<R> twy[N];
   __shared__ cx<R> buf[N * NK];
@@ -481,27 +619,31 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
     for (int c = 0; c < a.nch; ++c) {
       int tid = threadIdx.x;
       asm volatile("" : "+v"(tid));
-      if (c == 0) {
+      if (c > 0) {  // this channel's spectrum plane into the (free) registers
+        const cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride +
+                          (long long)col * a.g.Kk + (long long)z * N * NK;
 #pragma unroll
         for (int b = 0; b < ES; ++b) {
           const int e = tid + b * T;
           if (N * NK % T == 0 || e < N * NK) {
-            const int y = e / NK, k = e % NK;
-            buf[dif_pos<N>(y) * NK + k] = cmk<R>(sre[b], sim[b]);
+            const cx<R> v = sp[e];
+            sre[b] = v.re;
+            sim[b] = v.im;
           }
         }
-      } else {
-        const cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride +
-                          (long long)col * a.g.Kk + (long long)z * N * NK;
-        for (int e = tid; e < N * NK; e += T) {
+      }
+#pragma unroll
+      for (int b = 0; b < ES; ++b) {
+        const int e = tid + b * T;
+        if (N * NK % T == 0 || e < N * NK) {
           const int y = e / NK, k = e % NK;
-          buf[dif_pos<N>(y) * NK + k] = sp[e];
+          buf[dif_pos<N>(y) * NK + k] = cmk<R>(sre[b], sim[b]);
         }
       }
       __syncthreads();
-      // the registers are free after the first channel's LDS write
+      // the registers are free after the last channel's LDS write
       const int pn = p + (int)gridDim.x;
-      if (c == 0 && pn < nplanes) {
+      if (c == a.nch - 1 && pn < nplanes) {
         const cx<R> *sp = a.spec + (long long)a.chan_base * a.chan_stride +
                           (long long)(pn / nz) * a.g.Kk + (long long)(pn % nz) * N * NK;
 #pragma unroll
@@ -514,9 +656,9 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
           }
         }
       }
-      plane_inv_transform<R, N, T, EPT>(a, buf, twx, twy, c, plane0, acc0, acc1, tid);
+      plane_inv_transform<R, N, T, EPT, WG>(a, buf, twx, twy, c, plane0, acc0, acc1, tid);
     }
-    plane_inv_store<R, N, T, EPT>(a, col, plane0, acc0, acc1, threadIdx.x);
+    plane_inv_store<R, N, T, EPT, SG>(a, col, plane0, acc0, acc1, threadIdx.x);
   }
 }
 
@@ -530,18 +672,16 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
 template <typename R>
 struct ZMul {
   const YZArgs<R> *a;
-  int code, f0, valid;
+  int code, valid;
   const R *kxy2;  // kx^2 + ky^2 of pencil p (LDS)
+  const R *kz2;   // kz^2 (LDS)
   __device__ __forceinline__ cx<R> operator()(cx<R> v, int z, int p) const {
     if (p >= valid) return v;
-    const R k2 = kxy2[p] + a->kz2[z];
+    const R k2 = kxy2[p] + kz2[z];
     R f;
     switch (code) {
       case MUL_KIN:
         f = k2 * (R(0.5) * a->invS);
-        break;
-      case MUL_DIP:
-        f = a->U[(long long)z * a->g.ny * a->g.n0k + f0 + p] * a->invS;
         break;
       case MUL_PRE: {
         R d = a->mu + R(0.5) * k2;
@@ -552,16 +692,22 @@ struct ZMul {
       case MUL_ITPKIN:
         f = R(exp(-(double)a->dt * 0.5 * (double)k2)) * a->invS;
         break;
+      case MUL_DIP:
       case MUL_TABK:
       case MUL_TABM:
-        f = a->tab[code - MUL_TABK][(long long)z * a->g.ny * a->g.n0k + f0 + p];
-        break;
+        return v;  // tabulated factors: applied after the forward transform
       default:
         f = a->invS;
     }
     return cscale(v, f);
   }
 };
+
+// channels whose k-space factor is a table in HBM (the dipolar kernel, the
+// learned tables)
+__host__ __device__ __forceinline__ bool mul_is_table(int code) {
+  return code == MUL_DIP || code == MUL_TABK || code == MUL_TABM;
+}
 
 // Workgroups of a batch: the ny * n0k (y, kx) pencils of one spectrum, taken
 // in memory order (flat index f = y * n0k + kx), P consecutive pencils per
@@ -580,13 +726,34 @@ __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
   __shared__ cx<R> bufA[NZ * P];
   __shared__ cx<R> bufB[NZ * P];
   __shared__ R kxy2[P];
+  __shared__ R kz2[NZ];
+  constexpr int EU = NZ * P / T;  // tabulated factors per thread
+  static_assert(EU * T == NZ * P, "whole tile per thread set");
   const int ny = a.g.ny, n0k = a.g.n0k;
   const int npen = ny * n0k;
   const int per_b = z_tiles_per_batch<P>(n0k, ny);
   const int b = blockIdx.x / per_b;
   const int f0 = (blockIdx.x % per_b) * P;
   const int valid = npen - f0 < P ? npen - f0 : P;
-  for (int i = threadIdx.x; i < NZ; i += T) tw[i] = a.tw[i];
+  const int code = a.fwd_only ? MUL_ONE : a.mul[b / a.ncols];
+  // a table factor of element e = z * P + p of the tile, e = tid + q * T,
+  // requested now and applied once the forward transform is done
+  R uf[EU];
+  if (mul_is_table(code)) {
+    const R *src = code == MUL_DIP ? a.U : a.tab[code - MUL_TABK];
+    const R sc = code == MUL_DIP ? a.invS : R(1);
+#pragma unroll
+    for (int q = 0; q < EU; ++q) {
+      const int e = threadIdx.x + q * T;
+      const int z = e / P, pp = e % P;
+      const R u = src[(long long)z * npen + f0 + (pp < valid ? pp : 0)];
+      uf[q] = pp < valid ? u * sc : R(0);
+    }
+  }
+  for (int i = threadIdx.x; i < NZ; i += T) {
+    tw[i] = a.tw[i];
+    kz2[i] = a.kz2[i];
+  }
   if (threadIdx.x < P) {
     const int f = f0 + (int)threadIdx.x < npen ? f0 + (int)threadIdx.x : npen - 1;
     const int y = f / n0k;
@@ -595,7 +762,7 @@ __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
   __syncthreads();
   cx<R> *base = a.spec + (long long)b * a.g.Kk + f0;
   GlobIO<R> g{base, (long long)npen, valid, 1};
-  ZMul<R> mul{&a, a.mul[b / a.ncols], f0, valid, kxy2};
+  ZMul<R> mul{&a, code, valid, kxy2, kz2};
   constexpr int nst = make_fixed_plan(NZ).nst;
   cx<R> *F = (nst % 2 == 1) ? bufA : bufB;  // last LDS buffer of the forward
   cx<R> *O = (F == bufA) ? bufB : bufA;
@@ -603,16 +770,25 @@ __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
     fixed_stages<R, false, NZ, P, T, 0>(g, g, bufA, bufB, tw, NoMul{});
     return;
   }
-  if constexpr (nst == 1) {
+  // forward: global -> A -> B ... ; the last stage stores into the buffer
+  // it does not read (F)
+  if constexpr (nst == 1)
     fixed_stages<R, false, NZ, P, T, 0>(g, LdsIO<R>{bufA}, bufA, bufB, tw, mul);
-    fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{bufA}, g, bufB, bufA, tw, NoMul{});
-  } else {
-    // forward: global -> A -> B ... ; the last stage stores into the buffer
-    // it does not read
+  else
     fixed_stages<R, false, NZ, P, T, 0>(g, LdsIO<R>{F}, bufA, bufB, tw, mul);
-    // inverse: F -> O -> F ... -> global
-    fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{F}, g, O, F, tw, NoMul{});
+  if (mul_is_table(code)) {
+#pragma unroll
+    for (int q = 0; q < EU; ++q) {
+      const int e = threadIdx.x + q * T;
+      F[e] = cscale(F[e], uf[q]);
+    }
+    __syncthreads();
   }
+  // inverse: F -> O -> F ... -> global
+  if constexpr (nst == 1)
+    fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{bufA}, g, bufB, bufA, tw, NoMul{});
+  else
+    fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{F}, g, O, F, tw, NoMul{});
 }
 
 template <typename R>
@@ -678,9 +854,10 @@ bool plane_supported(const bdgx_ctx *c) {
   return g.nx == 32 || g.nx == 64 || g.nx == 128;
 }
 
-// Launch shapes of the whole-plane kernels: variant 0 = 4N threads,
-// variant 1 = 8N threads (one workgroup per plane either way).
-// persistent kernels: as many workgroups as fit on the device at once
+// Launch shapes of the whole-plane kernels: variants 0 / 1 = the one-plane
+// kernels with 4N / 8N threads, 2 / 3 = the persistent kernels with as many
+// workgroups as fit on the device at once (each then walks several planes),
+// 4N / 8N threads.
 template <typename R, int N, int T, bool INV>
 static void launch_plane_persist(const bdgx_ctx *c, const XArgs<R> &a, hipStream_t s) {
   static const int per_cu = [] {
@@ -736,13 +913,15 @@ int launch_plane_x(bdgx_ctx *c, const XArgs<R> &a, bool inverse) {
 }
 
 // Measured defaults (MI355X, 32 columns): at N = 128 FP64 (one 133 KB
-// plane per CU) runs faster with 8N threads, FP32 (two 67 KB planes per CU)
-// with 4N; 8N at N = 32, 64.  The z pass default is variant 0.
+// plane per CU) the persistent kernels with 4N threads; FP32 (two 67 KB
+// planes per CU) the persistent forward with 8N threads and the inverse with
+// one plane per 4N-thread workgroup; 8N threads, one plane per workgroup, at
+// N = 32, 64.  The z pass default is variant 0.
 void tune_defaults(bdgx_ctx *c) {
   for (int p = 0; p < 2; ++p) {
-    const int big = c->geo.nx == 128 ? (p == 0) : 1;
-    c->tune_var[p][BDGX_TUNE_PLANE_FWD] = big;
-    c->tune_var[p][BDGX_TUNE_PLANE_INV] = big;
+    const bool n128 = c->geo.nx == 128;
+    c->tune_var[p][BDGX_TUNE_PLANE_FWD] = n128 ? (p == 0 ? 2 : 3) : 1;
+    c->tune_var[p][BDGX_TUNE_PLANE_INV] = n128 ? (p == 0 ? 2 : 0) : 1;
     c->tune_var[p][BDGX_TUNE_Z] = 0;
     c->tune_var[p][BDGX_TUNE_COMBINE] = 0;
     c->tune_var[p][BDGX_TUNE_GRAM] = 0;

@@ -76,7 +76,7 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None):
 PASSES = ["xfwd", "yfwd", "z", "yinv", "xinv", "plane_fwd", "plane_inv"]
 def kernel_name(pname, cfg, variants):
     """Name of the FP64 kernel that runs pass `pname` under the tuned launch
-    variants (plane: 0/1 one plane per workgroup with 4N/8N threads, 2/3 the
+    variants (plane: 0/1 the one-plane kernel with 4N/8N threads, 2/3 the
     persistent kernel with 4N/8N threads; z: 256 or 128 threads at Nz = 128)."""
     nx, nz = cfg["N"][0], cfg["N"][2]
     v = variants.get(pname, 0)
