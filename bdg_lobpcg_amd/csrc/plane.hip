@@ -190,9 +190,33 @@ __device__ __forceinline__ void plane_fwd_transform(const XArgs<R> &a, cx<R> *bu
   dif_fft<R, false, N, NK, 1, NK, T, true, 0, 0>(buf, twy, tid);
   cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride + (long long)col * a.g.Kk +
               (long long)z * N * NK;
-  for (int e = tid; e < N * NK; e += T) {
-    const int y = e / NK, k = e % NK;
-    sp[e] = buf[dif_pos<N>(y) * NK + k];
+  if constexpr (sizeof(R) == 8 || T > 4 * N) {  // (no registers to spare)
+    for (int e = tid; e < N * NK; e += T) {
+      const int y = e / NK, k = e % NK;
+      sp[e] = buf[dif_pos<N>(y) * NK + k];
+    }
+    __syncthreads();
+    return;
+  }
+  // LDS reads in groups of SG per thread ahead of their stores
+  constexpr int ES = (N * NK + T - 1) / T;
+  constexpr int SG = 4;
+#pragma unroll
+  for (int b0 = 0; b0 < ES; b0 += SG) {
+    cx<R> v[SG];
+#pragma unroll
+    for (int i = 0; i < SG; ++i) {
+      const int b = b0 + i, e = tid + b * T;
+      if (b < ES && (N * NK % T == 0 || b < ES - 1 || e < N * NK)) {
+        const int y = e / NK, k = e % NK;
+        v[i] = buf[dif_pos<N>(y) * NK + k];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SG; ++i) {
+      const int b = b0 + i, e = tid + b * T;
+      if (b < ES && (N * NK % T == 0 || b < ES - 1 || e < N * NK)) sp[e] = v[i];
+    }
   }
   __syncthreads();
 }
@@ -550,12 +574,12 @@ __global__ void __launch_bounds__(T, (plane_waves_per_simd<R, N, T>())) k_plane_
 #pragma unroll
       for (int i = 0; i < LG; ++i) {
         const int e = tid + (b0 + i) * T;
-        if (b0 + i < ES && (N * NK % T == 0 || e < N * NK)) v[i] = sp[e];
+        if (b0 + i < ES && (N * NK % T == 0 || b0 + i < ES - 1 || e < N * NK)) v[i] = sp[e];
       }
 #pragma unroll
       for (int i = 0; i < LG; ++i) {
         const int e = tid + (b0 + i) * T;
-        if (b0 + i < ES && (N * NK % T == 0 || e < N * NK)) {
+        if (b0 + i < ES && (N * NK % T == 0 || b0 + i < ES - 1 || e < N * NK)) {
           const int y = e / NK, k = e % NK;
           buf[dif_pos<N>(y) * NK + k] = v[i];
         }
@@ -600,7 +624,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
 #pragma unroll
     for (int b = 0; b < ES; ++b) {
       const int e = threadIdx.x + b * T;
-      if (N * NK % T == 0 || e < N * NK) {
+      if (N * NK % T == 0 || b < ES - 1 || e < N * NK) {
         const cx<R> v = sp[e];
         sre[b] = v.re;
         sim[b] = v.im;
@@ -625,7 +649,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
 #pragma unroll
         for (int b = 0; b < ES; ++b) {
           const int e = tid + b * T;
-          if (N * NK % T == 0 || e < N * NK) {
+          if (N * NK % T == 0 || b < ES - 1 || e < N * NK) {
             const cx<R> v = sp[e];
             sre[b] = v.re;
             sim[b] = v.im;
@@ -635,7 +659,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
 #pragma unroll
       for (int b = 0; b < ES; ++b) {
         const int e = tid + b * T;
-        if (N * NK % T == 0 || e < N * NK) {
+        if (N * NK % T == 0 || b < ES - 1 || e < N * NK) {
           const int y = e / NK, k = e % NK;
           buf[dif_pos<N>(y) * NK + k] = cmk<R>(sre[b], sim[b]);
         }
@@ -649,7 +673,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
 #pragma unroll
         for (int b = 0; b < ES; ++b) {
           const int e = threadIdx.x + b * T;
-          if (N * NK % T == 0 || e < N * NK) {
+          if (N * NK % T == 0 || b < ES - 1 || e < N * NK) {
             const cx<R> v = sp[e];
             sre[b] = v.re;
             sim[b] = v.im;
