@@ -527,6 +527,154 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
   }
 }
 
+// RT consecutive rows of column `col` with no branch: the caller clamps
+// col and row to valid ones (a clamped column meets a zero row of C, a
+// clamped row only feeds output rows that are not stored)
+template <typename R, int RT>
+__device__ __forceinline__ void load_rows_nb(const R *A, long long lda, int col, long long row,
+                                             double out[RT]) {
+  const R *p = A + row + (long long)col * lda;
+  if constexpr (sizeof(R) == 8 && RT == 4) {
+    const double2 x = reinterpret_cast<const double2 *>(p)[0];
+    const double2 y = reinterpret_cast<const double2 *>(p)[1];
+    out[0] = x.x;
+    out[1] = x.y;
+    out[2] = y.x;
+    out[3] = y.y;
+  } else if constexpr (sizeof(R) == 8) {
+    const double2 x = reinterpret_cast<const double2 *>(p)[0];
+    out[0] = x.x;
+    out[1] = x.y;
+  } else if constexpr (RT == 4) {
+    const float4 x = *reinterpret_cast<const float4 *>(p);
+    out[0] = x.x;
+    out[1] = x.y;
+    out[2] = x.z;
+    out[3] = x.w;
+  } else {
+    const float2 x = *reinterpret_cast<const float2 *>(p);
+    out[0] = x.x;
+    out[1] = x.y;
+  }
+}
+
+// Persistent form of k_combine_mfma: the block's whole column tile of C
+// (k rounded up to 4 rows, zero-padded) is staged in LDS once, then the
+// block walks row tiles blockIdx.x, + gridDim.x, ...  The k loop runs in
+// groups of PD steps of 4 with PD steps of A in flight; every A load is
+// unconditional (clamped), so the loads of a step are awaited exactly.
+// Same MFMA sequence per output element as k_combine_mfma.  Needs
+// vector-aligned A and Y and nrows a multiple of RT.
+template <typename R, int MF, int RT, int PD>
+__global__ void __launch_bounds__(256) k_combine_pers(const R *A, long long lda, int k,
+                                                      const double *C, R *Y, long long ldy,
+                                                      int m, long long nrows, long long ntiles) {
+  extern __shared__ double cs[];
+  constexpr int CW = 16 * MF + 1;
+  const int kpad = (k + 3) & ~3;
+  const int nsteps = kpad >> 2;
+  const int j0 = blockIdx.y * 16 * MF;
+  for (int e = threadIdx.x; e < kpad * 16 * MF; e += 256) {
+    const int i = e % kpad, j = e / kpad;
+    cs[i * CW + j] = (i < k && j0 + j < m) ? C[i + (long long)(j0 + j) * k] : 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, kk = lane >> 4;
+  const long long rmax = nrows - RT;
+  // when the steps of a tile are a whole number of PD groups, the refills
+  // past a tile's last step fetch the next tile's first steps (into the
+  // buffers those steps use), otherwise they re-read a clamped column
+  const bool xt = nsteps % PD == 0;
+  auto lane_row = [&](long long t) {
+    const long long r = (t * 4 + wave) * (16 * RT) + RT * c;
+    return r < rmax ? r : rmax;
+  };
+  double av[PD][RT];
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long r0 = (tile * 4 + wave) * (16 * RT);
+    if (tile == blockIdx.x || !xt) {
+      const long long rl = lane_row(tile);
+#pragma unroll
+      for (int d = 0; d < PD; ++d) {
+        const int col = 4 * d + kk;
+        load_rows_nb<R, RT>(A, lda, col < k ? col : k - 1, rl, av[d]);
+      }
+    }
+    v4d acc[RT][MF];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int f = 0; f < MF; ++f) acc[t][f] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int s0 = 0; s0 < nsteps; s0 += PD) {
+#pragma unroll
+      for (int d = 0; d < PD; ++d) {
+        const int s = s0 + d;
+        if (s < nsteps) {
+          double bv[MF];
+#pragma unroll
+          for (int f = 0; f < MF; ++f) bv[f] = cs[(4 * s + kk) * CW + 16 * f + c];
+#pragma unroll
+          for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int f = 0; f < MF; ++f)
+              acc[t][f] =
+                  __builtin_amdgcn_mfma_f64_16x16x4f64(av[d][t], bv[f], acc[t][f], 0, 0, 0);
+        }
+        // refill this buffer with step s + PD
+        int sn = s + PD;
+        long long tn = tile;
+        if (xt && sn >= nsteps) {
+          sn -= nsteps;
+          tn += gridDim.x;
+        }
+        const int col = 4 * sn + kk;
+        load_rows_nb<R, RT>(A, lda, col < k ? col : k - 1, lane_row(tn), av[d]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < MF; ++f) {
+      const int col = j0 + 16 * f + c;
+      if (col >= m) continue;
+      R *yc = Y + (long long)col * ldy;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long long rr = r0 + RT * (kk + 4 * v);
+        if (rr >= nrows) continue;
+        if constexpr (sizeof(R) == 8 && RT == 4) {
+          reinterpret_cast<double2 *>(yc + rr)[0] = double2{acc[0][f][v], acc[1][f][v]};
+          reinterpret_cast<double2 *>(yc + rr)[1] = double2{acc[2][f][v], acc[3][f][v]};
+        } else if constexpr (sizeof(R) == 8) {
+          reinterpret_cast<double2 *>(yc + rr)[0] = double2{acc[0][f][v], acc[1][f][v]};
+        } else if constexpr (RT == 4) {
+          *reinterpret_cast<float4 *>(yc + rr) = float4{(float)acc[0][f][v], (float)acc[1][f][v],
+                                                        (float)acc[2][f][v], (float)acc[3][f][v]};
+        } else {
+          *reinterpret_cast<float2 *>(yc + rr) = float2{(float)acc[0][f][v], (float)acc[1][f][v]};
+        }
+      }
+    }
+  }
+}
+
+template <typename R, int MF, int RT, int PD>
+static void launch_combine_pers(const bdgx_ctx *c, unsigned gy, size_t lds, hipStream_t s,
+                                const R *A, long long lda, int k, const double *C, R *Y,
+                                long long ldy, int m, long long nrows) {
+  const long long ntiles = (nrows + 64 * RT - 1) / (64 * RT);
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, reinterpret_cast<const void *>(&k_combine_pers<R, MF, RT, PD>), 256, lds) !=
+          hipSuccess ||
+      nb < 1)
+    nb = 1;
+  long long gx = (long long)c->ncu * nb;
+  if (gx > ntiles) gx = ntiles;
+  hipLaunchKernelGGL((k_combine_pers<R, MF, RT, PD>), dim3((unsigned)gx, gy), dim3(256), lds, s,
+                     A, lda, k, C, Y, ldy, m, nrows, ntiles);
+}
+
 template <typename R, int MF>
 static void launch_combine(int rt, dim3 grid, hipStream_t s, const R *A, long long lda, int k,
                            const double *C, R *Y, long long ldy, int m, long long nrows,
@@ -540,19 +688,40 @@ static void launch_combine(int rt, dim3 grid, hipStream_t s, const R *A, long lo
 }
 
 // Launch variants (autotuned): 0 = 4 rows per lane (64 per wave), 1 = 2 rows
-// per lane (half the accumulators: more waves in flight per SIMD).
+// per lane (half the accumulators: more waves in flight per SIMD), 2 / 3 =
+// the persistent kernel with 2 rows per lane and 8 k steps in flight / 4
+// rows per lane and 4 steps (C tile in LDS once per block; used where it
+// fits in 64 KB of LDS and A, Y are vector-aligned, else as variant 1 / 0).
 template <typename R>
 static int combine_real_mfma(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k,
                              const double *dC, void *dY, uint64_t ldy, uint64_t m,
                              uint64_t nrows) {
   const int MF = m <= 16 ? 1 : (m <= 32 ? 2 : (m <= 48 ? 3 : 4));
-  const int rt = c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_COMBINE] ? 2 : 4;
+  const int var = c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_COMBINE];
+  const int rt = (var == 1 || var == 2) ? 2 : 4;
   const unsigned gx = (unsigned)((nrows + 64 * rt - 1) / (64 * rt));
   const unsigned gy = (unsigned)((m + 16 * MF - 1) / (16 * MF));
   const int vec_a = (lda % 4 == 0) && ((uintptr_t)dA % (4 * sizeof(R)) == 0);
   const int vec_y = (ldy % 4 == 0) && ((uintptr_t)dY % (4 * sizeof(R)) == 0);
   const R *A = reinterpret_cast<const R *>(dA);
   R *Y = reinterpret_cast<R *>(dY);
+  const size_t lds = sizeof(double) * (size_t)((k + 3) & ~3ull) * (size_t)(16 * MF + 1);
+  if (var >= 2 && vec_a && vec_y && nrows % 4 == 0 && lds <= 64 * 1024) {
+    const int K = (int)k, M = (int)m;
+    const long long la = (long long)lda, ly = (long long)ldy, nr = (long long)nrows;
+    switch (MF * 2 + (var == 3)) {
+      case 2: launch_combine_pers<R, 1, 2, 8>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+      case 3: launch_combine_pers<R, 1, 4, 4>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+      case 4: launch_combine_pers<R, 2, 2, 8>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+      case 5: launch_combine_pers<R, 2, 4, 4>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+      case 6: launch_combine_pers<R, 3, 2, 8>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+      case 7: launch_combine_pers<R, 3, 4, 4>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+      case 8: launch_combine_pers<R, 4, 2, 8>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+      default: launch_combine_pers<R, 4, 4, 4>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+    }
+    BDGX_HIP(hipGetLastError());
+    return BDGX_OK;
+  }
   const dim3 grid(gx, gy);
   switch (MF) {
     case 1: launch_combine<R, 1>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,

@@ -197,7 +197,8 @@ def blas(cfg, iters=10):
         by = S * (k + m) * 8
         fl = 2.0 * S * k * m
         out.append(dict(kernel="combine", shape=f"{k}->{m}",
-                        rows_per_lane=2 if variants["combine"] else 4, us=ms * 1e3,
+                        variant=variants["combine"], rows_per_lane=2 if variants["combine"] in (1, 2) else 4,
+                        persistent=variants["combine"] >= 2, us=ms * 1e3,
                         GBs=by / ms / 1e6,
                         TFs=fl / ms / 1e9, mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS,
                         rel_err=err))

@@ -164,10 +164,14 @@ int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iter
 /* Launch-shape autotuner (no reference counterpart; it takes the place of
  * the reference's FFTW_PATIENT planning, src/matmul_ctx.c:134-171).  The
  * fixed-size FFT kernels (whole-plane forward / inverse, fixed z pass) and
- * the block combine have launch variants: plane 0 = 4 Nx threads, 1 = 8 Nx
- * threads per plane; z 0 = 256 (Nz = 128) or 128 threads, 1 = half of that;
- * combine 0 = 4 rows per lane, 1 = 2 rows per lane; real Gram 0 = tiles of
- * up to 96 rows of p, 1 = up to 48.  bdgx_autotune times every variant of
+ * the block combine have launch variants: plane 0 / 1 = one workgroup per
+ * plane with 4 Nx / 8 Nx threads, 2 / 3 = as many workgroups as fit on the
+ * device, each walking several planes, with 4 Nx / 8 Nx threads; z 0 = 256
+ * (Nz = 128) or 128 threads, 1 = half of that; combine 0 = 4 rows per lane,
+ * 1 = 2 rows per lane, 2 / 3 = persistent workgroups that stage their tile
+ * of C in LDS once (2 / 4 rows per lane; where it fits in 64 KB and the
+ * blocks are vector-aligned, else as 1 / 0); real Gram 0 = tiles of up to 96
+ * rows of p, 1 = up to 48.  bdgx_autotune times every variant of
  * every kind for `prec` on `nbatch` columns (kinetic batches for the FFT
  * kinds, a 3 nbatch -> 2 nbatch combine and a 3 nbatch x nbatch Gram, the
  * solver's main shapes; 1 warm-up + 3 timed

@@ -119,10 +119,22 @@ def test_launch_variants_and_autotune(grid):
         rng = np.random.default_rng(3)
         A = rng.standard_normal((c.size, 20))
         Cm = rng.standard_normal((20, 36))
-        for v in (0, 1):
-            d.set_launch_variant(3, prec, v)  # block combine
+        for v in (0, 1, 2, 3):  # block combine; 2, 3 persistent
+            d.set_launch_variant(3, prec, v)
             e = relerr(d.combine(A, Cm, prec), A @ Cm)
             assert e < (1e-14 if prec == 0 else 1e-6), f"combine variant {v} prec {prec}: {e:.2e}"
+        # the solver's main shape (k = 96: whole groups of in-flight steps, so
+        # the persistent kernels prefetch across row tiles): identical bits
+        A96 = rng.standard_normal((c.size, 96))
+        C96 = rng.standard_normal((96, 64))
+        outs = []
+        for v in (0, 1, 2, 3):
+            d.set_launch_variant(3, prec, v)
+            outs.append(d.combine(A96, C96, prec))
+        e = relerr(outs[0], A96 @ C96)
+        assert e < (1e-14 if prec == 0 else 1e-6), f"combine 96 -> 64 prec {prec}: {e:.2e}"
+        for v in (1, 2, 3):
+            assert np.array_equal(outs[v], outs[0]), f"combine variant {v} differs from 0"
         Ag = rng.standard_normal((c.size, 90))
         Bg = rng.standard_normal((c.size, 20))
         for v in (0, 1):
@@ -134,7 +146,8 @@ def test_launch_variants_and_autotune(grid):
             assert e < 1e-12, f"gram variant {v} prec {prec}: {e:.2e}"
         chosen = d.autotune(prec, 8, force=True)
         assert chosen["plane_fwd"] in (0, 1, 2, 3) and chosen["plane_inv"] in (0, 1, 2, 3)
-        assert all(v in (0, 1) for k, v in chosen.items() if not k.startswith("plane"))
+        assert chosen["combine"] in (0, 1, 2, 3)
+        assert chosen["z"] in (0, 1) and chosen["gram"] in (0, 1)
         for op, ref in refs.items():
             assert relerr(d.apply(op, X, prec), ref) < TOL[prec]
 
