@@ -80,6 +80,63 @@ __device__ __forceinline__ void load4(const R *base, long long ld, long long row
   }
 }
 
+// Four waves' accumulator tiles summed in LDS (waves 2, 3 -> 0, 1, then
+// wave 1 -> 0, a fixed order), then wave 0 writes the block's partial tile.
+template <typename R, int FP, int FQ, bool CPLX>
+__device__ __forceinline__ void gram_epilogue(const GramMArgs<R> &a, double *red,
+                                              v4d (&acc)[FP * FQ * (CPLX ? 2 : 1)], int p0,
+                                              int q0) {
+  constexpr int NACC = FP * FQ * (CPLX ? 2 : 1);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, ks = lane >> 4;
+  if (wave >= 2) {
+    double *dst = red + (size_t)(wave - 2) * NACC * 256;
+#pragma unroll
+    for (int u = 0; u < NACC; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dst[(u * 4 + v) * 64 + lane] = acc[u][v];
+  }
+  __syncthreads();
+  if (wave < 2) {
+    const double *src = red + (size_t)wave * NACC * 256;
+#pragma unroll
+    for (int u = 0; u < NACC; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] += src[(u * 4 + v) * 64 + lane];
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int u = 0; u < NACC; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[(u * 4 + v) * 64 + lane] = acc[u][v];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int u = 0; u < NACC; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] += red[(u * 4 + v) * 64 + lane];
+  constexpr int W = CPLX ? 2 : 1;
+  double *out = a.part + (long long)blockIdx.x * a.p * a.q * W;
+  // accumulator element v of lane (c, ks) holds D[ks + 4*v][c]
+#pragma unroll
+  for (int f = 0; f < FP; ++f)
+#pragma unroll
+    for (int g = 0; g < FQ; ++g) {
+      const int j = q0 + 16 * g + c;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = p0 + 16 * f + ks + 4 * v;
+        if (i < a.p && j < a.q) {
+          const long long o = ((long long)i + (long long)j * a.p) * W;
+          out[o] = acc[f * FQ + g][v];
+          if constexpr (CPLX) out[o + 1] = acc[FP * FQ + f * FQ + g][v];
+        }
+      }
+    }
+}
+
 template <typename R, int FP, int FQ, bool CPLX>
 __global__ void __launch_bounds__(256) k_gram_mfma(GramMArgs<R> a) {
   constexpr int NACC = FP * FQ * (CPLX ? 2 : 1);
@@ -142,53 +199,127 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramMArgs<R> a) {
         for (int t = 0; t < 4; ++t) bv[g][t] = bn[g][t];
     }
   }
-  // sum the four waves' tiles in LDS: waves 2, 3 -> 0, 1, then wave 1 -> 0
-  if (wave >= 2) {
-    double *dst = red + (size_t)(wave - 2) * NACC * 256;
+  gram_epilogue<R, FP, FQ, CPLX>(a, red, acc, p0, q0);
+}
+
+// Real Gram with a software-pipelined row loop (A and B vector-aligned, B
+// not swapped): PD steps of 16 rows in flight per wave, every load of a full
+// step unconditional (columns past p or q are clamped to the last one: they
+// only feed output entries that are never written), so each step's operands
+// are awaited exactly; scheduling barriers keep a step's loads ahead of the
+// previous step's MFMAs.  Same MFMA sequence per output entry as
+// k_gram_mfma; a last partial step (rows past the wave's range zeroed) goes
+// through load4.
+template <typename R, int FP, int FQ, int PD>
+__global__ void __launch_bounds__(256) k_gram_pipe(GramMArgs<R> a) {
+  constexpr int NACC = FP * FQ;
+  extern __shared__ double red[];  // 2 * NACC * 256 doubles
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, ks = lane >> 4;
+  const int tp = blockIdx.y % a.tiles_p, tq = blockIdx.y / a.tiles_p;
+  const int p0 = tp * 16 * FP, q0 = tq * 16 * FQ;
+  const long long wg_r0 = (long long)blockIdx.x * a.rows_per_wg;
+  const long long rpw = a.rows_per_wg >> 2;
+  const long long r_lo = wg_r0 + wave * rpw;
+  long long r_hi = r_lo + rpw;
+  if (r_hi > a.nrows) r_hi = a.nrows;
+  v4d acc[NACC];
 #pragma unroll
-    for (int u = 0; u < NACC; ++u)
+  for (int u = 0; u < NACC; ++u) acc[u] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int nfull = r_hi > r_lo ? (int)((r_hi - r_lo) >> 4) : 0;
+  const R *pa[FP];
+  const R *pb[FQ];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) dst[(u * 4 + v) * 64 + lane] = acc[u][v];
+  for (int f = 0; f < FP; ++f) {
+    const int col = p0 + 16 * f + c;
+    pa[f] = a.A + (long long)(col < a.p ? col : a.p - 1) * a.lda + r_lo + 4 * ks;
   }
-  __syncthreads();
-  if (wave < 2) {
-    const double *src = red + (size_t)wave * NACC * 256;
 #pragma unroll
-    for (int u = 0; u < NACC; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] += src[(u * 4 + v) * 64 + lane];
+  for (int g = 0; g < FQ; ++g) {
+    const int col = q0 + 16 * g + c;
+    pb[g] = a.B + (long long)(col < a.q ? col : a.q - 1) * a.ldb + r_lo + 4 * ks;
   }
-  __syncthreads();
-  if (wave == 1) {
+  auto ld = [](const R *p, double out[4]) {
+    if constexpr (sizeof(R) == 8) {
+      const double2 x = reinterpret_cast<const double2 *>(p)[0];
+      const double2 y = reinterpret_cast<const double2 *>(p)[1];
+      out[0] = x.x;
+      out[1] = x.y;
+      out[2] = y.x;
+      out[3] = y.y;
+    } else {
+      const float4 x = *reinterpret_cast<const float4 *>(p);
+      out[0] = x.x;
+      out[1] = x.y;
+      out[2] = x.z;
+      out[3] = x.w;
+    }
+  };
+  if (nfull > 0) {
+    double av[PD][FP][4], bv[PD][FQ][4];
 #pragma unroll
-    for (int u = 0; u < NACC; ++u)
+    for (int d = 0; d < PD - 1; ++d) {
+      const long long off = 16LL * (d < nfull ? d : nfull - 1);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) red[(u * 4 + v) * 64 + lane] = acc[u][v];
-  }
-  __syncthreads();
-  if (wave != 0) return;
+      for (int f = 0; f < FP; ++f) ld(pa[f] + off, av[d][f]);
 #pragma unroll
-  for (int u = 0; u < NACC; ++u)
+      for (int g = 0; g < FQ; ++g) ld(pb[g] + off, bv[d][g]);
+    }
+    int s0 = 0;
+    for (; s0 + PD <= nfull; s0 += PD) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] += red[(u * 4 + v) * 64 + lane];
-  constexpr int W = CPLX ? 2 : 1;
-  double *out = a.part + (long long)blockIdx.x * a.p * a.q * W;
-  // accumulator element v of lane (c, ks) holds D[ks + 4*v][c]
+      for (int d = 0; d < PD; ++d) {
+        const int s = s0 + d;
+        const int dn = (d + PD - 1) % PD;
+        const int sn = s + PD - 1 < nfull ? s + PD - 1 : nfull - 1;
+        const long long off = 16LL * sn;
 #pragma unroll
-  for (int f = 0; f < FP; ++f)
+        for (int f = 0; f < FP; ++f) ld(pa[f] + off, av[dn][f]);
 #pragma unroll
-    for (int g = 0; g < FQ; ++g) {
-      const int j = q0 + 16 * g + c;
+        for (int g = 0; g < FQ; ++g) ld(pb[g] + off, bv[dn][g]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = p0 + 16 * f + ks + 4 * v;
-        if (i < a.p && j < a.q) {
-          const long long o = ((long long)i + (long long)j * a.p) * W;
-          out[o] = acc[f * FQ + g][v];
-          if constexpr (CPLX) out[o + 1] = acc[FP * FQ + f * FQ + g][v];
-        }
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int f = 0; f < FP; ++f)
+#pragma unroll
+            for (int g = 0; g < FQ; ++g)
+              acc[f * FQ + g] =
+                  __builtin_amdgcn_mfma_f64_16x16x4f64(av[d][f][t], bv[d][g][t], acc[f * FQ + g], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
+    // the last steps (fewer than PD) are already in their buffers
+#pragma unroll
+    for (int d = 0; d < PD - 1; ++d) {
+      if (s0 + d < nfull) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int f = 0; f < FP; ++f)
+#pragma unroll
+            for (int g = 0; g < FQ; ++g)
+              acc[f * FQ + g] =
+                  __builtin_amdgcn_mfma_f64_16x16x4f64(av[d][f][t], bv[d][g][t], acc[f * FQ + g], 0, 0, 0);
+      }
+    }
+  }
+  const long long rt = r_lo + 16LL * nfull;
+  if (rt < r_hi) {  // partial last step
+    double av[FP][4], bv[FQ][4];
+#pragma unroll
+    for (int f = 0; f < FP; ++f) load4(a.A, a.lda, rt + 4 * ks, p0 + 16 * f + c, a.p, r_hi, a, false, av[f]);
+#pragma unroll
+    for (int g = 0; g < FQ; ++g) load4(a.B, a.ldb, rt + 4 * ks, q0 + 16 * g + c, a.q, r_hi, a, false, bv[g]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int f = 0; f < FP; ++f)
+#pragma unroll
+        for (int g = 0; g < FQ; ++g)
+          acc[f * FQ + g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[f][t], bv[g][t], acc[f * FQ + g], 0, 0, 0);
+  }
+  gram_epilogue<R, FP, FQ, false>(a, red, acc, p0, q0);
 }
 
 // Sum of n partial vectors of length len: workgroup (x, y) adds chunks
@@ -237,6 +368,12 @@ static void launch_gram(const GramMArgs<R> &a, int nchunk, int tiles, hipStream_
   hipLaunchKernelGGL((k_gram_mfma<R, FP, FQ, CPLX>), dim3(nchunk, tiles), dim3(256), lds, s, a);
 }
 
+template <typename R, int FP, int FQ>
+static void launch_gram_pipe(const GramMArgs<R> &a, int nchunk, int tiles, hipStream_t s) {
+  const size_t lds = sizeof(double) * 2 * FP * FQ * 256;
+  hipLaunchKernelGGL((k_gram_pipe<R, FP, FQ, 3>), dim3(nchunk, tiles), dim3(256), lds, s, a);
+}
+
 template <typename R, bool CPLX>
 static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
                   const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows,
@@ -259,8 +396,9 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   const int tiles_q = (int)((q + 16 * FQ - 1) / (16 * FQ));
   const int tiles = tiles_p * tiles_q;
   const long long rows = (long long)nrows * W;
-  // about 1024 workgroups in total (2 per CU and 4 waves each)
-  long long nchunk = (1024 + tiles - 1) / tiles;
+  // one row chunk per CU, the same for every tile shape (so the launch
+  // variants sum the same rows in the same order)
+  long long nchunk = c->ncu > 0 ? c->ncu : 256;
   long long rpw = (rows + nchunk - 1) / nchunk;
   rpw = ((rpw + 63) / 64) * 64;
   if (rpw < 64) rpw = 64;
@@ -293,6 +431,20 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
     else if (FP == 1) launch_gram<R, 1, 2, true>(a, nc, tiles, c->stream);
     else if (FQ == 1) launch_gram<R, 3, 1, true>(a, nc, tiles, c->stream);
     else launch_gram<R, 3, 2, true>(a, nc, tiles, c->stream);
+  } else if (!swap_b && a.vec) {
+    if (FP == 2) {
+      if (FQ == 1) launch_gram_pipe<R, 2, 1>(a, nc, tiles, c->stream);
+      else launch_gram_pipe<R, 2, 2>(a, nc, tiles, c->stream);
+    } else if (FP == 3) {
+      if (FQ == 1) launch_gram_pipe<R, 3, 1>(a, nc, tiles, c->stream);
+      else launch_gram_pipe<R, 3, 2>(a, nc, tiles, c->stream);
+    } else if (FP == 4) {
+      if (FQ == 1) launch_gram_pipe<R, 4, 1>(a, nc, tiles, c->stream);
+      else launch_gram_pipe<R, 4, 2>(a, nc, tiles, c->stream);
+    } else {
+      if (FQ == 1) launch_gram_pipe<R, 6, 1>(a, nc, tiles, c->stream);
+      else launch_gram_pipe<R, 6, 2>(a, nc, tiles, c->stream);
+    }
   } else {
     if (FP == 2) {
       if (FQ == 1) launch_gram<R, 2, 1, false>(a, nc, tiles, c->stream);
