@@ -99,6 +99,7 @@ def lib():
         "bdgx_combine": (I, [P, I, P, U64, U64, P, P, U64, U64, U64]),
         "bdgx_residual": (I, [P, I, P, P, P, U64, U64, U64, P, I, P, U64, P]),
         "bdgx_convert": (I, [P, I, P, U64, I, P, U64, U64, U64]),
+        "bdgx_fill_uniform_xorshift": (I, [P, P, U64, U64, U64, U64, C.c_uint32]),
         "bdgx_itp_groundstate": (I, [P, P, D, D, P, D, D, U64, P, P]),
         "bdgx_malloc": (I, [C.POINTER(P), C.c_size_t]),
         "bdgx_free": (I, [P]),
@@ -287,6 +288,14 @@ class DeviceContext:
         b = DeviceBuffer(a.nbytes)
         check(lib().bdgx_memcpy_h2d(self.h, b.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes), "h2d")
         return b
+
+    def fill_uniform(self, nrows: int, ncols: int, steps_per_col: int, seed: int = 42) -> np.ndarray:
+        """bdgx_fill_uniform_xorshift into a fresh device block, downloaded."""
+        dt = np.complex128 if self.cplx else np.float64
+        b = DeviceBuffer(np.dtype(dt).itemsize * nrows * ncols)
+        check(lib().bdgx_fill_uniform_xorshift(self.h, b.ptr, nrows, nrows, ncols, steps_per_col,
+                                               seed), "bdgx_fill_uniform_xorshift")
+        return self.download(b, (nrows, ncols), dt)
 
     def download(self, b: DeviceBuffer, shape, dtype) -> np.ndarray:
         out = np.zeros(shape, dtype=dtype, order="F")
@@ -534,12 +543,25 @@ class Problem:
         self._keep += [Xf, cb]
         lib().bdg_set_init_mode(self.h, INIT_CUSTOM, C.cast(cb, C.c_void_p), None)
 
-    def set_init_uniform(self, seed: int = 42):
-        """BDG_INIT_CUSTOM with the built-in xorshift32 uniform block (u = v)."""
+    def set_init_uniform(self, seed: int = 42, host: bool = False):
+        """BDG_INIT_CUSTOM with the built-in xorshift32 uniform block (u = v).
+        bdg_solve generates this block on the device; host=True passes the
+        same C function through a Python callback instead, which bdg_solve
+        runs on the host like any user callback (same values)."""
         s = np.array([seed], dtype=np.uint32)
         self._keep.append(s)
-        lib().bdg_set_init_mode(self.h, INIT_CUSTOM,
-                                C.cast(lib().bdg_init_uniform_xorshift, C.c_void_p), _p(s))
+        if not host:
+            lib().bdg_set_init_mode(self.h, INIT_CUSTOM,
+                                    C.cast(lib().bdg_init_uniform_xorshift, C.c_void_p), _p(s))
+            return
+        fn = C.cast(lib().bdg_init_uniform_xorshift, INIT_FN)
+
+        def fill(bdg, ptr, n, m, param):
+            fn(bdg, ptr, n, m, param)
+
+        cb = INIT_FN(fill)
+        self._keep += [fn, cb]
+        lib().bdg_set_init_mode(self.h, INIT_CUSTOM, C.cast(cb, C.c_void_p), _p(s))
 
     def set_init_mode(self, mode, hint=None):
         lib().bdg_set_init_mode(self.h, int(mode), None,

@@ -262,6 +262,69 @@ __global__ void k_itp_mu(const double *psi, const double *tpsi, const double *V,
 
 using namespace bdgx;
 
+// ---- uniform start block: xorshift32 with jump-ahead -------------------
+// One xorshift32 step is a linear map T on GF(2)^32; a 32x32 GF(2) matrix is
+// stored as its 32 columns (column b = T applied to the unit vector b).
+namespace {
+struct Gf2 {
+  uint32_t col[32];
+};
+uint32_t gf2_apply(const Gf2 &m, uint32_t x) {
+  uint32_t r = 0;
+  for (int b = 0; b < 32; ++b)
+    if (x >> b & 1u) r ^= m.col[b];
+  return r;
+}
+Gf2 gf2_mul(const Gf2 &a, const Gf2 &b) {  // a * b (apply b first)
+  Gf2 r;
+  for (int k = 0; k < 32; ++k) r.col[k] = gf2_apply(a, b.col[k]);
+  return r;
+}
+uint32_t xs_step(uint32_t x) {
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  return x;
+}
+constexpr int kJumpBits = 40;  // jumps up to 2^40 steps
+}  // namespace
+
+// Elements [t * RUN, (t + 1) * RUN) of column blockIdx.y: the thread jumps
+// its column's start state ahead by t * RUN steps with the powers T^(2^b),
+// then steps sequentially.
+template <bool CPLX>
+__global__ void __launch_bounds__(256) k_fill_uniform(double *X, long long ld, long long nrows,
+                                                      const uint32_t *col_state,
+                                                      const uint32_t *pow2, int nbits) {
+  constexpr int RUN = 64;
+  const long long r0 = ((long long)blockIdx.x * 256 + threadIdx.x) * RUN;
+  if (r0 >= nrows) return;
+  uint32_t x = col_state[blockIdx.y];
+  for (int b = 0; b < nbits; ++b) {
+    if (((unsigned long long)r0 >> b) & 1ull) {
+      const uint32_t *m = pow2 + 32 * b;
+      uint32_t r = 0;
+      for (int k = 0; k < 32; ++k)
+        if (x >> k & 1u) r ^= m[k];
+      x = r;
+    }
+  }
+  double *col = X + (long long)blockIdx.y * ld * (CPLX ? 2 : 1);
+  const long long r1 = r0 + RUN < nrows ? r0 + RUN : nrows;
+  for (long long i = r0; i < r1; ++i) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    const double v = (double)(x & 0x7FFFFFFFu) / (double)0x7FFFFFFF - 0.5;
+    if (CPLX) {
+      col[2 * i] = v;
+      col[2 * i + 1] = 0.0;
+    } else {
+      col[i] = v;
+    }
+  }
+}
+
 extern "C" {
 
 const char *bdgx_error_string(int code) {
@@ -682,6 +745,16 @@ int bdgx_free(void *p) {
   if (p) BDGX_HIP(hipFree(p));
   return BDGX_OK;
 }
+int bdgx_host_alloc(void **p, size_t bytes) {
+  if (!p) return BDGX_EINVAL;
+  *p = nullptr;
+  BDGX_HIP(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+  return BDGX_OK;
+}
+int bdgx_host_free(void *p) {
+  if (p) BDGX_HIP(hipHostFree(p));
+  return BDGX_OK;
+}
 int bdgx_memcpy_h2d(bdgx_ctx *c, void *d, const void *s, size_t n) {
   BDGX_HIP(hipMemcpyAsync(d, s, n, hipMemcpyHostToDevice, c->stream));
   BDGX_HIP(hipStreamSynchronize(c->stream));
@@ -692,6 +765,57 @@ int bdgx_memcpy_d2h(bdgx_ctx *c, void *d, const void *s, size_t n) {
   BDGX_HIP(hipStreamSynchronize(c->stream));
   return BDGX_OK;
 }
+int bdgx_fill_uniform_xorshift(bdgx_ctx *c, void *dX, uint64_t ld, uint64_t nrows,
+                               uint64_t ncols, uint64_t steps_per_col, uint32_t seed) {
+  if (!c || !dX || ld < nrows || nrows == 0 || ncols == 0 || ncols > 65535 ||
+      nrows > (1ull << kJumpBits) || steps_per_col > (1ull << kJumpBits))
+    return BDGX_EINVAL;
+  // T^(2^b), b < kJumpBits
+  std::vector<uint32_t> pw(32 * kJumpBits);
+  Gf2 t;
+  for (int b = 0; b < 32; ++b) t.col[b] = xs_step(1u << b);
+  for (int b = 0; b < kJumpBits; ++b) {
+    memcpy(&pw[32 * b], t.col, sizeof(t.col));
+    t = gf2_mul(t, t);
+  }
+  // column start states: s_0 = seed, s_{j+1} = T^steps_per_col s_j
+  std::vector<uint32_t> st(ncols);
+  uint32_t s0 = seed;
+  for (uint64_t j = 0; j < ncols; ++j) {
+    st[j] = s0;
+    for (int b = 0; b < kJumpBits; ++b)
+      if (steps_per_col >> b & 1ull) {
+        Gf2 m;
+        memcpy(m.col, &pw[32 * b], sizeof(m.col));
+        s0 = gf2_apply(m, s0);
+      }
+  }
+  uint32_t *dtab = nullptr;
+  const size_t tb = sizeof(uint32_t) * (pw.size() + st.size());
+  BDGX_HIP(hipMalloc((void **)&dtab, tb));
+  int rc = BDGX_OK;
+  if (hipMemcpyAsync(dtab, pw.data(), sizeof(uint32_t) * pw.size(), hipMemcpyHostToDevice,
+                     c->stream) != hipSuccess ||
+      hipMemcpyAsync(dtab + pw.size(), st.data(), sizeof(uint32_t) * st.size(),
+                     hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = BDGX_EHIP;
+  if (!rc) {
+    const unsigned gx = (unsigned)((nrows + 256 * 64 - 1) / (256 * 64));
+    const dim3 grid(gx, (unsigned)ncols);
+    if (c->cplx)
+      hipLaunchKernelGGL(k_fill_uniform<true>, grid, dim3(256), 0, c->stream, (double *)dX,
+                         (long long)ld, (long long)nrows, dtab + pw.size(), dtab, kJumpBits);
+    else
+      hipLaunchKernelGGL(k_fill_uniform<false>, grid, dim3(256), 0, c->stream, (double *)dX,
+                         (long long)ld, (long long)nrows, dtab + pw.size(), dtab, kJumpBits);
+    if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
+  }
+  // the tables must outlive the kernel: wait, then free
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = BDGX_EHIP;
+  (void)hipFree(dtab);
+  return rc;
+}
+
 int bdgx_memset(bdgx_ctx *c, void *d, int v, size_t n) {
   BDGX_HIP(hipMemsetAsync(d, v, n, c->stream));
   return BDGX_OK;

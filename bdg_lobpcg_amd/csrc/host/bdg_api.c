@@ -44,6 +44,49 @@ static void *xmalloc_big(size_t n) {
   return p;
 }
 
+/* The two mode arrays of a solve, allocated on a helper thread while the
+ * device iterates: page-locked (the final download then runs at the link
+ * rate instead of through the runtime's staging of pageable memory), or, if
+ * that fails, huge-page memory touched page by page. */
+typedef struct {
+  size_t bytes;
+  void *u, *v;
+  int pinned;
+} result_prefault;
+
+static void touch_pages(void *p, size_t bytes) {
+  volatile char *c = (volatile char *)p;
+  for (size_t o = 0; o < bytes; o += 4096) c[o] = 0;
+}
+
+static void *prefault_main(void *arg) {
+  result_prefault *r = (result_prefault *)arg;
+  if (bdgx_host_alloc(&r->u, r->bytes) == BDGX_OK &&
+      bdgx_host_alloc(&r->v, r->bytes) == BDGX_OK) {
+    r->pinned = 1;
+    return NULL;
+  }
+  (void)bdgx_host_free(r->u);
+  r->u = NULL;
+  r->v = NULL;
+  r->u = xmalloc_big(r->bytes);
+  touch_pages(r->u, r->bytes);
+  r->v = xmalloc_big(r->bytes);
+  touch_pages(r->v, r->bytes);
+  return NULL;
+}
+
+static void release_prefault(result_prefault *r) {
+  if (r->pinned) {
+    (void)bdgx_host_free(r->u);
+    (void)bdgx_host_free(r->v);
+  } else {
+    free(r->u);
+    free(r->v);
+  }
+  r->u = r->v = NULL;
+}
+
 static void freep(void **p) {
   free(*p);
   *p = NULL;
@@ -79,6 +122,12 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
 
 static void free_results(bdg_t *b) {
   freep((void **)&b->eigvals);
+  if (b->modes_pinned) {
+    (void)bdgx_host_free(b->modes_u);
+    (void)bdgx_host_free(b->modes_v);
+    b->modes_u = b->modes_v = NULL;
+    b->modes_pinned = 0;
+  }
   freep(&b->modes_u);
   freep(&b->modes_v);
   freep((void **)&b->residuals);
@@ -509,19 +558,30 @@ int bdg_solve(bdg_t *b) {
     b->stats.status = rc;
     return rc;
   }
-  /* start block [u; v] on the host (zero-filled anonymous mapping, huge
-   * pages), v half to the device */
-  const size_t xbytes = n * m * es64;
-  void *X = mmap(NULL, xbytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
-  if (X == MAP_FAILED) BDG_FAIL("out of host memory (%zu bytes)", xbytes);
-  (void)madvise(X, xbytes, MADV_HUGEPAGE);
-  bdg_fill_start_block(b, X);
+  const double ts_fields = bdg_wtime();
   void *dV0 = NULL;
   rc = bdgx_malloc(&dV0, S * m * es64);
-  for (uint64_t j = 0; j < m && !rc; ++j)
-    rc = bdgx_memcpy_h2d(b->dev, (char *)dV0 + j * S * es64, (const char *)X + (j * n + S) * es64,
-                         S * es64);
-  munmap(X, xbytes);
+  double ts_block = ts_fields;
+  if (!rc && b->init_mode == BDG_INIT_CUSTOM && b->custom_init_fn == bdg_init_uniform_xorshift) {
+    /* the built-in uniform block (u = v): its v half generated on the
+     * device, the same values as the host fill */
+    const uint32_t seed = b->custom_init_param ? *(const uint32_t *)b->custom_init_param : 42u;
+    rc = bdgx_fill_uniform_xorshift(b->dev, dV0, S, S, m, S, seed);
+    ts_block = bdg_wtime();
+  } else if (!rc) {
+    /* start block [u; v] on the host (zero-filled anonymous mapping, huge
+     * pages), v half to the device */
+    const size_t xbytes = n * m * es64;
+    void *X = mmap(NULL, xbytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    if (X == MAP_FAILED) BDG_FAIL("out of host memory (%zu bytes)", xbytes);
+    (void)madvise(X, xbytes, MADV_HUGEPAGE);
+    bdg_fill_start_block(b, X);
+    ts_block = bdg_wtime();
+    for (uint64_t j = 0; j < m && !rc; ++j)
+      rc = bdgx_memcpy_h2d(b->dev, (char *)dV0 + j * S * es64,
+                           (const char *)X + (j * n + S) * es64, S * es64);
+    munmap(X, xbytes);
+  }
   if (rc) {
     bdgx_free(dV0);
     b->stats.status = rc;
@@ -553,6 +613,13 @@ int bdg_solve(bdg_t *b) {
   r.zflag = (int *)xcalloc(m, sizeof(int));
   r.res = (double *)xcalloc(m, sizeof(double));
   void *dX = NULL, *dMX = NULL;
+  result_prefault pf = {S * nev * es64, NULL, NULL, 0};
+  pthread_t pf_thread;
+  const int pf_on = pthread_create(&pf_thread, NULL, prefault_main, &pf) == 0;
+  if (b->verbosity > 0)
+    fprintf(stderr, "[bdg] setup: %.1f ms (fields %.1f, start block %.1f, upload %.1f)\n",
+            1e3 * (bdg_wtime() - t_start), 1e3 * (ts_fields - t_start),
+            1e3 * (ts_block - ts_fields), 1e3 * (bdg_wtime() - ts_block));
   rc = bdg_lobpcg_device(b->dev, &cfg, dV0, &r, &dX);
   b->stats.iterations = (uint64_t)r.iters;
   b->stats.column_applies = r.applies;
@@ -572,12 +639,19 @@ int bdg_solve(bdg_t *b) {
             (unsigned long long)r.applies_fp32, (unsigned long long)r.refreshes,
             bdg_wtime() - t_start, rc);
   bdgx_free(dV0);
+  /* result extraction, timed by phase when verbose */
+  const int tv = b->verbosity > 0;
+  double tx[5] = {bdg_wtime(), 0, 0, 0, 0};
   /* M X for the u half */
   if (!rc) {
     rc = bdgx_malloc(&dMX, S * m * es64);
     if (!rc) rc = bdgx_apply(b->dev, BDGX_OP_M, BDGX_FP64, dX, dMX, m, S);
   }
+  if (tv && !rc) rc = bdgx_sync(b->dev);
+  tx[1] = bdg_wtime();
+  if (pf_on) pthread_join(pf_thread, NULL);
   if (rc) {
+    release_prefault(&pf);
     bdgx_free(dX);
     bdgx_free(dMX);
     free(r.w2);
@@ -617,6 +691,7 @@ int bdg_solve(bdg_t *b) {
   if (!rc)
     rc = bdgx_residual(b->dev, BDGX_FP64, dMX, dX, dMX, S, nev, S, dzero, 0, NULL, S, dn2);
   if (!rc) rc = bdgx_memcpy_d2h(b->dev, hn2, dn2, sizeof(double) * 3 * nev);
+  tx[2] = bdg_wtime();
   if (!rc) {
     for (uint64_t j = 0; j < nev; ++j) {
       const double w = b->eigvals[j];
@@ -645,12 +720,27 @@ int bdg_solve(bdg_t *b) {
   if (!rc) rc = bdgx_malloc(&dU, S * nev * es64);
   if (!rc) rc = bdgx_combine(b->dev, BDGX_FP64, dMX, S, nev, dcu, dU, S, nev, S);
   if (!rc) rc = bdgx_combine(b->dev, BDGX_FP64, dX, S, nev, dcv, dMX, S, nev, S);
+  if (tv && !rc) rc = bdgx_sync(b->dev);
+  tx[3] = bdg_wtime();
   if (!rc) {
-    b->modes_u = xmalloc_big(S * nev * es64);
-    b->modes_v = xmalloc_big(S * nev * es64);
+    if (pf.u && pf.v) {
+      b->modes_u = pf.u;
+      b->modes_v = pf.v;
+      b->modes_pinned = pf.pinned;
+      pf.u = pf.v = NULL;
+    } else {
+      b->modes_u = xmalloc_big(S * nev * es64);
+      b->modes_v = xmalloc_big(S * nev * es64);
+    }
     rc = bdgx_memcpy_d2h(b->dev, b->modes_u, dU, S * nev * es64);
     if (!rc) rc = bdgx_memcpy_d2h(b->dev, b->modes_v, dMX, S * nev * es64);
   }
+  tx[4] = bdg_wtime();
+  if (tv)
+    fprintf(stderr,
+            "[bdg] results: %.1f ms (M X %.1f, norms %.1f, scaling %.1f, download %.1f)\n",
+            1e3 * (tx[4] - tx[0]), 1e3 * (tx[1] - tx[0]), 1e3 * (tx[2] - tx[1]),
+            1e3 * (tx[3] - tx[2]), 1e3 * (tx[4] - tx[3]));
   bdgx_free(dzero);
   bdgx_free(dn2);
   bdgx_free(dcu);
@@ -661,6 +751,7 @@ int bdg_solve(bdg_t *b) {
   free(hn2);
   free(cu);
   free(cv);
+  release_prefault(&pf);
   b->converged = conv;
   free(r.w2);
   free(r.zflag);

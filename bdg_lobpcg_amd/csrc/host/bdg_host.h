@@ -73,6 +73,7 @@ struct bdg_t {
   /* results */
   double *eigvals;
   void *modes_u, *modes_v;
+  int modes_pinned; /* modes_u / modes_v from bdgx_host_alloc */
   uint64_t converged;
   double *residuals;
   int *zero_modes;

@@ -103,7 +103,9 @@ void bdg_start_block(bdg_t *bdg, void *X);
 
 /* Built-in bdg_init_fn for BDG_INIT_CUSTOM: u = v = xrand - 0.5 from
  * xorshift32 (reference src/bdg_internal.h:53-65), column by column, with
- * param = const uint32_t *seed. */
+ * param = const uint32_t *seed.  When a problem's start block is this
+ * callback, bdg_solve generates its v half directly on the device
+ * (bdgx_fill_uniform_xorshift: the same values) instead of calling it. */
 void bdg_init_uniform_xorshift(const bdg_t *bdg, void *X, uint64_t n, uint64_t sizeSub,
                                void *param);
 

@@ -120,6 +120,17 @@ int bdgx_gather_cols(bdgx_ctx *ctx, int prec, const void *dSrc, uint64_t lds,
                      const int *idx, uint64_t ncols, void *dDst, uint64_t ldd,
                      uint64_t nrows);
 
+/* Uniform start block on the device (no reference counterpart: the values
+ * of bdg_init_uniform_xorshift, include/bdg/bdg_amd.h, i.e. the reference's
+ * xorshift32 + xrand, src/bdg_internal.h:53-65): element (i, j), i < nrows,
+ * of the FP64 block dX (ld elements; complex ctx: interleaved, imaginary 0)
+ * is xrand value number j * steps_per_col + i + 1 of the stream seeded with
+ * `seed`, minus 0.5.  Columns start at jumped-ahead states (xorshift32 is
+ * linear over GF(2)), so the block is generated in parallel yet equals the
+ * sequential host fill bit for bit. */
+int bdgx_fill_uniform_xorshift(bdgx_ctx *ctx, void *dX, uint64_t ld, uint64_t nrows,
+                               uint64_t ncols, uint64_t steps_per_col, uint32_t seed);
+
 /* Element-type conversion FP64 <-> FP32 of an nrows x ncols block. */
 int bdgx_convert(bdgx_ctx *ctx, int prec_src, const void *dsrc, uint64_t lds,
                  int prec_dst, void *ddst, uint64_t ldd, uint64_t nrows,
@@ -136,6 +147,10 @@ int bdgx_itp_groundstate(bdgx_ctx *ctx, const double *omega, double g,
 /* ---- memory, streams, timing ------------------------------------------ */
 int bdgx_malloc(void **dptr, size_t bytes);
 int bdgx_free(void *dptr);
+/* Page-locked host memory (hipHostMalloc / hipHostFree): copies between it
+ * and the device run at the full link rate. */
+int bdgx_host_alloc(void **hptr, size_t bytes);
+int bdgx_host_free(void *hptr);
 int bdgx_memcpy_h2d(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bdgx_memcpy_d2h(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bdgx_memset(bdgx_ctx *ctx, void *dst, int value, size_t bytes);

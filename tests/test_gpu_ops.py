@@ -247,3 +247,22 @@ def test_residual_on_selected_columns():
     ns = d.download(dns, (3 * len(sel),), np.float64)
     assert np.array_equal(Rs, R[:, sel])
     assert np.array_equal(ns.reshape(-1, 3), n2.reshape(-1, 3)[sel])
+
+
+@pytest.mark.parametrize("cplx", [False, True], ids=["real", "complex"])
+def test_fill_uniform_matches_oracle_start_block(cplx):
+    """bdgx_fill_uniform_xorshift (jump-ahead, one thread per 64 values)
+    equals the v half of the oracle's sequential uniform start block (u = v,
+    the stream advancing S values per column) bit for bit, including a
+    partial last run and several columns."""
+    import bdg_lobpcg_amd as B
+    N, L = (24, 20, 18), (10.0, 9.0, 8.0)
+    c, f = physics(N, L, cplx)
+    S, m = c.size, 7
+    X = c.start_block_uniform(42, m)
+    d = B.DeviceContext(N, L, cplx)
+    V = d.fill_uniform(S, m, S, 42)
+    np.testing.assert_array_equal(V, X[S:, :])
+    np.testing.assert_array_equal(X[:S, :], X[S:, :])
+    W = d.fill_uniform(S, 3, S, 7)
+    np.testing.assert_array_equal(W, c.start_block_uniform(7, 3)[S:, :])

@@ -124,3 +124,30 @@ def test_solve_batch_matches_sequential():
     for p, w, st in zip(bat, w_seq, st_seq):
         np.testing.assert_array_equal(p.eigenvalues(4), w)
         assert (p.stats()["iterations"], p.converged()) == st
+
+
+def test_device_start_block_gives_identical_solve():
+    """The built-in uniform start block generated on the device gives the
+    same iterations and eigenvalues as the same block filled on the host
+    through a callback."""
+    import bdg_lobpcg_amd as B
+    N, L = (32, 32, 32), (14.0, 14.0, 14.0)
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
+    out = {}
+    for host in (False, True):
+        p = B.Problem(N, L, False)
+        p.set_system()
+        p.set_trap_harmonic((1.0, 1.0, 1.0))
+        p.set_wavefunction(f["wf"])
+        p.set_contact(20.0)
+        p.set_dipolar(10.0, (0.0, 0.0, 1.0), f["R"])
+        p.set_mu(1.0)
+        p.set_solver_params(6, 8, 300, 1e-9)
+        p.set_init_uniform(42, host=host)
+        assert p.solve() == 0
+        out[host] = (p.eigenvalues(6), p.stats()["iterations"], p.modes(6)[1])
+        p.close()
+    (w0, i0, v0), (w1, i1, v1) = out[False], out[True]
+    assert i0 == i1
+    np.testing.assert_array_equal(w0, w1)
+    np.testing.assert_array_equal(v0, v1)
