@@ -634,10 +634,10 @@ int bdg_solve(bdg_t *b) {
   if (b->verbosity > 0)
     fprintf(stderr,
             "[bdg] %s solve: %d iterations, %llu column applies (%llu fp32), %llu refreshes, "
-            "%.3f s, status %d\n",
+            "%.3f s (Rayleigh-Ritz %.3f s, graph capture %.3f s), status %d\n",
             cfg.w_prec == BDGX_FP32 ? "mixed" : "fp64", r.iters, (unsigned long long)r.applies,
             (unsigned long long)r.applies_fp32, (unsigned long long)r.refreshes,
-            bdg_wtime() - t_start, rc);
+            bdg_wtime() - t_start, r.rr_seconds, r.capture_seconds, rc);
   bdgx_free(dV0);
   /* result extraction, timed by phase when verbose */
   const int tv = b->verbosity > 0;

@@ -131,6 +131,7 @@ typedef struct {
   uint64_t applies, applies_fp32, refreshes;
   int learned_its, learned_fallback_at;
   uint64_t graph_launches, graph_captures;
+  double capture_seconds; /* host time spent capturing and instantiating graphs */
   double rr_seconds;
   int status;
 } lobpcg_result;

@@ -491,6 +491,7 @@ static int chain_run(bdgx_ctx *ctx, const blk3 *A, const chain_args *c, int use_
     sl->seen = 0;
   }
   if (sl->seen++ == 0) return chain_all(ctx, A, c); /* first use: eager (sizes workspaces) */
+  const double tc0 = bdg_wtime();
   if (bdgx_graph_begin(ctx)) return chain_all(ctx, A, c);
   int rc = chain_all(ctx, A, c);
   if (rc) {
@@ -502,6 +503,7 @@ static int chain_run(bdgx_ctx *ctx, const blk3 *A, const chain_args *c, int use_
     return chain_all(ctx, A, c);
   }
   res->graph_captures++;
+  res->capture_seconds += bdg_wtime() - tc0;
   rc = bdgx_graph_launch(ctx, sl->g);
   if (!rc) res->graph_launches++;
   return rc;
@@ -572,6 +574,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   memset(gslots, 0, sizeof(gslots));
   res->graph_launches = 0;
   res->graph_captures = 0;
+  res->capture_seconds = 0.0;
   double *worst_hist = (double *)calloc((size_t)cfg->maxiter + 2, sizeof(double));
 
   /* launch shapes of the FFT kernels for this grid and block width (once per
