@@ -86,20 +86,25 @@ int ensure_scratch(bdgx_ctx *c, size_t bytes) {
   return BDGX_OK;
 }
 
-template <typename R>
-static int upload_conv(bdgx_ctx *c, R **dst, const double *src, size_t n) {
-  if (!*dst) BDGX_HIP(hipMalloc((void **)dst, sizeof(R) * (n ? n : 1)));
-  std::vector<R> tmp(n);
-  for (size_t i = 0; i < n; ++i) tmp[i] = (R)src[i];
-  BDGX_HIP(hipMemcpyAsync(*dst, tmp.data(), sizeof(R) * n, hipMemcpyHostToDevice, c->stream));
-  BDGX_HIP(hipStreamSynchronize(c->stream));
-  return BDGX_OK;
+// n reals: d[e] = (float)s[e]
+__global__ void k_d2f(const double *s, float *d, long long n) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    d[e] = (float)s[e];
 }
 
+// n reals (a complex array counts its real and imaginary parts): the FP64
+// copy uploaded as is, the FP32 copy converted from it on the device (same
+// round-to-nearest conversion as on the host)
 static int upload_both(bdgx_ctx *c, double **d64, float **d32, const double *src, size_t n) {
-  int rc = upload_conv<double>(c, d64, src, n);
-  if (rc) return rc;
-  return upload_conv<float>(c, d32, src, n);
+  if (!*d64) BDGX_HIP(hipMalloc((void **)d64, sizeof(double) * (n ? n : 1)));
+  if (!*d32) BDGX_HIP(hipMalloc((void **)d32, sizeof(float) * (n ? n : 1)));
+  if (n == 0) return BDGX_OK;
+  BDGX_HIP(hipMemcpyAsync(*d64, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_d2f, dim3(1024), dim3(256), 0, c->stream, *d64, *d32, (long long)n);
+  BDGX_HIP(hipGetLastError());
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  return BDGX_OK;
 }
 
 template <typename R>
@@ -145,12 +150,6 @@ __global__ void k_dipolar_kernel(double *U, int nx, int ny, int nz, int n0k, int
     const double f = 1.0 + 3.0 * cos(kR) / kR2 - 3.0 * sin(kR) / (kR2 * kR);
     U[e] = (3.0 * c2 - 1.0) * f;
   }
-}
-
-__global__ void k_d2f(const double *s, float *d, long long n) {
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
-       e += (long long)gridDim.x * blockDim.x)
-    d[e] = (float)s[e];
 }
 
 // ---------------------------------------------------------------------------
