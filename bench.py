@@ -395,6 +395,15 @@ def main():
         "column_applies": {"fp64": r64["stats"]["column_applies"],
                            "mixed": rmx["stats"]["column_applies"],
                            "mixed_learned": rlr["stats"]["column_applies"]},
+        # column applies (one column through M then K, or T_K then T_M) per
+        # second of time-to-solution, the whole solve included
+        "column_applies_per_second": {
+            "fp64": r64["stats"]["column_applies"] / t64,
+            "mixed": rmx["stats"]["column_applies"] / tmx,
+            "mixed_learned": rlr["stats"]["column_applies"] / tlr},
+        "rayleigh_ritz_seconds": {"fp64": r64["stats"]["seconds_rr"],
+                                  "mixed": rmx["stats"]["seconds_rr"],
+                                  "mixed_learned": rlr["stats"]["seconds_rr"]},
         "learned": dict(tinfo, training_seconds=t_train,
                         learned_iterations=rlr["stats"]["learned_iterations"],
                         fallback_at=rlr["stats"]["learned_fallback_at"]),
