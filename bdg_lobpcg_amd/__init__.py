@@ -100,6 +100,7 @@ def lib():
         "bdgx_residual": (I, [P, I, P, P, P, U64, U64, U64, P, I, P, U64, P]),
         "bdgx_convert": (I, [P, I, P, U64, I, P, U64, U64, U64]),
         "bdgx_fill_uniform_xorshift": (I, [P, P, U64, U64, U64, U64, C.c_uint32]),
+        "bdgx_gram_pair": (I, [P, I, P, U64, U64, P, U64, P, U64, U64, U64, P]),
         "bdgx_itp_groundstate": (I, [P, P, D, D, P, D, D, U64, P, P]),
         "bdgx_malloc": (I, [C.POINTER(P), C.c_size_t]),
         "bdgx_free": (I, [P]),
@@ -380,6 +381,17 @@ class DeviceContext:
                               B.shape[1], A.shape[0], int(swap_b), dG.ptr), "gram")
         G = self.download(dG, (A.shape[1], B.shape[1]), self.dtype64)
         return G
+
+    def gram_pair(self, A: np.ndarray, B1: np.ndarray, B2: np.ndarray, prec=FP64) -> np.ndarray:
+        """bdgx_gram_pair: A^H [B1 | B2] with A read once (real contexts)."""
+        A, B1, B2 = (np.asfortranarray(x, dtype=self.dtype(prec)) for x in (A, B1, B2))
+        dA, d1, d2 = self.upload(A), self.upload(B1), self.upload(B2)
+        q = B1.shape[1]
+        dG = DeviceBuffer(8 * A.shape[1] * 2 * q)
+        check(lib().bdgx_gram_pair(self.h, prec, dA.ptr, A.shape[0], A.shape[1], d1.ptr,
+                                   B1.shape[0], d2.ptr, B2.shape[0], q, A.shape[0], dG.ptr),
+              "gram_pair")
+        return self.download(dG, (A.shape[1], 2 * q), np.float64)
 
     def combine(self, A: np.ndarray, Cm: np.ndarray, prec=FP64) -> np.ndarray:
         A = np.asfortranarray(A, dtype=self.dtype(prec))

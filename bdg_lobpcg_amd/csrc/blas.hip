@@ -30,6 +30,9 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 template <typename R>
 struct GramMArgs {
   const R *A, *B;
+  const R *B2;             // columns q1.. of B (k_gram_pipe only), or null
+  int q1;                  // columns of B before B2 (q when B2 is null)
+  long long ldb2;
   long long lda, ldb;      // in R units of the real view
   long long nrows;         // rows of the real view
   long long half;          // element rows / 2 (swap_b)
@@ -236,8 +239,10 @@ __global__ void __launch_bounds__(256) k_gram_pipe(GramMArgs<R> a) {
   }
 #pragma unroll
   for (int g = 0; g < FQ; ++g) {
-    const int col = q0 + 16 * g + c;
-    pb[g] = a.B + (long long)(col < a.q ? col : a.q - 1) * a.ldb + r_lo + 4 * ks;
+    const int cq = q0 + 16 * g + c;
+    const int col = cq < a.q ? cq : a.q - 1;
+    pb[g] = (col < a.q1 ? a.B + (long long)col * a.ldb
+                        : a.B2 + (long long)(col - a.q1) * a.ldb2) + r_lo + 4 * ks;
   }
   auto ld = [](const R *p, double out[4]) {
     if constexpr (sizeof(R) == 8) {
@@ -310,7 +315,13 @@ __global__ void __launch_bounds__(256) k_gram_pipe(GramMArgs<R> a) {
 #pragma unroll
     for (int f = 0; f < FP; ++f) load4(a.A, a.lda, rt + 4 * ks, p0 + 16 * f + c, a.p, r_hi, a, false, av[f]);
 #pragma unroll
-    for (int g = 0; g < FQ; ++g) load4(a.B, a.ldb, rt + 4 * ks, q0 + 16 * g + c, a.q, r_hi, a, false, bv[g]);
+    for (int g = 0; g < FQ; ++g) {
+      const int cq = q0 + 16 * g + c;
+      if (cq < a.q1)
+        load4(a.B, a.ldb, rt + 4 * ks, cq, a.q1, r_hi, a, false, bv[g]);
+      else
+        load4(a.B2, a.ldb2, rt + 4 * ks, cq - a.q1, a.q - a.q1, r_hi, a, false, bv[g]);
+    }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -374,11 +385,16 @@ static void launch_gram_pipe(const GramMArgs<R> &a, int nchunk, int tiles, hipSt
   hipLaunchKernelGGL((k_gram_pipe<R, FP, FQ, 3>), dim3(nchunk, tiles), dim3(256), lds, s, a);
 }
 
+// G = A^H B (p x q).  With dB2 (real, unswapped, vector-aligned blocks
+// only) the right block is [B | B2], q columns each, and G is p x 2q: A is
+// read once for both.
 template <typename R, bool CPLX>
 static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
                   const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows,
-                  int swap_b, double *dG) {
+                  int swap_b, double *dG, const void *dB2 = nullptr, uint64_t ldb2 = 0) {
   if (p == 0 || q == 0) return BDGX_OK;
+  const uint64_t q1 = q;
+  if (dB2) q = 2 * q;
   const int W = CPLX ? 2 : 1;
   // tile shape: real up to 96 x 32, complex up to 48 x 32 (12 accumulators);
   // launch variant 1 (autotuned) caps real tiles at 48 rows of p: more p
@@ -412,6 +428,9 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   GramMArgs<R> a;
   a.A = reinterpret_cast<const R *>(dA);
   a.B = reinterpret_cast<const R *>(dB);
+  a.B2 = reinterpret_cast<const R *>(dB2);
+  a.q1 = (int)q1;
+  a.ldb2 = (long long)ldb2 * W;
   a.lda = (long long)lda * W;
   a.ldb = (long long)ldb * W;
   a.nrows = rows;
@@ -422,8 +441,10 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   a.tiles_p = tiles_p;
   a.swap_b = swap_b;
   a.wel = W;
-  const uintptr_t al = (uintptr_t)dA | (uintptr_t)dB;
-  a.vec = (a.lda % 4 == 0) && (a.ldb % 4 == 0) && (al % (4 * sizeof(R)) == 0);
+  const uintptr_t al = (uintptr_t)dA | (uintptr_t)dB | (uintptr_t)dB2;
+  a.vec = (a.lda % 4 == 0) && (a.ldb % 4 == 0) && (!dB2 || a.ldb2 % 4 == 0) &&
+          (al % (4 * sizeof(R)) == 0);
+  if (dB2 && (CPLX || swap_b || !a.vec)) return BDGX_EINVAL;
   a.part = reinterpret_cast<double *>(c->scratch);
   const int nc = (int)nchunk;
   if (CPLX) {
@@ -463,6 +484,17 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   BDGX_HIP(hipGetLastError());
   double *part = reinterpret_cast<double *>(c->scratch);
   return reduce_partials(c, part, nc, len, part + (size_t)nc * len, dG);
+}
+
+int gram_pair_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
+                       const void *dB, uint64_t ldb, const void *dB2, uint64_t ldb2,
+                       uint64_t q, uint64_t nrows, double *dG) {
+  if (c->cplx) return BDGX_EINVAL;
+  if (prec == BDGX_FP64)
+    return gram_t<double, false>(c, dA, lda, p, dB, ldb, q, nrows, 0, dG, dB2, ldb2);
+  if (prec == BDGX_FP32)
+    return gram_t<float, false>(c, dA, lda, p, dB, ldb, q, nrows, 0, dG, dB2, ldb2);
+  return BDGX_EINVAL;
 }
 
 int gram_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,

@@ -18,6 +18,9 @@ int apply_dispatch(bdgx_ctx *c, int op, int prec, const void *dX, void *dY,
 int gram_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
                   const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
                   double *dG);
+int gram_pair_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
+                       const void *dB, uint64_t ldb, const void *dB2, uint64_t ldb2,
+                       uint64_t q, uint64_t nrows, double *dG);
 int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
                      const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows);
 int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,
@@ -530,6 +533,13 @@ int bdgx_gram(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p, c
   if (!c || !dA || !dB || !dG) return BDGX_EINVAL;
   if (swap_b && (nrows & 1)) return BDGX_EINVAL;
   return gram_dispatch(c, prec, dA, lda, p, dB, ldb, q, nrows, swap_b, dG);
+}
+
+int bdgx_gram_pair(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
+                   const void *dB1, uint64_t ldb1, const void *dB2, uint64_t ldb2, uint64_t q,
+                   uint64_t nrows, double *dG) {
+  if (!c || !dA || !dB1 || !dB2 || !dG) return BDGX_EINVAL;
+  return gram_pair_dispatch(c, prec, dA, lda, p, dB1, ldb1, dB2, ldb2, q, nrows, dG);
 }
 
 int bdgx_combine(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,

@@ -258,11 +258,18 @@ static int grams_full(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, ui
 static int grams_cols_dev(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, int c0, int nc,
                           uint64_t S, int w, size_t es, double *dG) {
   const size_t csz = (size_t)k * nc * w;
-  int rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, colp(b->V, (uint64_t)c0, S, es), S,
-                     (uint64_t)nc, S, 0, dG);
-  if (!rc && !std)
-    rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, colp(b->MV, (uint64_t)c0, S, es), S,
-                   (uint64_t)nc, S, 0, dG + csz);
+  int rc = BDGX_EINVAL;
+  /* few new columns: V^H W and V^H M W in one pass over V */
+  if (!std && w == 1 && 2 * nc <= 32)
+    rc = bdgx_gram_pair(ctx, prec, b->V, S, (uint64_t)k, colp(b->V, (uint64_t)c0, S, es), S,
+                        colp(b->MV, (uint64_t)c0, S, es), S, (uint64_t)nc, S, dG);
+  if (rc) {
+    rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, colp(b->V, (uint64_t)c0, S, es), S,
+                   (uint64_t)nc, S, 0, dG);
+    if (!rc && !std)
+      rc = bdgx_gram(ctx, prec, b->V, S, (uint64_t)k, colp(b->MV, (uint64_t)c0, S, es), S,
+                     (uint64_t)nc, S, 0, dG + csz);
+  }
   if (!rc)
     rc = bdgx_gram(ctx, prec, std ? b->V : b->MV, S, (uint64_t)k,
                    colp(b->KMV, (uint64_t)c0, S, es), S, (uint64_t)nc, S, 0, dG + 2 * csz);

@@ -94,6 +94,15 @@ int bdgx_apply_host(bdgx_ctx *ctx, int op, const void *hX, void *hY,
 int bdgx_gram(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64_t p,
               const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows,
               int swap_b, double *dG);
+/* Two Grams with one left block read once (no reference counterpart; the
+ * solver's V^H V_W and V^H M V_W, src/solver.c:262 through the external
+ * iLOBPCG's Gram step): dG (p x 2q, column-major) = dA^H [dB1 | dB2], dB1
+ * and dB2 q columns each.  Real contexts, 16-byte aligned blocks with leading
+ * dimensions a multiple of 4 (else BDGX_EINVAL).  Each entry equals the
+ * corresponding entry of the separate bdgx_gram calls bit for bit. */
+int bdgx_gram_pair(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64_t p,
+                   const void *dB1, uint64_t ldb1, const void *dB2, uint64_t ldb2, uint64_t q,
+                   uint64_t nrows, double *dG);
 /* dY[:, 0:m] = dA[:, 0:k] * C  (C is k x m column-major FP64 / complex FP64
  * in device memory, ldc = k); dY must not alias dA. */
 int bdgx_combine(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda,

@@ -266,3 +266,25 @@ def test_fill_uniform_matches_oracle_start_block(cplx):
     np.testing.assert_array_equal(X[:S, :], X[S:, :])
     W = d.fill_uniform(S, 3, S, 7)
     np.testing.assert_array_equal(W, c.start_block_uniform(7, 3)[S:, :])
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+def test_gram_pair_equals_two_grams(prec):
+    """bdgx_gram_pair (A read once for [B1 | B2]) gives the two bdgx_gram
+    results bit for bit, at the solver's tail shape (36 x 2 + 2), a wider
+    pair (96 x 16 + 16) and a row count that ends in a partial step."""
+    import bdg_lobpcg_amd as B
+    N, L = (24, 20, 18), (10.0, 9.0, 8.0)
+    c, f = physics(N, L, False)
+    d = B.DeviceContext(N, L, False)
+    rng = np.random.default_rng(4)
+    for rows, p, q in ((c.size, 36, 2), (c.size, 96, 16), (c.size - 20, 20, 5)):
+        A = rng.standard_normal((rows, p))
+        B1 = rng.standard_normal((rows, q))
+        B2 = rng.standard_normal((rows, q))
+        G = d.gram_pair(A, B1, B2, prec)
+        np.testing.assert_array_equal(G[:, :q], d.gram(A, B1, prec))
+        np.testing.assert_array_equal(G[:, q:], d.gram(A, B2, prec))
+        dt = np.float32 if prec else np.float64
+        ref = A.astype(dt).astype(np.float64).T @ np.hstack([B1, B2]).astype(dt).astype(np.float64)
+        assert relerr(G, ref) < 1e-12
