@@ -247,3 +247,14 @@ int dn_rayleigh_ritz(int k, int m, int cplx, const double *GV, const double *GM,
   if (cplx) return rr_z(k, m, GV, GM, GA, w2, Y, zflags);
   return rr_d(k, m, GV, GM, GA, w2, Y, zflags);
 }
+
+void dn_qhgq(int k, int q, int cplx, const double *G, const double *Q, double *out) {
+  if (cplx)
+    qhgq_z(k, q, (const double complex *)G, (const double complex *)Q, (double complex *)out);
+  else
+    qhgq_d(k, q, G, Q, out);
+}
+
+int dn_chol_ok(int k, int cplx, const double *G, double tau) {
+  return cplx ? chol_ok_z(k, (const double complex *)G, tau) : chol_ok_d(k, G, tau);
+}

@@ -24,4 +24,13 @@ int dn_heev(int n, double *A, double *w);
 int dn_rayleigh_ritz(int k, int m, int cplx, const double *GV, const double *GM,
                      const double *GA, double *w2, double *Y, int *zflags);
 
+/* out (q x q) = Q^H G Q, G k x k Hermitian, Q k x q (complex interleaved
+ * when cplx): the Gram set of a combined basis V Q from the Gram of V. */
+void dn_qhgq(int k, int q, int cplx, const double *G, const double *Q, double *out);
+
+/* 1 when the diagonally scaled G (k x k, Hermitian) has a Cholesky factor
+ * whose every pivot is >= tau, else 0 (also when a diagonal entry is not
+ * positive). */
+int dn_chol_ok(int k, int cplx, const double *G, double tau);
+
 #endif

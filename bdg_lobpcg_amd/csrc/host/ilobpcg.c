@@ -122,56 +122,6 @@ static inline void setz(double *A, int ld, int i, int j, int w, double complex v
   if (w == 2) p[1] = cimag(v);
 }
 
-/* out (q x q) = Q^H G Q, G k x k, Q k x q */
-static void qhgq(const double *G, int k, const double *Q, int q, int w, double *out) {
-  double complex *T = (double complex *)malloc(sizeof(double complex) * (size_t)k * q);
-  for (int j = 0; j < q; ++j)
-    for (int i = 0; i < k; ++i) {
-      double complex s = 0;
-      for (int p = 0; p < k; ++p) s += getz(G, k, i, p, w) * getz(Q, k, p, j, w);
-      T[i + (size_t)j * k] = s;
-    }
-  for (int j = 0; j < q; ++j)
-    for (int i = 0; i < q; ++i) {
-      double complex s = 0;
-      for (int p = 0; p < k; ++p) s += conj(getz(Q, k, p, i, w)) * T[p + (size_t)j * k];
-      setz(out, q, i, j, w, s);
-    }
-  free(T);
-}
-
-/* Cholesky pivot test of the diagonally scaled Gram (real or complex) */
-static int gram_ok(const double *G, int k, int w) {
-  double complex *A = (double complex *)malloc(sizeof(double complex) * (size_t)k * k);
-  double *d = (double *)malloc(sizeof(double) * (size_t)k);
-  int ok = 1;
-  for (int i = 0; i < k; ++i) {
-    const double g = creal(getz(G, k, i, i, w));
-    if (!(g > 0)) ok = 0;
-    d[i] = g > 0 ? 1.0 / sqrt(g) : 0.0;
-  }
-  for (int j = 0; j < k && ok; ++j)
-    for (int i = 0; i < k; ++i) A[i + (size_t)j * k] = getz(G, k, i, j, w) * d[i] * d[j];
-  for (int j = 0; j < k && ok; ++j) {
-    double djj = creal(A[j + (size_t)j * k]);
-    for (int p = 0; p < j; ++p) djj -= creal(A[j + (size_t)p * k] * conj(A[j + (size_t)p * k]));
-    if (!(djj >= 1e-10)) {
-      ok = 0;
-      break;
-    }
-    const double l = sqrt(djj);
-    A[j + (size_t)j * k] = l;
-    for (int i = j + 1; i < k; ++i) {
-      double complex s = A[i + (size_t)j * k];
-      for (int p = 0; p < j; ++p) s -= A[i + (size_t)p * k] * conj(A[j + (size_t)p * k]);
-      A[i + (size_t)j * k] = s / l;
-    }
-  }
-  free(A);
-  free(d);
-  return ok;
-}
-
 /* Solve G Y = Z for Hermitian positive definite G (n x n) and n x r right-hand
  * sides, in place in Z, by Cholesky.  Returns 0, or -1 when a pivot is not
  * positive. */
@@ -218,6 +168,20 @@ static double signed_sqrt(double x) { return x >= 0 ? sqrt(x) : -sqrt(-x); }
 typedef struct {
   double *V, *M, *A; /* k x k each (complex interleaved when w = 2) */
 } gram3;
+
+/* Gram blocks of [X | P] = V_prev [C_X | C_P] (the first q columns of the
+ * new basis) from the previous Grams: Q^H G_prev Q, written into the
+ * top-left q x q blocks of the new k x k sets */
+static void project_grams(const gram3 *gp, int kprev, const double *hCp, int q, int k, int w,
+                          double *hT, gram3 *gn) {
+  const double *src[3] = {gp->V, gp->M, gp->A};
+  double *dst[3] = {gn->V, gn->M, gn->A};
+  for (int g = 0; g < 3; ++g) {
+    dn_qhgq(kprev, q, w == 2, src[g], hCp, hT);
+    for (int j = 0; j < q; ++j)
+      memcpy(dst[g] + (size_t)j * k * w, hT + (size_t)j * q * w, sizeof(double) * (size_t)q * w);
+  }
+}
 
 /* full device Grams on V[:, 0:k].  Product form: V^H V, V^H MV, MV^H KMV.
  * Standard form (std != 0, the "M" images hold K V and the "KM" images K^2 V):
@@ -702,16 +666,6 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     }
     void *other = (Rsrc == dR) ? dTmp : dR;
     const int full = !gvalid || cfg->gram_refresh <= 0 || ++since_full >= cfg->gram_refresh;
-    if (!full) {
-      /* [X | P] = V_prev [C_X | C_P]: Gram blocks from Q^H G_prev Q */
-      const int q = m + np;
-      double *gp[3] = {Gp.V, Gp.M, Gp.A}, *gn[3] = {G.V, G.M, G.A};
-      for (int g = 0; g < 3; ++g) {
-        qhgq(gp[g], kprev, hCp, q, w, hT);
-        for (int j = 0; j < q; ++j)
-          for (int i = 0; i < q; ++i) setz(gn[g], k, i, j, w, getz(hT, q, i, j, w));
-      }
-    }
     if (mixed && st.worst <= cfg->img_tol) img_done = 1;
     const int img32 = mixed && !img_done && cfg->img_tol > 0 && nact >= cfg->mixed_min_cols &&
                       st.worst > cfg->img_tol;
@@ -735,9 +689,12 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     ca.dG = dG;
     if (lp_on) res->learned_its++;
     if (!img32) {
-      /* W, FP64 images, Gram columns: one device chain (graph-replayed) */
+      /* W, FP64 images, Gram columns: one device chain (graph-replayed);
+       * the host projects the [X | P] Gram blocks while it runs */
       TRY(chain_run(ctx, &A, &ca, cfg->graphs, gslots, &gclock, res));
+      if (!full) project_grams(&Gp, kprev, hCp, m + np, k, w, hT, &G);
     } else {
+      if (!full) project_grams(&Gp, kprev, hCp, m + np, k, w, hT, &G);
       TRY(chain_w(ctx, &A, &ca));
       /* FP32 images of W, with W first projected out of span(X) (Euclidean,
        * FP64) so that the image error is relative to the new part of W */
@@ -789,14 +746,14 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     int ks = 0;
     for (int i = 0; i < k; ++i) sel[ks++] = i;
     sub_block(G.V, k, sel, ks, w, hS[0]);
-    if (np && !gram_ok(hS[0], ks, w)) {
+    if (np && !dn_chol_ok(ks, cplx, hS[0], 1e-10)) {
       ks = 0;
       for (int i = 0; i < m; ++i) sel[ks++] = i;
       for (int i = wc; i < k; ++i) sel[ks++] = i;
       sub_block(G.V, k, sel, ks, w, hS[0]);
       res->restarts++;
     }
-    if (!gram_ok(hS[0], ks, w)) {
+    if (!dn_chol_ok(ks, cplx, hS[0], 1e-10)) {
       /* [X, W] still dependent: the preconditioner collapsed the residuals
        * (e.g. a 1e8 gain on one mode); use W = R for this iteration */
       trr += bdg_wtime() - t0;

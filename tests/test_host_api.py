@@ -235,3 +235,44 @@ def test_host_rayleigh_ritz_matches_generalized_eigh(k, m):
     res = GA @ Y - (GM @ Y) * w2
     assert np.abs(res).max() < 1e-9 * abs(ref).max()
     assert not zf.any()
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+def test_host_gram_projection_and_independence_test(cplx):
+    """dn_qhgq (the Gram set of a combined basis V Q from the Gram of V,
+    used by the solver between full Gram recomputes) against numpy's Q^H G Q;
+    dn_chol_ok (the solver's dependence test of [X P W]) against numpy's
+    Cholesky of the diagonally scaled Gram: independent columns pass, a
+    column that is a combination of two others fails, a zero column fails."""
+    import ctypes as C
+    lib = B.lib()
+    lib.dn_qhgq.restype = None
+    lib.dn_qhgq.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.dn_chol_ok.restype = C.c_int
+    lib.dn_chol_ok.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_double]
+    rng = np.random.default_rng(7 + cplx)
+    dt = np.complex128 if cplx else np.float64
+
+    def rnd(*shape):
+        a = rng.standard_normal(shape)
+        return a + 1j * rng.standard_normal(shape) if cplx else a
+
+    for k, q in ((5, 3), (36, 34), (96, 64), (97, 65)):
+        V = rnd(4 * k, k)
+        G = np.asfortranarray((V.conj().T @ V).astype(dt))
+        Q = np.asfortranarray(rnd(k, q).astype(dt))
+        out = np.zeros((q, q), dtype=dt, order="F")
+        lib.dn_qhgq(k, q, cplx, G.ctypes.data, Q.ctypes.data, out.ctypes.data)
+        ref = Q.conj().T @ G @ Q
+        assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
+        assert lib.dn_chol_ok(k, cplx, G.ctypes.data, 1e-10) == 1
+        d = 1.0 / np.sqrt(np.real(np.diag(G)))
+        np.linalg.cholesky(d[:, None] * G * d[None, :])
+        Vd = V.copy()
+        Vd[:, k // 2] = 2.0 * V[:, 0] - 0.5 * V[:, k - 1]
+        Gd = np.asfortranarray((Vd.conj().T @ Vd).astype(dt))
+        assert lib.dn_chol_ok(k, cplx, Gd.ctypes.data, 1e-10) == 0
+        Vz = V.copy()
+        Vz[:, k - 1] = 0
+        Gz = np.asfortranarray((Vz.conj().T @ Vz).astype(dt))
+        assert lib.dn_chol_ok(k, cplx, Gz.ctypes.data, 1e-10) == 0
