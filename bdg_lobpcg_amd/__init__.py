@@ -65,7 +65,8 @@ class SolveStats(C.Structure):
                 ("restarts", C.c_uint64), ("learned_iterations", C.c_uint64),
                 ("learned_fallback_at", C.c_int), ("graph_launches", C.c_uint64),
                 ("graph_captures", C.c_uint64), ("seconds_total", C.c_double),
-                ("seconds_rr", C.c_double), ("status", C.c_int)]
+                ("seconds_rr", C.c_double), ("status", C.c_int),
+                ("frozen_skips", C.c_uint64)]
 
 
 def build() -> None:
@@ -97,6 +98,7 @@ def lib():
         "bdgx_apply_host": (I, [P, I, P, P, U64, U64]),
         "bdgx_gram": (I, [P, I, P, U64, U64, P, U64, U64, U64, I, P]),
         "bdgx_combine": (I, [P, I, P, U64, U64, P, P, U64, U64, U64]),
+        "bdgx_combine_cols": (I, [P, I, P, U64, U64, P, P, U64, P, U64, U64]),
         "bdgx_residual": (I, [P, I, P, P, P, U64, U64, U64, P, I, P, U64, P]),
         "bdgx_convert": (I, [P, I, P, U64, I, P, U64, U64, U64]),
         "bdgx_fill_uniform_xorshift": (I, [P, P, U64, U64, U64, U64, C.c_uint32]),
@@ -155,6 +157,7 @@ def lib():
         "bdg_set_mixed_params": (None, [P, D, U64]),
         "bdg_set_verbosity": (None, [P, I]),
         "bdg_set_graph_capture": (None, [P, I]),
+        "bdg_set_freeze": (None, [P, D]),
         "bdg_solve_batch": (I, [C.POINTER(P), I, I, C.POINTER(I)]),
         "bdgx_graph_begin": (I, [P]),
         "bdgx_graph_end": (I, [P, C.POINTER(P)]),
@@ -402,6 +405,18 @@ class DeviceContext:
                                  A.shape[0], Cm.shape[1], A.shape[0]), "combine")
         return self.download(dY, (A.shape[0], Cm.shape[1]), A.dtype)
 
+    def combine_cols(self, A: np.ndarray, Cm: np.ndarray, cols, Y0: np.ndarray) -> np.ndarray:
+        """Y = Y0 with Y[:, cols[j]] = A @ Cm[:, j] (FP64, at most 8 columns)."""
+        A = np.asfortranarray(A, dtype=self.dtype64)
+        Cm = np.asfortranarray(Cm, dtype=self.dtype64)
+        Y0 = np.asfortranarray(Y0, dtype=self.dtype64)
+        idx = np.ascontiguousarray(cols, dtype=np.int32)
+        dA, dC, dY = self.upload(A), self.upload(Cm), self.upload(Y0)
+        check(lib().bdgx_combine_cols(self.h, FP64, dA.ptr, A.shape[0], A.shape[1], dC.ptr,
+                                      dY.ptr, Y0.shape[0], idx.ctypes.data, idx.size,
+                                      A.shape[0]), "combine_cols")
+        return self.download(dY, Y0.shape, Y0.dtype)
+
     # -- learned preconditioner (FP32, real psi0) ---------------------------
     def lp_set_params(self, theta):
         th = np.ascontiguousarray(theta, dtype=np.float32)
@@ -530,6 +545,10 @@ class Problem:
 
     def set_graph_capture(self, on: bool):
         lib().bdg_set_graph_capture(self.h, int(bool(on)))
+
+    def set_freeze(self, tol: float):
+        """Converged columns moved by less than tol keep their vectors (0: off)."""
+        lib().bdg_set_freeze(self.h, float(tol))
 
     def set_learned_fallback(self, window=30, factor=0.5):
         lib().bdg_set_learned_fallback(self.h, int(window), float(factor))

@@ -938,6 +938,97 @@ static int combine_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k,
   return BDGX_OK;
 }
 
+// Few-column combine through a column map: Y[:, col[o]] = A[:, 0:k] C[:, o]
+// for o < n <= COLMAP_MAX (FP64, real or complex interleaved).  For the
+// update of a block in which most columns keep their vectors: A is read once,
+// only the n listed columns of Y are written.  One row per thread; the sum
+// over i runs in a fixed order (bitwise reproducible), and a unit coefficient
+// column copies its source column exactly.
+constexpr int COLMAP_MAX = BDGX_COMBINE_COLS_MAX;
+struct ColMap {
+  int n;
+  long long col[COLMAP_MAX];
+};
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256) k_combine_cols(const double *__restrict__ A, long long lda,
+                                                      int k, const double *__restrict__ C,
+                                                      double *__restrict__ Y, long long ldy,
+                                                      ColMap map, long long nrows) {
+  constexpr int W = CPLX ? 2 : 1;
+  const long long r = blockIdx.x * 256LL + threadIdx.x;
+  if (r >= nrows) return;
+  const int n = map.n;
+  double acc[COLMAP_MAX][W];
+#pragma unroll
+  for (int o = 0; o < COLMAP_MAX; ++o)
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc[o][w] = 0.0;
+  // four source columns per step: their loads are in flight together
+  int i = 0;
+  for (; i < k; i += 4) {
+    const int nt = k - i < 4 ? k - i : 4;
+    double a[4][W];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      // past the last column: load the last one again (unused), no branch
+      const int ic = t < nt ? i + t : k - 1;
+#pragma unroll
+      for (int w = 0; w < W; ++w) a[t][w] = A[(r + (long long)ic * lda) * W + w];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t >= nt) break;
+#pragma unroll
+      for (int o = 0; o < COLMAP_MAX; ++o) {
+        if (o >= n) break;
+        const double *c = C + ((long long)(i + t) + (long long)o * k) * W;
+        if (CPLX) {
+          acc[o][0] = fma(a[t][0], c[0], acc[o][0]);
+          acc[o][0] = fma(-a[t][W - 1], c[W - 1], acc[o][0]);
+          acc[o][W - 1] = fma(a[t][0], c[W - 1], acc[o][W - 1]);
+          acc[o][W - 1] = fma(a[t][W - 1], c[0], acc[o][W - 1]);
+        } else {
+          acc[o][0] = fma(a[t][0], c[0], acc[o][0]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < COLMAP_MAX; ++o) {
+    if (o >= n) break;
+#pragma unroll
+    for (int w = 0; w < W; ++w) Y[(r + map.col[o] * ldy) * W + w] = acc[o][w];
+  }
+}
+
+int combine_cols_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
+                          const double *dC, void *dY, uint64_t ldy, const int *cols,
+                          uint64_t ncols, uint64_t nrows) {
+  if (prec != BDGX_FP64 || ncols > (uint64_t)COLMAP_MAX || k > (1u << 20)) return BDGX_EINVAL;
+  if (ncols == 0 || nrows == 0) return BDGX_OK;
+  ColMap map;
+  memset(&map, 0, sizeof(map));
+  map.n = (int)ncols;
+  for (uint64_t o = 0; o < ncols; ++o) {
+    if (cols[o] < 0) return BDGX_EINVAL;
+    for (uint64_t q = 0; q < o; ++q)
+      if (cols[q] == cols[o]) return BDGX_EINVAL;
+    map.col[o] = cols[o];
+  }
+  const unsigned nb = (unsigned)((nrows + 255) / 256);
+  if (c->cplx)
+    hipLaunchKernelGGL(k_combine_cols<true>, dim3(nb), dim3(256), 0, c->stream,
+                       reinterpret_cast<const double *>(dA), (long long)lda, (int)k, dC,
+                       reinterpret_cast<double *>(dY), (long long)ldy, map, (long long)nrows);
+  else
+    hipLaunchKernelGGL(k_combine_cols<false>, dim3(nb), dim3(256), 0, c->stream,
+                       reinterpret_cast<const double *>(dA), (long long)lda, (int)k, dC,
+                       reinterpret_cast<double *>(dY), (long long)ldy, map, (long long)nrows);
+  BDGX_HIP(hipGetLastError());
+  return BDGX_OK;
+}
+
 int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
                      const double *dC, void *dY, uint64_t ldy, uint64_t m,
                      uint64_t nrows) {

@@ -23,6 +23,9 @@ int gram_pair_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint
                        uint64_t q, uint64_t nrows, double *dG);
 int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
                      const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows);
+int combine_cols_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
+                          const double *dC, void *dY, uint64_t ldy, const int *cols,
+                          uint64_t ncols, uint64_t nrows);
 int residual_dispatch(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,
                       uint64_t ld, uint64_t ncols, uint64_t nrows, const double *dlam,
                       int swap_x, void *dR, uint64_t ldr, double *dn2, const int *dcols);
@@ -546,6 +549,13 @@ int bdgx_combine(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k
                  const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows) {
   if (!c || !dA || !dC || !dY) return BDGX_EINVAL;
   return combine_dispatch(c, prec, dA, lda, k, dC, dY, ldy, m, nrows);
+}
+
+int bdgx_combine_cols(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
+                      const double *dC, void *dY, uint64_t ldy, const int *cols, uint64_t ncols,
+                      uint64_t nrows) {
+  if (!c || !dA || !dC || !dY || (!cols && ncols)) return BDGX_EINVAL;
+  return combine_cols_dispatch(c, prec, dA, lda, k, dC, dY, ldy, cols, ncols, nrows);
 }
 
 int bdgx_residual(bdgx_ctx *c, int prec, const void *dAX, const void *dX, const void *dMX,

@@ -117,6 +117,7 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
   b->lp_window = 30;
   b->lp_factor = 0.5;
   b->graphs = 1;
+  b->freeze_tol = 1e-6;
   return b;
 }
 
@@ -607,6 +608,7 @@ int bdg_solve(bdg_t *b) {
   cfg.lp_window = b->lp_window;
   cfg.lp_factor = b->lp_factor;
   cfg.graphs = b->graphs;
+  cfg.freeze_tol = b->freeze_tol;
   lobpcg_result r;
   memset(&r, 0, sizeof(r));
   r.w2 = (double *)xcalloc(m, sizeof(double));
@@ -628,6 +630,7 @@ int bdg_solve(bdg_t *b) {
   b->stats.learned_iterations = (uint64_t)r.learned_its;
   b->stats.learned_fallback_at = r.learned_fallback_at;
   b->stats.graph_launches = r.graph_launches;
+  b->stats.frozen_skips = r.frozen_skips;
   b->stats.graph_captures = r.graph_captures;
   b->stats.restarts = (uint64_t)r.restarts;
   b->stats.seconds_rr = r.rr_seconds;
@@ -850,6 +853,7 @@ void bdg_set_mixed_params(bdg_t *b, f64 img_tol, uint64_t refresh_every) {
 }
 void bdg_set_verbosity(bdg_t *b, int level) { b->verbosity = level; }
 void bdg_set_graph_capture(bdg_t *b, int on) { b->graphs = on ? 1 : 0; }
+void bdg_set_freeze(bdg_t *b, f64 tol) { b->freeze_tol = tol > 0 ? tol : 0.0; }
 void bdg_set_learned_fallback(bdg_t *b, int window, f64 factor) {
   b->lp_window = window > 0 ? window : 0;
   b->lp_factor = factor;

@@ -69,6 +69,7 @@ struct bdg_t {
   int lp_window;
   double lp_factor;
   int graphs; /* replay the per-iteration device chain as HIP graphs (default 1) */
+  double freeze_tol; /* converged columns moved less than this keep their vectors */
   int verbosity;
   /* results */
   double *eigvals;
@@ -121,6 +122,8 @@ typedef struct {
   double lp_factor;      /*   not dropped below lp_factor x its value lp_window   */
                          /*   iterations earlier                                   */
   int graphs;            /* replay the per-iteration device chain as HIP graphs   */
+  double freeze_tol;     /* converged columns whose update is below this keep     */
+                         /*   their vectors (0: every column updated)             */
 } lobpcg_cfg;
 
 typedef struct {
@@ -132,6 +135,7 @@ typedef struct {
   int learned_its, learned_fallback_at;
   uint64_t graph_launches, graph_captures;
   double capture_seconds; /* host time spent capturing and instantiating graphs */
+  uint64_t frozen_skips;  /* block columns the update left unwritten */
   double rr_seconds;
   int status;
 } lobpcg_result;

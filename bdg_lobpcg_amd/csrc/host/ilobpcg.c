@@ -183,6 +183,58 @@ static void project_grams(const gram3 *gp, int kprev, const double *hCp, int q, 
   }
 }
 
+/* Frozen columns: a converged column j whose Rayleigh-Ritz coefficient
+ * column c_j lies within tol of a unit multiple of e_j, relative to the
+ * column (||V (c_j - s e_j)|| / ||V e_j||, s = c_jj / |c_jj|, from the
+ * Euclidean Gram), keeps its vector: c_j := e_j, and w^2_j becomes its
+ * Rayleigh quotient.  Its residual is then the one last measured.  fz[j] = 1
+ * marks the frozen columns; returns their number. */
+static int freeze_cols(const gram3 *G, int k, int m, int w, const int *conv, double tol,
+                       double *hC, double *w2, int *fz) {
+  int n = 0;
+  double *d = (double *)malloc(sizeof(double) * (size_t)k * w);
+  for (int j = 0; j < m; ++j) {
+    fz[j] = 0;
+    if (!conv[j]) continue;
+    double *c = hC + (size_t)j * k * w;
+    double q = 0.0;
+    if (w == 1) {
+      const double s = c[j] >= 0 ? 1.0 : -1.0;
+      for (int i = 0; i < k; ++i) d[i] = c[i] - (i == j ? s : 0.0);
+      for (int l = 0; l < k; ++l) {
+        const double *g = G->V + (size_t)l * k;
+        double t = 0.0;
+        for (int i = 0; i < k; ++i) t += g[i] * d[i];
+        q += d[l] * t;
+      }
+    } else {
+      const double complex cjj = c[2 * j] + I * c[2 * j + 1];
+      const double a = cabs(cjj);
+      if (!(a > 0)) continue;
+      const double complex s = cjj / a;
+      double complex *dz = (double complex *)d;
+      for (int i = 0; i < k; ++i) dz[i] = (c[2 * i] + I * c[2 * i + 1]) - (i == j ? s : 0.0);
+      double complex qz = 0.0;
+      for (int l = 0; l < k; ++l) {
+        double complex t = 0.0;
+        for (int i = 0; i < k; ++i) t += conj(dz[i]) * getz(G->V, k, i, l, w);
+        qz += t * dz[l];
+      }
+      q = creal(qz);
+    }
+    const double gjj = creal(getz(G->V, k, j, j, w));
+    const double gm = creal(getz(G->M, k, j, j, w));
+    if (!(gjj > 0) || !(gm != 0) || !(q <= tol * tol * gjj)) continue;
+    memset(c, 0, sizeof(double) * (size_t)k * w);
+    c[(size_t)j * w] = 1.0;
+    w2[j] = creal(getz(G->A, k, j, j, w)) / gm;
+    fz[j] = 1;
+    ++n;
+  }
+  free(d);
+  return n;
+}
+
 /* full device Grams on V[:, 0:k].  Product form: V^H V, V^H MV, MV^H KMV.
  * Standard form (std != 0, the "M" images hold K V and the "KM" images K^2 V):
  * V^H V, V^H V, V^H K^2 V. */
@@ -547,6 +599,13 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   res->graph_captures = 0;
   res->capture_seconds = 0.0;
   double *worst_hist = (double *)calloc((size_t)cfg->maxiter + 2, sizeof(double));
+  /* frozen columns: fz marks this iteration's; same[j] says that both block
+   * buffers hold the same column j (then its update need not be written) */
+  int *fz = (int *)calloc((size_t)m, sizeof(int));
+  unsigned char *same = (unsigned char *)calloc((size_t)m, 1);
+  int *wcols = (int *)malloc(sizeof(int) * (size_t)(2 * m));
+  double *hCw = (double *)malloc(sizeof(double) * (size_t)kmax * BDGX_COMBINE_COLS_MAX * w);
+  res->frozen_skips = 0;
 
   /* launch shapes of the FFT kernels for this grid and block width (once per
    * context and precision) */
@@ -619,6 +678,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
          (cfg->refresh_every > 0 && it - refresh_it >= cfg->refresh_every))) {
       if (st.allc || st.worst <= cfg->img_tol) img_done = 1;
       TRY(refresh_images(ctx, cfg, &A, res));
+      memset(same, 0, (size_t)m);
       fresh = 1;
       gvalid = 0;
       refresh_it = it;
@@ -789,8 +849,30 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       for (int i = 0; i < ks; ++i) setz(hCn, k, sel[i], j, w, getz(hYs, ks, i, j, w));
     for (int j = 0; j < nact; ++j)
       for (int i = m; i < k; ++i) setz(hCn, k, i, m + j, w, getz(hCn, k, i, act[j], w));
-    TRY(bdgx_memcpy_h2d(ctx, dC, hCn, sizeof(double) * (size_t)k * nc * w));
-    TRY(combine3(ctx, BDGX_FP64, &A, 0, (uint64_t)k, dC, &B, 0, (uint64_t)nc, S, es));
+    if (cfg->freeze_tol > 0)
+      freeze_cols(&G, k, m, w, st.conv, cfg->freeze_tol, hCn, w2, fz);
+    /* write the columns that change, and the frozen ones that the other
+     * buffer does not hold yet (a unit coefficient column copies exactly) */
+    int nw = 0;
+    for (int j = 0; j < nc; ++j)
+      if (j >= m || !fz[j] || !same[j]) wcols[nw++] = j;
+    if (nw < nc && nw <= BDGX_COMBINE_COLS_MAX) {
+      for (int o = 0; o < nw; ++o)
+        memcpy(hCw + (size_t)o * k * w, hCn + (size_t)wcols[o] * k * w,
+               sizeof(double) * (size_t)k * w);
+      TRY(bdgx_memcpy_h2d(ctx, dC, hCw, sizeof(double) * (size_t)k * nw * w));
+      TRY(bdgx_combine_cols(ctx, BDGX_FP64, A.V, S, (uint64_t)k, dC, B.V, S, wcols,
+                            (uint64_t)nw, S));
+      TRY(bdgx_combine_cols(ctx, BDGX_FP64, A.MV, S, (uint64_t)k, dC, B.MV, S, wcols,
+                            (uint64_t)nw, S));
+      TRY(bdgx_combine_cols(ctx, BDGX_FP64, A.KMV, S, (uint64_t)k, dC, B.KMV, S, wcols,
+                            (uint64_t)nw, S));
+      res->frozen_skips += (uint64_t)(nc - nw);
+    } else {
+      TRY(bdgx_memcpy_h2d(ctx, dC, hCn, sizeof(double) * (size_t)k * nc * w));
+      TRY(combine3(ctx, BDGX_FP64, &A, 0, (uint64_t)k, dC, &B, 0, (uint64_t)nc, S, es));
+    }
+    for (int j = 0; j < m; ++j) same[j] = (unsigned char)fz[j];
     memcpy(Gp.V, G.V, sizeof(double) * (size_t)k * k * w);
     memcpy(Gp.M, G.M, sizeof(double) * (size_t)k * k * w);
     memcpy(Gp.A, G.A, sizeof(double) * (size_t)k * k * w);
@@ -861,5 +943,9 @@ done:
   free(st.sel);
   free(st.lsel);
   free(worst_hist);
+  free(fz);
+  free(same);
+  free(wcols);
+  free(hCw);
   return rc;
 }

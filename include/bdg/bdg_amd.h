@@ -37,6 +37,7 @@ typedef struct {
   double seconds_total;         /* bdg_solve wall clock                        */
   double seconds_rr;            /* host Rayleigh-Ritz time                     */
   int status;                   /* 0 ok, else a BDGX_E* code or -100 (RR)     */
+  uint64_t frozen_skips;        /* block columns the update left unwritten     */
 } bdg_solve_stats_t;
 
 void bdg_get_solve_stats(const bdg_t *bdg, bdg_solve_stats_t *out);
@@ -77,6 +78,12 @@ void bdg_set_learned_fallback(bdg_t *bdg, int window, f64 factor);
  * columns) as captured HIP graphs (default on; 0 runs every launch eagerly).
  * Results are the same either way. */
 void bdg_set_graph_capture(bdg_t *bdg, int on);
+
+/* Converged columns whose Rayleigh-Ritz update moves them by less than tol
+ * (relative, Euclidean, up to a phase) keep their vectors, and the block
+ * update writes only the columns that change (default 1e-6; 0 updates every
+ * column every iteration, as plain soft locking). */
+void bdg_set_freeze(bdg_t *bdg, f64 tol);
 
 /* Solve n independent, fully set-up problems concurrently: worker threads
  * (nthreads, 0 = one per problem) take problems in order and call bdg_solve

@@ -108,6 +108,17 @@ int bdgx_gram_pair(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64
 int bdgx_combine(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda,
                  uint64_t k, const double *dC, void *dY, uint64_t ldy,
                  uint64_t m, uint64_t nrows);
+#define BDGX_COMBINE_COLS_MAX 8
+/* dY[:, cols[j]] = dA[:, 0:k] * C[:, j] for j < ncols <= BDGX_COMBINE_COLS_MAX
+ * (C is k x ncols
+ * in device memory, ldc = k; cols a host array of distinct column indices):
+ * the update of a block in which most columns keep their vectors, written
+ * only where it changes.  FP64 only; the sum over the k source columns runs
+ * in a fixed order and a unit coefficient column copies its source column
+ * exactly.  dY must not alias dA. */
+int bdgx_combine_cols(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda,
+                      uint64_t k, const double *dC, void *dY, uint64_t ldy,
+                      const int *cols, uint64_t ncols, uint64_t nrows);
 /* R[:, j] = AX[:, j] - lambda_j * X'[:, j] with X' = Swap(X) when swap_x,
  * else X.  dnorms2 (3 * ncols FP64, device) receives ||R_j||^2, ||X_j||^2 and
  * ||MX_j||^2 (the last only when dMX != NULL, else 0).  dR may be NULL (norms

@@ -288,3 +288,46 @@ def test_gram_pair_equals_two_grams(prec):
         dt = np.float32 if prec else np.float64
         ref = A.astype(dt).astype(np.float64).T @ np.hstack([B1, B2]).astype(dt).astype(np.float64)
         assert relerr(G, ref) < 1e-12
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_combine_cols_writes_only_listed_columns(cplx):
+    """bdgx_combine_cols (the solver's update when most columns keep their
+    vectors): listed output columns equal A @ C (numpy, FP64), the other
+    columns of the output block are untouched, a unit coefficient column
+    copies its source column bit for bit, and the sum order is fixed (two
+    calls agree bit for bit).  Limits: FP32, more than 8 columns and
+    repeated column indices are rejected."""
+    import ctypes as Ct
+    import bdg_lobpcg_amd as B
+    N, L = (16, 12, 10), (8.0, 7.0, 6.0)
+    d = B.DeviceContext(N, L, cplx)
+    rng = np.random.default_rng(11 + cplx)
+    dt = np.complex128 if cplx else np.float64
+
+    def rnd(*shape):
+        a = rng.standard_normal(shape)
+        return (a + 1j * rng.standard_normal(shape)) if cplx else a
+
+    for rows, k, cols in ((1920, 36, [30, 31, 32, 33]), (1917, 5, [2]),
+                          (1920, 96, [0, 7, 12, 40, 41, 42, 60, 63])):
+        A = rnd(rows, k).astype(dt)
+        Cm = rnd(k, len(cols)).astype(dt)
+        Cm[:, 0] = 0
+        Cm[cols[0] % k, 0] = 1.0
+        Y0 = rnd(rows, 64).astype(dt)
+        Y = d.combine_cols(A, Cm, cols, Y0)
+        Y2 = d.combine_cols(A, Cm, cols, Y0)
+        np.testing.assert_array_equal(Y, Y2)
+        ref = A @ Cm
+        assert relerr(Y[:, cols], ref) < 1e-13
+        np.testing.assert_array_equal(Y[:, cols[0]], A[:, cols[0] % k])
+        rest = [j for j in range(64) if j not in cols]
+        np.testing.assert_array_equal(Y[:, rest], Y0[:, rest])
+    lib = B.lib()
+    cols = (Ct.c_int * 9)(*range(9))
+    dup = (Ct.c_int * 2)(3, 3)
+    buf = B.DeviceBuffer(8 * 64 * 16)
+    assert lib.bdgx_combine_cols(d.h, B.FP32, buf.ptr, 8, 4, buf.ptr, buf.ptr, 8, cols, 2, 8) != 0
+    assert lib.bdgx_combine_cols(d.h, B.FP64, buf.ptr, 8, 4, buf.ptr, buf.ptr, 8, cols, 9, 8) != 0
+    assert lib.bdgx_combine_cols(d.h, B.FP64, buf.ptr, 8, 4, buf.ptr, buf.ptr, 8, dup, 2, 8) != 0

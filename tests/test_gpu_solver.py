@@ -151,3 +151,41 @@ def test_device_start_block_gives_identical_solve():
     assert i0 == i1
     np.testing.assert_array_equal(w0, w1)
     np.testing.assert_array_equal(v0, v1)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_frozen_columns_match_soft_locking(cplx):
+    """Converged columns that the Rayleigh-Ritz leaves in place keep their
+    vectors and the block update skips them (bdg_set_freeze, default on):
+    same eigenvalues as plain soft locking (every column updated) to 1e-9,
+    every mode converged in both, and skipped column writes only with the
+    freeze on."""
+    import bdg_lobpcg_amd as B
+    N, L = (24, 24, 24), (12.0, 12.0, 12.0)
+    c, f = physics(N, L, cplx, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
+    nev, m = 6, 8
+    X0 = c.start_block_uniform(42, m)
+    out = {}
+    for tol in (0.0, 1e-6):
+        p = B.Problem(N, L, cplx)
+        p.set_system()
+        p.set_trap_harmonic((1.0, 1.0, 1.0))
+        p.set_wavefunction(f["wf"])
+        p.set_contact(20.0)
+        p.set_dipolar(10.0, (0.0, 0.0, 1.0), f["R"])
+        p.set_mu(1.0)
+        p.set_solver_params(nev, m, 400, 1e-9)
+        p.set_init_custom_block(X0)
+        p.set_freeze(tol)
+        assert p.solve() == 0
+        out[tol] = (p.eigenvalues(nev), p.zero_modes(nev), p.stats(), p.converged(),
+                    p.residuals(nev))
+        p.close()
+    (w0, z0, s0, c0, r0), (w1, z1, s1, c1, r1) = out[0.0], out[1e-6]
+    print("iterations", s0["iterations"], s1["iterations"], "skips", s1["frozen_skips"],
+          flush=True)
+    assert c0 >= nev and c1 >= nev
+    assert s0["frozen_skips"] == 0 and s1["frozen_skips"] > 0
+    nz = ~(z0 | z1)
+    np.testing.assert_allclose(w1[nz], w0[nz], rtol=1e-9)
+    assert np.all(r1[nz] < 1e-9)
