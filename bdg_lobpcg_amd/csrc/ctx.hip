@@ -779,6 +779,11 @@ int bdgx_memcpy_h2d(bdgx_ctx *c, void *d, const void *s, size_t n) {
   BDGX_HIP(hipStreamSynchronize(c->stream));
   return BDGX_OK;
 }
+int bdgx_memcpy_h2d_async(bdgx_ctx *c, void *d, const void *s, size_t n) {
+  if (!c || (n && (!d || !s))) return BDGX_EINVAL;
+  BDGX_HIP(hipMemcpyAsync(d, s, n, hipMemcpyHostToDevice, c->stream));
+  return BDGX_OK;
+}
 int bdgx_memcpy_d2h(bdgx_ctx *c, void *d, const void *s, size_t n) {
   BDGX_HIP(hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, c->stream));
   BDGX_HIP(hipStreamSynchronize(c->stream));

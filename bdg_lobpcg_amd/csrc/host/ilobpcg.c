@@ -183,53 +183,78 @@ static void project_grams(const gram3 *gp, int kprev, const double *hCp, int q, 
   }
 }
 
-/* Frozen columns: a converged column j whose Rayleigh-Ritz coefficient
- * column c_j lies within tol of a unit multiple of e_j, relative to the
- * column (||V (c_j - s e_j)|| / ||V e_j||, s = c_jj / |c_jj|, from the
- * Euclidean Gram), keeps its vector: c_j := e_j, and w^2_j becomes its
- * Rayleigh quotient.  Its residual is then the one last measured.  fz[j] = 1
- * marks the frozen columns; returns their number. */
+/* Frozen columns.  A cluster J of converged columns with equal Ritz values
+ * (relative difference <= 1e-11 in w^2: a degenerate multiplet, which the
+ * Rayleigh-Ritz may rotate freely, or a single column) whose new Ritz
+ * vectors V c_j all lie within tol of span(X_J) (relative Euclidean norm of
+ * the part of V c_j outside the rows J, from the Euclidean Gram) keeps its
+ * vectors: c_j := e_j, and w^2_j becomes the Rayleigh quotient of X_j; the
+ * residuals are then the ones last measured.  fz[j] = 1 marks the frozen
+ * columns; returns their number. */
+static double outside_part(const gram3 *G, int k, int w, const double *c, int j0, int j1,
+                           double *d) {
+  /* ||V d||^2 / ||V c||^2 with d = c outside the rows [j0, j1) */
+  double qd = 0.0, qc = 0.0;
+  if (w == 1) {
+    for (int i = 0; i < k; ++i) d[i] = (i >= j0 && i < j1) ? 0.0 : c[i];
+    for (int l = 0; l < k; ++l) {
+      const double *g = G->V + (size_t)l * k;
+      double td = 0.0, tc = 0.0;
+      for (int i = 0; i < k; ++i) {
+        td += g[i] * d[i];
+        tc += g[i] * c[i];
+      }
+      qd += d[l] * td;
+      qc += c[l] * tc;
+    }
+  } else {
+    double complex *dz = (double complex *)d;
+    const double complex *cz = (const double complex *)c;
+    const double complex *gz = (const double complex *)G->V;
+    for (int i = 0; i < k; ++i) dz[i] = (i >= j0 && i < j1) ? 0.0 : cz[i];
+    double complex sd = 0.0, sc = 0.0;
+    for (int l = 0; l < k; ++l) {
+      const double complex *g = gz + (size_t)l * k;
+      double complex td = 0.0, tc = 0.0;
+      for (int i = 0; i < k; ++i) {
+        td += conj(dz[i]) * g[i];
+        tc += conj(cz[i]) * g[i];
+      }
+      sd += td * dz[l];
+      sc += tc * cz[l];
+    }
+    qd = creal(sd);
+    qc = creal(sc);
+  }
+  return qc > 0 ? qd / qc : 1.0;
+}
+
 static int freeze_cols(const gram3 *G, int k, int m, int w, const int *conv, double tol,
                        double *hC, double *w2, int *fz) {
   int n = 0;
   double *d = (double *)malloc(sizeof(double) * (size_t)k * w);
-  for (int j = 0; j < m; ++j) {
-    fz[j] = 0;
-    if (!conv[j]) continue;
-    double *c = hC + (size_t)j * k * w;
-    double q = 0.0;
-    if (w == 1) {
-      const double s = c[j] >= 0 ? 1.0 : -1.0;
-      for (int i = 0; i < k; ++i) d[i] = c[i] - (i == j ? s : 0.0);
-      for (int l = 0; l < k; ++l) {
-        const double *g = G->V + (size_t)l * k;
-        double t = 0.0;
-        for (int i = 0; i < k; ++i) t += g[i] * d[i];
-        q += d[l] * t;
-      }
-    } else {
-      const double complex cjj = c[2 * j] + I * c[2 * j + 1];
-      const double a = cabs(cjj);
-      if (!(a > 0)) continue;
-      const double complex s = cjj / a;
-      double complex *dz = (double complex *)d;
-      for (int i = 0; i < k; ++i) dz[i] = (c[2 * i] + I * c[2 * i + 1]) - (i == j ? s : 0.0);
-      double complex qz = 0.0;
-      for (int l = 0; l < k; ++l) {
-        double complex t = 0.0;
-        for (int i = 0; i < k; ++i) t += conj(dz[i]) * getz(G->V, k, i, l, w);
-        qz += t * dz[l];
-      }
-      q = creal(qz);
+  for (int j = 0; j < m; ++j) fz[j] = 0;
+  for (int j0 = 0; j0 < m;) {
+    int j1 = j0 + 1;
+    while (j1 < m && fabs(w2[j1] - w2[j0]) <= 1e-11 * fmax(1.0, fabs(w2[j0]))) ++j1;
+    int ok = 1;
+    for (int j = j0; j < j1 && ok; ++j)
+      ok = conv[j] && outside_part(G, k, w, hC + (size_t)j * k * w, j0, j1, d) <= tol * tol;
+    for (int j = j0; j < j1 && ok; ++j) {
+      const double gm = creal(getz(G->M, k, j, j, w));
+      if (!(gm != 0)) ok = 0;
     }
-    const double gjj = creal(getz(G->V, k, j, j, w));
-    const double gm = creal(getz(G->M, k, j, j, w));
-    if (!(gjj > 0) || !(gm != 0) || !(q <= tol * tol * gjj)) continue;
-    memset(c, 0, sizeof(double) * (size_t)k * w);
-    c[(size_t)j * w] = 1.0;
-    w2[j] = creal(getz(G->A, k, j, j, w)) / gm;
-    fz[j] = 1;
-    ++n;
+    if (ok) {
+      for (int j = j0; j < j1; ++j) {
+        double *c = hC + (size_t)j * k * w;
+        memset(c, 0, sizeof(double) * (size_t)k * w);
+        c[(size_t)j * w] = 1.0;
+        w2[j] = creal(getz(G->A, k, j, j, w)) / creal(getz(G->M, k, j, j, w));
+        fz[j] = 1;
+        ++n;
+      }
+    }
+    j0 = j1;
   }
   free(d);
   return n;
@@ -250,23 +275,64 @@ static int grams_full_dev(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k
   return rc;
 }
 
-static int grams_full_host(bdgx_ctx *ctx, int std, int k, int w, const double *dG, gram3 *h) {
+/* Host staging of the per-iteration small copies (eigenvalues and
+ * coefficients up, norms and Grams down).  Page-locked when available: the
+ * uploads are then queued without waiting for the device, so the next
+ * kernels follow the running ones directly.  Each buffer is rewritten only
+ * after a later synchronous download, by which time its upload has run. */
+typedef struct {
+  double *lam, *C, *n2, *G;
+  void *block;
+  int pinned;
+} stage_t;
+
+static int stage_init(stage_t *sg, int m, int kmax, int w) {
+  const size_t nl = (size_t)m, nC = (size_t)kmax * 2 * m * w, nn = 3 * (size_t)m,
+               nG = 3 * (size_t)kmax * kmax * w, tot = nl + nC + nn + nG;
+  sg->pinned = bdgx_host_alloc(&sg->block, sizeof(double) * tot) == 0;
+  if (!sg->pinned) sg->block = malloc(sizeof(double) * tot);
+  if (!sg->block) return -1;
+  sg->lam = (double *)sg->block;
+  sg->C = sg->lam + nl;
+  sg->n2 = sg->C + nC;
+  sg->G = sg->n2 + nn;
+  return 0;
+}
+
+static void stage_free(stage_t *sg) {
+  if (!sg->block) return;
+  if (sg->pinned)
+    bdgx_host_free(sg->block);
+  else
+    free(sg->block);
+  sg->block = NULL;
+}
+
+/* upload bytes of src through the staging buffer buf (src may be buf) */
+static int stage_up(bdgx_ctx *ctx, const stage_t *sg, double *buf, void *dst, const double *src,
+                    size_t bytes) {
+  if (src != buf) memcpy(buf, src, bytes);
+  return sg->pinned ? bdgx_memcpy_h2d_async(ctx, dst, buf, bytes)
+                    : bdgx_memcpy_h2d(ctx, dst, buf, bytes);
+}
+
+static int grams_full_host(bdgx_ctx *ctx, const stage_t *sg, int std, int k, int w,
+                           const double *dG, gram3 *h) {
   const size_t gsz = (size_t)k * k * w;
-  double *tmp = (double *)malloc(sizeof(double) * 3 * gsz);
+  double *tmp = sg->G;
   int rc = bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * 3 * gsz);
   if (!rc) {
     memcpy(h->V, tmp, sizeof(double) * gsz);
     memcpy(h->M, std ? tmp : tmp + gsz, sizeof(double) * gsz);
     memcpy(h->A, tmp + 2 * gsz, sizeof(double) * gsz);
   }
-  free(tmp);
   return rc;
 }
 
-static int grams_full(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k, uint64_t S, int w,
-                      double *dG, gram3 *h) {
+static int grams_full(bdgx_ctx *ctx, const stage_t *sg, int prec, int std, const blk3 *b, int k,
+                      uint64_t S, int w, double *dG, gram3 *h) {
   int rc = grams_full_dev(ctx, prec, std, b, k, S, w, dG);
-  return rc ? rc : grams_full_host(ctx, std, k, w, dG, h);
+  return rc ? rc : grams_full_host(ctx, sg, std, k, w, dG, h);
 }
 
 /* device Grams of V[:, 0:k] against the columns [c0, c0+nc) of V; written
@@ -292,10 +358,10 @@ static int grams_cols_dev(bdgx_ctx *ctx, int prec, int std, const blk3 *b, int k
   return rc;
 }
 
-static int grams_cols_host(bdgx_ctx *ctx, int std, int k, int c0, int nc, int w,
-                           const double *dG, gram3 *h) {
+static int grams_cols_host(bdgx_ctx *ctx, const stage_t *sg, int std, int k, int c0, int nc,
+                           int w, const double *dG, gram3 *h) {
   const size_t csz = (size_t)k * nc * w;
-  double *tmp = (double *)malloc(sizeof(double) * 3 * csz);
+  double *tmp = sg->G;
   int rc = bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * 3 * csz);
   if (!rc) {
     if (std) memcpy(tmp + csz, tmp, sizeof(double) * csz);
@@ -308,7 +374,6 @@ static int grams_cols_host(bdgx_ctx *ctx, int std, int k, int c0, int nc, int w,
           setz(dst[g], k, c0 + j, i, w, conj(v));
         }
   }
-  free(tmp);
   return rc;
 }
 
@@ -327,21 +392,22 @@ typedef struct {
  * every column, dR column j = column j.  Partial (nsel > 0): only the
  * columns sel[0..nsel), packed into dR columns 0..nsel-1; the other columns
  * keep the state of the last full evaluation (they had converged). */
-static int eval_residuals(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const blk3 *A, const double *w2,
-                          const int *zf, double *dlam, double *dn2, double *hn2, void *dR,
-                          res_state *st, int nsel) {
+static int eval_residuals(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const stage_t *sg,
+                          const blk3 *A, const double *w2, const int *zf, double *dlam,
+                          double *dn2, void *dR, res_state *st, int nsel) {
   const uint64_t S = cfg->S;
   const int m = cfg->m, nev = cfg->nev;
+  double *hn2 = sg->n2;
   int rc;
   if (nsel == 0) {
-    rc = bdgx_memcpy_h2d(ctx, dlam, w2, sizeof(double) * (size_t)m);
+    rc = stage_up(ctx, sg, sg->lam, dlam, w2, sizeof(double) * (size_t)m);
     if (!rc)
       rc = bdgx_residual(ctx, BDGX_FP64, A->KMV, A->V, A->MV, S, (uint64_t)m, S, dlam, 0, dR, S,
                          dn2);
     if (!rc) rc = bdgx_memcpy_d2h(ctx, hn2, dn2, sizeof(double) * 3 * (size_t)m);
   } else {
-    for (int i = 0; i < nsel; ++i) st->lsel[i] = w2[st->sel[i]];
-    rc = bdgx_memcpy_h2d(ctx, dlam, st->lsel, sizeof(double) * (size_t)nsel);
+    for (int i = 0; i < nsel; ++i) sg->lam[i] = w2[st->sel[i]];
+    rc = stage_up(ctx, sg, sg->lam, dlam, sg->lam, sizeof(double) * (size_t)nsel);
     if (!rc)
       rc = bdgx_residual_cols(ctx, BDGX_FP64, A->KMV, A->V, A->MV, S, st->sel, (uint64_t)nsel, S,
                               dlam, 0, dR, S, dn2);
@@ -567,7 +633,6 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   double *hCp = (double *)calloc((size_t)kmax * 2 * m * w, sizeof(double)); /* previous */
   double *w2 = (double *)calloc((size_t)m, sizeof(double));
   int *zf = (int *)calloc((size_t)m, sizeof(int));
-  double *hn2 = (double *)malloc(sizeof(double) * 3 * (size_t)m);
   int *sel = (int *)malloc(sizeof(int) * (size_t)kmax);
   res_state st;
   st.resv = (double *)malloc(sizeof(double) * (size_t)m);
@@ -592,6 +657,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   res->learned_its = 0;
   res->learned_fallback_at = 0;
   int lp_on = cfg->learned;
+  int nfrozen = 0; /* columns frozen by the last update */
   gslot gslots[NGSLOT];
   unsigned long gclock = 0;
   memset(gslots, 0, sizeof(gslots));
@@ -606,6 +672,12 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
   int *wcols = (int *)malloc(sizeof(int) * (size_t)(2 * m));
   double *hCw = (double *)malloc(sizeof(double) * (size_t)kmax * BDGX_COMBINE_COLS_MAX * w);
   res->frozen_skips = 0;
+  stage_t sg;
+  memset(&sg, 0, sizeof(sg));
+  if (stage_init(&sg, m, kmax, w)) {
+    rc = BDGX_ENOMEM;
+    goto done;
+  }
 
   /* launch shapes of the FFT kernels for this grid and block width (once per
    * context and precision) */
@@ -631,7 +703,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     TRY(bdgx_apply(ctx, opM, BDGX_FP64, A.V, A.MV, (uint64_t)m, S));
     TRY(bdgx_apply(ctx, BDGX_OP_K, BDGX_FP64, A.MV, A.KMV, (uint64_t)m, S));
     res->applies += (uint64_t)m;
-    TRY(grams_full(ctx, BDGX_FP64, std, &A, m, S, w, dG, &G));
+    TRY(grams_full(ctx, &sg, BDGX_FP64, std, &A, m, S, w, dG, &G));
     const double t0 = bdg_wtime();
     const int got = dn_rayleigh_ritz(m, m, cplx, G.V, G.M, G.A, w2, hY, zf);
     trr += bdg_wtime() - t0;
@@ -663,11 +735,11 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       nsel = st.nact;
       for (int i = 0; i < nsel; ++i) st.sel[i] = st.act[i];
     }
-    TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st, nsel));
+    TRY(eval_residuals(ctx, cfg, &sg, &A, w2, zf, dlam, dn2, dR, &st, nsel));
     if (nsel == 0) {
       last_full = it;
     } else if (st.allc) {
-      TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st, 0));
+      TRY(eval_residuals(ctx, cfg, &sg, &A, w2, zf, dlam, dn2, dR, &st, 0));
       last_full = it;
     }
     /* mixed: FP64 images of X every refresh_every iterations while new
@@ -682,7 +754,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       fresh = 1;
       gvalid = 0;
       refresh_it = it;
-      TRY(eval_residuals(ctx, cfg, &A, w2, zf, dlam, dn2, hn2, dR, &st, 0));
+      TRY(eval_residuals(ctx, cfg, &sg, &A, w2, zf, dlam, dn2, dR, &st, 0));
       last_full = it;
     }
     const int nact = st.nact;
@@ -691,6 +763,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       fprintf(stderr, "[bdg] it %d t %.3f w0 %.10g wlast %.10g worst res %.3e active %d%s", it,
               bdg_wtime() - t_start, signed_sqrt(w2[0]), signed_sqrt(w2[nev - 1]), st.worst,
               nact, fresh ? "" : " (fp32 images)");
+      if (nfrozen) fprintf(stderr, " frozen %d", nfrozen);
       if (nact <= 8) {
         fprintf(stderr, " [");
         for (int j = 0; j < nact; ++j)
@@ -796,9 +869,9 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
     /* Gram set of [X | P | W] on the host */
     if (full) {
       since_full = 0;
-      TRY(grams_full_host(ctx, std, k, w, dG, &G));
+      TRY(grams_full_host(ctx, &sg, std, k, w, dG, &G));
     } else {
-      TRY(grams_cols_host(ctx, std, k, wc, nact, w, dG, &G));
+      TRY(grams_cols_host(ctx, &sg, std, k, wc, nact, w, dG, &G));
     }
     /* Rayleigh-Ritz on a column selection: all, or without P when [X P W]
      * is numerically dependent */
@@ -825,7 +898,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
                      colp(A.KMV, (uint64_t)wc, S, es), (uint64_t)nact, S));
       res->applies += (uint64_t)nact;
       res->precond_fallbacks++;
-      TRY(grams_full(ctx, BDGX_FP64, std, &A, k, S, w, dG, &G));
+      TRY(grams_full(ctx, &sg, BDGX_FP64, std, &A, k, S, w, dG, &G));
       since_full = 0;
       ks = 0;
       for (int i = 0; i < m; ++i) sel[ks++] = i;
@@ -849,8 +922,9 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       for (int i = 0; i < ks; ++i) setz(hCn, k, sel[i], j, w, getz(hYs, ks, i, j, w));
     for (int j = 0; j < nact; ++j)
       for (int i = m; i < k; ++i) setz(hCn, k, i, m + j, w, getz(hCn, k, i, act[j], w));
-    if (cfg->freeze_tol > 0)
-      freeze_cols(&G, k, m, w, st.conv, cfg->freeze_tol, hCn, w2, fz);
+    nfrozen = cfg->freeze_tol > 0 ? freeze_cols(&G, k, m, w, st.conv, cfg->freeze_tol, hCn,
+                                                w2, fz)
+                                   : 0;
     /* write the columns that change, and the frozen ones that the other
      * buffer does not hold yet (a unit coefficient column copies exactly) */
     int nw = 0;
@@ -860,7 +934,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
       for (int o = 0; o < nw; ++o)
         memcpy(hCw + (size_t)o * k * w, hCn + (size_t)wcols[o] * k * w,
                sizeof(double) * (size_t)k * w);
-      TRY(bdgx_memcpy_h2d(ctx, dC, hCw, sizeof(double) * (size_t)k * nw * w));
+      TRY(stage_up(ctx, &sg, sg.C, dC, hCw, sizeof(double) * (size_t)k * nw * w));
       TRY(bdgx_combine_cols(ctx, BDGX_FP64, A.V, S, (uint64_t)k, dC, B.V, S, wcols,
                             (uint64_t)nw, S));
       TRY(bdgx_combine_cols(ctx, BDGX_FP64, A.MV, S, (uint64_t)k, dC, B.MV, S, wcols,
@@ -869,7 +943,7 @@ int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lob
                             (uint64_t)nw, S));
       res->frozen_skips += (uint64_t)(nc - nw);
     } else {
-      TRY(bdgx_memcpy_h2d(ctx, dC, hCn, sizeof(double) * (size_t)k * nc * w));
+      TRY(stage_up(ctx, &sg, sg.C, dC, hCn, sizeof(double) * (size_t)k * nc * w));
       TRY(combine3(ctx, BDGX_FP64, &A, 0, (uint64_t)k, dC, &B, 0, (uint64_t)nc, S, es));
     }
     for (int j = 0; j < m; ++j) same[j] = (unsigned char)fz[j];
@@ -934,7 +1008,7 @@ done:
   free(hCp);
   free(w2);
   free(zf);
-  free(hn2);
+  stage_free(&sg);
   free(sel);
   free(st.resv);
   free(st.conv);

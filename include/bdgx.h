@@ -171,8 +171,13 @@ int bdgx_free(void *dptr);
  * and the device run at the full link rate. */
 int bdgx_host_alloc(void **hptr, size_t bytes);
 int bdgx_host_free(void *hptr);
+/* Copies on the ctx stream; both return after the copy has completed. */
 int bdgx_memcpy_h2d(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bdgx_memcpy_d2h(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Host-to-device copy queued on the ctx stream without waiting: src must be
+ * page-locked (bdgx_host_alloc) and must not change until the stream has
+ * passed the copy (a later bdgx_sync or synchronous copy on this ctx). */
+int bdgx_memcpy_h2d_async(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bdgx_memset(bdgx_ctx *ctx, void *dst, int value, size_t bytes);
 int bdgx_sync(bdgx_ctx *ctx);
 void *bdgx_stream(bdgx_ctx *ctx);          /* the ctx's hipStream_t */
