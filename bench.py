@@ -207,6 +207,33 @@ def blas(cfg, iters=10):
                         GBs=by / ms / 1e6,
                         TFs=fl / ms / 1e9, mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS,
                         rel_err=err))
+    # the update of a block whose other columns are frozen: 36 basis columns
+    # read once, the 4 changing ones written through a column map
+    import ctypes
+    k, cols = 36, [30, 31, 32, 33]
+    Cm = np.asfortranarray(rng.standard_normal((k, len(cols))))
+    dC = d.upload(Cm)
+    dY = B.DeviceBuffer(S * 34 * 8)
+    cidx = (ctypes.c_int * len(cols))(*cols)
+
+    def run():
+        B.check(B.lib().bdgx_combine_cols(d.h, B.FP64, dA.ptr, S, k, dC.ptr, dY.ptr, S, cidx,
+                                          len(cols), S), "combine_cols")
+
+    run()
+    Y = d.download(dY, (S, 34), np.float64)
+    rows = slice(0, S, 997)
+    ref = A[rows, :k] @ Cm
+    err = float(np.abs(Y[rows][:, cols] - ref).max() / np.abs(ref).max())
+    d.timer_begin()
+    for _ in range(iters):
+        run()
+    ms = d.timer_end() / iters
+    by = S * (k + len(cols)) * 8
+    fl = 2.0 * S * k * len(cols)
+    out.append(dict(kernel="combine_cols", shape=f"{k}->{len(cols)} of 34", us=ms * 1e3,
+                    GBs=by / ms / 1e6, TFs=fl / ms / 1e9,
+                    mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS, rel_err=err))
     d.close()
     for r in out:
         print(json.dumps(r), flush=True)
