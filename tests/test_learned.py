@@ -115,3 +115,39 @@ def test_device_zero_theta_is_the_analytic_preconditioner():
     R = rand_block(c.size, 4, False, seed=5)
     ref = c.apply("precondM", c.apply("precondK", R))
     assert relerr(d.lp_apply(R), ref) < 2e-5
+
+
+@pytest.mark.gpu
+def test_trained_network_gain_on_32c():
+    """What the trained network gains over the method without it, on the
+    32^3 dipolar configuration (box [-8,8]^3, g = 100, eps_dd = 0.5, nev =
+    block = 10, tol 1e-8): the network is trained on the GPU on pairs from a
+    related problem (g x 0.75, eps_dd x 0.875), then the same mixed solve runs
+    from the same start block with T_M T_K and with the network.  Both reach
+    the same omega (<= 1e-9 relative, non-zero modes) with every mode
+    converged; the network lowers the training loss and the iteration count."""
+    from bdg_lobpcg_amd.workloads import CONFIGS, ground_state, make_problem, train_learned
+    cfg = CONFIGS["cfg0_32c_d"]
+    theta, secs, info = train_learned(cfg)
+    print(f"training {secs:.1f} s, loss {info['loss_analytic']:.4f} -> "
+          f"{info['loss_learned']:.4f}", flush=True)
+    assert info["loss_learned"] < info["loss_analytic"]
+    psi, mu, _ = ground_state(cfg)
+    out = {}
+    for name, th in (("analytic", None), ("learned", theta)):
+        p = make_problem(cfg, psi, mu, precision=1)
+        if th is not None:
+            p.set_learned(th)
+        assert p.solve() == 0
+        out[name] = (p.eigenvalues(cfg["nev"]), p.zero_modes(cfg["nev"]), p.stats(),
+                     p.converged())
+        p.close()
+    (w0, z0, s0, c0), (w1, z1, s1, c1) = out["analytic"], out["learned"]
+    print(f"iterations: T_M T_K {s0['iterations']}, network {s1['iterations']} "
+          f"(learned iterations {s1['learned_iterations']}, fallback at "
+          f"{s1['learned_fallback_at']})", flush=True)
+    assert c0 >= cfg["nev"] and c1 >= cfg["nev"]
+    nz = ~(z0 | z1)
+    np.testing.assert_allclose(w1[nz], w0[nz], rtol=1e-9)
+    assert s1["learned_iterations"] > 0
+    assert s1["iterations"] < s0["iterations"]
