@@ -47,6 +47,25 @@ def rel_err_modes(w, ref, zero):
     return e_rel, e_abs0
 
 
+def max_cluster_angle(w, A, B, skip, rtol=1e-7):
+    """Largest principal angle (radians) between span(A[:, J]) and span(B[:, J])
+    over the clusters J of equal omega (degenerate modes compared as
+    subspaces), without the zero modes and the cluster holding the last mode
+    (its partners may lie beyond the computed modes)."""
+    from scipy.linalg import subspace_angles
+
+    n, worst, j0 = len(w), 0.0, 0
+    while j0 < n:
+        j1 = j0 + 1
+        while j1 < n and abs(w[j1] - w[j0]) <= rtol * max(1.0, abs(w[j0])):
+            j1 += 1
+        J = [j for j in range(j0, j1) if not skip[j]]
+        if J and j1 < n:
+            worst = max(worst, float(np.max(subspace_angles(A[:, J], B[:, J]))))
+        j0 = j1
+    return worst
+
+
 def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None, tol=None):
     from bdg_lobpcg_amd.workloads import make_problem
 
@@ -72,8 +91,10 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None, tol=None):
             f" converged {p.converged()}/{cfg['nev']}")
         if i >= warmup:
             times.append(t)
+        final = i == warmup + steps - 1
         last = dict(w=w, zero=p.zero_modes(cfg["nev"]), res=p.residuals(cfg["nev"]), stats=st,
-                    conv=p.converged(), arrays=p.arrays() if i == warmup + steps - 1 else None)
+                    conv=p.converged(), arrays=p.arrays() if final else None,
+                    v=p.modes(cfg["nev"])[1] if final else None)
         p.close()
     return float(np.mean(times)), last
 
@@ -384,6 +405,10 @@ def main():
     zero = r64["zero"] | rlr["zero"] | rmx["zero"]
     e_rel, e_abs0 = rel_err_modes(rlr["w"], r64["w"], zero)
     e_acc, _ = rel_err_modes(racc["w"], r64["w"], zero | racc["zero"])
+    # eigenvectors: principal angles against the FP64 modes (v half)
+    ang = {name: max_cluster_angle(r64["w"], r["v"], r64["v"], zero | r["zero"])
+           for name, r in (("mixed", rmx), ("mixed_learned", rlr), ("mixed_learned_at_omega_target", racc))}
+    log("max subspace angle vs FP64 modes: " + ", ".join(f"{k} {v:.2e}" for k, v in ang.items()))
     e_rel_mx, _ = rel_err_modes(rmx["w"], r64["w"], zero)
     log(f"FP64 {t64:.3f} s, mixed {tmx:.3f} s ({t64 / tmx:.2f}x), mixed+learned {tlr:.3f} s "
         f"({t64 / tlr:.2f}x); max rel err vs FP64: mixed {e_rel_mx:.2e}, mixed+learned {e_rel:.2e}")
@@ -429,6 +454,7 @@ def main():
         "speedup_mixed_only": t64 / tmx,
         "mixed_learned_vs_fp64_max_rel_err": e_rel,
         "mixed_vs_fp64_max_rel_err": e_rel_mx,
+        "max_subspace_angle_vs_fp64": ang,
         "goldstone_abs_diff": e_abs0,
         "iterations": {"fp64": r64["stats"]["iterations"], "mixed": rmx["stats"]["iterations"],
                        "mixed_learned": rlr["stats"]["iterations"]},

@@ -76,3 +76,23 @@ def rand_block(rows, cols, cplx, seed=0):
 
 def relerr(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+def cluster_angles(w, A, B, skip, rtol=1e-7):
+    """Largest principal angle (radians) between span(A[:, J]) and
+    span(B[:, J]) over the clusters J of equal w (relative rtol; degenerate
+    modes are compared as subspaces), leaving out the indices in skip (zero
+    modes) and the cluster holding the last index (its partners may lie
+    beyond the computed modes)."""
+    from scipy.linalg import subspace_angles
+    n = len(w)
+    worst, j0 = 0.0, 0
+    while j0 < n:
+        j1 = j0 + 1
+        while j1 < n and abs(w[j1] - w[j0]) <= rtol * max(1.0, abs(w[j0])):
+            j1 += 1
+        J = [j for j in range(j0, j1) if not skip[j]]
+        if J and j1 < n:
+            worst = max(worst, float(np.max(subspace_angles(A[:, J], B[:, J]))))
+        j0 = j1
+    return worst

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from tests.common import physics, relerr
+from tests.common import cluster_angles, physics, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +30,8 @@ def test_1d_uniform_bogoliubov(cplx):
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
 def test_3d_dipolar_vs_oracle(prec):
     """Small 3D dipolar problem: device solve vs the CPU reference solver
-    (numpy iLOBPCG on the oracle operators), same psi0 and start block."""
+    (numpy iLOBPCG on the oracle operators), same psi0 and start block:
+    omega to 1e-9, and the modes' subspaces to 1e-6 rad."""
     import bdg_lobpcg_amd as B
     from oracle.ilobpcg_ref import ilobpcg, solver_precond_weights
     N, L = (16, 16, 16), (12.0, 12.0, 12.0)
@@ -60,6 +61,11 @@ def test_3d_dipolar_vs_oracle(prec):
     np.testing.assert_allclose(w[nz], ref["w"][nz], rtol=1e-9)
     assert np.all(np.abs(w[z] - ref["w"][z]) < 1e-6)
     assert p.converged() >= nev
+    # eigenvectors: principal angles between the device and reference modes
+    # (v half), per cluster of equal omega
+    ang = cluster_angles(w, p.modes(nev)[1], ref["v"], z)
+    print(f"max subspace angle vs reference solver {ang:.2e}", flush=True)
+    assert ang < 1e-6
 
 
 def test_graph_capture_gives_identical_solve():
