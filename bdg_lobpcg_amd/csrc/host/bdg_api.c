@@ -523,7 +523,7 @@ void bdg_start_block(bdg_t *b, void *X) {
  * preconditioned residual collapses onto those grid points; inside the
  * condensate lt ~ mu, so the clamp only changes those points.  The operator
  * API (bdgx_apply PRECOND_*) keeps the reference weights. */
-static int upload_fields(bdg_t *b) {
+static int upload_fields(bdg_t *b, int learned) {
   const int drc = ensure_device(b);
   if (drc) return drc;
   const double floor_v = b->mu > 1e-8 ? b->mu : 1e-8;
@@ -536,8 +536,21 @@ static int upload_fields(bdg_t *b) {
   int rc = bdgx_set_fields(b->dev, b->ltK, b->ltM, b->wf, pKs, pMs, b->mu);
   free(pKs);
   free(pMs);
-  if (!rc && b->lp_theta) rc = bdgx_lp_set_params(b->dev, b->lp_theta);
+  /* the network's tables only when the solve will use it (real psi0 and
+   * K != M); otherwise the analytic T_M T_K runs and theta is ignored */
+  if (!rc && learned) rc = bdgx_lp_set_params(b->dev, b->lp_theta);
   return rc;
+}
+
+/* K == M: no dipolar term and identical local terms (the solver then runs
+ * the standard form) */
+static int k_equals_m(const bdg_t *b) {
+  return !b->dipolar && memcmp(b->ltK, b->ltM, sizeof(double) * b->size) == 0;
+}
+
+/* the learned network replaces T_M T_K only for real psi0 with K != M */
+static int learned_applies(const bdg_t *b) {
+  return b->lp_theta != NULL && !b->complex_psi0 && (b->state & BDG_HAS_WF) && !k_equals_m(b);
 }
 
 int bdg_solve(bdg_t *b) {
@@ -554,7 +567,8 @@ int bdg_solve(bdg_t *b) {
   }
   const int c = b->complex_psi0;
   const size_t es64 = sizeof(double) * (c ? 2 : 1);
-  int rc = upload_fields(b);
+  const int learned = learned_applies(b);
+  int rc = upload_fields(b, learned);
   if (rc) {
     b->stats.status = rc;
     return rc;
@@ -603,8 +617,8 @@ int bdg_solve(bdg_t *b) {
   cfg.img_tol = b->img_tol;
   cfg.refresh_every = (int)b->refresh_every;
   cfg.mixed_min_cols = (int)(m / 4 > 1 ? m / 4 : 1);
-  cfg.k_equals_m = !b->dipolar && memcmp(b->ltK, b->ltM, sizeof(double) * S) == 0;
-  cfg.learned = b->lp_theta != NULL && !c && !cfg.k_equals_m;
+  cfg.k_equals_m = k_equals_m(b);
+  cfg.learned = learned;
   cfg.lp_window = b->lp_window;
   cfg.lp_factor = b->lp_factor;
   cfg.graphs = b->graphs;
@@ -680,7 +694,8 @@ int bdg_solve(bdg_t *b) {
   for (uint64_t j = 0; j < nev; ++j) {
     b->residuals[j] = r.res[j];
     b->zero_modes[j] = r.zflag[j] || fabs(b->eigvals[j]) < 1e-3 * scale;
-    if (b->zero_modes[j] || r.res[j] < b->tol) ++conv;
+    /* the solver's own test: only M-null directions bypass the residual */
+    if (r.zflag[j] || r.res[j] < b->tol) ++conv;
   }
   /* column norms ||X_j||^2 and ||M X_j||^2 (the residual kernel with w = 0) */
   double *dzero = NULL, *dn2 = NULL, *dcu = NULL, *dcv = NULL;

@@ -422,21 +422,27 @@ static int eval_residuals(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const stage_t *s
     const double wj = signed_sqrt(w2[j]);
     const double aw = fmax(fabs(wj), 1e-300);
     const double rn = sqrt(hn2[3 * i]), xn = sqrt(hn2[3 * i + 1]), mn = sqrt(hn2[3 * i + 2]);
-    const double xnorm = sqrt(xn * xn + (mn / aw) * (mn / aw));
-    st->resv[j] = (rn / aw) / (xnorm * fmax(1.0, fabs(wj)));
+    if (fabs(wj) < cfg->zero_tol * scale) {
+      /* |w| too small for the pencil residual's 1/|w| scaling: absolute
+       * residual of the product form, ||K M x - w^2 x|| / (||x|| scale^2) */
+      st->resv[j] = rn / (fmax(xn, 1e-300) * scale * scale);
+    } else {
+      const double xnorm = sqrt(xn * xn + (mn / aw) * (mn / aw));
+      st->resv[j] = (rn / aw) / (xnorm * fmax(1.0, fabs(wj)));
+    }
   }
   st->nact = 0;
   st->allc = 1;
   st->worst = 0.0;
   int si = 0;
   for (int j = 0; j < m; ++j) {
-    const double wj = signed_sqrt(w2[j]);
-    const int zero = zf[j] || fabs(wj) < cfg->zero_tol * scale;
+    /* only the numerical null space of the M-Gram (zf) bypasses the test;
+     * a soft mode (|w| below zero_tol * scale) must meet it in absolute form */
     const int evaluated = !nsel || (si < nsel && st->sel[si] == j);
-    if (evaluated) st->conv[j] = zero || st->resv[j] < cfg->tol;
+    if (evaluated) st->conv[j] = zf[j] || st->resv[j] < cfg->tol;
     if (j < nev) {
       if (!st->conv[j]) st->allc = 0;
-      if (!zero) st->worst = fmax(st->worst, st->resv[j]);
+      if (!zf[j]) st->worst = fmax(st->worst, st->resv[j]);
     }
     if (!st->conv[j]) {
       st->rpos[st->nact] = nsel ? si : j;

@@ -96,3 +96,54 @@ def cluster_angles(w, A, B, skip, rtol=1e-7):
             worst = max(worst, float(np.max(subspace_angles(A[:, J], B[:, J]))))
         j0 = j1
     return worst
+
+
+def oracle_problem(cfg, psi, mu):
+    """Oracle context of a benchmark configuration (bdg_lobpcg_amd.workloads
+    CONFIGS entry) for psi0 / mu: trap, contact U = g n, dipolar mean field
+    with g_ddi = g eps_dd along z, mu (reference src/setup.c:17-177 order).
+    Returns (ctx, fields)."""
+    N, L, cplx = cfg["N"], cfg["L"], cfg["cplx"]
+    c = O.Ctx(N, L, cplx)
+    S = c.size
+    ltK, ltM = np.zeros(S), np.zeros(S)
+    c.trap_harmonic(cfg["omega"], ltK, ltM)
+    c.contact(psi, cfg["g"], cfg["g"], ltK, ltM)
+    c.set_fields(wf=psi)
+    c.set_dipolar(cfg["g"] * cfg["edd"], (0.0, 0.0, 1.0), cfg["R"])
+    gphi = c.dipolar_meanfield()
+    ltK += gphi
+    ltM += gphi
+    pK, pM = O.setup_mu(mu, ltK, ltM)
+    c.set_fields(ltK=ltK, ltM=ltM, pK=pK, pM=pM, mu=mu)
+    return c, dict(wf=psi, ltK=ltK, ltM=ltM, pK=pK, pM=pM, mu=mu)
+
+
+def oracle_mode_residuals(c, w, U, V, chunk=8):
+    """Residuals of device modes under the oracle's operators (FP64 C,
+    reference formulas), column by column:
+      first order  ||K u - w v|| + ||M v - w u|| over ||[u; v]|| max(1, |w|)
+                   (inf for w^2 < 0, where no real pair exists);
+      pencil       the solver's convergence measure for x = [M v / w; v]:
+                   (||K M v - w^2 v|| / |w|) / (||x|| max(1, |w|)), with w^2
+                   carrying the sign of the reported w.
+    Returns (first_order[nev], pencil[nev])."""
+    nev = len(w)
+    first = np.full(nev, np.inf)
+    pen = np.zeros(nev)
+    for j0 in range(0, nev, chunk):
+        J = slice(j0, min(nev, j0 + chunk))
+        u, v = U[:, J], V[:, J]
+        Ku, Mv = c.apply("K", u), c.apply("M", v)
+        KMv = c.apply("K", Mv)
+        wj = w[J]
+        aw = np.maximum(np.abs(wj), 1e-300)
+        w2 = np.sign(wj) * wj ** 2
+        rn = np.linalg.norm(KMv - v * w2[None, :], axis=0)
+        xnorm = np.sqrt(np.linalg.norm(v, axis=0) ** 2 + (np.linalg.norm(Mv, axis=0) / aw) ** 2)
+        pen[J] = (rn / aw) / (xnorm * np.maximum(1.0, aw))
+        xn = np.sqrt(np.linalg.norm(u, axis=0) ** 2 + np.linalg.norm(v, axis=0) ** 2)
+        r1 = (np.linalg.norm(Ku - v * wj[None, :], axis=0) +
+              np.linalg.norm(Mv - u * wj[None, :], axis=0)) / (xn * np.maximum(1.0, np.abs(wj)))
+        first[J] = np.where(wj > 0, r1, np.inf)
+    return first, pen

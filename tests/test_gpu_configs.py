@@ -1,0 +1,176 @@
+"""GPU parity at the benchmark configurations and for the psi0 generator.
+
+* the device imaginary-time generator against the oracle's
+  (orc_itp_groundstate), real / complex / dipolar;
+* the 32^3 configuration at full size: device FP64 and mixed solves against
+  the CPU reference solver (numpy iLOBPCG on the C oracle's operators), same
+  oracle-ITP psi0, parameters and seed-42 start block;
+* the 128^3 configuration at full size: every device FP64 mode checked with
+  the oracle's K and M (first-order and pencil residuals), and the
+  mixed solve's omega against the FP64 solve's;
+* a complex problem with learned-preconditioner parameters set solves with
+  the analytic preconditioner (the network is real-psi0 only).
+The result contract follows the reference's (src/solver.c:262-277,
+tests/test_bdg_3d_harmonic.c:51-108).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from tests.common import cluster_angles, gaussian_psi, oracle_mode_residuals, oracle_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("case", ["real_dipolar", "complex_vortex_dipolar", "real_1d"])
+def test_itp_device_matches_oracle(case):
+    """bdgx_itp_groundstate vs orc_itp_groundstate: same step definition,
+    FP64 on both sides; psi and mu to 1e-12."""
+    import bdg_lobpcg_amd as B
+    from oracle import oracle as O
+    if case == "real_dipolar":
+        N, L, cplx, om, g, gdd, R = (16, 16, 24), (12.0, 12.0, 16.0), False, (1, 1, .7), 60., 25., 6.
+        psi = gaussian_psi(N, L)
+    elif case == "complex_vortex_dipolar":
+        N, L, cplx, om, g, gdd, R = (16, 16, 20), (10.0, 10.0, 14.0), True, (2, 2, .5), 80., 40., 5.
+        psi = gaussian_psi(N, L, True, vortex=True)
+    else:
+        N, L, cplx, om, g, gdd, R = (64,), (16.0,), False, (1.0,), 10., 0., 1.
+        psi = gaussian_psi(N, L)
+    _log(f"itp parity case {case} N={N}")
+    ref, mu_ref = O.Ctx(N, L, cplx).itp_groundstate(psi, om, g, gdd, (0, 0, 1), R, 1e-3, 500)
+    d = B.DeviceContext(N, L, cplx)
+    got, mu = d.itp_groundstate(psi, om, g, gdd, (0, 0, 1), R, 1e-3, 500)
+    d.close()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    _log(f"  psi rel err {err:.2e}, mu {mu:.15g} vs {mu_ref:.15g}")
+    assert err < 1e-12
+    assert abs(mu - mu_ref) < 1e-12 * abs(mu_ref)
+
+
+# ------------------------------------------------------- 32^3 configuration
+@pytest.fixture(scope="module")
+def cfg0_reference():
+    """Oracle ITP psi0 and the CPU reference solve of the 32^3 configuration
+    (tol 1e-10, tighter than the configuration's 1e-8)."""
+    from bdg_lobpcg_amd.workloads import CONFIGS, initial_psi
+    from oracle import oracle as O
+    from oracle.ilobpcg_ref import ilobpcg, solver_precond_weights
+    cfg = CONFIGS["cfg0_32c_d"]
+    t0 = time.perf_counter()
+    psi, mu = O.Ctx(cfg["N"], cfg["L"], False).itp_groundstate(
+        initial_psi(cfg), cfg["omega"], cfg["g"], cfg["g"] * cfg["edd"], (0, 0, 1), cfg["R"],
+        cfg["dt"], cfg["itp_steps"])
+    c, f = oracle_problem(cfg, psi, mu)
+    S = c.size
+    kpsi = np.linalg.norm(c.apply("K", psi[:, None])) / np.linalg.norm(psi)
+    pKs, pMs = solver_precond_weights(f["ltK"], f["ltM"], mu)
+    c.set_fields(pK=pKs, pM=pMs, mu=mu)
+    X0 = c.start_block_uniform(42, cfg["m"])
+    ref = ilobpcg(lambda X: c.apply("K", X), lambda X: c.apply("M", X),
+                  lambda X: c.apply("precondK", X), lambda X: c.apply("precondM", X),
+                  X0[S:], cfg["nev"], tol=1e-10, maxiter=1500)
+    _log(f"cfg0 oracle: mu {mu:.12g}, ||K psi0||/||psi0|| {kpsi:.3e}, {ref['iters']} iterations,"
+         f" {time.perf_counter() - t0:.1f} s, w {ref['w']}")
+    assert ref["res"].max() < 1e-10
+    return cfg, psi, mu, ref
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_cfg0_full_size_vs_oracle_solver(cfg0_reference, prec):
+    """32^3, nev = block = 10, tol 1e-8: omega to 1e-9 relative (absolute
+    1e-9 for a mode with |w| < 1e-2), modes to 1e-6 rad per cluster."""
+    from bdg_lobpcg_amd.workloads import make_problem
+    cfg, psi, mu, ref = cfg0_reference
+    p = make_problem(cfg, psi, mu, precision=prec)
+    rc = p.solve()
+    st = p.stats()
+    _log(f"cfg0 device prec {prec}: rc {rc}, {st['iterations']} iterations")
+    assert rc == 0
+    nev = cfg["nev"]
+    w = p.eigenvalues(nev)
+    assert p.converged() >= nev
+    small = np.abs(ref["w"]) < 1e-2
+    rel = np.abs(w - ref["w"]) / np.where(small, 1.0, np.abs(ref["w"]))
+    _log(f"  max omega error vs oracle {rel.max():.2e}")
+    assert np.all(rel < 1e-9), (w, ref["w"])
+    ang = cluster_angles(w, p.modes(nev)[1], ref["v"], np.zeros(nev, bool))
+    _log(f"  max subspace angle {ang:.2e}")
+    assert ang < 1e-6
+    p.close()
+
+
+# ------------------------------------------------------ 128^3 configuration
+def test_cfg1_full_size_modes_under_oracle_operators():
+    """128^3 cigar, nev = block = 32: every FP64 device mode satisfies the
+    oracle's K u = w v, M v = w u (first-order residual <= 1e-7 for w > 0)
+    and meets the solve's tolerance 1e-8 in the solver's own pencil measure
+    evaluated with the oracle's K and M (up to 1.5x for the rounding of the
+    two operator implementations), and the mixed
+    solve's omega agree with FP64 to 1e-7 relative (absolute for |w| < 1e-2)."""
+    from bdg_lobpcg_amd.workloads import CONFIGS, ground_state, make_problem
+    cfg = CONFIGS["cfg1_128c_d"]
+    psi, mu, t_itp = ground_state(cfg)
+    _log(f"cfg1 psi0 {t_itp:.2f} s, mu {mu:.12g}")
+    out = {}
+    for prec in (0, 1):
+        p = make_problem(cfg, psi, mu, precision=prec)
+        t0 = time.perf_counter()
+        rc = p.solve()
+        st = p.stats()
+        _log(f"cfg1 prec {prec}: rc {rc}, {st['iterations']} iterations, "
+             f"{time.perf_counter() - t0:.2f} s, converged {p.converged()}")
+        assert rc == 0 and p.converged() >= cfg["nev"]
+        out[prec] = (p.eigenvalues(cfg["nev"]), p)
+    w64, p64 = out[0]
+    wmx, pmx = out[1]
+    small = np.abs(w64) < 1e-2
+    rel = np.abs(wmx - w64) / np.where(small, 1.0, np.abs(w64))
+    _log(f"  mixed vs fp64 omega max err {rel.max():.2e}")
+    assert np.all(rel < 1e-7)
+    pmx.close()
+    U, V = p64.modes(cfg["nev"])
+    p64.close()
+    c, _ = oracle_problem(cfg, psi, mu)
+    t0 = time.perf_counter()
+    first, pen = oracle_mode_residuals(c, w64, U, V)
+    fin = np.isfinite(first)
+    _log(f"  oracle residuals ({time.perf_counter() - t0:.1f} s): first-order max "
+         f"{first[fin].max():.2e} over {fin.sum()} modes, pencil max {pen.max():.2e}; "
+         f"w0 {w64[0]:.6g}")
+    assert first[fin].max() < 1e-7
+    assert pen.max() < 1.5 * cfg["tol"]
+
+
+def test_complex_problem_with_learned_theta_uses_analytic_preconditioner():
+    """Learned parameters on a complex problem: bdg_solve returns 0 and the
+    eigenvalues of the same solve without parameters (the network applies to
+    real psi0 only; the analytic T_M T_K runs)."""
+    import bdg_lobpcg_amd as B
+    from bdg_lobpcg_amd import learned as LP
+    N, L = (16, 16, 16), (12.0, 12.0, 12.0)
+    psi = gaussian_psi(N, L, True)
+    ws = []
+    for theta in (None, LP.init_theta(0)):
+        p = B.Problem(N, L, True)
+        p.set_system()
+        p.set_trap_harmonic((1.0, 1.0, 1.0))
+        p.set_wavefunction(psi)
+        p.set_contact(20.0)
+        p.set_dipolar(10.0, (0.0, 0.0, 1.0), 6.0)
+        p.set_mu(1.0)
+        p.set_solver_params(4, 6, 400, 1e-9)
+        p.set_init_uniform(42)
+        p.set_learned(theta)
+        rc = p.solve()
+        _log(f"complex 16^3 theta={'set' if theta is not None else 'none'}: rc {rc}")
+        assert rc == 0
+        ws.append(p.eigenvalues(4))
+        assert p.stats()["learned_iterations"] == 0
+        p.close()
+    np.testing.assert_array_equal(ws[0], ws[1])

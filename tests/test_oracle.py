@@ -332,13 +332,112 @@ def test_itp_noninteracting_is_stationary():
     assert abs(np.sum(out ** 2) * 0.125 - 1.0) < 1e-12
 
 
+def _gp_numpy(N, L, omega, g, gdd, R):
+    """H_GP restated in numpy (tests/golden/make_golden.py's NumpyBdG for
+    the FFTs and the cutoff kernel, independent of the oracle's C): returns
+    (H, energy) for real psi on the grid of reference src/setup.c:35,68-72."""
+    from tests.golden.make_golden import NumpyBdG
+    nb = NumpyBdG(N, L, False)
+    U = nb.kernel((0, 0, 1), R)
+    xs = [(np.arange(n) - n / 2) * l / n for n, l in zip(N, L)]
+    Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+    V = 0.5 * ((omega[0] * X) ** 2 + (omega[1] * Y) ** 2 + (omega[2] * Z) ** 2).ravel()
+    dV = float(np.prod([l / n for n, l in zip(N, L)]))
+
+    def parts(psi, gg, gd):
+        n = psi ** 2
+        phi = nb.inv(nb.fwd(n) * U).real / nb.S
+        T = nb.inv(nb.fwd(psi) * nb.k2 / 2).real / nb.S
+        return T, n, phi
+
+    def H(psi, gg=g, gd=gdd):
+        T, n, phi = parts(psi, gg, gd)
+        return T + (V + gg * n + gd * phi) * psi
+
+    def energy(psi):
+        T, n, phi = parts(psi, g, gdd)
+        return float(np.sum(psi * (T + (V + 0.5 * g * n + 0.5 * gdd * phi) * psi)) * dV)
+
+    def residual(psi, gg=g, gd=gdd):
+        Hp = H(psi, gg, gd)
+        mu = float(np.sum(psi * Hp) * dV)
+        return float(np.linalg.norm(Hp - mu * psi) / np.linalg.norm(psi)), mu
+
+    return residual, energy, X, Y, Z, dV
+
+
 def test_itp_energy_decreases_with_interactions():
-    N, L = (16, 16, 16), (12.0,) * 3
+    """Interacting dipolar ITP (orc_itp_groundstate): the GP energy, computed
+    by an independent numpy H_GP, decreases monotonically along the
+    imaginary-time path, the returned mu equals <psi|H_GP|psi>, and the norm
+    is conserved."""
+    N, L, g, gdd, R = (16, 16, 16), (12.0,) * 3, 50.0, 20.0, 6.0
+    residual, energy, X, Y, Z, dV = _gp_numpy(N, L, (1, 1, 1), g, gdd, R)
     c = O.Ctx(N, L, False)
-    g = (np.arange(16) - 8) * 0.75
-    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
     psi = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 2).ravel()
-    _, mu1 = c.itp_groundstate(psi, (1, 1, 1), 50.0, 20.0, (0, 0, 1), 6.0, 1e-3, 20)
-    p2, mu2 = c.itp_groundstate(psi, (1, 1, 1), 50.0, 20.0, (0, 0, 1), 6.0, 1e-3, 400)
-    assert np.isfinite(mu1) and np.isfinite(mu2)
-    assert abs(np.sum(p2 ** 2) * 0.75 ** 3 - 1.0) < 1e-12
+    E = []
+    for n in (20, 100, 400, 2000):
+        p, mu = c.itp_groundstate(psi, (1, 1, 1), g, gdd, (0, 0, 1), R, 1e-3, n)
+        assert abs(np.sum(p ** 2) * dV - 1.0) < 1e-12
+        _, mu_np = residual(p)
+        assert abs(mu - mu_np) < 1e-12 * abs(mu_np)
+        E.append(energy(p))
+    assert all(b < a for a, b in zip(E, E[1:])), E
+
+
+def test_itp_relaxes_to_the_gp_ground_state():
+    """The relaxed interacting state satisfies H_GP psi = mu psi with the
+    numpy H_GP up to the split-step's O(dt) floor: halving dt halves the GP
+    residual.  The same residual under a Hamiltonian with a wrong sign or
+    factor on g n or g_dd Phi is 30x larger, so such a mistake in the
+    oracle's step would fail here."""
+    N, L, g, gdd, R = (16, 16, 16), (12.0,) * 3, 50.0, 20.0, 6.0
+    residual, energy, X, Y, Z, dV = _gp_numpy(N, L, (1, 1, 1), g, gdd, R)
+    c = O.Ctx(N, L, False)
+    psi = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 2).ravel()
+    p1, _ = c.itp_groundstate(psi, (1, 1, 1), g, gdd, (0, 0, 1), R, 5e-3, 4000)
+    p2, _ = c.itp_groundstate(psi, (1, 1, 1), g, gdd, (0, 0, 1), R, 2.5e-3, 8000)
+    r1, r2 = residual(p1)[0], residual(p2)[0]
+    assert r1 < 1e-2 and 0.4 < r2 / r1 < 0.6, (r1, r2)
+    for gg, gd in ((-g, gdd), (g, 2 * gdd), (g, -gdd), (g, 0.0), (2 * g, gdd)):
+        assert residual(p2, gg, gd)[0] > 30 * r2
+
+
+def test_itp_complex_path_equals_real_path_times_phase():
+    """A constant phase commutes with every term of H_GP, so the complex
+    path of the oracle's ITP relaxes psi e^{i a} to (real result) e^{i a}
+    with the same mu."""
+    N, L, g, gdd, R = (12, 12, 16), (10.0, 10.0, 12.0), 40.0, 15.0, 5.0
+    _, _, X, Y, Z, _ = _gp_numpy(N, L, (1, 1.2, 0.8), g, gdd, R)
+    psi = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 2).ravel()
+    pr, mur = O.Ctx(N, L, False).itp_groundstate(psi, (1, 1.2, 0.8), g, gdd, (0, 0, 1), R,
+                                                 1e-3, 300)
+    ph = np.exp(0.7j)
+    pc, muc = O.Ctx(N, L, True).itp_groundstate(psi * ph, (1, 1.2, 0.8), g, gdd, (0, 0, 1), R,
+                                                1e-3, 300)
+    assert np.abs(pc - pr * ph).max() < 1e-12 * np.abs(pr).max()
+    assert abs(muc - mur) < 1e-12 * abs(mur)
+
+
+def test_reference_solver_soft_mode_meets_the_residual_test():
+    """A mode with |w| below zero_tol * scale but w != 0 is reported as a
+    zero mode, yet converges only through its absolute residual (it is not
+    counted converged on its magnitude alone): M = I, K = diag(d) with
+    w_0 = 1e-4, far below 1e-3 * max|w|."""
+    from oracle.ilobpcg_ref import ilobpcg
+    n, nev = 60, 4
+    d = np.concatenate([[1e-8], np.linspace(0.5, 9.0, n - 1)])
+    rng = np.random.default_rng(7)
+    K = lambda X: d[:, None] * X
+    M = lambda X: X
+    T = lambda X: X / (d[:, None] + 0.5)
+    X0 = rng.standard_normal((n, 6))
+    X0[:, 0] = 1e-4 * X0[:, 0]
+    X0[0, 0] = 1.0  # near the soft eigenvector: its Ritz value is flagged at once
+    one = ilobpcg(K, M, T, lambda X: X, X0, nev, tol=1e-10, maxiter=1)
+    assert one["zero"][0] and one["hist"][-1] > 1e-10  # not passed on magnitude
+    out = ilobpcg(K, M, T, lambda X: X, rng.standard_normal((n, 6)), nev, tol=1e-10,
+                  maxiter=300)
+    assert out["zero"][0] and out["res"][0] < 1e-10
+    assert abs(out["w"][0] - 1e-4) < 1e-9
+    np.testing.assert_allclose(out["w"][1:], np.sqrt(d[1:nev]), rtol=1e-9)
