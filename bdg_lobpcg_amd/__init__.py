@@ -97,6 +97,7 @@ def lib():
         "bdgx_apply": (I, [P, I, I, P, P, U64, U64]),
         "bdgx_apply_host": (I, [P, I, P, P, U64, U64]),
         "bdgx_gram": (I, [P, I, P, U64, U64, P, U64, U64, U64, I, P]),
+        "bdgx_gram_noalloc": (I, [P, I, P, U64, U64, P, U64, U64, U64, I, P]),
         "bdgx_combine": (I, [P, I, P, U64, U64, P, P, U64, U64, U64]),
         "bdgx_combine_cols": (I, [P, I, P, U64, U64, P, P, U64, P, U64, U64]),
         "bdgx_residual": (I, [P, I, P, P, P, U64, U64, U64, P, I, P, U64, P]),
@@ -384,6 +385,14 @@ class DeviceContext:
                               B.shape[1], A.shape[0], int(swap_b), dG.ptr), "gram")
         G = self.download(dG, (A.shape[1], B.shape[1]), self.dtype64)
         return G
+
+    def gram_noalloc_device(self, prec, dA, lda, p, dB, ldb, q, nrows, swap_b, dG) -> int:
+        """bdgx_gram_noalloc on device buffers; returns the status code."""
+        return int(lib().bdgx_gram_noalloc(self.h, prec, dA.ptr, lda, p, dB.ptr, ldb, q, nrows,
+                                           int(swap_b), dG.ptr))
+
+    def workspace_bytes(self) -> int:
+        return int(lib().bdgx_workspace_bytes(self.h))
 
     def gram_pair(self, A: np.ndarray, B1: np.ndarray, B2: np.ndarray, prec=FP64) -> np.ndarray:
         """bdgx_gram_pair: A^H [B1 | B2] with A read once (real contexts)."""

@@ -5,6 +5,8 @@
 // complex); all reductions accumulate in FP64.
 #include <string.h>
 
+#include <mutex>
+
 #include "bdgx_internal.hpp"
 #include "fft_core.hpp"
 
@@ -388,10 +390,17 @@ static void launch_gram_pipe(const GramMArgs<R> &a, int nchunk, int tiles, hipSt
 // G = A^H B (p x q).  With dB2 (real, unswapped, vector-aligned blocks
 // only) the right block is [B | B2], q columns each, and G is p x 2q: A is
 // read once for both.
+// fixed scratch for the partials (nullptr: the context's growing scratch)
+struct FixedScratch {
+  double *ptr;
+  size_t bytes;
+};
+
 template <typename R, bool CPLX>
 static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
                   const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows,
-                  int swap_b, double *dG, const void *dB2 = nullptr, uint64_t ldb2 = 0) {
+                  int swap_b, double *dG, const void *dB2 = nullptr, uint64_t ldb2 = 0,
+                  const FixedScratch *fixed = nullptr) {
   if (p == 0 || q == 0) return BDGX_OK;
   const uint64_t q1 = q;
   if (dB2) q = 2 * q;
@@ -423,8 +432,15 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   const long long len = (long long)p * q * W;
   const int g2 = nchunk > 64 ? (int)((nchunk + 63) / 64) : 0;
   const size_t need = sizeof(double) * (size_t)(nchunk + g2) * (size_t)len;
-  int rc = ensure_scratch(c, need);
-  if (rc) return rc;
+  double *part = nullptr;
+  if (fixed) {
+    if (need > fixed->bytes) return BDGX_ETOOBIG;
+    part = fixed->ptr;
+  } else {
+    int rc = ensure_scratch(c, need);
+    if (rc) return rc;
+    part = reinterpret_cast<double *>(c->scratch);
+  }
   GramMArgs<R> a;
   a.A = reinterpret_cast<const R *>(dA);
   a.B = reinterpret_cast<const R *>(dB);
@@ -445,7 +461,7 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   a.vec = (a.lda % 4 == 0) && (a.ldb % 4 == 0) && (!dB2 || a.ldb2 % 4 == 0) &&
           (al % (4 * sizeof(R)) == 0);
   if (dB2 && (CPLX || swap_b || !a.vec)) return BDGX_EINVAL;
-  a.part = reinterpret_cast<double *>(c->scratch);
+  a.part = part;
   const int nc = (int)nchunk;
   if (CPLX) {
     if (FP == 1 && FQ == 1) launch_gram<R, 1, 1, true>(a, nc, tiles, c->stream);
@@ -482,7 +498,6 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
     }
   }
   BDGX_HIP(hipGetLastError());
-  double *part = reinterpret_cast<double *>(c->scratch);
   return reduce_partials(c, part, nc, len, part + (size_t)nc * len, dG);
 }
 
@@ -494,6 +509,57 @@ int gram_pair_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint
     return gram_t<double, false>(c, dA, lda, p, dB, ldb, q, nrows, 0, dG, dB2, ldb2);
   if (prec == BDGX_FP32)
     return gram_t<float, false>(c, dA, lda, p, dB, ldb, q, nrows, 0, dG, dB2, ldb2);
+  return BDGX_EINVAL;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free Gram: the chunk partials live in one file-scope device
+// array per GPU instead of the context's scratch (which grows with
+// hipMalloc).  In host code `gram_noalloc_scratch` names a host-side
+// stand-in, never the device memory: its device address is looked up once
+// per device (gram_noalloc_setup, from bdgx_ctx_create with that device
+// current) and only that address is handed to kernels.
+static __device__ __align__(256) double
+    gram_noalloc_scratch[BDGX_GRAM_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kMaxDevices];  // device address per device id, or nullptr
+}  // namespace
+
+int gram_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return BDGX_EINVAL;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return BDGX_OK;
+  void *ptr = nullptr;
+  BDGX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(gram_noalloc_scratch)));
+  if (!ptr) return BDGX_EHIP;
+  g_noalloc_ptr[device] = reinterpret_cast<double *>(ptr);
+  return BDGX_OK;
+}
+
+int gram_noalloc_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
+                          const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
+                          double *dG) {
+  if (c->device < 0 || c->device >= kMaxDevices) return BDGX_EINVAL;
+  double *ptr;
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    ptr = g_noalloc_ptr[c->device];
+  }
+  if (!ptr) return BDGX_ESTATE;  // the context's set-up did not run on this device
+  const FixedScratch fs{ptr, BDGX_GRAM_NOALLOC_SCRATCH_BYTES};
+  if (prec == BDGX_FP64)
+    return c->cplx ? gram_t<double, true>(c, dA, lda, p, dB, ldb, q, nrows, swap_b, dG,
+                                          nullptr, 0, &fs)
+                   : gram_t<double, false>(c, dA, lda, p, dB, ldb, q, nrows, swap_b, dG,
+                                           nullptr, 0, &fs);
+  if (prec == BDGX_FP32)
+    return c->cplx ? gram_t<float, true>(c, dA, lda, p, dB, ldb, q, nrows, swap_b, dG,
+                                         nullptr, 0, &fs)
+                   : gram_t<float, false>(c, dA, lda, p, dB, ldb, q, nrows, swap_b, dG,
+                                          nullptr, 0, &fs);
   return BDGX_EINVAL;
 }
 

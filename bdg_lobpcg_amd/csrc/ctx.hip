@@ -21,6 +21,10 @@ int gram_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t 
 int gram_pair_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
                        const void *dB, uint64_t ldb, const void *dB2, uint64_t ldb2,
                        uint64_t q, uint64_t nrows, double *dG);
+int gram_noalloc_setup(int device);
+int gram_noalloc_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
+                          const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
+                          double *dG);
 int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
                      const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows);
 int combine_cols_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
@@ -339,6 +343,8 @@ const char *bdgx_error_string(int code) {
     case BDGX_EHIP: return hipGetErrorString(bdgx_last_hip_error);
     case BDGX_ESTATE: return "required field not uploaded";
     case BDGX_ENOMEM: return "out of memory";
+    case BDGX_ETOOBIG:
+      return "problem exceeds the fixed scratch of the allocation-free entry point";
     default: return "unknown error";
   }
 }
@@ -378,6 +384,12 @@ int bdgx_ctx_create(bdgx_ctx **out, uint64_t dim, const uint64_t *N, const doubl
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
+    return BDGX_EHIP;
+  }
+  // device address of the allocation-free Gram's fixed scratch (once per
+  // device, with this device current)
+  if (gram_noalloc_setup(c->device) != BDGX_OK) {
+    bdgx_ctx_destroy(c);
     return BDGX_EHIP;
   }
   // k^2 tables (reference src/matmul_ctx.c:102-113) and twiddles
@@ -536,6 +548,14 @@ int bdgx_gram(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p, c
   if (!c || !dA || !dB || !dG) return BDGX_EINVAL;
   if (swap_b && (nrows & 1)) return BDGX_EINVAL;
   return gram_dispatch(c, prec, dA, lda, p, dB, ldb, q, nrows, swap_b, dG);
+}
+
+int bdgx_gram_noalloc(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
+                      const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
+                      double *dG) {
+  if (!c || !dA || !dB || !dG) return BDGX_EINVAL;
+  if (swap_b && (nrows & 1)) return BDGX_EINVAL;
+  return gram_noalloc_dispatch(c, prec, dA, lda, p, dB, ldb, q, nrows, swap_b, dG);
 }
 
 int bdgx_gram_pair(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,

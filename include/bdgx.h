@@ -35,7 +35,8 @@ enum {
   BDGX_EINVAL = -1,   /* bad argument / unsupported shape */
   BDGX_EHIP = -2,     /* HIP runtime error */
   BDGX_ESTATE = -3,   /* required field not uploaded yet */
-  BDGX_ENOMEM = -4
+  BDGX_ENOMEM = -4,
+  BDGX_ETOOBIG = -5   /* beyond the fixed scratch of an allocation-free entry */
 };
 
 /* operator codes; each replaces the reference function named beside it */
@@ -94,6 +95,23 @@ int bdgx_apply_host(bdgx_ctx *ctx, int op, const void *hX, void *hY,
 int bdgx_gram(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64_t p,
               const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows,
               int swap_b, double *dG);
+/* bdgx_gram without allocating (no counterpart in the reference: an entry
+ * point for callers that cannot allocate device memory -- code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and result as bdgx_gram, bit for bit; the per-chunk
+ * partial sums live in one fixed device array of
+ * BDGX_GRAM_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends)
+ * instead of the context's scratch.  The partials of one call need
+ * 8 * (chunks + ceil(chunks / 64)) * p * q * (2 if complex) bytes, chunks =
+ * the number of compute units for tall blocks (256 on MI355X: p * q up to
+ * 8065 real, 4032 complex); larger calls return BDGX_ETOOBIG.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (other
+ * contexts or streams) race; calls in one stream order are safe.  Needs a
+ * context created on that device (bdgx_ctx_create looks the array up). */
+#define BDGX_GRAM_NOALLOC_SCRATCH_BYTES (16u << 20)
+int bdgx_gram_noalloc(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64_t p,
+                      const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
+                      double *dG);
 /* Two Grams with one left block read once (no reference counterpart; the
  * solver's V^H V_W and V^H M V_W, src/solver.c:262 through the external
  * iLOBPCG's Gram step): dG (p x 2q, column-major) = dA^H [dB1 | dB2], dB1
