@@ -23,6 +23,7 @@ _lib = None
 OP = {"kinetic": 0, "K": 1, "M": 2, "Lrep": 3, "Swap": 4, "precondK": 5,
       "precondM": 6, "precondLrep": 7, "dipolar_conv": 8}
 FP64, FP32 = 0, 1
+FORM_FIRST_ORDER, FORM_PRODUCT = 0, 1
 LP_NKNOT, LP_HIDDEN = 16, 16
 LP_NPARAM = 8 + 2 * LP_NKNOT + 29 * LP_HIDDEN
 PREC_FP64, PREC_MIXED = 0, 1
@@ -66,7 +67,8 @@ class SolveStats(C.Structure):
                 ("learned_fallback_at", C.c_int), ("graph_launches", C.c_uint64),
                 ("graph_captures", C.c_uint64), ("seconds_total", C.c_double),
                 ("seconds_rr", C.c_double), ("status", C.c_int),
-                ("frozen_skips", C.c_uint64)]
+                ("frozen_skips", C.c_uint64), ("iterations_fp32", C.c_uint64),
+                ("fp32_worst_residual", C.c_double)]
 
 
 def build() -> None:
@@ -147,6 +149,8 @@ def lib():
         # extensions
         "bdg_set_precision": (None, [P, I]),
         "bdg_get_solve_stats": (None, [P, C.POINTER(SolveStats)]),
+        "bdg_set_formulation": (None, [P, I]),
+        "bdg_set_fp32_phase": (None, [P, D, U64]),
         "bdg_residuals": (C.POINTER(D), [P]),
         "bdg_zero_modes": (C.POINTER(I), [P]),
         "bdg_device_context": (P, [P]),
@@ -548,6 +552,13 @@ class Problem:
 
     def set_mixed_params(self, img_tol=0.0, refresh_every=20):
         lib().bdg_set_mixed_params(self.h, float(img_tol), int(refresh_every))
+
+    def set_formulation(self, form):
+        """FORM_FIRST_ORDER (default) or FORM_PRODUCT."""
+        lib().bdg_set_formulation(self.h, int(form))
+
+    def set_fp32_phase(self, fp32_tol=1e-5, stall_window=20):
+        lib().bdg_set_fp32_phase(self.h, float(fp32_tol), int(stall_window))
 
     def set_verbosity(self, level):
         lib().bdg_set_verbosity(self.h, int(level))

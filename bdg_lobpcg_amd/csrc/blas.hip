@@ -576,13 +576,25 @@ int gram_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t 
 }
 
 // ---------------------------------------------------------------------------
-// combine: Y[r, j] = sum_i A[r, i] C[i, j]
+// combine: Y[r, j] = sum_i A[r, i] C[i, j], where the k columns of A come from
+// one block, or from two: columns i < k1 from A (ld lda), columns i >= k1
+// from A2 (ld lda2).
 constexpr int CJ = 16;  // output columns per pass
 
+template <typename R>
+struct Cols {
+  const R *A, *A2;
+  long long lda, lda2;
+  int k1;
+  // element offset of column i (in units of R, before the complex factor)
+  __device__ __forceinline__ const R *col(int i) const {
+    return i < k1 ? A + (long long)i * lda : A2 + (long long)(i - k1) * lda2;
+  }
+};
+
 template <typename R, bool CPLX>
-__global__ void __launch_bounds__(256) k_combine(const R *A, long long lda, int k,
-                                                 const double *C, R *Y, long long ldy,
-                                                 int m, long long nrows) {
+__global__ void __launch_bounds__(256) k_combine(Cols<R> A, int k, const double *C, R *Y,
+                                                 long long ldy, int m, long long nrows) {
   constexpr int W = CPLX ? 2 : 1;
   extern __shared__ __align__(16) char smem_raw[];
   double *Cs = reinterpret_cast<double *>(smem_raw);  // k x CJ (x W)
@@ -605,11 +617,12 @@ __global__ void __launch_bounds__(256) k_combine(const R *A, long long lda, int 
         for (int w = 0; w < W; ++w) acc[jj][w] = 0.0;
       for (int i = 0; i < k; ++i) {
         double ar, ai = 0.0;
+        const R *ac = A.col(i);
         if (CPLX) {
-          ar = (double)A[2 * (r + i * lda)];
-          ai = (double)A[2 * (r + i * lda) + 1];
+          ar = (double)ac[2 * r];
+          ai = (double)ac[2 * r + 1];
         } else {
-          ar = (double)A[r + i * lda];
+          ar = (double)ac[r];
         }
 #pragma unroll
         for (int jj = 0; jj < CJ; ++jj) {
@@ -651,15 +664,14 @@ __global__ void __launch_bounds__(256) k_combine(const R *A, long long lda, int 
 // the current one.
 // RT consecutive rows (r .. r+RT-1) of column `col`, zero past k / nrows
 template <typename R, int RT>
-__device__ __forceinline__ void load_rows(const R *A, long long lda, int col, int k,
-                                          long long row, long long nrows, int vec,
-                                          double out[RT]) {
+__device__ __forceinline__ void load_rows(const Cols<R> &A, int col, int k, long long row,
+                                          long long nrows, int vec, double out[RT]) {
   if (col >= k) {
 #pragma unroll
     for (int t = 0; t < RT; ++t) out[t] = 0.0;
     return;
   }
-  const R *p = A + row + (long long)col * lda;
+  const R *p = A.col(col) + row;
   if (vec && row + RT <= nrows) {
     if constexpr (sizeof(R) == 8 && RT == 4) {
       const double2 x = reinterpret_cast<const double2 *>(p)[0];
@@ -699,10 +711,9 @@ __device__ __forceinline__ void load_rows(const R *A, long long lda, int col, in
 // vector store.  C is staged in LDS KC rows at a time; A runs PD k steps
 // ahead of the MFMAs.
 template <typename R, int MF, int RT>
-__global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda, int k,
-                                                      const double *C, R *Y, long long ldy,
-                                                      int m, long long nrows, int vec_a,
-                                                      int vec_y) {
+__global__ void __launch_bounds__(256) k_combine_mfma(Cols<R> A, int k, const double *C, R *Y,
+                                                      long long ldy, int m, long long nrows,
+                                                      int vec_a, int vec_y) {
   constexpr int KC = 64;
   __shared__ double Cs[KC][16 * MF + 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -727,7 +738,7 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
     double av[PD][RT];
 #pragma unroll
     for (int q = 0; q < PD; ++q)
-      if (4 * q < nk) load_rows<R, RT>(A, lda, kc + 4 * q + kk, k, row, nrows, vec_a, av[q]);
+      if (4 * q < nk) load_rows<R, RT>(A, kc + 4 * q + kk, k, row, nrows, vec_a, av[q]);
     for (int k4 = 0; k4 < nk; k4 += 4 * PD) {
 #pragma unroll
       for (int q = 0; q < PD; ++q) {
@@ -744,7 +755,7 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
                 __builtin_amdgcn_mfma_f64_16x16x4f64(av[q][t], bv[f], acc[t][f], 0, 0, 0);
         // refill this slot with the step PD ahead
         if (kq + 4 * PD < nk)
-          load_rows<R, RT>(A, lda, kc + kq + 4 * PD + kk, k, row, nrows, vec_a, av[q]);
+          load_rows<R, RT>(A, kc + kq + 4 * PD + kk, k, row, nrows, vec_a, av[q]);
       }
     }
   }
@@ -781,9 +792,9 @@ __global__ void __launch_bounds__(256) k_combine_mfma(const R *A, long long lda,
 // col and row to valid ones (a clamped column meets a zero row of C, a
 // clamped row only feeds output rows that are not stored)
 template <typename R, int RT>
-__device__ __forceinline__ void load_rows_nb(const R *A, long long lda, int col, long long row,
+__device__ __forceinline__ void load_rows_nb(const Cols<R> &A, int col, long long row,
                                              double out[RT]) {
-  const R *p = A + row + (long long)col * lda;
+  const R *p = A.col(col) + row;
   if constexpr (sizeof(R) == 8 && RT == 4) {
     const double2 x = reinterpret_cast<const double2 *>(p)[0];
     const double2 y = reinterpret_cast<const double2 *>(p)[1];
@@ -816,9 +827,9 @@ __device__ __forceinline__ void load_rows_nb(const R *A, long long lda, int col,
 // Same MFMA sequence per output element as k_combine_mfma.  Needs
 // vector-aligned A and Y and nrows a multiple of RT.
 template <typename R, int MF, int RT, int PD>
-__global__ void __launch_bounds__(256) k_combine_pers(const R *A, long long lda, int k,
-                                                      const double *C, R *Y, long long ldy,
-                                                      int m, long long nrows, long long ntiles) {
+__global__ void __launch_bounds__(256) k_combine_pers(Cols<R> A, int k, const double *C, R *Y,
+                                                      long long ldy, int m, long long nrows,
+                                                      long long ntiles) {
   extern __shared__ double cs[];
   constexpr int CW = 16 * MF + 1;
   const int kpad = (k + 3) & ~3;
@@ -848,7 +859,7 @@ __global__ void __launch_bounds__(256) k_combine_pers(const R *A, long long lda,
 #pragma unroll
       for (int d = 0; d < PD; ++d) {
         const int col = 4 * d + kk;
-        load_rows_nb<R, RT>(A, lda, col < k ? col : k - 1, rl, av[d]);
+        load_rows_nb<R, RT>(A, col < k ? col : k - 1, rl, av[d]);
       }
     }
     v4d acc[RT][MF];
@@ -879,7 +890,7 @@ __global__ void __launch_bounds__(256) k_combine_pers(const R *A, long long lda,
           tn += gridDim.x;
         }
         const int col = 4 * sn + kk;
-        load_rows_nb<R, RT>(A, lda, col < k ? col : k - 1, lane_row(tn), av[d]);
+        load_rows_nb<R, RT>(A, col < k ? col : k - 1, lane_row(tn), av[d]);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
@@ -910,8 +921,8 @@ __global__ void __launch_bounds__(256) k_combine_pers(const R *A, long long lda,
 
 template <typename R, int MF, int RT, int PD>
 static void launch_combine_pers(const bdgx_ctx *c, unsigned gy, size_t lds, hipStream_t s,
-                                const R *A, long long lda, int k, const double *C, R *Y,
-                                long long ldy, int m, long long nrows) {
+                                const Cols<R> &A, int k, const double *C, R *Y, long long ldy,
+                                int m, long long nrows) {
   const long long ntiles = (nrows + 64 * RT - 1) / (64 * RT);
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -922,18 +933,18 @@ static void launch_combine_pers(const bdgx_ctx *c, unsigned gy, size_t lds, hipS
   long long gx = (long long)c->ncu * nb;
   if (gx > ntiles) gx = ntiles;
   hipLaunchKernelGGL((k_combine_pers<R, MF, RT, PD>), dim3((unsigned)gx, gy), dim3(256), lds, s,
-                     A, lda, k, C, Y, ldy, m, nrows, ntiles);
+                     A, k, C, Y, ldy, m, nrows, ntiles);
 }
 
 template <typename R, int MF>
-static void launch_combine(int rt, dim3 grid, hipStream_t s, const R *A, long long lda, int k,
+static void launch_combine(int rt, dim3 grid, hipStream_t s, const Cols<R> &A, int k,
                            const double *C, R *Y, long long ldy, int m, long long nrows,
                            int vec_a, int vec_y) {
   if (rt == 2)
-    hipLaunchKernelGGL((k_combine_mfma<R, MF, 2>), grid, dim3(256), 0, s, A, lda, k, C, Y, ldy, m,
+    hipLaunchKernelGGL((k_combine_mfma<R, MF, 2>), grid, dim3(256), 0, s, A, k, C, Y, ldy, m,
                        nrows, vec_a, vec_y);
   else
-    hipLaunchKernelGGL((k_combine_mfma<R, MF, 4>), grid, dim3(256), 0, s, A, lda, k, C, Y, ldy, m,
+    hipLaunchKernelGGL((k_combine_mfma<R, MF, 4>), grid, dim3(256), 0, s, A, k, C, Y, ldy, m,
                        nrows, vec_a, vec_y);
 }
 
@@ -943,63 +954,74 @@ static void launch_combine(int rt, dim3 grid, hipStream_t s, const R *A, long lo
 // rows per lane and 4 steps (C tile in LDS once per block; used where it
 // fits in 64 KB of LDS and A, Y are vector-aligned, else as variant 1 / 0).
 template <typename R>
-static int combine_real_mfma(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k,
-                             const double *dC, void *dY, uint64_t ldy, uint64_t m,
-                             uint64_t nrows) {
+static int combine_real_mfma(bdgx_ctx *c, const Cols<R> &A, uint64_t k, const double *dC,
+                             void *dY, uint64_t ldy, uint64_t m, uint64_t nrows) {
   const int MF = m <= 16 ? 1 : (m <= 32 ? 2 : (m <= 48 ? 3 : 4));
   const int var = c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_COMBINE];
   const int rt = (var == 1 || var == 2) ? 2 : 4;
   const unsigned gx = (unsigned)((nrows + 64 * rt - 1) / (64 * rt));
   const unsigned gy = (unsigned)((m + 16 * MF - 1) / (16 * MF));
-  const int vec_a = (lda % 4 == 0) && ((uintptr_t)dA % (4 * sizeof(R)) == 0);
+  auto aligned = [](const R *p, long long ld) {
+    return (ld % 4 == 0) && ((uintptr_t)p % (4 * sizeof(R)) == 0);
+  };
+  const int vec_a = aligned(A.A, A.lda) && ((int)k <= A.k1 || aligned(A.A2, A.lda2));
   const int vec_y = (ldy % 4 == 0) && ((uintptr_t)dY % (4 * sizeof(R)) == 0);
-  const R *A = reinterpret_cast<const R *>(dA);
   R *Y = reinterpret_cast<R *>(dY);
   const size_t lds = sizeof(double) * (size_t)((k + 3) & ~3ull) * (size_t)(16 * MF + 1);
   if (var >= 2 && vec_a && vec_y && nrows % 4 == 0 && lds <= 64 * 1024) {
     const int K = (int)k, M = (int)m;
-    const long long la = (long long)lda, ly = (long long)ldy, nr = (long long)nrows;
+    const long long ly = (long long)ldy, nr = (long long)nrows;
     switch (MF * 2 + (var == 3)) {
-      case 2: launch_combine_pers<R, 1, 2, 8>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
-      case 3: launch_combine_pers<R, 1, 4, 4>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
-      case 4: launch_combine_pers<R, 2, 2, 8>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
-      case 5: launch_combine_pers<R, 2, 4, 4>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
-      case 6: launch_combine_pers<R, 3, 2, 8>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
-      case 7: launch_combine_pers<R, 3, 4, 4>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
-      case 8: launch_combine_pers<R, 4, 2, 8>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
-      default: launch_combine_pers<R, 4, 4, 4>(c, gy, lds, c->stream, A, la, K, dC, Y, ly, M, nr); break;
+      case 2: launch_combine_pers<R, 1, 2, 8>(c, gy, lds, c->stream, A, K, dC, Y, ly, M, nr); break;
+      case 3: launch_combine_pers<R, 1, 4, 4>(c, gy, lds, c->stream, A, K, dC, Y, ly, M, nr); break;
+      case 4: launch_combine_pers<R, 2, 2, 8>(c, gy, lds, c->stream, A, K, dC, Y, ly, M, nr); break;
+      case 5: launch_combine_pers<R, 2, 4, 4>(c, gy, lds, c->stream, A, K, dC, Y, ly, M, nr); break;
+      case 6: launch_combine_pers<R, 3, 2, 8>(c, gy, lds, c->stream, A, K, dC, Y, ly, M, nr); break;
+      case 7: launch_combine_pers<R, 3, 4, 4>(c, gy, lds, c->stream, A, K, dC, Y, ly, M, nr); break;
+      case 8: launch_combine_pers<R, 4, 2, 8>(c, gy, lds, c->stream, A, K, dC, Y, ly, M, nr); break;
+      default: launch_combine_pers<R, 4, 4, 4>(c, gy, lds, c->stream, A, K, dC, Y, ly, M, nr); break;
     }
     BDGX_HIP(hipGetLastError());
     return BDGX_OK;
   }
   const dim3 grid(gx, gy);
   switch (MF) {
-    case 1: launch_combine<R, 1>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,
-                                 (long long)ldy, (int)m, (long long)nrows, vec_a, vec_y); break;
-    case 2: launch_combine<R, 2>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,
-                                 (long long)ldy, (int)m, (long long)nrows, vec_a, vec_y); break;
-    case 3: launch_combine<R, 3>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,
-                                 (long long)ldy, (int)m, (long long)nrows, vec_a, vec_y); break;
-    default: launch_combine<R, 4>(rt, grid, c->stream, A, (long long)lda, (int)k, dC, Y,
-                                  (long long)ldy, (int)m, (long long)nrows, vec_a, vec_y); break;
+    case 1: launch_combine<R, 1>(rt, grid, c->stream, A, (int)k, dC, Y, (long long)ldy, (int)m,
+                                 (long long)nrows, vec_a, vec_y); break;
+    case 2: launch_combine<R, 2>(rt, grid, c->stream, A, (int)k, dC, Y, (long long)ldy, (int)m,
+                                 (long long)nrows, vec_a, vec_y); break;
+    case 3: launch_combine<R, 3>(rt, grid, c->stream, A, (int)k, dC, Y, (long long)ldy, (int)m,
+                                 (long long)nrows, vec_a, vec_y); break;
+    default: launch_combine<R, 4>(rt, grid, c->stream, A, (int)k, dC, Y, (long long)ldy, (int)m,
+                                  (long long)nrows, vec_a, vec_y); break;
   }
   BDGX_HIP(hipGetLastError());
   return BDGX_OK;
 }
 
+// Y = [A | A2] C: k1 columns of A (ld lda), then k2 columns of A2 (ld lda2)
 template <typename R, bool CPLX>
-static int combine_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k,
-                     const double *dC, void *dY, uint64_t ldy, uint64_t m,
-                     uint64_t nrows) {
+static int combine_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t k1, const void *dA2,
+                     uint64_t lda2, uint64_t k2, const double *dC, void *dY, uint64_t ldy,
+                     uint64_t m, uint64_t nrows) {
   if (m == 0 || nrows == 0) return BDGX_OK;
-  if (!CPLX) return combine_real_mfma<R>(c, dA, lda, k, dC, dY, ldy, m, nrows);
+  const uint64_t k = k1 + k2;
+  if (k == 0 || k > (1u << 20)) return BDGX_EINVAL;
   const int W = CPLX ? 2 : 1;
+  Cols<R> A;
+  A.A = reinterpret_cast<const R *>(dA);
+  A.A2 = k2 ? reinterpret_cast<const R *>(dA2) : A.A;
+  // column strides in units of R (an interleaved complex element is two)
+  A.lda = (long long)lda * W;
+  A.lda2 = (long long)(k2 ? lda2 : lda) * W;
+  A.k1 = (int)k1;
+  if (!CPLX) return combine_real_mfma<R>(c, A, k, dC, dY, ldy, m, nrows);
   const size_t sm = sizeof(double) * (size_t)k * CJ * W;
   if (sm > 64 * 1024) return BDGX_EINVAL;
   const long long nb = ((long long)nrows + 255) / 256;
-  hipLaunchKernelGGL((k_combine<R, CPLX>), dim3((unsigned)nb), dim3(256), sm, c->stream,
-                     reinterpret_cast<const R *>(dA), (long long)lda, (int)k, dC,
-                     reinterpret_cast<R *>(dY), (long long)ldy, (int)m, (long long)nrows);
+  hipLaunchKernelGGL((k_combine<R, CPLX>), dim3((unsigned)nb), dim3(256), sm, c->stream, A,
+                     (int)k, dC, reinterpret_cast<R *>(dY), (long long)ldy, (int)m,
+                     (long long)nrows);
   BDGX_HIP(hipGetLastError());
   return BDGX_OK;
 }
@@ -1095,16 +1117,24 @@ int combine_cols_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, u
   return BDGX_OK;
 }
 
+int combine2_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k1,
+                      const void *dA2, uint64_t lda2, uint64_t k2, const double *dC, void *dY,
+                      uint64_t ldy, uint64_t m, uint64_t nrows) {
+  if (prec == BDGX_FP64)
+    return c->cplx ? combine_t<double, true>(c, dA, lda, k1, dA2, lda2, k2, dC, dY, ldy, m, nrows)
+                   : combine_t<double, false>(c, dA, lda, k1, dA2, lda2, k2, dC, dY, ldy, m,
+                                              nrows);
+  if (prec == BDGX_FP32)
+    return c->cplx ? combine_t<float, true>(c, dA, lda, k1, dA2, lda2, k2, dC, dY, ldy, m, nrows)
+                   : combine_t<float, false>(c, dA, lda, k1, dA2, lda2, k2, dC, dY, ldy, m,
+                                             nrows);
+  return BDGX_EINVAL;
+}
+
 int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
                      const double *dC, void *dY, uint64_t ldy, uint64_t m,
                      uint64_t nrows) {
-  if (prec == BDGX_FP64)
-    return c->cplx ? combine_t<double, true>(c, dA, lda, k, dC, dY, ldy, m, nrows)
-                   : combine_t<double, false>(c, dA, lda, k, dC, dY, ldy, m, nrows);
-  if (prec == BDGX_FP32)
-    return c->cplx ? combine_t<float, true>(c, dA, lda, k, dC, dY, ldy, m, nrows)
-                   : combine_t<float, false>(c, dA, lda, k, dC, dY, ldy, m, nrows);
-  return BDGX_EINVAL;
+  return combine2_dispatch(c, prec, dA, lda, k, nullptr, 0, 0, dC, dY, ldy, m, nrows);
 }
 
 // ---------------------------------------------------------------------------

@@ -25,6 +25,9 @@ int gram_noalloc_setup(int device);
 int gram_noalloc_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t p,
                           const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
                           double *dG);
+int combine2_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k1,
+                      const void *dA2, uint64_t lda2, uint64_t k2, const double *dC, void *dY,
+                      uint64_t ldy, uint64_t m, uint64_t nrows);
 int combine_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
                      const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows);
 int combine_cols_dispatch(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,
@@ -569,6 +572,13 @@ int bdgx_combine(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k
                  const double *dC, void *dY, uint64_t ldy, uint64_t m, uint64_t nrows) {
   if (!c || !dA || !dC || !dY) return BDGX_EINVAL;
   return combine_dispatch(c, prec, dA, lda, k, dC, dY, ldy, m, nrows);
+}
+
+int bdgx_combine2(bdgx_ctx *c, int prec, const void *dA1, uint64_t lda1, uint64_t k1,
+                  const void *dA2, uint64_t lda2, uint64_t k2, const double *dC, void *dY,
+                  uint64_t ldy, uint64_t m, uint64_t nrows) {
+  if (!c || !dA1 || !dC || !dY || (k2 && !dA2)) return BDGX_EINVAL;
+  return combine2_dispatch(c, prec, dA1, lda1, k1, dA2, lda2, k2, dC, dY, ldy, m, nrows);
 }
 
 int bdgx_combine_cols(bdgx_ctx *c, int prec, const void *dA, uint64_t lda, uint64_t k,

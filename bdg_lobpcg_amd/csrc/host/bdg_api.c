@@ -118,6 +118,9 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
   b->lp_factor = 0.5;
   b->graphs = 1;
   b->freeze_tol = 1e-6;
+  b->formulation = BDG_FORM_FIRST_ORDER;
+  b->fp32_tol = 1e-5;
+  b->fp32_stall = 40;
   return b;
 }
 
@@ -553,6 +556,119 @@ static int learned_applies(const bdg_t *b) {
   return b->lp_theta != NULL && !b->complex_psi0 && (b->state & BDG_HAS_WF) && !k_equals_m(b);
 }
 
+/* bdg_solve on the first-order pencil (lrep.c): one FP64 phase, or (mixed)
+ * an FP32 phase followed by an FP64 phase from its vectors.  dV0 is the v
+ * half of the start block (u = v for the built-in blocks; the reference's
+ * start blocks set u = v too, src/solver.c:166-259). */
+static int solve_first_order(bdg_t *b, const void *dV0, double t_start) {
+  const uint64_t S = b->size, m = b->sizeSub, nev = b->nev;
+  const int c = b->complex_psi0;
+  const size_t es64 = sizeof(double) * (c ? 2 : 1);
+  void *dU = NULL, *dV = NULL;
+  int rc = bdgx_malloc(&dU, S * m * es64);
+  if (!rc) rc = bdgx_malloc(&dV, S * m * es64);
+  lrep_cfg cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.S = S;
+  cfg.m = (int)m;
+  cfg.nev = (int)nev;
+  cfg.cplx = c;
+  cfg.verbosity = b->verbosity;
+  lrep_result r32, r64;
+  memset(&r32, 0, sizeof(r32));
+  memset(&r64, 0, sizeof(r64));
+  double *w2 = (double *)xcalloc(m, sizeof(double)), *resv = (double *)xcalloc(m, sizeof(double));
+  r32.w2 = r64.w2 = w2;
+  r32.res = r64.res = resv;
+  const void *start = dV0;
+  int it32 = 0;
+  if (!rc && b->precision == BDG_PREC_MIXED) {
+    cfg.maxiter = (int)b->maxIter;
+    cfg.tol = fmax(b->tol, b->fp32_tol);
+    cfg.stall_window = b->fp32_stall;
+    cfg.stall_below = 1e-4;
+    rc = bdg_lrep_device(b->dev, &cfg, BDGX_FP32, dV0, dV0, BDGX_FP64, &r32, dU, dV);
+    it32 = r32.iters;
+    b->stats.iterations_fp32 = (uint64_t)it32;
+    b->stats.fp32_worst_residual = r32.worst;
+    b->stats.column_applies_fp32 = r32.applies;
+    if (b->verbosity > 0)
+      fprintf(stderr, "[bdg] fp32 phase: %d iterations, worst residual %.3e, status %d\n", it32,
+              r32.worst, rc);
+    start = NULL;
+  }
+  if (!rc) {
+    cfg.maxiter = (int)b->maxIter - it32;
+    if (cfg.maxiter < 0) cfg.maxiter = 0;
+    cfg.tol = b->tol;
+    cfg.stall_window = 0;
+    rc = start ? bdg_lrep_device(b->dev, &cfg, BDGX_FP64, start, start, BDGX_FP64, &r64, dU, dV)
+               : bdg_lrep_device(b->dev, &cfg, BDGX_FP64, dU, dV, BDGX_FP64, &r64, dU, dV);
+  }
+  b->stats.iterations = (uint64_t)(it32 + r64.iters);
+  b->stats.column_applies = r32.applies + r64.applies;
+  b->stats.seconds_rr = r32.rr_seconds + r64.rr_seconds;
+  if (b->verbosity > 0)
+    fprintf(stderr, "[bdg] %s first-order solve: %d + %d iterations, %.3f s, status %d\n",
+            b->precision == BDG_PREC_MIXED ? "mixed" : "fp64", it32, r64.iters,
+            bdg_wtime() - t_start, rc);
+  /* results: w = signed sqrt(w2); [u; v] scaled to unit norm on the device */
+  double *dzero = NULL, *dn2 = NULL, *dcu = NULL;
+  double *hn2 = (double *)xcalloc(3 * nev, sizeof(double));
+  double *cz = (double *)xcalloc(nev * nev * (c ? 2 : 1), sizeof(double));
+  void *dOutU = NULL, *dOutV = NULL;
+  if (!rc) rc = bdgx_malloc((void **)&dzero, sizeof(double) * nev);
+  if (!rc) rc = bdgx_malloc((void **)&dn2, sizeof(double) * 3 * nev);
+  if (!rc) rc = bdgx_memset(b->dev, dzero, 0, sizeof(double) * nev);
+  if (!rc) rc = bdgx_residual(b->dev, BDGX_FP64, dU, dV, NULL, S, nev, S, dzero, 0, NULL, S, dn2);
+  if (!rc) rc = bdgx_memcpy_d2h(b->dev, hn2, dn2, sizeof(double) * 3 * nev);
+  if (!rc) {
+    b->eigvals = (double *)xcalloc(nev, sizeof(double));
+    b->residuals = (double *)xcalloc(nev, sizeof(double));
+    b->zero_modes = (int *)xcalloc(nev, sizeof(int));
+    double scale = 1.0;
+    for (uint64_t j = 0; j < nev; ++j) {
+      b->eigvals[j] = w2[j] >= 0 ? sqrt(w2[j]) : -sqrt(-w2[j]);
+      scale = fmax(scale, fabs(b->eigvals[j]));
+    }
+    uint64_t conv = 0;
+    for (uint64_t j = 0; j < nev; ++j) {
+      b->residuals[j] = resv[j];
+      b->zero_modes[j] = fabs(b->eigvals[j]) < 1e-3 * scale;
+      if (resv[j] < b->tol) ++conv;
+      const double nrm2 = hn2[3 * j] + hn2[3 * j + 1];
+      cz[(j + j * nev) * (c ? 2 : 1)] = nrm2 > 0 ? 1.0 / sqrt(nrm2) : 1.0;
+    }
+    b->converged = conv;
+    rc = bdgx_malloc((void **)&dcu, sizeof(double) * nev * nev * (c ? 2 : 1));
+    if (!rc) rc = bdgx_memcpy_h2d(b->dev, dcu, cz, sizeof(double) * nev * nev * (c ? 2 : 1));
+  }
+  if (!rc) rc = bdgx_malloc(&dOutU, S * nev * es64);
+  if (!rc) rc = bdgx_malloc(&dOutV, S * nev * es64);
+  if (!rc) rc = bdgx_combine(b->dev, BDGX_FP64, dU, S, nev, dcu, dOutU, S, nev, S);
+  if (!rc) rc = bdgx_combine(b->dev, BDGX_FP64, dV, S, nev, dcu, dOutV, S, nev, S);
+  if (!rc) {
+    b->modes_u = xmalloc_big(S * nev * es64);
+    b->modes_v = xmalloc_big(S * nev * es64);
+    rc = bdgx_memcpy_d2h(b->dev, b->modes_u, dOutU, S * nev * es64);
+    if (!rc) rc = bdgx_memcpy_d2h(b->dev, b->modes_v, dOutV, S * nev * es64);
+  }
+  bdgx_free(dzero);
+  bdgx_free(dn2);
+  bdgx_free(dcu);
+  bdgx_free(dOutU);
+  bdgx_free(dOutV);
+  bdgx_free(dU);
+  bdgx_free(dV);
+  free(hn2);
+  free(cz);
+  free(w2);
+  free(resv);
+  b->stats.status = rc;
+  b->stats.seconds_total = bdg_wtime() - t_start;
+  return rc;
+}
+
 int bdg_solve(bdg_t *b) {
   BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_solve");
   BDG_REQUIRE(b, BDG_HAS_MU, "bdg_solve");
@@ -600,6 +716,11 @@ int bdg_solve(bdg_t *b) {
   if (rc) {
     bdgx_free(dV0);
     b->stats.status = rc;
+    return rc;
+  }
+  if (b->formulation == BDG_FORM_FIRST_ORDER && !k_equals_m(b) && !learned) {
+    rc = solve_first_order(b, dV0, t_start);
+    bdgx_free(dV0);
     return rc;
   }
   lobpcg_cfg cfg;
@@ -789,6 +910,11 @@ const void *bdg_modes_v(const bdg_t *b) { return b->modes_v; }
 /* ---- extensions (include/bdg/bdg_amd.h) ------------------------------------ */
 
 void bdg_set_precision(bdg_t *b, bdg_precision_t p) { b->precision = (int)p; }
+void bdg_set_formulation(bdg_t *b, bdg_formulation_t f) { b->formulation = (int)f; }
+void bdg_set_fp32_phase(bdg_t *b, f64 fp32_tol, uint64_t stall_window) {
+  b->fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
+  b->fp32_stall = (int)stall_window;
+}
 /* ---- batches of independent problems ---------------------------------- */
 
 typedef struct {

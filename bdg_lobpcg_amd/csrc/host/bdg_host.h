@@ -69,6 +69,10 @@ struct bdg_t {
   int lp_window;
   double lp_factor;
   int graphs; /* replay the per-iteration device chain as HIP graphs (default 1) */
+  int formulation;    /* bdg_formulation_t                                       */
+  double fp32_tol;    /* mixed, first order: residual at which the FP32 phase    */
+  int fp32_stall;     /*   stops; columns also stop after this many iterations  */
+                      /*   without halving their residual (0: never)            */
   double freeze_tol; /* converged columns moved less than this keep their vectors */
   int verbosity;
   /* results */
@@ -145,6 +149,34 @@ typedef struct {
  * bdgx_free. */
 int bdg_lobpcg_device(bdgx_ctx *ctx, const lobpcg_cfg *cfg, const void *dX0, lobpcg_result *res,
                       void **dX);
+
+/* device LOBPCG on the first-order pencil (K u = w v, M v = w u); see lrep.c */
+typedef struct {
+  uint64_t S;
+  int m, nev, cplx;
+  int maxiter;
+  double tol;          /* a column is done when its residual is below this       */
+  int stall_window;    /* ... or (0: never) when its residual has not halved in  */
+  double stall_below;  /*   stall_window iterations and is below stall_below     */
+  int verbosity;
+} lrep_cfg;
+
+typedef struct {
+  double *w2;  /* m (negative: a pair with imaginary w) */
+  double *res; /* m */
+  int iters;
+  uint64_t applies; /* operator applies (K or M) on single columns */
+  double rr_seconds;
+  double worst;
+  int status;
+} lrep_result;
+
+/* One phase at storage precision prec, from the blocks dXU0, dXV0 (S x m,
+ * ld S, precision prec0); on success the Ritz vectors u, v (u negated for
+ * w^2 < 0, so that M v = w u with w = -|w|) are written to dXU, dXV
+ * (precision prec0; they may be the input blocks). */
+int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dXU0,
+                    const void *dXV0, int prec0, lrep_result *res, void *dXU, void *dXV);
 
 double bdg_wtime(void);
 

@@ -258,3 +258,27 @@ void dn_qhgq(int k, int q, int cplx, const double *G, const double *Q, double *o
 int dn_chol_ok(int k, int cplx, const double *G, double tau) {
   return cplx ? chol_ok_z(k, (const double complex *)G, tau) : chol_ok_d(k, G, tau);
 }
+
+int dn_lrep_rr(int ku, int kv, int m, int cplx, const double *GK, const double *GM,
+               const double *GW, double *w2, double *A, double *B) {
+  if (cplx)
+    return lrep_rr_z(ku, kv, m, (const double complex *)GK, (const double complex *)GM,
+                     (const double complex *)GW, w2, (double complex *)A, (double complex *)B);
+  return lrep_rr_d(ku, kv, m, GK, GM, GW, w2, A, B);
+}
+
+int dn_lrep_update(int k, int m, int cplx, const double *A, const int *act, int nact,
+                   double drop, double *CQ, double *R11) {
+  if (cplx)
+    return lrep_update_z(k, m, (const double complex *)A, act, nact, drop, (double complex *)CQ,
+                         (double complex *)R11);
+  return lrep_update_d(k, m, A, act, nact, drop, CQ, R11);
+}
+
+int dn_orth_coef(int nq, int nw, int cplx, const double *GQW, const double *GWW, double drop,
+                 double *C) {
+  if (cplx)
+    return orth_coef_z(nq, nw, (const double complex *)GQW, (const double complex *)GWW, drop,
+                       (double complex *)C);
+  return orth_coef_d(nq, nw, GQW, GWW, drop, C);
+}

@@ -33,4 +33,25 @@ void dn_qhgq(int k, int q, int cplx, const double *G, const double *Q, double *o
  * positive). */
 int dn_chol_ok(int k, int cplx, const double *G, double tau);
 
+/* First-order pencil Rayleigh-Ritz (lrep.c): Ritz pairs of
+ * G_K a = w G_W b, G_M b = w G_W^H a on (nearly) orthonormal bases of ku and
+ * kv columns (G_K ku x ku, G_M kv x kv, G_W ku x kv).  w2[m] ascending (a
+ * negative value: a pair with imaginary w, then K u = -|w| v, M v = |w| u),
+ * A (ku x m), B (kv x m), ||a||^2 + ||b||^2 = 1.  Returns the number of
+ * pairs, or < 0. */
+int dn_lrep_rr(int ku, int kv, int m, int cplx, const double *GK, const double *GM,
+               const double *GW, double *w2, double *A, double *B);
+
+/* Orthonormal update coefficients CQ (k x (m + p)) = [Q1 | Q2], A = Q1 R11,
+ * Q2 spanning the part of A[m:, act] (first m rows zeroed) outside span(A);
+ * parts below drop x their norm left out.  Returns m + p or < 0. */
+int dn_lrep_update(int k, int m, int cplx, const double *A, const int *act, int nact,
+                   double drop, double *CQ, double *R11);
+
+/* C ((nq + nw) x r) with [Q | W] C orthonormal and orthogonal to Q, from
+ * GQW = Q^H W and GWW = W^H W (Q orthonormal); directions that keep less
+ * than drop of their norm are left out.  Returns r. */
+int dn_orth_coef(int nq, int nw, int cplx, const double *GQW, const double *GWW, double drop,
+                 double *C);
+
 #endif

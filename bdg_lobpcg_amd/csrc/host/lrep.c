@@ -1,0 +1,514 @@
+/*
+ * Device LOBPCG on the first-order BdG pencil (host driver in C).
+ *
+ * The reference hands A = diag(K, M), B = Swap and T = [T_K; T_M] to an
+ * external iLOBPCG (reference src/solver.c:147-164, 262).  This solver works
+ * on that first-order form, K u = w v, M v = w u, with separate search
+ * spaces for the two halves (locally optimal block CG of Bai and Li for the
+ * linear-response eigenproblem):
+ *
+ *   U half: Z_U = [X_U | P_U | W_U] (S x k_U) and its image K Z_U;
+ *   V half: Z_V = [X_V | P_V | W_V] (S x k_V) and its image M Z_V;
+ *   residuals R_U = K u - s|w| v, R_V = M v - |w| u of the Ritz pairs
+ *   u = X_U R_A, v = X_V R_B (s = sign w^2); W_U = T_K R_U, W_V = T_M R_V;
+ *   Rayleigh-Ritz from G_K = Z_U^H K Z_U, G_M = Z_V^H M Z_V, G_W = Z_U^H Z_V
+ *   (dn_lrep_rr), FP64 on the host.
+ *
+ * Each half's basis is kept (nearly) orthonormal: new directions W are
+ * orthonormalised against [X | P] on the device before their images are
+ * formed (two passes, dn_orth_coef), and X, P are updated with orthonormal
+ * coefficients [Q1 | Q2] (dn_lrep_update: A = Q1 R_A, Q2 the new P
+ * directions).  No update then multiplies a stored vector or image by a
+ * large coefficient, so the rounding of FP32 storage and FP32 images is
+ * never amplified: the same iteration runs with FP32 basis, images and
+ * operator applies (Grams accumulate in FP64).
+ *
+ * One call is one phase at one storage precision, started from given X
+ * blocks; the mixed policy runs an FP32 phase to the residual FP32 can
+ * reach, then an FP64 phase started from its X blocks, whose first
+ * Rayleigh-Ritz (FP64 images of the FP32 vectors) is the FP64 refinement
+ * sweep and whose iterations, if any, take the residual to the tolerance.
+ */
+#include <complex.h>
+
+#include "bdg_host.h"
+#include "dense.h"
+
+#define TRY(x)         \
+  do {                 \
+    rc = (x);          \
+    if (rc) goto done; \
+  } while (0)
+
+static size_t lr_es(int prec, int cplx) {
+  return (size_t)(prec == BDGX_FP64 ? 8 : 4) * (cplx ? 2 : 1);
+}
+
+static char *lr_col(void *base, uint64_t col, uint64_t S, size_t es) {
+  return (char *)base + (size_t)col * S * es;
+}
+
+typedef struct {
+  void *Z[2], *Y[2]; /* basis and its image, two generations */
+  int np;            /* P columns of the current generation */
+} lr_half;
+
+/* host copy of a device Gram */
+static int lr_gram(bdgx_ctx *ctx, int prec, const void *A, uint64_t p, const void *Bm, uint64_t q,
+                   uint64_t S, double *dG, double *hG, int w) {
+  int rc = bdgx_gram(ctx, prec, A, S, p, Bm, S, q, S, 0, dG);
+  if (!rc) rc = bdgx_memcpy_d2h(ctx, hG, dG, sizeof(double) * p * q * w);
+  return rc;
+}
+
+/* Orthonormalise the nw columns Wt (temp) against Q = Z[:, 0:nq] and write
+ * the result to Z[:, nq:nq+r].  A second pass (against the first's output,
+ * which then sits in Z) runs when the first cancelled most of a column (kept
+ * less than 1/8 of its norm): the projection then loses orthogonality in
+ * proportion.  Returns 0 or an error; *ncols = r. */
+static int lr_orth(bdgx_ctx *ctx, int prec, int cplx, void *Z, int nq, void *Wt, int nw,
+                   uint64_t S, double drop, double *dG, double *dC, double *hG, double *hC,
+                   int *ncols) {
+  const int w = cplx ? 2 : 1;
+  const size_t es = lr_es(prec, cplx);
+  int rc = 0;
+  *ncols = 0;
+  if (nw == 0) return 0;
+  double *hQW = hG, *hWW = hG + (size_t)nq * nw * w;
+  if (nq) TRY(lr_gram(ctx, prec, Z, (uint64_t)nq, Wt, (uint64_t)nw, S, dG, hQW, w));
+  TRY(lr_gram(ctx, prec, Wt, (uint64_t)nw, Wt, (uint64_t)nw, S, dG, hWW, w));
+  double kept_min = 1.0;
+  for (int j = 0; j < nw; ++j) {
+    const double n2 = hWW[((size_t)j + (size_t)j * nw) * w];
+    double q2 = 0.0;
+    for (int i = 0; i < nq * w; ++i) q2 += hQW[(size_t)j * nq * w + i] * hQW[(size_t)j * nq * w + i];
+    if (n2 > 0) kept_min = fmin(kept_min, sqrt(fmax(0.0, 1.0 - q2 / n2)));
+  }
+  const int r = dn_orth_coef(nq, nw, cplx, hQW, hWW, drop, hC);
+  if (r <= 0) return 0;
+  TRY(bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)(nq + nw) * r * w));
+  TRY(bdgx_combine2(ctx, prec, Z, S, (uint64_t)nq, Wt, S, (uint64_t)nw, dC,
+                    lr_col(Z, (uint64_t)nq, S, es), S, (uint64_t)r, S));
+  *ncols = r;
+  if (kept_min > 0.125) return 0;
+  /* pass 2: [Q | W']^H W' in one Gram (W' now follows Q in Z) */
+  TRY(lr_gram(ctx, prec, Z, (uint64_t)(nq + r), lr_col(Z, (uint64_t)nq, S, es), (uint64_t)r, S,
+              dG, hG, w));
+  {
+    double *tmp = hG + (size_t)(nq + r) * r * w;
+    double *gq = tmp, *gw = tmp + (size_t)nq * r * w;
+    for (int j = 0; j < r; ++j) {
+      memcpy(gq + (size_t)j * nq * w, hG + (size_t)j * (nq + r) * w, sizeof(double) * nq * w);
+      memcpy(gw + (size_t)j * r * w, hG + ((size_t)j * (nq + r) + nq) * w, sizeof(double) * r * w);
+    }
+    const int r2 = dn_orth_coef(nq, r, cplx, gq, gw, 1e-2, hC);
+    *ncols = r2 > 0 ? r2 : 0;
+    if (r2 <= 0) return 0;
+    TRY(bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)(nq + r) * r2 * w));
+    TRY(bdgx_combine(ctx, prec, Z, S, (uint64_t)(nq + r), dC, Wt, S, (uint64_t)r2, S));
+    TRY(bdgx_convert(ctx, prec, Wt, S, prec, lr_col(Z, (uint64_t)nq, S, es), S, S,
+                     (uint64_t)r2));
+  }
+done:
+  return rc;
+}
+
+/* out (nu x nv, ld nu) = Qu^H G Qv, G ku x kv (ld ku), Qu ku x nu, Qv kv x nv */
+static void lr_project(int ku, int kv, int nu, int nv, int cplx, const double *G, const double *Qu,
+                       const double *Qv, double *tmp, double *out) {
+  if (!cplx) {
+    /* tmp (ku x nv) = G Qv */
+    for (int j = 0; j < nv; ++j)
+      for (int i = 0; i < ku; ++i) {
+        double a = 0.0;
+        for (int t = 0; t < kv; ++t) a += G[i + (size_t)t * ku] * Qv[t + (size_t)j * kv];
+        tmp[i + (size_t)j * ku] = a;
+      }
+    for (int j = 0; j < nv; ++j)
+      for (int i = 0; i < nu; ++i) {
+        double a = 0.0;
+        for (int t = 0; t < ku; ++t) a += Qu[t + (size_t)i * ku] * tmp[t + (size_t)j * ku];
+        out[i + (size_t)j * nu] = a;
+      }
+    return;
+  }
+  const double complex *g = (const double complex *)G, *qu = (const double complex *)Qu,
+                       *qv = (const double complex *)Qv;
+  double complex *t2 = (double complex *)tmp, *o = (double complex *)out;
+  for (int j = 0; j < nv; ++j)
+    for (int i = 0; i < ku; ++i) {
+      double complex a = 0.0;
+      for (int t = 0; t < kv; ++t) a += g[i + (size_t)t * ku] * qv[t + (size_t)j * kv];
+      t2[i + (size_t)j * ku] = a;
+    }
+  for (int j = 0; j < nv; ++j)
+    for (int i = 0; i < nu; ++i) {
+      double complex a = 0.0;
+      for (int t = 0; t < ku; ++t) a += conj(qu[t + (size_t)i * ku]) * t2[t + (size_t)j * ku];
+      o[i + (size_t)j * nu] = a;
+    }
+}
+
+/* Assemble a ku x kv Gram (ld ku) from its projected top-left block P
+ * (nu x nv, ld nu), the new columns Cn (ku x (kv - nv), ld ku: Z_a^H B_new)
+ * and the new rows given conjugate-transposed, Rn (kv x (ku - nu), ld kv:
+ * Z_b^H A_new); for a Hermitian Gram (herm) Rn is Cn itself. */
+static void lr_assemble(int ku, int kv, int nu, int nv, int w, const double *P, const double *Cn,
+                        const double *Rn, double *out) {
+  for (int j = 0; j < kv; ++j)
+    for (int i = 0; i < ku; ++i) {
+      double *d = out + ((size_t)i + (size_t)j * ku) * w;
+      const double *s;
+      int cj = 0;
+      if (j >= nv) {
+        s = Cn + ((size_t)i + (size_t)(j - nv) * ku) * w;
+      } else if (i >= nu) {
+        s = Rn + ((size_t)j + (size_t)(i - nu) * kv) * w;
+        cj = 1;
+      } else {
+        s = P + ((size_t)i + (size_t)j * nu) * w;
+      }
+      d[0] = s[0];
+      if (w == 2) d[1] = cj ? -s[1] : s[1];
+    }
+}
+
+int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dXU0,
+                    const void *dXV0, int prec0, lrep_result *res, void *dXU, void *dXV) {
+  const uint64_t S = cfg->S;
+  const int m = cfg->m, nev = cfg->nev, cplx = cfg->cplx;
+  const int w = cplx ? 2 : 1;
+  const size_t es = lr_es(prec, cplx);
+  const int kmax = 3 * m;
+  const int opT[2] = {BDGX_OP_PRECOND_K, BDGX_OP_PRECOND_M};
+  const int opA[2] = {BDGX_OP_K, BDGX_OP_M};
+  /* full device Gram recompute period (FP32: the projected Grams of rounded
+   * vectors drift from their stored values faster) */
+  const int refresh = prec == BDGX_FP64 ? 25 : 8;
+  int rc = 0, it = 0, g = 0;
+  lr_half H[2];
+  memset(H, 0, sizeof(H));
+  void *dR[2] = {NULL, NULL}, *dWt[2] = {NULL, NULL};
+  double *dG = NULL, *dC = NULL, *dZero = NULL, *dN2 = NULL;
+  const size_t gsz = (size_t)kmax * kmax * w;
+  double *hGK = (double *)calloc(gsz, sizeof(double));
+  double *hGM = (double *)calloc(gsz, sizeof(double));
+  double *hGW = (double *)calloc(gsz, sizeof(double));
+  double *hP[3] = {(double *)calloc(gsz, sizeof(double)), (double *)calloc(gsz, sizeof(double)),
+                   (double *)calloc(gsz, sizeof(double))};
+  double *hG = (double *)calloc(4 * gsz, sizeof(double));
+  double *hC = (double *)calloc(4 * gsz, sizeof(double));
+  double *hA = (double *)calloc((size_t)kmax * m * w, sizeof(double));
+  double *hB = (double *)calloc((size_t)kmax * m * w, sizeof(double));
+  double *hCQ[2] = {(double *)calloc(gsz, sizeof(double)), (double *)calloc(gsz, sizeof(double))};
+  double *hR[2] = {(double *)calloc((size_t)m * m * w, sizeof(double)),
+                   (double *)calloc((size_t)m * m * w, sizeof(double))};
+  double *w2 = (double *)calloc((size_t)m, sizeof(double));
+  double *hn2 = (double *)calloc(3 * (size_t)m, sizeof(double));
+  double *resv = (double *)calloc((size_t)m, sizeof(double));
+  double *best = (double *)calloc((size_t)m, sizeof(double));
+  int *best_it = (int *)calloc((size_t)m, sizeof(int));
+  int *conv = (int *)calloc((size_t)m, sizeof(int));
+  int *act = (int *)calloc((size_t)m, sizeof(int));
+  int *wact = (int *)calloc((size_t)m, sizeof(int)); /* columns whose W is in the basis */
+  int *sel = (int *)calloc((size_t)m, sizeof(int));
+  int k[2] = {0, 0}, nw[2] = {0, 0}, nwact = 0, nact = 0, last_full = -1000, since_full = 0;
+  int gvalid = 0; /* hGK/hGM/hGW hold the Grams of the current basis */
+  double trr = 0.0;
+  const double t0 = bdg_wtime();
+  res->iters = 0;
+  res->status = 0;
+  for (int j = 0; j < m; ++j) best[j] = HUGE_VAL;
+
+  TRY(bdgx_autotune(ctx, prec, (uint64_t)(m < 32 ? m : 32), 0));
+  for (int h = 0; h < 2; ++h) {
+    for (int q = 0; q < 2; ++q) {
+      TRY(bdgx_malloc(&H[h].Z[q], (size_t)S * kmax * es));
+      TRY(bdgx_malloc(&H[h].Y[q], (size_t)S * kmax * es));
+    }
+    TRY(bdgx_malloc(&dR[h], (size_t)S * m * es));
+    TRY(bdgx_malloc(&dWt[h], (size_t)S * m * es));
+  }
+  TRY(bdgx_malloc((void **)&dG, sizeof(double) * 4 * gsz));
+  TRY(bdgx_malloc((void **)&dC, sizeof(double) * 4 * gsz));
+  TRY(bdgx_malloc((void **)&dZero, sizeof(double) * (size_t)m));
+  TRY(bdgx_malloc((void **)&dN2, sizeof(double) * 3 * (size_t)m));
+  TRY(bdgx_memset(ctx, dZero, 0, sizeof(double) * (size_t)m));
+
+  /* start: X_U, X_V from the given blocks (converted), orthonormalised */
+  {
+    const void *src[2] = {dXU0, dXV0};
+    for (int h = 0; h < 2; ++h) {
+      TRY(bdgx_convert(ctx, prec0, src[h], S, prec, dWt[h], S, S, (uint64_t)m));
+      int r = 0;
+      TRY(lr_orth(ctx, prec, cplx, H[h].Z[0], 0, dWt[h], m, S, 1e-8, dG, dC, hG, hC, &r));
+      if (r < m) {
+        rc = -101; /* start block numerically dependent */
+        goto done;
+      }
+      TRY(bdgx_apply(ctx, opA[h], prec, H[h].Z[0], H[h].Y[0], (uint64_t)m, S));
+      k[h] = m;
+    }
+    res->applies += 2 * (uint64_t)m;
+  }
+
+  for (it = 0; it <= cfg->maxiter; ++it) {
+    /* Grams of the current basis: all on the device at the start and every
+     * `refresh` iterations, else the new W columns only */
+    if (!gvalid || ++since_full >= refresh) {
+      TRY(lr_gram(ctx, prec, H[0].Z[g], (uint64_t)k[0], H[0].Y[g], (uint64_t)k[0], S, dG, hGK, w));
+      TRY(lr_gram(ctx, prec, H[1].Z[g], (uint64_t)k[1], H[1].Y[g], (uint64_t)k[1], S, dG, hGM, w));
+      TRY(lr_gram(ctx, prec, H[0].Z[g], (uint64_t)k[0], H[1].Z[g], (uint64_t)k[1], S, dG, hGW, w));
+      since_full = 0;
+    } else {
+      /* hP[0..2]: projected [X | P] blocks (n0 x n0, n1 x n1, n0 x n1) */
+      const int n0 = k[0] - nw[0], n1 = k[1] - nw[1];
+      const size_t c0 = (size_t)k[0] * nw[0] * w, c1 = (size_t)k[1] * nw[1] * w;
+      double *gKc = hG, *gMc = gKc + c0, *gWc = gMc + c1, *gWr = gWc + (size_t)k[0] * nw[1] * w;
+      const void *W0 = lr_col(H[0].Z[g], (uint64_t)n0, S, es);
+      const void *W1 = lr_col(H[1].Z[g], (uint64_t)n1, S, es);
+      const void *KW0 = lr_col(H[0].Y[g], (uint64_t)n0, S, es);
+      const void *MW1 = lr_col(H[1].Y[g], (uint64_t)n1, S, es);
+      int paired = 0;
+      if (!cplx && nw[0] == nw[1] && nw[0] > 0) {
+        /* Z_U^H [K W_U | W_V] and Z_V^H [M W_V | W_U]: each basis read once */
+        const int q = nw[0];
+        double *tmp = hC;
+        if (bdgx_gram_pair(ctx, prec, H[0].Z[g], S, (uint64_t)k[0], KW0, S, W1, S, (uint64_t)q,
+                           S, dG) == 0 &&
+            bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * (size_t)k[0] * 2 * q) == 0) {
+          memcpy(gKc, tmp, sizeof(double) * (size_t)k[0] * q);
+          memcpy(gWc, tmp + (size_t)k[0] * q, sizeof(double) * (size_t)k[0] * q);
+          if (bdgx_gram_pair(ctx, prec, H[1].Z[g], S, (uint64_t)k[1], MW1, S, W0, S,
+                             (uint64_t)q, S, dG) == 0 &&
+              bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * (size_t)k[1] * 2 * q) == 0) {
+            memcpy(gMc, tmp, sizeof(double) * (size_t)k[1] * q);
+            memcpy(gWr, tmp + (size_t)k[1] * q, sizeof(double) * (size_t)k[1] * q);
+            paired = 1;
+          }
+        }
+      }
+      if (!paired) {
+        if (nw[0]) {
+          TRY(lr_gram(ctx, prec, H[0].Z[g], (uint64_t)k[0], KW0, (uint64_t)nw[0], S, dG, gKc, w));
+          TRY(lr_gram(ctx, prec, H[1].Z[g], (uint64_t)k[1], W0, (uint64_t)nw[0], S, dG, gWr, w));
+        }
+        if (nw[1]) {
+          TRY(lr_gram(ctx, prec, H[1].Z[g], (uint64_t)k[1], MW1, (uint64_t)nw[1], S, dG, gMc, w));
+          TRY(lr_gram(ctx, prec, H[0].Z[g], (uint64_t)k[0], W1, (uint64_t)nw[1], S, dG, gWc, w));
+        }
+      }
+      lr_assemble(k[0], k[0], n0, n0, w, hP[0], gKc, gKc, hGK);
+      lr_assemble(k[1], k[1], n1, n1, w, hP[1], gMc, gMc, hGM);
+      lr_assemble(k[0], k[1], n0, n1, w, hP[2], gWc, gWr, hGW);
+    }
+    gvalid = 1;
+    double tr = bdg_wtime();
+    const int got = dn_lrep_rr(k[0], k[1], m, cplx, hGK, hGM, hGW, w2, hA, hB);
+    if (got < m) {
+      rc = -100;
+      goto done;
+    }
+    /* orthonormal update coefficients; the new P from the columns whose W
+     * is in this basis */
+    int nc[2];
+    nc[0] = dn_lrep_update(k[0], m, cplx, hA, wact, nwact, 1e-3, hCQ[0], hR[0]);
+    nc[1] = dn_lrep_update(k[1], m, cplx, hB, wact, nwact, 1e-3, hCQ[1], hR[1]);
+    if (nc[0] < m || nc[1] < m) {
+      rc = -100;
+      goto done;
+    }
+    /* the Grams of the next [X | P], projected */
+    lr_project(k[0], k[0], nc[0], nc[0], cplx, hGK, hCQ[0], hCQ[0], hC, hP[0]);
+    lr_project(k[1], k[1], nc[1], nc[1], cplx, hGM, hCQ[1], hCQ[1], hC, hP[1]);
+    lr_project(k[0], k[1], nc[0], nc[1], cplx, hGW, hCQ[0], hCQ[1], hC, hP[2]);
+    trr += bdg_wtime() - tr;
+    for (int h = 0; h < 2; ++h) {
+      TRY(bdgx_memcpy_h2d(ctx, dC, hCQ[h], sizeof(double) * (size_t)k[h] * nc[h] * w));
+      TRY(bdgx_combine(ctx, prec, H[h].Z[g], S, (uint64_t)k[h], dC, H[h].Z[g ^ 1], S,
+                       (uint64_t)nc[h], S));
+      TRY(bdgx_combine(ctx, prec, H[h].Y[g], S, (uint64_t)k[h], dC, H[h].Y[g ^ 1], S,
+                       (uint64_t)nc[h], S));
+      H[h].np = nc[h] - m;
+      k[h] = nc[h];
+      nw[h] = 0;
+    }
+    g ^= 1;
+    /* residuals of the Ritz pairs u = X_U R_A, v = X_V R_B:
+     *   R_U = [K X_U | X_V] [R_A; -R_B s|w|],  R_V = [M X_V | X_U] [R_B; -R_A |w|]
+     * for the selected columns sel[0..nsel), packed in that order: all
+     * columns (the active ones first) every 10 iterations, else the active
+     * ones (the others keep the residual of the last full evaluation) */
+    int full = it - last_full >= 10 || nact == 0;
+    int nsel = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      nsel = 0;
+      if (full) {
+        for (int j = 0; j < m; ++j)
+          if (!conv[j]) sel[nsel++] = j;
+        for (int j = 0; j < m; ++j)
+          if (conv[j]) sel[nsel++] = j;
+      } else {
+        for (int i = 0; i < nact; ++i) sel[nsel++] = act[i];
+      }
+      for (int h = 0; h < 2; ++h) {
+        const double *Rs = hR[h], *Ro = hR[h ^ 1];
+        double *Cc = hC;
+        memset(Cc, 0, sizeof(double) * 2 * (size_t)m * nsel * w);
+        for (int c = 0; c < nsel; ++c) {
+          const int j = sel[c];
+          const double wt = sqrt(fabs(w2[j]));
+          const double f = (h == 0 && w2[j] < 0) ? -wt : wt;
+          for (int i = 0; i < m; ++i)
+            for (int q = 0; q < w; ++q) {
+              Cc[((size_t)i + (size_t)c * 2 * m) * w + q] = Rs[((size_t)i + (size_t)j * m) * w + q];
+              Cc[((size_t)(m + i) + (size_t)c * 2 * m) * w + q] =
+                  -f * Ro[((size_t)i + (size_t)j * m) * w + q];
+            }
+        }
+        TRY(bdgx_memcpy_h2d(ctx, dC, Cc, sizeof(double) * 2 * (size_t)m * nsel * w));
+        TRY(bdgx_combine2(ctx, prec, H[h].Y[g], S, (uint64_t)m, H[h ^ 1].Z[g], S, (uint64_t)m, dC,
+                          dR[h], S, (uint64_t)nsel, S));
+      }
+      TRY(bdgx_residual(ctx, prec, dR[0], dR[1], NULL, S, (uint64_t)nsel, S, dZero, 0, NULL, S,
+                        dN2));
+      TRY(bdgx_memcpy_d2h(ctx, hn2, dN2, sizeof(double) * 3 * (size_t)nsel));
+      /* ||A x - w B x|| / (||x|| max(1, |w|)), ||x||^2 from the coefficients
+       * (orthonormal bases) */
+      for (int c = 0; c < nsel; ++c) {
+        const int j = sel[c];
+        double xn2 = 0.0;
+        for (int h = 0; h < 2; ++h)
+          for (int i = 0; i < m * w; ++i)
+            xn2 += hR[h][(size_t)j * m * w + i] * hR[h][(size_t)j * m * w + i];
+        const double wt = sqrt(fabs(w2[j]));
+        resv[j] = sqrt(hn2[3 * c] + hn2[3 * c + 1]) / (sqrt(xn2) * fmax(1.0, wt));
+        if (resv[j] < 0.5 * best[j]) {
+          best[j] = resv[j];
+          best_it[j] = it;
+        }
+        /* FP32 phase: a column also leaves when its residual stopped falling
+         * (the rounding floor of FP32 images) */
+        conv[j] = resv[j] < cfg->tol ||
+                  (cfg->stall_window > 0 && resv[j] < cfg->stall_below &&
+                   it - best_it[j] >= cfg->stall_window);
+      }
+      if (full) last_full = it;
+      nact = 0;
+      for (int j = 0; j < m; ++j)
+        if (!conv[j]) act[nact++] = j;
+      int allc = 1;
+      for (int j = 0; j < nev; ++j)
+        if (!conv[j]) allc = 0;
+      /* a partial evaluation that finds everything converged is confirmed by
+       * a full one */
+      if (allc && !full) {
+        full = 1;
+        continue;
+      }
+      break;
+    }
+    double worst = 0.0;
+    int allc = 1;
+    for (int j = 0; j < nev; ++j) {
+      worst = fmax(worst, resv[j]);
+      if (!conv[j]) allc = 0;
+    }
+    if (cfg->verbosity > 1 || (cfg->verbosity == 1 && it % 10 == 0))
+      fprintf(stderr, "[bdg] %s it %d t %.3f w0 %.10g wlast %.10g worst res %.3e active %d\n",
+              prec == BDGX_FP64 ? "fp64" : "fp32", it, bdg_wtime() - t0,
+              w2[0] < 0 ? -sqrt(-w2[0]) : sqrt(w2[0]),
+              w2[nev - 1] < 0 ? -sqrt(-w2[nev - 1]) : sqrt(w2[nev - 1]), worst, nact);
+    res->worst = worst;
+    if (allc || it == cfg->maxiter) break;
+    /* new directions of the active columns, orthonormalised against
+     * [X | P], and their images.  The residual of act[i] sits in column
+     * pos[i] of dR (the evaluation order); when that is not i the columns
+     * are gathered first. */
+    int packed = 1;
+    for (int i = 0; i < nact; ++i) {
+      int p = 0;
+      while (p < nsel && sel[p] != act[i]) ++p;
+      wact[i] = p; /* position, reused below as the column list */
+      if (p != i) packed = 0;
+    }
+    for (int h = 0; h < 2; ++h) {
+      void *wsrc = dR[h], *wtmp = dWt[h];
+      if (!packed) {
+        TRY(bdgx_gather_cols(ctx, prec, dR[h], S, wact, (uint64_t)nact, dWt[h], S, S));
+        wsrc = dWt[h];
+        wtmp = dR[h];
+      }
+      TRY(bdgx_apply(ctx, opT[h], prec, wsrc, wtmp, (uint64_t)nact, S));
+      const int nq = k[h];
+      int r = 0;
+      TRY(lr_orth(ctx, prec, cplx, H[h].Z[g], nq, wtmp, nact, S, 1e-3, dG, dC, hG, hC, &r));
+      if (r > 0)
+        TRY(bdgx_apply(ctx, opA[h], prec, lr_col(H[h].Z[g], (uint64_t)nq, S, es),
+                       lr_col(H[h].Y[g], (uint64_t)nq, S, es), (uint64_t)r, S));
+      nw[h] = r;
+      k[h] = nq + r;
+      res->applies += (uint64_t)r;
+    }
+    nwact = nact;
+    memcpy(wact, act, sizeof(int) * (size_t)nact);
+    res->iters = it + 1;
+  }
+  /* output: u = X_U R_A, v = X_V R_B (u negated for w^2 < 0, so that
+   * M v = w u with the reported w = -|w|) into dXU, dXV (prec0) */
+  {
+    for (int j = 0; j < m; ++j)
+      if (w2[j] < 0)
+        for (int i = 0; i < m * w; ++i) hR[0][(size_t)j * m * w + i] = -hR[0][(size_t)j * m * w + i];
+    void *dst[2] = {dXU, dXV};
+    for (int h = 0; h < 2; ++h) {
+      TRY(bdgx_memcpy_h2d(ctx, dC, hR[h], sizeof(double) * (size_t)m * m * w));
+      if (prec0 == prec) {
+        TRY(bdgx_combine(ctx, prec, H[h].Z[g], S, (uint64_t)m, dC, dst[h], S, (uint64_t)m, S));
+      } else {
+        TRY(bdgx_combine(ctx, prec, H[h].Z[g], S, (uint64_t)m, dC, dWt[h], S, (uint64_t)m, S));
+        TRY(bdgx_convert(ctx, prec, dWt[h], S, prec0, dst[h], S, S, (uint64_t)m));
+      }
+    }
+    for (int j = 0; j < m; ++j) {
+      res->w2[j] = w2[j];
+      res->res[j] = resv[j];
+    }
+    TRY(bdgx_sync(ctx));
+  }
+done:
+  res->rr_seconds += trr;
+  res->status = rc;
+  for (int h = 0; h < 2; ++h) {
+    for (int q = 0; q < 2; ++q) {
+      bdgx_free(H[h].Z[q]);
+      bdgx_free(H[h].Y[q]);
+    }
+    bdgx_free(dR[h]);
+    bdgx_free(dWt[h]);
+    free(hCQ[h]);
+    free(hR[h]);
+  }
+  for (int q = 0; q < 3; ++q) free(hP[q]);
+  bdgx_free(dG);
+  bdgx_free(dC);
+  bdgx_free(dZero);
+  bdgx_free(dN2);
+  free(hGK);
+  free(hGM);
+  free(hGW);
+  free(hG);
+  free(hC);
+  free(hA);
+  free(hB);
+  free(w2);
+  free(hn2);
+  free(resv);
+  free(best);
+  free(best_it);
+  free(conv);
+  free(act);
+  free(wact);
+  free(sel);
+  return rc;
+}

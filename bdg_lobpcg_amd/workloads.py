@@ -130,7 +130,13 @@ def train_learned(cfg, steps=300, lr=1e-3, reg=0.3, snapshots=(4, 8, 14, 22, 32,
     t0 = time.perf_counter()
     tcfg = dict(cfg, g=cfg["g"] * 0.75, edd=cfg["edd"] * 0.875)
     psi_t, mu_t, _ = ground_state(tcfg)
-    mk = lambda maxiter: make_problem(tcfg, psi_t, mu_t, maxiter=maxiter)
+    def mk(maxiter):
+        # the network replaces the product form's T_M T_K: its training pairs
+        # come from that iteration
+        p = make_problem(tcfg, psi_t, mu_t, maxiter=maxiter)
+        p.set_formulation(1)
+        return p
+
     ref, d, snaps = LP.collect(mk, tcfg["nev"], snapshots=snapshots)
     th0 = LP.init_theta(0)
     loss0 = LP.evaluate(d, th0, snaps)

@@ -24,6 +24,25 @@ typedef enum {
 
 void bdg_set_precision(bdg_t *bdg, bdg_precision_t prec);
 
+/* Form of the eigen-iteration.
+ *   BDG_FORM_FIRST_ORDER (default): LOBPCG on the reference's first-order
+ *     pencil K u = w v, M v = w u (reference src/solver.c:155-164: A = diag(K,
+ *     M), B = Swap, T = [T_K; T_M]) with separate, orthonormal search spaces
+ *     for u and v.  With BDG_PREC_MIXED: an FP32 phase (FP32 basis, images,
+ *     operator applies; FP64 Grams and Rayleigh-Ritz) until the FP32 residual
+ *     target or the FP32 rounding floor, then an FP64 phase from its vectors
+ *     (FP64 sweep, FP64 iterations to tol if the sweep's residuals need them).
+ *   BDG_FORM_PRODUCT: LOBPCG on (M K M) v = w^2 M v for the v half (the
+ *     form used when K == M, and by the learned preconditioner).
+ * Both give the same eigenpairs. */
+typedef enum { BDG_FORM_FIRST_ORDER = 0, BDG_FORM_PRODUCT = 1 } bdg_formulation_t;
+void bdg_set_formulation(bdg_t *bdg, bdg_formulation_t form);
+
+/* Mixed precision, first-order form: the FP32 phase stops a column at
+ * residual max(tol, fp32_tol) (default 1e-5), or when its residual has not
+ * halved in stall_window iterations (default 40; 0: never) once below 1e-4. */
+void bdg_set_fp32_phase(bdg_t *bdg, f64 fp32_tol, uint64_t stall_window);
+
 typedef struct {
   uint64_t iterations;          /* LOBPCG iterations                           */
   uint64_t column_applies;      /* (M then K) column applies, incl. start      */
@@ -38,6 +57,8 @@ typedef struct {
   double seconds_rr;            /* host Rayleigh-Ritz time                     */
   int status;                   /* 0 ok, else a BDGX_E* code or -100 (RR)     */
   uint64_t frozen_skips;        /* block columns the update left unwritten     */
+  uint64_t iterations_fp32;     /* of the iterations, those of the FP32 phase  */
+  double fp32_worst_residual;   /* worst residual when the FP32 phase ended    */
 } bdg_solve_stats_t;
 
 void bdg_get_solve_stats(const bdg_t *bdg, bdg_solve_stats_t *out);

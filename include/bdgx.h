@@ -126,6 +126,15 @@ int bdgx_gram_pair(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64
 int bdgx_combine(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda,
                  uint64_t k, const double *dC, void *dY, uint64_t ldy,
                  uint64_t m, uint64_t nrows);
+/* dY[:, 0:m] = [dA1 | dA2] * C: k1 columns of dA1 (ld lda1) followed by k2
+ * columns of dA2 (ld lda2); C is (k1 + k2) x m column-major in device memory
+ * (no reference counterpart: the first-order solver's residuals
+ * [K X_U | X_V] [R_A; -R_B D] and its orthonormalisation of new directions
+ * against [X | P], which live in different blocks).  dY must alias neither
+ * source.  k2 = 0 is bdgx_combine. */
+int bdgx_combine2(bdgx_ctx *ctx, int prec, const void *dA1, uint64_t lda1, uint64_t k1,
+                  const void *dA2, uint64_t lda2, uint64_t k2, const double *dC, void *dY,
+                  uint64_t ldy, uint64_t m, uint64_t nrows);
 #define BDGX_COMBINE_COLS_MAX 8
 /* dY[:, cols[j]] = dA[:, 0:k] * C[:, j] for j < ncols <= BDGX_COMBINE_COLS_MAX
  * (C is k x ncols

@@ -122,14 +122,16 @@ def oracle_problem(cfg, psi, mu):
 def oracle_mode_residuals(c, w, U, V, chunk=8):
     """Residuals of device modes under the oracle's operators (FP64 C,
     reference formulas), column by column:
-      first order  ||K u - w v|| + ||M v - w u|| over ||[u; v]|| max(1, |w|)
-                   (inf for w^2 < 0, where no real pair exists);
-      pencil       the solver's convergence measure for x = [M v / w; v]:
-                   (||K M v - w^2 v|| / |w|) / (||x|| max(1, |w|)), with w^2
-                   carrying the sign of the reported w.
-    Returns (first_order[nev], pencil[nev])."""
+      first order  the reference pencil's ||A x - w B x|| / (||x|| max(1, |w|))
+                   for x = [u; v] (A = diag(K, M), B = Swap): K u = |w| v,
+                   M v = w u -- for w < 0 (a pair with imaginary frequency,
+                   reported as -|w| with M v = w u) K u = |w| v still holds;
+      product      (||K M v - w^2 v|| / |w|) / (||[M v / w; v]|| max(1, |w|)),
+                   w^2 carrying the sign of w (the product-form solver's
+                   convergence measure).
+    Returns (first_order[nev], product[nev])."""
     nev = len(w)
-    first = np.full(nev, np.inf)
+    first = np.zeros(nev)
     pen = np.zeros(nev)
     for j0 in range(0, nev, chunk):
         J = slice(j0, min(nev, j0 + chunk))
@@ -143,7 +145,7 @@ def oracle_mode_residuals(c, w, U, V, chunk=8):
         xnorm = np.sqrt(np.linalg.norm(v, axis=0) ** 2 + (np.linalg.norm(Mv, axis=0) / aw) ** 2)
         pen[J] = (rn / aw) / (xnorm * np.maximum(1.0, aw))
         xn = np.sqrt(np.linalg.norm(u, axis=0) ** 2 + np.linalg.norm(v, axis=0) ** 2)
-        r1 = (np.linalg.norm(Ku - v * wj[None, :], axis=0) +
-              np.linalg.norm(Mv - u * wj[None, :], axis=0)) / (xn * np.maximum(1.0, np.abs(wj)))
-        first[J] = np.where(wj > 0, r1, np.inf)
+        r1 = np.sqrt(np.linalg.norm(Ku - v * aw[None, :], axis=0) ** 2 +
+                     np.linalg.norm(Mv - u * wj[None, :], axis=0) ** 2)
+        first[J] = r1 / (xn * np.maximum(1.0, aw))
     return first, pen

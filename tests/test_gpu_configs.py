@@ -6,7 +6,7 @@
   the CPU reference solver (numpy iLOBPCG on the C oracle's operators), same
   oracle-ITP psi0, parameters and seed-42 start block;
 * the 128^3 configuration at full size: every device FP64 mode checked with
-  the oracle's K and M (first-order and pencil residuals), and the
+  the oracle's K and M (the reference pencil's residual), and the
   mixed solve's omega against the FP64 solve's;
 * a complex problem with learned-preconditioner parameters set solves with
   the analytic preconditioner (the network is real-psi0 only).
@@ -107,12 +107,12 @@ def test_cfg0_full_size_vs_oracle_solver(cfg0_reference, prec):
 
 # ------------------------------------------------------ 128^3 configuration
 def test_cfg1_full_size_modes_under_oracle_operators():
-    """128^3 cigar, nev = block = 32: every FP64 device mode satisfies the
-    oracle's K u = w v, M v = w u (first-order residual <= 1e-7 for w > 0)
-    and meets the solve's tolerance 1e-8 in the solver's own pencil measure
-    evaluated with the oracle's K and M (up to 1.5x for the rounding of the
-    two operator implementations), and the mixed
-    solve's omega agree with FP64 to 1e-7 relative (absolute for |w| < 1e-2)."""
+    """128^3 cigar, nev = block = 32: every FP64 device mode meets the
+    solve's tolerance 1e-8 in the reference pencil's residual
+    ||A x - w B x|| / (||x|| max(1, |w|)) evaluated with the oracle's K and M
+    (up to 1.5x for the rounding of the two operator implementations), and
+    the mixed solve's omega agree with FP64 to 1e-7 relative (absolute for
+    |w| < 1e-2)."""
     from bdg_lobpcg_amd.workloads import CONFIGS, ground_state, make_problem
     cfg = CONFIGS["cfg1_128c_d"]
     psi, mu, t_itp = ground_state(cfg)
@@ -139,12 +139,10 @@ def test_cfg1_full_size_modes_under_oracle_operators():
     c, _ = oracle_problem(cfg, psi, mu)
     t0 = time.perf_counter()
     first, pen = oracle_mode_residuals(c, w64, U, V)
-    fin = np.isfinite(first)
-    _log(f"  oracle residuals ({time.perf_counter() - t0:.1f} s): first-order max "
-         f"{first[fin].max():.2e} over {fin.sum()} modes, pencil max {pen.max():.2e}; "
+    _log(f"  oracle residuals ({time.perf_counter() - t0:.1f} s): pencil max {first.max():.2e} "
+         f"over {len(first)} modes (product-form measure max {pen.max():.2e}); "
          f"w0 {w64[0]:.6g}")
-    assert first[fin].max() < 1e-7
-    assert pen.max() < 1.5 * cfg["tol"]
+    assert first.max() < 1.5 * cfg["tol"]
 
 
 def test_complex_problem_with_learned_theta_uses_analytic_preconditioner():

@@ -27,24 +27,32 @@ def test_1d_uniform_bogoliubov(cplx):
     np.testing.assert_allclose(w[1:], [E1, E1, E2], rtol=1e-9)
 
 
+_REF = {}  # CPU reference solves, per psi0 kind (shared by the parameter cases)
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["first_order", "product"])
+@pytest.mark.parametrize("cplx", [False, True], ids=["real", "complex"])
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
-def test_3d_dipolar_vs_oracle(prec):
-    """Small 3D dipolar problem: device solve vs the CPU reference solver
-    (numpy iLOBPCG on the oracle operators), same psi0 and start block:
-    omega to 1e-9, and the modes' subspaces to 1e-6 rad."""
+def test_3d_dipolar_vs_oracle(prec, cplx, form):
+    """Small 3D dipolar problem: device solve (first-order or product form;
+    FP64, or mixed: FP32 phase + FP64 phase for the first-order form) vs the
+    CPU reference solver (numpy iLOBPCG on the oracle operators), same psi0
+    and start block: omega to 1e-9, and the modes' subspaces to 1e-6 rad."""
     import bdg_lobpcg_amd as B
     from oracle.ilobpcg_ref import ilobpcg, solver_precond_weights
     N, L = (16, 16, 16), (12.0, 12.0, 12.0)
-    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
+    c, f = physics(N, L, cplx, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
     S = c.size
     nev, m = 6, 8
     pKs, pMs = solver_precond_weights(f["ltK"], f["ltM"], f["mu"])
     c.set_fields(pK=pKs, pM=pMs, mu=f["mu"])
     X0 = c.start_block_uniform(42, m)
-    ref = ilobpcg(lambda X: c.apply("K", X), lambda X: c.apply("M", X),
-                  lambda X: c.apply("precondK", X), lambda X: c.apply("precondM", X),
-                  X0[S:], nev, tol=1e-11, maxiter=400)
-    p = B.Problem(N, L, False)
+    if cplx not in _REF:
+        _REF[cplx] = ilobpcg(lambda X: c.apply("K", X), lambda X: c.apply("M", X),
+                             lambda X: c.apply("precondK", X), lambda X: c.apply("precondM", X),
+                             X0[S:], nev, tol=1e-11, maxiter=400)
+    ref = _REF[cplx]
+    p = B.Problem(N, L, cplx)
     p.set_system()
     p.set_trap_harmonic((1.0, 1.0, 1.0))
     p.set_wavefunction(f["wf"])
@@ -53,8 +61,15 @@ def test_3d_dipolar_vs_oracle(prec):
     p.set_mu(1.0)
     p.set_solver_params(nev, m, 400, 1e-10)
     p.set_precision(prec)
+    p.set_formulation(form)
     p.set_init_custom_block(X0)
+    print(f"3d dipolar vs oracle: prec {prec} cplx {cplx} form {form}", flush=True)
     assert p.solve() == 0
+    st = p.stats()
+    print(f"  iterations {st['iterations']} (fp32 {st['iterations_fp32']}), "
+          f"fp32 applies {st['column_applies_fp32']}", flush=True)
+    if prec == 1 and form == 0:
+        assert st["iterations_fp32"] > 0 and st["column_applies_fp32"] > 0
     w = p.eigenvalues(nev)
     z = p.zero_modes(nev) | ref["zero"]
     nz = ~z
@@ -86,6 +101,7 @@ def test_graph_capture_gives_identical_solve():
         p.set_mu(1.0)
         p.set_solver_params(6, 8, 300, 1e-9)
         p.set_init_custom_block(X0)
+        p.set_formulation(B.FORM_PRODUCT)  # the product form replays graphs
         p.set_graph_capture(on)
         assert p.solve() == 0
         out[on] = (p.eigenvalues(6), p.stats())
@@ -182,6 +198,7 @@ def test_frozen_columns_match_soft_locking(cplx):
         p.set_mu(1.0)
         p.set_solver_params(nev, m, 400, 1e-9)
         p.set_init_custom_block(X0)
+        p.set_formulation(B.FORM_PRODUCT)  # frozen columns are a product-form feature
         p.set_freeze(tol)
         assert p.solve() == 0
         out[tol] = (p.eigenvalues(nev), p.zero_modes(nev), p.stats(), p.converged(),

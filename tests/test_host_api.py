@@ -276,3 +276,117 @@ def test_host_gram_projection_and_independence_test(cplx):
         Vz[:, k - 1] = 0
         Gz = np.asfortranarray((Vz.conj().T @ Vz).astype(dt))
         assert lib.dn_chol_ok(k, cplx, Gz.ctypes.data, 1e-10) == 0
+
+
+def _lrep_lib():
+    import ctypes as C
+    lib = B.lib()
+    lib.dn_lrep_rr.restype = C.c_int
+    lib.dn_lrep_rr.argtypes = [C.c_int] * 4 + [C.c_void_p] * 6
+    lib.dn_lrep_update.restype = C.c_int
+    lib.dn_lrep_update.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.c_double, C.c_void_p, C.c_void_p]
+    lib.dn_orth_coef.restype = C.c_int
+    lib.dn_orth_coef.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                 C.c_void_p]
+    return lib
+
+
+def _herm_pd(rng, n, cplx, lo, hi):
+    a = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+    Q, _ = np.linalg.qr(a)
+    return (Q * np.linspace(lo, hi, n)[None, :]) @ Q.conj().T
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+def test_host_first_order_rayleigh_ritz(cplx):
+    """dn_lrep_rr (the first-order pencil solver's host Rayleigh-Ritz): on
+    the full space its w^2 are the exact eigenvalues of M K (numpy eigvals of
+    the dense product, the first-order pencil K u = w v, M v = w u; one pair
+    with w^2 < 0 from a K with a negative eigenvalue); on random orthonormal
+    subspaces of unequal sizes its pairs satisfy the Galerkin conditions
+    G_K a = s|w| G_W b, G_M b = |w| G_W^H a with s = sign(w^2), normalised."""
+    lib = _lrep_lib()
+    rng = np.random.default_rng(11 + cplx)
+    dt = np.complex128 if cplx else np.float64
+    n, m = 40, 6
+    K = _herm_pd(rng, n, cplx, -0.02, 30.0)
+    M = _herm_pd(rng, n, cplx, 0.5, 25.0)
+    F = lambda a: np.asfortranarray(a.astype(dt))
+    # full space: Z_U = Z_V = I
+    w2 = np.zeros(m)
+    A = np.zeros((n, m), dtype=dt, order="F")
+    Bc = np.zeros((n, m), dtype=dt, order="F")
+    I = np.eye(n)
+    Kf, Mf, If = F(K), F(M), F(I)  # kept alive through the call
+    got = lib.dn_lrep_rr(n, n, m, cplx, Kf.ctypes.data, Mf.ctypes.data, If.ctypes.data,
+                         w2.ctypes.data, A.ctypes.data, Bc.ctypes.data)
+    assert got == m
+    ref = np.sort(np.linalg.eigvals(M @ K).real)[:m]
+    assert ref[0] < 0 < ref[1]
+    np.testing.assert_allclose(w2, ref, rtol=1e-10, atol=1e-12)
+    # subspaces of 17 and 14 columns
+    ku, kv = 17, 14
+    ZU, _ = np.linalg.qr(rng.standard_normal((n, ku)) + (1j * rng.standard_normal((n, ku)) if cplx else 0))
+    ZV, _ = np.linalg.qr(rng.standard_normal((n, kv)) + (1j * rng.standard_normal((n, kv)) if cplx else 0))
+    GK, GM, GW = ZU.conj().T @ K @ ZU, ZV.conj().T @ M @ ZV, ZU.conj().T @ ZV
+    A = np.zeros((ku, m), dtype=dt, order="F")
+    Bc = np.zeros((kv, m), dtype=dt, order="F")
+    GKf, GMf, GWf = F(GK), F(GM), F(GW)
+    got = lib.dn_lrep_rr(ku, kv, m, cplx, GKf.ctypes.data, GMf.ctypes.data, GWf.ctypes.data,
+                         w2.ctypes.data, A.ctypes.data, Bc.ctypes.data)
+    assert got == m
+    wt, sg = np.sqrt(np.abs(w2)), np.sign(w2)
+    r1 = GK @ A - GW @ Bc * (sg * wt)[None, :]
+    r2 = GM @ Bc - GW.conj().T @ A * wt[None, :]
+    assert max(np.abs(r1).max(), np.abs(r2).max()) < 1e-11 * 30
+    nrm = (np.abs(A) ** 2).sum(0) + (np.abs(Bc) ** 2).sum(0)
+    np.testing.assert_allclose(nrm, 1.0, rtol=1e-12)
+    assert np.all(np.diff(w2) >= 0)
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+def test_host_first_order_update_and_orthonormalisation(cplx):
+    """dn_lrep_update: [Q1 | Q2] orthonormal, A = Q1 R11 with R11 upper
+    triangular, span(Q2) = the part of A[m:, act] outside span(A), a
+    dependent P column dropped.  dn_orth_coef: [Q | W] C is orthonormal and
+    orthogonal to Q; a W column inside span(Q) is left out."""
+    lib = _lrep_lib()
+    rng = np.random.default_rng(3 + cplx)
+    dt = np.complex128 if cplx else np.float64
+
+    def rnd(*shape):
+        a = rng.standard_normal(shape)
+        return a + 1j * rng.standard_normal(shape) if cplx else a
+
+    k, m = 30, 8
+    A = np.asfortranarray(rnd(k, m).astype(dt))
+    act = np.array([1, 3, 4, 6], dtype=np.int32)
+    A[m:, 6] = 2.0 * A[m:, 3]  # its P part duplicates column 3's
+    A = np.asfortranarray(A)
+    CQ = np.zeros((k, k), dtype=dt, order="F")
+    R11 = np.zeros((m, m), dtype=dt, order="F")
+    nc = lib.dn_lrep_update(k, m, cplx, A.ctypes.data, act.ctypes.data, len(act), 1e-3,
+                            CQ.ctypes.data, R11.ctypes.data)
+    assert nc == m + 3
+    Q = CQ[:, :nc]
+    assert np.abs(Q.conj().T @ Q - np.eye(nc)).max() < 1e-13
+    assert np.abs(Q[:, :m] @ R11 - A).max() < 1e-13 * np.abs(A).max()
+    assert np.abs(np.tril(R11, -1)).max() == 0
+    A0 = A[:, act].copy()
+    A0[:m] = 0
+    # every P direction lies in span(Q)
+    assert np.abs(A0 - Q @ (Q.conj().T @ A0)).max() < 1e-12 * np.abs(A0).max()
+    # orthonormalisation of W against Q
+    n, nq, nw = 50, 7, 5
+    Qb, _ = np.linalg.qr(rnd(n, nq))
+    W = rnd(n, nw)
+    W[:, 2] = Qb @ rnd(nq) + 1e-6 * rnd(n)  # almost inside span(Q)
+    GQW = np.asfortranarray((Qb.conj().T @ W).astype(dt))
+    GWW = np.asfortranarray((W.conj().T @ W).astype(dt))
+    C = np.zeros((nq + nw, nw), dtype=dt, order="F")
+    r = lib.dn_orth_coef(nq, nw, cplx, GQW.ctypes.data, GWW.ctypes.data, 1e-3, C.ctypes.data)
+    assert r == nw - 1
+    Wn = np.concatenate([Qb, W], axis=1) @ C[:, :r]
+    assert np.abs(Wn.conj().T @ Wn - np.eye(r)).max() < 1e-10
+    assert np.abs(Qb.conj().T @ Wn).max() < 1e-10

@@ -122,8 +122,10 @@ def test_trained_network_gain_on_32c():
     """What the trained network gains over the method without it, on the
     32^3 dipolar configuration (box [-8,8]^3, g = 100, eps_dd = 0.5, nev =
     block = 10, tol 1e-8): the network is trained on the GPU on pairs from a
-    related problem (g x 0.75, eps_dd x 0.875), then the same mixed solve runs
-    from the same start block with T_M T_K and with the network.  Both reach
+    related problem (g x 0.75, eps_dd x 0.875), then the same mixed
+    product-form solve (the form whose preconditioner T_M T_K the network
+    replaces) runs from the same start block with T_M T_K and with the
+    network.  Both reach
     the same omega (<= 1e-9 relative, non-zero modes) with every mode
     converged; the network lowers the training loss and the iteration count."""
     from bdg_lobpcg_amd.workloads import CONFIGS, ground_state, make_problem, train_learned
@@ -136,6 +138,7 @@ def test_trained_network_gain_on_32c():
     out = {}
     for name, th in (("analytic", None), ("learned", theta)):
         p = make_problem(cfg, psi, mu, precision=1)
+        p.set_formulation(1)  # product form
         if th is not None:
             p.set_learned(th)
         assert p.solve() == 0

@@ -441,3 +441,54 @@ def test_reference_solver_soft_mode_meets_the_residual_test():
     assert out["zero"][0] and out["res"][0] < 1e-10
     assert abs(out["w"][0] - 1e-4) < 1e-9
     np.testing.assert_allclose(out["w"][1:], np.sqrt(d[1:nev]), rtol=1e-9)
+
+
+@pytest.mark.parametrize("tag", ["eig0", "eig1", "eig2"])
+def test_first_order_reference_solver_vs_dense(tag):
+    """The numpy first-order solver (oracle/lrep_ref.py, the restatement of
+    the device solver's iteration) reproduces the dense spectra (identity
+    preconditioner, as for the product-form reference solver above)."""
+    from oracle.lrep_ref import lrep_lobpcg
+    c, w_ref = _dense_case(tag)
+    S = c.size
+    nev, m = 8, 10
+    X0 = c.start_block_uniform(42, m)
+    r = lrep_lobpcg(lambda X: c.apply("K", X), lambda X: c.apply("M", X),
+                    lambda X: X, lambda X: X, X0[:S], X0[S:], nev, tol=1e-11, maxiter=800)
+    np.testing.assert_allclose(r["w"], w_ref[:nev], rtol=1e-9)
+    # the modes satisfy the first-order pencil under the oracle's operators
+    Ku, Mv = c.apply("K", r["u"]), c.apply("M", r["v"])
+    assert np.abs(Ku - r["v"] * r["w"]).max() < 1e-8 * max(1, r["w"].max())
+    assert np.abs(Mv - r["u"] * r["w"]).max() < 1e-8 * max(1, r["w"].max())
+
+
+def test_first_order_reference_solver_fp32_phase_then_fp64():
+    """3D dipolar problem with the solver's preconditioner: the first-order
+    iteration in FP64 gives the product-form reference solver's omega, and
+    the mixed policy (FP32-rounded storage and operators to residual 1e-5,
+    then FP64 from those vectors) gives the same omega to 1e-9."""
+    from oracle.ilobpcg_ref import ilobpcg, solver_precond_weights
+    from oracle.lrep_ref import lrep_lobpcg
+    from tests.common import physics
+    N, L = (12, 12, 16), (10.0, 10.0, 12.0)
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=8.0, mu=1.0)
+    S = c.size
+    pKs, pMs = solver_precond_weights(f["ltK"], f["ltM"], f["mu"])
+    c.set_fields(pK=pKs, pM=pMs, mu=f["mu"])
+    ops = [lambda X, o=o: c.apply(o, X) for o in ("K", "M", "precondK", "precondM")]
+    X0 = c.start_block_uniform(42, 8)
+    nev = 6
+    ref = ilobpcg(*ops, X0[S:], nev, tol=1e-11, maxiter=600)
+    r64 = lrep_lobpcg(*ops, X0[:S], X0[S:], nev, tol=1e-10, maxiter=600)
+    small = np.abs(ref["w"]) < 1e-2
+    scale = np.where(small, 1.0, np.abs(ref["w"]))
+    assert np.all(np.abs(r64["w"] - ref["w"]) / scale < 1e-9)
+    f32 = lambda X: np.asarray(X, np.complex64 if np.iscomplexobj(X) else np.float32)
+    o32 = [lambda X, op=op: op(np.asarray(X, np.float32).astype(np.float64)).astype(np.float32)
+           for op in ops]
+    r32 = lrep_lobpcg(*o32, X0[:S], X0[S:], 8, tol=1e-5, maxiter=600, rnd=f32)
+    U = np.asarray(r32["u"], np.float64)
+    V = np.asarray(r32["v"], np.float64)
+    rfin = lrep_lobpcg(*ops, U, V, nev, tol=1e-10, maxiter=600)
+    assert np.all(np.abs(rfin["w"] - ref["w"]) / scale < 1e-9)
+    assert rfin["iters"] < r64["iters"]
