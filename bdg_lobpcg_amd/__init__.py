@@ -151,6 +151,7 @@ def lib():
         "bdg_get_solve_stats": (None, [P, C.POINTER(SolveStats)]),
         "bdg_set_formulation": (None, [P, I]),
         "bdg_set_fp32_phase": (None, [P, D, U64]),
+        "bdg_set_locking": (None, [P, I]),
         "bdg_residuals": (C.POINTER(D), [P]),
         "bdg_zero_modes": (C.POINTER(I), [P]),
         "bdg_device_context": (P, [P]),
@@ -559,6 +560,11 @@ class Problem:
 
     def set_fp32_phase(self, fp32_tol=1e-5, stall_window=20):
         lib().bdg_set_fp32_phase(self.h, float(fp32_tol), int(stall_window))
+
+    def set_locking(self, max_active=-1):
+        """Lock converged pairs once at most max_active columns are active
+        (-1 automatic, 0 never)."""
+        lib().bdg_set_locking(self.h, int(max_active))
 
     def set_verbosity(self, level):
         lib().bdg_set_verbosity(self.h, int(level))

@@ -121,6 +121,7 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
   b->formulation = BDG_FORM_FIRST_ORDER;
   b->fp32_tol = 1e-5;
   b->fp32_stall = 40;
+  b->lock_max = -1;
   return b;
 }
 
@@ -574,6 +575,8 @@ static int solve_first_order(bdg_t *b, const void *dV0, double t_start) {
   cfg.nev = (int)nev;
   cfg.cplx = c;
   cfg.verbosity = b->verbosity;
+  /* automatic locking: once at most max(2, m / 8) columns are active */
+  cfg.lock_max = b->lock_max >= 0 ? b->lock_max : (int)(m / 8 > 2 ? m / 8 : 2);
   lrep_result r32, r64;
   memset(&r32, 0, sizeof(r32));
   memset(&r64, 0, sizeof(r64));
@@ -911,6 +914,7 @@ const void *bdg_modes_v(const bdg_t *b) { return b->modes_v; }
 
 void bdg_set_precision(bdg_t *b, bdg_precision_t p) { b->precision = (int)p; }
 void bdg_set_formulation(bdg_t *b, bdg_formulation_t f) { b->formulation = (int)f; }
+void bdg_set_locking(bdg_t *b, int max_active) { b->lock_max = max_active; }
 void bdg_set_fp32_phase(bdg_t *b, f64 fp32_tol, uint64_t stall_window) {
   b->fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
   b->fp32_stall = (int)stall_window;

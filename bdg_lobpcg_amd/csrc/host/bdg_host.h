@@ -73,6 +73,8 @@ struct bdg_t {
   double fp32_tol;    /* mixed, first order: residual at which the FP32 phase    */
   int fp32_stall;     /*   stops; columns also stop after this many iterations  */
                       /*   without halving their residual (0: never)            */
+  int lock_max;       /* first order: lock converged pairs once at most this   */
+                      /*   many columns are active (0: never, < 0: automatic)   */
   double freeze_tol; /* converged columns moved less than this keep their vectors */
   int verbosity;
   /* results */
@@ -159,6 +161,10 @@ typedef struct {
   int stall_window;    /* ... or (0: never) when its residual has not halved in  */
   double stall_below;  /*   stall_window iterations and is below stall_below     */
   int verbosity;
+  int lock_max;        /* lock the converged pairs once at most this many       */
+                       /*   columns are active (0: never), see lrep.c           */
+  const void *dUL, *dVL; /* locked pairs (S x nl, phase precision): new         */
+  int nl;                /*   directions are deflated against them              */
 } lrep_cfg;
 
 typedef struct {

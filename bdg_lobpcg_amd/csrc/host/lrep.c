@@ -23,6 +23,18 @@
  * never amplified: the same iteration runs with FP32 basis, images and
  * operator applies (Grams accumulate in FP64).
  *
+ * Locking.  Once at most cfg->lock_max columns are still active, the
+ * converged Ritz pairs (u_c, v_c) are locked and the active ones continue in
+ * a small block (a nested call with those pairs as its deflation set).
+ * Eigenvectors of the pencil with different w^2 are biorthogonal
+ * (u_i^H v_j = 0: the u are left and the v right eigenvectors of K M), so
+ * the remaining pairs lie in {u : V_L^H u = 0} x {v : U_L^H v = 0}; the new
+ * directions are projected there obliquely,
+ *   W_U <- W_U - U_L (V_L^H U_L)^-1 V_L^H W_U,
+ *   W_V <- W_V - V_L (U_L^H V_L)^-1 U_L^H W_V,
+ * and the tail iterations touch only the active columns and one pass over
+ * the locked ones, instead of the whole block.
+ *
  * One call is one phase at one storage precision, started from given X
  * blocks; the mixed policy runs an FP32 phase to the residual FP32 can
  * reach, then an FP64 phase started from its X blocks, whose first
@@ -173,6 +185,98 @@ static void lr_assemble(int ku, int kv, int nu, int nv, int w, const double *P, 
     }
 }
 
+/* Ainv = A^-1 (n x n, complex interleaved when cplx) by Gauss-Jordan with
+ * partial pivoting; returns 0, or -1 when A is singular to working precision */
+static int lr_invert(int n, int cplx, const double *A, double *Ainv) {
+  double complex *a = (double complex *)malloc(sizeof(double complex) * (size_t)n * n);
+  double complex *b = (double complex *)malloc(sizeof(double complex) * (size_t)n * n);
+  int bad = 0;
+  double amax = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      const size_t e = (size_t)i + (size_t)j * n;
+      a[e] = cplx ? A[2 * e] + I * A[2 * e + 1] : A[e];
+      b[e] = i == j ? 1.0 : 0.0;
+      amax = fmax(amax, cabs(a[e]));
+    }
+  for (int c = 0; c < n && !bad; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (cabs(a[r + (size_t)c * n]) > cabs(a[p + (size_t)c * n])) p = r;
+    if (!(cabs(a[p + (size_t)c * n]) > 1e-13 * amax)) {
+      bad = 1;
+      break;
+    }
+    if (p != c)
+      for (int j = 0; j < n; ++j) {
+        double complex t = a[c + (size_t)j * n];
+        a[c + (size_t)j * n] = a[p + (size_t)j * n];
+        a[p + (size_t)j * n] = t;
+        t = b[c + (size_t)j * n];
+        b[c + (size_t)j * n] = b[p + (size_t)j * n];
+        b[p + (size_t)j * n] = t;
+      }
+    const double complex d = 1.0 / a[c + (size_t)c * n];
+    for (int j = 0; j < n; ++j) {
+      a[c + (size_t)j * n] *= d;
+      b[c + (size_t)j * n] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double complex f = a[r + (size_t)c * n];
+      if (f == 0) continue;
+      for (int j = 0; j < n; ++j) {
+        a[r + (size_t)j * n] -= f * a[c + (size_t)j * n];
+        b[r + (size_t)j * n] -= f * b[c + (size_t)j * n];
+      }
+    }
+  }
+  for (size_t e = 0; e < (size_t)n * n && !bad; ++e) {
+    if (cplx) {
+      Ainv[2 * e] = creal(b[e]);
+      Ainv[2 * e + 1] = cimag(b[e]);
+    } else {
+      Ainv[e] = creal(b[e]);
+    }
+  }
+  free(a);
+  free(b);
+  return bad ? -1 : 0;
+}
+
+/* Oblique deflation of the nw columns W against the locked pairs:
+ * Wout = W - Lo Binv (Lp^H W), Lp (S x nl) the partner vectors, Lo the own
+ * half's locked vectors, Binv = (Lp^H Lo)^-1 (host, nl x nl). */
+static int lr_deflate(bdgx_ctx *ctx, int prec, int cplx, const void *Lo, const void *Lp, int nl,
+                      const double *Binv, const void *W, int nw, void *Wout, uint64_t S,
+                      double *dG, double *dC, double *hG, double *hC) {
+  const int w = cplx ? 2 : 1;
+  int rc = lr_gram(ctx, prec, Lp, (uint64_t)nl, W, (uint64_t)nw, S, dG, hG, w);
+  if (rc) return rc;
+  /* C = [-Binv (Lp^H W); I] ((nl + nw) x nw) for the combine over [Lo | W] */
+  memset(hC, 0, sizeof(double) * (size_t)(nl + nw) * nw * w);
+  for (int j = 0; j < nw; ++j) {
+    for (int i = 0; i < nl; ++i) {
+      double complex acc = 0.0;
+      for (int t = 0; t < nl; ++t) {
+        const size_t eb = (size_t)i + (size_t)t * nl, eg = (size_t)t + (size_t)j * nl;
+        const double complex bv = cplx ? Binv[2 * eb] + I * Binv[2 * eb + 1] : Binv[eb];
+        const double complex gv = cplx ? hG[2 * eg] + I * hG[2 * eg + 1] : hG[eg];
+        acc += bv * gv;
+      }
+      const size_t e = (size_t)i + (size_t)j * (nl + nw);
+      hC[e * w] = -creal(acc);
+      if (cplx) hC[e * w + 1] = -cimag(acc);
+    }
+    hC[((size_t)(nl + j) + (size_t)j * (nl + nw)) * w] = 1.0;
+  }
+  rc = bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)(nl + nw) * nw * w);
+  if (!rc)
+    rc = bdgx_combine2(ctx, prec, Lo, S, (uint64_t)nl, W, S, (uint64_t)nw, dC, Wout, S,
+                       (uint64_t)nw, S);
+  return rc;
+}
+
 int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dXU0,
                     const void *dXV0, int prec0, lrep_result *res, void *dXU, void *dXV) {
   const uint64_t S = cfg->S;
@@ -191,13 +295,17 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   void *dR[2] = {NULL, NULL}, *dWt[2] = {NULL, NULL};
   double *dG = NULL, *dC = NULL, *dZero = NULL, *dN2 = NULL;
   const size_t gsz = (size_t)kmax * kmax * w;
+  /* scratch Grams and coefficients also hold the locked pairs' (nl x nl)
+   * and deflation ((nl + m) x m) matrices */
+  const int kbuf = kmax > cfg->nl + m ? kmax : cfg->nl + m;
+  const size_t bsz = (size_t)kbuf * kbuf * w;
   double *hGK = (double *)calloc(gsz, sizeof(double));
   double *hGM = (double *)calloc(gsz, sizeof(double));
   double *hGW = (double *)calloc(gsz, sizeof(double));
   double *hP[3] = {(double *)calloc(gsz, sizeof(double)), (double *)calloc(gsz, sizeof(double)),
                    (double *)calloc(gsz, sizeof(double))};
-  double *hG = (double *)calloc(4 * gsz, sizeof(double));
-  double *hC = (double *)calloc(4 * gsz, sizeof(double));
+  double *hG = (double *)calloc(4 * bsz, sizeof(double));
+  double *hC = (double *)calloc(4 * bsz, sizeof(double));
   double *hA = (double *)calloc((size_t)kmax * m * w, sizeof(double));
   double *hB = (double *)calloc((size_t)kmax * m * w, sizeof(double));
   double *hCQ[2] = {(double *)calloc(gsz, sizeof(double)), (double *)calloc(gsz, sizeof(double))};
@@ -214,6 +322,8 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   int *sel = (int *)calloc((size_t)m, sizeof(int));
   int k[2] = {0, 0}, nw[2] = {0, 0}, nwact = 0, nact = 0, last_full = -1000, since_full = 0;
   int gvalid = 0; /* hGK/hGM/hGW hold the Grams of the current basis */
+  double *hBinv[2] = {NULL, NULL};
+  void *dUL = NULL, *dVL = NULL, *dU0 = NULL, *dV0 = NULL, *dOut = NULL;
   double trr = 0.0;
   const double t0 = bdg_wtime();
   res->iters = 0;
@@ -229,19 +339,44 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
     TRY(bdgx_malloc(&dR[h], (size_t)S * m * es));
     TRY(bdgx_malloc(&dWt[h], (size_t)S * m * es));
   }
-  TRY(bdgx_malloc((void **)&dG, sizeof(double) * 4 * gsz));
-  TRY(bdgx_malloc((void **)&dC, sizeof(double) * 4 * gsz));
+  TRY(bdgx_malloc((void **)&dG, sizeof(double) * 4 * bsz));
+  TRY(bdgx_malloc((void **)&dC, sizeof(double) * 4 * bsz));
   TRY(bdgx_malloc((void **)&dZero, sizeof(double) * (size_t)m));
   TRY(bdgx_malloc((void **)&dN2, sizeof(double) * 3 * (size_t)m));
   TRY(bdgx_memset(ctx, dZero, 0, sizeof(double) * (size_t)m));
+  /* locked pairs: (V_L^H U_L)^-1 for the U half, its adjoint for the V half */
+  const void *Lown[2] = {cfg->dUL, cfg->dVL}, *Lpar[2] = {cfg->dVL, cfg->dUL};
+  if (cfg->nl > 0) {
+    const int nl = cfg->nl;
+    hBinv[0] = (double *)calloc((size_t)nl * nl * w, sizeof(double));
+    hBinv[1] = (double *)calloc((size_t)nl * nl * w, sizeof(double));
+    TRY(lr_gram(ctx, prec, cfg->dVL, (uint64_t)nl, cfg->dUL, (uint64_t)nl, S, dG, hG, w));
+    if (lr_invert(nl, cplx, hG, hBinv[0])) {
+      rc = -102; /* locked pairs not biorthogonally independent */
+      goto done;
+    }
+    for (int j = 0; j < nl; ++j)
+      for (int i = 0; i < nl; ++i) {
+        const size_t a = (size_t)i + (size_t)j * nl, b = (size_t)j + (size_t)i * nl;
+        hBinv[1][a * w] = hBinv[0][b * w];
+        if (cplx) hBinv[1][a * w + 1] = -hBinv[0][b * w + 1];
+      }
+  }
 
-  /* start: X_U, X_V from the given blocks (converted), orthonormalised */
+  /* start: X_U, X_V from the given blocks (converted, deflated against the
+   * locked pairs), orthonormalised */
   {
     const void *src[2] = {dXU0, dXV0};
     for (int h = 0; h < 2; ++h) {
+      void *x0 = dWt[h];
       TRY(bdgx_convert(ctx, prec0, src[h], S, prec, dWt[h], S, S, (uint64_t)m));
+      if (cfg->nl > 0) {
+        TRY(lr_deflate(ctx, prec, cplx, Lown[h], Lpar[h], cfg->nl, hBinv[h], dWt[h], m, dR[h], S,
+                       dG, dC, hG, hC));
+        x0 = dR[h];
+      }
       int r = 0;
-      TRY(lr_orth(ctx, prec, cplx, H[h].Z[0], 0, dWt[h], m, S, 1e-8, dG, dC, hG, hC, &r));
+      TRY(lr_orth(ctx, prec, cplx, H[h].Z[0], 0, x0, m, S, 1e-8, dG, dC, hG, hC, &r));
       if (r < m) {
         rc = -101; /* start block numerically dependent */
         goto done;
@@ -420,6 +555,122 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
               w2[0] < 0 ? -sqrt(-w2[0]) : sqrt(w2[0]),
               w2[nev - 1] < 0 ? -sqrt(-w2[nev - 1]) : sqrt(w2[nev - 1]), worst, nact);
     res->worst = worst;
+    if (!allc && cfg->lock_max > 0 && cfg->nl == 0 && nact <= cfg->lock_max && nact < m &&
+        it < cfg->maxiter) {
+      /* lock the converged pairs; the active ones continue in a nested
+       * small-block solve deflated against them */
+      const int nc = m - nact;
+      int *cidx = (int *)malloc(sizeof(int) * (size_t)nc);
+      int ncv = 0, nev_in = 0;
+      for (int j = 0; j < m; ++j)
+        if (conv[j]) cidx[ncv++] = j;
+      for (int i = 0; i < nact; ++i)
+        if (act[i] < nev) ++nev_in;
+      TRY(bdgx_malloc(&dUL, (size_t)S * nc * es));
+      TRY(bdgx_malloc(&dVL, (size_t)S * nc * es));
+      TRY(bdgx_malloc(&dU0, (size_t)S * nact * es));
+      TRY(bdgx_malloc(&dV0, (size_t)S * nact * es));
+      void *dstL[2] = {dUL, dVL}, *dstA[2] = {dU0, dV0};
+      for (int h = 0; h < 2; ++h) {
+        /* Ritz vectors X R[:, C] and X R[:, J] */
+        for (int pass = 0; pass < 2; ++pass) {
+          const int *cols = pass == 0 ? cidx : act;
+          const int nn = pass == 0 ? nc : nact;
+          for (int c = 0; c < nn; ++c)
+            memcpy(hC + (size_t)c * m * w, hR[h] + (size_t)cols[c] * m * w,
+                   sizeof(double) * (size_t)m * w);
+          TRY(bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)m * nn * w));
+          TRY(bdgx_combine(ctx, prec, H[h].Z[g], S, (uint64_t)m, dC, pass == 0 ? dstL[h] : dstA[h],
+                           S, (uint64_t)nn, S));
+        }
+      }
+      TRY(bdgx_sync(ctx));
+      for (int h = 0; h < 2; ++h)
+        for (int q = 0; q < 2; ++q) {
+          bdgx_free(H[h].Z[q]);
+          bdgx_free(H[h].Y[q]);
+          H[h].Z[q] = H[h].Y[q] = NULL;
+        }
+      lrep_cfg icfg = *cfg;
+      icfg.m = nact;
+      icfg.nev = nev_in > 0 ? nev_in : 1;
+      icfg.maxiter = cfg->maxiter - it - 1;
+      icfg.lock_max = 0;
+      icfg.dUL = dUL;
+      icfg.dVL = dVL;
+      icfg.nl = nc;
+      lrep_result ires;
+      memset(&ires, 0, sizeof(ires));
+      double *iw2 = (double *)calloc((size_t)nact, sizeof(double));
+      double *ires_v = (double *)calloc((size_t)nact, sizeof(double));
+      ires.w2 = iw2;
+      ires.res = ires_v;
+      if (cfg->verbosity > 0)
+        fprintf(stderr, "[bdg] %s it %d: %d pairs locked, %d continue\n",
+                prec == BDGX_FP64 ? "fp64" : "fp32", it, nc, nact);
+      rc = bdg_lrep_device(ctx, &icfg, prec, dU0, dV0, prec, &ires, dU0, dV0);
+      res->applies += ires.applies;
+      trr += ires.rr_seconds;
+      res->iters = it + 1 + ires.iters;
+      if (!rc) {
+        /* merged pairs, sorted by w^2: locked ones (u negated for w^2 < 0)
+         * and the nested solve's (already in output form) */
+        int *src = (int *)malloc(sizeof(int) * (size_t)m);   /* < nc: locked, else nested */
+        double *fw2 = (double *)malloc(sizeof(double) * (size_t)m);
+        double *fres = (double *)malloc(sizeof(double) * (size_t)m);
+        for (int c = 0; c < nc; ++c) {
+          src[c] = c;
+          fw2[c] = w2[cidx[c]];
+          fres[c] = resv[cidx[c]];
+        }
+        for (int i = 0; i < nact; ++i) {
+          src[nc + i] = nc + i;
+          fw2[nc + i] = iw2[i];
+          fres[nc + i] = ires_v[i];
+        }
+        for (int a = 1; a < m; ++a) /* insertion sort by w^2 (stable) */
+          for (int b = a; b > 0 && fw2[b] < fw2[b - 1]; --b) {
+            double t = fw2[b];
+            fw2[b] = fw2[b - 1];
+            fw2[b - 1] = t;
+            t = fres[b];
+            fres[b] = fres[b - 1];
+            fres[b - 1] = t;
+            const int ti = src[b];
+            src[b] = src[b - 1];
+            src[b - 1] = ti;
+          }
+        void *dst[2] = {dXU, dXV};
+        if (prec0 != prec) TRY(bdgx_malloc(&dOut, (size_t)S * m * es));
+        for (int h = 0; h < 2; ++h) {
+          memset(hC, 0, sizeof(double) * (size_t)m * m * w);
+          for (int p = 0; p < m; ++p) {
+            const int sc = src[p];
+            const double sg = (h == 0 && sc < nc && fw2[p] < 0) ? -1.0 : 1.0;
+            hC[((size_t)sc + (size_t)p * m) * w] = sg;
+          }
+          TRY(bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)m * m * w));
+          TRY(bdgx_combine2(ctx, prec, dstL[h], S, (uint64_t)nc, dstA[h], S, (uint64_t)nact, dC,
+                            prec0 == prec ? dst[h] : dOut, S, (uint64_t)m, S));
+          if (prec0 != prec) TRY(bdgx_convert(ctx, prec, dOut, S, prec0, dst[h], S, S, (uint64_t)m));
+        }
+        worst = 0.0;
+        for (int j = 0; j < m; ++j) {
+          res->w2[j] = fw2[j];
+          res->res[j] = fres[j];
+          if (j < nev) worst = fmax(worst, fres[j]);
+        }
+        res->worst = worst;
+        TRY(bdgx_sync(ctx));
+        free(src);
+        free(fw2);
+        free(fres);
+      }
+      free(iw2);
+      free(ires_v);
+      free(cidx);
+      goto done;
+    }
     if (allc || it == cfg->maxiter) break;
     /* new directions of the active columns, orthonormalised against
      * [X | P], and their images.  The residual of act[i] sits in column
@@ -440,6 +691,14 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
         wtmp = dR[h];
       }
       TRY(bdgx_apply(ctx, opT[h], prec, wsrc, wtmp, (uint64_t)nact, S));
+      if (cfg->nl > 0) {
+        /* W into the deflated subspace (the residual in wsrc is consumed) */
+        TRY(lr_deflate(ctx, prec, cplx, Lown[h], Lpar[h], cfg->nl, hBinv[h], wtmp, nact, wsrc,
+                       S, dG, dC, hG, hC));
+        void *t = wtmp;
+        wtmp = wsrc;
+        wsrc = t;
+      }
       const int nq = k[h];
       int r = 0;
       TRY(lr_orth(ctx, prec, cplx, H[h].Z[g], nq, wtmp, nact, S, 1e-3, dG, dC, hG, hC, &r));
@@ -490,6 +749,13 @@ done:
     free(hR[h]);
   }
   for (int q = 0; q < 3; ++q) free(hP[q]);
+  free(hBinv[0]);
+  free(hBinv[1]);
+  bdgx_free(dUL);
+  bdgx_free(dVL);
+  bdgx_free(dU0);
+  bdgx_free(dV0);
+  bdgx_free(dOut);
   bdgx_free(dG);
   bdgx_free(dC);
   bdgx_free(dZero);

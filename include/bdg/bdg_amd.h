@@ -43,6 +43,11 @@ void bdg_set_formulation(bdg_t *bdg, bdg_formulation_t form);
  * halved in stall_window iterations (default 40; 0: never) once below 1e-4. */
 void bdg_set_fp32_phase(bdg_t *bdg, f64 fp32_tol, uint64_t stall_window);
 
+/* First-order form: once at most max_active columns are still active, the
+ * converged pairs are locked and the rest continue in a small block deflated
+ * against them (default -1: max(2, sizeSub / 8); 0: never lock). */
+void bdg_set_locking(bdg_t *bdg, int max_active);
+
 typedef struct {
   uint64_t iterations;          /* LOBPCG iterations                           */
   uint64_t column_applies;      /* (M then K) column applies, incl. start      */
