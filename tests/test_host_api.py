@@ -325,8 +325,15 @@ def test_host_first_order_rayleigh_ritz(cplx):
     ref = np.sort(np.linalg.eigvals(M @ K).real)[:m]
     assert ref[0] < 0 < ref[1]
     np.testing.assert_allclose(w2, ref, rtol=1e-10, atol=1e-12)
-    # subspaces of 17 and 14 columns
-    ku, kv = 17, 14
+    # subspaces of 17 and 14 columns (uncoupled directions of Z_U: the
+    # eigen-decomposition path with the Schur complement) and of 14 and 17
+    # (the Cholesky path)
+    for ku, kv in ((17, 14), (14, 17)):
+        _galerkin_case(lib, rng, cplx, dt, F, K, M, n, m, ku, kv)
+
+
+def _galerkin_case(lib, rng, cplx, dt, F, K, M, n, m, ku, kv):
+    w2 = np.zeros(m)
     ZU, _ = np.linalg.qr(rng.standard_normal((n, ku)) + (1j * rng.standard_normal((n, ku)) if cplx else 0))
     ZV, _ = np.linalg.qr(rng.standard_normal((n, kv)) + (1j * rng.standard_normal((n, kv)) if cplx else 0))
     GK, GM, GW = ZU.conj().T @ K @ ZU, ZV.conj().T @ M @ ZV, ZU.conj().T @ ZV
