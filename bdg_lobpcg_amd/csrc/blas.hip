@@ -246,7 +246,10 @@ __global__ void __launch_bounds__(256) k_gram_pipe(GramMArgs<R> a) {
     pb[g] = (col < a.q1 ? a.B + (long long)col * a.ldb
                         : a.B2 + (long long)(col - a.q1) * a.ldb2) + r_lo + 4 * ks;
   }
-  auto ld = [](const R *p, double out[4]) {
+  // operands stay in the storage type while in flight (an FP32 value is
+  // widened at its MFMA, after the scheduling barrier, so the load is
+  // awaited there and not where it is issued)
+  auto ld = [](const R *p, R out[4]) {
     if constexpr (sizeof(R) == 8) {
       const double2 x = reinterpret_cast<const double2 *>(p)[0];
       const double2 y = reinterpret_cast<const double2 *>(p)[1];
@@ -263,7 +266,7 @@ __global__ void __launch_bounds__(256) k_gram_pipe(GramMArgs<R> a) {
     }
   };
   if (nfull > 0) {
-    double av[PD][FP][4], bv[PD][FQ][4];
+    R av[PD][FP][4], bv[PD][FQ][4];
 #pragma unroll
     for (int d = 0; d < PD - 1; ++d) {
       const long long off = 16LL * (d < nfull ? d : nfull - 1);
@@ -291,8 +294,8 @@ __global__ void __launch_bounds__(256) k_gram_pipe(GramMArgs<R> a) {
           for (int f = 0; f < FP; ++f)
 #pragma unroll
             for (int g = 0; g < FQ; ++g)
-              acc[f * FQ + g] =
-                  __builtin_amdgcn_mfma_f64_16x16x4f64(av[d][f][t], bv[d][g][t], acc[f * FQ + g], 0, 0, 0);
+              acc[f * FQ + g] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  (double)av[d][f][t], (double)bv[d][g][t], acc[f * FQ + g], 0, 0, 0);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
@@ -306,8 +309,8 @@ __global__ void __launch_bounds__(256) k_gram_pipe(GramMArgs<R> a) {
           for (int f = 0; f < FP; ++f)
 #pragma unroll
             for (int g = 0; g < FQ; ++g)
-              acc[f * FQ + g] =
-                  __builtin_amdgcn_mfma_f64_16x16x4f64(av[d][f][t], bv[d][g][t], acc[f * FQ + g], 0, 0, 0);
+              acc[f * FQ + g] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  (double)av[d][f][t], (double)bv[d][g][t], acc[f * FQ + g], 0, 0, 0);
       }
     }
   }
@@ -333,6 +336,232 @@ __global__ void __launch_bounds__(256) k_gram_pipe(GramMArgs<R> a) {
           acc[f * FQ + g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[f][t], bv[g][t], acc[f * FQ + g], 0, 0, 0);
   }
   gram_epilogue<R, FP, FQ, false>(a, red, acc, p0, q0);
+}
+
+// Real Gram with the whole output in one workgroup (p, q <= 96): every
+// element of A and B is read from HBM once.  The workgroup walks its row
+// chunk in steps of RS rows; each step's RS x (32 TP + 32 TQ) block of
+// [A | B] columns (zero past p / q and past the chunk) is staged in LDS
+// (double-buffered: the next step's loads are in flight while the MFMAs of
+// the current one run), and wave (wp, wq) forms the (16 TP) x (16 TQ)
+// output block at rows 16 TP wp, columns 16 TQ wq from LDS.  Operands as in
+// k_gram_pipe: lane (c, ks) reads four consecutive rows of one column and
+// feeds them to four MFMAs.  FP64 storage, and FP32 storage by default:
+// v_mfma_f64_16x16x4f64 (FP32 values widened as they leave LDS: exact
+// products, FP64 sums).  With M32 (FP32 storage only):
+// v_mfma_f32_16x16x4f32, the products rounded to FP32 and summed in FP32
+// over the RS = 32 rows of a step, each step's sums added into FP64
+// accumulators (relative error ~1e-7).  One partial per workgroup.
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr bool kGramF32Mfma = false;
+
+template <typename R>
+struct GramLds {
+  static constexpr bool F32 = sizeof(R) == 4;
+  static constexpr int RS = F32 ? 32 : 16;       // rows per step
+  static constexpr int STR = RS + (F32 ? 4 : 2);  // LDS column stride (elements)
+};
+
+template <typename R, int TP, int TQ, bool M32>
+__global__ void __launch_bounds__(256) k_gram_lds(GramMArgs<R> a) {
+  constexpr bool F32 = GramLds<R>::F32 && M32;
+  constexpr bool S32 = GramLds<R>::F32;  // FP32 storage
+  constexpr int RS = GramLds<R>::RS, STR = GramLds<R>::STR;
+  constexpr int NCA = 32 * TP, NC = 32 * (TP + TQ);
+  constexpr int QPC = RS / 4;  // quads of 4 rows per column and step
+  constexpr int NQ = NC * QPC;
+  constexpr int NPASS = (NQ + 255) / 256;
+  extern __shared__ __align__(16) char smem_raw[];
+  R *lds = reinterpret_cast<R *>(smem_raw);  // 2 buffers of NC * STR
+  const int tid = threadIdx.x;
+  const long long r_lo = (long long)blockIdx.x * a.rows_per_wg;
+  long long r_hi = r_lo + a.rows_per_wg;
+  if (r_hi > a.nrows) r_hi = a.nrows;
+  const int nsteps = r_hi > r_lo ? (int)((r_hi - r_lo + RS - 1) / RS) : 0;
+  // this thread's quads: column base pointer (null: a zero column) and slot
+  const R *src[NPASS];
+  int slot[NPASS], qoff[NPASS];
+#pragma unroll
+  for (int i = 0; i < NPASS; ++i) {
+    const int e = tid + 256 * i;
+    const int col = e / QPC, qd = e % QPC;
+    src[i] = nullptr;
+    slot[i] = -1;
+    qoff[i] = 4 * qd;
+    if (e < NQ) {
+      slot[i] = col * STR + 4 * qd;
+      if (col < NCA) {
+        if (col < a.p) src[i] = a.A + (long long)col * a.lda;
+      } else {
+        const int cq = col - NCA;
+        if (cq < a.q)
+          src[i] = cq < a.q1 ? a.B + (long long)cq * a.ldb : a.B2 + (long long)(cq - a.q1) * a.ldb2;
+      }
+    }
+  }
+  R reg[NPASS][4];
+  auto fetch = [&](int s) {
+    const long long r0 = r_lo + (long long)s * RS;
+#pragma unroll
+    for (int i = 0; i < NPASS; ++i) {
+      const long long r = r0 + qoff[i];
+      if (src[i] && r + 4 <= r_hi) {
+        if constexpr (S32) {
+          const float4 x = *reinterpret_cast<const float4 *>(src[i] + r);
+          reg[i][0] = x.x;
+          reg[i][1] = x.y;
+          reg[i][2] = x.z;
+          reg[i][3] = x.w;
+        } else {
+          const double2 x = reinterpret_cast<const double2 *>(src[i] + r)[0];
+          const double2 y = reinterpret_cast<const double2 *>(src[i] + r)[1];
+          reg[i][0] = x.x;
+          reg[i][1] = x.y;
+          reg[i][2] = y.x;
+          reg[i][3] = y.y;
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) reg[i][t] = (src[i] && r + t < r_hi) ? src[i][r + t] : R(0);
+      }
+    }
+  };
+  auto stage = [&](int b) {
+    R *dst = lds + b * NC * STR;
+#pragma unroll
+    for (int i = 0; i < NPASS; ++i) {
+      if (slot[i] < 0) continue;
+      if constexpr (S32) {
+        *reinterpret_cast<float4 *>(dst + slot[i]) = float4{reg[i][0], reg[i][1], reg[i][2], reg[i][3]};
+      } else {
+        reinterpret_cast<double2 *>(dst + slot[i])[0] = double2{reg[i][0], reg[i][1]};
+        reinterpret_cast<double2 *>(dst + slot[i])[1] = double2{reg[i][2], reg[i][3]};
+      }
+    }
+  };
+  const int lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, ks = lane >> 4;
+  const int ca = (wave & 1) * 16 * TP, cb = NCA + (wave >> 1) * 16 * TQ;
+  v4d acc[TP][TQ];
+#pragma unroll
+  for (int f = 0; f < TP; ++f)
+#pragma unroll
+    for (int g = 0; g < TQ; ++g) acc[f][g] = v4d{0.0, 0.0, 0.0, 0.0};
+  if (nsteps > 0) {
+    fetch(0);
+    stage(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < nsteps; ++s) {
+    if (s + 1 < nsteps) fetch(s + 1);
+    const R *cur = lds + (s & 1) * NC * STR;
+    v4f acc32[TP][TQ];
+    if constexpr (F32) {
+#pragma unroll
+      for (int f = 0; f < TP; ++f)
+#pragma unroll
+        for (int g = 0; g < TQ; ++g) acc32[f][g] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int sub = 0; sub < RS / 16; ++sub) {
+      R av[TP][4], bv[TQ][4];
+#pragma unroll
+      for (int f = 0; f < TP; ++f) {
+        const R *p = cur + (ca + 16 * f + c) * STR + 16 * sub + 4 * ks;
+        if constexpr (S32) {
+          const float4 x = *reinterpret_cast<const float4 *>(p);
+          av[f][0] = x.x;
+          av[f][1] = x.y;
+          av[f][2] = x.z;
+          av[f][3] = x.w;
+        } else {
+          const double2 x = reinterpret_cast<const double2 *>(p)[0];
+          const double2 y = reinterpret_cast<const double2 *>(p)[1];
+          av[f][0] = x.x;
+          av[f][1] = x.y;
+          av[f][2] = y.x;
+          av[f][3] = y.y;
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < TQ; ++g) {
+        const R *p = cur + (cb + 16 * g + c) * STR + 16 * sub + 4 * ks;
+        if constexpr (S32) {
+          const float4 x = *reinterpret_cast<const float4 *>(p);
+          bv[g][0] = x.x;
+          bv[g][1] = x.y;
+          bv[g][2] = x.z;
+          bv[g][3] = x.w;
+        } else {
+          const double2 x = reinterpret_cast<const double2 *>(p)[0];
+          const double2 y = reinterpret_cast<const double2 *>(p)[1];
+          bv[g][0] = x.x;
+          bv[g][1] = x.y;
+          bv[g][2] = y.x;
+          bv[g][3] = y.y;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int f = 0; f < TP; ++f)
+#pragma unroll
+          for (int g = 0; g < TQ; ++g) {
+            if constexpr (F32)
+              acc32[f][g] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av[f][t], (float)bv[g][t],
+                                                                  acc32[f][g], 0, 0, 0);
+            else
+              acc[f][g] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[f][t], (double)bv[g][t],
+                                                                acc[f][g], 0, 0, 0);
+          }
+    }
+    if constexpr (F32) {
+#pragma unroll
+      for (int f = 0; f < TP; ++f)
+#pragma unroll
+        for (int g = 0; g < TQ; ++g)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[f][g][v] += (double)acc32[f][g][v];
+    }
+    if (s + 1 < nsteps) stage((s + 1) & 1);
+    __syncthreads();
+  }
+  // accumulator element v of lane (c, ks): D[ks + 4 v][c] for the FP64
+  // MFMA, D[4 ks + v][c] for the FP32 one
+  double *out = a.part + (long long)blockIdx.x * a.p * a.q;
+#pragma unroll
+  for (int f = 0; f < TP; ++f)
+#pragma unroll
+    for (int g = 0; g < TQ; ++g) {
+      const int j = cb - NCA + 16 * g + c;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = ca + 16 * f + (F32 ? 4 * ks + v : ks + 4 * v);
+        if (i < a.p && j < a.q) out[(long long)i + (long long)j * a.p] = acc[f][g][v];
+      }
+    }
+}
+
+template <typename R, int TP, int TQ>
+static void launch_gram_lds(const GramMArgs<R> &a, int nchunk, hipStream_t s) {
+  const size_t lds = sizeof(R) * 2 * 32 * (TP + TQ) * GramLds<R>::STR;
+  hipLaunchKernelGGL((k_gram_lds<R, TP, TQ, kGramF32Mfma>), dim3(nchunk), dim3(256), lds, s, a);
+}
+
+template <typename R>
+static void dispatch_gram_lds(const GramMArgs<R> &a, int nchunk, hipStream_t s) {
+  const int tp = (a.p + 31) / 32, tq = (a.q + 31) / 32;
+  switch (tp * 4 + tq) {
+    case 5: launch_gram_lds<R, 1, 1>(a, nchunk, s); break;
+    case 6: launch_gram_lds<R, 1, 2>(a, nchunk, s); break;
+    case 7: launch_gram_lds<R, 1, 3>(a, nchunk, s); break;
+    case 9: launch_gram_lds<R, 2, 1>(a, nchunk, s); break;
+    case 10: launch_gram_lds<R, 2, 2>(a, nchunk, s); break;
+    case 11: launch_gram_lds<R, 2, 3>(a, nchunk, s); break;
+    case 13: launch_gram_lds<R, 3, 1>(a, nchunk, s); break;
+    case 14: launch_gram_lds<R, 3, 2>(a, nchunk, s); break;
+    default: launch_gram_lds<R, 3, 3>(a, nchunk, s); break;
+  }
 }
 
 // Sum of n partial vectors of length len: workgroup (x, y) adds chunks
@@ -405,28 +634,39 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   const uint64_t q1 = q;
   if (dB2) q = 2 * q;
   const int W = CPLX ? 2 : 1;
-  // tile shape: real up to 96 x 32, complex up to 48 x 32 (12 accumulators);
-  // launch variant 1 (autotuned) caps real tiles at 48 rows of p: more p
-  // tiles re-read B, but each wave holds half the accumulators and operands
-  // (two waves per SIMD instead of one)
+  // tile shape: real up to 96 x 32, complex up to 48 x 32 (12 accumulators).
+  // The launch variant (autotuned) caps the real tile height: 0 = 96 rows of
+  // p, 1 = 48, 2 = 32, 3 = 64.  Lower tiles re-read B once per tile but hold
+  // fewer accumulators and operands per wave (more waves per SIMD).
   int FQ = q <= 16 ? 1 : 2;
   int FP;
-  if (CPLX)
+  if (CPLX) {
     FP = p <= 16 ? 1 : 3;
-  else if (c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_GRAM])
-    FP = p <= 32 ? 2 : 3;
-  else
-    FP = p <= 32 ? 2 : (p <= 64 ? 4 : 6);
+  } else {
+    static const int fp_cap[4] = {6, 3, 2, 4};
+    const int cap = fp_cap[c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_GRAM] & 3];
+    FP = p <= 32 ? 2 : (p <= 48 ? 3 : (p <= 64 ? 4 : 6));
+    if (FP > cap) FP = cap;
+  }
   const int tiles_p = (int)((p + 16 * FP - 1) / (16 * FP));
   const int tiles_q = (int)((q + 16 * FQ - 1) / (16 * FQ));
   const int tiles = tiles_p * tiles_q;
   const long long rows = (long long)nrows * W;
-  // one row chunk per CU, the same for every tile shape (so the launch
+  const uintptr_t al = (uintptr_t)dA | (uintptr_t)dB | (uintptr_t)dB2;
+  const bool vec = ((long long)lda * W % 4 == 0) && ((long long)ldb * W % 4 == 0) &&
+                   (!dB2 || (long long)ldb2 * W % 4 == 0) && (al % (4 * sizeof(R)) == 0);
+  // real, unswapped, vector-aligned, 32 < q <= 96 and p <= 96: the
+  // single-pass LDS kernel (two row chunks per CU), where the tiled kernels
+  // would read A once per 32 columns of B; else the tiled kernels with one
+  // row chunk per CU, the same for every tile shape (so their launch
   // variants sum the same rows in the same order)
+  const bool lds_path = !CPLX && !swap_b && vec && p <= 96 && q > 32 && q <= 96;
   long long nchunk = c->ncu > 0 ? c->ncu : 256;
+  if (lds_path) nchunk *= 2;
+  const long long rq = lds_path ? 32 : 64;
   long long rpw = (rows + nchunk - 1) / nchunk;
-  rpw = ((rpw + 63) / 64) * 64;
-  if (rpw < 64) rpw = 64;
+  rpw = ((rpw + rq - 1) / rq) * rq;
+  if (rpw < rq) rpw = rq;
   nchunk = (rows + rpw - 1) / rpw;
   if (nchunk < 1) nchunk = 1;
   const long long len = (long long)p * q * W;
@@ -457,13 +697,13 @@ static int gram_t(bdgx_ctx *c, const void *dA, uint64_t lda, uint64_t p,
   a.tiles_p = tiles_p;
   a.swap_b = swap_b;
   a.wel = W;
-  const uintptr_t al = (uintptr_t)dA | (uintptr_t)dB | (uintptr_t)dB2;
-  a.vec = (a.lda % 4 == 0) && (a.ldb % 4 == 0) && (!dB2 || a.ldb2 % 4 == 0) &&
-          (al % (4 * sizeof(R)) == 0);
+  a.vec = vec;
   if (dB2 && (CPLX || swap_b || !a.vec)) return BDGX_EINVAL;
   a.part = part;
   const int nc = (int)nchunk;
-  if (CPLX) {
+  if (lds_path) {
+    dispatch_gram_lds<R>(a, nc, c->stream);
+  } else if (CPLX) {
     if (FP == 1 && FQ == 1) launch_gram<R, 1, 1, true>(a, nc, tiles, c->stream);
     else if (FP == 1) launch_gram<R, 1, 2, true>(a, nc, tiles, c->stream);
     else if (FQ == 1) launch_gram<R, 3, 1, true>(a, nc, tiles, c->stream);
@@ -788,12 +1028,13 @@ __global__ void __launch_bounds__(256) k_combine_mfma(Cols<R> A, int k, const do
   }
 }
 
-// RT consecutive rows of column `col` with no branch: the caller clamps
+// RT consecutive rows of column `col` with no branch, in the storage type
+// (widened at their MFMA, so the load is awaited there): the caller clamps
 // col and row to valid ones (a clamped column meets a zero row of C, a
 // clamped row only feeds output rows that are not stored)
 template <typename R, int RT>
 __device__ __forceinline__ void load_rows_nb(const Cols<R> &A, int col, long long row,
-                                             double out[RT]) {
+                                             R out[RT]) {
   const R *p = A.col(col) + row;
   if constexpr (sizeof(R) == 8 && RT == 4) {
     const double2 x = reinterpret_cast<const double2 *>(p)[0];
@@ -851,7 +1092,7 @@ __global__ void __launch_bounds__(256) k_combine_pers(Cols<R> A, int k, const do
     const long long r = (t * 4 + wave) * (16 * RT) + RT * c;
     return r < rmax ? r : rmax;
   };
-  double av[PD][RT];
+  R av[PD][RT];
   for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const long long r0 = (tile * 4 + wave) * (16 * RT);
     if (tile == blockIdx.x || !xt) {
@@ -879,8 +1120,8 @@ __global__ void __launch_bounds__(256) k_combine_pers(Cols<R> A, int k, const do
           for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int f = 0; f < MF; ++f)
-              acc[t][f] =
-                  __builtin_amdgcn_mfma_f64_16x16x4f64(av[d][t], bv[f], acc[t][f], 0, 0, 0);
+              acc[t][f] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[d][t], bv[f],
+                                                               acc[t][f], 0, 0, 0);
         }
         // refill this buffer with step s + PD
         int sn = s + PD;

@@ -1006,7 +1006,7 @@ int bdgx_get_launch_variant(const bdgx_ctx *c, int kind, int prec, int *variant)
 
 int bdgx_set_launch_variant(bdgx_ctx *c, int kind, int prec, int variant) {
   if (!c || kind < 0 || kind > 4 || (prec != BDGX_FP64 && prec != BDGX_FP32) || variant < 0 ||
-      variant > (kind <= BDGX_TUNE_PLANE_INV || kind == BDGX_TUNE_COMBINE ? 3 : 1))
+      variant > (kind == BDGX_TUNE_Z ? 1 : 3))
     return BDGX_EINVAL;
   c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind] = variant;
   return BDGX_OK;

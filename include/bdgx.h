@@ -103,12 +103,14 @@ int bdgx_gram(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64_t p,
  * BDGX_GRAM_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends)
  * instead of the context's scratch.  The partials of one call need
  * 8 * (chunks + ceil(chunks / 64)) * p * q * (2 if complex) bytes, chunks =
- * the number of compute units for tall blocks (256 on MI355X: p * q up to
- * 8065 real, 4032 complex); larger calls return BDGX_ETOOBIG.  Every call on
+ * twice the number of compute units for tall real blocks with p <= 96 and
+ * 32 < q <= 96 (the single-pass kernel), else the number of compute units
+ * (256 on MI355X: p * q up to 8065 real with p <= 96 and 32 < q <= 96,
+ * 16131 other real, 8065 complex); larger calls return BDGX_ETOOBIG.  Every call on
  * one GPU shares that array, so calls that overlap on one GPU (other
  * contexts or streams) race; calls in one stream order are safe.  Needs a
  * context created on that device (bdgx_ctx_create looks the array up). */
-#define BDGX_GRAM_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define BDGX_GRAM_NOALLOC_SCRATCH_BYTES (32u << 20)
 int bdgx_gram_noalloc(bdgx_ctx *ctx, int prec, const void *dA, uint64_t lda, uint64_t p,
                       const void *dB, uint64_t ldb, uint64_t q, uint64_t nrows, int swap_b,
                       double *dG);
@@ -237,8 +239,8 @@ int bdgx_bench_pass(bdgx_ctx *ctx, int pass, int prec, uint64_t nbatch, int iter
  * (Nz = 128) or 128 threads, 1 = half of that; combine 0 = 4 rows per lane,
  * 1 = 2 rows per lane, 2 / 3 = persistent workgroups that stage their tile
  * of C in LDS once (2 / 4 rows per lane; where it fits in 64 KB and the
- * blocks are vector-aligned, else as 1 / 0); real Gram 0 = tiles of up to 96
- * rows of p, 1 = up to 48.  bdgx_autotune times every variant of
+ * blocks are vector-aligned, else as 1 / 0); real Gram 0 / 1 / 2 / 3 =
+ * tiles of up to 96 / 48 / 32 / 64 rows of p.  bdgx_autotune times every variant of
  * every kind for `prec` on `nbatch` columns (kinetic batches for the FFT
  * kinds, a 3 nbatch -> 2 nbatch combine and a 3 nbatch x nbatch Gram, the
  * solver's main shapes; 1 warm-up + 3 timed

@@ -137,17 +137,21 @@ def test_launch_variants_and_autotune(grid):
             assert np.array_equal(outs[v], outs[0]), f"combine variant {v} differs from 0"
         Ag = rng.standard_normal((c.size, 90))
         Bg = rng.standard_normal((c.size, 20))
-        for v in (0, 1):
-            d.set_launch_variant(4, prec, v)  # Gram tile height
+        gs = []
+        for v in (0, 1, 2, 3):
+            d.set_launch_variant(4, prec, v)  # Gram tile height 96 / 48 / 32 / 64
             G = d.gram(Ag, Bg, prec)
             ref = (Ag.astype(np.float32 if prec else np.float64).astype(np.float64).T
                    @ Bg.astype(np.float32 if prec else np.float64).astype(np.float64))
             e = relerr(G, ref)
             assert e < 1e-12, f"gram variant {v} prec {prec}: {e:.2e}"
+            gs.append(G)
+        for v in (1, 2, 3):  # same sums in the same order
+            assert np.array_equal(gs[v], gs[0]), f"gram variant {v} differs from 0"
         chosen = d.autotune(prec, 8, force=True)
         assert chosen["plane_fwd"] in (0, 1, 2, 3) and chosen["plane_inv"] in (0, 1, 2, 3)
         assert chosen["combine"] in (0, 1, 2, 3)
-        assert chosen["z"] in (0, 1) and chosen["gram"] in (0, 1)
+        assert chosen["z"] in (0, 1) and chosen["gram"] in (0, 1, 2, 3)
         for op, ref in refs.items():
             assert relerr(d.apply(op, X, prec), ref) < TOL[prec]
 

@@ -84,7 +84,7 @@ def test_noalloc_gram_refuses_oversize():
     N, L = (64, 64, 64), (12.0, 12.0, 12.0)
     d = B.DeviceContext(N, L, False)
     S = d.size
-    p = q = 96  # 96 * 96 partials per chunk exceed the fixed scratch
+    p = q = 96  # 96 * 96 partials per chunk (2 chunks per CU) exceed the fixed scratch
     dA = B.DeviceBuffer(8 * S * p)
     dG = B.DeviceBuffer(8 * p * q)
     rc = d.gram_noalloc_device(B.FP64, dA, S, p, dA, S, q, S, 0, dG)
