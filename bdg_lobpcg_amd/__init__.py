@@ -152,6 +152,7 @@ def lib():
         "bdg_set_formulation": (None, [P, I]),
         "bdg_set_fp32_phase": (None, [P, D, U64]),
         "bdg_set_locking": (None, [P, I]),
+        "bdg_set_lock_guards": (None, [P, I]),
         "bdg_residuals": (C.POINTER(D), [P]),
         "bdg_zero_modes": (C.POINTER(I), [P]),
         "bdg_device_context": (P, [P]),
@@ -565,6 +566,11 @@ class Problem:
         """Lock converged pairs once at most max_active columns are active
         (-1 automatic, 0 never)."""
         lib().bdg_set_locking(self.h, int(max_active))
+
+    def set_lock_guards(self, guards=4):
+        """Guard columns carried by the small block after locking (first-order
+        form): Ritz pairs just beyond the block, iterated, never returned."""
+        lib().bdg_set_lock_guards(self.h, int(guards))
 
     def set_verbosity(self, level):
         lib().bdg_set_verbosity(self.h, int(level))

@@ -122,6 +122,7 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
   b->fp32_tol = 1e-5;
   b->fp32_stall = 40;
   b->lock_max = -1;
+  b->lock_guards = 4;
   return b;
 }
 
@@ -577,6 +578,7 @@ static int solve_first_order(bdg_t *b, const void *dV0, double t_start) {
   cfg.verbosity = b->verbosity;
   /* automatic locking: once at most max(2, m / 8) columns are active */
   cfg.lock_max = b->lock_max >= 0 ? b->lock_max : (int)(m / 8 > 2 ? m / 8 : 2);
+  cfg.lock_guards = b->lock_guards;
   lrep_result r32, r64;
   memset(&r32, 0, sizeof(r32));
   memset(&r64, 0, sizeof(r64));
@@ -915,6 +917,13 @@ const void *bdg_modes_v(const bdg_t *b) { return b->modes_v; }
 void bdg_set_precision(bdg_t *b, bdg_precision_t p) { b->precision = (int)p; }
 void bdg_set_formulation(bdg_t *b, bdg_formulation_t f) { b->formulation = (int)f; }
 void bdg_set_locking(bdg_t *b, int max_active) { b->lock_max = max_active; }
+void bdg_set_lock_guards(bdg_t *b, int guards) {
+  if (guards < 0) {
+    fprintf(stderr, "bdg_set_lock_guards: guards must be >= 0\n");
+    exit(EXIT_FAILURE);
+  }
+  b->lock_guards = guards;
+}
 void bdg_set_fp32_phase(bdg_t *b, f64 fp32_tol, uint64_t stall_window) {
   b->fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
   b->fp32_stall = (int)stall_window;

@@ -73,6 +73,7 @@ struct bdg_t {
   double fp32_tol;    /* mixed, first order: residual at which the FP32 phase    */
   int fp32_stall;     /*   stops; columns also stop after this many iterations  */
                       /*   without halving their residual (0: never)            */
+  int lock_guards;    /* first order: guard columns after locking              */
   int lock_max;       /* first order: lock converged pairs once at most this   */
                       /*   many columns are active (0: never, < 0: automatic)   */
   double freeze_tol; /* converged columns moved less than this keep their vectors */
@@ -163,6 +164,8 @@ typedef struct {
   int verbosity;
   int lock_max;        /* lock the converged pairs once at most this many       */
                        /*   columns are active (0: never), see lrep.c           */
+  int lock_guards;     /* extra Ritz pairs beyond the block that the small      */
+                       /*   block after locking carries as guard columns        */
   const void *dUL, *dVL; /* locked pairs (S x nl, phase precision): new         */
   int nl;                /*   directions are deflated against them              */
 } lrep_cfg;

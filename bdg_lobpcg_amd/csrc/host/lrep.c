@@ -321,8 +321,10 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   int *wact = (int *)calloc((size_t)m, sizeof(int)); /* columns whose W is in the basis */
   int *sel = (int *)calloc((size_t)m, sizeof(int));
   int k[2] = {0, 0}, nw[2] = {0, 0}, nwact = 0, nact = 0, last_full = -1000, since_full = 0;
+  int kprev[2] = {0, 0}; /* basis size of the last Rayleigh-Ritz */
   int gvalid = 0; /* hGK/hGM/hGW hold the Grams of the current basis */
   double *hBinv[2] = {NULL, NULL};
+  double *gA = NULL, *gB = NULL, *gw2 = NULL; /* guard Ritz pairs at locking */
   void *dUL = NULL, *dVL = NULL, *dU0 = NULL, *dV0 = NULL, *dOut = NULL;
   double trr = 0.0;
   const double t0 = bdg_wtime();
@@ -465,6 +467,7 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
       TRY(bdgx_combine(ctx, prec, H[h].Y[g], S, (uint64_t)k[h], dC, H[h].Y[g ^ 1], S,
                        (uint64_t)nc[h], S));
       H[h].np = nc[h] - m;
+      kprev[h] = k[h];
       k[h] = nc[h];
       nw[h] = 0;
     }
@@ -568,8 +571,23 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
         if (act[i] < nev) ++nev_in;
       TRY(bdgx_malloc(&dUL, (size_t)S * nc * es));
       TRY(bdgx_malloc(&dVL, (size_t)S * nc * es));
-      TRY(bdgx_malloc(&dU0, (size_t)S * nact * es));
-      TRY(bdgx_malloc(&dV0, (size_t)S * nact * es));
+      /* guard columns for the small block: Ritz pairs m .. m + ng - 1 of
+       * this iteration's Rayleigh-Ritz, whose basis (now the other buffer)
+       * and Grams are still intact */
+      int ng = cfg->lock_guards;
+      if (ng > kprev[0] - m) ng = kprev[0] - m;
+      if (ng > kprev[1] - m) ng = kprev[1] - m;
+      if (ng < 0) ng = 0;
+      if (ng > 0) {
+        gA = (double *)calloc((size_t)kprev[0] * (m + ng) * w, sizeof(double));
+        gB = (double *)calloc((size_t)kprev[1] * (m + ng) * w, sizeof(double));
+        gw2 = (double *)calloc((size_t)(m + ng), sizeof(double));
+        const int got2 = dn_lrep_rr(kprev[0], kprev[1], m + ng, cplx, hGK, hGM, hGW, gw2, gA, gB);
+        ng = got2 > m ? got2 - m : 0;
+      }
+      const int mi = nact + ng;
+      TRY(bdgx_malloc(&dU0, (size_t)S * mi * es));
+      TRY(bdgx_malloc(&dV0, (size_t)S * mi * es));
       void *dstL[2] = {dUL, dVL}, *dstA[2] = {dU0, dV0};
       for (int h = 0; h < 2; ++h) {
         /* Ritz vectors X R[:, C] and X R[:, J] */
@@ -583,6 +601,13 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
           TRY(bdgx_combine(ctx, prec, H[h].Z[g], S, (uint64_t)m, dC, pass == 0 ? dstL[h] : dstA[h],
                            S, (uint64_t)nn, S));
         }
+        if (ng > 0) {
+          /* guards after the active columns: old basis x coefficients m.. */
+          const double *gc = (h == 0 ? gA : gB) + (size_t)kprev[h] * m * w;
+          TRY(bdgx_memcpy_h2d(ctx, dC, gc, sizeof(double) * (size_t)kprev[h] * ng * w));
+          TRY(bdgx_combine(ctx, prec, H[h].Z[g ^ 1], S, (uint64_t)kprev[h], dC,
+                           lr_col(dstA[h], (uint64_t)nact, S, es), S, (uint64_t)ng, S));
+        }
       }
       TRY(bdgx_sync(ctx));
       for (int h = 0; h < 2; ++h)
@@ -592,22 +617,23 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
           H[h].Z[q] = H[h].Y[q] = NULL;
         }
       lrep_cfg icfg = *cfg;
-      icfg.m = nact;
       icfg.nev = nev_in > 0 ? nev_in : 1;
       icfg.maxiter = cfg->maxiter - it - 1;
+      icfg.m = mi;
       icfg.lock_max = 0;
+      icfg.lock_guards = 0;
       icfg.dUL = dUL;
       icfg.dVL = dVL;
       icfg.nl = nc;
       lrep_result ires;
       memset(&ires, 0, sizeof(ires));
-      double *iw2 = (double *)calloc((size_t)nact, sizeof(double));
-      double *ires_v = (double *)calloc((size_t)nact, sizeof(double));
+      double *iw2 = (double *)calloc((size_t)mi, sizeof(double));
+      double *ires_v = (double *)calloc((size_t)mi, sizeof(double));
       ires.w2 = iw2;
       ires.res = ires_v;
       if (cfg->verbosity > 0)
-        fprintf(stderr, "[bdg] %s it %d: %d pairs locked, %d continue\n",
-                prec == BDGX_FP64 ? "fp64" : "fp32", it, nc, nact);
+        fprintf(stderr, "[bdg] %s it %d: %d pairs locked, %d continue (+%d guards)\n",
+                prec == BDGX_FP64 ? "fp64" : "fp32", it, nc, nact, ng);
       rc = bdg_lrep_device(ctx, &icfg, prec, dU0, dV0, prec, &ires, dU0, dV0);
       res->applies += ires.applies;
       trr += ires.rr_seconds;
@@ -751,6 +777,9 @@ done:
   for (int q = 0; q < 3; ++q) free(hP[q]);
   free(hBinv[0]);
   free(hBinv[1]);
+  free(gA);
+  free(gB);
+  free(gw2);
   bdgx_free(dUL);
   bdgx_free(dVL);
   bdgx_free(dU0);

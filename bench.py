@@ -40,10 +40,15 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 # residual tolerance of the mixed solve that stops where the mixed path's
 # eigenvalue target (relative omega error <= 1e-7 against FP64) is reached.
-# Measured at 128^3: residual 1e-5 leaves mode 1 (w = 0.46) 4e-6 off in the
-# mixed solve (9e-10 in an FP64 solve), 3e-6 leaves it 7e-9 off.
-ACC_TOL = 3e-6
+# Measured at 128^3: the error of mode 1 (w = 0.46) is not bounded by the
+# square of its residual -- residual 1e-5 left it 4e-6 off, 2.6e-6 left it
+# 9.4e-7 or 7e-9 off from one run to another, 7e-7 left it 3e-9 to 8e-9 off.
+ACC_TOL = 1e-6
 FORM_NAMES = {0: "first-order", 1: "product"}
+# --policy: (precision 0 FP64 / 1 mixed, residual tolerance or None for the
+# configuration's own, formulation)
+POLICIES = {"fp64": (0, None, 0), "fp64_product": (0, None, 1), "mixed": (1, None, 0),
+            "mixed_at_omega_target": (1, ACC_TOL, 0)}
 
 
 def log(*a):
@@ -189,99 +194,116 @@ def probe(cfg):
     d.close()
 
 
-GRAM_SHAPES = [(96, 32, False), (96, 96, False), (96, 96, True), (36, 2, False)]
-COMBINE_SHAPES = [(96, 64), (96, 32), (36, 34)]
+# (p, q, A^T A, paired): the product form's [X|P|W] x W shapes and the
+# first-order solver's (full refresh 96 x 96, paired W Grams 96 x (32 + 32),
+# orthogonalisation 64 x 32 and 32 x 32)
+GRAM_SHAPES = [(96, 32, False, False), (96, 96, False, False), (96, 96, True, False),
+               (36, 2, False, False), (96, 32, False, True), (64, 32, False, False),
+               (32, 32, True, False)]
+COMBINE_SHAPES = [(96, 64), (96, 32), (64, 32), (36, 34)]
 FP64_MFMA_PEAK_TFS = 78.6
 
 
 def blas(cfg, iters=10):
-    """Gram V^T Y and combine V C at the solver's shapes for this grid (S rows,
-    basis [X|P|W] of 3m columns), FP64, HIP events on the context stream:
-    achieved TF/s against the FP64 MFMA peak and GB/s against HBM."""
+    """Gram V^T Y and combine V C at the solvers' shapes for this grid (S rows,
+    basis [X|P|W] of 3m columns), FP64 and FP32 storage (FP64 arithmetic),
+    HIP events on the context stream: GB/s against HBM and achieved TF/s."""
+    import ctypes
+
     import bdg_lobpcg_amd as B
 
     d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
-    variants = d.autotune(B.FP64, cfg["m"])
     S = int(np.prod(cfg["N"]))
     rng = np.random.default_rng(1)
     kmax = 96
     A = np.asfortranarray(rng.standard_normal((S, kmax)))
     Bm = np.asfortranarray(rng.standard_normal((S, kmax)))
-    dA, dB = d.upload(A), d.upload(Bm)
-    dG = B.DeviceBuffer(8 * kmax * kmax)
+    dG = B.DeviceBuffer(8 * kmax * kmax * 2)
     out = []
-    for p, q, same in GRAM_SHAPES:
-        dY = dA if same else dB
+    for prec, pname, dt in ((B.FP64, "fp64", np.float64), (B.FP32, "fp32", np.float32)):
+        variants = d.autotune(prec, cfg["m"])
+        es = np.dtype(dt).itemsize
+        Ad, Bd = A.astype(dt), Bm.astype(dt)
+        dA, dB = d.upload(np.asfortranarray(Ad)), d.upload(np.asfortranarray(Bd))
+        for p, q, same, pair in GRAM_SHAPES:
+            dY = dA if same else dB
 
-        def run():
-            B.check(B.lib().bdgx_gram(d.h, B.FP64, dA.ptr, S, p, dY.ptr, S, q, S, 0, dG.ptr), "gram")
+            def run():
+                if pair:  # [B[:, :q] | B[:, q:2q]] as two blocks
+                    B2 = ctypes.c_void_p(dB.ptr.value + es * S * q)
+                    B.check(B.lib().bdgx_gram_pair(d.h, prec, dA.ptr, S, p, dB.ptr, S, B2, S, q,
+                                                   S, dG.ptr),
+                            "gram_pair")
+                else:
+                    B.check(B.lib().bdgx_gram(d.h, prec, dA.ptr, S, p, dY.ptr, S, q, S, 0, dG.ptr),
+                            "gram")
 
-        run()
-        G = d.download(dG, (p, q), np.float64)
-        ref = A[:, :p].T @ (A if same else Bm)[:, :q]
-        err = float(np.abs(G - ref).max() / np.abs(ref).max())
-        d.timer_begin()
-        for _ in range(iters):
             run()
-        ms = d.timer_end() / iters
-        by = S * (p + (0 if same else q)) * 8
-        fl = 2.0 * S * p * q
-        out.append(dict(kernel="gram", shape=f"{p}x{q}" + (" (A^T A)" if same else ""), us=ms * 1e3,
-                        GBs=by / ms / 1e6, TFs=fl / ms / 1e9,
-                        mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS, rel_err=err))
-    for k, m in COMBINE_SHAPES:
-        Cm = np.asfortranarray(rng.standard_normal((k, m)))
-        dC = d.upload(Cm)
-        dY = B.DeviceBuffer(S * m * 8)
+            qq = 2 * q if pair else q
+            G = d.download(dG, (p, qq), np.float64)
+            Ar = Ad[:, :p].astype(np.float64)
+            ref = Ar.T @ (Ad if same else Bd)[:, :qq].astype(np.float64)
+            err = float(np.abs(G - ref).max() / np.abs(ref).max())
+            d.timer_begin()
+            for _ in range(iters):
+                run()
+            ms = d.timer_end() / iters
+            by = S * (p + (0 if same else qq)) * es
+            fl = 2.0 * S * p * qq
+            out.append(dict(prec=pname, kernel="gram_pair" if pair else "gram",
+                            shape=f"{p}x{qq}" + (" (A^T A)" if same else ""), us=ms * 1e3,
+                            GBs=by / ms / 1e6, TFs=fl / ms / 1e9, rel_err=err,
+                            variant=variants["gram"]))
+        for k, m in COMBINE_SHAPES:
+            Cm = np.asfortranarray(rng.standard_normal((k, m)))
+            dC = d.upload(Cm)
+            dY = B.DeviceBuffer(S * m * es)
 
-        def run():
-            B.check(B.lib().bdgx_combine(d.h, B.FP64, dA.ptr, S, k, dC.ptr, dY.ptr, S, m, S),
-                    "combine")
+            def run():
+                B.check(B.lib().bdgx_combine(d.h, prec, dA.ptr, S, k, dC.ptr, dY.ptr, S, m, S),
+                        "combine")
 
-        run()
-        Y = d.download(dY, (S, m), np.float64)
-        rows = slice(0, S, 997)
-        ref = A[rows, :k] @ Cm
-        err = float(np.abs(Y[rows] - ref).max() / np.abs(ref).max())
-        d.timer_begin()
-        for _ in range(iters):
             run()
-        ms = d.timer_end() / iters
-        by = S * (k + m) * 8
-        fl = 2.0 * S * k * m
-        out.append(dict(kernel="combine", shape=f"{k}->{m}",
-                        variant=variants["combine"], rows_per_lane=2 if variants["combine"] in (1, 2) else 4,
-                        persistent=variants["combine"] >= 2, us=ms * 1e3,
-                        GBs=by / ms / 1e6,
-                        TFs=fl / ms / 1e9, mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS,
-                        rel_err=err))
-    # the update of a block whose other columns are frozen: 36 basis columns
-    # read once, the 4 changing ones written through a column map
-    import ctypes
-    k, cols = 36, [30, 31, 32, 33]
-    Cm = np.asfortranarray(rng.standard_normal((k, len(cols))))
-    dC = d.upload(Cm)
-    dY = B.DeviceBuffer(S * 34 * 8)
-    cidx = (ctypes.c_int * len(cols))(*cols)
+            Y = d.download(dY, (S, m), dt)
+            rows = slice(0, S, 997)
+            ref = Ad[rows, :k].astype(np.float64) @ Cm
+            err = float(np.abs(Y[rows] - ref).max() / np.abs(ref).max())
+            d.timer_begin()
+            for _ in range(iters):
+                run()
+            ms = d.timer_end() / iters
+            by = S * (k + m) * es
+            fl = 2.0 * S * k * m
+            out.append(dict(prec=pname, kernel="combine", shape=f"{k}->{m}",
+                            variant=variants["combine"], us=ms * 1e3, GBs=by / ms / 1e6,
+                            TFs=fl / ms / 1e9, rel_err=err))
+        if prec == B.FP64:
+            # the update of a block whose other columns are frozen: 36 basis
+            # columns read once, the 4 changing ones written through a column map
+            k, cols = 36, [30, 31, 32, 33]
+            Cm = np.asfortranarray(rng.standard_normal((k, len(cols))))
+            dC = d.upload(Cm)
+            dY = B.DeviceBuffer(S * 34 * 8)
+            cidx = (ctypes.c_int * len(cols))(*cols)
 
-    def run():
-        B.check(B.lib().bdgx_combine_cols(d.h, B.FP64, dA.ptr, S, k, dC.ptr, dY.ptr, S, cidx,
-                                          len(cols), S), "combine_cols")
+            def run():
+                B.check(B.lib().bdgx_combine_cols(d.h, B.FP64, dA.ptr, S, k, dC.ptr, dY.ptr, S,
+                                                  cidx, len(cols), S), "combine_cols")
 
-    run()
-    Y = d.download(dY, (S, 34), np.float64)
-    rows = slice(0, S, 997)
-    ref = A[rows, :k] @ Cm
-    err = float(np.abs(Y[rows][:, cols] - ref).max() / np.abs(ref).max())
-    d.timer_begin()
-    for _ in range(iters):
-        run()
-    ms = d.timer_end() / iters
-    by = S * (k + len(cols)) * 8
-    fl = 2.0 * S * k * len(cols)
-    out.append(dict(kernel="combine_cols", shape=f"{k}->{len(cols)} of 34", us=ms * 1e3,
-                    GBs=by / ms / 1e6, TFs=fl / ms / 1e9,
-                    mfma_frac=fl / ms / 1e9 / FP64_MFMA_PEAK_TFS, rel_err=err))
+            run()
+            Y = d.download(dY, (S, 34), np.float64)
+            rows = slice(0, S, 997)
+            ref = A[rows, :k] @ Cm
+            err = float(np.abs(Y[rows][:, cols] - ref).max() / np.abs(ref).max())
+            d.timer_begin()
+            for _ in range(iters):
+                run()
+            ms = d.timer_end() / iters
+            by = S * (k + len(cols)) * 8
+            fl = 2.0 * S * k * len(cols)
+            out.append(dict(prec=pname, kernel="combine_cols", shape=f"{k}->{len(cols)} of 34",
+                            us=ms * 1e3, GBs=by / ms / 1e6, TFs=fl / ms / 1e9, rel_err=err))
+        del dA, dB
     d.close()
     for r in out:
         print(json.dumps(r), flush=True)
@@ -418,6 +440,8 @@ def main():
                     help="solve the N-problem batch workload sequentially and concurrently")
     ap.add_argument("--blas", action="store_true",
                     help="time the Gram and combine kernels at the solver's shapes and exit")
+    ap.add_argument("--policy", choices=sorted(POLICIES), default=None,
+                    help="run psi0 and one solve policy only (for kernel traces) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the solver runs on a single GPU "
@@ -443,6 +467,14 @@ def main():
     psi, mu, t_itp = ground_state(cfg)
     log(f"psi0 by device ITP: {t_itp:.2f} s, mu = {mu:.10f}")
     st, wu = args.steps, args.warmup
+    if args.policy:
+        prec, tol, form = POLICIES[args.policy]
+        t, r = solve_timed(cfg, psi, mu, prec, st, wu, tol=tol, form=form)
+        print(json.dumps({"policy": args.policy, "config": args.config, "seconds": t,
+                          "iterations": r["stats"]["iterations"],
+                          "iterations_fp32": r["stats"]["iterations_fp32"],
+                          "converged": r["conv"]}), flush=True)
+        return
     t64, r64 = solve_timed(cfg, psi, mu, 0, st, wu, form=0)
     t64p, r64p = solve_timed(cfg, psi, mu, 0, st, wu, form=1)
     tmx, rmx = solve_timed(cfg, psi, mu, 1, st, wu, form=0)

@@ -47,6 +47,13 @@ void bdg_set_fp32_phase(bdg_t *bdg, f64 fp32_tol, uint64_t stall_window);
  * converged pairs are locked and the rest continue in a small block deflated
  * against them (default -1: max(2, sizeSub / 8); 0: never lock). */
 void bdg_set_locking(bdg_t *bdg, int max_active);
+/* First-order formulation: the small block that continues after locking also
+ * carries this many guard columns, started from the Ritz pairs just beyond
+ * the block (default 4).  They are iterated but need not converge and are
+ * not returned; the last wanted pairs then converge at the rate set by the
+ * gap to the first pair beyond the guards instead of the first beyond the
+ * block. */
+void bdg_set_lock_guards(bdg_t *bdg, int guards);
 
 typedef struct {
   uint64_t iterations;          /* LOBPCG iterations                           */
