@@ -4,8 +4,13 @@ system -> trap -> wavefunction -> local interactions -> dipolar -> mu.
 
 Conventions settled here (the configuration text leaves them open):
   * psi0: normalised Gaussian of width 1 (sum |psi|^2 dV = 1), relaxed by
-    `itp_steps` imaginary-time split-step steps of `dt` on the device
-    (bdgx_itp_groundstate), mu = <psi| H_GP |psi> of the final state;
+    imaginary-time split-step steps of `dt` on the device
+    (bdgx_itp_groundstate), mu = <psi| H_GP |psi> of the final state: the
+    stated `itp_steps` where the configuration gives a step count (32^3:
+    2000, 128^3: 4000); where it only asks for a relaxed state (the complex
+    96x96x192 tube, the batch) chunks of `itp_steps` until the imaginary-time
+    residual ||psi' - psi|| / (dt ||psi||) (psi' one more step from psi, not
+    kept) is at most `itp_tol` (at most `itp_max` steps);
   * contact: U_K = U_M = g * density;
   * dipolar coupling g_ddi = g * eps_dd (the reference examples' convention,
     examples/3d_dipolar_atom.c:39-48), dipoles along z, cutoff R as given
@@ -27,7 +32,8 @@ CONFIGS = {
                         tol=1e-8, cplx=False),
     "cfg2_96x96x192_z": dict(N=(96, 96, 192), L=(20.0, 20.0, 80.0), omega=(3.0, 3.0, 0.3),
                              g=3000.0, edd=0.95, R=10.0, itp_steps=2000, dt=1e-3, nev=64, m=64,
-                             tol=1e-8, cplx=True, modulation=0.3, mod_period=4.0, vortex=True),
+                             tol=1e-8, cplx=True, modulation=0.3, mod_period=4.0, vortex=True,
+                             itp_tol=1e-8, itp_max=40000),
     "cfg4_256c_d": dict(N=(256, 256, 256), L=(32.0, 32.0, 32.0), omega=(1.0, 1.0, 2.0),
                         g=5000.0, edd=0.7, R=16.0, itp_steps=2000, dt=1e-3, nev=128, m=128,
                         tol=1e-8, cplx=False),
@@ -53,17 +59,36 @@ def initial_psi(cfg):
     return psi / np.sqrt(np.sum(np.abs(psi) ** 2) * dV)
 
 
-def ground_state(cfg, steps=None):
-    """psi0 and mu by device imaginary-time evolution; returns (psi, mu, seconds)."""
+def ground_state(cfg, steps=None, info=None):
+    """psi0 and mu by device imaginary-time evolution; returns (psi, mu,
+    seconds).  `steps` overrides the configuration's step rule with a fixed
+    count.  `info` (a dict), if given, receives the steps taken and the
+    imaginary-time residual of the last step."""
     import bdg_lobpcg_amd as B
 
     d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
     t0 = time.perf_counter()
-    psi, mu = d.itp_groundstate(initial_psi(cfg), cfg["omega"], cfg["g"], cfg["g"] * cfg["edd"],
-                                (0.0, 0.0, 1.0), cfg["R"], cfg["dt"],
-                                cfg["itp_steps"] if steps is None else steps)
+
+    def run(psi, n):
+        return d.itp_groundstate(psi, cfg["omega"], cfg["g"], cfg["g"] * cfg["edd"],
+                                 (0.0, 0.0, 1.0), cfg["R"], cfg["dt"], n)
+
+    def residual(psi):  # one more step's change (the step is not kept)
+        nxt, _ = run(psi, 1)
+        return float(np.linalg.norm(nxt - psi) / (cfg["dt"] * np.linalg.norm(psi)))
+
+    chunk = cfg["itp_steps"] if steps is None else steps
+    relax = steps is None and cfg.get("itp_tol") is not None
+    psi, mu = run(initial_psi(cfg), chunk)
+    total = chunk
+    while relax and total + chunk <= cfg["itp_max"] and residual(psi) > cfg["itp_tol"]:
+        psi, mu = run(psi, chunk)
+        total += chunk
     dt = time.perf_counter() - t0
+    res = residual(psi) if info is not None else float("nan")
     d.close()
+    if info is not None:
+        info.update(itp_steps=total, itp_residual=res)
     return psi, mu, dt
 
 
@@ -114,8 +139,8 @@ def batch_configs(n=16, seed=20260601):
         asp = 0.25 * 16.0 ** xr()  # log-uniform on [0.25, 4]
         lz = 16.0 / np.sqrt(asp) * N[2] / N[0]
         out.append(dict(N=N, L=(16.0, 16.0, lz), omega=(1.0, 1.0, asp), g=g, edd=edd,
-                        R=min(16.0, lz) / 2, itp_steps=2000, dt=1e-3, nev=16, m=16, tol=1e-8,
-                        cplx=False, name=f"batch{i}"))
+                        R=min(16.0, lz) / 2, itp_steps=2000, itp_tol=1e-8, itp_max=40000,
+                        dt=1e-3, nev=16, m=16, tol=1e-8, cplx=False, name=f"batch{i}"))
     return out
 
 

@@ -8,6 +8,10 @@
 * the 128^3 configuration at full size: every device FP64 mode checked with
   the oracle's K and M (the reference pencil's residual), and the
   mixed solve's omega against the FP64 solve's;
+* the complex 96x96x192 configuration at full size: the first-order FP64
+  solve's modes checked with the oracle's complex K and M;
+* the 48^3 members of the many-problems workload: FP64 and mixed solves,
+  every FP64 mode under the oracle's operators;
 * a complex problem with learned-preconditioner parameters set solves with
   the analytic preconditioner (the network is real-psi0 only).
 The result contract follows the reference's (src/solver.c:262-277,
@@ -143,6 +147,77 @@ def test_cfg1_full_size_modes_under_oracle_operators():
          f"over {len(first)} modes (product-form measure max {pen.max():.2e}); "
          f"w0 {w64[0]:.6g}")
     assert first.max() < 1.5 * cfg["tol"]
+
+
+# ------------------------------------------- complex 96x96x192 configuration
+def test_cfg2_full_size_modes_under_oracle_operators():
+    """Complex 96x96x192 vortex tube, nev = block = 64, psi0 relaxed by
+    device ITP to an imaginary-time residual <= 1e-8: the first-order FP64
+    solve converges every mode, and the lowest and highest eight modes meet
+    the solve's tolerance in the reference pencil's residual evaluated with
+    the oracle's (complex) K and M (up to 1.5x)."""
+    from bdg_lobpcg_amd.workloads import CONFIGS, ground_state, make_problem
+    cfg = CONFIGS["cfg2_96x96x192_z"]
+    info = {}
+    psi, mu, t_itp = ground_state(cfg, info=info)
+    _log(f"cfg2 psi0 {info['itp_steps']} ITP steps, {t_itp:.2f} s, residual "
+         f"{info['itp_residual']:.2e}, mu {mu:.12g}")
+    assert info["itp_residual"] <= cfg["itp_tol"]
+    nev = cfg["nev"]
+    p = make_problem(cfg, psi, mu, precision=0)
+    t0 = time.perf_counter()
+    rc = p.solve()
+    st = p.stats()
+    _log(f"cfg2 fp64 first-order: rc {rc}, {st['iterations']} iterations, "
+         f"{time.perf_counter() - t0:.2f} s, converged {p.converged()}")
+    assert rc == 0 and p.converged() >= nev
+    w = p.eigenvalues(nev)
+    U, V = p.modes(nev)
+    p.close()
+    J = np.r_[0:8, nev - 8:nev]
+    c, _ = oracle_problem(cfg, psi, mu)
+    t0 = time.perf_counter()
+    first, _ = oracle_mode_residuals(c, w[J], U[:, J], V[:, J])
+    _log(f"  oracle residuals of modes {list(J)} ({time.perf_counter() - t0:.1f} s): "
+         f"max {first.max():.2e}; w0 {w[0]:.6g}, w[{nev - 1}] {w[-1]:.6g}")
+    assert first.max() < 1.5 * cfg["tol"]
+
+
+# ----------------------------------------------------- many-problems workload
+def test_cfg3_smallest_members_fp64_and_mixed():
+    """The 48^3 members of the 16-problem workload (batch_configs): psi0
+    relaxed to an imaginary-time residual <= 1e-8; first-order FP64 and mixed
+    solves converge all 16 modes; every FP64 mode meets the tolerance in the
+    reference pencil's residual under the oracle's operators (up to 1.5x);
+    the mixed omega agree with FP64 to 1e-7 (relative, absolute for the
+    Goldstone pair)."""
+    from bdg_lobpcg_amd.workloads import batch_configs, ground_state, make_problem
+    members = [c for c in batch_configs(16) if tuple(c["N"]) == (48, 48, 48)]
+    assert len(members) >= 2
+    for cfg in members:
+        info = {}
+        psi, mu, _ = ground_state(cfg, info=info)
+        assert info["itp_residual"] <= cfg["itp_tol"]
+        out = {}
+        for prec in (0, 1):
+            p = make_problem(cfg, psi, mu, precision=prec)
+            rc = p.solve()
+            _log(f"{cfg['name']} (g {cfg['g']:.0f}, eps_dd {cfg['edd']:.3f}, aspect "
+                 f"{cfg['omega'][2]:.3f}, ITP {info['itp_steps']}): prec {prec} rc {rc}, "
+                 f"{p.stats()['iterations']} iterations, converged {p.converged()}")
+            assert rc == 0 and p.converged() >= cfg["nev"]
+            out[prec] = (p.eigenvalues(cfg["nev"]), p.modes(cfg["nev"]))
+            p.close()
+        w64, (U, V) = out[0]
+        wmx = out[1][0]
+        small = np.abs(w64) < 5e-2 * max(1.0, np.abs(w64).max())
+        err = np.abs(wmx - w64) / np.where(small, 1.0, np.abs(w64))
+        c, _ = oracle_problem(cfg, psi, mu)
+        first, _ = oracle_mode_residuals(c, w64, U, V)
+        _log(f"  mixed vs fp64 max err {err.max():.2e}; oracle pencil residual max "
+             f"{first.max():.2e}")
+        assert np.all(err < 1e-7)
+        assert first.max() < 1.5 * cfg["tol"]
 
 
 def test_complex_problem_with_learned_theta_uses_analytic_preconditioner():
