@@ -154,3 +154,38 @@ def test_trained_network_gain_on_32c():
     np.testing.assert_allclose(w1[nz], w0[nz], rtol=1e-9)
     assert s1["learned_iterations"] > 0
     assert s1["iterations"] < s0["iterations"]
+
+
+@pytest.mark.gpu
+def test_stall_fallback_with_a_useless_network():
+    """A network that only passes high-k content (radial gain exp(-40) at
+    k = 0 rising to 1 at the top knot, in both spectral layers) gives search
+    directions that do not reduce the residual: the stall rule hands over to
+    T_M T_K (fallback recorded), and the solve still converges every mode to
+    the analytic solve's omega (<= 1e-9 relative, non-zero modes)."""
+    from bdg_lobpcg_amd import LP_NKNOT, LP_NPARAM
+    from bdg_lobpcg_amd.workloads import CONFIGS, ground_state, make_problem
+    cfg = CONFIGS["cfg0_32c_d"]
+    psi, mu, _ = ground_state(cfg)
+    theta = np.zeros(LP_NPARAM)
+    ramp = np.linspace(-40.0, 0.0, LP_NKNOT)
+    theta[8:8 + LP_NKNOT] = ramp               # kappa_K
+    theta[8 + LP_NKNOT:8 + 2 * LP_NKNOT] = ramp  # kappa_M
+    out = {}
+    for name, th in (("analytic", None), ("useless", theta)):
+        p = make_problem(cfg, psi, mu, precision=1)
+        p.set_formulation(1)  # product form (the network replaces its T_M T_K)
+        if th is not None:
+            p.set_learned(th)
+        assert p.solve() == 0
+        out[name] = (p.eigenvalues(cfg["nev"]), p.zero_modes(cfg["nev"]), p.stats(),
+                     p.converged())
+        p.close()
+    (w0, z0, s0, c0), (w1, z1, s1, c1) = out["analytic"], out["useless"]
+    print(f"iterations: T_M T_K {s0['iterations']}, useless network {s1['iterations']} "
+          f"(learned iterations {s1['learned_iterations']}, fallback at "
+          f"{s1['learned_fallback_at']})", flush=True)
+    assert s1["learned_fallback_at"] > 0
+    assert c0 >= cfg["nev"] and c1 >= cfg["nev"]
+    nz = ~(z0 | z1)
+    np.testing.assert_allclose(w1[nz], w0[nz], rtol=1e-9)
