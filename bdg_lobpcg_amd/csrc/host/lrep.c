@@ -73,6 +73,100 @@ static int lr_gram(bdgx_ctx *ctx, int prec, const void *A, uint64_t p, const voi
   return rc;
 }
 
+/* Queued uploads of small host matrices (coefficients) through page-locked
+ * staging: each piece is copied into the next free part of the staging
+ * region and its copy queued on the stream, so the host never waits for an
+ * upload.  The region is reused once the stream has drained: every download
+ * (lr_get) and every bdgx_sync drains it. */
+typedef struct {
+  double *h;  /* page-locked */
+  size_t cap; /* doubles */
+  size_t off;
+} lr_stage;
+
+static int lr_put(bdgx_ctx *ctx, lr_stage *st, double *dDst, const double *src, size_t n) {
+  if (!st->h || n > st->cap) return bdgx_memcpy_h2d(ctx, dDst, src, sizeof(double) * n);
+  if (st->off + n > st->cap) {
+    const int rc = bdgx_sync(ctx);
+    if (rc) return rc;
+    st->off = 0;
+  }
+  double *p = st->h + st->off;
+  memcpy(p, src, sizeof(double) * n);
+  st->off += (n + 31) & ~(size_t)31;
+  return bdgx_memcpy_h2d_async(ctx, dDst, p, sizeof(double) * n);
+}
+
+static int lr_get(bdgx_ctx *ctx, lr_stage *st, double *hDst, const double *dSrc, size_t n) {
+  const int rc = bdgx_memcpy_d2h(ctx, hDst, dSrc, sizeof(double) * n);
+  st->off = 0;
+  return rc;
+}
+
+/* C = [-Binv G; I] ((nl + nw) x nw) from G = Lp^H W (nl x nw): the
+ * coefficients of the oblique deflation W - Lo Binv (Lp^H W) over [Lo | W] */
+static void lr_deflate_coef(int nl, int nw, int cplx, const double *Binv, const double *G,
+                            double *C) {
+  const int w = cplx ? 2 : 1;
+  memset(C, 0, sizeof(double) * (size_t)(nl + nw) * nw * w);
+  for (int j = 0; j < nw; ++j) {
+    for (int i = 0; i < nl; ++i) {
+      double complex acc = 0.0;
+      for (int t = 0; t < nl; ++t) {
+        const size_t eb = (size_t)i + (size_t)t * nl, eg = (size_t)t + (size_t)j * nl;
+        const double complex bv = cplx ? Binv[2 * eb] + I * Binv[2 * eb + 1] : Binv[eb];
+        const double complex gv = cplx ? G[2 * eg] + I * G[2 * eg + 1] : G[eg];
+        acc += bv * gv;
+      }
+      const size_t e = (size_t)i + (size_t)j * (nl + nw);
+      C[e * w] = -creal(acc);
+      if (cplx) C[e * w + 1] = -cimag(acc);
+    }
+    C[((size_t)(nl + j) + (size_t)j * (nl + nw)) * w] = 1.0;
+  }
+}
+
+/* Smallest fraction of a column's norm that survives the projection out of
+ * span(Q), Q orthonormal, from GQW = Q^H W (nq x nw) and GWW = W^H W */
+static double lr_kept_min(int nq, int nw, int w, const double *hQW, const double *hWW) {
+  double kept_min = 1.0;
+  for (int j = 0; j < nw; ++j) {
+    const double n2 = hWW[((size_t)j + (size_t)j * nw) * w];
+    double q2 = 0.0;
+    for (int i = 0; i < nq * w; ++i) q2 += hQW[(size_t)j * nq * w + i] * hQW[(size_t)j * nq * w + i];
+    if (n2 > 0) kept_min = fmin(kept_min, sqrt(fmax(0.0, 1.0 - q2 / n2)));
+  }
+  return kept_min;
+}
+
+/* Second orthonormalisation pass over the r columns W' = Z[:, nq:nq+r] (the
+ * first pass cancelled most of a column, which costs orthogonality in
+ * proportion): [Q | W']^H W' in one Gram, W' <- [Q | W'] C through the temp
+ * Wt.  *ncols = the columns kept. */
+static int lr_orth2(bdgx_ctx *ctx, int prec, int cplx, void *Z, int nq, int r, void *Wt,
+                    uint64_t S, double *dG, double *dC, double *hG, double *hC, int *ncols) {
+  const int w = cplx ? 2 : 1;
+  const size_t es = lr_es(prec, cplx);
+  int rc = lr_gram(ctx, prec, Z, (uint64_t)(nq + r), lr_col(Z, (uint64_t)nq, S, es),
+                   (uint64_t)r, S, dG, hG, w);
+  if (rc) return rc;
+  double *tmp = hG + (size_t)(nq + r) * r * w;
+  double *gq = tmp, *gw = tmp + (size_t)nq * r * w;
+  for (int j = 0; j < r; ++j) {
+    memcpy(gq + (size_t)j * nq * w, hG + (size_t)j * (nq + r) * w, sizeof(double) * nq * w);
+    memcpy(gw + (size_t)j * r * w, hG + ((size_t)j * (nq + r) + nq) * w, sizeof(double) * r * w);
+  }
+  const int r2 = dn_orth_coef(nq, r, cplx, gq, gw, 1e-2, hC);
+  *ncols = r2 > 0 ? r2 : 0;
+  if (r2 <= 0) return 0;
+  rc = bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)(nq + r) * r2 * w);
+  if (!rc) rc = bdgx_combine(ctx, prec, Z, S, (uint64_t)(nq + r), dC, Wt, S, (uint64_t)r2, S);
+  if (!rc)
+    rc = bdgx_convert(ctx, prec, Wt, S, prec, lr_col(Z, (uint64_t)nq, S, es), S, S,
+                      (uint64_t)r2);
+  return rc;
+}
+
 /* Orthonormalise the nw columns Wt (temp) against Q = Z[:, 0:nq] and write
  * the result to Z[:, nq:nq+r].  A second pass (against the first's output,
  * which then sits in Z) runs when the first cancelled most of a column (kept
@@ -89,13 +183,7 @@ static int lr_orth(bdgx_ctx *ctx, int prec, int cplx, void *Z, int nq, void *Wt,
   double *hQW = hG, *hWW = hG + (size_t)nq * nw * w;
   if (nq) TRY(lr_gram(ctx, prec, Z, (uint64_t)nq, Wt, (uint64_t)nw, S, dG, hQW, w));
   TRY(lr_gram(ctx, prec, Wt, (uint64_t)nw, Wt, (uint64_t)nw, S, dG, hWW, w));
-  double kept_min = 1.0;
-  for (int j = 0; j < nw; ++j) {
-    const double n2 = hWW[((size_t)j + (size_t)j * nw) * w];
-    double q2 = 0.0;
-    for (int i = 0; i < nq * w; ++i) q2 += hQW[(size_t)j * nq * w + i] * hQW[(size_t)j * nq * w + i];
-    if (n2 > 0) kept_min = fmin(kept_min, sqrt(fmax(0.0, 1.0 - q2 / n2)));
-  }
+  const double kept_min = lr_kept_min(nq, nw, w, hQW, hWW);
   const int r = dn_orth_coef(nq, nw, cplx, hQW, hWW, drop, hC);
   if (r <= 0) return 0;
   TRY(bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)(nq + nw) * r * w));
@@ -103,24 +191,7 @@ static int lr_orth(bdgx_ctx *ctx, int prec, int cplx, void *Z, int nq, void *Wt,
                     lr_col(Z, (uint64_t)nq, S, es), S, (uint64_t)r, S));
   *ncols = r;
   if (kept_min > 0.125) return 0;
-  /* pass 2: [Q | W']^H W' in one Gram (W' now follows Q in Z) */
-  TRY(lr_gram(ctx, prec, Z, (uint64_t)(nq + r), lr_col(Z, (uint64_t)nq, S, es), (uint64_t)r, S,
-              dG, hG, w));
-  {
-    double *tmp = hG + (size_t)(nq + r) * r * w;
-    double *gq = tmp, *gw = tmp + (size_t)nq * r * w;
-    for (int j = 0; j < r; ++j) {
-      memcpy(gq + (size_t)j * nq * w, hG + (size_t)j * (nq + r) * w, sizeof(double) * nq * w);
-      memcpy(gw + (size_t)j * r * w, hG + ((size_t)j * (nq + r) + nq) * w, sizeof(double) * r * w);
-    }
-    const int r2 = dn_orth_coef(nq, r, cplx, gq, gw, 1e-2, hC);
-    *ncols = r2 > 0 ? r2 : 0;
-    if (r2 <= 0) return 0;
-    TRY(bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)(nq + r) * r2 * w));
-    TRY(bdgx_combine(ctx, prec, Z, S, (uint64_t)(nq + r), dC, Wt, S, (uint64_t)r2, S));
-    TRY(bdgx_convert(ctx, prec, Wt, S, prec, lr_col(Z, (uint64_t)nq, S, es), S, S,
-                     (uint64_t)r2));
-  }
+  return lr_orth2(ctx, prec, cplx, Z, nq, r, Wt, S, dG, dC, hG, hC, ncols);
 done:
   return rc;
 }
@@ -253,23 +324,7 @@ static int lr_deflate(bdgx_ctx *ctx, int prec, int cplx, const void *Lo, const v
   const int w = cplx ? 2 : 1;
   int rc = lr_gram(ctx, prec, Lp, (uint64_t)nl, W, (uint64_t)nw, S, dG, hG, w);
   if (rc) return rc;
-  /* C = [-Binv (Lp^H W); I] ((nl + nw) x nw) for the combine over [Lo | W] */
-  memset(hC, 0, sizeof(double) * (size_t)(nl + nw) * nw * w);
-  for (int j = 0; j < nw; ++j) {
-    for (int i = 0; i < nl; ++i) {
-      double complex acc = 0.0;
-      for (int t = 0; t < nl; ++t) {
-        const size_t eb = (size_t)i + (size_t)t * nl, eg = (size_t)t + (size_t)j * nl;
-        const double complex bv = cplx ? Binv[2 * eb] + I * Binv[2 * eb + 1] : Binv[eb];
-        const double complex gv = cplx ? hG[2 * eg] + I * hG[2 * eg + 1] : hG[eg];
-        acc += bv * gv;
-      }
-      const size_t e = (size_t)i + (size_t)j * (nl + nw);
-      hC[e * w] = -creal(acc);
-      if (cplx) hC[e * w + 1] = -cimag(acc);
-    }
-    hC[((size_t)(nl + j) + (size_t)j * (nl + nw)) * w] = 1.0;
-  }
+  lr_deflate_coef(nl, nw, cplx, Binv, hG, hC);
   rc = bdgx_memcpy_h2d(ctx, dC, hC, sizeof(double) * (size_t)(nl + nw) * nw * w);
   if (!rc)
     rc = bdgx_combine2(ctx, prec, Lo, S, (uint64_t)nl, W, S, (uint64_t)nw, dC, Wout, S,
@@ -325,6 +380,7 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   int gvalid = 0; /* hGK/hGM/hGW hold the Grams of the current basis */
   double *hBinv[2] = {NULL, NULL};
   double *gA = NULL, *gB = NULL, *gw2 = NULL; /* guard Ritz pairs at locking */
+  lr_stage st = {NULL, 0, 0};
   void *dUL = NULL, *dVL = NULL, *dU0 = NULL, *dV0 = NULL, *dOut = NULL;
   double trr = 0.0;
   const double t0 = bdg_wtime();
@@ -343,6 +399,8 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   }
   TRY(bdgx_malloc((void **)&dG, sizeof(double) * 4 * bsz));
   TRY(bdgx_malloc((void **)&dC, sizeof(double) * 4 * bsz));
+  st.cap = 8 * bsz;
+  if (bdgx_host_alloc((void **)&st.h, sizeof(double) * st.cap)) st.h = NULL; /* then synchronous */
   TRY(bdgx_malloc((void **)&dZero, sizeof(double) * (size_t)m));
   TRY(bdgx_malloc((void **)&dN2, sizeof(double) * 3 * (size_t)m));
   TRY(bdgx_memset(ctx, dZero, 0, sizeof(double) * (size_t)m));
@@ -393,9 +451,19 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
     /* Grams of the current basis: all on the device at the start and every
      * `refresh` iterations, else the new W columns only */
     if (!gvalid || ++since_full >= refresh) {
-      TRY(lr_gram(ctx, prec, H[0].Z[g], (uint64_t)k[0], H[0].Y[g], (uint64_t)k[0], S, dG, hGK, w));
-      TRY(lr_gram(ctx, prec, H[1].Z[g], (uint64_t)k[1], H[1].Y[g], (uint64_t)k[1], S, dG, hGM, w));
-      TRY(lr_gram(ctx, prec, H[0].Z[g], (uint64_t)k[0], H[1].Z[g], (uint64_t)k[1], S, dG, hGW, w));
+      /* three Grams, one download */
+      const size_t nK = (size_t)k[0] * k[0] * w, nM = (size_t)k[1] * k[1] * w;
+      const size_t nW = (size_t)k[0] * k[1] * w;
+      TRY(bdgx_gram(ctx, prec, H[0].Z[g], S, (uint64_t)k[0], H[0].Y[g], S, (uint64_t)k[0], S, 0,
+                    dG));
+      TRY(bdgx_gram(ctx, prec, H[1].Z[g], S, (uint64_t)k[1], H[1].Y[g], S, (uint64_t)k[1], S, 0,
+                    dG + nK));
+      TRY(bdgx_gram(ctx, prec, H[0].Z[g], S, (uint64_t)k[0], H[1].Z[g], S, (uint64_t)k[1], S, 0,
+                    dG + nK + nM));
+      TRY(lr_get(ctx, &st, hG, dG, nK + nM + nW));
+      memcpy(hGK, hG, sizeof(double) * nK);
+      memcpy(hGM, hG + nK, sizeof(double) * nM);
+      memcpy(hGW, hG + nK + nM, sizeof(double) * nW);
       since_full = 0;
     } else {
       /* hP[0..2]: projected [X | P] blocks (n0 x n0, n1 x n1, n0 x n1) */
@@ -408,32 +476,45 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
       const void *MW1 = lr_col(H[1].Y[g], (uint64_t)n1, S, es);
       int paired = 0;
       if (!cplx && nw[0] == nw[1] && nw[0] > 0) {
-        /* Z_U^H [K W_U | W_V] and Z_V^H [M W_V | W_U]: each basis read once */
+        /* Z_U^H [K W_U | W_V] and Z_V^H [M W_V | W_U]: each basis read once;
+         * both into dG, one download */
         const int q = nw[0];
+        const size_t n0p = (size_t)k[0] * 2 * q, n1p = (size_t)k[1] * 2 * q;
         double *tmp = hC;
         if (bdgx_gram_pair(ctx, prec, H[0].Z[g], S, (uint64_t)k[0], KW0, S, W1, S, (uint64_t)q,
                            S, dG) == 0 &&
-            bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * (size_t)k[0] * 2 * q) == 0) {
+            bdgx_gram_pair(ctx, prec, H[1].Z[g], S, (uint64_t)k[1], MW1, S, W0, S, (uint64_t)q,
+                           S, dG + n0p) == 0) {
+          TRY(lr_get(ctx, &st, tmp, dG, n0p + n1p));
           memcpy(gKc, tmp, sizeof(double) * (size_t)k[0] * q);
           memcpy(gWc, tmp + (size_t)k[0] * q, sizeof(double) * (size_t)k[0] * q);
-          if (bdgx_gram_pair(ctx, prec, H[1].Z[g], S, (uint64_t)k[1], MW1, S, W0, S,
-                             (uint64_t)q, S, dG) == 0 &&
-              bdgx_memcpy_d2h(ctx, tmp, dG, sizeof(double) * (size_t)k[1] * 2 * q) == 0) {
-            memcpy(gMc, tmp, sizeof(double) * (size_t)k[1] * q);
-            memcpy(gWr, tmp + (size_t)k[1] * q, sizeof(double) * (size_t)k[1] * q);
-            paired = 1;
-          }
+          memcpy(gMc, tmp + n0p, sizeof(double) * (size_t)k[1] * q);
+          memcpy(gWr, tmp + n0p + (size_t)k[1] * q, sizeof(double) * (size_t)k[1] * q);
+          paired = 1;
         }
       }
       if (!paired) {
+        /* the four W-column Grams into dG side by side, one download */
+        const size_t o1 = c0, o2 = o1 + (size_t)k[1] * nw[0] * w, o3 = o2 + c1;
+        const size_t ntot = o3 + (size_t)k[0] * nw[1] * w;
         if (nw[0]) {
-          TRY(lr_gram(ctx, prec, H[0].Z[g], (uint64_t)k[0], KW0, (uint64_t)nw[0], S, dG, gKc, w));
-          TRY(lr_gram(ctx, prec, H[1].Z[g], (uint64_t)k[1], W0, (uint64_t)nw[0], S, dG, gWr, w));
+          TRY(bdgx_gram(ctx, prec, H[0].Z[g], S, (uint64_t)k[0], KW0, S, (uint64_t)nw[0], S, 0,
+                        dG));
+          TRY(bdgx_gram(ctx, prec, H[1].Z[g], S, (uint64_t)k[1], W0, S, (uint64_t)nw[0], S, 0,
+                        dG + o1));
         }
         if (nw[1]) {
-          TRY(lr_gram(ctx, prec, H[1].Z[g], (uint64_t)k[1], MW1, (uint64_t)nw[1], S, dG, gMc, w));
-          TRY(lr_gram(ctx, prec, H[0].Z[g], (uint64_t)k[0], W1, (uint64_t)nw[1], S, dG, gWc, w));
+          TRY(bdgx_gram(ctx, prec, H[1].Z[g], S, (uint64_t)k[1], MW1, S, (uint64_t)nw[1], S, 0,
+                        dG + o2));
+          TRY(bdgx_gram(ctx, prec, H[0].Z[g], S, (uint64_t)k[0], W1, S, (uint64_t)nw[1], S, 0,
+                        dG + o3));
         }
+        double *tmp = hC;
+        TRY(lr_get(ctx, &st, tmp, dG, ntot));
+        memcpy(gKc, tmp, sizeof(double) * c0);
+        memcpy(gWr, tmp + o1, sizeof(double) * (size_t)k[1] * nw[0] * w);
+        memcpy(gMc, tmp + o2, sizeof(double) * c1);
+        memcpy(gWc, tmp + o3, sizeof(double) * (size_t)k[0] * nw[1] * w);
       }
       lr_assemble(k[0], k[0], n0, n0, w, hP[0], gKc, gKc, hGK);
       lr_assemble(k[1], k[1], n1, n1, w, hP[1], gMc, gMc, hGM);
@@ -455,17 +536,29 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
       rc = -100;
       goto done;
     }
-    /* the Grams of the next [X | P], projected */
+    trr += bdg_wtime() - tr;
+    /* the update: both halves' coefficients in one upload (each half's
+     * block at its own offset of dC), then the four combines, queued */
+    {
+      const size_t off1 = (size_t)k[0] * nc[0] * w;
+      TRY(lr_put(ctx, &st, dC, hCQ[0], off1));
+      TRY(lr_put(ctx, &st, dC + off1, hCQ[1], (size_t)k[1] * nc[1] * w));
+      for (int h = 0; h < 2; ++h) {
+        const double *dCh = dC + (h ? off1 : 0);
+        TRY(bdgx_combine(ctx, prec, H[h].Z[g], S, (uint64_t)k[h], dCh, H[h].Z[g ^ 1], S,
+                         (uint64_t)nc[h], S));
+        TRY(bdgx_combine(ctx, prec, H[h].Y[g], S, (uint64_t)k[h], dCh, H[h].Y[g ^ 1], S,
+                         (uint64_t)nc[h], S));
+      }
+    }
+    /* the Grams of the next [X | P], projected on the host while the device
+     * runs the update */
+    tr = bdg_wtime();
     lr_project(k[0], k[0], nc[0], nc[0], cplx, hGK, hCQ[0], hCQ[0], hC, hP[0]);
     lr_project(k[1], k[1], nc[1], nc[1], cplx, hGM, hCQ[1], hCQ[1], hC, hP[1]);
     lr_project(k[0], k[1], nc[0], nc[1], cplx, hGW, hCQ[0], hCQ[1], hC, hP[2]);
     trr += bdg_wtime() - tr;
     for (int h = 0; h < 2; ++h) {
-      TRY(bdgx_memcpy_h2d(ctx, dC, hCQ[h], sizeof(double) * (size_t)k[h] * nc[h] * w));
-      TRY(bdgx_combine(ctx, prec, H[h].Z[g], S, (uint64_t)k[h], dC, H[h].Z[g ^ 1], S,
-                       (uint64_t)nc[h], S));
-      TRY(bdgx_combine(ctx, prec, H[h].Y[g], S, (uint64_t)k[h], dC, H[h].Y[g ^ 1], S,
-                       (uint64_t)nc[h], S));
       H[h].np = nc[h] - m;
       kprev[h] = k[h];
       k[h] = nc[h];
@@ -504,13 +597,16 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
                   -f * Ro[((size_t)i + (size_t)j * m) * w + q];
             }
         }
-        TRY(bdgx_memcpy_h2d(ctx, dC, Cc, sizeof(double) * 2 * (size_t)m * nsel * w));
-        TRY(bdgx_combine2(ctx, prec, H[h].Y[g], S, (uint64_t)m, H[h ^ 1].Z[g], S, (uint64_t)m, dC,
-                          dR[h], S, (uint64_t)nsel, S));
+        /* each half's coefficients in a region of dC of its own, after the
+         * update's (at most 2 x 3m x 2m) */
+        double *dCh = dC + (size_t)(6 + h) * 2 * m * m * w;
+        TRY(lr_put(ctx, &st, dCh, Cc, 2 * (size_t)m * nsel * w));
+        TRY(bdgx_combine2(ctx, prec, H[h].Y[g], S, (uint64_t)m, H[h ^ 1].Z[g], S, (uint64_t)m,
+                          dCh, dR[h], S, (uint64_t)nsel, S));
       }
       TRY(bdgx_residual(ctx, prec, dR[0], dR[1], NULL, S, (uint64_t)nsel, S, dZero, 0, NULL, S,
                         dN2));
-      TRY(bdgx_memcpy_d2h(ctx, hn2, dN2, sizeof(double) * 3 * (size_t)nsel));
+      TRY(lr_get(ctx, &st, hn2, dN2, 3 * (size_t)nsel));
       /* ||A x - w B x|| / (||x|| max(1, |w|)), ||x||^2 from the coefficients
        * (orthonormal bases) */
       for (int c = 0; c < nsel; ++c) {
@@ -709,31 +805,85 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
       wact[i] = p; /* position, reused below as the column list */
       if (p != i) packed = 0;
     }
+    /* both halves go through each phase together, so each phase waits for
+     * the device once: preconditioner applies, deflation against the
+     * locked pairs (one download), orthonormalisation against [X | P] (one
+     * download), images */
+    void *wsrc[2], *wtmp[2];
     for (int h = 0; h < 2; ++h) {
-      void *wsrc = dR[h], *wtmp = dWt[h];
+      wsrc[h] = dR[h];
+      wtmp[h] = dWt[h];
       if (!packed) {
         TRY(bdgx_gather_cols(ctx, prec, dR[h], S, wact, (uint64_t)nact, dWt[h], S, S));
-        wsrc = dWt[h];
-        wtmp = dR[h];
+        wsrc[h] = dWt[h];
+        wtmp[h] = dR[h];
       }
-      TRY(bdgx_apply(ctx, opT[h], prec, wsrc, wtmp, (uint64_t)nact, S));
-      if (cfg->nl > 0) {
-        /* W into the deflated subspace (the residual in wsrc is consumed) */
-        TRY(lr_deflate(ctx, prec, cplx, Lown[h], Lpar[h], cfg->nl, hBinv[h], wtmp, nact, wsrc,
-                       S, dG, dC, hG, hC));
-        void *t = wtmp;
-        wtmp = wsrc;
-        wsrc = t;
+      TRY(bdgx_apply(ctx, opT[h], prec, wsrc[h], wtmp[h], (uint64_t)nact, S));
+    }
+    const int nl = cfg->nl;
+    if (nl > 0) {
+      /* W into the deflated subspace (the residual in wsrc is consumed) */
+      const size_t ngl = (size_t)nl * nact * w, ncf = (size_t)(nl + nact) * nact * w;
+      for (int h = 0; h < 2; ++h)
+        TRY(bdgx_gram(ctx, prec, Lpar[h], S, (uint64_t)nl, wtmp[h], S, (uint64_t)nact, S, 0,
+                      dG + h * ngl));
+      TRY(lr_get(ctx, &st, hG, dG, 2 * ngl));
+      for (int h = 0; h < 2; ++h) {
+        lr_deflate_coef(nl, nact, cplx, hBinv[h], hG + h * ngl, hC);
+        TRY(lr_put(ctx, &st, dC + h * ncf, hC, ncf));
+        TRY(bdgx_combine2(ctx, prec, Lown[h], S, (uint64_t)nl, wtmp[h], S, (uint64_t)nact,
+                          dC + h * ncf, wsrc[h], S, (uint64_t)nact, S));
+        void *t = wtmp[h];
+        wtmp[h] = wsrc[h];
+        wsrc[h] = t;
       }
-      const int nq = k[h];
-      int r = 0;
-      TRY(lr_orth(ctx, prec, cplx, H[h].Z[g], nq, wtmp, nact, S, 1e-3, dG, dC, hG, hC, &r));
-      if (r > 0)
-        TRY(bdgx_apply(ctx, opA[h], prec, lr_col(H[h].Z[g], (uint64_t)nq, S, es),
-                       lr_col(H[h].Y[g], (uint64_t)nq, S, es), (uint64_t)r, S));
-      nw[h] = r;
-      k[h] = nq + r;
-      res->applies += (uint64_t)r;
+    }
+    {
+      /* [X | P]^H W and W^H W of both halves, one download */
+      int nq[2], r[2];
+      double kept[2];
+      size_t oQ[2], oW[2], tot = 0;
+      for (int h = 0; h < 2; ++h) {
+        nq[h] = k[h];
+        oQ[h] = tot;
+        tot += (size_t)nq[h] * nact * w;
+        oW[h] = tot;
+        tot += (size_t)nact * nact * w;
+        TRY(bdgx_gram(ctx, prec, H[h].Z[g], S, (uint64_t)nq[h], wtmp[h], S, (uint64_t)nact, S, 0,
+                      dG + oQ[h]));
+        TRY(bdgx_gram(ctx, prec, wtmp[h], S, (uint64_t)nact, wtmp[h], S, (uint64_t)nact, S, 0,
+                      dG + oW[h]));
+      }
+      TRY(lr_get(ctx, &st, hG, dG, tot));
+      /* a region of dC after the deflation's */
+      const size_t odef = nl > 0 ? 2 * (size_t)(nl + nact) * nact * w : 0;
+      const size_t o8 = (size_t)8 * m * m * w;
+      double *dCo = dC + (odef > o8 ? odef : o8);
+      for (int h = 0; h < 2; ++h) {
+        kept[h] = lr_kept_min(nq[h], nact, w, hG + oQ[h], hG + oW[h]);
+        r[h] = dn_orth_coef(nq[h], nact, cplx, hG + oQ[h], hG + oW[h], 1e-3, hC);
+        if (r[h] <= 0) {
+          r[h] = 0;
+          continue;
+        }
+        const size_t nco = (size_t)(nq[h] + nact) * r[h] * w;
+        TRY(lr_put(ctx, &st, dCo, hC, nco));
+        TRY(bdgx_combine2(ctx, prec, H[h].Z[g], S, (uint64_t)nq[h], wtmp[h], S, (uint64_t)nact,
+                          dCo, lr_col(H[h].Z[g], (uint64_t)nq[h], S, es), S, (uint64_t)r[h],
+                          S));
+        dCo += nco;
+      }
+      for (int h = 0; h < 2; ++h) {
+        if (r[h] > 0 && kept[h] <= 0.125)
+          TRY(lr_orth2(ctx, prec, cplx, H[h].Z[g], nq[h], r[h], wtmp[h], S, dG, dC, hG, hC,
+                       &r[h]));
+        if (r[h] > 0)
+          TRY(bdgx_apply(ctx, opA[h], prec, lr_col(H[h].Z[g], (uint64_t)nq[h], S, es),
+                         lr_col(H[h].Y[g], (uint64_t)nq[h], S, es), (uint64_t)r[h], S));
+        nw[h] = r[h];
+        k[h] = nq[h] + r[h];
+        res->applies += (uint64_t)r[h];
+      }
     }
     nwact = nact;
     memcpy(wact, act, sizeof(int) * (size_t)nact);
@@ -780,6 +930,10 @@ done:
   free(gA);
   free(gB);
   free(gw2);
+  if (st.h) {
+    (void)bdgx_sync(ctx); /* queued uploads read the staging region */
+    bdgx_host_free(st.h);
+  }
   bdgx_free(dUL);
   bdgx_free(dVL);
   bdgx_free(dU0);
