@@ -88,7 +88,16 @@ static void tridiagonalize(int n, double *A, double *Q, double *d, double *e) {
   free(qv);
 }
 
-/* implicit QR on the tridiagonal (d, e) with vectors accumulated in Z */
+/* sqrt(a^2 + b^2): the plain formula where it cannot overflow or lose all
+ * digits to underflow, hypot otherwise (hypot costs several times more) */
+static inline double hyp(double a, double b) {
+  const double m = fmax(fabs(a), fabs(b));
+  if (m < 1e150 && m > 1e-150) return sqrt(a * a + b * b);
+  return hypot(a, b);
+}
+
+/* implicit QR on the tridiagonal (d, e) with vectors accumulated in Z (or
+ * eigenvalues only when Z is NULL) */
 static int tridiag_qr(int n, double *d, double *e, double *Z) {
   int q = n - 1;
   int guard = 0;
@@ -103,12 +112,12 @@ static int tridiag_qr(int n, double *d, double *e, double *Z) {
     /* Wilkinson shift from the trailing 2x2 of the block [p, q] */
     const double dl = 0.5 * (d[q - 1] - d[q]);
     const double eq = e[q - 1];
-    const double den = dl + (dl >= 0 ? 1.0 : -1.0) * hypot(dl, eq);
+    const double den = dl + (dl >= 0 ? 1.0 : -1.0) * hyp(dl, eq);
     const double mu = den != 0.0 ? d[q] - eq * eq / den : d[q];
     double x = d[p] - mu, z = e[p];
     double bulge = 0.0;
     for (int k = p; k < q; ++k) {
-      const double r = hypot(x, z);
+      const double r = hyp(x, z);
       double c = 1.0, s = 0.0;
       if (r != 0.0) {
         c = x / r;
@@ -123,11 +132,13 @@ static int tridiag_qr(int n, double *d, double *e, double *Z) {
         bulge = s * e[k + 1];
         e[k + 1] = c * e[k + 1];
       }
-      double *zk = Z + (size_t)k * n, *zk1 = Z + (size_t)(k + 1) * n;
-      for (int i = 0; i < n; ++i) {
-        const double u = zk[i], w = zk1[i];
-        zk[i] = c * u + s * w;
-        zk1[i] = -s * u + c * w;
+      if (Z) {
+        double *zk = Z + (size_t)k * n, *zk1 = Z + (size_t)(k + 1) * n;
+        for (int i = 0; i < n; ++i) {
+          const double u = zk[i], w = zk1[i];
+          zk[i] = c * u + s * w;
+          zk1[i] = -s * u + c * w;
+        }
       }
       x = e[k];
       z = bulge;
@@ -137,8 +148,8 @@ static int tridiag_qr(int n, double *d, double *e, double *Z) {
 }
 
 static void sort_eig(int n, double *w, double *V, int ldv, int vlen) {
-  /* selection sort (n is small) */
-  double *tmp = (double *)malloc(sizeof(double) * (size_t)vlen);
+  /* selection sort (n is small); V may be NULL */
+  double *tmp = V ? (double *)malloc(sizeof(double) * (size_t)vlen) : NULL;
   for (int i = 0; i < n; ++i) {
     int k = i;
     for (int j = i + 1; j < n; ++j)
@@ -147,12 +158,131 @@ static void sort_eig(int n, double *w, double *V, int ldv, int vlen) {
       const double t = w[i];
       w[i] = w[k];
       w[k] = t;
-      memcpy(tmp, V + (size_t)i * ldv, sizeof(double) * vlen);
-      memcpy(V + (size_t)i * ldv, V + (size_t)k * ldv, sizeof(double) * vlen);
-      memcpy(V + (size_t)k * ldv, tmp, sizeof(double) * vlen);
+      if (V) {
+        memcpy(tmp, V + (size_t)i * ldv, sizeof(double) * vlen);
+        memcpy(V + (size_t)i * ldv, V + (size_t)k * ldv, sizeof(double) * vlen);
+        memcpy(V + (size_t)k * ldv, tmp, sizeof(double) * vlen);
+      }
     }
   }
   free(tmp);
+}
+
+/* LU factorisation with partial pivoting of the tridiagonal T - s I (the
+ * layout of LAPACK's dgttrf: dl multipliers, d / du / du2 the diagonals of
+ * U, sw[i] = 1 when rows i and i+1 were exchanged); pivots smaller than
+ * tiny are replaced by +-tiny */
+static void trid_lu(int n, const double *dd, const double *e, double s, double tiny, double *dl,
+                    double *d, double *du, double *du2, int *sw) {
+  for (int i = 0; i < n; ++i) d[i] = dd[i] - s;
+  for (int i = 0; i + 1 < n; ++i) {
+    dl[i] = e[i];
+    du[i] = e[i];
+    du2[i] = 0.0;
+    sw[i] = 0;
+  }
+  for (int i = 0; i + 1 < n; ++i) {
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (fabs(d[i]) < tiny) d[i] = d[i] < 0 ? -tiny : tiny;
+      const double f = dl[i] / d[i];
+      dl[i] = f;
+      d[i + 1] -= f * du[i];
+    } else {
+      const double f = d[i] / dl[i];
+      d[i] = dl[i];
+      dl[i] = f;
+      const double t = du[i];
+      du[i] = d[i + 1];
+      d[i + 1] = t - f * d[i + 1];
+      if (i + 2 < n) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -f * du2[i];
+      }
+      sw[i] = 1;
+    }
+  }
+  if (fabs(d[n - 1]) < tiny) d[n - 1] = d[n - 1] < 0 ? -tiny : tiny;
+}
+
+/* x <- (T - s I)^-1 x with the factors of trid_lu */
+static void trid_solve(int n, const double *dl, const double *d, const double *du,
+                       const double *du2, const int *sw, double *x) {
+  for (int i = 0; i + 1 < n; ++i) {
+    if (!sw[i]) {
+      x[i + 1] -= dl[i] * x[i];
+    } else {
+      const double t = x[i];
+      x[i] = x[i + 1];
+      x[i + 1] = t - dl[i] * x[i];
+    }
+  }
+  x[n - 1] /= d[n - 1];
+  if (n > 1) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / d[n - 2];
+  for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
+}
+
+int dn_trid_low(int n, const double *d0, const double *e0, int m, double *w, double *Y) {
+  if (n <= 0 || m <= 0) return 0;
+  if (m > n) m = n;
+  double *d = (double *)malloc(sizeof(double) * (size_t)n);
+  double *e = (double *)malloc(sizeof(double) * (size_t)n);
+  double *lam = (double *)malloc(sizeof(double) * (size_t)n);
+  double *f[4];
+  for (int q = 0; q < 4; ++q) f[q] = (double *)malloc(sizeof(double) * (size_t)n);
+  int *sw = (int *)malloc(sizeof(int) * (size_t)n);
+  memcpy(d, d0, sizeof(double) * (size_t)n);
+  for (int i = 0; i + 1 < n; ++i) e[i] = e0[i];
+  e[n - 1] = 0.0;
+  double tn = 0.0; /* 1-norm of T */
+  for (int i = 0; i < n; ++i)
+    tn = fmax(tn, fabs(d0[i]) + (i > 0 ? fabs(e0[i - 1]) : 0.0) + (i + 1 < n ? fabs(e0[i]) : 0.0));
+  if (!(tn > 0)) tn = 1.0;
+  int rc = tridiag_qr(n, d, e, NULL);
+  memcpy(lam, d, sizeof(double) * (size_t)n);
+  sort_eig(n, lam, NULL, 0, 0);
+  const double ortol = 1e-3 * tn, pertol = 10.0 * DBL_EPSILON * tn, tiny = DBL_EPSILON * tn;
+  int c0 = 0;          /* first vector of the current cluster */
+  double sprev = 0.0;  /* previous shift */
+  uint32_t seed = 0x9e3779b9u;
+  for (int j = 0; j < m; ++j) {
+    double s = lam[j];
+    if (j > 0 && lam[j] - lam[j - 1] > ortol) c0 = j;
+    if (j > c0 && s - sprev < pertol) s = sprev + pertol; /* distinct shifts in a cluster */
+    sprev = s;
+    trid_lu(n, d0, e0, s, tiny, f[0], f[1], f[2], f[3], sw);
+    double *y = Y + (size_t)j * n;
+    for (int i = 0; i < n; ++i) { /* fixed pseudo-random start */
+      seed ^= seed << 13;
+      seed ^= seed >> 17;
+      seed ^= seed << 5;
+      y[i] = (double)(seed & 0xffffff) / 16777216.0 - 0.5;
+    }
+    for (int itr = 0; itr < 3; ++itr) {
+      trid_solve(n, f[0], f[1], f[2], f[3], sw, y);
+      for (int pass = 0; pass < 2; ++pass) /* against the cluster's earlier vectors */
+        for (int c = c0; c < j; ++c) {
+          const double *yc = Y + (size_t)c * n;
+          double dot = 0.0;
+          for (int i = 0; i < n; ++i) dot += yc[i] * y[i];
+          for (int i = 0; i < n; ++i) y[i] -= dot * yc[i];
+        }
+      double nrm = 0.0;
+      for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
+      nrm = sqrt(nrm);
+      if (!(nrm > 0)) {
+        rc = -1;
+        break;
+      }
+      for (int i = 0; i < n; ++i) y[i] /= nrm;
+    }
+    w[j] = lam[j];
+  }
+  free(d);
+  free(e);
+  free(lam);
+  for (int q = 0; q < 4; ++q) free(f[q]);
+  free(sw);
+  return rc;
 }
 
 int dn_syev(int n, double *A, double *w) {
@@ -265,6 +395,11 @@ int dn_lrep_rr(int ku, int kv, int m, int cplx, const double *GK, const double *
     return lrep_rr_z(ku, kv, m, (const double complex *)GK, (const double complex *)GM,
                      (const double complex *)GW, w2, (double complex *)A, (double complex *)B);
   return lrep_rr_d(ku, kv, m, GK, GM, GW, w2, A, B);
+}
+
+int dn_eig_low(int n, int m, int cplx, const double *G, double *w, double *X) {
+  if (cplx) return eig_low_z(n, m, (const double complex *)G, w, (double complex *)X);
+  return eig_low_d(n, m, G, w, X);
 }
 
 int dn_lrep_update(int k, int m, int cplx, const double *A, const int *act, int nact,

@@ -14,6 +14,17 @@ int dn_syev(int n, double *A, double *w);
  * convention; A is overwritten by the eigenvectors. */
 int dn_heev(int n, double *A, double *w);
 
+/* The m lowest eigenpairs of the real symmetric tridiagonal matrix with
+ * diagonal d[n] and off-diagonal e[n-1]: w[m] ascending and Y (n x m)
+ * orthonormal, by implicit QR without vectors and inverse iteration
+ * (Gram-Schmidt inside clusters closer than 1e-3 ||T||).  Returns 0 or -1. */
+int dn_trid_low(int n, const double *d, const double *e, int m, double *w, double *Y);
+
+/* The m lowest eigenpairs of Hermitian G (n x n, complex interleaved when
+ * cplx; not modified): w[m] ascending, X (n x m) orthonormal eigenvectors.
+ * Returns 0 or < 0. */
+int dn_eig_low(int n, int m, int cplx, const double *G, double *w, double *X);
+
 /* Rayleigh-Ritz for the Hermitian-definite pencil (G_A, G_M) on a basis
  * with Euclidean Gram G_V (all k x k, complex interleaved when cplx):
  *   w2      out: the m smallest Ritz values (w^2), ascending

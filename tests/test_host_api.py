@@ -397,3 +397,44 @@ def test_host_first_order_update_and_orthonormalisation(cplx):
     Wn = np.concatenate([Qb, W], axis=1) @ C[:, :r]
     assert np.abs(Wn.conj().T @ Wn - np.eye(r)).max() < 1e-10
     assert np.abs(Qb.conj().T @ Wn).max() < 1e-10
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("kind", ["random", "degenerate_pairs", "block_diagonal"])
+def test_host_lowest_eigenpairs(cplx, kind):
+    """dn_eig_low (Householder tridiagonalisation, eigenvalues by implicit QR
+    without vectors, inverse iteration for the lowest m vectors, Gram-Schmidt
+    inside clusters) against numpy.linalg.eigh: eigenvalues to 1e-12
+    relative to ||G||, residuals ||G x - w x|| <= 1e-12 ||G||, orthonormal
+    vectors; exactly degenerate pairs (the spectra of the solver's
+    projected problems have many) and a matrix whose off-diagonal blocks
+    vanish (some reflectors are the identity)."""
+    import ctypes as C
+    import bdg_lobpcg_amd as B
+    lib = B.lib()
+    lib.dn_eig_low.restype = C.c_int
+    lib.dn_eig_low.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(7 + 3 * cplx)
+    dt = np.complex128 if cplx else np.float64
+    for n, m in ((96, 32), (40, 40), (7, 3), (2, 1), (1, 1)):
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0))
+        if kind == "random":
+            lam = rng.standard_normal(n) * 3.0
+        elif kind == "degenerate_pairs":
+            lam = np.repeat(np.sort(rng.standard_normal((n + 1) // 2)), 2)[:n]
+        else:
+            lam = rng.standard_normal(n)
+        G = (Q * lam[None, :]) @ Q.conj().T
+        if kind == "block_diagonal" and n > 4:
+            G[: n // 2, n // 2:] = 0
+            G[n // 2:, : n // 2] = 0
+        G = np.asfortranarray(0.5 * (G + G.conj().T)).astype(dt)
+        w = np.zeros(m)
+        X = np.zeros((n, m), dtype=dt, order="F")
+        assert lib.dn_eig_low(n, m, int(cplx), G.ctypes.data, w.ctypes.data, X.ctypes.data) == 0
+        ref = np.linalg.eigvalsh(G)[:m]
+        gn = max(np.abs(G).max(), 1e-300)
+        np.testing.assert_allclose(w, ref, atol=1e-12 * gn * n)
+        R = G @ X - X * w[None, :]
+        assert np.abs(R).max() < 1e-12 * gn * n, (n, m, np.abs(R).max())
+        np.testing.assert_allclose(X.conj().T @ X, np.eye(m), atol=1e-12 * n)
