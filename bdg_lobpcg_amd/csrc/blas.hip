@@ -1066,19 +1066,29 @@ __device__ __forceinline__ void load_rows_nb(const Cols<R> &A, int col, long lon
 // groups of PD steps of 4 with PD steps of A in flight; every A load is
 // unconditional (clamped), so the loads of a step are awaited exactly.
 // Same MFMA sequence per output element as k_combine_mfma.  Needs
-// vector-aligned A and Y and nrows a multiple of RT.
-template <typename R, int MF, int RT, int PD>
+// vector-aligned A and Y and nrows a multiple of RT.  With M32 (FP32
+// storage only): v_mfma_f32_16x16x4f32, the coefficients rounded to FP32
+// and the sums over k in FP32.
+constexpr bool kCombineF32Mfma = false;
+
+template <typename R, int MF, int RT, int PD, bool M32 = false>
 __global__ void __launch_bounds__(256) k_combine_pers(Cols<R> A, int k, const double *C, R *Y,
                                                       long long ldy, int m, long long nrows,
                                                       long long ntiles) {
+  static_assert(!M32 || sizeof(R) == 4, "FP32 MFMA combine needs FP32 storage");
   extern __shared__ double cs[];
+  float *csf = reinterpret_cast<float *>(cs);
   constexpr int CW = 16 * MF + 1;
   const int kpad = (k + 3) & ~3;
   const int nsteps = kpad >> 2;
   const int j0 = blockIdx.y * 16 * MF;
   for (int e = threadIdx.x; e < kpad * 16 * MF; e += 256) {
     const int i = e % kpad, j = e / kpad;
-    cs[i * CW + j] = (i < k && j0 + j < m) ? C[i + (long long)(j0 + j) * k] : 0.0;
+    const double cv = (i < k && j0 + j < m) ? C[i + (long long)(j0 + j) * k] : 0.0;
+    if constexpr (M32)
+      csf[i * CW + j] = (float)cv;
+    else
+      cs[i * CW + j] = cv;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1104,24 +1114,42 @@ __global__ void __launch_bounds__(256) k_combine_pers(Cols<R> A, int k, const do
       }
     }
     v4d acc[RT][MF];
+    v4f acc32[RT][MF];
 #pragma unroll
     for (int t = 0; t < RT; ++t)
 #pragma unroll
-      for (int f = 0; f < MF; ++f) acc[t][f] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int f = 0; f < MF; ++f) {
+        if constexpr (M32)
+          acc32[t][f] = v4f{0.f, 0.f, 0.f, 0.f};
+        else
+          acc[t][f] = v4d{0.0, 0.0, 0.0, 0.0};
+      }
     for (int s0 = 0; s0 < nsteps; s0 += PD) {
 #pragma unroll
       for (int d = 0; d < PD; ++d) {
         const int s = s0 + d;
         if (s < nsteps) {
-          double bv[MF];
+          if constexpr (M32) {
+            float bv[MF];
 #pragma unroll
-          for (int f = 0; f < MF; ++f) bv[f] = cs[(4 * s + kk) * CW + 16 * f + c];
+            for (int f = 0; f < MF; ++f) bv[f] = csf[(4 * s + kk) * CW + 16 * f + c];
 #pragma unroll
-          for (int t = 0; t < RT; ++t)
+            for (int t = 0; t < RT; ++t)
 #pragma unroll
-            for (int f = 0; f < MF; ++f)
-              acc[t][f] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[d][t], bv[f],
-                                                               acc[t][f], 0, 0, 0);
+              for (int f = 0; f < MF; ++f)
+                acc32[t][f] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av[d][t], bv[f],
+                                                                   acc32[t][f], 0, 0, 0);
+          } else {
+            double bv[MF];
+#pragma unroll
+            for (int f = 0; f < MF; ++f) bv[f] = cs[(4 * s + kk) * CW + 16 * f + c];
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+              for (int f = 0; f < MF; ++f)
+                acc[t][f] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[d][t], bv[f],
+                                                                 acc[t][f], 0, 0, 0);
+          }
         }
         // refill this buffer with step s + PD
         int sn = s + PD;
@@ -1135,6 +1163,8 @@ __global__ void __launch_bounds__(256) k_combine_pers(Cols<R> A, int k, const do
         __builtin_amdgcn_sched_barrier(0);
       }
     }
+    // accumulator element v of lane (c, kk) is row kk + 4 v of the 16-row
+    // MFMA block for the FP64 MFMA, row 4 kk + v for the FP32 one
 #pragma unroll
     for (int f = 0; f < MF; ++f) {
       const int col = j0 + 16 * f + c;
@@ -1142,9 +1172,14 @@ __global__ void __launch_bounds__(256) k_combine_pers(Cols<R> A, int k, const do
       R *yc = Y + (long long)col * ldy;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const long long rr = r0 + RT * (kk + 4 * v);
+        const long long rr = r0 + RT * (M32 ? 4 * kk + v : kk + 4 * v);
         if (rr >= nrows) continue;
-        if constexpr (sizeof(R) == 8 && RT == 4) {
+        if constexpr (M32 && RT == 4) {
+          *reinterpret_cast<float4 *>(yc + rr) =
+              float4{acc32[0][f][v], acc32[1][f][v], acc32[2][f][v], acc32[3][f][v]};
+        } else if constexpr (M32) {
+          *reinterpret_cast<float2 *>(yc + rr) = float2{acc32[0][f][v], acc32[1][f][v]};
+        } else if constexpr (sizeof(R) == 8 && RT == 4) {
           reinterpret_cast<double2 *>(yc + rr)[0] = double2{acc[0][f][v], acc[1][f][v]};
           reinterpret_cast<double2 *>(yc + rr)[1] = double2{acc[2][f][v], acc[3][f][v]};
         } else if constexpr (sizeof(R) == 8) {
@@ -1164,17 +1199,18 @@ template <typename R, int MF, int RT, int PD>
 static void launch_combine_pers(const bdgx_ctx *c, unsigned gy, size_t lds, hipStream_t s,
                                 const Cols<R> &A, int k, const double *C, R *Y, long long ldy,
                                 int m, long long nrows) {
+  constexpr bool M32 = kCombineF32Mfma && sizeof(R) == 4;
   const long long ntiles = (nrows + 64 * RT - 1) / (64 * RT);
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, reinterpret_cast<const void *>(&k_combine_pers<R, MF, RT, PD>), 256, lds) !=
+          &nb, reinterpret_cast<const void *>(&k_combine_pers<R, MF, RT, PD, M32>), 256, lds) !=
           hipSuccess ||
       nb < 1)
     nb = 1;
   long long gx = (long long)c->ncu * nb;
   if (gx > ntiles) gx = ntiles;
-  hipLaunchKernelGGL((k_combine_pers<R, MF, RT, PD>), dim3((unsigned)gx, gy), dim3(256), lds, s,
-                     A, k, C, Y, ldy, m, nrows, ntiles);
+  hipLaunchKernelGGL((k_combine_pers<R, MF, RT, PD, M32>), dim3((unsigned)gx, gy), dim3(256), lds,
+                     s, A, k, C, Y, ldy, m, nrows, ntiles);
 }
 
 template <typename R, int MF>
