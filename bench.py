@@ -14,15 +14,18 @@ block 32) with psi0 and all fields resident on the device.  Timed solves:
     solve measured in the same run;
   * the learned preconditioner (trained on the GPU beforehand; it replaces
     the product form's T_M T_K) in a mixed product-form solve.
-The roofline of the dominant FFT kernel is measured live with HIP events on
-the context stream (bdgx_bench_pass).  The CPU baseline times full iterations
+`roofline` is the value solve's dominant kernel by kernel time, the FP32
+update combine [X|P|W] C (96 -> 64 columns), bound by the FP64 MFMA it
+computes on; `roofline_fft` the FFT pass with the longest FP64 launch.  Both
+are timed live with HIP events on the context stream.  The CPU baseline times full iterations
 of the numpy first-order solver on the FP64 C oracle (oracle/lrep_ref.py) on
 the host cores and reports the oracle's residuals of the FP64 device modes.
 
 `--blas` times the Gram and block-combine kernels at the solver's shapes
-(FP64 MFMA utilisation).  `--probe` only launches the FP64 FFT pass kernels and a copy kernel of known
-size a few times (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE traffic runs,
-whose summary in profiles/ supplies `roofline.traffic`).
+(FP64 MFMA utilisation).  `--probe` only launches the FP64 FFT pass kernels,
+the update combine and a copy kernel of known size a few times (for
+rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE traffic runs, whose summaries in
+profiles/ supply the `traffic` of both rooflines).
 """
 from __future__ import annotations
 
@@ -173,22 +176,28 @@ def roofline(cfg):
 
 
 def probe(cfg):
-    """Launch the three FP64 FFT pass kernels (32 spectrum batches) and a
-    1 GiB device copy (k_convert<double, double>) three times each."""
+    """Launch the three FP64 FFT pass kernels (32 spectrum batches), the
+    update combine 96 -> 64 in FP32 and FP64 storage, and a 1 GiB device copy
+    (k_convert<double, double>) three times each."""
     import bdg_lobpcg_amd as B
 
     d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
     d.autotune(B.FP64, cfg["m"])
+    d.autotune(B.FP32, cfg["m"])
     info = {}
     for name in ("plane_fwd", "z", "plane_inv"):
         _, by = d.bench_pass(PASSES.index(name), B.FP64, cfg["m"], iters=3)
         info[name] = by
+    for prec in (B.FP32, B.FP64):
+        _, by, _ = combine_timing(d, cfg, prec, iters=3)
+        info[f"combine_{'fp32' if prec == B.FP32 else 'fp64'}"] = by
     n = (1 << 30) // 8
     src, dst = B.DeviceBuffer(8 * n), B.DeviceBuffer(8 * n)
     for _ in range(3):
         B.check(B.lib().bdgx_convert(d.h, B.FP64, src.ptr, n, B.FP64, dst.ptr, n, n, 1), "copy")
     d.sync()
-    print(json.dumps({"probe": "fp64 plane_fwd, z, plane_inv x3 each (+1 warm-up), copy 1 GiB x3",
+    print(json.dumps({"probe": "fp64 plane_fwd, z, plane_inv x3 each (+1 warm-up), combine 96 -> 64 "
+                               "fp32 and fp64 x3 each (+1 warm-up), copy 1 GiB x3",
                       "algorithmic_bytes": info, "copy_bytes_read": 8 * n,
                       "copy_bytes_written": 8 * n}), flush=True)
     d.close()
@@ -368,17 +377,73 @@ def batch(nprob=16, precision=1, nthreads=0):
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a
-    1 GiB copy), or None if the summary does not list it."""
-    path = os.path.join(HERE, "profiles", "traffic_fft_passes_128c.md")
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith("| `") and kernel in line:
-                    return float(line.split("|")[-2])
-    except OSError:
-        pass
+    summaries (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on
+    a 1 GiB copy), or None if no summary lists it."""
+    for name in ("traffic_fft_passes_128c.md", "traffic_combine_128c.md"):
+        try:
+            with open(os.path.join(HERE, "profiles", name)) as f:
+                for line in f:
+                    if line.startswith("| `") and f"`{kernel}`" in line:
+                        return float(line.split("|")[-2])
+        except OSError:
+            pass
     return None
+
+
+COMBINE_K, COMBINE_M = 96, 64
+
+
+def combine_timing(d, cfg, prec, iters=10):
+    """The value solve's dominant kernel: the update combine [X|P|W] C of
+    3m -> 2m columns (S rows, the FP32 phase's storage), HIP events on the
+    context stream over `iters` launches.  Returns (ms per launch,
+    algorithmic bytes, algorithmic flops)."""
+    import bdg_lobpcg_amd as B
+
+    S = int(np.prod(cfg["N"]))
+    k, m = COMBINE_K, COMBINE_M
+    dt = np.float32 if prec == B.FP32 else np.float64
+    rng = np.random.default_rng(5)
+    dA = d.upload(np.asfortranarray(rng.standard_normal((S, k)).astype(dt)))
+    dC = d.upload(np.asfortranarray(rng.standard_normal((k, m))))
+    dY = B.DeviceBuffer(S * m * np.dtype(dt).itemsize)
+
+    def run():
+        B.check(B.lib().bdgx_combine(d.h, prec, dA.ptr, S, k, dC.ptr, dY.ptr, S, m, S), "combine")
+
+    run()
+    d.timer_begin()
+    for _ in range(iters):
+        run()
+    ms = d.timer_end() / iters
+    return ms, S * (k + m) * np.dtype(dt).itemsize, 2.0 * S * k * m
+
+
+FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+
+
+def combine_roofline(cfg):
+    """Roofline of the value solve's dominant kernel (FP32 storage, FP64
+    arithmetic on the FP64 MFMA: 19 flop/B against a ridge of 9.8, so the
+    MFMA bounds it)."""
+    import bdg_lobpcg_amd as B
+
+    d = B.DeviceContext(cfg["N"], cfg["L"], cfg["cplx"])
+    v = d.autotune(B.FP32, cfg["m"])
+    ms, by, fl = combine_timing(d, cfg, B.FP32)
+    d.close()
+    mf = 4 if COMBINE_M > 48 else 3
+    pers = v["combine"] >= 2
+    rt = 2 if v["combine"] in (1, 2) else 4
+    name = (f"k_combine_pers<float, {mf}, {rt}, {8 if rt == 2 else 4}, false>" if pers
+            else f"k_combine_mfma<float, {mf}, {rt}>")
+    tf = fl / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": tf, "peak": FP64_MFMA_PEAK, "unit": "TFLOP/s",
+            "frac": tf / FP64_MFMA_PEAK, "traffic": measured_traffic(name),
+            "kernel": f"{name} (update [X|P|W] C, {COMBINE_K} -> {COMBINE_M} columns, FP32 "
+                      f"storage, FP64 MFMA)",
+            "flops_per_launch": fl, "bytes_per_launch": by, "ms_per_launch": ms,
+            "hbm_GBs": by / (ms * 1e-3) / 1e9}
 
 
 def cpu_baseline(cfg, psi, mu, r64, t64, iters=3):
@@ -540,6 +605,10 @@ def main():
         f"mixed at residual {ACC_TOL:g} {tacc:.3f} s ({t64 / tacc:.2f}x), "
         f"mixed+learned product {tlr:.3f} s")
     rf = roofline(cfg)
+    croof = combine_roofline(cfg)
+    log(f"dominant kernel {croof['kernel']}: {croof['ms_per_launch'] * 1e3:.1f} us, "
+        f"{croof['achieved']:.1f} TF/s ({croof['frac']:.2f} of FP64 MFMA peak), "
+        f"{croof['hbm_GBs']:.0f} GB/s")
     variants = {k: rf.pop(k) for k in [k for k in rf if k.endswith("_variants")]}
     log("passes: " + ", ".join(f"{k} {v['GBs']:.0f} GB/s" for k, v in rf.items()))
     log(f"autotuned launch variants: {variants}")
@@ -613,11 +682,12 @@ def main():
         "psi0_seconds": t_itp,
         "psi0": dict(itp_info, note="imaginary-time residual ||psi' - psi|| / (dt ||psi||) of "
                                     "one more step; the configuration fixes 4000 steps"),
-        "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["GBs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom_kernel),
-                     "kernel": f"{dom_kernel} ({KERNEL_ROLE.get(dom_pass, dom_pass)})",
-                     "bytes_per_launch": dom["bytes"], "ms_per_launch": dom["ms"]},
+        "roofline": croof,
+        "roofline_fft": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(dom_kernel),
+                         "kernel": f"{dom_kernel} ({KERNEL_ROLE.get(dom_pass, dom_pass)})",
+                         "bytes_per_launch": dom["bytes"], "ms_per_launch": dom["ms"]},
         "passes": {k: round(v["GBs"], 1) for k, v in rf.items()},
         "launch_variants": variants,
         "cpu_baseline": cpu,
