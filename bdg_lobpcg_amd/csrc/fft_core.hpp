@@ -128,6 +128,45 @@ struct Bfly<R, INV, 8> {
   }
 };
 
+// 16 = 4 x 4: x[4 n1 + n2] -> DFT4 over n1, twiddle w16^(n2 k1), DFT4 over
+// n2 -> X[k1 + 4 k2] (natural order out)
+template <typename R, bool INV>
+__device__ __forceinline__ cx<R> tw16(cx<R> a, R c, R s) {  // a * exp(-+ i theta)
+  return cmul(a, cmk<R>(c, INV ? s : -s));
+}
+template <typename R, bool INV>
+struct Bfly<R, INV, 16> {
+  static __device__ __forceinline__ void run(cx<R> *v, const cx<R> *, int) {
+    const R c1 = R(0.92387953251128675613), s1 = R(0.38268343236508977173);
+    const R h = R(0.70710678118654752440);
+    cx<R> a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cx<R> t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      Bfly<R, INV, 4>::run(t, nullptr, 0);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+    }
+    // w16^j for j = n2 k1 in {1, 2, 3, 4, 6, 9}: (cos, sin) of 2 pi j / 16
+    a[1][1] = tw16<R, INV>(a[1][1], c1, s1);
+    a[1][2] = tw16<R, INV>(a[1][2], h, h);
+    a[1][3] = tw16<R, INV>(a[1][3], s1, c1);
+    a[2][1] = tw16<R, INV>(a[2][1], h, h);
+    a[2][2] = rot90<R, INV>(a[2][2]);
+    a[2][3] = tw16<R, INV>(a[2][3], -h, h);
+    a[3][1] = tw16<R, INV>(a[3][1], s1, c1);
+    a[3][2] = tw16<R, INV>(a[3][2], -h, h);
+    a[3][3] = tw16<R, INV>(a[3][3], -c1, -s1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cx<R> t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Bfly<R, INV, 4>::run(t, nullptr, 0);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
 // generic odd prime radix from the n-point twiddle table
 template <typename R, bool INV>
 struct Bfly<R, INV, 7> {

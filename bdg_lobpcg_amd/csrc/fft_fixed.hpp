@@ -10,18 +10,26 @@
 
 namespace bdgx {
 
-// radix plan: as many 8s as possible, then 4, 2, 3, 5
+// radix plan: as many 8s as possible, then 4, 2, 3, 5; with r16 a leftover
+// 2 is merged with the first 8 into a 16 (128 = 16 x 8: one LDS round trip
+// fewer per transform, at 16 complex values per thread in the butterfly)
+constexpr bool kFixedRadix16 = true;
+
 struct FixedPlan {
   int nst;
   int rad[12];
   int ns[12];
 };
 
-constexpr FixedPlan make_fixed_plan(int n) {
+constexpr FixedPlan make_fixed_plan(int n, bool r16 = kFixedRadix16) {
   FixedPlan p{};
   int m = n, nf = 0;
   int f[12] = {};
   while (m % 8 == 0 && m > 1) { f[nf++] = 8; m /= 8; }
+  if (r16 && nf > 0 && m == 2) {
+    f[0] = 16;
+    m = 1;
+  }
   while (m % 4 == 0 && m > 1) { f[nf++] = 4; m /= 4; }
   while (m % 2 == 0 && m > 1) { f[nf++] = 2; m /= 2; }
   while (m % 3 == 0 && m > 1) { f[nf++] = 3; m /= 3; }

@@ -28,16 +28,16 @@ namespace bdgx {
 
 // In-place mixed-radix transforms over NP pencils, element i of pencil p at
 // buf[p*PST + i*EST].  Decimation in frequency (DIF) takes natural order and
-// leaves X[k] at position dif_pos<N>(k) (digits of k in the plan's radices,
+// leaves X[k] at position dif_pos<N, R16>(k) (digits of k in the plan's radices,
 // first radix least significant, reversed); decimation in time (DIT) takes
 // that order and returns natural order.  Every butterfly reads and writes the
 // same RAD positions, so a stage needs no staging buffer and only one barrier;
 // each thread keeps one butterfly in registers at a time.  PFAST maps
 // consecutive threads to consecutive pencils (pencils adjacent in memory),
 // otherwise to consecutive butterflies of one pencil.
-template <int N>
+template <int N, bool R16>
 __device__ __forceinline__ int dif_pos(int k) {
-  constexpr FixedPlan pl = make_fixed_plan(N);
+  constexpr FixedPlan pl = make_fixed_plan(N, R16);
   int p = 0, span = N;
 #pragma unroll
   for (int s = 0; s < pl.nst; ++s) {
@@ -112,29 +112,29 @@ __device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw, int tid) {
 
 // DIF: stages S = 0 .. nst-1 with block length L = N / ns[S]
 template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int SWZ,
-          int S>
+          int S, bool R16>
 __device__ __forceinline__ void dif_fft(cx<R> *buf, const cx<R> *tw, int tid) {
-  constexpr FixedPlan pl = make_fixed_plan(N);
+  constexpr FixedPlan pl = make_fixed_plan(N, R16);
   if constexpr (S < pl.nst) {
     ip_stage<R, INV, false, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST, SWZ>(buf, tw, tid);
-    dif_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S + 1>(buf, tw, tid);
+    dif_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S + 1, R16>(buf, tw, tid);
   }
 }
 
 // DIT: the same stages in reverse order (S = nst-1 .. 0)
 template <typename R, bool INV, int N, int NP, int PST, int EST, int T, bool PFAST, int SWZ,
-          int S>
+          int S, bool R16>
 __device__ __forceinline__ void dit_fft(cx<R> *buf, const cx<R> *tw, int tid) {
-  constexpr FixedPlan pl = make_fixed_plan(N);
+  constexpr FixedPlan pl = make_fixed_plan(N, R16);
   if constexpr (S >= 0) {
     ip_stage<R, INV, true, N, pl.rad[S], N / pl.ns[S], NP, PST, EST, T, PFAST, SWZ>(buf, tw, tid);
-    dit_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S - 1>(buf, tw, tid);
+    dit_fft<R, INV, N, NP, PST, EST, T, PFAST, SWZ, S - 1, R16>(buf, tw, tid);
   }
 }
 
-template <int N>
+template <int N, bool R16>
 constexpr int last_stage() {
-  return make_fixed_plan(N).nst - 1;
+  return make_fixed_plan(N, R16).nst - 1;
 }
 
 // rows handled per round of the in-place untangle / repack: the largest
@@ -148,13 +148,13 @@ constexpr int group_rows(int row_len) {
 
 // x transform, untangle, y transform of the weighted plane of channel c held
 // in LDS (complex rows of row pairs, swizzled), then the spectrum plane out
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool R16 = kFixedRadix16>
 __device__ __forceinline__ void plane_fwd_transform(const XArgs<R> &a, cx<R> *buf,
                                                     const cx<R> *twx, const cx<R> *twy, int c,
                                                     int col, int z, int tid) {
   constexpr int NK = N / 2 + 1;
   constexpr int SW = (int)sizeof(cx<R>);
-  dif_fft<R, false, N, N / 2, N, 1, T, false, SW, 0>(buf, twx, tid);
+  dif_fft<R, false, N, N / 2, N, 1, T, false, SW, 0, R16>(buf, twx, tid);
   // untangle Z_q (digit-reversed in k) into the half spectra of rows 2q,
   // 2q+1 (natural kx, row stride NK); rounds of G rows from the top down,
   // so that no row is overwritten before it is read
@@ -168,8 +168,8 @@ __device__ __forceinline__ void plane_fwd_transform(const XArgs<R> &a, cx<R> *bu
       const int t = tid + b * T;
       if (NU % T == 0 || t < NU) {
         const int q = q0 + t / NK, k = t % NK;
-        const cx<R> zk = buf[swz<SW>(q * N + dif_pos<N>(k))];
-        const cx<R> zn = buf[swz<SW>(q * N + dif_pos<N>(k == 0 ? 0 : N - k))];
+        const cx<R> zk = buf[swz<SW>(q * N + dif_pos<N, R16>(k))];
+        const cx<R> zn = buf[swz<SW>(q * N + dif_pos<N, R16>(k == 0 ? 0 : N - k))];
         ua[b] = cmk<R>(R(0.5) * (zk.re + zn.re), R(0.5) * (zk.im - zn.im));
         ub[b] = cmk<R>(R(0.5) * (zk.im + zn.im), R(-0.5) * (zk.re - zn.re));
       }
@@ -187,13 +187,13 @@ __device__ __forceinline__ void plane_fwd_transform(const XArgs<R> &a, cx<R> *bu
     __syncthreads();
   }
   // y transform over the NK adjacent pencils; rows come out digit-reversed
-  dif_fft<R, false, N, NK, 1, NK, T, true, 0, 0>(buf, twy, tid);
+  dif_fft<R, false, N, NK, 1, NK, T, true, 0, 0, R16>(buf, twy, tid);
   cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride + (long long)col * a.g.Kk +
               (long long)z * N * NK;
   if constexpr (sizeof(R) == 8 || T > 4 * N) {  // (no registers to spare)
     for (int e = tid; e < N * NK; e += T) {
       const int y = e / NK, k = e % NK;
-      sp[e] = buf[dif_pos<N>(y) * NK + k];
+      sp[e] = buf[dif_pos<N, R16>(y) * NK + k];
     }
     __syncthreads();
     return;
@@ -209,7 +209,7 @@ __device__ __forceinline__ void plane_fwd_transform(const XArgs<R> &a, cx<R> *bu
       const int b = b0 + i, e = tid + b * T;
       if (b < ES && (N * NK % T == 0 || b < ES - 1 || e < N * NK)) {
         const int y = e / NK, k = e % NK;
-        v[i] = buf[dif_pos<N>(y) * NK + k];
+        v[i] = buf[dif_pos<N, R16>(y) * NK + k];
       }
     }
 #pragma unroll
@@ -236,6 +236,15 @@ constexpr int plane_waves_per_simd() {
 template <typename R, int N, int T>
 constexpr int plane_load_group() {
   return plane_two_per_cu<R, N, T>() ? 2 : 4;
+}
+// Radix-16 stages in the inverse kernels except the FP64 persistent one,
+// which holds the next plane's spectrum and the output rows: there radix 16
+// spills about 100 VGPRs and runs 16% slower (the FP32 kernel held to 128
+// VGPRs spills 25 and still gains 3%).  The radix plan only orders values
+// inside one kernel's LDS; HBM planes are natural order either way.
+template <typename R, int N, int T, bool PERSIST>
+constexpr bool plane_inv_r16() {
+  return kFixedRadix16 && !(PERSIST && sizeof(R) == 8);
 }
 
 // One plane per workgroup, nothing held across the transform (fewer
@@ -377,14 +386,14 @@ __global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes
 
 // y inverse, repack, x inverse of channel c's spectrum plane held in LDS
 // (rows digit-reversed), then acc += post-weighted real rows
-template <typename R, int N, int T, int EPT, int PGMAX>
+template <typename R, int N, int T, int EPT, int PGMAX, bool R16>
 __device__ __forceinline__ void plane_inv_transform(const XArgs<R> &a, cx<R> *buf,
                                                     const cx<R> *twx, const cx<R> *twy, int c,
                                                     long long plane0, R (&acc0)[EPT],
                                                     R (&acc1)[EPT], int tid) {
   constexpr int NK = N / 2 + 1;
   constexpr int SW = (int)sizeof(cx<R>);
-  dit_fft<R, true, N, NK, 1, NK, T, true, 0, last_stage<N>()>(buf, twy, tid);
+  dit_fft<R, true, N, NK, 1, NK, T, true, 0, last_stage<N, R16>(), R16>(buf, twy, tid);
   // repack: complex row q = spectrum(row 2q) + i spectrum(row 2q+1),
   // extended to all kx by Hermitian symmetry, digit-reversed in x; rounds
   // of G rows from the bottom up
@@ -418,12 +427,12 @@ __device__ __forceinline__ void plane_inv_transform(const XArgs<R> &a, cx<R> *bu
       const int t = tid + b * T;
       if (NQ % T == 0 || t < NQ) {
         const int q = q0 + t / N, k = t % N;
-        buf[swz<SW>(q * N + dif_pos<N>(k))] = zq[b];
+        buf[swz<SW>(q * N + dif_pos<N, R16>(k))] = zq[b];
       }
     }
     __syncthreads();
   }
-  dit_fft<R, true, N, N / 2, N, 1, T, false, SW, last_stage<N>()>(buf, twx, tid);
+  dit_fft<R, true, N, N / 2, N, 1, T, false, SW, last_stage<N, R16>(), R16>(buf, twx, tid);
   const bool weighted = a.post[c] != POST_ONE;
   const R ps = a.posts[c];
   const R *pw = a.postw[c] + plane0;
@@ -581,12 +590,13 @@ __global__ void __launch_bounds__(T, (plane_waves_per_simd<R, N, T>())) k_plane_
         const int e = tid + (b0 + i) * T;
         if (b0 + i < ES && (N * NK % T == 0 || b0 + i < ES - 1 || e < N * NK)) {
           const int y = e / NK, k = e % NK;
-          buf[dif_pos<N>(y) * NK + k] = v[i];
+          buf[dif_pos<N, plane_inv_r16<R, N, T, false>()>(y) * NK + k] = v[i];
         }
       }
     }
     __syncthreads();
-    plane_inv_transform<R, N, T, EPT, LG>(a, buf, twx, twy, c, plane0, acc0, acc1, tid);
+    plane_inv_transform<R, N, T, EPT, LG, plane_inv_r16<R, N, T, false>()>(a, buf, twx, twy, c,
+                                                                           plane0, acc0, acc1, tid);
   }
   plane_inv_store<R, N, T, EPT, LG>(a, col, plane0, acc0, acc1, threadIdx.x);
 }
@@ -661,7 +671,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
         const int e = tid + b * T;
         if (N * NK % T == 0 || b < ES - 1 || e < N * NK) {
           const int y = e / NK, k = e % NK;
-          buf[dif_pos<N>(y) * NK + k] = cmk<R>(sre[b], sim[b]);
+          buf[dif_pos<N, plane_inv_r16<R, N, T, true>()>(y) * NK + k] = cmk<R>(sre[b], sim[b]);
         }
       }
       __syncthreads();
@@ -680,7 +690,8 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
           }
         }
       }
-      plane_inv_transform<R, N, T, EPT, WG>(a, buf, twx, twy, c, plane0, acc0, acc1, tid);
+      plane_inv_transform<R, N, T, EPT, WG, plane_inv_r16<R, N, T, true>()>(a, buf, twx, twy, c,
+                                                                          plane0, acc0, acc1, tid);
     }
     plane_inv_store<R, N, T, EPT, SG>(a, col, plane0, acc0, acc1, threadIdx.x);
   }
