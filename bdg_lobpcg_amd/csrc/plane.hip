@@ -72,6 +72,16 @@ __device__ __forceinline__ int swz(int e) {
   }
 }
 
+// Swizzle of the x layout of an N-point plane: the permutation stays inside
+// its chunk (16 FP64 / 32 FP32 values), so a row must be whole chunks for
+// the in-place untangle and repack rounds (which rewrite rows in place) to
+// touch disjoint LDS ranges; other row lengths (N = 48, 80 in FP32) run
+// unswizzled.
+template <typename R, int N>
+constexpr int plane_swz() {
+  return N % (sizeof(R) == 8 ? 16 : 32) == 0 ? (int)sizeof(cx<R>) : 0;
+}
+
 template <typename R, bool INV, bool DIT, int N, int RAD, int L, int NP, int PST, int EST,
           int T, bool PFAST, int SWZ>
 __device__ __forceinline__ void ip_stage(cx<R> *buf, const cx<R> *tw, int tid) {
@@ -138,12 +148,19 @@ constexpr int last_stage() {
 }
 
 // rows handled per round of the in-place untangle / repack: the largest
-// power of two that keeps at most `per` staged values per thread
+// power of two dividing N/2 that keeps at most `per` staged values per
+// thread (the rounds cover the N/2 row pairs exactly)
 template <int N, int T, int per>
 constexpr int group_rows(int row_len) {
   int p = 1;
-  while (2 * p <= N / 2 && 2 * p * row_len <= per * T) p *= 2;
+  while ((N / 2) % (2 * p) == 0 && 2 * p * row_len <= per * T) p *= 2;
   return p;
+}
+
+// the largest group size <= g that divides n (loads issued in whole groups)
+constexpr int div_group(int n, int g) {
+  while (g > 1 && n % g) --g;
+  return g;
 }
 
 // x transform, untangle, y transform of the weighted plane of channel c held
@@ -153,7 +170,7 @@ __device__ __forceinline__ void plane_fwd_transform(const XArgs<R> &a, cx<R> *bu
                                                     const cx<R> *twx, const cx<R> *twy, int c,
                                                     int col, int z, int tid) {
   constexpr int NK = N / 2 + 1;
-  constexpr int SW = (int)sizeof(cx<R>);
+  constexpr int SW = plane_swz<R, N>();
   dif_fft<R, false, N, N / 2, N, 1, T, false, SW, 0, R16>(buf, twx, tid);
   // untangle Z_q (digit-reversed in k) into the half spectra of rows 2q,
   // 2q+1 (natural kx, row stride NK); rounds of G rows from the top down,
@@ -253,10 +270,10 @@ constexpr bool plane_inv_r16() {
 // flight together.
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
-  constexpr int SW = (int)sizeof(cx<R>);
+  constexpr int SW = plane_swz<R, N>();
   constexpr int EP = N * N / 2 / T;  // row-pair values per thread
   static_assert(EP * T == N * N / 2, "whole row pairs per thread");
-  constexpr int LG = EP < plane_load_group<R, N, T>() ? EP : plane_load_group<R, N, T>();
+  constexpr int LG = div_group(EP, plane_load_group<R, N, T>());
   static_assert(EP % LG == 0, "load groups");
   __shared__ cx<R> twx[N];
   __shared__ cx<R> twy[N];
@@ -322,7 +339,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
 template <typename R, int N, int T>
 __global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes) {
   constexpr int NK = N / 2 + 1;
-  constexpr int SW = (int)sizeof(cx<R>);
+  constexpr int SW = plane_swz<R, N>();
   constexpr int EP = N * N / 2 / T;  // row-pair values per thread
   static_assert(EP * T == N * N / 2, "whole row pairs per thread");
   __shared__ cx<R> twx[N];
@@ -392,7 +409,7 @@ __device__ __forceinline__ void plane_inv_transform(const XArgs<R> &a, cx<R> *bu
                                                     long long plane0, R (&acc0)[EPT],
                                                     R (&acc1)[EPT], int tid) {
   constexpr int NK = N / 2 + 1;
-  constexpr int SW = (int)sizeof(cx<R>);
+  constexpr int SW = plane_swz<R, N>();
   dit_fft<R, true, N, NK, 1, NK, T, true, 0, last_stage<N, R16>(), R16>(buf, twy, tid);
   // repack: complex row q = spectrum(row 2q) + i spectrum(row 2q+1),
   // extended to all kx by Hermitian symmetry, digit-reversed in x; rounds
@@ -439,7 +456,7 @@ __device__ __forceinline__ void plane_inv_transform(const XArgs<R> &a, cx<R> *bu
   if constexpr (PGMAX > 1) {
     // weights requested in groups of PG values per thread, a group's loads
     // all in flight before the first is used
-    constexpr int PG = EPT < PGMAX ? EPT : PGMAX;
+    constexpr int PG = div_group(EPT, PGMAX);
     static_assert(EPT % PG == 0, "weight groups");
 #pragma unroll
     for (int b0 = 0; b0 < EPT; b0 += PG) {
@@ -497,7 +514,7 @@ template <typename R, int N, int T, int EPT, int SGMAX>
 __device__ __forceinline__ void plane_inv_store(const XArgs<R> &a, int col, long long plane0,
                                                 const R (&acc0)[EPT], const R (&acc1)[EPT],
                                                 int tid) {
-  constexpr int SG = EPT < SGMAX ? EPT : SGMAX;
+  constexpr int SG = div_group(EPT, SGMAX);
   static_assert(EPT % SG == 0, "store groups");
   const R *xp = a.X + (long long)col * a.ld + (long long)a.in_half * a.g.S + plane0;
   R *yp = a.Y + (long long)col * a.ld + (long long)a.out_half * a.g.S + plane0;
@@ -829,7 +846,7 @@ __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
 template <typename R>
 bool zpass_fixed_supported(const bdgx_ctx *c) {
   const int nz = c->geo.nz;
-  return nz == 32 || nz == 64 || nz == 128;
+  return nz == 32 || nz == 48 || nz == 64 || nz == 80 || nz == 96 || nz == 128 || nz == 192;
 }
 
 // Launch shapes of the fixed z pass: variant 0 = (P, T) with T = 256 at
@@ -858,9 +875,25 @@ int launch_zpass_fixed(bdgx_ctx *c, const YZArgs<R> &z) {
       if (v) launch_z_nt<R, 64, 64>(z, grid, c->stream);
       else launch_z_nt<R, 64, 128>(z, grid, c->stream);
       break;
+    case 48:
+      if (v) launch_z_nt<R, 48, 64>(z, grid, c->stream);
+      else launch_z_nt<R, 48, 128>(z, grid, c->stream);
+      break;
+    case 80:
+      if (v) launch_z_nt<R, 80, 64>(z, grid, c->stream);
+      else launch_z_nt<R, 80, 128>(z, grid, c->stream);
+      break;
+    case 96:
+      if (v) launch_z_nt<R, 96, 64>(z, grid, c->stream);
+      else launch_z_nt<R, 96, 128>(z, grid, c->stream);
+      break;
     case 128:
       if (v) launch_z_nt<R, 128, 128>(z, grid, c->stream);
       else launch_z_nt<R, 128, 256>(z, grid, c->stream);
+      break;
+    case 192:
+      if (v) launch_z_nt<R, 192, 128>(z, grid, c->stream);
+      else launch_z_nt<R, 192, 256>(z, grid, c->stream);
       break;
     default: return BDGX_EINVAL;
   }
@@ -872,6 +905,220 @@ template bool zpass_fixed_supported<float>(const bdgx_ctx *);
 template bool zpass_fixed_supported<double>(const bdgx_ctx *);
 template int launch_zpass_fixed<float>(bdgx_ctx *, const YZArgs<float> &);
 template int launch_zpass_fixed<double>(bdgx_ctx *, const YZArgs<double> &);
+
+// ---------------------------------------------------------------------------
+// Whole-plane x/y transforms for complex vectors (c2c, Nx = Ny = N).  One
+// workgroup owns one z-plane of one column; the N x N complex plane sits in
+// LDS in the swizzled row-major x layout (147 KB at N = 96 FP64).
+//
+//   forward  read the complex plane into registers once; per channel:
+//            pre-weight (real weight, conj(psi0), |x|^2), x transform of the
+//            N rows, y transform of the N columns (both DIF, in place), write
+//            the spectrum plane spec[y][kx] in natural order;
+//   inverse  per channel: read the spectrum plane into digit-reversed
+//            positions, y inverse of the columns, x inverse of the rows (DIT,
+//            in place), accumulate the post-weighted values (posts, real
+//            weight, psi0) in registers; then add lt * x and write the output
+//            plane (complex, or its real part).
+//
+// The formulas per channel are those of the pencil kernels k_xfwd_cplx /
+// k_xinv_cplx; only the order of the FFT work differs.
+template <typename R, int N>
+constexpr bool cplane_fits() {
+  return (size_t)N * N * sizeof(cx<R>) + 2 * N * sizeof(cx<R>) <= 160 * 1024;
+}
+
+template <typename R, int N, int T>
+__global__ void __launch_bounds__(T) k_cplane_fwd(XArgs<R> a) {
+  constexpr int SW = plane_swz<R, N>();
+  constexpr bool R16 = kFixedRadix16;
+  constexpr int EP = N * N / T;  // plane values per thread
+  static_assert(EP * T == N * N, "whole plane per thread set");
+  __shared__ cx<R> twx[N];
+  __shared__ cx<R> twy[N];
+  __shared__ cx<R> buf[N * N];
+  const int tid = threadIdx.x;
+  const int nz = a.g.nz;
+  const int z = blockIdx.x % nz;
+  const int col = blockIdx.x / nz;
+  for (int i = tid; i < N; i += T) {
+    twx[i] = a.tw[i];
+    twy[i] = a.tw_y[i];
+  }
+  const long long plane0 = (long long)z * N * N;
+  const cx<R> *xp = reinterpret_cast<const cx<R> *>(a.X) + (long long)col * a.ld +
+                    (long long)a.in_half * a.g.S + plane0;
+  cx<R> xv[EP];
+#pragma unroll
+  for (int b = 0; b < EP; ++b) xv[b] = xp[tid + b * T];
+  for (int c = 0; c < a.nch; ++c) {
+    const int pre = a.pre[c];
+    if (pre == PRE_REAL) {
+      const R *pw = a.prew[c] + plane0;
+      R q[EP];
+#pragma unroll
+      for (int b = 0; b < EP; ++b) q[b] = pw[tid + b * T];
+#pragma unroll
+      for (int b = 0; b < EP; ++b) buf[swz<SW>(tid + b * T)] = cscale(xv[b], q[b]);
+    } else if (pre == PRE_CONJ_PSI) {
+      const cx<R> *pw = reinterpret_cast<const cx<R> *>(a.prew[c]) + plane0;
+      constexpr int LG = div_group(EP, 4);
+#pragma unroll
+      for (int b0 = 0; b0 < EP; b0 += LG) {
+        cx<R> q[LG];
+#pragma unroll
+        for (int i = 0; i < LG; ++i) q[i] = pw[tid + (b0 + i) * T];
+#pragma unroll
+        for (int i = 0; i < LG; ++i)
+          buf[swz<SW>(tid + (b0 + i) * T)] = cmul(cconj(q[i]), xv[b0 + i]);
+      }
+    } else if (pre == PRE_ABS2) {
+#pragma unroll
+      for (int b = 0; b < EP; ++b)
+        buf[swz<SW>(tid + b * T)] = cmk<R>(xv[b].re * xv[b].re + xv[b].im * xv[b].im, R(0));
+    } else {
+#pragma unroll
+      for (int b = 0; b < EP; ++b) buf[swz<SW>(tid + b * T)] = xv[b];
+    }
+    __syncthreads();
+    // x transform of the N rows, then y transform of the N columns (pencil
+    // p = x position, adjacent pencils adjacent in LDS)
+    dif_fft<R, false, N, N, N, 1, T, false, SW, 0, R16>(buf, twx, tid);
+    dif_fft<R, false, N, N, 1, N, T, true, SW, 0, R16>(buf, twy, tid);
+    cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride + (long long)col * a.g.Kk +
+                plane0;
+    constexpr int SG = div_group(EP, 4);
+#pragma unroll
+    for (int b0 = 0; b0 < EP; b0 += SG) {
+      cx<R> v[SG];
+#pragma unroll
+      for (int i = 0; i < SG; ++i) {
+        const int e = tid + (b0 + i) * T;
+        const int y = e / N, k = e % N;
+        v[i] = buf[swz<SW>(dif_pos<N, R16>(y) * N + dif_pos<N, R16>(k))];
+      }
+#pragma unroll
+      for (int i = 0; i < SG; ++i) sp[tid + (b0 + i) * T] = v[i];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R, int N, int T>
+__global__ void __launch_bounds__(T) k_cplane_inv(XArgs<R> a) {
+  constexpr int SW = plane_swz<R, N>();
+  constexpr bool R16 = kFixedRadix16;
+  constexpr int EP = N * N / T;
+  static_assert(EP * T == N * N, "whole plane per thread set");
+  constexpr int LG = div_group(EP, 4);
+  __shared__ cx<R> twx[N];
+  __shared__ cx<R> twy[N];
+  __shared__ cx<R> buf[N * N];
+  const int nz = a.g.nz;
+  const int z = blockIdx.x % nz;
+  const int col = blockIdx.x / nz;
+  for (int i = threadIdx.x; i < N; i += T) {
+    twx[i] = a.tw[i];
+    twy[i] = a.tw_y[i];
+  }
+  const long long plane0 = (long long)z * N * N;
+  cx<R> acc[EP];
+#pragma unroll
+  for (int b = 0; b < EP; ++b) acc[b] = cmk<R>(0, 0);
+  for (int c = 0; c < a.nch; ++c) {
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const cx<R> *sp = a.spec + (long long)(a.chan_base + c) * a.chan_stride +
+                      (long long)col * a.g.Kk + plane0;
+#pragma unroll
+    for (int b0 = 0; b0 < EP; b0 += LG) {
+      cx<R> v[LG];
+#pragma unroll
+      for (int i = 0; i < LG; ++i) v[i] = sp[tid + (b0 + i) * T];
+#pragma unroll
+      for (int i = 0; i < LG; ++i) {
+        const int e = tid + (b0 + i) * T;
+        const int y = e / N, k = e % N;
+        buf[swz<SW>(dif_pos<N, R16>(y) * N + dif_pos<N, R16>(k))] = v[i];
+      }
+    }
+    __syncthreads();
+    dit_fft<R, true, N, N, 1, N, T, true, SW, last_stage<N, R16>(), R16>(buf, twy, tid);
+    dit_fft<R, true, N, N, N, 1, T, false, SW, last_stage<N, R16>(), R16>(buf, twx, tid);
+    const int post = a.post[c];
+    const R ps = a.posts[c];
+#pragma unroll
+    for (int b0 = 0; b0 < EP; b0 += LG) {
+      cx<R> w[LG];
+      if (post == POST_REAL) {
+        const R *pw = a.postw[c] + plane0;
+#pragma unroll
+        for (int i = 0; i < LG; ++i) w[i] = cmk<R>(ps * pw[tid + (b0 + i) * T], R(0));
+      } else if (post == POST_PSI) {
+        const cx<R> *pw = reinterpret_cast<const cx<R> *>(a.postw[c]) + plane0;
+#pragma unroll
+        for (int i = 0; i < LG; ++i) w[i] = cscale(pw[tid + (b0 + i) * T], ps);
+      } else {
+#pragma unroll
+        for (int i = 0; i < LG; ++i) w[i] = cmk<R>(ps, R(0));
+      }
+#pragma unroll
+      for (int i = 0; i < LG; ++i) {
+        const cx<R> zv = buf[swz<SW>(tid + (b0 + i) * T)];
+        acc[b0 + i] = acc[b0 + i] + (post == POST_PSI ? cmul(zv, w[i]) : cscale(zv, w[i].re));
+      }
+    }
+    __syncthreads();
+  }
+  const int tid = threadIdx.x;
+  const long long obase = (long long)col * a.ld + (long long)a.out_half * a.g.S + plane0;
+  if (a.lt) {
+    const cx<R> *xp = reinterpret_cast<const cx<R> *>(a.X) + (long long)col * a.ld +
+                      (long long)a.in_half * a.g.S + plane0;
+    const R *lt = a.lt + plane0;
+#pragma unroll
+    for (int b0 = 0; b0 < EP; b0 += LG) {
+      cx<R> u[LG];
+      R l[LG];
+#pragma unroll
+      for (int i = 0; i < LG; ++i) {
+        u[i] = xp[tid + (b0 + i) * T];
+        l[i] = lt[tid + (b0 + i) * T];
+      }
+#pragma unroll
+      for (int i = 0; i < LG; ++i) acc[b0 + i] = acc[b0 + i] + cscale(u[i], l[i]);
+    }
+  }
+  if (a.out_real) {
+#pragma unroll
+    for (int b = 0; b < EP; ++b) a.Y[obase + tid + b * T] = acc[b].re;
+  } else {
+    cx<R> *yp = reinterpret_cast<cx<R> *>(a.Y) + obase;
+#pragma unroll
+    for (int b = 0; b < EP; ++b) yp[tid + b * T] = acc[b];
+  }
+}
+
+template <typename R, int N, int T>
+static void launch_cplane(const XArgs<R> &a, bool inverse, hipStream_t s) {
+  const unsigned grid = (unsigned)(a.g.nz * a.ncols);
+  if (inverse)
+    hipLaunchKernelGGL((k_cplane_inv<R, N, T>), dim3(grid), dim3(T), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_cplane_fwd<R, N, T>), dim3(grid), dim3(T), 0, s, a);
+}
+
+// complex variants: even = 4N threads, odd = 8N threads (no persistent
+// forms; variants 2 / 3 repeat 0 / 1)
+template <typename R, int N>
+static void launch_cplane_v(const XArgs<R> &a, bool inverse, int v, hipStream_t s) {
+  if constexpr (cplane_fits<R, N>()) {
+    if (v & 1)
+      launch_cplane<R, N, 8 * N>(a, inverse, s);
+    else
+      launch_cplane<R, N, 4 * N>(a, inverse, s);
+  }
+}
 
 template <typename R, int N, int T>
 static void launch_plane(const XArgs<R> &a, bool inverse, hipStream_t s) {
@@ -885,8 +1132,11 @@ static void launch_plane(const XArgs<R> &a, bool inverse, hipStream_t s) {
 template <typename R>
 bool plane_supported(const bdgx_ctx *c) {
   const Geo &g = c->geo;
-  if (c->cplx || c->dim != 3 || g.nx != g.ny) return false;
-  return g.nx == 32 || g.nx == 64 || g.nx == 128;
+  if (c->dim != 3 || g.nx != g.ny) return false;
+  if (c->cplx)  // the complex plane in LDS: N = 128 only in FP32
+    return g.nx == 32 || g.nx == 48 || g.nx == 64 || g.nx == 80 || g.nx == 96 ||
+           (g.nx == 128 && sizeof(R) == 4);
+  return g.nx == 32 || g.nx == 48 || g.nx == 64 || g.nx == 80 || g.nx == 96 || g.nx == 128;
 }
 
 // Launch shapes of the whole-plane kernels: variants 0 / 1 = the one-plane
@@ -937,9 +1187,26 @@ int launch_plane_x(bdgx_ctx *c, const XArgs<R> &a, bool inverse) {
     return BDGX_EINVAL;
   const int v = c->tune_var[sizeof(R) == 8 ? 0 : 1][inverse ? BDGX_TUNE_PLANE_INV
                                                               : BDGX_TUNE_PLANE_FWD];
+  if (c->cplx) {
+    if (!plane_supported<R>(c)) return BDGX_EINVAL;
+    switch (c->geo.nx) {
+      case 32: launch_cplane_v<R, 32>(a, inverse, v, c->stream); break;
+      case 48: launch_cplane_v<R, 48>(a, inverse, v, c->stream); break;
+      case 64: launch_cplane_v<R, 64>(a, inverse, v, c->stream); break;
+      case 80: launch_cplane_v<R, 80>(a, inverse, v, c->stream); break;
+      case 96: launch_cplane_v<R, 96>(a, inverse, v, c->stream); break;
+      case 128: launch_cplane_v<R, 128>(a, inverse, v, c->stream); break;
+      default: return BDGX_EINVAL;
+    }
+    BDGX_HIP(hipGetLastError());
+    return BDGX_OK;
+  }
   switch (c->geo.nx) {
     case 32: launch_plane_v<R, 32>(c, a, inverse, v, c->stream); break;
+    case 48: launch_plane_v<R, 48>(c, a, inverse, v, c->stream); break;
     case 64: launch_plane_v<R, 64>(c, a, inverse, v, c->stream); break;
+    case 80: launch_plane_v<R, 80>(c, a, inverse, v, c->stream); break;
+    case 96: launch_plane_v<R, 96>(c, a, inverse, v, c->stream); break;
     case 128: launch_plane_v<R, 128>(c, a, inverse, v, c->stream); break;
     default: return BDGX_EINVAL;
   }

@@ -20,6 +20,10 @@ GRIDS = [
     ((16, 12, 32), (12.0, 10.0, 16.0)),    # fixed-size z pass: Nyquist-column tiles
     ((32, 32, 64), (16.0, 16.0, 24.0)),    # plane kernels + fixed z pass
     ((24, 8, 128), (12.0, 8.0, 30.0)),     # fixed z pass, partial kx tiles
+    ((48, 48, 80), (16.0, 16.0, 20.0)),    # plane and fixed z kernels, radix 3 / 5 sizes
+    ((80, 80, 48), (18.0, 18.0, 12.0)),
+    ((96, 96, 8), (20.0, 20.0, 8.0)),
+    ((12, 10, 192), (10.0, 9.0, 60.0)),    # fixed z pass, Nz = 192
 ]
 OPS_HALF = ["kinetic", "K", "M", "precondK", "precondM", "dipolar_conv"]
 OPS_FULL = ["Lrep", "Swap", "precondLrep"]
@@ -98,7 +102,10 @@ def test_gram_combine(cplx, prec):
 
 @pytest.mark.parametrize("grid", [((32, 32, 64), (16.0, 16.0, 24.0)),
                                   ((16, 12, 32), (12.0, 10.0, 16.0)),
-                                  ((128, 128, 4), (24.0, 24.0, 4.0))],
+                                  ((128, 128, 4), (24.0, 24.0, 4.0)),
+                                  ((48, 48, 80), (16.0, 16.0, 20.0)),
+                                  ((80, 80, 48), (18.0, 18.0, 12.0)),
+                                  ((96, 96, 8), (20.0, 20.0, 8.0))],
                          ids=lambda g: "x".join(map(str, g[0])))
 def test_launch_variants_and_autotune(grid):
     """Every launch variant of the plane and fixed z kernels gives the
@@ -335,3 +342,24 @@ def test_combine_cols_writes_only_listed_columns(cplx):
     assert lib.bdgx_combine_cols(d.h, B.FP32, buf.ptr, 8, 4, buf.ptr, buf.ptr, 8, cols, 2, 8) != 0
     assert lib.bdgx_combine_cols(d.h, B.FP64, buf.ptr, 8, 4, buf.ptr, buf.ptr, 8, cols, 9, 8) != 0
     assert lib.bdgx_combine_cols(d.h, B.FP64, buf.ptr, 8, 4, buf.ptr, buf.ptr, 8, dup, 2, 8) != 0
+
+
+@pytest.mark.parametrize("grid", [((32, 32, 16), (16.0, 16.0, 10.0)),
+                                  ((96, 96, 8), (20.0, 20.0, 8.0))],
+                         ids=lambda g: "x".join(map(str, g[0])))
+def test_complex_plane_launch_variants(grid):
+    """Both launch shapes (4N and 8N threads) of the complex whole-plane
+    kernels, forward and inverse, give the oracle's result for every channel
+    kind (real weight, conj(psi0) / psi0 weights, dipolar two-channel M)."""
+    N, L = grid
+    c, f = physics(N, L, True, dipolar=True)
+    d = device_twin(N, L, True, f)
+    X = rand_block(c.size, 3, True, seed=7)
+    refs = {op: c.apply(op, X) for op in ("K", "M", "precondK", "dipolar_conv")}
+    for prec in (0, 1):
+        for kind in (0, 1):
+            for v in (0, 1):
+                d.set_launch_variant(kind, prec, v)
+                for op, ref in refs.items():
+                    e = relerr(d.apply(op, X, prec), ref)
+                    assert e < TOL[prec], f"kind {kind} variant {v} prec {prec} {op}: {e:.2e}"
