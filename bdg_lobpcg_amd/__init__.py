@@ -143,6 +143,9 @@ def lib():
         "bdg_modes_u": (P, [P]),
         "bdg_modes_v": (P, [P]),
         "bdg_load_wavefunction": (I, [P, C.c_char_p]),
+        "bdg_save_wavefunction_bin": (I, [P, C.c_char_p]),
+        "bdg_load_wavefunction_bin": (I, [P, C.c_char_p]),
+        "bdg_write_modes_bin": (I, [P, C.c_char_p]),
         "bdg_write_eigenvalues": (I, [P, C.c_char_p]),
         "bdg_write_mode_u": (I, [P, U64, C.c_char_p]),
         "bdg_write_mode_v": (I, [P, U64, C.c_char_p]),
@@ -153,6 +156,7 @@ def lib():
         "bdg_set_fp32_phase": (None, [P, D, U64]),
         "bdg_set_locking": (None, [P, I]),
         "bdg_set_lock_guards": (None, [P, I]),
+        "bdg_set_device_sweeps": (None, [P, I]),
         "bdg_residuals": (C.POINTER(D), [P]),
         "bdg_zero_modes": (C.POINTER(I), [P]),
         "bdg_device_context": (P, [P]),
@@ -495,6 +499,11 @@ class Problem:
     def __del__(self):
         self.close()
 
+    def reset(self):
+        """bdg_reset: physics cleared, grid / device context / start-block
+        strategy (and a reuse block) kept."""
+        lib().bdg_reset(self.h)
+
     # setup, in the reference's order
     def set_system(self):
         lib().bdg_set_system(self.h)
@@ -571,6 +580,23 @@ class Problem:
         """Guard columns carried by the small block after locking (first-order
         form): Ritz pairs just beyond the block, iterated, never returned."""
         lib().bdg_set_lock_guards(self.h, int(guards))
+
+    def set_device_sweeps(self, on: bool):
+        """Keep the modes on the device after a solve so that reuse_modes
+        builds the next start block there (default on)."""
+        lib().bdg_set_device_sweeps(self.h, int(bool(on)))
+
+    def reuse_modes(self, noise_frac: float) -> int:
+        return int(lib().bdg_reuse_modes(self.h, float(noise_frac)))
+
+    def save_wavefunction_bin(self, path) -> int:
+        return int(lib().bdg_save_wavefunction_bin(self.h, str(path).encode()))
+
+    def load_wavefunction_bin(self, path) -> int:
+        return int(lib().bdg_load_wavefunction_bin(self.h, str(path).encode()))
+
+    def write_modes_bin(self, path) -> int:
+        return int(lib().bdg_write_modes_bin(self.h, str(path).encode()))
 
     def set_verbosity(self, level):
         lib().bdg_set_verbosity(self.h, int(level))
@@ -699,3 +725,25 @@ class Problem:
     def device_context(self):
         """Non-owning DeviceContext of this problem (fields as of the last solve)."""
         return DeviceContext.wrap(self.device_context_handle(), self.N, self.L, self.cplx)
+
+
+def read_bin(path):
+    """Read a file of bdg_save_wavefunction_bin / bdg_write_modes_bin:
+    returns (header dict, psi0) for kind 1 and (header, eigenvalues, U, V)
+    with U, V of shape (S, nev) for kind 2."""
+    with open(path, "rb") as f:
+        raw = f.read(64)
+        if len(raw) != 64 or raw[:8] != b"BDGBIN1\0":
+            raise ValueError(f"{path}: not a BDGBIN1 file")
+        kind, cplx = np.frombuffer(raw, np.uint32, 2, 8)
+        dim, n0, n1, n2, count = (int(v) for v in np.frombuffer(raw, np.uint64, 5, 16))
+        hdr = dict(kind=int(kind), complex=bool(cplx), dim=dim, N=(n0, n1, n2)[:dim],
+                   count=count)
+        S = n0 * n1 * n2
+        dt = np.complex128 if cplx else np.float64
+        if kind == 1:
+            return hdr, np.fromfile(f, dt, S)
+        w = np.fromfile(f, np.float64, count)
+        U = np.fromfile(f, dt, S * count).reshape(count, S).T
+        V = np.fromfile(f, dt, S * count).reshape(count, S).T
+        return hdr, w, U, V

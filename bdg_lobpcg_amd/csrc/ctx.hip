@@ -337,7 +337,175 @@ __global__ void __launch_bounds__(256) k_fill_uniform(double *X, long long ld, l
   }
 }
 
+// ---- reuse block of a parameter sweep (reference src/bdg.c:112-207) ------
+// Column j < nev: [U_j; V_j] plus sigma_j (z0, z1), sigma_j = noise *
+// ||[U_j; V_j]||, (z0, z1) the Box-Muller pair of the xrand values number
+// 2 (j S + i) + 1, + 2 of the xorshift32 stream; column j >= nev: both halves
+// |z0| |psi0_i|.  Column norms: kReuseChunks fixed row chunks per column,
+// summed in a fixed order (deterministic).
+constexpr int kReuseChunks = 64;
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256) k_colsq_partial(const double *U, const double *V,
+                                                       long long ld, long long S,
+                                                       double *part) {
+  constexpr int W = CPLX ? 2 : 1;
+  const int j = blockIdx.y;
+  const long long len = (S + kReuseChunks - 1) / kReuseChunks;
+  const long long r0 = (long long)blockIdx.x * len;
+  const long long r1 = r0 + len < S ? r0 + len : S;
+  const double *u = U + (long long)j * ld * W, *v = V + (long long)j * ld * W;
+  double s = 0.0;
+  for (long long i = r0 * W + threadIdx.x; i < r1 * W; i += 256) s += u[i] * u[i] + v[i] * v[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long long)j * kReuseChunks + blockIdx.x] = red[0];
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256) k_reuse_fill(const double *U, const double *V,
+                                                    long long ldm, int nev, const double *part,
+                                                    double noise, const double *wf, double *OU,
+                                                    double *OV, long long ldo, long long S,
+                                                    const uint32_t *col_state,
+                                                    const uint32_t *pow2, int nbits) {
+  constexpr int RUN = 64;
+  constexpr int W = CPLX ? 2 : 1;
+  const int j = blockIdx.y;
+  const long long r0 = ((long long)blockIdx.x * 256 + threadIdx.x) * RUN;
+  if (r0 >= S) return;
+  double sigma = 0.0;
+  if (j < nev) {
+    double s = 0.0;
+    for (int q = 0; q < kReuseChunks; ++q) s += part[(long long)j * kReuseChunks + q];
+    sigma = noise * sqrt(s);
+  }
+  // two stream values per row: jump 2 r0 steps from the column's start
+  uint32_t x = col_state[j];
+  const unsigned long long jump = 2ull * (unsigned long long)r0;
+  for (int b = 0; b < nbits; ++b) {
+    if ((jump >> b) & 1ull) {
+      const uint32_t *m = pow2 + 32 * b;
+      uint32_t r = 0;
+      for (int k = 0; k < 32; ++k)
+        if (x >> k & 1u) r ^= m[k];
+      x = r;
+    }
+  }
+  const long long r1 = r0 + RUN < S ? r0 + RUN : S;
+  double *ou = OU + (long long)j * ldo * W, *ov = OV + (long long)j * ldo * W;
+  for (long long i = r0; i < r1; ++i) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    double u1 = (double)(x & 0x7FFFFFFFu) / (double)0x7FFFFFFF;
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    const double u2 = (double)(x & 0x7FFFFFFFu) / (double)0x7FFFFFFF;
+    u1 = fmax(u1, 1e-10);
+    const double rr = sqrt(-2.0 * log(u1));
+    const double ang = 2.0 * M_PI * u2;
+    if (j < nev) {
+      const double *u = U + (long long)j * ldm * W, *v = V + (long long)j * ldm * W;
+      ou[W * i] = u[W * i] + sigma * (rr * cos(ang));
+      ov[W * i] = v[W * i] + sigma * (rr * sin(ang));
+      if (CPLX) {
+        ou[W * i + 1] = u[W * i + 1];
+        ov[W * i + 1] = v[W * i + 1];
+      }
+    } else {
+      double a = 1.0;
+      if (wf) a = CPLX ? sqrt(wf[2 * i] * wf[2 * i] + wf[2 * i + 1] * wf[2 * i + 1]) : fabs(wf[i]);
+      const double val = fabs(rr * cos(ang)) * a;
+      ou[W * i] = val;
+      ov[W * i] = val;
+      if (CPLX) {
+        ou[W * i + 1] = 0.0;
+        ov[W * i + 1] = 0.0;
+      }
+    }
+  }
+}
+
 extern "C" {
+
+int bdgx_reuse_block(bdgx_ctx *c, const void *dU, const void *dV, uint64_t ldm, uint64_t nev,
+                     void *dOU, void *dOV, uint64_t ldo, uint64_t ncols, uint64_t nrows,
+                     double noise_frac, uint32_t seed) {
+  if (!c || !dOU || !dOV || nrows == 0 || ncols == 0 || ncols > 65535 || nev > ncols ||
+      ldo < nrows || (nev > 0 && (!dU || !dV || ldm < nrows)) ||
+      2 * nrows * ncols > (1ull << kJumpBits))
+    return BDGX_EINVAL;
+  std::vector<uint32_t> pw(32 * kJumpBits);
+  Gf2 t;
+  for (int b = 0; b < 32; ++b) t.col[b] = xs_step(1u << b);
+  for (int b = 0; b < kJumpBits; ++b) {
+    memcpy(&pw[32 * b], t.col, sizeof(t.col));
+    t = gf2_mul(t, t);
+  }
+  // column start states: 2 nrows stream values per column
+  std::vector<uint32_t> st(ncols);
+  uint32_t s0 = seed;
+  const uint64_t per_col = 2 * nrows;
+  for (uint64_t j = 0; j < ncols; ++j) {
+    st[j] = s0;
+    for (int b = 0; b < kJumpBits; ++b)
+      if (per_col >> b & 1ull) {
+        Gf2 m;
+        memcpy(m.col, &pw[32 * b], sizeof(m.col));
+        s0 = gf2_apply(m, s0);
+      }
+  }
+  uint32_t *dtab = nullptr;
+  double *dpart = nullptr;
+  const size_t tb = sizeof(uint32_t) * (pw.size() + st.size());
+  BDGX_HIP(hipMalloc((void **)&dtab, tb));
+  int rc = BDGX_OK;
+  if (hipMalloc((void **)&dpart, sizeof(double) * kReuseChunks * (nev ? nev : 1)) != hipSuccess)
+    rc = BDGX_EHIP;
+  if (!rc && (hipMemcpyAsync(dtab, pw.data(), sizeof(uint32_t) * pw.size(),
+                             hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+              hipMemcpyAsync(dtab + pw.size(), st.data(), sizeof(uint32_t) * st.size(),
+                             hipMemcpyHostToDevice, c->stream) != hipSuccess))
+    rc = BDGX_EHIP;
+  const double *wf = c->has_wf ? c->f64.wf : nullptr;
+  const double *U = (const double *)dU, *V = (const double *)dV;
+  if (!rc && nev > 0) {
+    const dim3 grid(kReuseChunks, (unsigned)nev);
+    if (c->cplx)
+      hipLaunchKernelGGL(k_colsq_partial<true>, grid, dim3(256), 0, c->stream, U, V,
+                         (long long)ldm, (long long)nrows, dpart);
+    else
+      hipLaunchKernelGGL(k_colsq_partial<false>, grid, dim3(256), 0, c->stream, U, V,
+                         (long long)ldm, (long long)nrows, dpart);
+    if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
+  }
+  if (!rc) {
+    const unsigned gx = (unsigned)((nrows + 256 * 64 - 1) / (256 * 64));
+    const dim3 grid(gx, (unsigned)ncols);
+    if (c->cplx)
+      hipLaunchKernelGGL(k_reuse_fill<true>, grid, dim3(256), 0, c->stream, U, V, (long long)ldm,
+                         (int)nev, dpart, noise_frac, wf, (double *)dOU, (double *)dOV,
+                         (long long)ldo, (long long)nrows, dtab + pw.size(), dtab, kJumpBits);
+    else
+      hipLaunchKernelGGL(k_reuse_fill<false>, grid, dim3(256), 0, c->stream, U, V,
+                         (long long)ldm, (int)nev, dpart, noise_frac, wf, (double *)dOU,
+                         (double *)dOV, (long long)ldo, (long long)nrows, dtab + pw.size(), dtab,
+                         kJumpBits);
+    if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
+  }
+  // the tables and partials must outlive the kernels: wait, then free
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = BDGX_EHIP;
+  (void)hipFree(dtab);
+  if (dpart) (void)hipFree(dpart);
+  return rc;
+}
 
 const char *bdgx_error_string(int code) {
   switch (code) {

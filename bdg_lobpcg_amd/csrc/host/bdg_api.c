@@ -123,6 +123,7 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
   b->fp32_stall = 40;
   b->lock_max = -1;
   b->lock_guards = 4;
+  b->device_sweeps = 1;
   return b;
 }
 
@@ -138,13 +139,23 @@ static void free_results(bdg_t *b) {
   freep(&b->modes_v);
   freep((void **)&b->residuals);
   freep((void **)&b->zero_modes);
+  (void)bdgx_free(b->d_modes_u);
+  (void)bdgx_free(b->d_modes_v);
+  b->d_modes_u = b->d_modes_v = NULL;
   b->converged = 0;
+}
+
+static void free_device_reuse(bdg_t *b) {
+  (void)bdgx_free(b->d_reuse_u);
+  (void)bdgx_free(b->d_reuse_v);
+  b->d_reuse_u = b->d_reuse_v = NULL;
 }
 
 void bdg_free(bdg_t **pb) {
   if (!pb || !*pb) return;
   bdg_t *b = *pb;
   free_results(b);
+  free_device_reuse(b);
   freep(&b->reuse_buf);
   for (int d = 0; d < 3; ++d) freep((void **)&b->kx2[d]);
   freep((void **)&b->ltK);
@@ -320,6 +331,22 @@ int bdg_reuse_modes(bdg_t *b, f64 noise_frac) {
   const uint64_t S = b->size, n = 2 * S, nev = b->nev, m = b->sizeSub;
   const int c = b->complex_psi0;
   freep(&b->reuse_buf);
+  free_device_reuse(b);
+  if (b->dev && b->d_modes_u && b->d_modes_v) {
+    /* device-resident sweep: the block is built next to the modes and
+     * stays on the device for the next solve */
+    const size_t es = sizeof(double) * (c ? 2 : 1);
+    int rc = bdgx_malloc(&b->d_reuse_u, S * m * es);
+    if (!rc) rc = bdgx_malloc(&b->d_reuse_v, S * m * es);
+    if (!rc)
+      rc = bdgx_reuse_block(b->dev, b->d_modes_u, b->d_modes_v, S, nev < m ? nev : m,
+                            b->d_reuse_u, b->d_reuse_v, S, m, S, noise_frac, 12345u);
+    if (rc) BDG_FAIL("bdg_reuse_modes: device reuse block failed: %s", bdgx_error_string(rc));
+    b->reuse_n = n;
+    b->reuse_cols = m;
+    b->init_mode = BDG_INIT_REUSE;
+    return 0;
+  }
   b->reuse_buf = xcalloc(n * m, sizeof(double) * (c ? 2 : 1));
   b->reuse_n = n;
   b->reuse_cols = m;
@@ -507,6 +534,22 @@ void bdg_start_block(bdg_t *b, void *X) {
   BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_start_block");
   const bdg_init_mode_t mode = b->init_mode;
   void *keep = b->reuse_buf;
+  if (mode == BDG_INIT_REUSE && b->d_reuse_u && b->reuse_n == 2 * b->size &&
+      b->reuse_cols == b->sizeSub) {
+    /* the device-resident block, column by column into [u; v] */
+    const size_t es = sizeof(double) * (b->complex_psi0 ? 2 : 1);
+    const uint64_t S = b->size;
+    int rc = 0;
+    for (uint64_t j = 0; j < b->sizeSub && !rc; ++j) {
+      rc = bdgx_memcpy_d2h(b->dev, (char *)X + 2 * j * S * es,
+                           (const char *)b->d_reuse_u + j * S * es, S * es);
+      if (!rc)
+        rc = bdgx_memcpy_d2h(b->dev, (char *)X + (2 * j + 1) * S * es,
+                             (const char *)b->d_reuse_v + j * S * es, S * es);
+    }
+    if (rc) BDG_FAIL("bdg_start_block: download failed: %s", bdgx_error_string(rc));
+    return;
+  }
   if (mode == BDG_INIT_REUSE && keep) {
     /* copy without consuming the buffer */
     const size_t es = sizeof(double) * (b->complex_psi0 ? 2 : 1);
@@ -562,7 +605,7 @@ static int learned_applies(const bdg_t *b) {
  * an FP32 phase followed by an FP64 phase from its vectors.  dV0 is the v
  * half of the start block (u = v for the built-in blocks; the reference's
  * start blocks set u = v too, src/solver.c:166-259). */
-static int solve_first_order(bdg_t *b, const void *dV0, double t_start) {
+static int solve_first_order(bdg_t *b, const void *dU0, const void *dV0, double t_start) {
   const uint64_t S = b->size, m = b->sizeSub, nev = b->nev;
   const int c = b->complex_psi0;
   const size_t es64 = sizeof(double) * (c ? 2 : 1);
@@ -585,14 +628,14 @@ static int solve_first_order(bdg_t *b, const void *dV0, double t_start) {
   double *w2 = (double *)xcalloc(m, sizeof(double)), *resv = (double *)xcalloc(m, sizeof(double));
   r32.w2 = r64.w2 = w2;
   r32.res = r64.res = resv;
-  const void *start = dV0;
+  const void *start = dV0, *startU = dU0;
   int it32 = 0;
   if (!rc && b->precision == BDG_PREC_MIXED) {
     cfg.maxiter = (int)b->maxIter;
     cfg.tol = fmax(b->tol, b->fp32_tol);
     cfg.stall_window = b->fp32_stall;
     cfg.stall_below = 1e-4;
-    rc = bdg_lrep_device(b->dev, &cfg, BDGX_FP32, dV0, dV0, BDGX_FP64, &r32, dU, dV);
+    rc = bdg_lrep_device(b->dev, &cfg, BDGX_FP32, dU0, dV0, BDGX_FP64, &r32, dU, dV);
     it32 = r32.iters;
     b->stats.iterations_fp32 = (uint64_t)it32;
     b->stats.fp32_worst_residual = r32.worst;
@@ -607,7 +650,7 @@ static int solve_first_order(bdg_t *b, const void *dV0, double t_start) {
     if (cfg.maxiter < 0) cfg.maxiter = 0;
     cfg.tol = b->tol;
     cfg.stall_window = 0;
-    rc = start ? bdg_lrep_device(b->dev, &cfg, BDGX_FP64, start, start, BDGX_FP64, &r64, dU, dV)
+    rc = start ? bdg_lrep_device(b->dev, &cfg, BDGX_FP64, startU, start, BDGX_FP64, &r64, dU, dV)
                : bdg_lrep_device(b->dev, &cfg, BDGX_FP64, dU, dV, BDGX_FP64, &r64, dU, dV);
   }
   b->stats.iterations = (uint64_t)(it32 + r64.iters);
@@ -661,8 +704,13 @@ static int solve_first_order(bdg_t *b, const void *dV0, double t_start) {
   bdgx_free(dzero);
   bdgx_free(dn2);
   bdgx_free(dcu);
-  bdgx_free(dOutU);
-  bdgx_free(dOutV);
+  if (!rc && b->device_sweeps) {  /* the modes stay on the device for a sweep's reuse block */
+    b->d_modes_u = dOutU;
+    b->d_modes_v = dOutV;
+  } else {
+    bdgx_free(dOutU);
+    bdgx_free(dOutV);
+  }
   bdgx_free(dU);
   bdgx_free(dV);
   free(hn2);
@@ -695,10 +743,23 @@ int bdg_solve(bdg_t *b) {
     return rc;
   }
   const double ts_fields = bdg_wtime();
-  void *dV0 = NULL;
-  rc = bdgx_malloc(&dV0, S * m * es64);
+  void *dV0 = NULL, *dU0 = NULL; /* dU0: u half where it differs from v (reuse) */
   double ts_block = ts_fields;
-  if (!rc && b->init_mode == BDG_INIT_CUSTOM && b->custom_init_fn == bdg_init_uniform_xorshift) {
+  if (b->init_mode == BDG_INIT_REUSE && b->d_reuse_u && b->d_reuse_v && b->reuse_n == n &&
+      b->reuse_cols == m) {
+    /* device-resident sweep: the reuse block built on the device by
+     * bdg_reuse_modes, consumed here (as the host block is) */
+    dU0 = b->d_reuse_u;
+    dV0 = b->d_reuse_v;
+    b->d_reuse_u = b->d_reuse_v = NULL;
+    b->init_mode = BDG_INIT_WF_WEIGHTED;
+  } else {
+    free_device_reuse(b);
+    rc = bdgx_malloc(&dV0, S * m * es64);
+  }
+  if (dU0) {
+    /* start block already on the device */
+  } else if (!rc && b->init_mode == BDG_INIT_CUSTOM && b->custom_init_fn == bdg_init_uniform_xorshift) {
     /* the built-in uniform block (u = v): its v half generated on the
      * device, the same values as the host fill */
     const uint32_t seed = b->custom_init_param ? *(const uint32_t *)b->custom_init_param : 42u;
@@ -711,23 +772,35 @@ int bdg_solve(bdg_t *b) {
     void *X = mmap(NULL, xbytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
     if (X == MAP_FAILED) BDG_FAIL("out of host memory (%zu bytes)", xbytes);
     (void)madvise(X, xbytes, MADV_HUGEPAGE);
+    /* a reuse block has u != v: both halves go up */
+    const int two_halves = b->init_mode == BDG_INIT_REUSE && b->reuse_buf != NULL;
     bdg_fill_start_block(b, X);
     ts_block = bdg_wtime();
-    for (uint64_t j = 0; j < m && !rc; ++j)
+    if (two_halves) rc = bdgx_malloc(&dU0, S * m * es64);
+    for (uint64_t j = 0; j < m && !rc; ++j) {
       rc = bdgx_memcpy_h2d(b->dev, (char *)dV0 + j * S * es64,
                            (const char *)X + (j * n + S) * es64, S * es64);
+      if (!rc && dU0)
+        rc = bdgx_memcpy_h2d(b->dev, (char *)dU0 + j * S * es64, (const char *)X + j * n * es64,
+                             S * es64);
+    }
     munmap(X, xbytes);
   }
   if (rc) {
     bdgx_free(dV0);
+    bdgx_free(dU0);
     b->stats.status = rc;
     return rc;
   }
   if (b->formulation == BDG_FORM_FIRST_ORDER && !k_equals_m(b) && !learned) {
-    rc = solve_first_order(b, dV0, t_start);
+    rc = solve_first_order(b, dU0 ? dU0 : dV0, dV0, t_start);
     bdgx_free(dV0);
+    bdgx_free(dU0);
     return rc;
   }
+  /* the product form iterates on the v half only */
+  bdgx_free(dU0);
+  dU0 = NULL;
   lobpcg_cfg cfg;
   memset(&cfg, 0, sizeof(cfg));
   cfg.S = S;
@@ -889,9 +962,14 @@ int bdg_solve(bdg_t *b) {
   bdgx_free(dn2);
   bdgx_free(dcu);
   bdgx_free(dcv);
-  bdgx_free(dU);
+  if (!rc && b->device_sweeps) {  /* u in dU, v in the head of dMX: kept for a sweep's reuse block */
+    b->d_modes_u = dU;
+    b->d_modes_v = dMX;
+  } else {
+    bdgx_free(dU);
+    bdgx_free(dMX);
+  }
   bdgx_free(dX);
-  bdgx_free(dMX);
   free(hn2);
   free(cu);
   free(cv);
@@ -1007,6 +1085,7 @@ void bdg_set_mixed_params(bdg_t *b, f64 img_tol, uint64_t refresh_every) {
 }
 void bdg_set_verbosity(bdg_t *b, int level) { b->verbosity = level; }
 void bdg_set_graph_capture(bdg_t *b, int on) { b->graphs = on ? 1 : 0; }
+void bdg_set_device_sweeps(bdg_t *b, int on) { b->device_sweeps = on ? 1 : 0; }
 void bdg_set_freeze(bdg_t *b, f64 tol) { b->freeze_tol = tol > 0 ? tol : 0.0; }
 void bdg_set_learned_fallback(bdg_t *b, int window, f64 factor) {
   b->lp_window = window > 0 ? window : 0;

@@ -92,6 +92,13 @@ struct bdg_t {
   void *custom_init_param;
   void *reuse_buf;
   uint64_t reuse_n, reuse_cols;
+  /* device-resident sweeps: the modes of the last solve kept on the device
+   * (S x nev per half, FP64; freed with the results), and the reuse start
+   * block built there from them (S x reuse_cols per half; survives reset,
+   * consumed by the next solve) */
+  void *d_modes_u, *d_modes_v;
+  void *d_reuse_u, *d_reuse_v;
+  int device_sweeps; /* keep d_modes after a solve (default 1) */
 };
 
 /* xorshift32 / uniform (0, 1] (reference src/bdg_internal.h:53-65) */

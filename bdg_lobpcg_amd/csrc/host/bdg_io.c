@@ -102,3 +102,92 @@ int bdg_write_mode_u(const bdg_t *b, uint64_t idx, const char *base) {
 int bdg_write_mode_v(const bdg_t *b, uint64_t idx, const char *base) {
   return write_mode(b, idx, b->modes_v, base, "v");
 }
+
+/* ---- binary fast path (no reference counterpart) ---------------------------
+ * The text formats above write ~30 bytes per value through stdio; at 128^3
+ * and beyond a wavefunction or a set of modes is hundreds of MB of text.
+ * The binary files hold the same values bit for bit: a 64-byte header
+ *   char magic[8] "BDGBIN1"; uint32 kind (1 wavefunction, 2 modes);
+ *   uint32 complex; uint64 dim; uint64 N[3]; uint64 count (1, or nev);
+ *   uint64 reserved
+ * followed by little-endian FP64 payload (interleaved re, im when complex):
+ * kind 1: psi0 (S values); kind 2: the nev eigenvalues (real), then modes_u
+ * (S x nev, column-major), then modes_v. */
+typedef struct {
+  char magic[8];
+  uint32_t kind, cplx;
+  uint64_t dim, N[3], count, reserved;
+} bdg_bin_header;
+
+static const char kBinMagic[8] = "BDGBIN1";
+
+static void bin_header(const bdg_t *b, uint32_t kind, uint64_t count, bdg_bin_header *h) {
+  memset(h, 0, sizeof(*h));
+  memcpy(h->magic, kBinMagic, sizeof(kBinMagic));
+  h->kind = kind;
+  h->cplx = (uint32_t)b->complex_psi0;
+  h->dim = b->dim;
+  for (int d = 0; d < 3; ++d) h->N[d] = d < (int)b->dim ? b->N[d] : 1;
+  h->count = count;
+}
+
+static int write_all(FILE *f, const void *p, size_t bytes) {
+  return fwrite(p, 1, bytes, f) == bytes ? 0 : -1;
+}
+
+int bdg_save_wavefunction_bin(const bdg_t *b, const char *filename) {
+  if (!b->wf) return -3;
+  FILE *f = fopen(filename, "wb");
+  if (!f) return -1;
+  bdg_bin_header h;
+  bin_header(b, 1, 1, &h);
+  const size_t bytes = b->size * sizeof(double) * (b->complex_psi0 ? 2 : 1);
+  int rc = write_all(f, &h, sizeof(h));
+  if (!rc) rc = write_all(f, b->wf, bytes);
+  if (fclose(f) != 0) rc = -1;
+  return rc;
+}
+
+int bdg_load_wavefunction_bin(bdg_t *b, const char *filename) {
+  BDG_REQUIRE(b, BDG_HAS_SYSTEM, "bdg_load_wavefunction_bin");
+  FILE *f = fopen(filename, "rb");
+  if (!f) return -1;
+  bdg_bin_header h, want;
+  bin_header(b, 1, 1, &want);
+  if (fread(&h, 1, sizeof(h), f) != sizeof(h) || memcmp(h.magic, kBinMagic, 8) != 0 ||
+      h.kind != 1 || h.cplx != want.cplx || h.dim != want.dim || h.count != 1 ||
+      memcmp(h.N, want.N, sizeof(h.N)) != 0) {
+    fclose(f);
+    return -2;
+  }
+  const size_t n = b->size * (b->complex_psi0 ? 2 : 1);
+  double *wf = (double *)malloc(n * sizeof(double));
+  if (!wf) {
+    fclose(f);
+    return -1;
+  }
+  const size_t got = fread(wf, sizeof(double), n, f);
+  fclose(f);
+  if (got != n) {
+    free(wf);
+    return -2;
+  }
+  bdg_set_wavefunction(b, wf, b->size);
+  free(wf);
+  return 0;
+}
+
+int bdg_write_modes_bin(const bdg_t *b, const char *filename) {
+  if (!b->eigvals || !b->modes_u || !b->modes_v) return -3;
+  FILE *f = fopen(filename, "wb");
+  if (!f) return -1;
+  bdg_bin_header h;
+  bin_header(b, 2, b->nev, &h);
+  const size_t mb = b->size * b->nev * sizeof(double) * (b->complex_psi0 ? 2 : 1);
+  int rc = write_all(f, &h, sizeof(h));
+  if (!rc) rc = write_all(f, b->eigvals, b->nev * sizeof(double));
+  if (!rc) rc = write_all(f, b->modes_u, mb);
+  if (!rc) rc = write_all(f, b->modes_v, mb);
+  if (fclose(f) != 0) rc = -1;
+  return rc;
+}

@@ -55,6 +55,25 @@ void bdg_set_locking(bdg_t *bdg, int max_active);
  * block. */
 void bdg_set_lock_guards(bdg_t *bdg, int guards);
 
+/* Device-resident parameter sweeps (default on): the modes of a solve stay
+ * in device memory beside the host results (2 x S x nev values), so that
+ * bdg_reuse_modes builds the reuse start block on the device and the next
+ * bdg_solve starts from it without a host round trip (the sweep of reference
+ * src/bdg.c:55-207).  Off: the modes live on the host only and the reuse
+ * block is built there, as in the reference. */
+void bdg_set_device_sweeps(bdg_t *bdg, int on);
+
+/* Binary fast path beside the reference's text formats (reference
+ * src/io.c:13-165 writes ~30 bytes of text per value): a 64-byte header
+ * (magic "BDGBIN1", kind 1 wavefunction / 2 modes, complex flag, dim, N[3],
+ * count) and raw little-endian FP64 values, bit-exact.  Modes files hold the
+ * nev eigenvalues, then modes_u and modes_v (S x nev each, column-major).
+ * Codes as the text functions: 0, -1 (file), -2 (format, or header not
+ * matching this grid / variant), -3 (nothing to write). */
+int bdg_save_wavefunction_bin(const bdg_t *bdg, const char *filename);
+int bdg_load_wavefunction_bin(bdg_t *bdg, const char *filename);
+int bdg_write_modes_bin(const bdg_t *bdg, const char *filename);
+
 typedef struct {
   uint64_t iterations;          /* LOBPCG iterations                           */
   uint64_t column_applies;      /* (M then K) column applies, incl. start      */

@@ -180,6 +180,23 @@ int bdgx_gather_cols(bdgx_ctx *ctx, int prec, const void *dSrc, uint64_t lds,
 int bdgx_fill_uniform_xorshift(bdgx_ctx *ctx, void *dX, uint64_t ld, uint64_t nrows,
                                uint64_t ncols, uint64_t steps_per_col, uint32_t seed);
 
+/* Reuse start block of a parameter sweep, built on the device from the
+ * device-resident modes of the last solve (replaces the host loops of
+ * reference src/bdg.c:112-207, bdg_reuse_modes).  FP64 blocks, S = nrows
+ * rows per half, complex ctx: interleaved.  Column j < nev:
+ *   OU_j = U_j + sigma_j z0,  OV_j = V_j + sigma_j z1,
+ *   sigma_j = noise_frac * ||[U_j; V_j]||,
+ * (z0, z1) the Box-Muller pair sqrt(-2 ln max(u1, 1e-10)) (cos, sin)(2 pi u2)
+ * of the xrand values u1, u2 number 2 (j S + i) + 1, + 2 of the xorshift32
+ * stream seeded with `seed` (the reference uses 12345); complex: the noise
+ * goes to the real parts.  Columns nev <= j < ncols: both halves
+ * |z0| * |psi0_i| (|psi0| of the context's wavefunction, 1 where none is
+ * set).  Values equal the sequential host recipe up to the rounding of the
+ * device log / cos / sin and of the norm's summation order. */
+int bdgx_reuse_block(bdgx_ctx *ctx, const void *dU, const void *dV, uint64_t ldm, uint64_t nev,
+                     void *dOU, void *dOV, uint64_t ldo, uint64_t ncols, uint64_t nrows,
+                     double noise_frac, uint32_t seed);
+
 /* Element-type conversion FP64 <-> FP32 of an nrows x ncols block. */
 int bdgx_convert(bdgx_ctx *ctx, int prec_src, const void *dsrc, uint64_t lds,
                  int prec_dst, void *ddst, uint64_t ldd, uint64_t nrows,

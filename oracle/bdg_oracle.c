@@ -858,4 +858,49 @@ void orc_start_block_uniform(const orc_ctx *c, uint32_t seed, int64_t ncols,
     }
 }
 
+/* Reuse block of a parameter sweep, reference src/bdg.c:112-207
+ * (bdg_reuse_modes): S rows per half, modes U, V (S x nev, FP64 or
+ * interleaved complex), wf (S values, or NULL), output X (2S x m, [u; v]
+ * column-major).  One xorshift32 stream from `seed`, two xrand values per
+ * row, columns in order: columns j < nev get the modes plus
+ * sigma_j (z0, z1), sigma_j = noise ||[u_j; v_j]|| (complex: noise on the
+ * real parts); columns nev <= j < m get |z0| |wf| in both halves. */
+void orc_reuse_block(int64_t S, int64_t nev, int64_t m, int cplx, const double *U,
+                     const double *V, const double *wf, double noise, uint32_t seed,
+                     double *X) {
+  const int64_t n = 2 * S, w = cplx ? 2 : 1;
+  uint32_t st = seed;
+  for (int64_t j = 0; j < m; j++) {
+    double sigma = 0.0;
+    if (j < nev) {
+      double cn = 0.0;
+      for (int64_t i = 0; i < S * w; i++)
+        cn += U[j * S * w + i] * U[j * S * w + i] + V[j * S * w + i] * V[j * S * w + i];
+      sigma = noise * sqrt(cn);
+    }
+    for (int64_t i = 0; i < S; i++) {
+      double u1 = orc_xrand(&st);
+      const double u2 = orc_xrand(&st);
+      u1 = fmax(u1, 1e-10);
+      const double r = sqrt(-2.0 * log(u1));
+      const double z0 = r * cos(2.0 * M_PI * u2), z1 = r * sin(2.0 * M_PI * u2);
+      double *xu = X + (j * n + i) * w, *xv = X + (j * n + S + i) * w;
+      if (j < nev) {
+        const double *u = U + (j * S + i) * w, *v = V + (j * S + i) * w;
+        xu[0] = u[0] + sigma * z0;
+        xv[0] = v[0] + sigma * z1;
+        if (cplx) {
+          xu[1] = u[1];
+          xv[1] = v[1];
+        }
+      } else {
+        double a = 1.0;
+        if (wf) a = cplx ? hypot(wf[2 * i], wf[2 * i + 1]) : fabs(wf[i]);
+        xu[0] = xv[0] = fabs(z0) * a;
+        if (cplx) xu[1] = xv[1] = 0.0;
+      }
+    }
+  }
+}
+
 int orc_num_threads(void) { return omp_get_max_threads(); }

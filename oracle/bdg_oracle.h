@@ -91,6 +91,10 @@ int orc_itp_groundstate(const orc_ctx *ctx, const double *omega, double g,
 /* xorshift32 uniform (0,1] (reference src/bdg_internal.h:53-65). */
 uint32_t orc_xorshift32(uint32_t *state);
 double orc_xrand(uint32_t *state);
+/* reuse block of a sweep (reference src/bdg.c:112-207) */
+void orc_reuse_block(int64_t S, int64_t nev, int64_t m, int cplx, const double *U,
+                     const double *V, const double *wf, double noise, uint32_t seed,
+                     double *X);
 /* Column-major n x ncols block, u = v = xrand-0.5 (seed given). */
 void orc_start_block_uniform(const orc_ctx *ctx, uint32_t seed, int64_t ncols,
                              double *X);

@@ -65,6 +65,8 @@ def lib():
         L.orc_xrand.restype = C.c_double
         L.orc_xrand.argtypes = [C.POINTER(C.c_uint32)]
         L.orc_start_block_uniform.argtypes = [P, C.c_uint32, C.c_int64, P]
+        L.orc_reuse_block.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int, P, P, P,
+                                      C.c_double, C.c_uint32, P]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -207,6 +209,21 @@ def setup_mu(mu, ltK, ltM):
 def xrand_stream(seed: int, count: int) -> np.ndarray:
     s = C.c_uint32(seed)
     return np.array([lib().orc_xrand(C.byref(s)) for _ in range(count)])
+
+
+def reuse_block(U, V, m, noise, wf=None, seed=12345):
+    """Reuse start block [u; v] (2S x m) of a sweep from modes U, V (S x nev)
+    (reference src/bdg.c:112-207, bdg_reuse_modes)."""
+    cplx = np.iscomplexobj(U) or np.iscomplexobj(V)
+    dt = np.complex128 if cplx else np.float64
+    U = np.asfortranarray(U, dtype=dt)
+    V = np.asfortranarray(V, dtype=dt)
+    S, nev = U.shape
+    X = np.zeros((2 * S, m), dtype=dt, order="F")
+    w = None if wf is None else np.ascontiguousarray(wf, dtype=dt)
+    lib().orc_reuse_block(S, nev, int(m), int(cplx), _p(U), _p(V),
+                          None if w is None else _p(w), float(noise), int(seed), _p(X))
+    return X
 
 
 def num_threads() -> int:

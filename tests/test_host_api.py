@@ -200,6 +200,47 @@ def test_io_formats_and_error_codes(tmp_path):
     assert lib.bdg_reuse_modes(p.h, 0.1) == -3
 
 
+@pytest.mark.parametrize("cplx", [False, True], ids=["real", "complex"])
+def test_binary_wavefunction_round_trip_and_codes(tmp_path, cplx):
+    """Binary fast path beside the text loader: a saved psi0 loads back bit
+    for bit into a handle of the same grid; a file of another grid or
+    variant, a truncated file and a missing file give -2 / -2 / -1; nothing
+    to save gives -3; modes without results give -3."""
+    N, L = (6, 5, 4), (3.0, 2.5, 2.0)
+    rng = np.random.default_rng(1)
+    S = int(np.prod(N))
+    psi = rng.standard_normal(S) + (1j * rng.standard_normal(S) if cplx else 0)
+    p = B.Problem(N, L, cplx)
+    p.set_system()
+    f = tmp_path / "psi.bin"
+    assert p.save_wavefunction_bin(f) == -3
+    p.set_wavefunction(psi)
+    assert p.save_wavefunction_bin(f) == 0
+    assert f.stat().st_size == 64 + S * 8 * (2 if cplx else 1)
+    hdr, got = B.read_bin(f)
+    assert hdr == dict(kind=1, complex=cplx, dim=3, N=N, count=1)
+    np.testing.assert_array_equal(got, psi)
+    q = B.Problem(N, L, cplx)
+    q.set_system()
+    assert q.load_wavefunction_bin(f) == 0
+    g = tmp_path / "psi2.bin"
+    assert q.save_wavefunction_bin(g) == 0
+    assert f.read_bytes() == g.read_bytes()
+    other = B.Problem((6, 5, 5), L, cplx)
+    other.set_system()
+    assert other.load_wavefunction_bin(f) == -2
+    variant = B.Problem(N, L, not cplx)
+    variant.set_system()
+    assert variant.load_wavefunction_bin(f) == -2
+    t = tmp_path / "short.bin"
+    t.write_bytes(f.read_bytes()[:-8])
+    r = B.Problem(N, L, cplx)
+    r.set_system()
+    assert r.load_wavefunction_bin(t) == -2
+    assert r.load_wavefunction_bin(tmp_path / "missing.bin") == -1
+    assert r.write_modes_bin(tmp_path / "modes.bin") == -3
+
+
 @pytest.mark.parametrize("k,m", [(12, 5), (40, 16), (96, 32)])
 def test_host_rayleigh_ritz_matches_generalized_eigh(k, m):
     """The solver's host Rayleigh-Ritz (dense.c / rr_impl.inc: Euclidean

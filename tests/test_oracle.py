@@ -241,6 +241,56 @@ def test_xrand_stream():
     np.testing.assert_array_equal(O.xrand_stream(42, 3), out)
 
 
+@pytest.mark.parametrize("cplx", [False, True], ids=["real", "complex"])
+def test_reuse_block_recipe(cplx):
+    """Oracle reuse block (reference src/bdg.c:112-207) pinned by what the
+    reference's recipe fixes: the shape and a nonzero buffer
+    (tests/test_io.c:332-365); noise 0 copies the modes exactly and fills
+    the extra columns with |z0| |psi0| >= 0; the stream position of every
+    (row, column): with unit-norm modes and noise 1 the added pair is
+    (z0, z1) of xrand values 2 (j S + i) + 1, + 2 of seed 12345 (a swapped
+    pair, a restarted stream per column or a dropped column shifts it); and
+    the Box-Muller law: z0, z1 standard normal and uncorrelated."""
+    rng = np.random.default_rng(4)
+    S, nev, m = 40, 2, 4
+    dt = np.complex128 if cplx else np.float64
+    U = rng.standard_normal((S, nev)).astype(dt)
+    V = rng.standard_normal((S, nev)).astype(dt)
+    if cplx:
+        U += 1j * rng.standard_normal((S, nev))
+        V += 1j * rng.standard_normal((S, nev))
+    wf = rng.standard_normal(S) + (1j * rng.standard_normal(S) if cplx else 0)
+    X0 = O.reuse_block(U, V, m, 0.0, wf)
+    assert X0.shape == (2 * S, m)
+    np.testing.assert_array_equal(X0[:S, :nev], U)
+    np.testing.assert_array_equal(X0[S:, :nev], V)
+    assert np.all(X0[:, nev:].real >= 0) and np.all(X0[:, nev:].imag == 0)
+    np.testing.assert_array_equal(X0[:S, nev:], X0[S:, nev:])
+    # unit columns, noise 1: the noise is the Box-Muller pair of the stream
+    nrm = np.sqrt(np.sum(np.abs(U) ** 2 + np.abs(V) ** 2, axis=0))
+    Un, Vn = U / nrm, V / nrm
+    X1 = O.reuse_block(Un, Vn, m, 1.0, wf)
+    xr = O.xrand_stream(12345, 2 * S * m).reshape(m, S, 2)
+    u1 = np.maximum(xr[..., 0], 1e-10)
+    rr = np.sqrt(-2 * np.log(u1))
+    z0 = (rr * np.cos(2 * np.pi * xr[..., 1])).T
+    z1 = (rr * np.sin(2 * np.pi * xr[..., 1])).T
+    np.testing.assert_allclose((X1[:S, :nev] - Un).real, z0[:, :nev], atol=1e-13)
+    np.testing.assert_allclose((X1[S:, :nev] - Vn).real, z1[:, :nev], atol=1e-13)
+    np.testing.assert_array_equal(X1[:S, :nev].imag, Un.imag)
+    np.testing.assert_allclose(X1[:S, nev:].real, np.abs(z0[:, nev:]) * np.abs(wf)[:, None],
+                               rtol=1e-14)
+    # the law of the noise over a long column
+    Sl = 100000
+    Ul = np.zeros((Sl, 1), dt)
+    Ul[0, 0] = 1.0
+    Xl = O.reuse_block(Ul, np.zeros((Sl, 1), dt), 1, 1.0)
+    a, b = Xl[1:Sl, 0].real, Xl[Sl + 1:, 0].real
+    assert abs(a.mean()) < 0.02 and abs(b.mean()) < 0.02
+    assert abs(a.var() - 1) < 0.03 and abs(b.var() - 1) < 0.03
+    assert abs(np.mean(a * b)) < 0.02
+
+
 # -------------------------------------------------- eigenvalue pins --
 def _dense_case(tag):
     N, L = tuple(FIX[f"{tag}_N"]), tuple(FIX[f"{tag}_L"])
