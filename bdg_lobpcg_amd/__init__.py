@@ -68,7 +68,7 @@ class SolveStats(C.Structure):
                 ("graph_captures", C.c_uint64), ("seconds_total", C.c_double),
                 ("seconds_rr", C.c_double), ("status", C.c_int),
                 ("frozen_skips", C.c_uint64), ("iterations_fp32", C.c_uint64),
-                ("fp32_worst_residual", C.c_double)]
+                ("fp32_worst_residual", C.c_double), ("fp64_sweep_only", C.c_int)]
 
 
 def build() -> None:
@@ -124,6 +124,9 @@ def lib():
         "bdgx_lp_set_params": (I, [P, P]),
         "bdgx_lp_apply": (I, [P, I, P, P, U64, U64]),
         "bdgx_lp_backward": (I, [P, P, P, U64, U64, P]),
+        "bdgx_lp_apply_layer": (I, [P, I, P, P, U64, U64]),
+        "bdgx_mem_info": (I, [P, P]),
+        "bdgx_lp_backward_layer": (I, [P, I, P, P, U64, U64, P]),
         # public API
         "bdg_alloc": (P, [U64, P, P, I]),
         "bdg_free": (None, [C.POINTER(P)]),
@@ -452,6 +455,27 @@ class DeviceContext:
         check(lib().bdgx_lp_backward(self.h, dR.ptr, dG.ptr, int(ncols), self.size,
                                      g.ctypes.data_as(C.c_void_p)), "bdgx_lp_backward")
         return g
+
+    def lp_apply_layer_device(self, layer, dR, dW, ncols, r_off=0, w_off=0):
+        """dW = T'_layer dR (FP32 columns; offsets in columns)"""
+        check(lib().bdgx_lp_apply_layer(self.h, int(layer), dR.ptr.value + 4 * self.size * r_off,
+                                        dW.ptr.value + 4 * self.size * w_off, int(ncols),
+                                        self.size), "bdgx_lp_apply_layer")
+
+    def lp_backward_layer_device(self, layer, dR, dG, ncols, g_off=0) -> np.ndarray:
+        g = np.zeros(LP_NPARAM, dtype=np.float32)
+        check(lib().bdgx_lp_backward_layer(self.h, int(layer), dR.ptr,
+                                           dG.ptr.value + 4 * self.size * g_off, int(ncols),
+                                           self.size, g.ctypes.data_as(C.c_void_p)),
+              "bdgx_lp_backward_layer")
+        return g
+
+    def lp_apply_layer(self, layer, R: np.ndarray) -> np.ndarray:
+        R = np.asfortranarray(R, dtype=np.float32)
+        dR = self.upload(R)
+        dW = DeviceBuffer(R.nbytes)
+        self.lp_apply_layer_device(layer, dR, dW, R.shape[1])
+        return self.download(dW, R.shape, np.float32)
 
     def lp_apply(self, R: np.ndarray) -> np.ndarray:
         R = np.asfortranarray(R, dtype=np.float32)

@@ -958,6 +958,11 @@ int bdgx_malloc(void **p, size_t bytes) {
   BDGX_HIP(hipMalloc(p, bytes ? bytes : 1));
   return BDGX_OK;
 }
+int bdgx_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return BDGX_EINVAL;
+  BDGX_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return BDGX_OK;
+}
 int bdgx_free(void *p) {
   if (p) BDGX_HIP(hipFree(p));
   return BDGX_OK;
@@ -1174,7 +1179,7 @@ int bdgx_get_launch_variant(const bdgx_ctx *c, int kind, int prec, int *variant)
 
 int bdgx_set_launch_variant(bdgx_ctx *c, int kind, int prec, int variant) {
   if (!c || kind < 0 || kind > 4 || (prec != BDGX_FP64 && prec != BDGX_FP32) || variant < 0 ||
-      variant > (kind == BDGX_TUNE_Z ? 1 : 3))
+      variant > (kind == BDGX_TUNE_Z ? 3 : kind <= BDGX_TUNE_PLANE_INV ? 7 : 3))
     return BDGX_EINVAL;
   c->tune_var[prec == BDGX_FP64 ? 0 : 1][kind] = variant;
   return BDGX_OK;

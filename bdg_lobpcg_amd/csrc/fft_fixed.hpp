@@ -111,10 +111,11 @@ __device__ __forceinline__ void fixed_stage(const SRC &src, const DST &dst, cons
 // stage writes `last` (through `mul`) instead.  Barriers follow every stage.
 // With nst stages the last LDS buffer written before `last` is X when nst is
 // even, Y when odd (for nst == 1 none is).
-template <typename R, bool INV, int N, int P, int NT, int S, class FIRST, class LAST, class MUL>
+template <typename R, bool INV, int N, int P, int NT, int S, bool R16 = kFixedRadix16,
+          class FIRST, class LAST, class MUL>
 __device__ __forceinline__ void fixed_stages(const FIRST &first, const LAST &last, cx<R> *X,
                                              cx<R> *Y, const cx<R> *tw, const MUL &mul) {
-  constexpr FixedPlan pl = make_fixed_plan(N);
+  constexpr FixedPlan pl = make_fixed_plan(N, R16);
   if constexpr (S < pl.nst) {
     constexpr int RAD = pl.rad[S];
     constexpr int NS = pl.ns[S];
@@ -130,7 +131,7 @@ __device__ __forceinline__ void fixed_stages(const FIRST &first, const LAST &las
     else
       fixed_stage<R, INV, N, RAD, NS, P, NT>(in, out, tw, NoMul{});
     __syncthreads();
-    fixed_stages<R, INV, N, P, NT, S + 1>(first, last, X, Y, tw, mul);
+    fixed_stages<R, INV, N, P, NT, S + 1, R16>(first, last, X, Y, tw, mul);
   }
 }
 

@@ -605,7 +605,8 @@ static int learned_applies(const bdg_t *b) {
  * an FP32 phase followed by an FP64 phase from its vectors.  dV0 is the v
  * half of the start block (u = v for the built-in blocks; the reference's
  * start blocks set u = v too, src/solver.c:166-259). */
-static int solve_first_order(bdg_t *b, const void *dU0, const void *dV0, double t_start) {
+static int solve_first_order(bdg_t *b, const void *dU0, const void *dV0, int learned,
+                             double t_start) {
   const uint64_t S = b->size, m = b->sizeSub, nev = b->nev;
   const int c = b->complex_psi0;
   const size_t es64 = sizeof(double) * (c ? 2 : 1);
@@ -622,6 +623,11 @@ static int solve_first_order(bdg_t *b, const void *dU0, const void *dV0, double 
   /* automatic locking: once at most max(2, m / 8) columns are active */
   cfg.lock_max = b->lock_max >= 0 ? b->lock_max : (int)(m / 8 > 2 ? m / 8 : 2);
   cfg.lock_guards = b->lock_guards;
+  /* the learned spectral layers replace precondK / precondM in the FP32
+   * phase (mixed precision only; the FP64 phase keeps the analytic ones) */
+  cfg.learned = learned;
+  cfg.lp_window = b->lp_window;
+  cfg.lp_factor = b->lp_factor;
   lrep_result r32, r64;
   memset(&r32, 0, sizeof(r32));
   memset(&r64, 0, sizeof(r64));
@@ -640,6 +646,8 @@ static int solve_first_order(bdg_t *b, const void *dU0, const void *dV0, double 
     b->stats.iterations_fp32 = (uint64_t)it32;
     b->stats.fp32_worst_residual = r32.worst;
     b->stats.column_applies_fp32 = r32.applies;
+    b->stats.learned_iterations = (uint64_t)r32.learned_its;
+    b->stats.learned_fallback_at = r32.learned_fallback_at;
     if (b->verbosity > 0)
       fprintf(stderr, "[bdg] fp32 phase: %d iterations, worst residual %.3e, status %d\n", it32,
               r32.worst, rc);
@@ -792,8 +800,8 @@ int bdg_solve(bdg_t *b) {
     b->stats.status = rc;
     return rc;
   }
-  if (b->formulation == BDG_FORM_FIRST_ORDER && !k_equals_m(b) && !learned) {
-    rc = solve_first_order(b, dU0 ? dU0 : dV0, dV0, t_start);
+  if (b->formulation == BDG_FORM_FIRST_ORDER && !k_equals_m(b)) {
+    rc = solve_first_order(b, dU0 ? dU0 : dV0, dV0, learned, t_start);
     bdgx_free(dV0);
     bdgx_free(dU0);
     return rc;

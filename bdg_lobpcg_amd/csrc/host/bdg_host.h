@@ -175,6 +175,11 @@ typedef struct {
                        /*   block after locking carries as guard columns        */
   const void *dUL, *dVL; /* locked pairs (S x nl, phase precision): new         */
   int nl;                /*   directions are deflated against them              */
+  int learned;           /* FP32 phase: W_U = T'_K R_U, W_V = T'_M R_V from the   */
+                         /*   learned spectral layers (bdgx_lp_apply_layer)     */
+  int lp_window;         /* ... handed back to precondK / precondM for the rest  */
+  double lp_factor;      /*   of the solve once the worst residual is above      */
+                         /*   lp_factor x its value lp_window iterations earlier */
 } lrep_cfg;
 
 typedef struct {
@@ -185,6 +190,8 @@ typedef struct {
   double rr_seconds;
   double worst;
   int status;
+  int learned_its;         /* iterations whose W came from the learned layers */
+  int learned_fallback_at; /* iteration of the hand-back, or 0                */
 } lrep_result;
 
 /* One phase at storage precision prec, from the blocks dXU0, dXV0 (S x m,

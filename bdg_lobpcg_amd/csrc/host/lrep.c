@@ -338,7 +338,9 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   const int m = cfg->m, nev = cfg->nev, cplx = cfg->cplx;
   const int w = cplx ? 2 : 1;
   const size_t es = lr_es(prec, cplx);
-  const int kmax = 3 * m;
+  /* basis [X | P | W]; a phase of no iterations (the refinement sweep:
+   * Rayleigh-Ritz of the start block and its residuals) holds X only */
+  const int kmax = cfg->maxiter > 0 ? 3 * m : m;
   const int opT[2] = {BDGX_OP_PRECOND_K, BDGX_OP_PRECOND_M};
   const int opA[2] = {BDGX_OP_K, BDGX_OP_M};
   /* full device Gram recompute period (FP32: the projected Grams of rounded
@@ -387,6 +389,10 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   res->iters = 0;
   res->status = 0;
   for (int j = 0; j < m; ++j) best[j] = HUGE_VAL;
+  /* learned per-half preconditioner in the FP32 phase, with its stall rule
+   * over the worst residual of the wanted pairs */
+  int lp_on = cfg->learned && prec == BDGX_FP32;
+  double *whist = lp_on ? (double *)calloc((size_t)cfg->maxiter + 2, sizeof(double)) : NULL;
 
   TRY(bdgx_autotune(ctx, prec, (uint64_t)(m < 32 ? m : 32), 0));
   for (int h = 0; h < 2; ++h) {
@@ -654,6 +660,17 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
               w2[0] < 0 ? -sqrt(-w2[0]) : sqrt(w2[0]),
               w2[nev - 1] < 0 ? -sqrt(-w2[nev - 1]) : sqrt(w2[nev - 1]), worst, nact);
     res->worst = worst;
+    if (lp_on) {
+      whist[it] = worst;
+      if (cfg->lp_window > 0 && it >= cfg->lp_window &&
+          worst > cfg->lp_factor * whist[it - cfg->lp_window]) {
+        lp_on = 0;
+        res->learned_fallback_at = it;
+        if (cfg->verbosity > 0)
+          fprintf(stderr, "[bdg] fp32 it %d: learned preconditioner stalled, precondK/M from here\n",
+                  it);
+      }
+    }
     if (!allc && cfg->lock_max > 0 && cfg->nl == 0 && nact <= cfg->lock_max && nact < m &&
         it < cfg->maxiter) {
       /* lock the converged pairs; the active ones continue in a nested
@@ -721,6 +738,7 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
       icfg.dUL = dUL;
       icfg.dVL = dVL;
       icfg.nl = nc;
+      icfg.learned = lp_on;
       lrep_result ires;
       memset(&ires, 0, sizeof(ires));
       double *iw2 = (double *)calloc((size_t)mi, sizeof(double));
@@ -732,6 +750,9 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
                 prec == BDGX_FP64 ? "fp64" : "fp32", it, nc, nact, ng);
       rc = bdg_lrep_device(ctx, &icfg, prec, dU0, dV0, prec, &ires, dU0, dV0);
       res->applies += ires.applies;
+      res->learned_its += ires.learned_its;
+      if (ires.learned_fallback_at && !res->learned_fallback_at)
+        res->learned_fallback_at = it + 1 + ires.learned_fallback_at;
       trr += ires.rr_seconds;
       res->iters = it + 1 + ires.iters;
       if (!rc) {
@@ -818,8 +839,10 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
         wsrc[h] = dWt[h];
         wtmp[h] = dR[h];
       }
-      TRY(bdgx_apply(ctx, opT[h], prec, wsrc[h], wtmp[h], (uint64_t)nact, S));
+      TRY(lp_on ? bdgx_lp_apply_layer(ctx, h, wsrc[h], wtmp[h], (uint64_t)nact, S)
+                : bdgx_apply(ctx, opT[h], prec, wsrc[h], wtmp[h], (uint64_t)nact, S));
     }
+    if (lp_on) ++res->learned_its;
     const int nl = cfg->nl;
     if (nl > 0) {
       /* W into the deflated subspace (the residual in wsrc is consumed) */
@@ -943,6 +966,7 @@ done:
   bdgx_free(dC);
   bdgx_free(dZero);
   bdgx_free(dN2);
+  free(whist);
   free(hGK);
   free(hGM);
   free(hGW);

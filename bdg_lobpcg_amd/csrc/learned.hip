@@ -603,4 +603,71 @@ int bdgx_lp_backward(bdgx_ctx *c, const float *dR, const float *dGW, uint64_t nc
   return BDGX_OK;
 }
 
+// One spectral layer on its own (layer 0: T'_K, 1: T'_M): the first-order
+// solver's per-half preconditioner W_U = T'_K R_U, W_V = T'_M R_V.
+int bdgx_lp_apply_layer(bdgx_ctx *c, int layer, const void *dR, void *dW, uint64_t ncols,
+                        uint64_t ld) {
+  if (!c || (layer != 0 && layer != 1) || c->cplx || !dR || !dW || dR == dW ||
+      ld != (uint64_t)c->geo.S)
+    return BDGX_EINVAL;
+  if (!c->has_lp) return BDGX_ESTATE;
+  if (ncols == 0) return BDGX_OK;
+  return lp_filter_f32(c, layer, 0, reinterpret_cast<const float *>(dR),
+                       reinterpret_cast<float *>(dW), (long long)ncols, c->geo.S);
+}
+
+// Parameter gradient of sum <dGW, T'_X R> for layer X alone (the K-layer
+// part of bdgx_lp_backward with the layer's own offsets): alpha_X from
+// dGW . W, beta_X from R . b_X C_X(a_X dGW), kappa_X from the spectra of
+// (dGW, R).  Entries of the other layer and of the stencil are zero.
+int bdgx_lp_backward_layer(bdgx_ctx *c, int layer, const float *dR, const float *dGW,
+                           uint64_t ncols, uint64_t ld, float *h_grad) {
+  if (!c || (layer != 0 && layer != 1) || !dR || !dGW || !h_grad || c->cplx ||
+      ld != (uint64_t)c->geo.S)
+    return BDGX_EINVAL;
+  if (!c->has_lp) return BDGX_ESTATE;
+  for (int i = 0; i < BDGX_LP_NPARAM; ++i) h_grad[i] = 0.f;
+  if (ncols == 0) return BDGX_OK;
+  const long long S = c->geo.S, n = (long long)ncols;
+  Fields<float> &F = c->f32;
+  // W, T (S x n each), X2 (2S x n)
+  int rc = ensure_lp_tmp(c, sizeof(float) * (size_t)S * n * 4);
+  if (rc) return rc;
+  float *W = reinterpret_cast<float *>(c->lp_tmp);
+  float *T = W + S * n, *X2 = T + S * n;
+  rc = ensure_scratch(c, sizeof(double) * BDGX_LP_NPARAM);
+  if (rc) return rc;
+  double *dg = reinterpret_cast<double *>(c->scratch);
+  BDGX_HIP(hipMemsetAsync(dg, 0, sizeof(double) * BDGX_LP_NPARAM, c->stream));
+  const unsigned gb = grid_for(S * n);
+  const hipStream_t st = c->stream;
+  const int oa = layer == 0 ? 0 : 4, ob = oa + 2, ok = 8 + layer * LPK;
+  rc = lp_filter_f32(c, layer, 0, dR, W, n, S);
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, st, dGW, (const float *)W,
+                       (const float *)nullptr, c->lp_rho, S, (int)n, S, dg + oa);
+    rc = lp_filter_f32(c, layer, 1, dGW, T, n, S);
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_pointgrad, dim3(gb), dim3(256), 0, st, dR, (const float *)T,
+                       (const float *)F.lpb[layer], c->lp_rho, S, (int)n, S, dg + ob);
+    hipLaunchKernelGGL(k_lp_pack2, dim3(gb), dim3(256), 0, st, dGW, dR, S, (int)n, S, X2);
+    rc = lp_spectra_f32(c, layer, X2, n, 2 * S);
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_lp_knotgrad, dim3(grid_for(c->geo.Kk * n)), dim3(256), 0, st,
+                       reinterpret_cast<const cx<float> *>(c->spec), (int)n, c->geo, F.kx2[0],
+                       F.kx2[1], F.kx2[2], lp_kmax(c), (const float *)F.lpg[layer], dg + ok);
+    if (hipGetLastError() != hipSuccess) rc = BDGX_EHIP;
+  }
+  std::vector<double> hg(BDGX_LP_NPARAM, 0.0);
+  if (!rc && hipMemcpyAsync(hg.data(), dg, sizeof(double) * BDGX_LP_NPARAM, hipMemcpyDeviceToHost,
+                            st) != hipSuccess)
+    rc = BDGX_EHIP;
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = BDGX_EHIP;
+  if (rc) return rc;
+  for (int i = 0; i < BDGX_LP_NPARAM; ++i) h_grad[i] = (float)hg[i];
+  return BDGX_OK;
+}
+
 }  // extern "C"

@@ -945,7 +945,10 @@ static int autotune_t(bdgx_ctx *c, uint64_t nbatch) {
     };
     int best = c->tune_var[pi][kind];
     float best_ms = 0.f;
-    const int nvar = kind == BDGX_TUNE_Z ? 2 : 4;
+    const int nvar = kind == BDGX_TUNE_Z ? zpass_variants(c)
+                     : (kind == BDGX_TUNE_PLANE_FWD || kind == BDGX_TUNE_PLANE_INV)
+                         ? plane_variants(c)
+                         : 4;
     for (int v = 0; v < nvar && !rc; ++v) {
       c->tune_var[pi][kind] = v;
       rc = one();

@@ -268,7 +268,7 @@ constexpr bool plane_inv_r16() {
 // registers: more workgroups per CU where the LDS allows it).  Each channel
 // reads its input values in groups of LG per thread, a group's loads all in
 // flight together.
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool R16>
 __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
   constexpr int SW = plane_swz<R, N>();
   constexpr int EP = N * N / 2 / T;  // row-pair values per thread
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
         buf[swz<SW>(tid + (b0 + i) * T)] = cmk<R>(x0[i] * q0[i], x1[i] * q1[i]);
     }
     __syncthreads();
-    plane_fwd_transform<R, N, T>(a, buf, twx, twy, c, col, z, tid);
+    plane_fwd_transform<R, N, T, R16>(a, buf, twx, twy, c, col, z, tid);
   }
 }
 
@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_real(XArgs<R> a) {
 // + gridDim.x, ... and loads the next plane's values into registers while the
 // current plane is transformed in LDS, so that its HBM reads overlap the LDS
 // work.
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool R16>
 __global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes) {
   constexpr int NK = N / 2 + 1;
   constexpr int SW = plane_swz<R, N>();
@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_persist(XArgs<R> a, int nplanes
       // the registers are free after the last channel's LDS write: fetch
       // the next plane now, its loads in flight during the transform
       if (c == a.nch - 1 && p + (int)gridDim.x < nplanes) load(p + gridDim.x);
-      plane_fwd_transform<R, N, T>(a, buf, twx, twy, c, col, z, tid);
+      plane_fwd_transform<R, N, T, R16>(a, buf, twx, twy, c, col, z, tid);
     }
   }
 }
@@ -562,7 +562,7 @@ __device__ __forceinline__ void plane_inv_store(const XArgs<R> &a, int col, long
 // One plane per workgroup, nothing held across the transform but the
 // output accumulators; each channel's spectrum plane is read in groups of
 // LG values per thread, a group's loads all in flight together.
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool R16>
 __global__ void __launch_bounds__(T, (plane_waves_per_simd<R, N, T>())) k_plane_inv_real(XArgs<R> a) {
   constexpr int NK = N / 2 + 1;
   constexpr int EPT = (N * N / 2 + T - 1) / T;
@@ -607,12 +607,12 @@ __global__ void __launch_bounds__(T, (plane_waves_per_simd<R, N, T>())) k_plane_
         const int e = tid + (b0 + i) * T;
         if (b0 + i < ES && (N * NK % T == 0 || b0 + i < ES - 1 || e < N * NK)) {
           const int y = e / NK, k = e % NK;
-          buf[dif_pos<N, plane_inv_r16<R, N, T, false>()>(y) * NK + k] = v[i];
+          buf[dif_pos<N, R16>(y) * NK + k] = v[i];
         }
       }
     }
     __syncthreads();
-    plane_inv_transform<R, N, T, EPT, LG, plane_inv_r16<R, N, T, false>()>(a, buf, twx, twy, c,
+    plane_inv_transform<R, N, T, EPT, LG, R16>(a, buf, twx, twy, c,
                                                                            plane0, acc0, acc1, tid);
   }
   plane_inv_store<R, N, T, EPT, LG>(a, col, plane0, acc0, acc1, threadIdx.x);
@@ -623,7 +623,7 @@ __global__ void __launch_bounds__(T, (plane_waves_per_simd<R, N, T>())) k_plane_
 // LDS (all of a thread's loads in flight at once); the next plane's first
 // channel is loaded while the last channel of the current plane is
 // transformed, so those HBM reads overlap the LDS work.
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool R16>
 __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes) {
   constexpr int NK = N / 2 + 1;
   constexpr int EPT = (N * N / 2 + T - 1) / T;
@@ -688,7 +688,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
         const int e = tid + b * T;
         if (N * NK % T == 0 || b < ES - 1 || e < N * NK) {
           const int y = e / NK, k = e % NK;
-          buf[dif_pos<N, plane_inv_r16<R, N, T, true>()>(y) * NK + k] = cmk<R>(sre[b], sim[b]);
+          buf[dif_pos<N, R16>(y) * NK + k] = cmk<R>(sre[b], sim[b]);
         }
       }
       __syncthreads();
@@ -707,7 +707,7 @@ __global__ void __launch_bounds__(T) k_plane_inv_persist(XArgs<R> a, int nplanes
           }
         }
       }
-      plane_inv_transform<R, N, T, EPT, WG, plane_inv_r16<R, N, T, true>()>(a, buf, twx, twy, c,
+      plane_inv_transform<R, N, T, EPT, WG, R16>(a, buf, twx, twy, c,
                                                                           plane0, acc0, acc1, tid);
     }
     plane_inv_store<R, N, T, EPT, SG>(a, col, plane0, acc0, acc1, threadIdx.x);
@@ -772,7 +772,7 @@ __host__ __device__ __forceinline__ int z_tiles_per_batch(int n0k, int ny) {
   return (n0k * ny + P - 1) / P;
 }
 
-template <typename R, int NZ, int P, int T>
+template <typename R, int NZ, int P, int T, bool R16>
 __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
   __shared__ cx<R> tw[NZ];
   __shared__ cx<R> bufA[NZ * P];
@@ -815,19 +815,19 @@ __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
   cx<R> *base = a.spec + (long long)b * a.g.Kk + f0;
   GlobIO<R> g{base, (long long)npen, valid, 1};
   ZMul<R> mul{&a, code, valid, kxy2, kz2};
-  constexpr int nst = make_fixed_plan(NZ).nst;
+  constexpr int nst = make_fixed_plan(NZ, R16).nst;
   cx<R> *F = (nst % 2 == 1) ? bufA : bufB;  // last LDS buffer of the forward
   cx<R> *O = (F == bufA) ? bufB : bufA;
   if (a.fwd_only) {
-    fixed_stages<R, false, NZ, P, T, 0>(g, g, bufA, bufB, tw, NoMul{});
+    fixed_stages<R, false, NZ, P, T, 0, R16>(g, g, bufA, bufB, tw, NoMul{});
     return;
   }
   // forward: global -> A -> B ... ; the last stage stores into the buffer
   // it does not read (F)
   if constexpr (nst == 1)
-    fixed_stages<R, false, NZ, P, T, 0>(g, LdsIO<R>{bufA}, bufA, bufB, tw, mul);
+    fixed_stages<R, false, NZ, P, T, 0, R16>(g, LdsIO<R>{bufA}, bufA, bufB, tw, mul);
   else
-    fixed_stages<R, false, NZ, P, T, 0>(g, LdsIO<R>{F}, bufA, bufB, tw, mul);
+    fixed_stages<R, false, NZ, P, T, 0, R16>(g, LdsIO<R>{F}, bufA, bufB, tw, mul);
   if (mul_is_table(code)) {
 #pragma unroll
     for (int q = 0; q < EU; ++q) {
@@ -838,9 +838,9 @@ __global__ void __launch_bounds__(T) k_zpass_fixed(YZArgs<R> a) {
   }
   // inverse: F -> O -> F ... -> global
   if constexpr (nst == 1)
-    fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{bufA}, g, bufB, bufA, tw, NoMul{});
+    fixed_stages<R, true, NZ, P, T, 0, R16>(LdsIO<R>{bufA}, g, bufB, bufA, tw, NoMul{});
   else
-    fixed_stages<R, true, NZ, P, T, 0>(LdsIO<R>{F}, g, O, F, tw, NoMul{});
+    fixed_stages<R, true, NZ, P, T, 0, R16>(LdsIO<R>{F}, g, O, F, tw, NoMul{});
 }
 
 template <typename R>
@@ -849,14 +849,36 @@ bool zpass_fixed_supported(const bdgx_ctx *c) {
   return nz == 32 || nz == 48 || nz == 64 || nz == 80 || nz == 96 || nz == 128 || nz == 192;
 }
 
-// Launch shapes of the fixed z pass: variant 0 = (P, T) with T = 256 at
-// NZ = 128 and 128 below; variant 1 halves T (two butterflies per thread in
-// the radix-8 stages).  P = 8 (FP64) or 16 (FP32) adjacent kx pencils: one
-// 128-byte segment per z row.
-template <typename R, int NZ, int T>
+// Radix plans of the fixed kernels: a size whose plan ends in 8 x ... x 2
+// (N = 128) also has the plan with the 2 merged into a radix-16 first stage
+// (one LDS round trip fewer, 16 values per butterfly in registers).  Both are
+// compiled where they differ and the autotuner times both.
+constexpr bool has_alt_plan(int n) {
+  return make_fixed_plan(n, true).nst != make_fixed_plan(n, false).nst;
+}
+
+// Launch shapes of the fixed z pass: variant bit 0 halves T (T = 256 at
+// NZ >= 128, 128 below: two butterflies per thread in the radix-8 stages),
+// bit 1 takes the other radix plan (where NZ has one).  P = 8 (FP64) or 16
+// (FP32) adjacent kx pencils: one 128-byte segment per z row.
+template <typename R, int NZ, int T, bool R16>
 static void launch_z_nt(const YZArgs<R> &z, dim3 grid, hipStream_t s) {
   constexpr int P = sizeof(R) == 8 ? 8 : 16;
-  hipLaunchKernelGGL((k_zpass_fixed<R, NZ, P, T>), grid, dim3(T), 0, s, z);
+  hipLaunchKernelGGL((k_zpass_fixed<R, NZ, P, T, R16>), grid, dim3(T), 0, s, z);
+}
+
+template <typename R, int NZ, int TB>
+static void launch_z_v(const YZArgs<R> &z, dim3 grid, int v, hipStream_t s) {
+  constexpr bool D = kFixedRadix16;
+  if constexpr (has_alt_plan(NZ)) {
+    if (v & 2) {
+      if (v & 1) launch_z_nt<R, NZ, TB / 2, !D>(z, grid, s);
+      else launch_z_nt<R, NZ, TB, !D>(z, grid, s);
+      return;
+    }
+  }
+  if (v & 1) launch_z_nt<R, NZ, TB / 2, D>(z, grid, s);
+  else launch_z_nt<R, NZ, TB, D>(z, grid, s);
 }
 
 template <typename R>
@@ -867,34 +889,13 @@ int launch_zpass_fixed(bdgx_ctx *c, const YZArgs<R> &z) {
   const dim3 grid((unsigned)(z_tiles_per_batch<P>(g.n0k, g.ny) * z.nb));
   const int v = c->tune_var[sizeof(R) == 8 ? 0 : 1][BDGX_TUNE_Z];
   switch (g.nz) {
-    case 32:
-      if (v) launch_z_nt<R, 32, 64>(z, grid, c->stream);
-      else launch_z_nt<R, 32, 128>(z, grid, c->stream);
-      break;
-    case 64:
-      if (v) launch_z_nt<R, 64, 64>(z, grid, c->stream);
-      else launch_z_nt<R, 64, 128>(z, grid, c->stream);
-      break;
-    case 48:
-      if (v) launch_z_nt<R, 48, 64>(z, grid, c->stream);
-      else launch_z_nt<R, 48, 128>(z, grid, c->stream);
-      break;
-    case 80:
-      if (v) launch_z_nt<R, 80, 64>(z, grid, c->stream);
-      else launch_z_nt<R, 80, 128>(z, grid, c->stream);
-      break;
-    case 96:
-      if (v) launch_z_nt<R, 96, 64>(z, grid, c->stream);
-      else launch_z_nt<R, 96, 128>(z, grid, c->stream);
-      break;
-    case 128:
-      if (v) launch_z_nt<R, 128, 128>(z, grid, c->stream);
-      else launch_z_nt<R, 128, 256>(z, grid, c->stream);
-      break;
-    case 192:
-      if (v) launch_z_nt<R, 192, 128>(z, grid, c->stream);
-      else launch_z_nt<R, 192, 256>(z, grid, c->stream);
-      break;
+    case 32: launch_z_v<R, 32, 128>(z, grid, v, c->stream); break;
+    case 48: launch_z_v<R, 48, 128>(z, grid, v, c->stream); break;
+    case 64: launch_z_v<R, 64, 128>(z, grid, v, c->stream); break;
+    case 80: launch_z_v<R, 80, 128>(z, grid, v, c->stream); break;
+    case 96: launch_z_v<R, 96, 128>(z, grid, v, c->stream); break;
+    case 128: launch_z_v<R, 128, 256>(z, grid, v, c->stream); break;
+    case 192: launch_z_v<R, 192, 256>(z, grid, v, c->stream); break;
     default: return BDGX_EINVAL;
   }
   BDGX_HIP(hipGetLastError());
@@ -1120,13 +1121,13 @@ static void launch_cplane_v(const XArgs<R> &a, bool inverse, int v, hipStream_t 
   }
 }
 
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool R16>
 static void launch_plane(const XArgs<R> &a, bool inverse, hipStream_t s) {
   const unsigned grid = (unsigned)(a.g.nz * a.ncols);
   if (inverse)
-    hipLaunchKernelGGL((k_plane_inv_real<R, N, T>), dim3(grid), dim3(T), 0, s, a);
+    hipLaunchKernelGGL((k_plane_inv_real<R, N, T, R16>), dim3(grid), dim3(T), 0, s, a);
   else
-    hipLaunchKernelGGL((k_plane_fwd_real<R, N, T>), dim3(grid), dim3(T), 0, s, a);
+    hipLaunchKernelGGL((k_plane_fwd_real<R, N, T, R16>), dim3(grid), dim3(T), 0, s, a);
 }
 
 template <typename R>
@@ -1139,16 +1140,18 @@ bool plane_supported(const bdgx_ctx *c) {
   return g.nx == 32 || g.nx == 48 || g.nx == 64 || g.nx == 80 || g.nx == 96 || g.nx == 128;
 }
 
-// Launch shapes of the whole-plane kernels: variants 0 / 1 = the one-plane
-// kernels with 4N / 8N threads, 2 / 3 = the persistent kernels with as many
-// workgroups as fit on the device at once (each then walks several planes),
-// 4N / 8N threads.
-template <typename R, int N, int T, bool INV>
+// Launch shapes of the whole-plane kernels: variant bits 0-1 = the one-plane
+// kernels with 4N / 8N threads (0 / 1) or the persistent kernels with as
+// many workgroups as fit on the device at once, each then walking several
+// planes, 4N / 8N threads (2 / 3); bit 2 = the other radix plan (where N has
+// one; the default plan is radix 16 first except in the FP64 persistent
+// inverse, which spills with it).
+template <typename R, int N, int T, bool INV, bool R16>
 static void launch_plane_persist(const bdgx_ctx *c, const XArgs<R> &a, hipStream_t s) {
   static const int per_cu = [] {
     int nb = 0;
-    const void *fn = INV ? reinterpret_cast<const void *>(&k_plane_inv_persist<R, N, T>)
-                         : reinterpret_cast<const void *>(&k_plane_fwd_persist<R, N, T>);
+    const void *fn = INV ? reinterpret_cast<const void *>(&k_plane_inv_persist<R, N, T, R16>)
+                         : reinterpret_cast<const void *>(&k_plane_fwd_persist<R, N, T, R16>);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, T, 0) != hipSuccess || nb < 1)
       nb = 1;
     return nb;
@@ -1158,27 +1161,51 @@ static void launch_plane_persist(const bdgx_ctx *c, const XArgs<R> &a, hipStream
   if (grid > nplanes) grid = nplanes;
   if (grid < 1) grid = 1;
   if (INV)
-    hipLaunchKernelGGL((k_plane_inv_persist<R, N, T>), dim3(grid), dim3(T), 0, s, a, nplanes);
+    hipLaunchKernelGGL((k_plane_inv_persist<R, N, T, R16>), dim3(grid), dim3(T), 0, s, a,
+                       nplanes);
   else
-    hipLaunchKernelGGL((k_plane_fwd_persist<R, N, T>), dim3(grid), dim3(T), 0, s, a, nplanes);
+    hipLaunchKernelGGL((k_plane_fwd_persist<R, N, T, R16>), dim3(grid), dim3(T), 0, s, a,
+                       nplanes);
 }
 
-// variants: 0 = 4N threads, 1 = 8N threads, 2 / 3 = persistent with 4N / 8N
+// default radix plan of each plane kernel, and the launch of shape v & 3
+// with the default plan (ALT = false) or the other one
+template <typename R, int N, int T, bool INV, bool PERSIST>
+constexpr bool plane_default_r16() {
+  return INV ? plane_inv_r16<R, N, T, PERSIST>() : kFixedRadix16;
+}
+template <typename R, int N, bool INV, bool ALT>
+static void launch_plane_shape(const bdgx_ctx *c, const XArgs<R> &a, int shape, hipStream_t s) {
+  switch (shape) {
+    case 2:
+      launch_plane_persist<R, N, 4 * N, INV, plane_default_r16<R, N, 4 * N, INV, true>() != ALT>(
+          c, a, s);
+      break;
+    case 3:
+      launch_plane_persist<R, N, 8 * N, INV, plane_default_r16<R, N, 8 * N, INV, true>() != ALT>(
+          c, a, s);
+      break;
+    case 1:
+      launch_plane<R, N, 8 * N, plane_default_r16<R, N, 8 * N, INV, false>() != ALT>(a, INV, s);
+      break;
+    default:
+      launch_plane<R, N, 4 * N, plane_default_r16<R, N, 4 * N, INV, false>() != ALT>(a, INV, s);
+  }
+}
+
 template <typename R, int N>
 static void launch_plane_v(const bdgx_ctx *c, const XArgs<R> &a, bool inverse, int v,
                            hipStream_t s) {
-  if (v == 2 && inverse)
-    launch_plane_persist<R, N, 4 * N, true>(c, a, s);
-  else if (v == 3 && inverse)
-    launch_plane_persist<R, N, 8 * N, true>(c, a, s);
-  else if (v == 2)
-    launch_plane_persist<R, N, 4 * N, false>(c, a, s);
-  else if (v == 3)
-    launch_plane_persist<R, N, 8 * N, false>(c, a, s);
-  else if (v)
-    launch_plane<R, N, 8 * N>(a, inverse, s);
-  else
-    launch_plane<R, N, 4 * N>(a, inverse, s);
+  const int shape = v & 3;
+  if constexpr (has_alt_plan(N)) {
+    if (v & 4) {
+      if (inverse) launch_plane_shape<R, N, true, true>(c, a, shape, s);
+      else launch_plane_shape<R, N, false, true>(c, a, shape, s);
+      return;
+    }
+  }
+  if (inverse) launch_plane_shape<R, N, true, false>(c, a, shape, s);
+  else launch_plane_shape<R, N, false, false>(c, a, shape, s);
 }
 
 template <typename R>
@@ -1230,6 +1257,13 @@ void tune_defaults(bdgx_ctx *c) {
     c->tuned[p] = 0;
   }
 }
+
+// number of launch variants the autotuner times: shapes x radix plans
+int plane_variants(const bdgx_ctx *c) {
+  if (c->cplx) return 2;
+  return has_alt_plan(c->geo.nx) ? 8 : 4;
+}
+int zpass_variants(const bdgx_ctx *c) { return has_alt_plan(c->geo.nz) ? 4 : 2; }
 
 template bool plane_supported<float>(const bdgx_ctx *);
 template bool plane_supported<double>(const bdgx_ctx *);

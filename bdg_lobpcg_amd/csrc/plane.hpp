@@ -57,5 +57,8 @@ template <typename R>
 int launch_zpass_fixed(bdgx_ctx *c, const YZArgs<R> &z);
 // measured default launch variants of the plane and fixed z kernels
 void tune_defaults(bdgx_ctx *c);
+// launch variants (shapes x radix plans) of the plane / fixed z kernels
+int plane_variants(const bdgx_ctx *c);
+int zpass_variants(const bdgx_ctx *c);
 
 }  // namespace bdgx

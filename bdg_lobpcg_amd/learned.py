@@ -217,3 +217,148 @@ def collect(make_problem, nev, snapshots=(4, 8, 14, 22, 32), log=None):
         if log:
             log(f"snapshot after {k} iterations")
     return ref, d, snaps
+
+
+# ---------------------------------------------------------------------------
+# First-order form: one spectral layer per half.  The first-order solver
+# (K u = w v, M v = w u, separate U and V search spaces) takes its new
+# directions W_U = T'_K R_U, W_V = T'_M R_V; the pairs of a snapshot are,
+# per half X in {U, V} (columns of the Ritz block, [u; v] of unit norm):
+#
+#     R_U = K u - |w| v,   R_V = M v - w u          (the solver's residuals)
+#     E_X = (I - Q_X Q_X^T) Pi_X x                  (x = u or v)
+#
+# Q_X an orthonormal basis of the snapshot's X block, Pi_X the projector on
+# the converged X block.  Loss: the mean over both halves of the cosine loss
+# above, each half through its own layer.
+
+class SnapshotFO:
+    """Per half: [Q | W | E | W_p] (4m FP32 columns) and R (m FP32 columns)."""
+
+    def __init__(self, S, m):
+        self.m = m
+        self.B4 = [DeviceBuffer(4 * 4 * S * m) for _ in range(2)]
+        self.R = [DeviceBuffer(4 * S * m) for _ in range(2)]
+
+
+def _pack2(d, A, B, m):
+    """[A | B] (S x 2m FP64) from two FP64 S x m blocks."""
+    S = d.size
+    out = DeviceBuffer(8 * S * 2 * m)
+    check(lib().bdgx_convert(d.h, FP64, A.ptr, S, FP64, out.ptr, S, S, m), "convert")
+    check(lib().bdgx_convert(d.h, FP64, B.ptr, S, FP64, out.ptr.value + 8 * S * m, S, S, m),
+          "convert")
+    return out
+
+
+def make_snapshot_fo(d, U, V, w, Ustar, Vstar) -> SnapshotFO:
+    """U, V, w: a (truncated) first-order solve's modes (M v = w u) and
+    frequencies; Ustar, Vstar: the converged modes.  Builds R, E, Q of both
+    halves on the device."""
+    dev = _Dev(d)
+    S, m = U.shape
+    n = Ustar.shape[1]
+    dU, dV = d.upload(np.asfortranarray(U)), d.upload(np.asfortranarray(V))
+    dKU, dMV = DeviceBuffer(8 * S * m), DeviceBuffer(8 * S * m)
+    dev.apply("K", FP64, dU, dKU, m)
+    dev.apply("M", FP64, dV, dMV, m)
+    aw = np.abs(np.asarray(w, dtype=np.float64))
+    snap = SnapshotFO(S, m)
+    for h, (dX, dImg, dOther, coef, Xs) in enumerate(
+            ((dU, dKU, dV, aw, Ustar), (dV, dMV, dU, np.asarray(w, np.float64), Vstar))):
+        dR64 = DeviceBuffer(8 * S * m)
+        dev.combine(FP64, _pack2(d, dImg, dOther, m), 2 * m, np.vstack([np.eye(m), -np.diag(coef)]),
+                    dR64)
+        G = dev.gram(FP64, dX, m, dX, m)
+        dQ = DeviceBuffer(8 * S * m)
+        dev.combine(FP64, dX, m, _chol_inv_t(G), dQ)
+        dXs = d.upload(np.asfortranarray(Xs))
+        Gs = dev.gram(FP64, dXs, n, dXs, n)
+        Z = dev.gram(FP64, dXs, n, dX, m)
+        dPi = DeviceBuffer(8 * S * m)
+        dev.combine(FP64, dXs, n, np.linalg.solve(Gs, Z), dPi)
+        QtP = dev.gram(FP64, dQ, m, dPi, m)
+        dE64 = DeviceBuffer(8 * S * m)
+        dev.combine(FP64, _pack2(d, dQ, dPi, m), 2 * m, np.vstack([-QtP, np.eye(m)]), dE64)
+        dev.convert(FP64, dQ, FP32, snap.B4[h], m, 0, 0)
+        dev.convert(FP64, dE64, FP32, snap.B4[h], m, 0, 2 * m)
+        dev.convert(FP64, dR64, FP32, snap.R[h], m)
+    d.sync()
+    return snap
+
+
+def loss_and_grad_fo(d, theta, snap: SnapshotFO, weighted=False):
+    """Mean cosine loss of the two projected layer outputs and its gradient.
+    weighted: columns weighted by their share of the snapshot's squared error
+    |E_j|^2 / sum |E|^2 instead of 1 / m (converged columns, whose E is
+    rounding noise, then carry no weight)."""
+    dev = _Dev(d)
+    S, m = d.size, snap.m
+    d.lp_set_params(theta)
+    loss, grad = 0.0, np.zeros(LP_NPARAM)
+    for h in (0, 1):
+        B4, R = snap.B4[h], snap.R[h]
+        d.lp_apply_layer_device(h, R, B4, m, 0, m)
+        QtW = dev.gram(FP32, B4, m, B4, m, 0, m)
+        dev.combine(FP32, B4, 2 * m, np.vstack([-QtW, np.eye(m)]), B4, 0, 3 * m)
+        EW = dev.gram(FP32, B4, 2 * m, B4, 2 * m, 2 * m, 2 * m)
+        ee = np.maximum(np.diag(EW)[:m], 1e-300)
+        ww = np.maximum(np.diag(EW)[m:], 1e-300)
+        c = np.diag(EW[:m, m:])
+        wt = ee / ee.sum() if weighted else np.full(m, 1.0 / m)
+        loss += 0.5 * float(np.sum(wt * (1.0 - c * c / (ww * ee))))
+        alpha = 0.5 * wt * (-2.0 * c / (ww * ee))
+        beta = 0.5 * wt * (2.0 * c * c / (ww * ww * ee))
+        dev.combine(FP32, B4, 2 * m, np.vstack([np.diag(alpha), np.diag(beta)]), B4, 2 * m, m)
+        QtD = dev.gram(FP32, B4, m, B4, m, 0, m)
+        dev.combine(FP32, B4, 2 * m, np.vstack([-QtD, np.eye(m)]), B4, 0, 3 * m)
+        grad += d.lp_backward_layer_device(h, R, B4, m, 3 * m).astype(np.float64)
+    return loss, grad
+
+
+def train_fo(d, snaps, theta0=None, steps=60, lr=0.02, reg=0.0, log=None, weighted=False):
+    """Adam over the first-order snapshots (one per step, cycling), pulled
+    toward theta0 with weight reg.  Only the spectral-layer entries move."""
+    th = np.zeros(LP_NPARAM) if theta0 is None else np.asarray(theta0, np.float64).copy()
+    th0 = th.copy()
+    mom, vel = np.zeros_like(th), np.zeros_like(th)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(1, steps + 1):
+        loss, g = loss_and_grad_fo(d, th.astype(np.float32), snaps[(it - 1) % len(snaps)],
+                                   weighted)
+        hist.append(loss)
+        if reg > 0:
+            g = g + 2.0 * reg * (th - th0)
+        mom = b1 * mom + (1 - b1) * g
+        vel = b2 * vel + (1 - b2) * g * g
+        th -= lr * (mom / (1 - b1 ** it)) / (np.sqrt(vel / (1 - b2 ** it)) + eps)
+        if log and (it % 10 == 0 or it == 1):
+            log(f"step {it}: loss {loss:.5f} ({time.perf_counter() - t0:.1f} s)")
+    return th.astype(np.float32), hist
+
+
+def evaluate_fo(d, theta, snaps, weighted=False):
+    return float(np.mean([loss_and_grad_fo(d, theta, s, weighted)[0] for s in snaps]))
+
+
+def collect_fo(make_problem, nev, snapshots=(4, 8, 14, 22, 32, 48, 70, 100, 140), log=None):
+    """Converged first-order solve plus truncated solves of the same problem
+    (all FP64, first-order form); returns (problem, its device context,
+    snapshots)."""
+    ref = make_problem(maxiter=2000)
+    ref.solve()
+    Ustar, Vstar = ref.modes(nev)
+    snaps = []
+    d = ref.device_context()
+    for k in snapshots:
+        p = make_problem(maxiter=k)
+        p.solve()
+        U, V = p.modes(nev)
+        snaps.append(make_snapshot_fo(d, U, V, p.eigenvalues(nev), Ustar, Vstar))
+        p.close()
+        if log:
+            log(f"first-order snapshot after {k} iterations")
+    return ref, d, snaps
+

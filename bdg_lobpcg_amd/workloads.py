@@ -145,16 +145,34 @@ def batch_configs(n=16, seed=20260601):
 
 
 def train_learned(cfg, steps=300, lr=1e-3, reg=0.3, snapshots=(4, 8, 14, 22, 32, 48, 70),
-                  log=None):
+                  log=None, form="product", weighted=False):
     """Learned-preconditioner parameters for cfg's grid, trained on the GPU on
     pairs from a related problem (g x 0.75, eps_dd x 0.875: same grid and trap,
     different condensate), so that the evaluation problem is not in the
-    training set.  Returns (theta, seconds, info)."""
+    training set.  form "product": the whole network, for the product form's
+    T_M T_K; "first_order": the two spectral layers, for the first-order
+    form's per-half T_K, T_M.  Returns (theta, seconds, info)."""
     from bdg_lobpcg_amd import learned as LP
 
     t0 = time.perf_counter()
     tcfg = dict(cfg, g=cfg["g"] * 0.75, edd=cfg["edd"] * 0.875)
     psi_t, mu_t, _ = ground_state(tcfg)
+    if form == "first_order":
+        def mk_fo(maxiter):
+            return make_problem(tcfg, psi_t, mu_t, maxiter=maxiter)
+
+        ref, d, snaps = LP.collect_fo(mk_fo, tcfg["nev"], snapshots=snapshots, log=log)
+        th0 = np.zeros(len(LP.init_theta(0)), dtype=np.float32)
+        loss0 = LP.evaluate_fo(d, th0, snaps, weighted)
+        th, hist = LP.train_fo(d, snaps, th0, steps=steps, lr=lr, reg=reg, log=log,
+                               weighted=weighted)
+        loss1 = LP.evaluate_fo(d, th, snaps, weighted)
+        d.close()
+        ref.close()
+        return th, time.perf_counter() - t0, dict(
+            form=form, weighted=weighted, loss_analytic=loss0, loss_learned=loss1, steps=steps,
+            lr=lr, reg=reg,
+            snapshots=list(snapshots), train_problem=f"g={tcfg['g']}, eps_dd={tcfg['edd']}")
     def mk(maxiter):
         # the network replaces the product form's T_M T_K: its training pairs
         # come from that iteration

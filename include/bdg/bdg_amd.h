@@ -90,6 +90,10 @@ typedef struct {
   uint64_t frozen_skips;        /* block columns the update left unwritten     */
   uint64_t iterations_fp32;     /* of the iterations, those of the FP32 phase  */
   double fp32_worst_residual;   /* worst residual when the FP32 phase ended    */
+  int fp64_sweep_only;          /* mixed, first order: the FP64 phase was one  */
+                                /* refinement sweep (Rayleigh-Ritz and         */
+                                /* residuals on the FP32 vectors) because its  */
+                                /* full basis did not fit in device memory     */
 } bdg_solve_stats_t;
 
 void bdg_get_solve_stats(const bdg_t *bdg, bdg_solve_stats_t *out);

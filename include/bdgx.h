@@ -213,6 +213,8 @@ int bdgx_itp_groundstate(bdgx_ctx *ctx, const double *omega, double g,
 /* ---- memory, streams, timing ------------------------------------------ */
 int bdgx_malloc(void **dptr, size_t bytes);
 int bdgx_free(void *dptr);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int bdgx_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* Page-locked host memory (hipHostMalloc / hipHostFree): copies between it
  * and the device run at the full link rate. */
 int bdgx_host_alloc(void **hptr, size_t bytes);
@@ -318,6 +320,15 @@ int bdgx_lp_apply(bdgx_ctx *ctx, int prec, const void *dR, void *dW, uint64_t nc
 /* h_grad = d/dtheta sum(dGW * N(dR)) (device forward + adjoint, FP64 sums). */
 int bdgx_lp_backward(bdgx_ctx *ctx, const float *dR, const float *dGW, uint64_t ncols,
                      uint64_t ld, float *h_grad);
+/* One spectral layer alone, the first-order solver's per-half preconditioner
+ * (the learned counterpart of precondK / precondM applied to R_U / R_V):
+ * dW = T'_X dR, layer 0 = K, 1 = M (FP32, dW != dR). */
+int bdgx_lp_apply_layer(bdgx_ctx *ctx, int layer, const void *dR, void *dW, uint64_t ncols,
+                        uint64_t ld);
+/* h_grad = d/dtheta sum(dGW * T'_X(dR)): nonzero only in alpha_X, beta_X,
+ * kappa_X. */
+int bdgx_lp_backward_layer(bdgx_ctx *ctx, int layer, const float *dR, const float *dGW,
+                           uint64_t ncols, uint64_t ld, float *h_grad);
 
 #ifdef __cplusplus
 }

@@ -180,6 +180,35 @@ class LearnedRef:
         return np.concatenate([g["aK"], g["bK"], g["aM"], g["bM"], g["kK"], g["kM"],
                                g["w"].ravel(), g["d"], g["v"]])
 
+    # -- one spectral layer alone (the first-order solver's per-half
+    #    preconditioner: T'_K on R_U, T'_M on R_V) --------------------------
+    def layer_forward(self, theta, R, layer):
+        X = "K" if layer == 0 else "M"
+        T = self.tables(unpack(theta))
+        out = np.zeros_like(np.asarray(R, float))
+        for j in range(R.shape[1]):
+            r = R[:, j].reshape(self.shape)
+            out[:, j] = (T["a" + X] * self.filt(T["g" + X], T["b" + X] * r)).ravel()
+        return out
+
+    def layer_grad(self, theta, R, dW, layer):
+        """Gradient of sum(dW * layer_forward(theta, R, layer))."""
+        X = "K" if layer == 0 else "M"
+        T = self.tables(unpack(theta))
+        oa, ob, ok = (0, 2, 8) if layer == 0 else (4, 6, 8 + NKNOT)
+        g = np.zeros(NPARAM)
+        for j in range(R.shape[1]):
+            r = R[:, j].reshape(self.shape)
+            dw = dW[:, j].reshape(self.shape)
+            w = T["a" + X] * self.filt(T["g" + X], T["b" + X] * r)
+            g[oa:oa + 2] += [np.sum(dw * w * f) for f in self.phi]
+            t = self.filt(T["g" + X], T["a" + X] * dw)
+            g[ob:ob + 2] += [np.sum(r * t * T["b" + X] * f) for f in self.phi]
+            A = np.fft.rfftn(T["a" + X] * dw)
+            B = np.fft.rfftn(T["b" + X] * r)
+            g[ok:ok + NKNOT] += self.knot_sum(self.wt * np.real(np.conj(A) * B) * T["g" + X])
+        return g
+
     def knot_sum(self, q):
         out = np.zeros(NKNOT)
         np.add.at(out, self.ki.ravel(), ((1 - self.kf) * q).ravel())

@@ -105,7 +105,8 @@ def test_gram_combine(cplx, prec):
                                   ((128, 128, 4), (24.0, 24.0, 4.0)),
                                   ((48, 48, 80), (16.0, 16.0, 20.0)),
                                   ((80, 80, 48), (18.0, 18.0, 12.0)),
-                                  ((96, 96, 8), (20.0, 20.0, 8.0))],
+                                  ((96, 96, 8), (20.0, 20.0, 8.0)),
+                                  ((32, 32, 128), (16.0, 16.0, 30.0))],
                          ids=lambda g: "x".join(map(str, g[0])))
 def test_launch_variants_and_autotune(grid):
     """Every launch variant of the plane and fixed z kernels gives the
@@ -118,7 +119,10 @@ def test_launch_variants_and_autotune(grid):
     refs = {op: c.apply(op, X) for op in ("K", "M", "precondK")}
     for prec in (0, 1):
         for kind in range(3):
-            for v in ((0, 1, 2, 3) if kind <= 1 else (0, 1)):  # planes: + persistent forms
+            # planes: 4 shapes (one-plane / persistent, 4N / 8N threads) x 2
+            # radix plans; z: 2 shapes x 2 radix plans (the second plan
+            # exists at N = 128 only; elsewhere those variants repeat)
+            for v in (range(8) if kind <= 1 else range(4)):
                 d.set_launch_variant(kind, prec, v)
                 for op, ref in refs.items():
                     e = relerr(d.apply(op, X, prec), ref)
@@ -156,9 +160,9 @@ def test_launch_variants_and_autotune(grid):
         for v in (1, 2, 3):  # same sums in the same order
             assert np.array_equal(gs[v], gs[0]), f"gram variant {v} differs from 0"
         chosen = d.autotune(prec, 8, force=True)
-        assert chosen["plane_fwd"] in (0, 1, 2, 3) and chosen["plane_inv"] in (0, 1, 2, 3)
+        assert chosen["plane_fwd"] in range(8) and chosen["plane_inv"] in range(8)
         assert chosen["combine"] in (0, 1, 2, 3)
-        assert chosen["z"] in (0, 1) and chosen["gram"] in (0, 1, 2, 3)
+        assert chosen["z"] in range(4) and chosen["gram"] in (0, 1, 2, 3)
         for op, ref in refs.items():
             assert relerr(d.apply(op, X, prec), ref) < TOL[prec]
 

@@ -46,6 +46,33 @@ def test_gradient_matches_finite_differences():
     np.testing.assert_allclose(g[idx], fd, rtol=2e-5, atol=1e-7 * np.abs(fd).max())
 
 
+@pytest.mark.parametrize("layer", [0, 1], ids=["K", "M"])
+def test_layer_alone_zero_theta_and_gradient(layer):
+    """One spectral layer (the first-order solver's per-half preconditioner):
+    at theta = 0 it is the oracle's precondK / precondM; its gradient against
+    central finite differences; entries outside the layer are zero."""
+    c, f, net = _small()
+    R = rand_block(c.size, 2, False, seed=8)
+    ref = c.apply("precondK" if layer == 0 else "precondM", R)
+    assert relerr(net.layer_forward(np.zeros(LR.NPARAM), R, layer), ref) < 1e-12
+    rng = np.random.default_rng(9)
+    th = np.zeros(LR.NPARAM)
+    th[:8] = 0.3 * rng.standard_normal(8)
+    th[8:8 + 2 * LR.NKNOT] = 0.2 * rng.standard_normal(2 * LR.NKNOT)
+    dW = rand_block(c.size, 2, False, seed=10)
+    g = net.layer_grad(th, R, dW, layer)
+    f0 = lambda t: float(np.sum(dW * net.layer_forward(t, R, layer)))
+    own = (list(range(0, 4)) + list(range(8, 8 + LR.NKNOT)) if layer == 0 else
+           list(range(4, 8)) + list(range(8 + LR.NKNOT, 8 + 2 * LR.NKNOT)))
+    h = 1e-5
+    for i in own:
+        e = np.zeros_like(th)
+        e[i] = h
+        fd = (f0(th + e) - f0(th - e)) / (2 * h)
+        assert abs(g[i] - fd) <= 2e-5 * abs(fd) + 1e-7 * np.abs(g).max(), (i, g[i], fd)
+    assert np.all(np.delete(g, own) == 0)
+
+
 def test_cosine_loss_gradient():
     rng = np.random.default_rng(3)
     W, E = rng.standard_normal((50, 3)), rng.standard_normal((50, 3))
@@ -76,6 +103,29 @@ def _theta(seed):
     o = 8 + 2 * LR.NKNOT + 27 * LR.HID
     th[o:] = 0.3 * rng.standard_normal(2 * LR.HID)
     return th
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,L", [((16, 12, 10), (12.0, 10.0, 9.0)), ((32, 32, 32), (16.0, 16.0, 12.0))],
+                         ids=["pencil", "plane"])
+def test_device_layer_forward_and_gradient(N, L):
+    """bdgx_lp_apply_layer / bdgx_lp_backward_layer (the first-order solver's
+    per-half learned preconditioner) against the numpy restatement."""
+    c, f, d, net = _device(N, L)
+    th = _theta(13)
+    d.lp_set_params(th.astype(np.float32))
+    t64 = th.astype(np.float32).astype(np.float64)
+    R = rand_block(c.size, 3, False, seed=23)
+    G = rand_block(c.size, 3, False, seed=24)
+    R64, G64 = R.astype(np.float32).astype(np.float64), G.astype(np.float32).astype(np.float64)
+    print(f"layer case N={N}", flush=True)
+    for layer in (0, 1):
+        W = d.lp_apply_layer(layer, R)
+        assert relerr(W, net.layer_forward(t64, R64, layer)) < 2e-5
+        dR, dG = d.upload(R.astype(np.float32)), d.upload(G.astype(np.float32))
+        g = d.lp_backward_layer_device(layer, dR, dG, 3)
+        gr = net.layer_grad(t64, R64, G64, layer)
+        assert np.abs(g - gr).max() < 1e-3 * np.abs(gr).max(), (layer, g[:8], gr[:8])
 
 
 @pytest.mark.gpu
