@@ -160,6 +160,7 @@ def lib():
         "bdg_set_locking": (None, [P, I]),
         "bdg_set_lock_guards": (None, [P, I]),
         "bdg_set_device_sweeps": (None, [P, I]),
+        "bdg_set_fp64_sweep_only": (None, [P, I]),
         "bdg_residuals": (C.POINTER(D), [P]),
         "bdg_zero_modes": (C.POINTER(I), [P]),
         "bdg_device_context": (P, [P]),
@@ -609,6 +610,11 @@ class Problem:
         """Keep the modes on the device after a solve so that reuse_modes
         builds the next start block there (default on)."""
         lib().bdg_set_device_sweeps(self.h, int(bool(on)))
+
+    def set_fp64_sweep_only(self, on: bool):
+        """Mixed first-order solves: the FP64 phase is one refinement sweep
+        (automatic where the FP64 basis does not fit in device memory)."""
+        lib().bdg_set_fp64_sweep_only(self.h, int(bool(on)))
 
     def reuse_modes(self, noise_frac: float) -> int:
         return int(lib().bdg_reuse_modes(self.h, float(noise_frac)))

@@ -605,14 +605,27 @@ static int learned_applies(const bdg_t *b) {
  * an FP32 phase followed by an FP64 phase from its vectors.  dV0 is the v
  * half of the start block (u = v for the built-in blocks; the reference's
  * start blocks set u = v too, src/solver.c:166-259). */
-static int solve_first_order(bdg_t *b, const void *dU0, const void *dV0, int learned,
-                             double t_start) {
+/* frees a start block pair (u and v may share one buffer) */
+static void free_start(void **pU0, void **pV0) {
+  if (*pU0 && *pU0 != *pV0) bdgx_free(*pU0);
+  bdgx_free(*pV0);
+  *pU0 = *pV0 = NULL;
+}
+
+/* Device bytes of one phase's working set: per half the basis [X | P | W]
+ * and its images, double-buffered (4 S x 3m), and the residual and
+ * new-direction blocks (2 S x m) -- lrep.c's allocations. */
+static double phase_bytes(uint64_t S, uint64_t m, size_t es) {
+  return 2.0 * (4.0 * 3.0 * (double)m + 2.0 * (double)m) * (double)S * (double)es;
+}
+
+static int solve_first_order(bdg_t *b, void **pU0, void **pV0, int learned, double t_start) {
   const uint64_t S = b->size, m = b->sizeSub, nev = b->nev;
   const int c = b->complex_psi0;
-  const size_t es64 = sizeof(double) * (c ? 2 : 1);
+  const size_t es64 = sizeof(double) * (c ? 2 : 1), es32 = sizeof(float) * (c ? 2 : 1);
+  const int mixed = b->precision == BDG_PREC_MIXED;
   void *dU = NULL, *dV = NULL;
-  int rc = bdgx_malloc(&dU, S * m * es64);
-  if (!rc) rc = bdgx_malloc(&dV, S * m * es64);
+  int rc = 0;
   lrep_cfg cfg;
   memset(&cfg, 0, sizeof(cfg));
   cfg.S = S;
@@ -634,14 +647,23 @@ static int solve_first_order(bdg_t *b, const void *dU0, const void *dV0, int lea
   double *w2 = (double *)xcalloc(m, sizeof(double)), *resv = (double *)xcalloc(m, sizeof(double));
   r32.w2 = r64.w2 = w2;
   r32.res = r64.res = resv;
-  const void *start = dV0, *startU = dU0;
   int it32 = 0;
-  if (!rc && b->precision == BDG_PREC_MIXED) {
+  if (mixed) {
+    /* FP32 phase on FP32 copies of the start halves, which then receive its
+     * vectors; the FP64 start block is released first, so that at the
+     * largest grids the phase has the memory to itself */
+    void *sU = NULL, *sV = NULL;
+    rc = bdgx_malloc(&sU, S * m * es32);
+    if (!rc) rc = bdgx_malloc(&sV, S * m * es32);
+    if (!rc) rc = bdgx_convert(b->dev, BDGX_FP64, *pU0, S, BDGX_FP32, sU, S, S, m);
+    if (!rc) rc = bdgx_convert(b->dev, BDGX_FP64, *pV0, S, BDGX_FP32, sV, S, S, m);
+    if (!rc) rc = bdgx_sync(b->dev);
+    free_start(pU0, pV0);
     cfg.maxiter = (int)b->maxIter;
     cfg.tol = fmax(b->tol, b->fp32_tol);
     cfg.stall_window = b->fp32_stall;
     cfg.stall_below = 1e-4;
-    rc = bdg_lrep_device(b->dev, &cfg, BDGX_FP32, dU0, dV0, BDGX_FP64, &r32, dU, dV);
+    if (!rc) rc = bdg_lrep_device(b->dev, &cfg, BDGX_FP32, sU, sV, BDGX_FP32, &r32, sU, sV);
     it32 = r32.iters;
     b->stats.iterations_fp32 = (uint64_t)it32;
     b->stats.fp32_worst_residual = r32.worst;
@@ -651,16 +673,51 @@ static int solve_first_order(bdg_t *b, const void *dU0, const void *dV0, int lea
     if (b->verbosity > 0)
       fprintf(stderr, "[bdg] fp32 phase: %d iterations, worst residual %.3e, status %d\n", it32,
               r32.worst, rc);
-    start = NULL;
+    /* the FP64 phase starts from the FP32 vectors */
+    if (!rc) rc = bdgx_malloc(&dU, S * m * es64);
+    if (!rc) rc = bdgx_malloc(&dV, S * m * es64);
+    if (!rc) rc = bdgx_convert(b->dev, BDGX_FP32, sU, S, BDGX_FP64, dU, S, S, m);
+    if (!rc) rc = bdgx_convert(b->dev, BDGX_FP32, sV, S, BDGX_FP64, dV, S, S, m);
+    if (!rc) rc = bdgx_sync(b->dev);
+    bdgx_free(sU);
+    bdgx_free(sV);
+  } else {
+    rc = bdgx_malloc(&dU, S * m * es64);
+    if (!rc) rc = bdgx_malloc(&dV, S * m * es64);
   }
   if (!rc) {
     cfg.maxiter = (int)b->maxIter - it32;
     if (cfg.maxiter < 0) cfg.maxiter = 0;
     cfg.tol = b->tol;
     cfg.stall_window = 0;
-    rc = start ? bdg_lrep_device(b->dev, &cfg, BDGX_FP64, startU, start, BDGX_FP64, &r64, dU, dV)
-               : bdg_lrep_device(b->dev, &cfg, BDGX_FP64, dU, dV, BDGX_FP64, &r64, dU, dV);
+    /* where the FP64 basis does not fit the device (256^3 at block 128:
+     * ~400 GB), the FP64 phase of the mixed policy is the refinement sweep
+     * alone: Rayleigh-Ritz on the FP64 images of the FP32 vectors and the
+     * FP64 residuals, no further iterations */
+    size_t fr = 0, tot = 0;
+    if (mixed && b->sweep_only) {
+      cfg.maxiter = 0;
+      b->stats.fp64_sweep_only = 1;
+    } else if (cfg.maxiter > 0 && bdgx_mem_info(&fr, &tot) == 0 &&
+        phase_bytes(S, m, es64) + 2e9 > (double)fr) {
+      if (!mixed) {
+        if (b->verbosity > 0)
+          fprintf(stderr, "[bdg] fp64 solve needs %.1f GB, %.1f GB free\n",
+                  phase_bytes(S, m, es64) / 1e9, (double)fr / 1e9);
+        rc = BDGX_ENOMEM;
+      } else {
+        cfg.maxiter = 0;
+        b->stats.fp64_sweep_only = 1;
+        if (b->verbosity > 0)
+          fprintf(stderr, "[bdg] fp64 basis needs %.1f GB, %.1f GB free: refinement sweep only\n",
+                  phase_bytes(S, m, es64) / 1e9, (double)fr / 1e9);
+      }
+    }
   }
+  if (!rc)
+    rc = mixed ? bdg_lrep_device(b->dev, &cfg, BDGX_FP64, dU, dV, BDGX_FP64, &r64, dU, dV)
+               : bdg_lrep_device(b->dev, &cfg, BDGX_FP64, *pU0, *pV0, BDGX_FP64, &r64, dU, dV);
+  free_start(pU0, pV0);
   b->stats.iterations = (uint64_t)(it32 + r64.iters);
   b->stats.column_applies = r32.applies + r64.applies;
   b->stats.seconds_rr = r32.rr_seconds + r64.rr_seconds;
@@ -801,10 +858,10 @@ int bdg_solve(bdg_t *b) {
     return rc;
   }
   if (b->formulation == BDG_FORM_FIRST_ORDER && !k_equals_m(b)) {
-    rc = solve_first_order(b, dU0 ? dU0 : dV0, dV0, learned, t_start);
-    bdgx_free(dV0);
-    bdgx_free(dU0);
-    return rc;
+    /* the start halves (u = v unless a reuse block) pass to the solver,
+     * which frees them as soon as it can */
+    if (!dU0) dU0 = dV0;
+    return solve_first_order(b, &dU0, &dV0, learned, t_start);
   }
   /* the product form iterates on the v half only */
   bdgx_free(dU0);
@@ -1094,6 +1151,7 @@ void bdg_set_mixed_params(bdg_t *b, f64 img_tol, uint64_t refresh_every) {
 void bdg_set_verbosity(bdg_t *b, int level) { b->verbosity = level; }
 void bdg_set_graph_capture(bdg_t *b, int on) { b->graphs = on ? 1 : 0; }
 void bdg_set_device_sweeps(bdg_t *b, int on) { b->device_sweeps = on ? 1 : 0; }
+void bdg_set_fp64_sweep_only(bdg_t *b, int on) { b->sweep_only = on ? 1 : 0; }
 void bdg_set_freeze(bdg_t *b, f64 tol) { b->freeze_tol = tol > 0 ? tol : 0.0; }
 void bdg_set_learned_fallback(bdg_t *b, int window, f64 factor) {
   b->lp_window = window > 0 ? window : 0;

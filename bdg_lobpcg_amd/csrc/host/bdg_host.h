@@ -99,6 +99,7 @@ struct bdg_t {
   void *d_modes_u, *d_modes_v;
   void *d_reuse_u, *d_reuse_v;
   int device_sweeps; /* keep d_modes after a solve (default 1) */
+  int sweep_only;    /* mixed, first order: FP64 phase = refinement sweep only */
 };
 
 /* xorshift32 / uniform (0, 1] (reference src/bdg_internal.h:53-65) */

@@ -36,7 +36,7 @@ CONFIGS = {
                              itp_tol=1e-8, itp_max=40000),
     "cfg4_256c_d": dict(N=(256, 256, 256), L=(32.0, 32.0, 32.0), omega=(1.0, 1.0, 2.0),
                         g=5000.0, edd=0.7, R=16.0, itp_steps=2000, dt=1e-3, nev=128, m=128,
-                        tol=1e-8, cplx=False),
+                        tol=1e-8, cplx=False, itp_tol=1e-8, itp_max=40000),
 }
 
 

@@ -527,6 +527,36 @@ def config_run(args, cfg, psi, mu, t_itp, itp_info):
     }), flush=True)
 
 
+def large_run(args, cfg, psi, mu, t_itp, itp_info, nmodes_check=16):
+    """The 256^3 configuration (block 128): its FP64 basis (~400 GB) does not
+    fit one MI355X, so only the mixed policy runs -- FP32 phase, then the FP64
+    refinement sweep on the FP32 vectors -- and the result is checked with
+    the FP64 C oracle's operators: the reference pencil's residual of the
+    lowest `nmodes_check` modes (K u = |w| v, M v = w u).  One JSON line."""
+    from tests.common import oracle_problem, oracle_mode_residuals
+    t, r = solve_timed(cfg, psi, mu, 1, 1, 0, form=0)
+    st = r["stats"]
+    t0 = time.perf_counter()
+    c, _ = oracle_problem(cfg, psi, mu)
+    k = min(nmodes_check, cfg["nev"])
+    first, _ = oracle_mode_residuals(c, r["w"][:k], r["u"][:, :k], r["v"][:, :k])
+    t_or = time.perf_counter() - t0
+    log(f"oracle residuals of the lowest {k} modes: max {first.max():.3e} ({t_or:.1f} s)")
+    print(json.dumps({
+        "workload": f"{args.config}: {'x'.join(map(str, cfg['N']))}, real psi0, nev {cfg['nev']}, "
+                    f"block {cfg['m']}, tol {cfg['tol']}",
+        "psi0": dict(itp_info, seconds=t_itp, mu=mu),
+        "mixed_seconds": t, "iterations": st["iterations"],
+        "iterations_fp32": st["iterations_fp32"], "fp64_sweep_only": bool(st["fp64_sweep_only"]),
+        "fp32_worst_residual": st["fp32_worst_residual"],
+        "column_applies": st["column_applies"], "column_applies_fp32": st["column_applies_fp32"],
+        "converged_to_tol": r["conv"], "max_residual_device": float(np.max(r["res"])),
+        "oracle_pencil_residual": {"modes": k, "max": float(first.max()),
+                                   "per_mode": [float(x) for x in first], "seconds": t_or},
+        "omega": [float(x) for x in r["w"]],
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -577,6 +607,9 @@ def main():
                           "iterations": r["stats"]["iterations"],
                           "iterations_fp32": r["stats"]["iterations_fp32"],
                           "converged": r["conv"]}), flush=True)
+        return
+    if args.config == "cfg4_256c_d":
+        large_run(args, cfg, psi, mu, t_itp, itp_info)
         return
     if args.config != "cfg1_128c_d":
         config_run(args, cfg, psi, mu, t_itp, itp_info)

@@ -63,6 +63,13 @@ void bdg_set_lock_guards(bdg_t *bdg, int guards);
  * block is built there, as in the reference. */
 void bdg_set_device_sweeps(bdg_t *bdg, int on);
 
+/* Mixed precision, first-order form: the FP64 phase is the refinement sweep
+ * alone (Rayleigh-Ritz on the FP64 images of the FP32 vectors and the FP64
+ * residuals, no iterations).  It is chosen automatically where the FP64
+ * basis does not fit in device memory (256^3 at block 128); on = 1 forces
+ * it. */
+void bdg_set_fp64_sweep_only(bdg_t *bdg, int on);
+
 /* Binary fast path beside the reference's text formats (reference
  * src/io.c:13-165 writes ~30 bytes of text per value): a 64-byte header
  * (magic "BDGBIN1", kind 1 wavefunction / 2 modes, complex flag, dim, N[3],

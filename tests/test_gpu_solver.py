@@ -212,3 +212,41 @@ def test_frozen_columns_match_soft_locking(cplx):
     nz = ~(z0 | z1)
     np.testing.assert_allclose(w1[nz], w0[nz], rtol=1e-9)
     assert np.all(r1[nz] < 1e-9)
+
+
+def test_mixed_fp64_sweep_only():
+    """Mixed first-order solve whose FP64 phase is the refinement sweep alone
+    (the path the 256^3 configuration takes because its FP64 basis does not
+    fit the device; forced here at 16^3): the FP32 phase's vectors, one FP64
+    Rayleigh-Ritz, FP64 residuals.  Omega within 1e-7 of the FP64 solve (the
+    Ritz values of vectors with residual r are off by O(r^2)), residuals at
+    the FP32 phase's level, no FP64 iterations."""
+    import bdg_lobpcg_amd as B
+    N, L = (16, 16, 16), (12.0, 12.0, 12.0)
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
+    out = {}
+    for sweep in (False, True):
+        p = B.Problem(N, L, False)
+        p.set_system()
+        p.set_trap_harmonic((1.0, 1.0, 1.0))
+        p.set_wavefunction(f["wf"])
+        p.set_contact(20.0)
+        p.set_dipolar(10.0, (0.0, 0.0, 1.0), f["R"])
+        p.set_mu(1.0)
+        p.set_solver_params(6, 8, 400, 1e-10)
+        p.set_precision(0 if not sweep else 1)
+        p.set_init_uniform(42)
+        p.set_fp64_sweep_only(sweep)
+        print(f"sweep-only case: {sweep}", flush=True)
+        assert p.solve() == 0
+        out[sweep] = (p.eigenvalues(6), p.residuals(6), p.stats())
+        p.close()
+    w64, _, _ = out[False]
+    w, res, st = out[True]
+    print(f"  iterations {st['iterations']} (fp32 {st['iterations_fp32']}), res {res.max():.2e}",
+          flush=True)
+    assert st["fp64_sweep_only"] == 1 and st["iterations"] == st["iterations_fp32"]
+    small = np.abs(w64) < 1e-2
+    err = np.abs(w - w64) / np.where(small, 1.0, np.abs(w64))
+    assert np.all(err < 1e-7), (w, w64)
+    assert np.all(res < 1e-4)
