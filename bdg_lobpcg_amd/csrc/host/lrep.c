@@ -338,9 +338,14 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   const int m = cfg->m, nev = cfg->nev, cplx = cfg->cplx;
   const int w = cplx ? 2 : 1;
   const size_t es = lr_es(prec, cplx);
-  /* basis [X | P | W]; a phase of no iterations (the refinement sweep:
-   * Rayleigh-Ritz of the start block and its residuals) holds X only */
-  const int kmax = cfg->maxiter > 0 ? 3 * m : m;
+  /* basis [X | P | W] of at most kmax columns: the host Grams and the
+   * coefficient scratch (dG, dC, hG, hC: the update's, the residuals' and
+   * the deflation's matrices at fixed offsets) are sized from it.  The
+   * device basis of a phase of no iterations (the refinement sweep:
+   * Rayleigh-Ritz of the start block and its residuals) holds X only:
+   * kcols columns. */
+  const int kmax = 3 * m;
+  const int kcols = cfg->maxiter > 0 ? kmax : m;
   const int opT[2] = {BDGX_OP_PRECOND_K, BDGX_OP_PRECOND_M};
   const int opA[2] = {BDGX_OP_K, BDGX_OP_M};
   /* full device Gram recompute period (FP32: the projected Grams of rounded
@@ -397,8 +402,8 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
   TRY(bdgx_autotune(ctx, prec, (uint64_t)(m < 32 ? m : 32), 0));
   for (int h = 0; h < 2; ++h) {
     for (int q = 0; q < 2; ++q) {
-      TRY(bdgx_malloc(&H[h].Z[q], (size_t)S * kmax * es));
-      TRY(bdgx_malloc(&H[h].Y[q], (size_t)S * kmax * es));
+      TRY(bdgx_malloc(&H[h].Z[q], (size_t)S * kcols * es));
+      TRY(bdgx_malloc(&H[h].Y[q], (size_t)S * kcols * es));
     }
     TRY(bdgx_malloc(&dR[h], (size_t)S * m * es));
     TRY(bdgx_malloc(&dWt[h], (size_t)S * m * es));
