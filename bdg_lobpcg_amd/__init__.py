@@ -68,7 +68,8 @@ class SolveStats(C.Structure):
                 ("graph_captures", C.c_uint64), ("seconds_total", C.c_double),
                 ("seconds_rr", C.c_double), ("status", C.c_int),
                 ("frozen_skips", C.c_uint64), ("iterations_fp32", C.c_uint64),
-                ("fp32_worst_residual", C.c_double), ("fp64_sweep_only", C.c_int)]
+                ("fp32_worst_residual", C.c_double), ("fp64_sweep_only", C.c_int),
+                ("fp64_group_size", C.c_int)]
 
 
 def build() -> None:
@@ -161,6 +162,7 @@ def lib():
         "bdg_set_lock_guards": (None, [P, I]),
         "bdg_set_device_sweeps": (None, [P, I]),
         "bdg_set_fp64_sweep_only": (None, [P, I]),
+        "bdg_set_fp64_groups": (None, [P, I]),
         "bdg_residuals": (C.POINTER(D), [P]),
         "bdg_zero_modes": (C.POINTER(I), [P]),
         "bdg_device_context": (P, [P]),
@@ -615,6 +617,11 @@ class Problem:
         """Mixed first-order solves: the FP64 phase is one refinement sweep
         (automatic where the FP64 basis does not fit in device memory)."""
         lib().bdg_set_fp64_sweep_only(self.h, int(bool(on)))
+
+    def set_fp64_groups(self, group: int):
+        """Mixed first-order solves: FP64 phase in groups of `group` modes
+        (automatic where the block's FP64 basis does not fit the device)."""
+        lib().bdg_set_fp64_groups(self.h, int(group))
 
     def reuse_modes(self, noise_frac: float) -> int:
         return int(lib().bdg_reuse_modes(self.h, float(noise_frac)))

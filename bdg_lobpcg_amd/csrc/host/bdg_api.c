@@ -619,6 +619,57 @@ static double phase_bytes(uint64_t S, uint64_t m, size_t es) {
   return 2.0 * (4.0 * 3.0 * (double)m + 2.0 * (double)m) * (double)S * (double)es;
 }
 
+/* The FP64 phase in groups, where the block's FP64 basis does not fit the
+ * device: modes g0 .. g0+G-1 plus guard columns start from the FP32 phase's
+ * vectors (columns g0 .. of dU / dV, refined in place) and are iterated in a
+ * block deflated against the groups already refined (columns 0 .. g0-1,
+ * locked), as the solver's own locking does with converged pairs. */
+static int fp64_groups(bdg_t *b, const lrep_cfg *base, int G, void *dU, void *dV, double *w2,
+                       double *resv, lrep_result *acc) {
+  const uint64_t S = b->size;
+  const int m = (int)b->sizeSub, nev = (int)b->nev;
+  const size_t es = sizeof(double) * (b->complex_psi0 ? 2 : 1);
+  int rc = 0;
+  for (int g0 = 0; g0 < nev && !rc; g0 += G) {
+    const int gn = nev - g0 < G ? nev - g0 : G;
+    int ng = base->lock_guards > 0 ? base->lock_guards : 0;
+    if (g0 + gn + ng > m) ng = m - g0 - gn;
+    const int mb = gn + ng;
+    lrep_cfg cfg = *base;
+    cfg.m = mb;
+    cfg.nev = gn;
+    cfg.lock_max = 0;
+    cfg.lock_guards = 0;
+    cfg.nl = g0;
+    cfg.dUL = g0 ? dU : NULL;
+    cfg.dVL = g0 ? dV : NULL;
+    cfg.maxiter = base->maxiter - acc->iters;
+    if (cfg.maxiter < 1) cfg.maxiter = 1;
+    lrep_result r;
+    memset(&r, 0, sizeof(r));
+    double *gw2 = (double *)xcalloc((size_t)mb, sizeof(double));
+    double *gres = (double *)xcalloc((size_t)mb, sizeof(double));
+    r.w2 = gw2;
+    r.res = gres;
+    void *u0 = (char *)dU + (size_t)g0 * S * es, *v0 = (char *)dV + (size_t)g0 * S * es;
+    rc = bdg_lrep_device(b->dev, &cfg, BDGX_FP64, u0, v0, BDGX_FP64, &r, u0, v0);
+    for (int j = 0; j < gn; ++j) {
+      w2[g0 + j] = gw2[j];
+      resv[g0 + j] = gres[j];
+    }
+    acc->iters += r.iters;
+    acc->applies += r.applies;
+    acc->rr_seconds += r.rr_seconds;
+    acc->worst = fmax(acc->worst, r.worst);
+    if (b->verbosity > 0)
+      fprintf(stderr, "[bdg] fp64 group %d..%d (+%d guards, %d locked): %d iterations, worst "
+              "residual %.3e, status %d\n", g0, g0 + gn - 1, ng, g0, r.iters, r.worst, rc);
+    free(gw2);
+    free(gres);
+  }
+  return rc;
+}
+
 static int solve_first_order(bdg_t *b, void **pU0, void **pV0, int learned, double t_start) {
   const uint64_t S = b->size, m = b->sizeSub, nev = b->nev;
   const int c = b->complex_psi0;
@@ -691,30 +742,48 @@ static int solve_first_order(bdg_t *b, void **pU0, void **pV0, int learned, doub
     cfg.tol = b->tol;
     cfg.stall_window = 0;
     /* where the FP64 basis does not fit the device (256^3 at block 128:
-     * ~400 GB), the FP64 phase of the mixed policy is the refinement sweep
-     * alone: Rayleigh-Ritz on the FP64 images of the FP32 vectors and the
-     * FP64 residuals, no further iterations */
+     * ~480 GB), the FP64 phase of the mixed policy iterates the modes in
+     * groups (the largest of m/2, m/4, ... whose basis fits), each deflated
+     * against the groups before it; where not even that fits, it is the
+     * refinement sweep alone (Rayleigh-Ritz on the FP64 images of the FP32
+     * vectors and the FP64 residuals, no iterations) */
     size_t fr = 0, tot = 0;
+    int group = 0;
     if (mixed && b->sweep_only) {
       cfg.maxiter = 0;
       b->stats.fp64_sweep_only = 1;
+    } else if (mixed && b->fp64_groups > 0 && (uint64_t)b->fp64_groups < nev) {
+      group = b->fp64_groups;
+      b->stats.fp64_group_size = group;
     } else if (cfg.maxiter > 0 && bdgx_mem_info(&fr, &tot) == 0 &&
-        phase_bytes(S, m, es64) + 2e9 > (double)fr) {
+               phase_bytes(S, m, es64) + 2e9 > (double)fr) {
       if (!mixed) {
         if (b->verbosity > 0)
           fprintf(stderr, "[bdg] fp64 solve needs %.1f GB, %.1f GB free\n",
                   phase_bytes(S, m, es64) / 1e9, (double)fr / 1e9);
         rc = BDGX_ENOMEM;
       } else {
-        cfg.maxiter = 0;
-        b->stats.fp64_sweep_only = 1;
+        const int ng = b->lock_guards > 0 ? b->lock_guards : 0;
+        int G = (int)m / 2;
+        /* 10% and 8 GB of headroom for the FP64 spectrum workspace, the
+         * Gram partials and the allocator's granularity */
+        while (G >= 8 && 1.1 * phase_bytes(S, (uint64_t)(G + ng), es64) + 8e9 > (double)fr) G /= 2;
+        if (G >= 8 && (uint64_t)G < nev) {
+          group = G;
+          b->stats.fp64_group_size = G;
+        } else {
+          cfg.maxiter = 0;
+          b->stats.fp64_sweep_only = 1;
+        }
         if (b->verbosity > 0)
-          fprintf(stderr, "[bdg] fp64 basis needs %.1f GB, %.1f GB free: refinement sweep only\n",
-                  phase_bytes(S, m, es64) / 1e9, (double)fr / 1e9);
+          fprintf(stderr, "[bdg] fp64 basis needs %.1f GB, %.1f GB free: %s %d\n",
+                  phase_bytes(S, m, es64) / 1e9, (double)fr / 1e9,
+                  group ? "groups of" : "refinement sweep only", group);
       }
     }
+    if (!rc && group) rc = fp64_groups(b, &cfg, group, dU, dV, w2, resv, &r64);
   }
-  if (!rc)
+  if (!rc && !b->stats.fp64_group_size)
     rc = mixed ? bdg_lrep_device(b->dev, &cfg, BDGX_FP64, dU, dV, BDGX_FP64, &r64, dU, dV)
                : bdg_lrep_device(b->dev, &cfg, BDGX_FP64, *pU0, *pV0, BDGX_FP64, &r64, dU, dV);
   free_start(pU0, pV0);
@@ -1152,6 +1221,7 @@ void bdg_set_verbosity(bdg_t *b, int level) { b->verbosity = level; }
 void bdg_set_graph_capture(bdg_t *b, int on) { b->graphs = on ? 1 : 0; }
 void bdg_set_device_sweeps(bdg_t *b, int on) { b->device_sweeps = on ? 1 : 0; }
 void bdg_set_fp64_sweep_only(bdg_t *b, int on) { b->sweep_only = on ? 1 : 0; }
+void bdg_set_fp64_groups(bdg_t *b, int group) { b->fp64_groups = group > 0 ? group : 0; }
 void bdg_set_freeze(bdg_t *b, f64 tol) { b->freeze_tol = tol > 0 ? tol : 0.0; }
 void bdg_set_learned_fallback(bdg_t *b, int window, f64 factor) {
   b->lp_window = window > 0 ? window : 0;

@@ -100,6 +100,7 @@ struct bdg_t {
   void *d_reuse_u, *d_reuse_v;
   int device_sweeps; /* keep d_modes after a solve (default 1) */
   int sweep_only;    /* mixed, first order: FP64 phase = refinement sweep only */
+  int fp64_groups;   /* mixed, first order: FP64 phase in groups of this size  */
 };
 
 /* xorshift32 / uniform (0, 1] (reference src/bdg_internal.h:53-65) */

@@ -138,19 +138,39 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None, tol=None, fo
 
 
 PASSES = ["xfwd", "yfwd", "z", "yinv", "xinv", "plane_fwd", "plane_inv"]
+
+
+def _alt_plan(n):
+    """n has a second radix plan (8 x ... x 8 x 2 with the 2 merged into a
+    radix-16 first stage): n = 2 * 8^k, k >= 1."""
+    k = 0
+    while n % 8 == 0 and n > 1:
+        n //= 8
+        k += 1
+    return k >= 1 and n == 2
+
+
 def kernel_name(pname, cfg, variants):
     """Name of the FP64 kernel that runs pass `pname` under the tuned launch
-    variants (plane: 0/1 the one-plane kernel with 4N/8N threads, 2/3 the
-    persistent kernel with 4N/8N threads; z: 256 or 128 threads at Nz = 128)."""
+    variants (plane: bits 0-1 = 0/1 the one-plane kernel with 4N/8N threads,
+    2/3 the persistent kernel with 4N/8N threads, bit 2 = the other radix
+    plan; z: bit 0 = half the threads of 256 (Nz >= 128) or 128, bit 1 = the
+    other radix plan).  The last template argument is the radix-16 flag: the
+    default plan is radix 16 first except in the FP64 persistent inverse."""
     nx, nz = cfg["N"][0], cfg["N"][2]
     v = variants.get(pname, 0)
     if pname in ("plane_fwd", "plane_inv"):
         d = "fwd" if pname == "plane_fwd" else "inv"
-        k = f"k_plane_{d}_persist" if v >= 2 else f"k_plane_{d}_real"
-        return f"{k}<double, {nx}, {(8 if v & 1 else 4) * nx}>"
+        shape = v & 3
+        k = f"k_plane_{d}_persist" if shape >= 2 else f"k_plane_{d}_real"
+        r16 = not (d == "inv" and shape >= 2)
+        if _alt_plan(nx) and v & 4:
+            r16 = not r16
+        return f"{k}<double, {nx}, {(8 if shape & 1 else 4) * nx}, {str(r16).lower()}>"
     if pname == "z":
-        t = (256 if nz == 128 else 128) >> v
-        return f"k_zpass_fixed<double, {nz}, 8, {t}>"
+        t = (256 if nz >= 128 else 128) >> (v & 1)
+        r16 = not (_alt_plan(nz) and v & 2)
+        return f"k_zpass_fixed<double, {nz}, 8, {t}, {str(r16).lower()}>"
     return pname
 
 

@@ -65,10 +65,15 @@ void bdg_set_device_sweeps(bdg_t *bdg, int on);
 
 /* Mixed precision, first-order form: the FP64 phase is the refinement sweep
  * alone (Rayleigh-Ritz on the FP64 images of the FP32 vectors and the FP64
- * residuals, no iterations).  It is chosen automatically where the FP64
- * basis does not fit in device memory (256^3 at block 128); on = 1 forces
- * it. */
+ * residuals, no iterations).  Where the block's FP64 basis does not fit in
+ * device memory (256^3 at block 128) the FP64 phase iterates the modes in
+ * groups instead (bdg_solve_stats_t.fp64_group_size), and takes the sweep
+ * alone only where not even a group fits; on = 1 forces the sweep. */
 void bdg_set_fp64_sweep_only(bdg_t *bdg, int on);
+/* Mixed precision, first-order form: run the FP64 phase in groups of
+ * `group` modes (each deflated against the groups before it) even where the
+ * whole block fits; 0 (default): only where it does not fit. */
+void bdg_set_fp64_groups(bdg_t *bdg, int group);
 
 /* Binary fast path beside the reference's text formats (reference
  * src/io.c:13-165 writes ~30 bytes of text per value): a 64-byte header
@@ -101,6 +106,10 @@ typedef struct {
                                 /* refinement sweep (Rayleigh-Ritz and         */
                                 /* residuals on the FP32 vectors) because its  */
                                 /* full basis did not fit in device memory     */
+  int fp64_group_size;          /* mixed, first order: the FP64 phase refined  */
+                                /* the modes in groups of this many (each     */
+                                /* deflated against the ones before) because   */
+                                /* the block's FP64 basis did not fit; 0: no  */
 } bdg_solve_stats_t;
 
 void bdg_get_solve_stats(const bdg_t *bdg, bdg_solve_stats_t *out);

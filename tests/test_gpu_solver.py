@@ -214,6 +214,43 @@ def test_frozen_columns_match_soft_locking(cplx):
     assert np.all(r1[nz] < 1e-9)
 
 
+def test_mixed_fp64_groups():
+    """Mixed first-order solve whose FP64 phase refines the modes in groups
+    (the 256^3 configuration's path, forced here at 16^3 with groups of 3 of
+    9 modes): each group iterated in FP64 from the FP32 phase's vectors and
+    deflated against the groups before it.  Omega to 1e-9 of the FP64 solve,
+    every mode converged to the tolerance."""
+    import bdg_lobpcg_amd as B
+    N, L = (16, 16, 16), (12.0, 12.0, 12.0)
+    c, f = physics(N, L, False, dipolar=True, g=20.0, gdd=10.0, mu=1.0)
+    out = {}
+    for grouped in (False, True):
+        p = B.Problem(N, L, False)
+        p.set_system()
+        p.set_trap_harmonic((1.0, 1.0, 1.0))
+        p.set_wavefunction(f["wf"])
+        p.set_contact(20.0)
+        p.set_dipolar(10.0, (0.0, 0.0, 1.0), f["R"])
+        p.set_mu(1.0)
+        p.set_solver_params(9, 12, 600, 1e-10)
+        p.set_precision(1 if grouped else 0)
+        p.set_init_uniform(42)
+        p.set_fp64_groups(3 if grouped else 0)
+        print(f"fp64 groups case: {grouped}", flush=True)
+        assert p.solve() == 0
+        out[grouped] = (p.eigenvalues(9), p.residuals(9), p.stats(), p.converged())
+        p.close()
+    w64 = out[False][0]
+    w, res, st, conv = out[True]
+    print(f"  iterations {st['iterations']} (fp32 {st['iterations_fp32']}), res {res.max():.2e}",
+          flush=True)
+    assert st["fp64_group_size"] == 3 and conv >= 9
+    assert np.all(res < 1e-10)
+    small = np.abs(w64) < 1e-2
+    err = np.abs(w - w64) / np.where(small, 1.0, np.abs(w64))
+    assert np.all(err < np.where(small, 1e-7, 1e-9)), (w, w64)
+
+
 def test_mixed_fp64_sweep_only():
     """Mixed first-order solve whose FP64 phase is the refinement sweep alone
     (the path the 256^3 configuration takes because its FP64 basis does not
