@@ -50,6 +50,10 @@ HBM_PEAK_GBS = 8000.0
 # another, <= 1e-6 left it 4e-10 to 2.4e-7 off (up to ~0.1 x the residual).
 # 2e-7 keeps that worst case near 5e-8.
 ACC_TOL = 2e-7
+# the FP32 phase of such a solve runs to max(ACC_TOL, FP32_PHASE_TOL): below
+# ~1e-6 the FP32 images' rounding stalls it (at residual 2e-7 it ran 191
+# instead of 143 FP32 iterations at 128^3) and FP64 iterations are cheaper
+FP32_PHASE_TOL = 1e-6
 FORM_NAMES = {0: "first-order", 1: "product"}
 # --policy: (precision 0 FP64 / 1 mixed, residual tolerance or None for the
 # configuration's own, formulation)
@@ -107,7 +111,8 @@ def solve_timed(cfg, psi, mu, precision, steps, warmup, theta=None, tol=None, fo
         p = make_problem(cfg, psi, mu, precision=precision, tol=tol)
         p.set_formulation(form)
         if tol is not None:
-            p.set_fp32_phase(tol, 40)  # the FP32 phase runs to the solve's own target
+            # the FP32 phase runs to the solve's own target or its own floor
+            p.set_fp32_phase(max(tol, FP32_PHASE_TOL), 40)
         p.set_verbosity(1)
         if theta is not None:
             p.set_learned(theta)
