@@ -18,9 +18,30 @@
 
 #include <complex.h>
 #include <float.h>
+#include <immintrin.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* x . y in four independent 4-wide FMA chains (a single running sum makes
+ * the dot products of the Gram-Schmidt steps latency bound) */
+static inline double vdot(int n, const double *x, const double *y) {
+  __m256d s0 = _mm256_setzero_pd(), s1 = s0, s2 = s0, s3 = s0;
+  int i = 0;
+  for (; i + 16 <= n; i += 16) {
+    s0 = _mm256_fmadd_pd(_mm256_loadu_pd(x + i), _mm256_loadu_pd(y + i), s0);
+    s1 = _mm256_fmadd_pd(_mm256_loadu_pd(x + i + 4), _mm256_loadu_pd(y + i + 4), s1);
+    s2 = _mm256_fmadd_pd(_mm256_loadu_pd(x + i + 8), _mm256_loadu_pd(y + i + 8), s2);
+    s3 = _mm256_fmadd_pd(_mm256_loadu_pd(x + i + 12), _mm256_loadu_pd(y + i + 12), s3);
+  }
+  for (; i + 4 <= n; i += 4)
+    s0 = _mm256_fmadd_pd(_mm256_loadu_pd(x + i), _mm256_loadu_pd(y + i), s0);
+  const __m256d s = _mm256_add_pd(_mm256_add_pd(s0, s1), _mm256_add_pd(s2, s3));
+  const __m128d h = _mm_add_pd(_mm256_castpd256_pd128(s), _mm256_extractf128_pd(s, 1));
+  double r = _mm_cvtsd_f64(_mm_add_sd(h, _mm_unpackhi_pd(h, h)));
+  for (; i < n; ++i) r += x[i] * y[i];
+  return r;
+}
 
 /* ---------------------------------------------------------------------- */
 static void tridiagonalize(int n, double *A, double *Q, double *d, double *e) {
@@ -133,8 +154,17 @@ static int tridiag_qr(int n, double *d, double *e, double *Z) {
         e[k + 1] = c * e[k + 1];
       }
       if (Z) {
+        /* columns k, k+1 of Z rotated, four rows per FMA pair (this vector
+         * work overlaps the latency-bound scalar chain of the next steps) */
         double *zk = Z + (size_t)k * n, *zk1 = Z + (size_t)(k + 1) * n;
-        for (int i = 0; i < n; ++i) {
+        const __m256d vc = _mm256_set1_pd(c), vs = _mm256_set1_pd(s);
+        int i = 0;
+        for (; i + 4 <= n; i += 4) {
+          const __m256d u = _mm256_loadu_pd(zk + i), w = _mm256_loadu_pd(zk1 + i);
+          _mm256_storeu_pd(zk + i, _mm256_fmadd_pd(vc, u, _mm256_mul_pd(vs, w)));
+          _mm256_storeu_pd(zk1 + i, _mm256_fnmadd_pd(vs, u, _mm256_mul_pd(vc, w)));
+        }
+        for (; i < n; ++i) {
           const double u = zk[i], w = zk1[i];
           zk[i] = c * u + s * w;
           zk1[i] = -s * u + c * w;
@@ -169,9 +199,10 @@ static void sort_eig(int n, double *w, double *V, int ldv, int vlen) {
 }
 
 /* LU factorisation with partial pivoting of the tridiagonal T - s I (the
- * layout of LAPACK's dgttrf: dl multipliers, d / du / du2 the diagonals of
- * U, sw[i] = 1 when rows i and i+1 were exchanged); pivots smaller than
- * tiny are replaced by +-tiny */
+ * layout of LAPACK's dgttrf: dl multipliers, du / du2 the upper diagonals
+ * of U, sw[i] = 1 when rows i and i+1 were exchanged; d returns the
+ * reciprocals of U's diagonal); pivots smaller than tiny are replaced by
+ * +-tiny */
 static void trid_lu(int n, const double *dd, const double *e, double s, double tiny, double *dl,
                     double *d, double *du, double *du2, int *sw) {
   for (int i = 0; i < n; ++i) d[i] = dd[i] - s;
@@ -202,23 +233,22 @@ static void trid_lu(int n, const double *dd, const double *e, double s, double t
     }
   }
   if (fabs(d[n - 1]) < tiny) d[n - 1] = d[n - 1] < 0 ? -tiny : tiny;
+  for (int i = 0; i < n; ++i) d[i] = 1.0 / d[i];
 }
 
-/* x <- (T - s I)^-1 x with the factors of trid_lu */
-static void trid_solve(int n, const double *dl, const double *d, const double *du,
+/* x <- (T - s I)^-1 x with the factors of trid_lu (the row exchanges as
+ * selects, not branches: their pattern is data dependent) */
+static void trid_solve(int n, const double *dl, const double *rd, const double *du,
                        const double *du2, const int *sw, double *x) {
   for (int i = 0; i + 1 < n; ++i) {
-    if (!sw[i]) {
-      x[i + 1] -= dl[i] * x[i];
-    } else {
-      const double t = x[i];
-      x[i] = x[i + 1];
-      x[i + 1] = t - dl[i] * x[i];
-    }
+    const double a = x[i], b = x[i + 1];
+    const double lo = sw[i] ? b : a, hi = sw[i] ? a : b;
+    x[i] = lo;
+    x[i + 1] = hi - dl[i] * lo;
   }
-  x[n - 1] /= d[n - 1];
-  if (n > 1) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / d[n - 2];
-  for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
+  x[n - 1] *= rd[n - 1];
+  if (n > 1) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) * rd[n - 2];
+  for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) * rd[i];
 }
 
 int dn_trid_low(int n, const double *d0, const double *e0, int m, double *w, double *Y) {
@@ -262,13 +292,11 @@ int dn_trid_low(int n, const double *d0, const double *e0, int m, double *w, dou
       for (int pass = 0; pass < 2; ++pass) /* against the cluster's earlier vectors */
         for (int c = c0; c < j; ++c) {
           const double *yc = Y + (size_t)c * n;
-          double dot = 0.0;
-          for (int i = 0; i < n; ++i) dot += yc[i] * y[i];
+          const double dot = vdot(n, yc, y);
+#pragma omp simd
           for (int i = 0; i < n; ++i) y[i] -= dot * yc[i];
         }
-      double nrm = 0.0;
-      for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
-      nrm = sqrt(nrm);
+      const double nrm = sqrt(vdot(n, y, y));
       if (!(nrm > 0)) {
         rc = -1;
         break;
@@ -383,6 +411,15 @@ void dn_qhgq(int k, int q, int cplx, const double *G, const double *Q, double *o
     qhgq_z(k, q, (const double complex *)G, (const double complex *)Q, (double complex *)out);
   else
     qhgq_d(k, q, G, Q, out);
+}
+
+void dn_project(int ku, int kv, int nu, int nv, int cplx, const double *G, const double *Qu,
+                const double *Qv, double *out) {
+  if (cplx)
+    project_z(ku, kv, nu, nv, (const double complex *)G, (const double complex *)Qu,
+              (const double complex *)Qv, (double complex *)out);
+  else
+    project_d(ku, kv, nu, nv, G, Qu, Qv, out);
 }
 
 int dn_chol_ok(int k, int cplx, const double *G, double tau) {

@@ -39,6 +39,11 @@ int dn_rayleigh_ritz(int k, int m, int cplx, const double *GV, const double *GM,
  * when cplx): the Gram set of a combined basis V Q from the Gram of V. */
 void dn_qhgq(int k, int q, int cplx, const double *G, const double *Q, double *out);
 
+/* out (nu x nv) = Qu^H G Qv for G (ku x kv), Qu (ku x nu), Qv (kv x nv)
+ * (complex interleaved when cplx): the Gram between two combined bases. */
+void dn_project(int ku, int kv, int nu, int nv, int cplx, const double *G, const double *Qu,
+                const double *Qv, double *out);
+
 /* 1 when the diagonally scaled G (k x k, Hermitian) has a Cholesky factor
  * whose every pivot is >= tau, else 0 (also when a diagonal entry is not
  * positive). */

@@ -196,41 +196,6 @@ done:
   return rc;
 }
 
-/* out (nu x nv, ld nu) = Qu^H G Qv, G ku x kv (ld ku), Qu ku x nu, Qv kv x nv */
-static void lr_project(int ku, int kv, int nu, int nv, int cplx, const double *G, const double *Qu,
-                       const double *Qv, double *tmp, double *out) {
-  if (!cplx) {
-    /* tmp (ku x nv) = G Qv */
-    for (int j = 0; j < nv; ++j)
-      for (int i = 0; i < ku; ++i) {
-        double a = 0.0;
-        for (int t = 0; t < kv; ++t) a += G[i + (size_t)t * ku] * Qv[t + (size_t)j * kv];
-        tmp[i + (size_t)j * ku] = a;
-      }
-    for (int j = 0; j < nv; ++j)
-      for (int i = 0; i < nu; ++i) {
-        double a = 0.0;
-        for (int t = 0; t < ku; ++t) a += Qu[t + (size_t)i * ku] * tmp[t + (size_t)j * ku];
-        out[i + (size_t)j * nu] = a;
-      }
-    return;
-  }
-  const double complex *g = (const double complex *)G, *qu = (const double complex *)Qu,
-                       *qv = (const double complex *)Qv;
-  double complex *t2 = (double complex *)tmp, *o = (double complex *)out;
-  for (int j = 0; j < nv; ++j)
-    for (int i = 0; i < ku; ++i) {
-      double complex a = 0.0;
-      for (int t = 0; t < kv; ++t) a += g[i + (size_t)t * ku] * qv[t + (size_t)j * kv];
-      t2[i + (size_t)j * ku] = a;
-    }
-  for (int j = 0; j < nv; ++j)
-    for (int i = 0; i < nu; ++i) {
-      double complex a = 0.0;
-      for (int t = 0; t < ku; ++t) a += conj(qu[t + (size_t)i * ku]) * t2[t + (size_t)j * ku];
-      o[i + (size_t)j * nu] = a;
-    }
-}
 
 /* Assemble a ku x kv Gram (ld ku) from its projected top-left block P
  * (nu x nv, ld nu), the new columns Cn (ku x (kv - nv), ld ku: Z_a^H B_new)
@@ -565,9 +530,9 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
     /* the Grams of the next [X | P], projected on the host while the device
      * runs the update */
     tr = bdg_wtime();
-    lr_project(k[0], k[0], nc[0], nc[0], cplx, hGK, hCQ[0], hCQ[0], hC, hP[0]);
-    lr_project(k[1], k[1], nc[1], nc[1], cplx, hGM, hCQ[1], hCQ[1], hC, hP[1]);
-    lr_project(k[0], k[1], nc[0], nc[1], cplx, hGW, hCQ[0], hCQ[1], hC, hP[2]);
+    dn_project(k[0], k[0], nc[0], nc[0], cplx, hGK, hCQ[0], hCQ[0], hP[0]);
+    dn_project(k[1], k[1], nc[1], nc[1], cplx, hGM, hCQ[1], hCQ[1], hP[1]);
+    dn_project(k[0], k[1], nc[0], nc[1], cplx, hGW, hCQ[0], hCQ[1], hP[2]);
     trr += bdg_wtime() - tr;
     for (int h = 0; h < 2; ++h) {
       H[h].np = nc[h] - m;

@@ -443,8 +443,9 @@ def test_host_first_order_update_and_orthonormalisation(cplx):
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("kind", ["random", "degenerate_pairs", "block_diagonal"])
 def test_host_lowest_eigenpairs(cplx, kind):
-    """dn_eig_low (Householder tridiagonalisation, eigenvalues by implicit QR
-    without vectors, inverse iteration for the lowest m vectors, Gram-Schmidt
+    """dn_eig_low (Householder tridiagonalisation; up to n = 128 implicit QR
+    with the rotations accumulated, beyond it eigenvalues by QR without
+    vectors and inverse iteration for the lowest m vectors, Gram-Schmidt
     inside clusters) against numpy.linalg.eigh: eigenvalues to 1e-12
     relative to ||G||, residuals ||G x - w x|| <= 1e-12 ||G||, orthonormal
     vectors; exactly degenerate pairs (the spectra of the solver's
@@ -457,7 +458,7 @@ def test_host_lowest_eigenpairs(cplx, kind):
     lib.dn_eig_low.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     rng = np.random.default_rng(7 + 3 * cplx)
     dt = np.complex128 if cplx else np.float64
-    for n, m in ((96, 32), (40, 40), (7, 3), (2, 1), (1, 1)):
+    for n, m in ((96, 32), (192, 64), (40, 40), (7, 3), (2, 1), (1, 1)):
         Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0))
         if kind == "random":
             lam = rng.standard_normal(n) * 3.0
