@@ -94,6 +94,7 @@ def lib():
         "bdgx_ctx_ksize": (U64, [P]),
         "bdgx_error_string": (C.c_char_p, [I]),
         "bdgx_set_fields": (I, [P, P, P, P, P, P, D]),
+        "bdgx_derive_precond": (I, [P, D]),
         "bdgx_dipolar_set_kernel": (I, [P, D, P, D]),
         "bdgx_dipolar_get_kernel": (I, [P, P]),
         "bdgx_dipolar_meanfield": (I, [P, P]),
@@ -112,6 +113,7 @@ def lib():
         "bdgx_free": (I, [P]),
         "bdgx_memcpy_h2d": (I, [P, P, P, C.c_size_t]),
         "bdgx_memcpy_d2h": (I, [P, P, P, C.c_size_t]),
+        "bdgx_memcpy_d2h_pair": (I, [P, P, P, P, P, C.c_size_t]),
         "bdgx_memset": (I, [P, P, I, C.c_size_t]),
         "bdgx_sync": (I, [P]),
         "bdgx_stream": (P, [P]),
@@ -288,6 +290,9 @@ class DeviceContext:
         check(lib().bdgx_set_fields(self.h, arr(ltK, np.float64), arr(ltM, np.float64),
                                     arr(wf, self.dtype64), arr(pK, np.float64),
                                     arr(pM, np.float64), float(mu)), "bdgx_set_fields")
+
+    def derive_precond(self, floor):
+        check(lib().bdgx_derive_precond(self.h, float(floor)), "bdgx_derive_precond")
 
     def set_dipolar(self, g_ddi, direction, cutoff):
         d = np.ascontiguousarray(direction, dtype=np.float64)

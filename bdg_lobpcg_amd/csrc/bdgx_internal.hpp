@@ -70,6 +70,7 @@ struct bdgx_ctx {
   bdgx::Geo geo;
   bdgx::AxisPlan plan[3];
   hipStream_t stream;
+  hipStream_t aux;  // second copy stream (created on first use)
   hipEvent_t ev0, ev1;
   bdgx::Fields<double> f64;
   bdgx::Fields<float> f32;

@@ -1065,14 +1065,17 @@ __device__ __forceinline__ void load_rows_nb(const Cols<R> &A, int col, long lon
 // block walks row tiles blockIdx.x, + gridDim.x, ...  The k loop runs in
 // groups of PD steps of 4 with PD steps of A in flight; every A load is
 // unconditional (clamped), so the loads of a step are awaited exactly.
-// Same MFMA sequence per output element as k_combine_mfma.  Needs
-// vector-aligned A and Y and nrows a multiple of RT.  With M32 (FP32
+// Same MFMA sequence per output element as k_combine_mfma.  Held to two
+// waves per SIMD (accumulators in VGPRs: without the bound the FP32
+// 64-column instance took 316 registers, one wave per SIMD, and ran the
+// 96 -> 64 update 6% slower).  Needs vector-aligned A and Y and nrows a
+// multiple of RT.  With M32 (FP32
 // storage only): v_mfma_f32_16x16x4f32, the coefficients rounded to FP32
 // and the sums over k in FP32.
 constexpr bool kCombineF32Mfma = false;
 
 template <typename R, int MF, int RT, int PD, bool M32 = false>
-__global__ void __launch_bounds__(256) k_combine_pers(Cols<R> A, int k, const double *C, R *Y,
+__global__ void __launch_bounds__(256, 2) k_combine_pers(Cols<R> A, int k, const double *C, R *Y,
                                                       long long ldy, int m, long long nrows,
                                                       long long ntiles) {
   static_assert(!M32 || sizeof(R) == 4, "FP32 MFMA combine needs FP32 storage");

@@ -601,6 +601,7 @@ void bdgx_ctx_destroy(bdgx_ctx *c) {
   if (c->lp_tmp) (void)hipFree(c->lp_tmp);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -626,6 +627,36 @@ int bdgx_set_fields(bdgx_ctx *c, const double *ltK, const double *ltM, const voi
   if (pK && pM) c->has_p = 1;
   if (wf) c->has_wf = 1;
   c->mu = mu;
+  return BDGX_OK;
+}
+
+// pK = 1 / sqrt(max(floor, ltK + mu)), pM from ltM; FP32 copies converted
+__global__ void k_derive_precond(const double *ltK, const double *ltM, double mu, double floor,
+                                 double *pK, double *pM, float *pK32, float *pM32, long long n) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const double a = 1.0 / sqrt(fmax(floor, ltK[e] + mu));
+    const double b = 1.0 / sqrt(fmax(floor, ltM[e] + mu));
+    pK[e] = a;
+    pM[e] = b;
+    pK32[e] = (float)a;
+    pM32[e] = (float)b;
+  }
+}
+
+int bdgx_derive_precond(bdgx_ctx *c, double floor) {
+  if (!c || !c->has_lt) return BDGX_EINVAL;
+  const size_t S = (size_t)c->geo.S;
+  if (!c->f64.pK) BDGX_HIP(hipMalloc((void **)&c->f64.pK, sizeof(double) * (S ? S : 1)));
+  if (!c->f64.pM) BDGX_HIP(hipMalloc((void **)&c->f64.pM, sizeof(double) * (S ? S : 1)));
+  if (!c->f32.pK) BDGX_HIP(hipMalloc((void **)&c->f32.pK, sizeof(float) * (S ? S : 1)));
+  if (!c->f32.pM) BDGX_HIP(hipMalloc((void **)&c->f32.pM, sizeof(float) * (S ? S : 1)));
+  hipLaunchKernelGGL(k_derive_precond, dim3(1024), dim3(256), 0, c->stream, c->f64.ltK,
+                     c->f64.ltM, c->mu, floor, c->f64.pK, c->f64.pM, c->f32.pK, c->f32.pM,
+                     (long long)S);
+  BDGX_HIP(hipGetLastError());
+  BDGX_HIP(hipStreamSynchronize(c->stream));
+  c->has_p = 1;
   return BDGX_OK;
 }
 
@@ -990,6 +1021,24 @@ int bdgx_memcpy_h2d_async(bdgx_ctx *c, void *d, const void *s, size_t n) {
 int bdgx_memcpy_d2h(bdgx_ctx *c, void *d, const void *s, size_t n) {
   BDGX_HIP(hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, c->stream));
   BDGX_HIP(hipStreamSynchronize(c->stream));
+  return BDGX_OK;
+}
+int bdgx_memcpy_d2h_pair(bdgx_ctx *c, void *d1, const void *s1, void *d2, const void *s2,
+                         size_t n) {
+  if (!c) return BDGX_EINVAL;
+  if (!c->aux) BDGX_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+  // the second copy waits for the work queued so far on the context stream
+  hipEvent_t ready;
+  BDGX_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ready, c->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->aux, ready, 0);
+  if (e == hipSuccess) e = hipMemcpyAsync(d1, s1, n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d2, s2, n, hipMemcpyDeviceToHost, c->aux);
+  const hipError_t e1 = hipStreamSynchronize(c->stream), e2 = hipStreamSynchronize(c->aux);
+  (void)hipEventDestroy(ready);
+  BDGX_HIP(e);
+  BDGX_HIP(e1);
+  BDGX_HIP(e2);
   return BDGX_OK;
 }
 int bdgx_fill_uniform_xorshift(bdgx_ctx *c, void *dX, uint64_t ld, uint64_t nrows,

@@ -67,6 +67,13 @@ int bdgx_set_fields(bdgx_ctx *ctx, const double *ltK, const double *ltM,
                     const void *wf, const double *pK, const double *pM,
                     double mu);
 
+/* The preconditioner weights from the uploaded local terms, on the device:
+ * pK = 1 / sqrt(max(floor, ltK + mu)), pM likewise from ltM (FP64, and the
+ * FP32 copies converted from them); mu as given to bdgx_set_fields.  The
+ * same values as host arrays passed as pK, pM (IEEE square root and
+ * division), without their computation and upload on the host. */
+int bdgx_derive_precond(bdgx_ctx *ctx, double floor);
+
 /* Mark every physics field absent (allocations are kept); the next uploads
  * re-define the problem.  Device side of bdg_reset (reference src/bdg.c:55-84). */
 int bdgx_reset_fields(bdgx_ctx *ctx);
@@ -222,6 +229,11 @@ int bdgx_host_free(void *hptr);
 /* Copies on the ctx stream; both return after the copy has completed. */
 int bdgx_memcpy_h2d(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bdgx_memcpy_d2h(bdgx_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Two device-to-host copies of `bytes` each, after the work queued on the
+ * ctx stream, on two streams at once (two copy engines); returns after both
+ * have completed. */
+int bdgx_memcpy_d2h_pair(bdgx_ctx *ctx, void *dst1, const void *src1, void *dst2,
+                         const void *src2, size_t bytes);
 /* Host-to-device copy queued on the ctx stream without waiting: src must be
  * page-locked (bdgx_host_alloc) and must not change until the stream has
  * passed the copy (a later bdgx_sync or synchronous copy on this ctx). */

@@ -367,3 +367,27 @@ def test_complex_plane_launch_variants(grid):
                 for op, ref in refs.items():
                     e = relerr(d.apply(op, X, prec), ref)
                     assert e < TOL[prec], f"kind {kind} variant {v} prec {prec} {op}: {e:.2e}"
+
+
+@pytest.mark.parametrize("cplx", [False, True], ids=["real", "complex"])
+def test_derived_precond_weights_match_host_weights(cplx):
+    """bdgx_derive_precond (the solver's preconditioner weights 1 / sqrt(max(
+    floor, lt + mu)) computed on the device from the uploaded local terms)
+    against the same weights computed on the host with numpy and uploaded:
+    precondK and precondM agree bit for bit in FP64 and FP32."""
+    N, L = (32, 32, 12), (16.0, 16.0, 10.0)
+    c, f = physics(N, L, cplx, dipolar=True)
+    mu = float(f["mu"])
+    floor = max(mu, 1e-8)
+    pK = 1.0 / np.sqrt(np.maximum(floor, f["ltK"] + mu))
+    pM = 1.0 / np.sqrt(np.maximum(floor, f["ltM"] + mu))
+    host = device_twin(N, L, cplx, dict(f, pK=pK, pM=pM))
+    dev = device_twin(N, L, cplx, dict(f, pK=None, pM=None))
+    dev.derive_precond(floor)
+    X = rand_block(c.size, 3, cplx, seed=5)
+    print(f"case N={N} cplx={cplx}", flush=True)
+    for prec in (0, 1):
+        for op in ("precondK", "precondM"):
+            a = host.apply(op, X, prec)
+            b = dev.apply(op, X, prec)
+            assert np.array_equal(a, b), (op, prec, float(np.abs(a - b).max()))
