@@ -679,8 +679,11 @@ static int solve_first_order(bdg_t *b, void **pU0, void **pV0, int learned, doub
   cfg.nev = (int)nev;
   cfg.cplx = c;
   cfg.verbosity = b->verbosity;
-  /* automatic locking: once at most max(2, m / 8) columns are active */
-  cfg.lock_max = b->lock_max >= 0 ? b->lock_max : (int)(m / 8 > 2 ? m / 8 : 2);
+  /* automatic locking: once at most max(2, m / 4) columns are active (at
+   * 128^3 with m = 32 the FP64 solve took 128 iterations, 0.96 s, against
+   * 172, 1.08 s when locking at m / 8: the small deflated block converges
+   * the last pairs in fewer, cheaper iterations) */
+  cfg.lock_max = b->lock_max >= 0 ? b->lock_max : (int)(m / 4 > 2 ? m / 4 : 2);
   cfg.lock_guards = b->lock_guards;
   /* the learned spectral layers replace precondK / precondM in the FP32
    * phase (mixed precision only; the FP64 phase keeps the analytic ones) */

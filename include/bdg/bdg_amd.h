@@ -45,7 +45,7 @@ void bdg_set_fp32_phase(bdg_t *bdg, f64 fp32_tol, uint64_t stall_window);
 
 /* First-order form: once at most max_active columns are still active, the
  * converged pairs are locked and the rest continue in a small block deflated
- * against them (default -1: max(2, sizeSub / 8); 0: never lock). */
+ * against them (default -1: max(2, sizeSub / 4); 0: never lock). */
 void bdg_set_locking(bdg_t *bdg, int max_active);
 /* First-order formulation: the small block that continues after locking also
  * carries this many guard columns, started from the Ritz pairs just beyond
