@@ -76,6 +76,21 @@ static void *prefault_main(void *arg) {
   return NULL;
 }
 
+/* the helper thread, started once the solver's own device allocations are
+ * done (page-locking the host arrays holds the runtime's memory lock:
+ * started earlier, it stalled the solver's first allocations by ~28 ms) */
+typedef struct {
+  result_prefault pf;
+  pthread_t th;
+  int on; /* 0 not started, 1 running or done, -1 could not start */
+} prefault_job;
+
+static void prefault_start(void *arg) {
+  prefault_job *j = (prefault_job *)arg;
+  if (j->on) return;
+  j->on = pthread_create(&j->th, NULL, prefault_main, &j->pf) == 0 ? 1 : -1;
+}
+
 static void release_prefault(result_prefault *r) {
   if (r->pinned) {
     (void)bdgx_host_free(r->u);
@@ -700,9 +715,11 @@ static int solve_first_order(bdg_t *b, void **pU0, void **pV0, int learned, doub
   /* the host arrays of the modes, page-locked on a helper thread while the
    * device iterates (their download then runs at the link rate: pageable
    * memory took ~0.15 s for the 1 GB of 32 modes at 128^3) */
-  result_prefault pf = {S * nev * es64, NULL, NULL, 0};
-  pthread_t pf_thread;
-  const int pf_on = pthread_create(&pf_thread, NULL, prefault_main, &pf) == 0;
+  prefault_job pj;
+  memset(&pj, 0, sizeof(pj));
+  pj.pf.bytes = S * nev * es64;
+  cfg.on_iterating = prefault_start;
+  cfg.on_iterating_arg = &pj;
   if (mixed) {
     /* FP32 phase on FP32 copies of the start halves, which then receive its
      * vectors; the FP64 start block is released first, so that at the
@@ -834,20 +851,20 @@ static int solve_first_order(bdg_t *b, void **pU0, void **pV0, int learned, doub
   if (!rc) rc = bdgx_malloc(&dOutV, S * nev * es64);
   if (!rc) rc = bdgx_combine(b->dev, BDGX_FP64, dU, S, nev, dcu, dOutU, S, nev, S);
   if (!rc) rc = bdgx_combine(b->dev, BDGX_FP64, dV, S, nev, dcu, dOutV, S, nev, S);
-  if (pf_on) pthread_join(pf_thread, NULL);
+  if (pj.on == 1) pthread_join(pj.th, NULL);
   if (!rc) {
-    if (pf_on && pf.u && pf.v) {
-      b->modes_u = pf.u;
-      b->modes_v = pf.v;
-      b->modes_pinned = pf.pinned;
-      pf.u = pf.v = NULL;
+    if (pj.pf.u && pj.pf.v) {
+      b->modes_u = pj.pf.u;
+      b->modes_v = pj.pf.v;
+      b->modes_pinned = pj.pf.pinned;
+      pj.pf.u = pj.pf.v = NULL;
     } else {
       b->modes_u = xmalloc_big(S * nev * es64);
       b->modes_v = xmalloc_big(S * nev * es64);
     }
     rc = bdgx_memcpy_d2h_pair(b->dev, b->modes_u, dOutU, b->modes_v, dOutV, S * nev * es64);
   }
-  release_prefault(&pf);
+  release_prefault(&pj.pf);
   bdgx_free(dzero);
   bdgx_free(dn2);
   bdgx_free(dcu);

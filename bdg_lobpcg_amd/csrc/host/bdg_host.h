@@ -182,6 +182,8 @@ typedef struct {
   int lp_window;         /* ... handed back to precondK / precondM for the rest  */
   double lp_factor;      /*   of the solve once the worst residual is above      */
                          /*   lp_factor x its value lp_window iterations earlier */
+  void (*on_iterating)(void *); /* called once (may be NULL) when the block's     */
+  void *on_iterating_arg;       /*   allocations are done and the iterations start */
 } lrep_cfg;
 
 typedef struct {

@@ -423,6 +423,7 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
     res->applies += 2 * (uint64_t)m;
   }
 
+  if (cfg->on_iterating) cfg->on_iterating(cfg->on_iterating_arg);
   for (it = 0; it <= cfg->maxiter; ++it) {
     /* Grams of the current basis: all on the device at the start and every
      * `refresh` iterations, else the new W columns only */
@@ -709,6 +710,7 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
       icfg.dVL = dVL;
       icfg.nl = nc;
       icfg.learned = lp_on;
+      icfg.on_iterating = NULL;
       lrep_result ires;
       memset(&ires, 0, sizeof(ires));
       double *iw2 = (double *)calloc((size_t)mi, sizeof(double));
