@@ -54,6 +54,68 @@ typedef struct {
   int pinned;
 } result_prefault;
 
+/* Page-locked host blocks of finished results, kept for later solves of
+ * this process: page-locking takes the runtime's memory lock for as long as
+ * the pages take to pin (tens of ms per GB, now and then seconds), which
+ * stalls the solver's own HIP calls on the other thread; a block taken from
+ * here needs no pinning.  At most PIN_POOL blocks of at most PIN_POOL_MAX
+ * bytes each are kept; the rest are released. */
+#define PIN_POOL 4
+#define PIN_POOL_MAX ((size_t)4 << 30)
+static struct {
+  void *p;
+  size_t bytes;
+} pin_pool[PIN_POOL];
+static pthread_mutex_t pin_mu = PTHREAD_MUTEX_INITIALIZER;
+
+/* the smallest kept block of at least `bytes`, or NULL */
+static void *pin_take(size_t bytes) {
+  pthread_mutex_lock(&pin_mu);
+  int best = -1;
+  for (int i = 0; i < PIN_POOL; ++i)
+    if (pin_pool[i].p && pin_pool[i].bytes >= bytes &&
+        (best < 0 || pin_pool[i].bytes < pin_pool[best].bytes))
+      best = i;
+  void *p = NULL;
+  if (best >= 0) {
+    p = pin_pool[best].p;
+    pin_pool[best].p = NULL;
+    pin_pool[best].bytes = 0;
+  }
+  pthread_mutex_unlock(&pin_mu);
+  return p;
+}
+
+/* keep a page-locked block (a full pool gives up its smallest block for a
+ * larger one) or release it */
+static void pin_give(void *p, size_t bytes) {
+  if (!p) return;
+  void *drop = p;
+  if (bytes <= PIN_POOL_MAX) {
+    pthread_mutex_lock(&pin_mu);
+    int slot = -1;
+    for (int i = 0; i < PIN_POOL && slot < 0; ++i)
+      if (!pin_pool[i].p) slot = i;
+    if (slot < 0) {
+      int small = 0;
+      for (int i = 1; i < PIN_POOL; ++i)
+        if (pin_pool[i].bytes < pin_pool[small].bytes) small = i;
+      if (pin_pool[small].bytes < bytes) {
+        slot = small;
+        drop = pin_pool[small].p;
+      }
+    } else {
+      drop = NULL;
+    }
+    if (slot >= 0) {
+      pin_pool[slot].p = p;
+      pin_pool[slot].bytes = bytes;
+    }
+    pthread_mutex_unlock(&pin_mu);
+  }
+  if (drop) (void)bdgx_host_free(drop);
+}
+
 static void touch_pages(void *p, size_t bytes) {
   volatile char *c = (volatile char *)p;
   for (size_t o = 0; o < bytes; o += 4096) c[o] = 0;
@@ -61,12 +123,13 @@ static void touch_pages(void *p, size_t bytes) {
 
 static void *prefault_main(void *arg) {
   result_prefault *r = (result_prefault *)arg;
-  if (bdgx_host_alloc(&r->u, r->bytes) == BDGX_OK &&
-      bdgx_host_alloc(&r->v, r->bytes) == BDGX_OK) {
+  if ((r->u || bdgx_host_alloc(&r->u, r->bytes) == BDGX_OK) &&
+      (r->v || bdgx_host_alloc(&r->v, r->bytes) == BDGX_OK)) {
     r->pinned = 1;
     return NULL;
   }
-  (void)bdgx_host_free(r->u);
+  pin_give(r->u, r->bytes);
+  pin_give(r->v, r->bytes);
   r->u = NULL;
   r->v = NULL;
   r->u = xmalloc_big(r->bytes);
@@ -88,13 +151,26 @@ typedef struct {
 static void prefault_start(void *arg) {
   prefault_job *j = (prefault_job *)arg;
   if (j->on) return;
+  /* blocks kept from earlier solves first: no thread when both are there */
+  j->pf.u = pin_take(j->pf.bytes);
+  j->pf.v = pin_take(j->pf.bytes);
+  if (j->pf.u && j->pf.v) {
+    j->pf.pinned = 1;
+    j->on = 2;
+    return;
+  }
   j->on = pthread_create(&j->th, NULL, prefault_main, &j->pf) == 0 ? 1 : -1;
+  if (j->on < 0) {
+    pin_give(j->pf.u, j->pf.bytes);
+    pin_give(j->pf.v, j->pf.bytes);
+    j->pf.u = j->pf.v = NULL;
+  }
 }
 
 static void release_prefault(result_prefault *r) {
   if (r->pinned) {
-    (void)bdgx_host_free(r->u);
-    (void)bdgx_host_free(r->v);
+    pin_give(r->u, r->bytes);
+    pin_give(r->v, r->bytes);
   } else {
     free(r->u);
     free(r->v);
@@ -145,8 +221,8 @@ bdg_t *bdg_alloc(uint64_t dim, const uint64_t *N, const f64 *L, int complex_psi0
 static void free_results(bdg_t *b) {
   freep((void **)&b->eigvals);
   if (b->modes_pinned) {
-    (void)bdgx_host_free(b->modes_u);
-    (void)bdgx_host_free(b->modes_v);
+    pin_give(b->modes_u, b->modes_bytes);
+    pin_give(b->modes_v, b->modes_bytes);
     b->modes_u = b->modes_v = NULL;
     b->modes_pinned = 0;
   }
@@ -857,6 +933,7 @@ static int solve_first_order(bdg_t *b, void **pU0, void **pV0, int learned, doub
       b->modes_u = pj.pf.u;
       b->modes_v = pj.pf.v;
       b->modes_pinned = pj.pf.pinned;
+      b->modes_bytes = pj.pf.bytes;
       pj.pf.u = pj.pf.v = NULL;
     } else {
       b->modes_u = xmalloc_big(S * nev * es64);
@@ -1000,8 +1077,14 @@ int bdg_solve(bdg_t *b) {
   r.res = (double *)xcalloc(m, sizeof(double));
   void *dX = NULL, *dMX = NULL;
   result_prefault pf = {S * nev * es64, NULL, NULL, 0};
+  pf.u = pin_take(pf.bytes);
+  pf.v = pin_take(pf.bytes);
   pthread_t pf_thread;
-  const int pf_on = pthread_create(&pf_thread, NULL, prefault_main, &pf) == 0;
+  int pf_on = 0;
+  if (pf.u && pf.v)
+    pf.pinned = 1;
+  else
+    pf_on = pthread_create(&pf_thread, NULL, prefault_main, &pf) == 0;
   if (b->verbosity > 0)
     fprintf(stderr, "[bdg] setup: %.1f ms (fields %.1f, start block %.1f, upload %.1f)\n",
             1e3 * (bdg_wtime() - t_start), 1e3 * (ts_fields - t_start),
@@ -1115,6 +1198,7 @@ int bdg_solve(bdg_t *b) {
       b->modes_u = pf.u;
       b->modes_v = pf.v;
       b->modes_pinned = pf.pinned;
+      b->modes_bytes = pf.bytes;
       pf.u = pf.v = NULL;
     } else {
       b->modes_u = xmalloc_big(S * nev * es64);

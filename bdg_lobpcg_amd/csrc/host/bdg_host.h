@@ -81,7 +81,8 @@ struct bdg_t {
   /* results */
   double *eigvals;
   void *modes_u, *modes_v;
-  int modes_pinned; /* modes_u / modes_v from bdgx_host_alloc */
+  int modes_pinned; /* modes_u / modes_v page-locked (from the pinned pool) */
+  size_t modes_bytes; /* bytes of each of modes_u / modes_v */
   uint64_t converged;
   double *residuals;
   int *zero_modes;

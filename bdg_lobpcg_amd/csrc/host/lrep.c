@@ -611,9 +611,14 @@ int bdg_lrep_device(bdgx_ctx *ctx, const lrep_cfg *cfg, int prec, const void *dX
       int allc = 1;
       for (int j = 0; j < nev; ++j)
         if (!conv[j]) allc = 0;
-      /* a partial evaluation that finds everything converged is confirmed by
-       * a full one */
-      if (allc && !full) {
+      /* a partial evaluation that finds everything converged, or after which
+       * the converged pairs would be locked, is confirmed by a full one: the
+       * other columns' residuals can be up to 10 iterations old, and a pair
+       * that has drifted since must neither end the solve nor be locked
+       * (locked pairs are final) */
+      const int will_lock = !allc && cfg->lock_max > 0 && cfg->nl == 0 && nact <= cfg->lock_max &&
+                            nact < m && it < cfg->maxiter;
+      if ((allc || will_lock) && !full) {
         full = 1;
         continue;
       }
