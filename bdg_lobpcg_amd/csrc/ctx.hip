@@ -3,7 +3,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "bdgx_internal.hpp"
@@ -432,6 +434,38 @@ __global__ void __launch_bounds__(256) k_reuse_fill(const double *U, const doubl
     }
   }
 }
+
+// device blocks kept between bdgx_free and bdgx_malloc (below)
+namespace {
+constexpr size_t kDevCacheMax = (size_t)64 << 30;
+struct DevCache {
+  std::mutex mu;
+  std::map<std::pair<int, size_t>, std::vector<void *>> freeb;  // (device, bytes) -> blocks
+  std::map<void *, std::pair<int, size_t>> live;                // blocks handed out
+  size_t kept[64] = {0};                                         // bytes kept per device
+};
+DevCache &dev_cache() {
+  static DevCache c;
+  return c;
+}
+void release_kept(int dev) {
+  DevCache &c = dev_cache();
+  std::vector<void *> drop;
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    for (auto it = c.freeb.begin(); it != c.freeb.end();) {
+      if (it->first.first == dev) {
+        for (void *q : it->second) drop.push_back(q);
+        it = c.freeb.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    if (dev >= 0 && dev < 64) c.kept[dev] = 0;
+  }
+  for (void *q : drop) (void)hipFree(q);
+}
+}  // namespace
 
 extern "C" {
 
@@ -984,18 +1018,79 @@ int bdgx_itp_groundstate(bdgx_ctx *c, const double *omega, double g, double g_dd
   return BDGX_OK;
 }
 
+// Device blocks released by bdgx_free are kept for the next bdgx_malloc of
+// the same size on the same device (a solve allocates the same sizes at
+// the same points every time: ~0.3 ms per hipMalloc / hipFree of a large
+// block, at locking and between precision phases).  A release waits for
+// the device first, as hipFree does (work queued on any stream may still
+// use the block).  At most kDevCacheMax bytes are kept; when hipMalloc runs
+// out of memory the kept blocks are released and it is tried again, and
+// bdgx_mem_info counts kept blocks as free.
+
 int bdgx_malloc(void **p, size_t bytes) {
   if (!p) return BDGX_EINVAL;
-  BDGX_HIP(hipMalloc(p, bytes ? bytes : 1));
+  const size_t b = bytes ? bytes : 1;
+  int dev = 0;
+  BDGX_HIP(hipGetDevice(&dev));
+  DevCache &c = dev_cache();
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.freeb.find({dev, b});
+    if (it != c.freeb.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      if (dev >= 0 && dev < 64) c.kept[dev] -= b;
+      c.live[*p] = {dev, b};
+      return BDGX_OK;
+    }
+  }
+  hipError_t e = hipMalloc(p, b);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    release_kept(dev);
+    e = hipMalloc(p, b);
+  }
+  BDGX_HIP(e);
+  std::lock_guard<std::mutex> g(c.mu);
+  c.live[*p] = {dev, b};
   return BDGX_OK;
 }
 int bdgx_mem_info(size_t *free_bytes, size_t *total_bytes) {
   if (!free_bytes || !total_bytes) return BDGX_EINVAL;
   BDGX_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  int dev = 0;
+  BDGX_HIP(hipGetDevice(&dev));
+  DevCache &c = dev_cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  if (dev >= 0 && dev < 64) *free_bytes += c.kept[dev];
   return BDGX_OK;
 }
 int bdgx_free(void *p) {
-  if (p) BDGX_HIP(hipFree(p));
+  if (!p) return BDGX_OK;
+  DevCache &c = dev_cache();
+  std::pair<int, size_t> key{-1, 0};
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.live.find(p);
+    if (it != c.live.end()) {
+      key = it->second;
+      c.live.erase(it);
+    }
+  }
+  if (key.first < 0 || key.first >= 64) {  // not from bdgx_malloc
+    BDGX_HIP(hipFree(p));
+    return BDGX_OK;
+  }
+  BDGX_HIP(hipDeviceSynchronize());
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    if (c.kept[key.first] + key.second <= kDevCacheMax) {
+      c.freeb[key].push_back(p);
+      c.kept[key.first] += key.second;
+      return BDGX_OK;
+    }
+  }
+  BDGX_HIP(hipFree(p));
   return BDGX_OK;
 }
 int bdgx_host_alloc(void **p, size_t bytes) {

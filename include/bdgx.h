@@ -218,9 +218,13 @@ int bdgx_itp_groundstate(bdgx_ctx *ctx, const double *omega, double g,
                          double *mu_out);
 
 /* ---- memory, streams, timing ------------------------------------------ */
+/* Device memory.  bdgx_free waits for the device (as hipFree does) and keeps
+ * the block for the next bdgx_malloc of the same size on the same device
+ * (up to 64 GiB kept per device; released when hipMalloc runs short). */
 int bdgx_malloc(void **dptr, size_t bytes);
 int bdgx_free(void *dptr);
-/* Free and total device memory of the current device (hipMemGetInfo). */
+/* Free and total device memory of the current device (hipMemGetInfo), the
+ * blocks kept by bdgx_free counted as free. */
 int bdgx_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* Page-locked host memory (hipHostMalloc / hipHostFree): copies between it
  * and the device run at the full link rate. */
